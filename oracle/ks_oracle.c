@@ -1,0 +1,1040 @@
+/*
+ * ks_oracle.c -- CPU restatement of SLEPc 3.22's Krylov-Schur path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Loaded by tests/, by smoke() and by
+ * bench.py's cpu_baseline leg; never by the product.  It follows the reference
+ * line by line so that its trajectory (restarts, DGKS passes, truncation sizes)
+ * is the one SLEPc's CPU path takes on the same inputs:
+ *
+ *   CSR MatMult               PETSc MatMult_SeqAIJ row loop (called at
+ *                             src/sys/classes/bv/interface/bvops.c:879)
+ *   BVMAT ops                 src/sys/classes/bv/impls/mat/bvmat.c:38-71,110-176,250-260
+ *                             src/sys/classes/bv/interface/bvblas.c:24-278
+ *   CGS + DGKS                src/sys/classes/bv/interface/bvorthog.c:91-217,380-427
+ *   coefficient buffer        include/slepc/private/bvimpl.h:121-141,289-415
+ *   Arnoldi / Lanczos         src/sys/classes/bv/interface/bvkrylov.c:56-226
+ *   DSHEP (compact)           src/sys/classes/ds/impls/hep/dshep.c:137-426,643-670
+ *   DSNHEP                    src/sys/classes/ds/impls/dsutil.c:21-171,
+ *                             src/sys/classes/ds/impls/nhep/dsnhep.c:101-226,318-425
+ *   DS interface              src/sys/classes/ds/interface/dsops.c:63-160,232-275,329-344
+ *                             src/sys/classes/ds/interface/dspriv.c:172-270
+ *   sorting criteria          src/sys/slepcsc.c:40-190
+ *   KS restart loop           src/eps/impls/krylov/krylovschur/krylovschur.c:93-194,227-337
+ *   convergence test          src/eps/impls/krylov/epskrylov.c:207-295
+ *   EPS driver                src/eps/interface/epssolve.c:119-208,666-873
+ *                             src/eps/interface/epssetup.c:286-418,648-732
+ *                             src/eps/interface/epsdefault.c:28-50,105-169,224-302
+ *
+ * Third-party arithmetic at the boundary: BLAS/LAPACK (MKL via libmkl_rt,
+ * LP64) for gemv/gemm/nrm2/lartg/rot/steqr/gehrd/orghr/hseqr/trexc/trevc/lascl,
+ * exactly the routines the reference calls.
+ */
+#include "ks_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#include <omp.h>
+
+/* ---------------------------------------------------------------- BLAS/LAPACK */
+extern void   dgemv_(const char*, const int*, const int*, const double*, const double*, const int*,
+                     const double*, const int*, const double*, double*, const int*);
+extern void   dgemm_(const char*, const char*, const int*, const int*, const int*, const double*,
+                     const double*, const int*, const double*, const int*, const double*, double*,
+                     const int*);
+extern double dnrm2_(const int*, const double*, const int*);
+extern void   dlartg_(double*, double*, double*, double*, double*);
+extern void   drot_(const int*, double*, const int*, double*, const int*, const double*, const double*);
+extern void   dsteqr_(const char*, const int*, double*, double*, double*, const int*, double*, int*);
+extern void   dgehrd_(const int*, const int*, const int*, double*, const int*, double*, double*,
+                      const int*, int*);
+extern void   dorghr_(const int*, const int*, const int*, double*, const int*, const double*, double*,
+                      const int*, int*);
+extern void   dhseqr_(const char*, const char*, const int*, const int*, const int*, double*,
+                      const int*, double*, double*, double*, const int*, double*, const int*, int*);
+extern void   dtrexc_(const char*, const int*, double*, const int*, double*, const int*, int*, int*,
+                      double*, int*);
+extern void   dtrevc_(const char*, const char*, int*, const int*, const double*, const int*, double*,
+                      const int*, double*, const int*, const int*, int*, double*, int*);
+extern void   dlascl_(const char*, const int*, const int*, const double*, const double*, const int*,
+                      const int*, double*, const int*, int*);
+
+#define OR_ERR(code, ...) do { fprintf(stderr, "ks_oracle: " __VA_ARGS__); fputc('\n', stderr); return (code); } while (0)
+#define OR_CALL(x) do { int _e = (x); if (_e) return _e; } while (0)
+#define MACH_EPS DBL_EPSILON
+
+static double wtime(void) { return omp_get_wtime(); }
+
+/* ------------------------------------------------------------------ RNG */
+static unsigned long long splitmix64(unsigned long long z)
+{
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+void or_random_uniform(unsigned long long seed, int n, double *out)
+{
+  unsigned long long base = seed * 0x9E3779B97F4A7C15ULL;
+  for (int i = 0; i < n; i++) {
+    unsigned long long r = splitmix64(base + (unsigned long long)i);
+    out[i] = 2.0 * ((double)(r >> 11) * 0x1.0p-53) - 1.0;
+  }
+}
+
+/* ------------------------------------------------------------------ SpMV */
+int or_spmv(int n, const int *rowptr, const int *colidx, const double *val,
+            const double *x, double *y)
+{
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    double s = 0.0;
+    for (int p = rowptr[i]; p < rowptr[i + 1]; p++) s += val[p] * x[colidx[p]];
+    y[i] = s;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ sorting (slepcsc.c) */
+typedef struct { int which; double target; } or_sc;
+
+/* SlepcCompare* (slepcsc.c:152-190): result < 0 means a goes first */
+static int sc_compare(const or_sc *sc, double ar, double ai, double br, double bi)
+{
+  double a = 0, b = 0;
+  switch (sc->which) {
+  case OR_LARGEST_MAGNITUDE:  a = hypot(ar, ai); b = hypot(br, bi); return a < b ? 1 : (a > b ? -1 : 0);
+  case OR_SMALLEST_MAGNITUDE: a = hypot(ar, ai); b = hypot(br, bi); return a > b ? 1 : (a < b ? -1 : 0);
+  case OR_LARGEST_REAL:       a = ar; b = br; return a < b ? 1 : (a > b ? -1 : 0);
+  case OR_SMALLEST_REAL:      a = ar; b = br; return a > b ? 1 : (a < b ? -1 : 0);
+  case OR_LARGEST_IMAGINARY:  a = fabs(ai); b = fabs(bi); return a < b ? 1 : (a > b ? -1 : 0);
+  case OR_SMALLEST_IMAGINARY: a = fabs(ai); b = fabs(bi); return a > b ? 1 : (a < b ? -1 : 0);
+  case OR_TARGET_MAGNITUDE:   a = hypot(ar - sc->target, ai); b = hypot(br - sc->target, bi);
+                              return a > b ? 1 : (a < b ? -1 : 0);
+  case OR_TARGET_REAL:        a = fabs(ar - sc->target); b = fabs(br - sc->target);
+                              return a > b ? 1 : (a < b ? -1 : 0);
+  }
+  return 0;
+}
+
+/* SlepcSortEigenvalues (slepcsc.c:89-150) */
+static void sort_eigenvalues(const or_sc *sc, int n, const double *eigr, const double *eigi, int *perm)
+{
+  for (int i = n - 1; i >= 0; i--) {
+    double re = eigr[perm[i]], im = eigi[perm[i]];
+    int j = i + 1, tmp;
+    if (im != 0) { i--; im = eigi[perm[i]]; }
+    while (j < n) {
+      int result = sc_compare(sc, re, im, eigr[perm[j]], eigi[perm[j]]);
+      if (result <= 0) break;
+      if (!im) {
+        if (eigi[perm[j]] == 0.0) {
+          tmp = perm[j - 1]; perm[j - 1] = perm[j]; perm[j] = tmp; j++;
+        } else {
+          tmp = perm[j - 1]; perm[j - 1] = perm[j]; perm[j] = perm[j + 1]; perm[j + 1] = tmp; j += 2;
+        }
+      } else {
+        if (eigi[perm[j]] == 0.0) {
+          tmp = perm[j - 2]; perm[j - 2] = perm[j]; perm[j] = perm[j - 1]; perm[j - 1] = tmp; j++;
+        } else {
+          tmp = perm[j - 2]; perm[j - 2] = perm[j]; perm[j] = tmp;
+          tmp = perm[j - 1]; perm[j - 1] = perm[j + 1]; perm[j + 1] = tmp; j += 2;
+        }
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ BV (BVMAT semantics) */
+typedef struct {
+  int n, m, nc, ld;         /* local rows, columns, constraints, leading dimension */
+  int l, k;                 /* active columns (l may be -nc) */
+  double *V;                /* ld x (nc+m), constraints first */
+  double *buffer;           /* (nc+m) x m coefficient buffer (bvbasic.c:755-786) */
+  int refine;
+  double eta;
+  long passes;
+  unsigned long long seed, rcount;
+} obv;
+
+#define COL(bv, j) ((bv)->V + (size_t)((bv)->nc + (j)) * (size_t)(bv)->ld)
+
+/* BVDotVec_Mat -> BVDotVec_BLAS_Private (bvmat.c:110-130, bvblas.c:240-261) */
+static void bv_dotvec(obv *bv, const double *y, double *q)
+{
+  int n = bv->n, k = bv->k - bv->l, one = 1, ld = bv->ld;
+  double done = 1.0, zero = 0.0;
+  if (n && k) dgemv_("C", &n, &k, &done, COL(bv, bv->l), &ld, y, &one, &zero, q, &one);
+  else for (int i = 0; i < k; i++) q[i] = 0.0;
+}
+
+/* BVMultVec_Mat -> BVMultVec_BLAS_Private (bvmat.c:38-53, bvblas.c:56-67) */
+static void bv_multvec(obv *bv, double alpha, double beta, double *y, const double *q)
+{
+  int n = bv->n, k = bv->k - bv->l, one = 1, ld = bv->ld;
+  if (n) dgemv_("N", &n, &k, &alpha, COL(bv, bv->l), &ld, q, &one, &beta, y, &one);
+}
+
+/* BVNorm_Mat for one column, NORM_2 (bvmat.c:165-176, bvlapack.c:49) */
+static double bv_normcol(obv *bv, int j)
+{
+  int n = bv->n, one = 1;
+  return dnrm2_(&n, COL(bv, j), &one);
+}
+
+/* BVScale_BLAS_Private (bvblas.c:266-278) */
+static void bv_scalecol(obv *bv, int j, double alpha)
+{
+  double *x = COL(bv, j);
+  if (alpha == 0.0) { memset(x, 0, sizeof(double) * (size_t)bv->n); return; }
+  if (alpha == 1.0) return;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < bv->n; i++) x[i] *= alpha;
+}
+
+static void bv_copycol(obv *bv, int j, int i)
+{
+  memcpy(COL(bv, i), COL(bv, j), sizeof(double) * (size_t)bv->n);
+}
+
+/* BVSetRandomColumn: uniform(-1,1) from the counter generator */
+static void bv_randomcol(obv *bv, int j)
+{
+  or_random_uniform(bv->seed + 1000003ULL * (++bv->rcount), bv->n, COL(bv, j));
+}
+
+/* BVMultInPlace_BLAS_Private (bvblas.c:74-106): row-blocked gemm through a work panel */
+static void multinplace_blas(int m, int k, int s, int e, double *A, int lda, const double *B, int ldb)
+{
+  const int bs = 64;
+  int n = e - s;
+  if (m <= 0 || n <= 0) return;
+  int nblk = (m + bs - 1) / bs;
+#pragma omp parallel
+  {
+    double *work = (double*)malloc(sizeof(double) * (size_t)bs * (size_t)n);
+    double one = 1.0, zero = 0.0;
+    const double *pb = B + (size_t)s * ldb;
+#pragma omp for schedule(static)
+    for (int b = 0; b < nblk; b++) {
+      int r0 = b * bs, rows = (r0 + bs <= m) ? bs : m - r0;
+      dgemm_("N", "N", &rows, &n, &k, &one, A + r0, &lda, pb, &ldb, &zero, work, &rows);
+      for (int j = 0; j < n; j++) memcpy(A + (size_t)(s + j) * lda + r0, work + (size_t)j * rows, sizeof(double) * rows);
+    }
+    free(work);
+  }
+}
+
+/* BVMultInPlace_Mat (bvmat.c:55-71) */
+static void bv_multinplace(obv *bv, const double *Q, int ldq, int s, int e)
+{
+  if (s >= e || !bv->n) return;
+  multinplace_blas(bv->n, bv->k - bv->l, s - bv->l, e - bv->l, COL(bv, bv->l), bv->ld,
+                   Q + (size_t)bv->l * ldq + bv->l, ldq);
+}
+
+int or_multinplace(int n, int ld, double *V, int nc, int l, int k,
+                   const double *Q, int ldq, int s, int e)
+{
+  obv bv = {0};
+  bv.n = n; bv.ld = ld; bv.nc = nc; bv.V = V; bv.l = l; bv.k = k;
+  bv_multinplace(&bv, Q, ldq, s, e);
+  return 0;
+}
+
+/* BV_SafeSqrt (bvimpl.h:121-141), definite inner product */
+static int safe_sqrt(double alpha, double *res)
+{
+  if (!(alpha > -10 * MACH_EPS)) OR_ERR(-62, "Invalid inner product: %g", alpha);
+  *res = (alpha < 0.0) ? 0.0 : sqrt(alpha);
+  return 0;
+}
+
+/* BVOrthogonalizeCGS1 (bvorthog.c:91-132) on column j, h = buffer column j, c = buffer column 0 */
+static int cgs1(obv *bv, int j, double *onorm, double *norm)
+{
+  int nc = bv->nc, ldb = bv->nc + bv->m, ksave;
+  double *c = bv->buffer, *h = bv->buffer + (size_t)j * ldb, beta = 0.0;
+  bv->passes++;
+  bv->k = j;
+  if (onorm || norm) {
+    ksave = bv->k; bv->k = j + 1;                  /* BVDotColumnInc (bvorthog.c:32-47) */
+    bv_dotvec(bv, COL(bv, j), c);
+    bv->k = ksave;
+    OR_CALL(safe_sqrt(c[nc + j], &beta));          /* BV_SquareRoot */
+  } else bv_dotvec(bv, COL(bv, j), c);
+  bv_multvec(bv, -1.0, 1.0, COL(bv, j), c);        /* BVMultColumn */
+  if (onorm) *onorm = beta;
+  if (norm) {
+    double sum = 0.0;
+    for (int i = 0; i < nc + j; i++) sum += c[i] * c[i];   /* BV_SquareSum */
+    *norm = beta * beta - sum;
+    if (*norm <= 0.0) *norm = bv_normcol(bv, j);
+    else *norm = sqrt(*norm);
+  }
+  for (int i = 0; i < nc + j; i++) h[i] += c[i];   /* BV_AddCoefficients */
+  return 0;
+}
+
+/* BVOrthogonalizeGS (bvorthog.c:145-217), column mode (v == NULL) */
+static int orthogonalize_gs(obv *bv, int j, double *norm, int *lindep)
+{
+  int nc = bv->nc, ldb = bv->nc + bv->m, l;
+  double onrm = 0.0, nrm = 0.0, *h = bv->buffer + (size_t)j * ldb;
+  int dolindep = lindep != NULL;
+  for (int i = 0; i < nc + j; i++) h[i] = 0.0;     /* BV_CleanCoefficients */
+  switch (bv->refine) {
+  case OR_REFINE_IFNEEDED:
+    OR_CALL(cgs1(bv, j, &onrm, &nrm));
+    l = 1;
+    while (l < 3 && nrm && fabs(nrm) < bv->eta * fabs(onrm)) {
+      l++;
+      OR_CALL(cgs1(bv, j, &onrm, &nrm));
+    }
+    if (dolindep) *lindep = !(nrm && fabs(nrm) >= bv->eta * fabs(onrm));
+    break;
+  case OR_REFINE_NEVER:
+    OR_CALL(cgs1(bv, j, NULL, NULL));
+    if (norm || dolindep) nrm = bv_normcol(bv, j);
+    if (dolindep) *lindep = !nrm;
+    break;
+  case OR_REFINE_ALWAYS:
+    OR_CALL(cgs1(bv, j, NULL, NULL));
+    OR_CALL(cgs1(bv, j, dolindep ? &onrm : NULL, (norm || dolindep) ? &nrm : NULL));
+    if (dolindep) *lindep = !(nrm && fabs(nrm) >= bv->eta * fabs(onrm));
+    break;
+  }
+  if (norm) {
+    *norm = nrm;
+    h[nc + j] = (dolindep && *lindep) ? 0.0 : nrm;   /* BV_SetValue */
+  }
+  return 0;
+}
+
+/* BVOrthogonalizeColumn (bvorthog.c:285-320): l = -nc, k saved */
+static int orthogonalize_column(obv *bv, int j, double *norm, int *lindep)
+{
+  int ksave = bv->k, lsave = bv->l;
+  bv->l = -bv->nc;
+  OR_CALL(orthogonalize_gs(bv, j, norm, lindep));
+  bv->k = ksave; bv->l = lsave;
+  return 0;
+}
+
+/* BVOrthonormalizeColumn (bvorthog.c:380-427), replace = FALSE */
+static int orthonormalize_column(obv *bv, int j, double *norm, int *lindep)
+{
+  double nrm;
+  int lnd = 0;
+  OR_CALL(orthogonalize_column(bv, j, &nrm, &lnd));
+  if (nrm != 1.0 && nrm != 0.0) bv_scalecol(bv, j, 1.0 / nrm);
+  if (norm) *norm = nrm;
+  if (lindep) *lindep = lnd;
+  return 0;
+}
+
+int or_orthonormalize_column(int n, int ld, double *V, int nc, int j, int refine,
+                             double eta, double *H, double *norm, int *lindep, int *npasses)
+{
+  obv bv = {0};
+  int m = j + 1;
+  bv.n = n; bv.ld = ld; bv.nc = nc; bv.m = m; bv.V = V;
+  bv.refine = refine; bv.eta = eta > 0 ? eta : 0.7071;
+  bv.buffer = (double*)calloc((size_t)(nc + m) * m, sizeof(double));
+  if (!bv.buffer) return -1;
+  int e = orthonormalize_column(&bv, j, norm, lindep);
+  if (!e && H) memcpy(H, bv.buffer + (size_t)j * (nc + m), sizeof(double) * (size_t)(nc + j + 1));
+  if (npasses) *npasses = (int)bv.passes;
+  free(bv.buffer);
+  return e;
+}
+
+/* ------------------------------------------------------------------ DS */
+enum { DS_RAW = 0, DS_INTERMEDIATE = 1, DS_CONDENSED = 2, DS_TRUNCATED = 3 };
+enum { DS_HEP = 1, DS_NHEP = 2 };
+
+typedef struct {
+  int type, ld, n, l, k, t, state, compact, extrarow;
+  double *A, *T, *Q, *X, *work;
+  int *perm, *iwork;
+  or_sc sc;
+} ods;
+
+static int ds_alloc(ods *ds, int type, int ld)
+{
+  memset(ds, 0, sizeof(*ds));
+  ds->type = type; ds->ld = ld;
+  ds->A = (double*)calloc((size_t)ld * ld, sizeof(double));
+  ds->T = (double*)calloc((size_t)ld * 3, sizeof(double));
+  ds->Q = (double*)calloc((size_t)ld * ld, sizeof(double));
+  ds->X = (double*)calloc((size_t)ld * ld, sizeof(double));
+  ds->work = (double*)calloc((size_t)ld * ld + 8 * (size_t)ld, sizeof(double));
+  ds->perm = (int*)calloc((size_t)ld, sizeof(int));
+  ds->iwork = (int*)calloc((size_t)ld, sizeof(int));
+  if (!ds->A || !ds->T || !ds->Q || !ds->X || !ds->work || !ds->perm || !ds->iwork) return -1;
+  return 0;
+}
+
+static void ds_free(ods *ds)
+{
+  free(ds->A); free(ds->T); free(ds->Q); free(ds->X); free(ds->work); free(ds->perm); free(ds->iwork);
+}
+
+/* DSSetDimensions (dsops.c:130-160) */
+static void ds_setdims(ods *ds, int n, int l, int k)
+{
+  ds->n = n; ds->t = n; ds->l = l; ds->k = k;
+}
+
+/* DSSetIdentity (dspriv.c:377-395) */
+static void ds_setidentity(ods *ds, double *M)
+{
+  memset(M + (size_t)ds->l * ds->ld, 0, sizeof(double) * (size_t)ds->ld * (ds->n - ds->l));
+  for (int i = ds->l; i < ds->n; i++) M[i + (size_t)i * ds->ld] = 1.0;
+}
+
+/* DSArrowTridiag (dshep.c:221-262): Rutishauser reduction of an arrowhead */
+static void arrow_tridiag(int n, double *d, double *e, double *Q, int ld)
+{
+  int one = 1;
+  double c, s, p, off, temp;
+  if (n <= 2) return;
+  for (int j = 0; j < n - 2; j++) {
+    temp = e[j + 1];
+    dlartg_(&temp, &e[j], &c, &s, &e[j + 1]);
+    s = -s;
+    temp = d[j + 1];
+    e[j] = c * s * (temp - d[j]);
+    d[j + 1] = s * s * d[j] + c * c * temp;
+    d[j] = c * c * d[j] + s * s * temp;
+    int j2 = j + 2;
+    drot_(&j2, Q + (size_t)j * ld, &one, Q + (size_t)(j + 1) * ld, &one, &c, &s);
+    for (int i = j - 1; i >= 0; i--) {
+      off = -s * e[i];
+      e[i] = c * e[i];
+      temp = e[i + 1];
+      dlartg_(&temp, &off, &c, &s, &e[i + 1]);
+      s = -s;
+      temp = (d[i] - d[i + 1]) * s - 2.0 * c * e[i];
+      p = s * temp;
+      d[i + 1] += p;
+      d[i] -= p;
+      e[i] = -e[i] - c * temp;
+      drot_(&j2, Q + (size_t)i * ld, &one, Q + (size_t)(i + 1) * ld, &one, &c, &s);
+    }
+  }
+}
+
+/* DSSolve_HEP_QR (dshep.c:383-426) with DSIntermediate_HEP compact (dshep.c:267-321) */
+static int ds_solve_hep(ods *ds, double *wr, double *wi)
+{
+  int n = ds->n, l = ds->l, ld = ds->ld, n1, info;
+  double *d = ds->T, *e = ds->T + ld;
+  if (!ds->compact) OR_ERR(-56, "oracle DSHEP supports compact storage only");
+  ds_setidentity(ds, ds->Q);
+  n1 = ds->k - l + 1; if (n1 < 0) n1 = 0;
+  if (ds->state < DS_INTERMEDIATE) arrow_tridiag(n1, d + l, e + l, ds->Q + l + (size_t)l * ld, ld);
+  for (int i = 0; i < l; i++) wr[i] = d[i];
+  n1 = n - l;
+  dsteqr_("V", &n1, d + l, e + l, ds->Q + l + (size_t)l * ld, &ld, ds->work, &info);
+  if (info) OR_ERR(-76, "steqr info %d", info);
+  for (int i = l; i < n; i++) wr[i] = d[i];
+  for (int i = 0; i < n - 1; i++) e[i] = 0.0;
+  if (wi) for (int i = l; i < n; i++) wi[i] = 0.0;
+  return 0;
+}
+
+/* DSSortEigenvaluesReal_Private (dspriv.c:224-246) */
+static void ds_sort_real(ods *ds, const double *eig, int *perm)
+{
+  int n = ds->t, l = ds->l;
+  for (int i = l + 1; i < n; i++) {
+    double re = eig[perm[i]];
+    int j = i - 1;
+    int result = sc_compare(&ds->sc, re, 0.0, eig[perm[j]], 0.0);
+    while (result < 0 && j >= l) {
+      int tmp = perm[j]; perm[j] = perm[j + 1]; perm[j + 1] = tmp; j--;
+      if (j >= l) result = sc_compare(&ds->sc, re, 0.0, eig[perm[j]], 0.0);
+    }
+  }
+}
+
+/* DSPermuteColumns_Private (dspriv.c:248-270) */
+static void ds_permute_columns(ods *ds, int istart, int iend, int n, double *Q, int *perm)
+{
+  int ld = ds->ld;
+  for (int i = istart; i < iend; i++) {
+    int p = perm[i];
+    if (p != i) {
+      int j = i + 1;
+      while (perm[j] != i) j++;
+      perm[j] = p; perm[i] = i;
+      for (int k = 0; k < n; k++) {
+        double r = Q[k + (size_t)p * ld]; Q[k + (size_t)p * ld] = Q[k + (size_t)i * ld]; Q[k + (size_t)i * ld] = r;
+      }
+    }
+  }
+}
+
+/* DSSort_HEP (dshep.c:323-347) */
+static void ds_sort_hep(ods *ds, double *wr)
+{
+  int n = ds->n, l = ds->l;
+  double *d = ds->T;
+  for (int i = 0; i < n; i++) ds->perm[i] = i;
+  ds_sort_real(ds, d, ds->perm);
+  for (int i = l; i < n; i++) wr[i] = d[ds->perm[i]];
+  ds_permute_columns(ds, l, n, n, ds->Q, ds->perm);
+  for (int i = l; i < n; i++) d[i] = wr[i];
+}
+
+/* DSUpdateExtraRow_HEP, compact (dshep.c:349-381) */
+static void ds_update_extra_row_hep(ods *ds)
+{
+  int n = ds->n, ld = ds->ld;
+  double *e = ds->T + ld, beta = e[n - 1];
+  for (int i = 0; i < n; i++) e[i] = beta * ds->Q[n - 1 + (size_t)i * ld];
+  ds->k = n;
+}
+
+/* DSSolve_NHEP_Private (dsutil.c:21-87) */
+static int ds_solve_nhep(ods *ds, double *wr, double *wi)
+{
+  int n = ds->n, ld = ds->ld, ilo = ds->l + 1, lwork = 6 * ld, info;
+  double *A = ds->A, *Q = ds->Q, *tau = ds->work, *work = ds->work + ld;
+  memset(Q, 0, sizeof(double) * (size_t)ld * ld);
+  for (int i = 0; i < n; i++) Q[i + (size_t)i * ld] = 1.0;
+  if (n == 1) { wr[0] = A[0]; if (wi) wi[0] = 0.0; return 0; }
+  if (ds->state < DS_INTERMEDIATE) {
+    dgehrd_(&n, &ilo, &n, A, &ld, tau, work, &lwork, &info);
+    if (info) OR_ERR(-76, "gehrd info %d", info);
+    for (int j = 0; j < n - 1; j++)
+      for (int i = j + 2; i < n; i++) { Q[i + (size_t)j * ld] = A[i + (size_t)j * ld]; A[i + (size_t)j * ld] = 0.0; }
+    dorghr_(&n, &ilo, &n, Q, &ld, tau, work, &lwork, &info);
+    if (info) OR_ERR(-76, "orghr info %d", info);
+  }
+  dhseqr_("S", "V", &n, &ilo, &n, A, &ld, wr, wi, Q, &ld, work, &lwork, &info);
+  for (int j = 0; j < ds->l; j++) {
+    if (j == n - 1 || A[j + 1 + (size_t)j * ld] == 0.0) { wr[j] = A[j + (size_t)j * ld]; wi[j] = 0.0; }
+    else {
+      wr[j] = A[j + (size_t)j * ld]; wr[j + 1] = A[j + (size_t)j * ld];
+      wi[j] = sqrt(fabs(A[j + 1 + (size_t)j * ld])) * sqrt(fabs(A[j + (size_t)(j + 1) * ld]));
+      wi[j + 1] = -wi[j];
+      j++;
+    }
+  }
+  if (info) OR_ERR(-76, "hseqr info %d", info);
+  return 0;
+}
+
+/* DSSort_NHEP_Total (dsutil.c:93-171) */
+static int ds_sort_nhep(ods *ds, double *wr, double *wi)
+{
+  int n = ds->n, ld = ds->ld, info;
+  double *T = ds->A, *Q = ds->Q;
+  for (int i = ds->l; i < n - 1; i++) {
+    double re = wr[i], im = wi[i];
+    int pos = 0, j = i + 1;
+    if (im != 0) j = i + 2;
+    for (; j < n; j++) {
+      if (sc_compare(&ds->sc, re, im, wr[j], wi[j]) > 0) { re = wr[j]; im = wi[j]; pos = j; }
+      if (wi[j] != 0) j++;
+    }
+    if (pos) {
+      int ifst = pos + 1, ilst = i + 1;
+      dtrexc_("V", &n, T, &ld, Q, &ld, &ifst, &ilst, ds->work, &info);
+      if (info) OR_ERR(-76, "trexc info %d", info);
+      for (j = i; j < n; j++) {
+        wr[j] = T[j + (size_t)j * ld];
+        if (j < n - 1 && T[j + 1 + (size_t)j * ld] != 0.0) {
+          wi[j] = sqrt(fabs(T[j + 1 + (size_t)j * ld])) * sqrt(fabs(T[j + (size_t)(j + 1) * ld]));
+          wr[j + 1] = wr[j]; wi[j + 1] = -wi[j]; j++;
+        } else wi[j] = 0.0;
+      }
+    }
+    if (wi[i] != 0) i++;
+  }
+  return 0;
+}
+
+/* DSUpdateExtraRow_NHEP (dsnhep.c:318-340) */
+static void ds_update_extra_row_nhep(ods *ds)
+{
+  int n = ds->n, ld = ds->ld, inc = 1;
+  double one = 1.0, zero = 0.0, *x = ds->work, *y = ds->work + ld;
+  for (int i = 0; i < n; i++) x[i] = ds->A[n + (size_t)i * ld];
+  dgemv_("C", &n, &n, &one, ds->Q, &ld, x, &inc, &zero, y, &inc);
+  for (int i = 0; i < n; i++) ds->A[n + (size_t)i * ld] = y[i];
+  ds->k = n;
+}
+
+/* DSVectors_NHEP_Eigen_Some (dsnhep.c:101-167), right vectors */
+static int ds_vectors_nhep_some(ods *ds, int *k, double *rnorm)
+{
+  int n = ds->n, ld = ds->ld, mm, mout, info, inc = 1, cols = 1, zero = 0, iscomplex = 0;
+  double sone = 1.0, szero = 0.0, norm, done = 1.0;
+  int *select = ds->iwork;
+  double *Y = ds->X + (size_t)(*k) * ld;
+  for (int i = 0; i < n; i++) select[i] = 0;
+  select[*k] = 1;
+  if (*k < n - 1 && ds->A[*k + 1 + (size_t)(*k) * ld] != 0.0) iscomplex = 1;
+  mm = iscomplex ? 2 : 1;
+  if (iscomplex) select[*k + 1] = 1;
+  dtrevc_("R", "S", select, &n, ds->A, &ld, Y, &ld, Y, &ld, &mm, &mout, ds->work, &info);
+  if (info) OR_ERR(-76, "trevc info %d", info);
+  if (mout != mm) OR_ERR(-60, "Inconsistent arguments");
+  if (ds->state >= DS_CONDENSED) {
+    double *w = ds->work;
+    memcpy(w, Y, sizeof(double) * (size_t)mout * ld);
+    dgemv_("N", &n, &n, &sone, ds->Q, &ld, w, &inc, &szero, Y, &inc);
+    if (iscomplex) dgemv_("N", &n, &n, &sone, ds->Q, &ld, w + ld, &inc, &szero, Y + ld, &inc);
+    norm = dnrm2_(&n, Y, &inc);
+    if (iscomplex) { norm = hypot(norm, dnrm2_(&n, Y + ld, &inc)); cols = 2; }
+    dlascl_("G", &zero, &zero, &norm, &done, &n, &cols, Y, &ld, &info);
+    if (info) OR_ERR(-76, "lascl info %d", info);
+  }
+  if (iscomplex) (*k)++;
+  if (rnorm) *rnorm = iscomplex ? hypot(Y[n - 1], Y[n - 1 + ld]) : fabs(Y[n - 1]);
+  return 0;
+}
+
+/* DSVectors_NHEP_Eigen_All (dsnhep.c:168-226), right vectors */
+static int ds_vectors_nhep_all(ods *ds)
+{
+  int n = ds->n, ld = ds->ld, mout, info, inc = 1, cols, zero = 0;
+  double norm, done = 1.0;
+  const char *back = "A";
+  if (ds->state >= DS_CONDENSED) { back = "B"; memcpy(ds->X, ds->Q, sizeof(double) * (size_t)ld * ld); }
+  dtrevc_("R", back, NULL, &n, ds->A, &ld, NULL, &ld, ds->X, &ld, &n, &mout, ds->work, &info);
+  if (info) OR_ERR(-76, "trevc info %d", info);
+  for (int i = 0; i < n; i++) {
+    int iscomplex = (i < n - 1 && ds->A[i + 1 + (size_t)i * ld] != 0.0);
+    cols = 1;
+    norm = dnrm2_(&n, ds->X + (size_t)i * ld, &inc);
+    if (iscomplex) { norm = hypot(norm, dnrm2_(&n, ds->X + (size_t)(i + 1) * ld, &inc)); cols = 2; }
+    dlascl_("G", &zero, &zero, &norm, &done, &n, &cols, ds->X + (size_t)i * ld, &ld, &info);
+    if (info) OR_ERR(-76, "lascl info %d", info);
+    if (iscomplex) i++;
+  }
+  return 0;
+}
+
+/* DSVectors_HEP (dshep.c:137-172) */
+static void ds_vectors_hep(ods *ds, int *j, double *rnorm)
+{
+  int ld = ds->ld;
+  if (j) {
+    if (ds->state >= DS_CONDENSED) {
+      memcpy(ds->X + (size_t)(*j) * ld, ds->Q + (size_t)(*j) * ld, sizeof(double) * ld);
+      if (rnorm) *rnorm = fabs(ds->Q[ds->n - 1 + (size_t)(*j) * ld]);
+    } else {
+      memset(ds->X + (size_t)(*j) * ld, 0, sizeof(double) * ld);
+      ds->X[*j + (size_t)(*j) * ld] = 1.0;
+      if (rnorm) *rnorm = 0.0;
+    }
+  } else {
+    if (ds->state >= DS_CONDENSED) memcpy(ds->X, ds->Q, sizeof(double) * (size_t)ld * ld);
+    else ds_setidentity(ds, ds->X);
+  }
+}
+
+static int ds_solve(ods *ds, double *wr, double *wi)
+{
+  if (ds->state >= DS_CONDENSED) return 0;
+  OR_CALL(ds->type == DS_HEP ? ds_solve_hep(ds, wr, wi) : ds_solve_nhep(ds, wr, wi));
+  ds->state = DS_CONDENSED;
+  return 0;
+}
+
+static int ds_sort(ods *ds, double *wr, double *wi)
+{
+  for (int i = 0; i < ds->n; i++) ds->perm[i] = i;
+  if (ds->type == DS_HEP) { ds_sort_hep(ds, wr); return 0; }
+  return ds_sort_nhep(ds, wr, wi);
+}
+
+static void ds_update_extra_row(ods *ds)
+{
+  if (ds->type == DS_HEP) ds_update_extra_row_hep(ds);
+  else ds_update_extra_row_nhep(ds);
+}
+
+static int ds_vectors(ods *ds, int *j, double *rnorm)
+{
+  if (ds->type == DS_HEP) { ds_vectors_hep(ds, j, rnorm); return 0; }
+  return j ? ds_vectors_nhep_some(ds, j, rnorm) : ds_vectors_nhep_all(ds);
+}
+
+/* DSGetTruncateSize_Default (dsops.c:329-344) */
+static void ds_truncate_size(ods *ds, int l, int n, int *k)
+{
+  if (ds->type == DS_HEP) return;
+  double val = ds->A[(l + *k) + (size_t)(l + *k - 1) * ds->ld];
+  if (val != 0.0) {
+    if (l + *k < n - 1) (*k)++;
+    else (*k)--;
+  }
+}
+
+/* DSTruncate_HEP / DSTruncate_NHEP (dshep.c:643-670, dsnhep.c:394-425) */
+static void ds_truncate(ods *ds, int n, int trim)
+{
+  int ld = ds->ld, l = ds->l;
+  int fullA = (ds->type == DS_NHEP || !ds->compact) && ds->extrarow;
+  if (trim) {
+    if (fullA) for (int i = l; i < ds->n; i++) ds->A[ds->n + (size_t)i * ld] = 0.0;
+    ds->l = 0; ds->k = 0; ds->n = n; ds->t = n;
+  } else {
+    if (fullA && ds->k == ds->n) {
+      for (int i = l; i < n; i++) ds->A[n + (size_t)i * ld] = ds->A[ds->n + (size_t)i * ld];
+      for (int i = l; i < ds->n; i++) ds->A[ds->n + (size_t)i * ld] = 0.0;
+    }
+    ds->k = ds->extrarow ? n : 0;
+    ds->t = ds->n;
+    ds->n = n;
+  }
+  ds->state = trim ? DS_RAW : DS_TRUNCATED;
+}
+
+int or_ds_run(int type, int ld, int n, int l, int k, int compact, int extrarow,
+              int state, double *A, double *T, int which, double *wr, double *wi,
+              double *Q, int jvec, double *rnorm, double *X)
+{
+  ods ds;
+  double *wi_local = NULL;
+  if (ds_alloc(&ds, type, ld)) return -1;
+  ds.compact = compact; ds.extrarow = extrarow; ds.state = state;
+  ds.sc.which = which;
+  ds_setdims(&ds, n, l, k);
+  if (A) memcpy(ds.A, A, sizeof(double) * (size_t)ld * ld);
+  if (T) memcpy(ds.T, T, sizeof(double) * (size_t)ld * 3);
+  if (!wi) { wi_local = (double*)calloc((size_t)ld, sizeof(double)); wi = wi_local; }
+  int e = ds_solve(&ds, wr, wi);
+  if (!e) e = ds_sort(&ds, wr, wi);
+  if (!e && extrarow) ds_update_extra_row(&ds);
+  if (!e && jvec >= 0) { int j = jvec; e = ds_vectors(&ds, &j, rnorm); }
+  if (!e) e = ds_vectors(&ds, NULL, NULL);
+  if (!e) {
+    if (A) memcpy(A, ds.A, sizeof(double) * (size_t)ld * ld);
+    if (T) memcpy(T, ds.T, sizeof(double) * (size_t)ld * 3);
+    if (Q) memcpy(Q, ds.Q, sizeof(double) * (size_t)ld * ld);
+    if (X) memcpy(X, ds.X, sizeof(double) * (size_t)ld * ld);
+  }
+  free(wi_local);
+  ds_free(&ds);
+  return e;
+}
+
+/* ------------------------------------------------------------------ EPS + Krylov-Schur */
+typedef struct {
+  int n, nev, ncv, mpd, max_it, which, hermitian, lock, nconv, its, reason;
+  double tol, target, keep;
+  const int *rowptr, *colidx;
+  const double *val;
+  obv V;
+  ods ds;
+  double *eigr, *eigi, *errest;
+  int *perm;
+  or_sc sc;
+  long nmatmult;
+  double t_ds;
+} oeps;
+
+static void op_mult(oeps *eps, const double *x, double *y)
+{
+  or_spmv(eps->n, eps->rowptr, eps->colidx, eps->val, x, y);
+  eps->nmatmult++;
+}
+
+/* BVMatArnoldi / BVMatLanczos (bvkrylov.c:56-226) */
+static int mat_krylov(oeps *eps, int k, int *m, double *beta, int *breakdown)
+{
+  obv *bv = &eps->V;
+  ods *ds = &eps->ds;
+  int lindep = 0, ldb = bv->nc + bv->m, ld = ds->ld;
+  const double *a = bv->buffer;
+  for (int j = k; j < *m; j++) {
+    op_mult(eps, COL(bv, j), COL(bv, j + 1));                     /* BVMatMultColumn */
+    if (j == eps->n - 1) {                                         /* BV_OrthogonalizeColumn_Safe */
+      int save = bv->refine;
+      bv->refine = OR_REFINE_NEVER;
+      OR_CALL(orthogonalize_column(bv, j + 1, NULL, NULL));
+      bv->refine = save;
+      *beta = 0.0; lindep = 1;
+    } else OR_CALL(orthonormalize_column(bv, j + 1, beta, &lindep));
+    if (lindep) { *m = j + 1; break; }
+  }
+  *breakdown = lindep;
+  if (eps->hermitian) {
+    double *alpha = ds->T, *betat = ds->T + ld;
+    for (int j = k; j < *m; j++) {
+      alpha[j] = a[bv->nc + j + (size_t)(j + 1) * ldb];
+      betat[j] = a[bv->nc + j + 1 + (size_t)(j + 1) * ldb];
+    }
+  } else {
+    double *h = ds->A;
+    for (int j = k; j < *m - 1; j++) memcpy(h + (size_t)j * ld, a + bv->nc + (size_t)(j + 1) * ldb, sizeof(double) * (j + 2));
+    memcpy(h + (size_t)(*m - 1) * ld, a + bv->nc + (size_t)(*m) * ldb, sizeof(double) * (*m));
+    if (ld > *m) h[*m + (size_t)(*m - 1) * ld] = a[bv->nc + *m + (size_t)(*m) * ldb];
+  }
+  return 0;
+}
+
+/* EPSGetStartVector (epssolve.c:841-873) */
+static int get_start_vector(oeps *eps, int i, int have_initial, int *breakdown)
+{
+  double norm;
+  int lindep = 0;
+  if (i > 0 || !have_initial) bv_randomcol(&eps->V, i);
+  OR_CALL(orthogonalize_column(&eps->V, i, &norm, &lindep));
+  if (breakdown) *breakdown = lindep;
+  else if (lindep || norm == 0.0) {
+    if (!i) OR_ERR(-77, "Initial vector is zero or belongs to the deflation space");
+    OR_ERR(-80, "Unable to generate more start vectors");
+  }
+  bv_scalecol(&eps->V, i, 1.0 / norm);
+  return 0;
+}
+
+/* EPSKrylovConvergence (epskrylov.c:207-295), STSHIFT sigma=0, trivial RG, relative test */
+static int krylov_convergence(oeps *eps, int kini, int nits, double beta, int *kout)
+{
+  int k, marker = -1;
+  double resnorm;
+  for (k = kini; k < kini + nits; k++) {
+    double re = eps->eigr[k], im = eps->eigi[k], w;
+    int newk = k;
+    OR_CALL(ds_vectors(&eps->ds, &newk, &resnorm));
+    resnorm *= beta;                                               /* corrf = 1 */
+    w = hypot(re, im);                                             /* EPSConvergedRelative */
+    eps->errest[k] = (w != 0.0) ? resnorm / w : DBL_MAX;
+    if (marker == -1 && eps->errest[k] >= eps->tol) marker = k;
+    if (newk == k + 1) { eps->errest[k + 1] = eps->errest[k]; k++; }
+    if (marker != -1) break;
+  }
+  if (marker != -1) k = marker;
+  *kout = k;
+  return 0;
+}
+
+/* EPSSolve_KrylovSchur_Default (krylovschur.c:227-337), Ritz extraction */
+static int ks_solve(oeps *eps, int have_initial, int restart_cap)
+{
+  int k = 0, l = 0, nv, nconv, breakdown = 0;
+  double beta = 0.0, t0;
+  obv *V = &eps->V;
+  ods *ds = &eps->ds;
+
+  OR_CALL(get_start_vector(eps, 0, have_initial, NULL));
+  l = 0;
+  while (eps->reason == OR_CONVERGED_ITERATING) {
+    eps->its++;
+    nv = eps->nconv + eps->mpd < eps->ncv ? eps->nconv + eps->mpd : eps->ncv;
+    ds_setdims(ds, nv, eps->nconv, eps->nconv + l);
+    OR_CALL(mat_krylov(eps, eps->nconv + l, &nv, &beta, &breakdown));
+    ds_setdims(ds, nv, eps->nconv, eps->nconv + l);
+    ds->state = l ? DS_RAW : DS_INTERMEDIATE;
+    V->l = eps->nconv; V->k = nv;                                  /* BVSetActiveColumns */
+
+    t0 = wtime();
+    OR_CALL(ds_solve(ds, eps->eigr, eps->eigi));
+    OR_CALL(ds_sort(ds, eps->eigr, eps->eigi));
+    ds_update_extra_row(ds);
+    OR_CALL(krylov_convergence(eps, eps->nconv, nv - eps->nconv, beta, &k));
+    eps->t_ds += wtime() - t0;
+
+    /* EPSStoppingBasic (epsdefault.c:290-302) */
+    if (k >= eps->nev) eps->reason = OR_CONVERGED_TOL;
+    else if (eps->its >= eps->max_it) eps->reason = OR_DIVERGED_ITS;
+    if (restart_cap > 0 && eps->its >= restart_cap && eps->reason == OR_CONVERGED_ITERATING)
+      eps->reason = OR_DIVERGED_ITS;
+    nconv = k;
+
+    if (eps->reason != OR_CONVERGED_ITERATING || breakdown || k == nv) l = 0;
+    else {
+      l = (int)((nv - k) * eps->keep); if (l < 1) l = 1;
+      if (!eps->hermitian) ds_truncate_size(ds, k, nv, &l);
+    }
+    if (!eps->lock && l > 0) { l += k; k = 0; }
+
+    if (eps->reason == OR_CONVERGED_ITERATING) {
+      if (breakdown || k == nv) {
+        if (k < eps->nev) {
+          OR_CALL(get_start_vector(eps, k, have_initial, &breakdown));
+          if (breakdown) eps->reason = OR_DIVERGED_BREAKDOWN;
+        }
+      } else ds_truncate(ds, k + l, 0);
+    }
+    /* BVMultInPlace(V, Q, nconv, k+l) */
+    bv_multinplace(V, ds->Q, ds->ld, eps->nconv, k + l);
+    if (eps->reason == OR_CONVERGED_ITERATING && !breakdown) bv_copycol(V, nv, k + l);
+    eps->nconv = k;
+    (void)nconv;
+  }
+  ds_truncate(ds, eps->nconv, 1);
+  return 0;
+}
+
+/* EPSComputeResidualNorm_Private (epssolve.c:666-718), standard problem */
+static double residual_norm(oeps *eps, double kr, double ki, const double *xr, const double *xi,
+                            double *u, double *w)
+{
+  int n = eps->n, one = 1;
+  if (ki == 0 || fabs(ki) < fabs(kr * MACH_EPS)) {
+    op_mult(eps, xr, u);
+    if (fabs(kr) > MACH_EPS) for (int i = 0; i < n; i++) u[i] -= kr * xr[i];
+    return dnrm2_(&n, u, &one);
+  } else {
+    double nr, ni;
+    op_mult(eps, xr, u);
+    if (hypot(kr, ki) > MACH_EPS) for (int i = 0; i < n; i++) u[i] += -kr * xr[i] + ki * xi[i];
+    nr = dnrm2_(&n, u, &one);
+    op_mult(eps, xi, u);
+    if (hypot(kr, ki) > MACH_EPS) for (int i = 0; i < n; i++) u[i] += -kr * xi[i] - ki * xr[i];
+    ni = dnrm2_(&n, u, &one);
+    (void)w;
+    return hypot(nr, ni);
+  }
+}
+
+int or_eps_solve(int n, const int *rowptr, const int *colidx, const double *val,
+                 const or_opts *opts, const double *v0, int nds, const double *defl,
+                 double *eigr, double *eigi, double *errest, double *err, double *X,
+                 or_info *info)
+{
+  oeps eps;
+  int e = 0, ncv, mpd, requested;
+  double t0;
+  memset(&eps, 0, sizeof(eps));
+  if (opts->nthreads > 0) omp_set_num_threads(opts->nthreads);
+
+  eps.n = n; eps.rowptr = rowptr; eps.colidx = colidx; eps.val = val;
+  eps.nev = opts->nev; eps.which = opts->which ? opts->which : OR_LARGEST_MAGNITUDE;
+  eps.target = opts->target; eps.hermitian = opts->hermitian;
+  eps.tol = opts->tol > 0 ? opts->tol : 1e-8;
+  eps.keep = opts->keep > 0 ? opts->keep : 0.5;
+  eps.lock = opts->lock;
+  eps.sc.which = eps.which; eps.sc.target = eps.target;
+
+  /* EPSSetUp (epssetup.c:344-345) and EPSSetDimensions_Default (epssetup.c:648-671) */
+  if (eps.nev > n) eps.nev = n;
+  ncv = opts->ncv > 0 ? opts->ncv : -1;
+  mpd = opts->mpd > 0 ? opts->mpd : -1;
+  if (ncv > n) ncv = n;
+  if (ncv > 0) {
+    if (!(ncv >= eps.nev + 1 || (ncv == eps.nev && ncv == n))) OR_ERR(-62, "The value of ncv must be at least nev+1");
+  } else if (mpd > 0) ncv = n < eps.nev + mpd ? n : eps.nev + mpd;
+  else ncv = n < (2 * eps.nev > eps.nev + 15 ? 2 * eps.nev : eps.nev + 15) ? n : (2 * eps.nev > eps.nev + 15 ? 2 * eps.nev : eps.nev + 15);
+  if (mpd <= 0) mpd = ncv;
+  if (ncv > eps.nev + mpd) OR_ERR(-62, "The value of ncv must not be larger than nev+mpd");
+  eps.ncv = ncv; eps.mpd = mpd;
+  eps.max_it = opts->max_it > 0 ? opts->max_it : (100 > 2 * n / ncv ? 100 : 2 * n / ncv);
+
+  /* EPSAllocateSolution(eps, 1) (epssetup.c:692-732) */
+  requested = ncv + 1;
+  eps.V.n = n; eps.V.m = requested; eps.V.nc = nds > 0 ? nds : 0; eps.V.ld = n;
+  eps.V.refine = opts->refine; eps.V.eta = opts->eta > 0 ? opts->eta : 0.7071;
+  eps.V.seed = 0;
+  eps.V.V = (double*)calloc((size_t)n * (eps.V.nc + requested), sizeof(double));
+  eps.V.buffer = (double*)calloc((size_t)(eps.V.nc + requested) * requested, sizeof(double));
+  eps.eigr = (double*)calloc(requested, sizeof(double));
+  eps.eigi = (double*)calloc(requested, sizeof(double));
+  eps.errest = (double*)calloc(requested, sizeof(double));
+  eps.perm = (int*)calloc(requested, sizeof(int));
+  if (!eps.V.V || !eps.V.buffer || !eps.eigr || !eps.eigi || !eps.errest || !eps.perm) { e = -55; goto done; }
+  if ((e = ds_alloc(&eps.ds, eps.hermitian ? DS_HEP : DS_NHEP, ncv + 1))) goto done;
+  eps.ds.compact = eps.hermitian; eps.ds.extrarow = 1; eps.ds.sc = eps.sc;
+
+  /* deflation space: BVInsertConstraints (orthonormalized, bvfunc.c:411) */
+  if (eps.V.nc) {
+    obv *bv = &eps.V;
+    int nc = bv->nc;
+    memcpy(bv->V, defl, sizeof(double) * (size_t)n * nc);
+    bv->nc = 0; bv->m = nc;               /* orthonormalize constraints among themselves */
+    for (int j = 0; j < nc; j++) {
+      double nrm; int lnd;
+      bv->l = 0; bv->k = j;
+      if ((e = orthonormalize_column(bv, j, &nrm, &lnd))) goto done;
+    }
+    bv->nc = nc; bv->m = requested;
+    memset(bv->buffer, 0, sizeof(double) * (size_t)(nc + requested) * requested);
+  }
+  /* initial space (one vector): BVInsertVecs(V,0,&k,IS,TRUE) (bvfunc.c:358-371) */
+  if (v0) {
+    double nrm; int lnd = 0;
+    memcpy(COL(&eps.V, 0), v0, sizeof(double) * (size_t)n);
+    eps.V.l = 0; eps.V.k = 0;
+    if ((e = orthogonalize_column(&eps.V, 0, &nrm, &lnd))) goto done;
+    if (nrm == 0.0 || lnd) { e = -77; fprintf(stderr, "ks_oracle: initial vector is zero or in the deflation space\n"); goto done; }
+    bv_scalecol(&eps.V, 0, 1.0 / nrm);
+  }
+
+  for (int i = 0; i < ncv; i++) { eps.eigr[i] = eps.eigi[i] = eps.errest[i] = 0.0; eps.perm[i] = i; }
+  eps.reason = OR_CONVERGED_ITERATING;
+  eps.V.l = 0; eps.V.k = 0;
+
+  t0 = wtime();
+  if ((e = ks_solve(&eps, v0 != NULL, opts->restart_cap))) goto done;
+
+  /* EPSComputeVectors_Schur (epsdefault.c:105-169) for NHEP; Hermitian: no-op */
+  if (!eps.hermitian && eps.nconv > 0) {
+    if ((e = ds_vectors(&eps.ds, NULL, NULL))) goto done;
+    eps.V.l = 0; eps.V.k = eps.nconv;
+    bv_multinplace(&eps.V, eps.ds.X, eps.ds.ld, 0, eps.nconv);
+  }
+  /* reorder conjugate pairs (epssolve.c:161-175) */
+  for (int i = 0; i < eps.nconv - 1; i++) {
+    if (eps.eigi[i] != 0) {
+      if (eps.eigi[i] < 0) {
+        eps.eigi[i] = -eps.eigi[i]; eps.eigi[i + 1] = -eps.eigi[i + 1];
+        bv_scalecol(&eps.V, i + 1, -1.0);
+      }
+      i++;
+    }
+  }
+  sort_eigenvalues(&eps.sc, eps.nconv, eps.eigr, eps.eigi, eps.perm);
+  if (info) info->t_solve = wtime() - t0;
+
+  /* outputs in EPSGetEigenpair order: eigenvalue perm[i], vector BV_GetEigenvector (bvimpl.h:421-444) */
+  {
+    double *u = (double*)malloc(sizeof(double) * (size_t)n), *zero = NULL;
+    for (int i = 0; i < eps.nconv; i++) {
+      int k = eps.perm[i];
+      double kr = eps.eigr[k], ki = eps.eigi[k];
+      const double *xr, *xi = NULL;
+      if (ki > 0) { xr = COL(&eps.V, k); xi = COL(&eps.V, k + 1); }
+      else if (ki < 0) { xr = COL(&eps.V, k - 1); xi = COL(&eps.V, k); }
+      else xr = COL(&eps.V, k);
+      eigr[i] = kr; eigi[i] = ki; errest[i] = eps.errest[k];
+      if (X) {
+        if (ki < 0) for (int r = 0; r < n; r++) X[(size_t)i * n + r] = -xi[r];   /* imaginary part, negated */
+        else if (ki > 0) memcpy(X + (size_t)i * n, xr, sizeof(double) * (size_t)n);
+        else memcpy(X + (size_t)i * n, xr, sizeof(double) * (size_t)n);
+      }
+      if (err) {
+        double r;
+        if (ki == 0) r = residual_norm(&eps, kr, 0.0, xr, NULL, u, NULL);
+        else if (ki > 0) r = residual_norm(&eps, kr, ki, xr, xi, u, NULL);
+        else {   /* second of the pair: xi = -V(:,k) */
+          double *nxi = (double*)malloc(sizeof(double) * (size_t)n);
+          for (int q = 0; q < n; q++) nxi[q] = -xi[q];
+          r = residual_norm(&eps, kr, ki, xr, nxi, u, NULL);
+          free(nxi);
+        }
+        err[i] = r / hypot(kr, ki);
+      }
+    }
+    free(u); (void)zero;
+  }
+  if (info) {
+    info->nconv = eps.nconv; info->its = eps.its; info->reason = eps.reason;
+    info->nmatmult = eps.nmatmult; info->gs_passes = eps.V.passes; info->t_ds = eps.t_ds;
+  }
+done:
+  free(eps.V.V); free(eps.V.buffer); free(eps.eigr); free(eps.eigi); free(eps.errest); free(eps.perm);
+  if (eps.ds.A) ds_free(&eps.ds);
+  return e;
+}

@@ -1,0 +1,157 @@
+"""ctypes wrapper of libks_oracle.so -- the CPU restatement of SLEPc's
+Krylov-Schur path.
+
+TEST INFRASTRUCTURE ONLY (the checker): imported by tests/, by
+``__graft_entry__.smoke()`` and by bench.py's cpu_baseline leg, never by the
+product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libks_oracle.so")
+
+LARGEST_MAGNITUDE, SMALLEST_MAGNITUDE, LARGEST_REAL, SMALLEST_REAL = 1, 2, 3, 4
+LARGEST_IMAGINARY, SMALLEST_IMAGINARY, TARGET_MAGNITUDE, TARGET_REAL = 5, 6, 7, 8
+REFINE_IFNEEDED, REFINE_NEVER, REFINE_ALWAYS = 0, 1, 2
+
+
+class Opts(C.Structure):
+    _fields_ = [("nev", C.c_int), ("ncv", C.c_int), ("mpd", C.c_int), ("max_it", C.c_int),
+                ("tol", C.c_double), ("which", C.c_int), ("target", C.c_double),
+                ("hermitian", C.c_int), ("keep", C.c_double), ("lock", C.c_int),
+                ("eta", C.c_double), ("refine", C.c_int), ("nthreads", C.c_int),
+                ("restart_cap", C.c_int)]
+
+
+class Info(C.Structure):
+    _fields_ = [("nconv", C.c_int), ("its", C.c_int), ("reason", C.c_int),
+                ("nmatmult", C.c_long), ("gs_passes", C.c_long),
+                ("t_solve", C.c_double), ("t_ds", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (run `make -C oracle`)")
+        _lib = C.CDLL(LIB_PATH)
+    return _lib
+
+
+def _p(a, ct=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ct))
+
+
+def _csr_args(csr):
+    rowptr, colidx, val = csr
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+    colidx = np.ascontiguousarray(colidx, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    return rowptr, colidx, val
+
+
+def random_uniform(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n)
+    lib().or_random_uniform(C.c_ulonglong(seed), C.c_int(n), _p(out))
+    return out
+
+
+def spmv(csr, x: np.ndarray) -> np.ndarray:
+    rowptr, colidx, val = _csr_args(csr)
+    n = len(rowptr) - 1
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty(n)
+    rc = lib().or_spmv(C.c_int(n), _p(rowptr, C.c_int), _p(colidx, C.c_int), _p(val), _p(x), _p(y))
+    assert rc == 0
+    return y
+
+
+def orthonormalize_column(V: np.ndarray, j: int, nc: int = 0, refine: int = REFINE_IFNEEDED,
+                          eta: float = 0.7071):
+    """V is (n, ncols) float64 Fortran-ordered, modified in place."""
+    assert V.flags.f_contiguous and V.dtype == np.float64
+    n = V.shape[0]
+    H = np.zeros(nc + j + 1)
+    norm = C.c_double()
+    lindep = C.c_int()
+    npasses = C.c_int()
+    rc = lib().or_orthonormalize_column(C.c_int(n), C.c_int(n), _p(V), C.c_int(nc), C.c_int(j),
+                                        C.c_int(refine), C.c_double(eta), _p(H), C.byref(norm),
+                                        C.byref(lindep), C.byref(npasses))
+    if rc:
+        raise RuntimeError(f"or_orthonormalize_column failed ({rc})")
+    return H, norm.value, bool(lindep.value), npasses.value
+
+
+def multinplace(V: np.ndarray, Q: np.ndarray, l: int, k: int, s: int, e: int, nc: int = 0):
+    assert V.flags.f_contiguous and Q.flags.f_contiguous
+    n = V.shape[0]
+    rc = lib().or_multinplace(C.c_int(n), C.c_int(n), _p(V), C.c_int(nc), C.c_int(l), C.c_int(k),
+                              _p(Q), C.c_int(Q.shape[0]), C.c_int(s), C.c_int(e))
+    assert rc == 0
+
+
+def ds_run(dstype: str, ld: int, n: int, l: int = 0, k: int = 0, compact: bool = False,
+           extrarow: bool = False, state: int = 0, A=None, T=None, which: int = LARGEST_MAGNITUDE,
+           jvec: int = -1):
+    """DSSolve + DSSort + [DSUpdateExtraRow] + DSVectors; returns a dict."""
+    t = 1 if dstype == "hep" else 2
+    A = None if A is None else np.array(A, dtype=np.float64, order="F", copy=True)
+    T = None if T is None else np.array(T, dtype=np.float64, order="F", copy=True)
+    wr = np.zeros(ld)
+    wi = np.zeros(ld)
+    Q = np.zeros((ld, ld), order="F")
+    X = np.zeros((ld, ld), order="F")
+    rnorm = C.c_double(0.0)
+    rc = lib().or_ds_run(C.c_int(t), C.c_int(ld), C.c_int(n), C.c_int(l), C.c_int(k),
+                         C.c_int(int(compact)), C.c_int(int(extrarow)), C.c_int(state),
+                         _p(A), _p(T), C.c_int(which), _p(wr), _p(wi), _p(Q), C.c_int(jvec),
+                         C.byref(rnorm), _p(X))
+    if rc:
+        raise RuntimeError(f"or_ds_run failed ({rc})")
+    return dict(wr=wr[:n], wi=wi[:n], A=A, T=T, Q=Q, X=X, rnorm=rnorm.value)
+
+
+def eps_solve(csr, nev: int, ncv: int = 0, mpd: int = 0, tol: float = 1e-8,
+              which: int = LARGEST_MAGNITUDE, hermitian: bool = True, target: float = 0.0,
+              v0=None, deflation=None, max_it: int = 0, keep: float = 0.5, lock: bool = True,
+              eta: float = 0.7071, refine: int = REFINE_IFNEEDED, nthreads: int = 0,
+              restart_cap: int = 0, want_vectors: bool = True):
+    rowptr, colidx, val = _csr_args(csr)
+    n = len(rowptr) - 1
+    o = Opts(nev=nev, ncv=ncv, mpd=mpd, max_it=max_it, tol=tol, which=which, target=target,
+             hermitian=int(hermitian), keep=keep, lock=int(lock), eta=eta, refine=refine,
+             nthreads=nthreads, restart_cap=restart_cap)
+    ncv_eff = ncv if ncv > 0 else max(2 * nev, nev + 15)
+    ncv_eff = min(ncv_eff, n) if ncv_eff > 0 else n
+    size = max(ncv_eff, nev) + 2
+    eigr = np.zeros(size)
+    eigi = np.zeros(size)
+    errest = np.zeros(size)
+    err = np.zeros(size)
+    X = np.zeros((n, size), order="F") if want_vectors else None
+    v0a = None if v0 is None else np.ascontiguousarray(v0, dtype=np.float64)
+    nds = 0
+    defl = None
+    if deflation is not None:
+        defl = np.asfortranarray(deflation, dtype=np.float64)
+        nds = defl.shape[1]
+    info = Info()
+    rc = lib().or_eps_solve(C.c_int(n), _p(rowptr, C.c_int), _p(colidx, C.c_int), _p(val),
+                            C.byref(o), _p(v0a), C.c_int(nds), _p(defl), _p(eigr), _p(eigi),
+                            _p(errest), _p(err), _p(X), C.byref(info))
+    if rc:
+        raise RuntimeError(f"or_eps_solve failed ({rc})")
+    k = info.nconv
+    return dict(nconv=k, its=info.its, reason=info.reason, eigr=eigr[:k].copy(),
+                eigi=eigi[:k].copy(), errest=errest[:k].copy(), err=err[:k].copy(),
+                X=None if X is None else X[:, :k].copy(), nmatmult=info.nmatmult,
+                gs_passes=info.gs_passes, t_solve=info.t_solve, t_ds=info.t_ds)

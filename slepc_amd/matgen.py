@@ -1,0 +1,163 @@
+"""Seeded synthetic CSR matrices for the Krylov-Schur workloads.
+
+Every generator returns ``(rowptr, colidx, val)`` as numpy arrays
+(int32, int32, float64) in PETSc SeqAIJ layout: rows in order, column
+indices strictly increasing inside a row.  Generators are deterministic for a
+given seed on any machine (numpy ``default_rng`` / PCG64).
+
+* ``laplacian_1d``      -- the 1-D Laplacian of src/eps/tests/test2.c:40-46.
+* ``laplacian_2d``      -- the 2-D 5-point Laplacian of src/eps/tutorials/ex2.c:44-51
+                           (row II = i*n + j, stencil [-1,-1,4,-1,-1], Dirichlet).
+* ``varcoef_laplacian_3d`` -- 3-D 7-point variable-coefficient Laplacian,
+                           lognormal(0,1) face coefficients, diagonal = sum of the
+                           cell's six face coefficients (boundary faces included,
+                           which makes the matrix SPD).
+* ``convection_diffusion_2d`` -- nonsymmetric 5-point upwind convection-diffusion.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _stencil_to_csr(cols: np.ndarray, vals: np.ndarray, mask: np.ndarray):
+    """cols/vals/mask are (n, s) arrays with each row's entries already in
+    increasing column order; masked-out entries are dropped."""
+    counts = mask.sum(axis=1, dtype=np.int64)
+    rowptr = np.zeros(cols.shape[0] + 1, dtype=np.int64)
+    np.cumsum(counts, out=rowptr[1:])
+    if rowptr[-1] > np.iinfo(np.int32).max:
+        raise ValueError("nnz exceeds 32-bit index range")
+    return (rowptr.astype(np.int32), cols[mask].astype(np.int32),
+            np.ascontiguousarray(vals[mask], dtype=np.float64))
+
+
+def laplacian_1d(n: int):
+    i = np.arange(n, dtype=np.int64)
+    cols = np.stack([i - 1, i, i + 1], axis=1)
+    vals = np.broadcast_to(np.array([-1.0, 2.0, -1.0]), cols.shape).copy()
+    mask = (cols >= 0) & (cols < n)
+    return _stencil_to_csr(cols, vals, mask)
+
+
+def laplacian_2d(n: int, m: int | None = None):
+    """ex2.c ordering: II = i*n + j, i in [0,m), j in [0,n)."""
+    m = n if m is None else m
+    N = n * m
+    II = np.arange(N, dtype=np.int64)
+    i = II // n
+    j = II - i * n
+    cols = np.stack([II - n, II - 1, II, II + 1, II + n], axis=1)
+    mask = np.stack([i > 0, j > 0, np.ones(N, bool), j < n - 1, i < m - 1], axis=1)
+    vals = np.broadcast_to(np.array([-1.0, -1.0, 4.0, -1.0, -1.0]), cols.shape).copy()
+    return _stencil_to_csr(cols, vals, mask)
+
+
+def laplacian_2d_eigenvalues(n: int, m: int | None = None) -> np.ndarray:
+    """All eigenvalues of laplacian_2d (closed form), descending."""
+    m = n if m is None else m
+    p = np.arange(1, n + 1)
+    q = np.arange(1, m + 1)
+    lp = 4.0 * np.sin(p * np.pi / (2 * (n + 1))) ** 2
+    lq = 4.0 * np.sin(q * np.pi / (2 * (m + 1))) ** 2
+    return np.sort((lp[:, None] + lq[None, :]).ravel())[::-1]
+
+
+def varcoef_laplacian_3d(N: int, seed: int = 1):
+    """Cell (x,y,z) has row index x + N*(y + N*z).  Face coefficients are
+    drawn in the order ax (z,y,x-face), ay (z,y-face,x), az (z-face,y,x),
+    each lognormal(0,1) from default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    ax = rng.lognormal(0.0, 1.0, size=(N, N, N + 1))
+    ay = rng.lognormal(0.0, 1.0, size=(N, N + 1, N))
+    az = rng.lognormal(0.0, 1.0, size=(N + 1, N, N))
+    diag = (ax[:, :, :-1] + ax[:, :, 1:] + ay[:, :-1, :] + ay[:, 1:, :]
+            + az[:-1, :, :] + az[1:, :, :]).ravel()
+    n = N ** 3
+    idx = np.arange(n, dtype=np.int64)
+    x = idx % N
+    y = (idx // N) % N
+    z = idx // (N * N)
+    NN = N * N
+    cols = np.stack([idx - NN, idx - N, idx - 1, idx, idx + 1, idx + N, idx + NN], axis=1)
+    vals = np.empty((n, 7), dtype=np.float64)
+    vals[:, 0] = -az[:-1, :, :].ravel()       # face between z-1 and z
+    vals[:, 1] = -ay[:, :-1, :].ravel()
+    vals[:, 2] = -ax[:, :, :-1].ravel()
+    vals[:, 3] = diag
+    vals[:, 4] = -ax[:, :, 1:].ravel()
+    vals[:, 5] = -ay[:, 1:, :].ravel()
+    vals[:, 6] = -az[1:, :, :].ravel()
+    mask = np.stack([z > 0, y > 0, x > 0, np.ones(n, bool), x < N - 1, y < N - 1, z < N - 1], axis=1)
+    del ax, ay, az, diag
+    return _stencil_to_csr(cols, vals, mask)
+
+
+def convection_diffusion_2d(N: int, peclet: float = 10.0, seed: int = 2):
+    """-div(k grad u) + w . grad u on the unit square, N x N interior cells,
+    Dirichlet boundary, first-order upwind convection with constant velocity
+    w = (1, 1) scaled so that the grid Peclet number |w| h / (2 k_mean) equals
+    ``peclet``; k is uniform(0.5, 2) per cell (default_rng(seed)); face
+    diffusion is the harmonic mean of the two cells.  Row x + N*y."""
+    rng = np.random.default_rng(seed)
+    k = rng.uniform(0.5, 2.0, size=(N, N))          # [y, x]
+    h = 1.0 / (N + 1)
+    kmean = 1.25
+    wmag = peclet * 2.0 * kmean / h
+    wx = wy = wmag / np.sqrt(2.0)
+    # face diffusion coefficients (harmonic mean; boundary faces use the cell value)
+    kx = np.empty((N, N + 1))
+    kx[:, 1:-1] = 2.0 * k[:, :-1] * k[:, 1:] / (k[:, :-1] + k[:, 1:])
+    kx[:, 0] = k[:, 0]
+    kx[:, -1] = k[:, -1]
+    ky = np.empty((N + 1, N))
+    ky[1:-1, :] = 2.0 * k[:-1, :] * k[1:, :] / (k[:-1, :] + k[1:, :])
+    ky[0, :] = k[0, :]
+    ky[-1, :] = k[-1, :]
+    inv_h2 = 1.0 / (h * h)
+    west = -(kx[:, :-1] * inv_h2) - wx / h      # upwind for wx > 0 takes u_{x-1}
+    east = -(kx[:, 1:] * inv_h2)
+    south = -(ky[:-1, :] * inv_h2) - wy / h
+    north = -(ky[1:, :] * inv_h2)
+    diag = (kx[:, :-1] + kx[:, 1:] + ky[:-1, :] + ky[1:, :]) * inv_h2 + (wx + wy) / h
+    n = N * N
+    idx = np.arange(n, dtype=np.int64)
+    x = idx % N
+    y = idx // N
+    cols = np.stack([idx - N, idx - 1, idx, idx + 1, idx + N], axis=1)
+    vals = np.stack([south.ravel(), west.ravel(), diag.ravel(), east.ravel(), north.ravel()], axis=1)
+    mask = np.stack([y > 0, x > 0, np.ones(n, bool), x < N - 1, y < N - 1], axis=1)
+    return _stencil_to_csr(cols, vals, mask)
+
+
+def csr_to_dense(rowptr, colidx, val, n: int | None = None) -> np.ndarray:
+    n = len(rowptr) - 1 if n is None else n
+    A = np.zeros((len(rowptr) - 1, n))
+    rows = np.repeat(np.arange(len(rowptr) - 1), np.diff(rowptr))
+    A[rows, colidx] = val
+    return A
+
+
+def random_sparse_symmetric(n: int, density: float, seed: int):
+    """Small symmetric sparse test matrix with a dominant diagonal."""
+    rng = np.random.default_rng(seed)
+    M = (rng.random((n, n)) < density) * rng.standard_normal((n, n))
+    M = np.triu(M, 1)
+    M = M + M.T + np.diag(rng.uniform(-3, 3, n))
+    return dense_to_csr(M)
+
+
+def random_sparse_nonsymmetric(n: int, density: float, seed: int):
+    rng = np.random.default_rng(seed)
+    M = (rng.random((n, n)) < density) * rng.standard_normal((n, n))
+    M = M + np.diag(rng.uniform(-3, 3, n))
+    return dense_to_csr(M)
+
+
+def dense_to_csr(M: np.ndarray):
+    mask = M != 0
+    n = M.shape[0]
+    counts = mask.sum(axis=1)
+    rowptr = np.zeros(n + 1, dtype=np.int32)
+    np.cumsum(counts, out=rowptr[1:])
+    cols = np.nonzero(mask)[1].astype(np.int32)
+    return rowptr, cols, np.ascontiguousarray(M[mask], dtype=np.float64)
