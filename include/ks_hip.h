@@ -1,0 +1,185 @@
+/*
+ * ks_hip.h -- C ABI of libks_hip.so, the MI355X (gfx950) kernels of the
+ * Krylov-Schur hot path.  Plain pointers and sizes only; every call returns
+ * 0 (KS_OK) or a negative KS_ERR_* code.  Device pointers are HIP device
+ * memory obtained from ks_malloc (or any hipMalloc'd buffer).  All work is
+ * stream-ordered on the context's HIP stream; calls that return host values
+ * synchronize that stream.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the SLEPc 3.22 source tree; PETSc functions are outside it).
+ */
+#ifndef KS_HIP_H
+#define KS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  KS_OK = 0,
+  KS_ERR_ARG = -1,        /* bad argument (sizes, null pointers, out of range) */
+  KS_ERR_HIP = -2,        /* HIP runtime error */
+  KS_ERR_NOMEM = -3,      /* device allocation failed */
+  KS_ERR_NODEV = -4,      /* no gfx950 device visible */
+  KS_ERR_INNER = -5,      /* invalid inner product (negative self dot beyond tolerance) */
+  KS_ERR_RANGE = -6       /* size beyond a kernel's supported maximum */
+};
+
+typedef enum { KS_F64 = 0, KS_F32 = 1 } ks_prec;
+
+/* orthogonalization refinement, BVOrthogRefineType (include/slepcbv.h) */
+typedef enum { KS_REFINE_IFNEEDED = 0, KS_REFINE_NEVER = 1, KS_REFINE_ALWAYS = 2 } ks_refine;
+
+/* maximum number of columns one Gram-Schmidt pass or restart GEMM handles
+   (ncv <= 256 plus constraints) */
+#define KS_MAX_COLS 320
+
+typedef struct ks_ctx_s *ks_ctx;
+typedef struct ks_csr_s *ks_csr;
+
+/* Device-side state of the fused CGS+DGKS orthonormalization (the scalars of
+   BVOrthogonalizeGS, src/sys/classes/bv/interface/bvorthog.c:145-217, kept on
+   the GPU so an Arnoldi expansion needs no host round trip). */
+typedef struct {
+  int32_t active;          /* 1 until a column is found linearly dependent */
+  int32_t breakdown_col;   /* column index where lindep occurred, or -1 */
+  int32_t col;             /* column being orthogonalized */
+  int32_t pass_done;       /* CGS passes completed on that column */
+  int32_t done;            /* 1 once the column is finalized */
+  int32_t lindep;          /* lindep flag of the last finalized column */
+  int32_t error;           /* KS_ERR_* raised on the device, 0 if none */
+  int32_t need_norm;       /* estimate beta^2 - sum c^2 <= 0: use the exact norm */
+  int64_t total_passes;    /* executed CGS passes since the last reset */
+  double  onrm;            /* |v| before the current pass */
+  double  nrm;             /* |q| after the current pass (estimate or exact) */
+  double  beta_last;       /* norm of the last finalized column (H(j+1,j)) */
+  double  scale;           /* 1/nrm to apply to the finalized column, or 1 */
+  double  sumsq;           /* sum of squares of the current pass coefficients */
+  double  exact_sq;        /* exact |q|^2 from the update pass */
+} ks_status;
+
+/* ---------------------------------------------------------------- runtime */
+/* Selects the device; fails with KS_ERR_NODEV unless it is gfx950. */
+int ks_init(int device);
+/* Context = one HIP stream (default priority) + reduction workspace + status. */
+int ks_ctx_create(ks_ctx *ctx);
+int ks_ctx_destroy(ks_ctx ctx);
+int ks_ctx_sync(ks_ctx ctx);
+void *ks_ctx_stream(ks_ctx ctx);
+int ks_malloc(void **ptr, size_t bytes);
+int ks_free(void *ptr);
+int ks_memcpy_htod(ks_ctx ctx, void *dst, const void *src, size_t bytes);
+int ks_memcpy_dtoh(ks_ctx ctx, void *dst, const void *src, size_t bytes);
+int ks_memcpy_dtod(ks_ctx ctx, void *dst, const void *src, size_t bytes);
+int ks_memset_zero(ks_ctx ctx, void *dst, size_t bytes);   /* fill kernel, not hipMemset */
+/* stream-ordered host->device copy that returns without waiting; src must stay
+   untouched until the stream passes this point (use pinned memory) */
+int ks_memcpy_htod_async(ks_ctx ctx, void *dst, const void *src, size_t bytes);
+int ks_host_alloc(void **ptr, size_t bytes);   /* pinned host memory */
+int ks_host_free(void *ptr);
+const char *ks_error_string(int code);
+int ks_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* ---------------------------------------------------------------- CSR matrix
+ * Replaces PETSc MatMult_SeqAIJHIPSPARSE / MatMult_SeqAIJ, called from
+ * BVMatMultColumn (src/sys/classes/bv/interface/bvops.c:879), STApply_Generic
+ * (src/sys/classes/st/interface/stsolve.c:22) and EPSComputeResidualNorm_Private
+ * (src/eps/interface/epssolve.c:688).
+ * Uploads the host CSR (int32 indices, values in vprec) and builds the
+ * row-block schedule from the nnz/row histogram. */
+int ks_csr_create(ks_csr *A, ks_ctx ctx, int nrows, int ncols, int64_t nnz,
+                  const int32_t *rowptr, const int32_t *colidx, const void *val, ks_prec vprec);
+/* Same, from arrays already resident on the device (copied, not adopted). */
+int ks_csr_create_device(ks_csr *A, ks_ctx ctx, int nrows, int ncols, int64_t nnz,
+                         const int32_t *d_rowptr, const int32_t *d_colidx, const void *d_val,
+                         ks_prec vprec);
+int ks_csr_destroy(ks_csr A);
+/* nrows, nnz, max row length and the kernel variant chosen (0 stream, 1 vector) */
+int ks_csr_info(ks_csr A, int *nrows, int64_t *nnz, int *max_row, int *variant, int *nblocks);
+/* y = A x.  x, y device vectors in xprec.  When gate != 0 the product is
+   skipped once the context status says an expansion broke down. */
+int ks_spmv(ks_ctx ctx, ks_csr A, const void *x, void *y, ks_prec xprec, int gate);
+
+/* ---------------------------------------------------------------- BV kernels
+ * V is column-major with leading dimension ld (elements), columns in prec. */
+
+/* q[0..k) = V(:,0..k)^T y.  Replaces BVDotVec_BLAS_Private
+   (src/sys/classes/bv/interface/bvblas.c:240-261) / BVDotVec_BLAS_HIP. */
+int ks_bv_dotvec(ks_ctx ctx, ks_prec prec, int n, const void *V, int ld, int k,
+                 const void *y, double *d_q);
+/* y = beta*y + alpha*V(:,0..k) q.  Replaces BVMultVec_BLAS_Private (bvblas.c:56-67). */
+int ks_bv_multvec(ks_ctx ctx, ks_prec prec, int n, double alpha, const void *V, int ld, int k,
+                  const double *d_q, double beta, void *y);
+
+/* Orthonormalize column w = V(:,k) against V(:,0..k) with CGS and the
+   refinement policy (DGKS for IFNEEDED, eta), then scale it by 1/|q|:
+   BVOrthonormalizeColumn (src/sys/classes/bv/interface/bvorthog.c:380-427)
+   with BVOrthogonalizeGS/BVOrthogonalizeCGS1 (bvorthog.c:91-217) and the
+   coefficient helpers (include/slepc/private/bvimpl.h:289-415) fused into
+   stream-ordered kernels.  d_buf is the coefficient buffer with leading
+   dimension ldbuf: column 0 receives the scratch coefficients c, column
+   `hcol` accumulates h and gets |q| at row k (0 if lindep).  Fully
+   asynchronous: the decision to run a second/third pass, lindep and the
+   scaling factor stay on the device (ks_status).  `col` is the column index
+   recorded in the status.  scale=0 skips the normalization
+   (BVOrthogonalizeColumn semantics); safe=1 orthogonalizes with a single pass
+   and reports lindep with norm 0 (BV_OrthogonalizeColumn_Safe, bvimpl.h:446-465). */
+int ks_bv_orthonormalize_column(ks_ctx ctx, ks_prec prec, int n, void *V, int ld, int k,
+                                double *d_buf, int ldbuf, int hcol, int col, ks_refine refine,
+                                double eta, int scale, int safe);
+/* Reset the status for a new expansion (active=1, breakdown_col=-1). */
+int ks_status_reset(ks_ctx ctx);
+/* Copy the status to the host (synchronizes the stream). */
+int ks_status_get(ks_ctx ctx, ks_status *out);
+
+/* x *= alpha (alpha==0 zeroes).  Replaces BVScale_BLAS_Private (bvblas.c:266-278). */
+int ks_bv_scale(ks_ctx ctx, ks_prec prec, int n, void *x, double alpha);
+/* |x|_2 to the host.  Replaces BVNorm_LAPACK_Private NORM_2 (bvlapack.c:37-86). */
+int ks_bv_norm(ks_ctx ctx, ks_prec prec, int n, const void *x, double *nrm);
+/* y += alpha x */
+int ks_bv_axpy(ks_ctx ctx, ks_prec prec, int n, double alpha, const void *x, void *y);
+/* y = x (device copy).  Replaces BVCopyColumn_Mat (bvmat.c:250-260). */
+int ks_bv_copy(ks_ctx ctx, ks_prec prec, int n, const void *x, void *y);
+/* x = uniform(-1,1) from the counter generator
+   u_i = 2*(splitmix64(seed*0x9E3779B97F4A7C15 + i) >> 11)*2^-53 - 1.
+   Replaces BVSetRandomColumn (src/sys/classes/bv/interface/bvops.c:360-379). */
+int ks_bv_random(ks_ctx ctx, ks_prec prec, int n, void *x, uint64_t seed);
+/* convert between precisions (y in py, x in px) */
+int ks_bv_convert(ks_ctx ctx, int n, ks_prec px, const void *x, ks_prec py, void *y);
+
+/* In-place restart GEMM on the FP64 matrix cores:
+   V(:,s:e-1) = V(:,0:k-1) * Q(0:k-1, s:e-1), Q device, column-major, ldq.
+   Replaces BVMultInPlace_BLAS_Private (bvblas.c:74-106) / BVMultInPlace_BLAS_HIP
+   (src/sys/classes/bv/impls/hip/bvhip.hip.cpp:65-112) without the n x (e-s)
+   temporary.  Requires k <= KS_MAX_COLS and e <= KS_MAX_COLS. */
+int ks_bv_multinplace(ks_ctx ctx, ks_prec prec, int n, void *V, int ld, int k,
+                      const double *d_Q, int ldq, int s, int e);
+
+/* ---------------------------------------------------------------- profiling
+ * When enabled, every kernel launch of the context is bracketed by HIP events
+ * on the context stream; collect() accumulates per-class device time, the
+ * algorithmic HBM bytes and flops of the launches that did work. */
+enum { KS_PROF_SPMV = 0, KS_PROF_GS_DOT1, KS_PROF_GS_UPD1, KS_PROF_GS_DOTR, KS_PROF_GS_UPDR,
+       KS_PROF_SCALE, KS_PROF_NORM, KS_PROF_AXPY, KS_PROF_COPY, KS_PROF_MULTINPLACE,
+       KS_PROF_DOTVEC, KS_PROF_MULTVEC, KS_PROF_OTHER, KS_PROF_NCLASS };
+typedef struct {
+  int64_t launches;      /* launches that did work */
+  int64_t skipped;       /* launches gated off on the device */
+  double  ms;            /* device time of the launches that did work */
+  double  bytes;         /* algorithmic HBM bytes */
+  double  flops;
+} ks_prof_entry;
+int ks_prof_enable(ks_ctx ctx, int on);
+int ks_prof_collect(ks_ctx ctx);
+int ks_prof_get(ks_ctx ctx, int cls, ks_prof_entry *out);
+int ks_prof_reset(ks_ctx ctx);
+const char *ks_prof_name(int cls);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

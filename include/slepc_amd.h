@@ -1,0 +1,266 @@
+/*
+ * slepc_amd.h -- SLEPc-shaped host API of libslepc_amd.so (C, no HIP types).
+ *
+ * The object model mirrors SLEPc 3.22: EPS (solver), BV (basis vectors), DS
+ * (dense projected problem), ST (spectral transformation, shift with sigma=0),
+ * plus the PETSc Mat/Vec pieces the Krylov-Schur path touches.  Types are
+ * selected through the same registries: EPSRegister / BVRegister / MatRegister
+ * / DSRegister with EPSSetType / BVSetType / MatSetType / DSSetType.
+ *
+ * The MI355X pieces are the plugins registered at SlepcInitialize():
+ *   BVHIPKS   ("hipks")   BVCreate_HIPKS  -- basis vectors resident in HBM, fused
+ *                                            CGS+DGKS, FP64-MFMA restart GEMM
+ *   MATHIPCSR ("hipcsr")  MatCreate_HIPCSR -- device CSR with the adaptive SpMV
+ * Function names, argument meaning and error behaviour follow the reference;
+ * every function returns a PetscErrorCode (0 on success).
+ */
+#ifndef SLEPC_AMD_H
+#define SLEPC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int    PetscErrorCode;
+typedef int    PetscInt;
+typedef double PetscScalar;
+typedef double PetscReal;
+typedef int    PetscBool;
+#define PETSC_TRUE  1
+#define PETSC_FALSE 0
+#define PETSC_DETERMINE (-1)
+#define PETSC_DECIDE    (-1)
+#define PETSC_CURRENT   (-2)
+#define PETSC_DEFAULT   (-2)
+
+/* error codes (PETSc numbering) */
+#define PETSC_SUCCESS               0
+#define PETSC_ERR_MEM               55
+#define PETSC_ERR_SUP               56
+#define PETSC_ERR_ORDER             58
+#define PETSC_ERR_LIB               76
+#define PETSC_ERR_PLIB              77
+#define PETSC_ERR_ARG_SIZ           60
+#define PETSC_ERR_ARG_WRONG         62
+#define PETSC_ERR_ARG_OUTOFRANGE    63
+#define PETSC_ERR_ARG_WRONGSTATE    73
+#define PETSC_ERR_ARG_NULL          85
+#define PETSC_ERR_USER_INPUT        90
+#define PETSC_ERR_CONV_FAILED       91
+#define PETSC_ERR_NOT_CONVERGED     PETSC_ERR_CONV_FAILED
+#define PETSC_ERR_GPU               97
+
+typedef struct _p_Vec *Vec;
+typedef struct _p_Mat *Mat;
+typedef struct _p_BV  *BV;
+typedef struct _p_DS  *DS;
+typedef struct _p_ST  *ST;
+typedef struct _p_EPS *EPS;
+typedef const char *VecType;
+typedef const char *MatType;
+typedef const char *BVType;
+typedef const char *DSType;
+typedef const char *STType;
+typedef const char *EPSType;
+
+#define MATHIPCSR      "hipcsr"
+#define MATSEQDENSE    "seqdense"
+#define BVHIPKS        "hipks"
+#define DSHEP          "hep"
+#define DSNHEP         "nhep"
+#define STSHIFT        "shift"
+#define EPSKRYLOVSCHUR "krylovschur"
+
+/* value precision of device data (FP64 default; FP32 for the mixed form) */
+typedef enum { SLEPC_AMD_F64 = 0, SLEPC_AMD_F32 = 1 } SlepcAMDPrecision;
+
+typedef enum { NORM_1 = 0, NORM_2 = 1, NORM_FROBENIUS = 2, NORM_INFINITY = 3 } NormType;
+
+/* ------------------------------------------------------------ system */
+PetscErrorCode SlepcInitialize(int device);
+PetscErrorCode SlepcFinalize(void);
+PetscErrorCode SlepcInitialized(PetscBool *flg);
+/* last error message recorded by the library */
+const char    *SlepcAMDGetLastError(void);
+
+/* ------------------------------------------------------------ Vec (device, FP64 or FP32) */
+PetscErrorCode VecCreateHIP(PetscInt n, SlepcAMDPrecision prec, Vec *v);
+PetscErrorCode VecDuplicate(Vec v, Vec *w);
+PetscErrorCode VecDestroy(Vec *v);
+PetscErrorCode VecGetSize(Vec v, PetscInt *n);
+PetscErrorCode VecSet(Vec v, PetscScalar alpha);
+PetscErrorCode VecCopy(Vec x, Vec y);
+PetscErrorCode VecAXPY(Vec y, PetscScalar alpha, Vec x);
+PetscErrorCode VecScale(Vec x, PetscScalar alpha);
+PetscErrorCode VecNorm(Vec x, NormType type, PetscReal *nrm);
+/* host <-> device transfer of the whole vector (FP64 host arrays) */
+PetscErrorCode VecSetValuesHost(Vec v, const PetscScalar *values);
+PetscErrorCode VecGetValuesHost(Vec v, PetscScalar *values);
+/* uniform(-1,1) values from the seeded counter generator, generated on the device */
+PetscErrorCode VecSetRandomSeeded(Vec v, uint64_t seed);
+/* device pointer of the vector storage */
+PetscErrorCode VecHIPGetArray(Vec v, void **d_ptr);
+
+/* ------------------------------------------------------------ Mat */
+PetscErrorCode MatRegister(const char name[], PetscErrorCode (*create)(Mat));
+PetscErrorCode MatCreate(Mat *A);
+PetscErrorCode MatSetSizes(Mat A, PetscInt m, PetscInt n, PetscInt M, PetscInt N);
+PetscErrorCode MatSetType(Mat A, MatType type);
+PetscErrorCode MatGetType(Mat A, MatType *type);
+PetscErrorCode MatGetSize(Mat A, PetscInt *M, PetscInt *N);
+PetscErrorCode MatMult(Mat A, Vec x, Vec y);
+PetscErrorCode MatDestroy(Mat *A);
+/* CSR arrays on the host (copied to the device); row pointers and column
+   indices 32-bit, values FP64 */
+PetscErrorCode MatSeqAIJSetPreallocationCSR(Mat A, const PetscInt i[], const PetscInt j[], const PetscScalar v[]);
+/* same with FP32 values */
+PetscErrorCode MatHIPCSRSetArraysF32(Mat A, const PetscInt i[], const PetscInt j[], const float v[]);
+/* CSR arrays already on the device (copied) */
+PetscErrorCode MatHIPCSRSetDeviceArrays(Mat A, PetscInt nnz, const PetscInt *d_i, const PetscInt *d_j,
+                                        const void *d_v, SlepcAMDPrecision vprec);
+PetscErrorCode MatHIPCSRGetInfo(Mat A, PetscInt *nnz, PetscInt *max_row, PetscInt *variant, PetscInt *nblocks);
+PetscErrorCode MatCreateSeqDense(PetscInt m, PetscInt n, PetscScalar *data, Mat *A);
+PetscErrorCode MatDenseGetArray(Mat A, PetscScalar **array);
+PetscErrorCode MatDenseRestoreArray(Mat A, PetscScalar **array);
+PetscErrorCode MatDenseGetLDA(Mat A, PetscInt *lda);
+PetscErrorCode MatCreateHIPCSR(PetscInt n, const PetscInt i[], const PetscInt j[], const PetscScalar v[], Mat *A);
+
+/* ------------------------------------------------------------ BV */
+typedef enum { BV_ORTHOG_CGS = 0, BV_ORTHOG_MGS = 1 } BVOrthogType;
+typedef enum { BV_ORTHOG_REFINE_IFNEEDED = 0, BV_ORTHOG_REFINE_NEVER = 1, BV_ORTHOG_REFINE_ALWAYS = 2 } BVOrthogRefineType;
+PetscErrorCode BVRegister(const char name[], PetscErrorCode (*create)(BV));
+PetscErrorCode BVCreate(BV *bv);
+PetscErrorCode BVSetSizes(BV bv, PetscInt n, PetscInt N, PetscInt m);
+PetscErrorCode BVSetSizesFromVec(BV bv, Vec t, PetscInt m);
+PetscErrorCode BVGetSizes(BV bv, PetscInt *n, PetscInt *N, PetscInt *m);
+PetscErrorCode BVSetType(BV bv, BVType type);
+PetscErrorCode BVGetType(BV bv, BVType *type);
+PetscErrorCode BVHIPKSSetPrecision(BV bv, SlepcAMDPrecision prec);
+PetscErrorCode BVHIPKSGetPrecision(BV bv, SlepcAMDPrecision *prec);
+PetscErrorCode BVSetOrthogonalization(BV bv, BVOrthogType type, BVOrthogRefineType refine, PetscReal eta, int block);
+PetscErrorCode BVSetActiveColumns(BV bv, PetscInt l, PetscInt k);
+PetscErrorCode BVGetActiveColumns(BV bv, PetscInt *l, PetscInt *k);
+PetscErrorCode BVGetColumn(BV bv, PetscInt j, Vec *v);
+PetscErrorCode BVRestoreColumn(BV bv, PetscInt j, Vec *v);
+PetscErrorCode BVSetRandomColumn(BV bv, PetscInt j);
+PetscErrorCode BVScaleColumn(BV bv, PetscInt j, PetscScalar alpha);
+PetscErrorCode BVNormColumn(BV bv, PetscInt j, NormType type, PetscReal *norm);
+PetscErrorCode BVCopyColumn(BV bv, PetscInt j, PetscInt i);
+PetscErrorCode BVCopyVec(BV bv, PetscInt j, Vec w);
+PetscErrorCode BVInsertVec(BV bv, PetscInt j, Vec w);
+PetscErrorCode BVInsertVecs(BV bv, PetscInt s, PetscInt *m, Vec *W, PetscBool orth);
+PetscErrorCode BVInsertConstraints(BV bv, PetscInt *nc, Vec *C);
+PetscErrorCode BVGetNumConstraints(BV bv, PetscInt *nc);
+PetscErrorCode BVDotVec(BV bv, Vec y, PetscScalar m[]);
+PetscErrorCode BVMultVec(BV bv, PetscScalar alpha, PetscScalar beta, Vec y, PetscScalar q[]);
+PetscErrorCode BVMultInPlace(BV bv, Mat Q, PetscInt s, PetscInt e);
+PetscErrorCode BVOrthogonalizeColumn(BV bv, PetscInt j, PetscScalar H[], PetscReal *norm, PetscBool *lindep);
+PetscErrorCode BVOrthonormalizeColumn(BV bv, PetscInt j, PetscBool replace, PetscReal *norm, PetscBool *lindep);
+PetscErrorCode BVMatMultColumn(BV bv, Mat A, PetscInt j);
+PetscErrorCode BVMatArnoldi(BV bv, Mat A, Mat H, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown);
+PetscErrorCode BVMatLanczos(BV bv, Mat A, Mat T, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown);
+/* host copy of the coefficient buffer, (nc+m) x m column-major */
+PetscErrorCode BVGetBufferHost(BV bv, const PetscScalar **a, PetscInt *ld);
+PetscErrorCode BVGetColumnValuesHost(BV bv, PetscInt j, PetscScalar *values);
+PetscErrorCode BVSetColumnValuesHost(BV bv, PetscInt j, const PetscScalar *values);
+PetscErrorCode BVDestroy(BV *bv);
+/* the plugin constructor of the device basis type */
+PetscErrorCode BVCreate_HIPKS(BV bv);
+
+/* ------------------------------------------------------------ DS */
+typedef enum { DS_STATE_RAW = 0, DS_STATE_INTERMEDIATE = 1, DS_STATE_CONDENSED = 2, DS_STATE_TRUNCATED = 3 } DSStateType;
+typedef enum { DS_MAT_A = 0, DS_MAT_T = 3, DS_MAT_Q = 5, DS_MAT_X = 7, DS_NUM_MAT = 12 } DSMatType;
+PetscErrorCode DSRegister(const char name[], PetscErrorCode (*create)(DS));
+PetscErrorCode DSCreate(DS *ds);
+PetscErrorCode DSSetType(DS ds, DSType type);
+PetscErrorCode DSAllocate(DS ds, PetscInt ld);
+PetscErrorCode DSGetLeadingDimension(DS ds, PetscInt *ld);
+PetscErrorCode DSSetDimensions(DS ds, PetscInt n, PetscInt l, PetscInt k);
+PetscErrorCode DSGetDimensions(DS ds, PetscInt *n, PetscInt *l, PetscInt *k, PetscInt *t);
+PetscErrorCode DSSetState(DS ds, DSStateType state);
+PetscErrorCode DSGetState(DS ds, DSStateType *state);
+PetscErrorCode DSSetCompact(DS ds, PetscBool comp);
+PetscErrorCode DSSetExtraRow(DS ds, PetscBool ext);
+PetscErrorCode DSGetArray(DS ds, DSMatType m, PetscScalar **a);
+PetscErrorCode DSRestoreArray(DS ds, DSMatType m, PetscScalar **a);
+PetscErrorCode DSGetMat(DS ds, DSMatType m, Mat *A);
+PetscErrorCode DSRestoreMat(DS ds, DSMatType m, Mat *A);
+PetscErrorCode DSSolve(DS ds, PetscScalar eigr[], PetscScalar eigi[]);
+PetscErrorCode DSSort(DS ds, PetscScalar eigr[], PetscScalar eigi[], PetscScalar rr[], PetscScalar ri[], PetscInt *k);
+PetscErrorCode DSUpdateExtraRow(DS ds);
+PetscErrorCode DSVectors(DS ds, DSMatType mat, PetscInt *j, PetscReal *rnorm);
+PetscErrorCode DSGetTruncateSize(DS ds, PetscInt l, PetscInt n, PetscInt *k);
+PetscErrorCode DSTruncate(DS ds, PetscInt n, PetscBool trim);
+/* sorting criterion: which (EPSWhich values) and target */
+PetscErrorCode DSSetSortCriterion(DS ds, int which, PetscScalar target);
+PetscErrorCode DSDestroy(DS *ds);
+
+/* ------------------------------------------------------------ ST */
+PetscErrorCode STCreate(ST *st);
+PetscErrorCode STSetMatrices(ST st, PetscInt n, Mat A[]);
+PetscErrorCode STGetOperator(ST st, Mat *Op);
+PetscErrorCode STRestoreOperator(ST st, Mat *Op);
+PetscErrorCode STApply(ST st, Vec x, Vec y);
+PetscErrorCode STDestroy(ST *st);
+
+/* ------------------------------------------------------------ EPS */
+typedef enum { EPS_HEP = 1, EPS_GHEP = 2, EPS_NHEP = 3, EPS_GNHEP = 4 } EPSProblemType;
+typedef enum { EPS_LARGEST_MAGNITUDE = 1, EPS_SMALLEST_MAGNITUDE = 2, EPS_LARGEST_REAL = 3,
+               EPS_SMALLEST_REAL = 4, EPS_LARGEST_IMAGINARY = 5, EPS_SMALLEST_IMAGINARY = 6,
+               EPS_TARGET_MAGNITUDE = 7, EPS_TARGET_REAL = 8 } EPSWhich;
+typedef enum { EPS_CONVERGED_TOL = 1, EPS_CONVERGED_USER = 3, EPS_DIVERGED_ITS = -1,
+               EPS_DIVERGED_BREAKDOWN = -2, EPS_CONVERGED_ITERATING = 0 } EPSConvergedReason;
+typedef enum { EPS_ERROR_ABSOLUTE = 0, EPS_ERROR_RELATIVE = 1, EPS_ERROR_BACKWARD = 2 } EPSErrorType;
+typedef PetscErrorCode (*EPSMonitorFn)(EPS, PetscInt its, PetscInt nconv, PetscScalar *eigr,
+                                       PetscScalar *eigi, PetscReal *errest, PetscInt nest, void *ctx);
+
+PetscErrorCode EPSRegister(const char name[], PetscErrorCode (*create)(EPS));
+PetscErrorCode EPSCreate(EPS *eps);
+PetscErrorCode EPSSetType(EPS eps, EPSType type);
+PetscErrorCode EPSGetType(EPS eps, EPSType *type);
+PetscErrorCode EPSSetOperators(EPS eps, Mat A, Mat B);
+PetscErrorCode EPSSetProblemType(EPS eps, EPSProblemType type);
+PetscErrorCode EPSSetWhichEigenpairs(EPS eps, EPSWhich which);
+PetscErrorCode EPSSetTarget(EPS eps, PetscScalar target);
+PetscErrorCode EPSSetDimensions(EPS eps, PetscInt nev, PetscInt ncv, PetscInt mpd);
+PetscErrorCode EPSGetDimensions(EPS eps, PetscInt *nev, PetscInt *ncv, PetscInt *mpd);
+PetscErrorCode EPSSetTolerances(EPS eps, PetscReal tol, PetscInt maxits);
+PetscErrorCode EPSGetTolerances(EPS eps, PetscReal *tol, PetscInt *maxits);
+PetscErrorCode EPSSetInitialSpace(EPS eps, PetscInt n, Vec is[]);
+PetscErrorCode EPSSetDeflationSpace(EPS eps, PetscInt n, Vec v[]);
+PetscErrorCode EPSKrylovSchurSetRestart(EPS eps, PetscReal keep);
+PetscErrorCode EPSKrylovSchurSetLocking(EPS eps, PetscBool lock);
+PetscErrorCode EPSSetBV(EPS eps, BV bv);
+PetscErrorCode EPSGetBV(EPS eps, BV *bv);
+PetscErrorCode EPSGetDS(EPS eps, DS *ds);
+PetscErrorCode EPSGetST(EPS eps, ST *st);
+PetscErrorCode EPSMonitorSet(EPS eps, EPSMonitorFn fn, void *ctx);
+PetscErrorCode EPSSetUp(EPS eps);
+PetscErrorCode EPSSolve(EPS eps);
+PetscErrorCode EPSGetConverged(EPS eps, PetscInt *nconv);
+PetscErrorCode EPSGetIterationNumber(EPS eps, PetscInt *its);
+PetscErrorCode EPSGetConvergedReason(EPS eps, EPSConvergedReason *reason);
+PetscErrorCode EPSGetEigenvalue(EPS eps, PetscInt i, PetscScalar *eigr, PetscScalar *eigi);
+PetscErrorCode EPSGetEigenvector(EPS eps, PetscInt i, Vec Vr, Vec Vi);
+PetscErrorCode EPSGetEigenpair(EPS eps, PetscInt i, PetscScalar *eigr, PetscScalar *eigi, Vec Vr, Vec Vi);
+PetscErrorCode EPSGetErrorEstimate(EPS eps, PetscInt i, PetscReal *errest);
+PetscErrorCode EPSComputeError(EPS eps, PetscInt i, EPSErrorType type, PetscReal *error);
+PetscErrorCode EPSDestroy(EPS *eps);
+/* solve statistics: SpMV count, executed CGS passes, restarts, host DS seconds, solve seconds */
+PetscErrorCode EPSGetSolveStats(EPS eps, int64_t *nmatmult, int64_t *gs_passes, PetscInt *its,
+                                PetscReal *t_ds, PetscReal *t_solve);
+
+/* the plugin constructors of the solver and the device matrix type */
+PetscErrorCode EPSCreate_KrylovSchur(EPS eps);
+PetscErrorCode MatCreate_HIPCSR(Mat A);
+
+/* the kernel-library context used by the objects (ks_ctx of ks_hip.h) */
+void *SlepcAMDGetDefaultContext(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,605 @@
+// ks_bv.hip -- basis-vector kernels for gfx950: fused CGS+DGKS Gram-Schmidt,
+// tall-skinny V^T y / V q, FP64-MFMA in-place restart GEMM, norm/scale/axpy tails.
+//
+// Layout: V column-major, leading dimension ld (elements), FP64 or FP32.
+// Reductions are row-partitioned: each workgroup owns RPT*256 consecutive rows,
+// keeps its slice of the vector in registers, sweeps the columns in tiles of
+// KS_CT, reduces each tile across its four wavefronts (DPP/shuffle + LDS) and
+// writes one partial row; the last workgroup of each group of 32 and then the
+// last group finish the sums in a fixed order (deterministic, no float atomics).
+#include "ks_internal.hpp"
+
+#include <cmath>
+
+#define KS_CT 8          // columns per register tile in the dot kernels
+#define KS_MIP_ROWS 64   // rows per workgroup of the restart GEMM (4 waves x 16)
+#define KS_MIP_KC 16     // k-chunk of Q staged in LDS
+
+namespace {
+
+enum { GS_FIRST = 0, GS_REPEAT = 1, GS_PLAIN = 2 };
+
+template <typename T> struct acc_t { using type = double; };
+template <> struct acc_t<float> { using type = float; };
+
+__device__ __forceinline__ bool gs_gate(const ks_status *st, int mode, int col, int pass) {
+  if (mode == GS_FIRST) return st->active != 0;
+  if (mode == GS_REPEAT) return st->active && st->col == col && !st->done && st->pass_done == pass - 1;
+  return true;
+}
+
+// partial[b][0..ntot) = V(rows_b, 0..k)^T w(rows_b)  (+ w^T w at column k unless PLAIN)
+template <typename T, int RPT, int MODE>
+__global__ __launch_bounds__(KS_BLOCK) void k_gs_dot(const T *__restrict__ V, long ld, int k,
+                                                    const T *__restrict__ w, int n, int nblocks,
+                                                    double *partial, double *gpartial, unsigned *counters,
+                                                    ks_status *st, double *cout, double *hcol, int col,
+                                                    int pass, int *ran, double deftol) {
+  using AT = typename acc_t<T>::type;
+  if (MODE != GS_PLAIN && !gs_gate(st, MODE, col, pass)) return;
+  if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = 1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ntot = MODE == GS_PLAIN ? k : k + 1;
+  __shared__ double sh[KS_WAVES * KS_CT];
+  __shared__ double fin[KS_NCOLP];
+  const size_t row0 = (size_t)blockIdx.x * KS_BLOCK * RPT;
+  T wv[RPT];
+  bool ok[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const size_t i = row0 + (size_t)r * KS_BLOCK + tid;
+    ok[r] = i < (size_t)n;
+    wv[r] = ok[r] ? w[i] : (T)0;
+  }
+  for (int c0 = 0; c0 < ntot; c0 += KS_CT) {
+    AT acc[KS_CT];
+#pragma unroll
+    for (int t = 0; t < KS_CT; t++) acc[t] = 0;
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+      if (!ok[r]) continue;
+      const size_t i = row0 + (size_t)r * KS_BLOCK + tid;
+#pragma unroll
+      for (int t = 0; t < KS_CT; t++) {
+        const int c = c0 + t;
+        if (c < k) acc[t] += (AT)V[i + (size_t)c * ld] * (AT)wv[r];
+      }
+    }
+    double self = 0.0;
+    if (MODE != GS_PLAIN && c0 + KS_CT > k) {   // this tile holds the self dot (w^T w, FP64)
+#pragma unroll
+      for (int r = 0; r < RPT; r++) self += (double)wv[r] * (double)wv[r];
+    }
+    // reduce the tile across the workgroup
+#pragma unroll
+    for (int t = 0; t < KS_CT; t++) {
+      double v = (MODE != GS_PLAIN && c0 + t == k) ? self : (double)acc[t];
+      v = ks_wave_sum(v);
+      if (lane == 0) sh[wave * KS_CT + t] = v;
+    }
+    __syncthreads();
+    if (tid < KS_CT && c0 + tid < ntot) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < KS_WAVES; q++) s += sh[q * KS_CT + tid];
+      partial[(size_t)blockIdx.x * ntot + c0 + tid] = s;
+    }
+    __syncthreads();
+  }
+  if (!ks_reduce_finish(partial, gpartial, counters, nblocks, ntot, fin)) return;
+  // finishing workgroup.  With constraints the accumulated column may be the
+  // scratch column itself (column 0), so it is cleaned before c is written.
+  if (MODE == GS_FIRST) {
+    for (int c = tid; c <= k; c += KS_BLOCK) hcol[c] = 0.0;   // BV_CleanCoefficients
+    __syncthreads();
+  }
+  for (int c = tid; c < k; c += KS_BLOCK) cout[c] = fin[c];
+  if (MODE == GS_PLAIN) return;
+  if (tid == 0) {
+    const double self = fin[k];
+    double sumsq = 0.0;
+    for (int c = 0; c < k; c++) sumsq += fin[c] * fin[c];    // BV_SquareSum
+    st->sumsq = sumsq;
+    if (self < -deftol) {                                    // BV_SafeSqrt
+      st->error = KS_ERR_INNER;
+      st->active = 0;
+    }
+    st->onrm = self < 0.0 ? 0.0 : sqrt(self);
+    if (MODE == GS_FIRST) {
+      st->col = col;
+      st->pass_done = 0;
+      st->done = 0;
+      st->lindep = 0;
+    }
+  }
+}
+
+// GS: w -= V(:,0..k) c, partial |w|^2, DGKS decision in the finishing workgroup.
+// PLAIN: w = beta*w + alpha*V c (no reduction).
+template <typename T, int RPT, int MODE>
+__global__ __launch_bounds__(KS_BLOCK) void k_gs_update(const T *__restrict__ V, long ld, int k, T *__restrict__ w,
+                                                       int n, int nblocks, const double *__restrict__ cin,
+                                                       double alpha, double beta, double *partial,
+                                                       double *gpartial, unsigned *counters, ks_status *st,
+                                                       double *hcol, int col, int pass, int refine, double eta,
+                                                       int do_scale, int safe, int *ran) {
+  using AT = typename acc_t<T>::type;
+  if (MODE == GS_FIRST) {
+    if (!(st->active && st->col == col && st->pass_done == 0 && !st->done)) return;
+  } else if (MODE == GS_REPEAT) {
+    if (!gs_gate(st, MODE, col, pass)) return;
+  }
+  if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = 1;
+  const int tid = threadIdx.x;
+  __shared__ AT sc[KS_MAX_COLS];
+  __shared__ double red[KS_WAVES];
+  __shared__ double fin[2];
+  for (int c = tid; c < k; c += KS_BLOCK) sc[c] = (AT)(MODE == GS_PLAIN ? alpha * cin[c] : cin[c]);
+  __syncthreads();
+  const size_t row0 = (size_t)blockIdx.x * KS_BLOCK * RPT;
+  double sq = 0.0;
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const size_t i = row0 + (size_t)r * KS_BLOCK + tid;
+    if (i >= (size_t)n) continue;
+    AT a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int c = 0;
+    for (; c + 4 <= k; c += 4) {
+      a0 += (AT)V[i + (size_t)c * ld] * sc[c];
+      a1 += (AT)V[i + (size_t)(c + 1) * ld] * sc[c + 1];
+      a2 += (AT)V[i + (size_t)(c + 2) * ld] * sc[c + 2];
+      a3 += (AT)V[i + (size_t)(c + 3) * ld] * sc[c + 3];
+    }
+    for (; c < k; c++) a0 += (AT)V[i + (size_t)c * ld] * sc[c];
+    const AT dot = (a0 + a1) + (a2 + a3);
+    AT res;
+    if (MODE == GS_PLAIN) res = (beta == 0.0 ? (AT)0 : (AT)beta * (AT)w[i]) + dot;
+    else res = (AT)w[i] - dot;
+    w[i] = (T)res;
+    sq += (double)res * (double)res;
+  }
+  if (MODE == GS_PLAIN) return;
+  double bs = ks_block_sum(sq, red);
+  if (tid == 0) partial[blockIdx.x] = bs;
+  if (!ks_reduce_finish(partial, gpartial, counters, nblocks, 1, fin)) return;
+  for (int c = tid; c < k; c += KS_BLOCK) hcol[c] += cin[c];   // BV_AddCoefficients
+  if (tid == 0) {
+    const double exact = sqrt(fin[0]);
+    st->exact_sq = fin[0];
+    st->total_passes += 1;
+    const int l = ++st->pass_done;
+    const double onrm = st->onrm;
+    double nrm = 0.0;
+    int lindep = 0, finalize = 1, have_norm = 1;
+    if (safe) {                                    // BV_OrthogonalizeColumn_Safe
+      lindep = 1;
+      have_norm = 0;
+    } else if (refine == KS_REFINE_NEVER) {
+      nrm = exact;
+      lindep = !(nrm != 0.0);
+    } else {
+      if (refine == KS_REFINE_ALWAYS && l < 2) {
+        finalize = 0;
+      } else {
+        nrm = onrm * onrm - st->sumsq;             // estimate |v'| from |v|
+        nrm = (nrm <= 0.0) ? exact : sqrt(nrm);
+        if (refine == KS_REFINE_IFNEEDED && l < 3 && nrm != 0.0 && fabs(nrm) < eta * fabs(onrm)) finalize = 0;
+        else lindep = !(nrm != 0.0 && fabs(nrm) >= eta * fabs(onrm));
+      }
+      st->nrm = nrm;
+    }
+    if (finalize) {
+      st->done = 1;
+      st->lindep = lindep;
+      st->nrm = nrm;
+      st->beta_last = nrm;
+      if (have_norm) hcol[k] = lindep ? 0.0 : nrm;   // BV_SetValue
+      st->scale = (do_scale && nrm != 1.0 && nrm != 0.0) ? 1.0 / nrm : 1.0;
+      if (lindep) {
+        st->active = 0;
+        st->breakdown_col = col;
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_scale_status(T *__restrict__ w, int n, const ks_status *__restrict__ st, int col, int *ran) {
+  if (!(st->col == col && st->done)) return;
+  const double s = st->scale;
+  if (s == 1.0) return;
+  if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = 1;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < (size_t)n; i += stride) w[i] = (T)((typename acc_t<T>::type)w[i] * (typename acc_t<T>::type)s);
+}
+
+template <typename T>
+__global__ void k_scale(T *__restrict__ x, int n, double alpha) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  using AT = typename acc_t<T>::type;
+  if (alpha == 0.0)
+    for (; i < (size_t)n; i += stride) x[i] = (T)0;
+  else
+    for (; i < (size_t)n; i += stride) x[i] = (T)((AT)x[i] * (AT)alpha);
+}
+
+template <typename T>
+__global__ void k_axpy(T *__restrict__ y, const T *__restrict__ x, int n, double alpha) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  using AT = typename acc_t<T>::type;
+  for (; i < (size_t)n; i += stride) y[i] = (T)((AT)y[i] + (AT)alpha * (AT)x[i]);
+}
+
+template <typename TX, typename TY>
+__global__ void k_convert(TY *__restrict__ y, const TX *__restrict__ x, int n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < (size_t)n; i += stride) y[i] = (TY)x[i];
+}
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ void k_random(T *__restrict__ x, int n, uint64_t seed) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const uint64_t base = seed * 0x9E3779B97F4A7C15ULL;
+  for (; i < (size_t)n; i += stride) {
+    const uint64_t r = splitmix64(base + (uint64_t)i);
+    x[i] = (T)(2.0 * ((double)(r >> 11) * 0x1.0p-53) - 1.0);
+  }
+}
+
+template <typename T, int RPT>
+__global__ __launch_bounds__(KS_BLOCK) void k_norm(const T *__restrict__ x, int n, int nblocks, double *partial,
+                                                  double *gpartial, unsigned *counters, double *out) {
+  __shared__ double red[KS_WAVES];
+  __shared__ double fin[2];
+  const size_t row0 = (size_t)blockIdx.x * KS_BLOCK * RPT;
+  double s = 0.0;
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const size_t i = row0 + (size_t)r * KS_BLOCK + threadIdx.x;
+    if (i < (size_t)n) {
+      const double v = (double)x[i];
+      s += v * v;
+    }
+  }
+  const double bs = ks_block_sum(s, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+  if (!ks_reduce_finish(partial, gpartial, counters, nblocks, 1, fin)) return;
+  if (threadIdx.x == 0) out[0] = sqrt(fin[0]);
+}
+
+// ---------------------------------------------------------------- restart GEMM
+// V(:, s:s+N) = V(:, 0:K) * Q(0:K, s:s+N) on v_mfma_f64_16x16x4f64.
+// Each wavefront owns 16 rows and all N output columns (NT 16-wide tiles kept in
+// the accumulators).  A fragments come straight from HBM (lane l: row l%16,
+// k = l/16), Q chunks are staged through LDS.  A wave reads every input column
+// of its rows before it stores any output, and rows are disjoint across waves,
+// so the update is in place without a temporary.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <typename T, int NT>
+__global__ __launch_bounds__(KS_BLOCK) void k_multinplace_mfma(T *__restrict__ V, long ld, int n, int K,
+                                                              const double *__restrict__ Q, int ldq, int s,
+                                                              int N) {
+  __shared__ double sQ[KS_MIP_KC][NT * 16];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int rbase = blockIdx.x * KS_MIP_ROWS + wave * 16;
+  const int ra = rbase + (lane & 15);   // A-fragment row of this lane
+  const int kq = lane >> 4;             // k offset of this lane inside a 4-step
+  v4d acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < K; k0 += KS_MIP_KC) {
+    __syncthreads();
+    for (int idx = tid; idx < KS_MIP_KC * NT * 16; idx += KS_BLOCK) {
+      const int kk = idx / (NT * 16), j = idx - kk * (NT * 16);
+      sQ[kk][j] = (k0 + kk < K && j < N) ? Q[(size_t)(k0 + kk) + (size_t)(s + j) * ldq] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KS_MIP_KC; kk += 4) {
+      const int kc = k0 + kk + kq;
+      const double a = (ra < n && kc < K) ? (double)V[(size_t)ra + (size_t)kc * ld] : 0.0;
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const double b = sQ[kk + kq][t * 16 + (lane & 15)];
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  // D layout of v_mfma_f64_16x16x4f64: lane l holds rows 4*(l/16)+q, column l%16
+  const int rowd = rbase + 4 * (lane >> 4);
+  const int cold = lane & 15;
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int c = t * 16 + cold;
+    if (c < N) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int row = rowd + q;
+        if (row < n) V[(size_t)row + (size_t)(s + c) * ld] = (T)acc[t][q];
+      }
+    }
+  }
+}
+
+template <typename T, int NT>
+void launch_mip(ks_ctx ctx, T *V, long ld, int n, int K, const double *Q, int ldq, int s, int N) {
+  dim3 grid((n + KS_MIP_ROWS - 1) / KS_MIP_ROWS), block(KS_BLOCK);
+  hipLaunchKernelGGL((k_multinplace_mfma<T, NT>), grid, block, 0, ctx->stream, V, ld, n, K, Q, ldq, s, N);
+}
+
+template <typename T>
+int dispatch_mip(ks_ctx ctx, T *V, long ld, int n, int K, const double *Q, int ldq, int s, int N) {
+  const int nt = (N + 15) / 16;
+  switch (nt) {
+#define KS_MIP_CASE(X) case X: launch_mip<T, X>(ctx, V, ld, n, K, Q, ldq, s, N); break;
+    KS_MIP_CASE(1) KS_MIP_CASE(2) KS_MIP_CASE(3) KS_MIP_CASE(4) KS_MIP_CASE(5)
+    KS_MIP_CASE(6) KS_MIP_CASE(7) KS_MIP_CASE(8) KS_MIP_CASE(9) KS_MIP_CASE(10)
+    KS_MIP_CASE(11) KS_MIP_CASE(12) KS_MIP_CASE(13) KS_MIP_CASE(14) KS_MIP_CASE(15)
+    KS_MIP_CASE(16) KS_MIP_CASE(17) KS_MIP_CASE(18) KS_MIP_CASE(19) KS_MIP_CASE(20)
+#undef KS_MIP_CASE
+    default: return KS_ERR_RANGE;
+  }
+  return KS_OK;
+}
+
+// ---------------------------------------------------------------- launch helpers
+inline int ew_blocks(int n) {
+  int b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return b;
+}
+
+template <typename T, int MODE>
+void launch_dot(ks_ctx ctx, int rpt, int nb, const T *V, long ld, int k, const T *w, int n, double *cout,
+                double *hcol, int col, int pass, int *ran) {
+  dim3 grid(nb), block(KS_BLOCK);
+  const double deftol = 10.0 * 2.220446049250313e-16;
+#define KS_DOT_CASE(R)                                                                                   \
+  case R:                                                                                                \
+    hipLaunchKernelGGL((k_gs_dot<T, R, MODE>), grid, block, 0, ctx->stream, V, ld, k, w, n, nb,          \
+                       ctx->d_partial, ctx->d_gpartial, ctx->d_counters, ctx->d_status, cout, hcol, col, \
+                       pass, ran, deftol);                                                               \
+    break;
+  switch (rpt) { KS_DOT_CASE(1) KS_DOT_CASE(2) KS_DOT_CASE(4) KS_DOT_CASE(8) KS_DOT_CASE(16) }
+#undef KS_DOT_CASE
+}
+
+template <typename T, int MODE>
+void launch_upd(ks_ctx ctx, int rpt, int nb, const T *V, long ld, int k, T *w, int n, const double *cin,
+                double alpha, double beta, double *hcol, int col, int pass, int refine, double eta, int do_scale,
+                int safe, int *ran) {
+  dim3 grid(nb), block(KS_BLOCK);
+#define KS_UPD_CASE(R)                                                                                      \
+  case R:                                                                                                   \
+    hipLaunchKernelGGL((k_gs_update<T, R, MODE>), grid, block, 0, ctx->stream, V, ld, k, w, n, nb, cin,     \
+                       alpha, beta, ctx->d_partial, ctx->d_gpartial, ctx->d_counters, ctx->d_status, hcol,  \
+                       col, pass, refine, eta, do_scale, safe, ran);                                        \
+    break;
+  switch (rpt) { KS_UPD_CASE(1) KS_UPD_CASE(2) KS_UPD_CASE(4) KS_UPD_CASE(8) KS_UPD_CASE(16) }
+#undef KS_UPD_CASE
+}
+
+int split_checked(int n, int *rpt, int *nb) {
+  ks_rows_split(n, rpt, nb);
+  KS_CHECK(*nb <= KS_MAXB, KS_ERR_RANGE, "vector length %d exceeds the reduction capacity", n);
+  return KS_OK;
+}
+
+template <typename T>
+int orthonormalize_t(ks_ctx ctx, int n, T *V, long ld, int k, double *buf, int ldbuf, int hcol_idx, int col,
+                     ks_refine refine, double eta, int do_scale, int safe) {
+  int rpt, nb, e;
+  if ((e = split_checked(n, &rpt, &nb))) return e;
+  const double s = sizeof(T);
+  T *w = V + (size_t)k * ld;
+  double *cbuf = buf;                              // buffer column 0: scratch coefficients
+  double *hcol = buf + (size_t)hcol_idx * ldbuf;   // buffer column hcol_idx: accumulated h
+  const int passes = safe || refine == KS_REFINE_NEVER ? 1 : (refine == KS_REFINE_ALWAYS ? 2 : 3);
+  for (int p = 1; p <= passes; p++) {
+    const bool first = p == 1;
+    int *ran = ks_prof_begin(ctx, first ? KS_PROF_GS_DOT1 : KS_PROF_GS_DOTR, (k + 1.0) * n * s, 2.0 * (k + 1.0) * n,
+                             true);
+    if (first) launch_dot<T, GS_FIRST>(ctx, rpt, nb, V, ld, k, w, n, cbuf, hcol, col, p, ran);
+    else launch_dot<T, GS_REPEAT>(ctx, rpt, nb, V, ld, k, w, n, cbuf, hcol, col, p, ran);
+    ks_prof_end(ctx);
+    ran = ks_prof_begin(ctx, first ? KS_PROF_GS_UPD1 : KS_PROF_GS_UPDR, (k + 2.0) * n * s, 2.0 * k * n + 2.0 * n,
+                        true);
+    if (first)
+      launch_upd<T, GS_FIRST>(ctx, rpt, nb, V, ld, k, w, n, cbuf, -1.0, 1.0, hcol, col, p, refine, eta, do_scale,
+                              safe, ran);
+    else
+      launch_upd<T, GS_REPEAT>(ctx, rpt, nb, V, ld, k, w, n, cbuf, -1.0, 1.0, hcol, col, p, refine, eta, do_scale,
+                               safe, ran);
+    ks_prof_end(ctx);
+  }
+  if (do_scale) {
+    int *ran = ks_prof_begin(ctx, KS_PROF_SCALE, 2.0 * n * s, (double)n, true);
+    hipLaunchKernelGGL((k_scale_status<T>), dim3(ew_blocks(n)), dim3(256), 0, ctx->stream, w, n, ctx->d_status, col,
+                       ran);
+    ks_prof_end(ctx);
+  }
+  KS_HIP(hipGetLastError());
+  return KS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ks_bv_orthonormalize_column(ks_ctx ctx, ks_prec prec, int n, void *V, int ld, int k, double *d_buf, int ldbuf,
+                                int hcol, int col, ks_refine refine, double eta, int scale, int safe) {
+  KS_CHECK(ctx && V && d_buf, KS_ERR_ARG, "null orthonormalize argument");
+  KS_CHECK(n > 0 && ld >= n && k >= 0 && k < KS_MAX_COLS && ldbuf > k && hcol >= 0, KS_ERR_ARG,
+           "bad orthonormalize sizes n=%d ld=%d k=%d ldbuf=%d", n, ld, k, ldbuf);
+  KS_CHECK(refine >= KS_REFINE_IFNEEDED && refine <= KS_REFINE_ALWAYS, KS_ERR_ARG, "bad refinement type");
+  if (prec == KS_F64)
+    return orthonormalize_t<double>(ctx, n, (double *)V, ld, k, d_buf, ldbuf, hcol, col, refine, eta, scale, safe);
+  return orthonormalize_t<float>(ctx, n, (float *)V, ld, k, d_buf, ldbuf, hcol, col, refine, eta, scale, safe);
+}
+
+int ks_bv_dotvec(ks_ctx ctx, ks_prec prec, int n, const void *V, int ld, int k, const void *y, double *d_q) {
+  KS_CHECK(ctx && y && d_q && (V || k == 0), KS_ERR_ARG, "null dotvec argument");
+  KS_CHECK(n > 0 && ld >= n && k >= 0 && k <= KS_MAX_COLS, KS_ERR_ARG, "bad dotvec sizes");
+  if (k == 0) return KS_OK;
+  int rpt, nb, e;
+  if ((e = split_checked(n, &rpt, &nb))) return e;
+  const double s = prec == KS_F64 ? 8.0 : 4.0;
+  ks_prof_begin(ctx, KS_PROF_DOTVEC, (k + 1.0) * n * s, 2.0 * k * n, false);
+  if (prec == KS_F64)
+    launch_dot<double, GS_PLAIN>(ctx, rpt, nb, (const double *)V, ld, k, (const double *)y, n, d_q, nullptr, 0, 0,
+                                 nullptr);
+  else
+    launch_dot<float, GS_PLAIN>(ctx, rpt, nb, (const float *)V, ld, k, (const float *)y, n, d_q, nullptr, 0, 0,
+                                nullptr);
+  ks_prof_end(ctx);
+  KS_HIP(hipGetLastError());
+  return KS_OK;
+}
+
+int ks_bv_multvec(ks_ctx ctx, ks_prec prec, int n, double alpha, const void *V, int ld, int k, const double *d_q,
+                  double beta, void *y) {
+  KS_CHECK(ctx && y && (k == 0 || (V && d_q)), KS_ERR_ARG, "null multvec argument");
+  KS_CHECK(n > 0 && ld >= n && k >= 0 && k <= KS_MAX_COLS, KS_ERR_ARG, "bad multvec sizes");
+  int rpt, nb, e;
+  if ((e = split_checked(n, &rpt, &nb))) return e;
+  const double s = prec == KS_F64 ? 8.0 : 4.0;
+  ks_prof_begin(ctx, KS_PROF_MULTVEC, (k + 2.0) * n * s, 2.0 * k * n, false);
+  if (prec == KS_F64)
+    launch_upd<double, GS_PLAIN>(ctx, rpt, nb, (const double *)V, ld, k, (double *)y, n, d_q, alpha, beta, nullptr,
+                                 0, 0, 0, 0.0, 0, 0, nullptr);
+  else
+    launch_upd<float, GS_PLAIN>(ctx, rpt, nb, (const float *)V, ld, k, (float *)y, n, d_q, alpha, beta, nullptr, 0,
+                                0, 0, 0.0, 0, 0, nullptr);
+  ks_prof_end(ctx);
+  KS_HIP(hipGetLastError());
+  return KS_OK;
+}
+
+int ks_bv_scale(ks_ctx ctx, ks_prec prec, int n, void *x, double alpha) {
+  KS_CHECK(ctx && x && n >= 0, KS_ERR_ARG, "bad scale arguments");
+  if (n == 0 || alpha == 1.0) return KS_OK;
+  const double s = prec == KS_F64 ? 8.0 : 4.0;
+  ks_prof_begin(ctx, KS_PROF_SCALE, 2.0 * n * s, (double)n, false);
+  if (prec == KS_F64)
+    hipLaunchKernelGGL((k_scale<double>), dim3(ew_blocks(n)), dim3(256), 0, ctx->stream, (double *)x, n, alpha);
+  else
+    hipLaunchKernelGGL((k_scale<float>), dim3(ew_blocks(n)), dim3(256), 0, ctx->stream, (float *)x, n, alpha);
+  ks_prof_end(ctx);
+  KS_HIP(hipGetLastError());
+  return KS_OK;
+}
+
+int ks_bv_norm(ks_ctx ctx, ks_prec prec, int n, const void *x, double *nrm) {
+  KS_CHECK(ctx && x && nrm && n > 0, KS_ERR_ARG, "bad norm arguments");
+  int rpt, nb, e;
+  if ((e = split_checked(n, &rpt, &nb))) return e;
+  const double s = prec == KS_F64 ? 8.0 : 4.0;
+  ks_prof_begin(ctx, KS_PROF_NORM, (double)n * s, 2.0 * n, false);
+  dim3 grid(nb), block(KS_BLOCK);
+#define KS_NORM_CASE(T, R)                                                                                       \
+  case R:                                                                                                        \
+    hipLaunchKernelGGL((k_norm<T, R>), grid, block, 0, ctx->stream, (const T *)x, n, nb, ctx->d_partial,         \
+                       ctx->d_gpartial, ctx->d_counters, ctx->d_scalar);                                         \
+    break;
+  if (prec == KS_F64) {
+    switch (rpt) { KS_NORM_CASE(double, 1) KS_NORM_CASE(double, 2) KS_NORM_CASE(double, 4) KS_NORM_CASE(double, 8) KS_NORM_CASE(double, 16) }
+  } else {
+    switch (rpt) { KS_NORM_CASE(float, 1) KS_NORM_CASE(float, 2) KS_NORM_CASE(float, 4) KS_NORM_CASE(float, 8) KS_NORM_CASE(float, 16) }
+  }
+#undef KS_NORM_CASE
+  ks_prof_end(ctx);
+  KS_HIP(hipGetLastError());
+  KS_HIP(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  KS_HIP(hipStreamSynchronize(ctx->stream));
+  *nrm = ctx->h_scalar[0];
+  return KS_OK;
+}
+
+int ks_bv_axpy(ks_ctx ctx, ks_prec prec, int n, double alpha, const void *x, void *y) {
+  KS_CHECK(ctx && x && y && n >= 0, KS_ERR_ARG, "bad axpy arguments");
+  if (n == 0) return KS_OK;
+  const double s = prec == KS_F64 ? 8.0 : 4.0;
+  ks_prof_begin(ctx, KS_PROF_AXPY, 3.0 * n * s, 2.0 * n, false);
+  if (prec == KS_F64)
+    hipLaunchKernelGGL((k_axpy<double>), dim3(ew_blocks(n)), dim3(256), 0, ctx->stream, (double *)y,
+                       (const double *)x, n, alpha);
+  else
+    hipLaunchKernelGGL((k_axpy<float>), dim3(ew_blocks(n)), dim3(256), 0, ctx->stream, (float *)y,
+                       (const float *)x, n, alpha);
+  ks_prof_end(ctx);
+  KS_HIP(hipGetLastError());
+  return KS_OK;
+}
+
+int ks_bv_copy(ks_ctx ctx, ks_prec prec, int n, const void *x, void *y) {
+  KS_CHECK(ctx && x && y && n >= 0, KS_ERR_ARG, "bad copy arguments");
+  if (n == 0 || x == y) return KS_OK;
+  const size_t s = prec == KS_F64 ? 8 : 4;
+  ks_prof_begin(ctx, KS_PROF_COPY, 2.0 * n * s, 0.0, false);
+  KS_HIP(hipMemcpyAsync(y, x, s * n, hipMemcpyDeviceToDevice, ctx->stream));
+  ks_prof_end(ctx);
+  return KS_OK;
+}
+
+int ks_bv_random(ks_ctx ctx, ks_prec prec, int n, void *x, uint64_t seed) {
+  KS_CHECK(ctx && x && n >= 0, KS_ERR_ARG, "bad random arguments");
+  if (n == 0) return KS_OK;
+  ks_prof_begin(ctx, KS_PROF_OTHER, (double)n * (prec == KS_F64 ? 8 : 4), 0.0, false);
+  if (prec == KS_F64)
+    hipLaunchKernelGGL((k_random<double>), dim3(ew_blocks(n)), dim3(256), 0, ctx->stream, (double *)x, n, seed);
+  else
+    hipLaunchKernelGGL((k_random<float>), dim3(ew_blocks(n)), dim3(256), 0, ctx->stream, (float *)x, n, seed);
+  ks_prof_end(ctx);
+  KS_HIP(hipGetLastError());
+  return KS_OK;
+}
+
+int ks_bv_convert(ks_ctx ctx, int n, ks_prec px, const void *x, ks_prec py, void *y) {
+  KS_CHECK(ctx && x && y && n >= 0, KS_ERR_ARG, "bad convert arguments");
+  if (n == 0) return KS_OK;
+  if (px == py) return ks_bv_copy(ctx, px, n, x, y);
+  ks_prof_begin(ctx, KS_PROF_OTHER, 12.0 * n, 0.0, false);
+  if (px == KS_F64)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(ew_blocks(n)), dim3(256), 0, ctx->stream, (float *)y,
+                       (const double *)x, n);
+  else
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(ew_blocks(n)), dim3(256), 0, ctx->stream, (double *)y,
+                       (const float *)x, n);
+  ks_prof_end(ctx);
+  KS_HIP(hipGetLastError());
+  return KS_OK;
+}
+
+int ks_bv_multinplace(ks_ctx ctx, ks_prec prec, int n, void *V, int ld, int k, const double *d_Q, int ldq, int s,
+                      int e) {
+  KS_CHECK(ctx && V && d_Q, KS_ERR_ARG, "null multinplace argument");
+  KS_CHECK(n > 0 && ld >= n && k >= 0 && ldq >= k && s >= 0, KS_ERR_ARG, "bad multinplace sizes");
+  KS_CHECK(k <= KS_MAX_COLS && e <= KS_MAX_COLS, KS_ERR_RANGE, "multinplace supports at most %d columns",
+           KS_MAX_COLS);
+  if (s >= e) return KS_OK;
+  const int N = e - s;
+  const double sz = prec == KS_F64 ? 8.0 : 4.0;
+  ks_prof_begin(ctx, KS_PROF_MULTINPLACE, (double)n * k * sz + (double)n * N * sz, 2.0 * n * (double)k * N, false);
+  int err = prec == KS_F64 ? dispatch_mip<double>(ctx, (double *)V, ld, n, k, d_Q, ldq, s, N)
+                           : dispatch_mip<float>(ctx, (float *)V, ld, n, k, d_Q, ldq, s, N);
+  ks_prof_end(ctx);
+  if (err) return err;
+  KS_HIP(hipGetLastError());
+  return KS_OK;
+}
+
+}  // extern "C"

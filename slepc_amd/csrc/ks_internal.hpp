@@ -1,0 +1,148 @@
+// ks_internal.hpp -- shared pieces of libks_hip.so (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/ks_hip.h"
+
+#define KS_BLOCK 256                 // threads per block of the reduction kernels
+#define KS_WAVES (KS_BLOCK / 64)
+#define KS_MAXB 8192                 // maximum blocks of one row-partitioned reduction
+#define KS_GROUP 32                  // blocks per first-level reduction group
+#define KS_MAXG (KS_MAXB / KS_GROUP)
+#define KS_NCOLP (KS_MAX_COLS + 2)   // partial row width: columns + self dot + spare
+#define KS_SPMV_NNZ 1024             // products staged in LDS per SpMV row block
+#define KS_SPMV_ROWS 256             // rows per SpMV row block (one per thread)
+
+#define KS_HIP(call)                                                                    \
+  do {                                                                                  \
+    hipError_t _e = (call);                                                             \
+    if (_e != hipSuccess) {                                                             \
+      fprintf(stderr, "ks_hip: %s failed: %s (%s:%d)\n", #call, hipGetErrorString(_e),  \
+              __FILE__, __LINE__);                                                      \
+      return KS_ERR_HIP;                                                                \
+    }                                                                                   \
+  } while (0)
+
+#define KS_CHECK(cond, code, ...)                                                       \
+  do {                                                                                  \
+    if (!(cond)) {                                                                      \
+      fprintf(stderr, "ks_hip: " __VA_ARGS__);                                          \
+      fputc('\n', stderr);                                                              \
+      return (code);                                                                    \
+    }                                                                                   \
+  } while (0)
+
+struct ks_prof_rec {
+  int cls;
+  int ran_slot;            // index into the device ran-flag array, -1 if always runs
+  double bytes, flops;
+  hipEvent_t ev0, ev1;
+};
+
+struct ks_ctx_s {
+  hipStream_t stream = nullptr;
+  ks_status *d_status = nullptr;
+  double *d_partial = nullptr;     // [KS_MAXB][KS_NCOLP]
+  double *d_gpartial = nullptr;    // [KS_MAXG][KS_NCOLP]
+  unsigned *d_counters = nullptr;  // [KS_MAXG + 1], zero between launches
+  double *d_scalar = nullptr;      // small device scratch (norm results)
+  double *h_scalar = nullptr;      // pinned host scratch
+  // profiling
+  bool prof = false;
+  std::vector<ks_prof_rec> recs;
+  std::vector<hipEvent_t> event_pool;
+  int *d_ran = nullptr;            // per-launch "did work" flags
+  int ran_cap = 0, ran_used = 0;
+  ks_prof_entry acc[KS_PROF_NCLASS] = {};
+};
+
+struct ks_csr_s {
+  int nrows = 0, ncols = 0, max_row = 0, nblocks = 0, variant = 0;
+  int64_t nnz = 0;
+  ks_prec vprec = KS_F64;
+  int32_t *d_rowptr = nullptr, *d_colidx = nullptr, *d_blk = nullptr;
+  void *d_val = nullptr;
+};
+
+// ---------------------------------------------------------------- profiling hooks
+// begin() returns a pointer to the device ran-flag for gated kernels (or null).
+extern "C" int *ks_prof_begin(ks_ctx ctx, int cls, double bytes, double flops, bool gated);
+extern "C" void ks_prof_end(ks_ctx ctx);
+
+// ---------------------------------------------------------------- device helpers
+__device__ __forceinline__ double ks_wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-wide sum of one double; result valid in thread 0.  sh: >= KS_WAVES doubles.
+__device__ __forceinline__ double ks_block_sum(double v, double *sh) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  v = ks_wave_sum(v);
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < KS_WAVES; w++) s += sh[w];
+  }
+  return s;
+}
+
+// Two-level "last block finishes" reduction of per-block partial rows.
+// Every block has written partial[blockIdx.x * ncol + c] for c < ncol.  Returns
+// true in exactly one block, whose threads then find the totals in out[0..ncol)
+// (shared memory).  Summation order is fixed, so results are deterministic.
+// Counters are returned to zero by the finishing blocks.
+__device__ __forceinline__ bool ks_reduce_finish(double *partial, double *gpartial,
+                                                 unsigned *counters, int nblocks, int ncol,
+                                                 double *out) {
+  __shared__ unsigned s_ticket;
+  const int g = blockIdx.x / KS_GROUP;
+  const int gbeg = g * KS_GROUP;
+  const int gend = min(nblocks, gbeg + KS_GROUP);
+  const int ngroups = (nblocks + KS_GROUP - 1) / KS_GROUP;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_ticket = atomicAdd(&counters[g], 1u);
+  __syncthreads();
+  if (s_ticket != (unsigned)(gend - gbeg - 1)) return false;
+  __threadfence();
+  for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
+    double s = 0.0;
+    for (int b = gbeg; b < gend; b++) s += __builtin_nontemporal_load(&partial[(size_t)b * ncol + c]);
+    gpartial[(size_t)g * ncol + c] = s;
+  }
+  if (threadIdx.x == 0) counters[g] = 0u;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_ticket = atomicAdd(&counters[KS_MAXG], 1u);
+  __syncthreads();
+  if (s_ticket != (unsigned)(ngroups - 1)) return false;
+  __threadfence();
+  for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
+    double s = 0.0;
+    for (int q = 0; q < ngroups; q++) s += __builtin_nontemporal_load(&gpartial[(size_t)q * ncol + c]);
+    out[c] = s;
+  }
+  if (threadIdx.x == 0) counters[KS_MAXG] = 0u;
+  __syncthreads();
+  return true;
+}
+
+// rows per thread (1,2,4,8,16) and block count for a row-partitioned reduction
+inline void ks_rows_split(int n, int *rpt, int *nblocks) {
+  int r = 1;
+  while (r < 16 && (int64_t)(n + KS_BLOCK * r - 1) / (KS_BLOCK * r) > 1024) r <<= 1;
+  int64_t nb = (n + (int64_t)KS_BLOCK * r - 1) / ((int64_t)KS_BLOCK * r);
+  if (nb < 1) nb = 1;
+  *rpt = r;
+  *nblocks = (int)nb;
+}
+

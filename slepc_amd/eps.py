@@ -1,0 +1,329 @@
+"""Python front-end of libslepc_amd.so (include/slepc_amd.h), named after
+slepc4py's EPS/BV/Mat/Vec so that tests read like the reference's.  Every call
+goes through the C host framework, which drives the HIP kernels; there is no
+CPU fallback and loading fails loudly when the libraries are not built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import ks as _ks
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libslepc_amd.so")
+
+F64, F32 = 0, 1
+_lib = None
+_initialized = False
+
+
+class SlepcError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _ks.lib()   # load the kernel library first (RTLD_GLOBAL)
+        if not os.path.exists(LIB_PATH):
+            raise SlepcError(f"{LIB_PATH} is not built; run `python -m slepc_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.SlepcAMDGetLastError.restype = C.c_char_p
+        L.SlepcAMDGetDefaultContext.restype = C.c_void_p
+        _lib = L
+    return _lib
+
+
+def _chk(code, what):
+    if code != 0:
+        msg = lib().SlepcAMDGetLastError().decode(errors="replace")
+        raise SlepcError(f"{what} failed with error {code}: {msg}")
+
+
+def initialize(device: int = 0):
+    global _initialized
+    if not _initialized:
+        _chk(lib().SlepcInitialize(C.c_int(device)), "SlepcInitialize")
+        _initialized = True
+
+
+def context_handle() -> C.c_void_p:
+    initialize()
+    return C.c_void_p(lib().SlepcAMDGetDefaultContext())
+
+
+class Vec:
+    def __init__(self, n: int, prec: int = F64):
+        initialize()
+        self.h = C.c_void_p()
+        self.n = n
+        _chk(lib().VecCreateHIP(C.c_int(n), C.c_int(prec), C.byref(self.h)), "VecCreateHIP")
+
+    @classmethod
+    def from_array(cls, a, prec: int = F64):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        v = cls(len(a), prec)
+        v.set(a)
+        return v
+
+    def set(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == (self.n,)
+        _chk(lib().VecSetValuesHost(self.h, a.ctypes.data_as(C.POINTER(C.c_double))), "VecSetValuesHost")
+
+    @classmethod
+    def random(cls, n: int, seed: int, prec: int = F64):
+        v = cls(n, prec)
+        _chk(lib().VecSetRandomSeeded(v.h, C.c_uint64(seed)), "VecSetRandomSeeded")
+        return v
+
+    def array(self) -> np.ndarray:
+        out = np.empty(self.n)
+        _chk(lib().VecGetValuesHost(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "VecGetValuesHost")
+        return out
+
+    def norm(self) -> float:
+        r = C.c_double()
+        _chk(lib().VecNorm(self.h, C.c_int(1), C.byref(r)), "VecNorm")
+        return r.value
+
+    def destroy(self):
+        if self.h:
+            lib().VecDestroy(C.byref(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Mat:
+    """MATHIPCSR matrix from host CSR arrays (32-bit indices)."""
+
+    def __init__(self, csr, prec: int = F64):
+        initialize()
+        rowptr, colidx, val = csr
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.colidx = np.ascontiguousarray(colidx, dtype=np.int32)
+        self.n = len(self.rowptr) - 1
+        self.h = C.c_void_p()
+        L = lib()
+        _chk(L.MatCreate(C.byref(self.h)), "MatCreate")
+        _chk(L.MatSetSizes(self.h, self.n, self.n, self.n, self.n), "MatSetSizes")
+        _chk(L.MatSetType(self.h, b"hipcsr"), "MatSetType")
+        ip = self.rowptr.ctypes.data_as(C.c_void_p)
+        jp = self.colidx.ctypes.data_as(C.c_void_p)
+        if prec == F64:
+            v = np.ascontiguousarray(val, dtype=np.float64)
+            _chk(L.MatSeqAIJSetPreallocationCSR(self.h, ip, jp, v.ctypes.data_as(C.c_void_p)),
+                 "MatSeqAIJSetPreallocationCSR")
+        else:
+            v = np.ascontiguousarray(val, dtype=np.float32)
+            _chk(L.MatHIPCSRSetArraysF32(self.h, ip, jp, v.ctypes.data_as(C.c_void_p)), "MatHIPCSRSetArraysF32")
+
+    def mult(self, x: Vec, y: Vec):
+        _chk(lib().MatMult(self.h, x.h, y.h), "MatMult")
+
+    def info(self):
+        nnz, mr, var, nb = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _chk(lib().MatHIPCSRGetInfo(self.h, C.byref(nnz), C.byref(mr), C.byref(var), C.byref(nb)),
+             "MatHIPCSRGetInfo")
+        return dict(nnz=nnz.value, max_row=mr.value, variant=var.value, nblocks=nb.value)
+
+    def destroy(self):
+        if self.h:
+            lib().MatDestroy(C.byref(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class EPS:
+    # EPSProblemType
+    HEP, NHEP = 1, 3
+    # EPSWhich
+    LARGEST_MAGNITUDE, SMALLEST_MAGNITUDE, LARGEST_REAL, SMALLEST_REAL = 1, 2, 3, 4
+    LARGEST_IMAGINARY, SMALLEST_IMAGINARY, TARGET_MAGNITUDE, TARGET_REAL = 5, 6, 7, 8
+    # EPSConvergedReason
+    CONVERGED_TOL, DIVERGED_ITS, DIVERGED_BREAKDOWN = 1, -1, -2
+    # EPSErrorType
+    ERROR_ABSOLUTE, ERROR_RELATIVE = 0, 1
+    REFINE_IFNEEDED, REFINE_NEVER, REFINE_ALWAYS = 0, 1, 2
+
+    def __init__(self):
+        initialize()
+        self.h = C.c_void_p()
+        _chk(lib().EPSCreate(C.byref(self.h)), "EPSCreate")
+        self._keep = []
+        self.n = None
+
+    # ---- settings
+    def setOperators(self, A: Mat):
+        self._A = A
+        self.n = A.n
+        _chk(lib().EPSSetOperators(self.h, A.h, None), "EPSSetOperators")
+
+    def setProblemType(self, t):
+        _chk(lib().EPSSetProblemType(self.h, C.c_int(t)), "EPSSetProblemType")
+
+    def setWhichEigenpairs(self, w):
+        _chk(lib().EPSSetWhichEigenpairs(self.h, C.c_int(w)), "EPSSetWhichEigenpairs")
+
+    def setTarget(self, t: float):
+        _chk(lib().EPSSetTarget(self.h, C.c_double(t)), "EPSSetTarget")
+
+    def setDimensions(self, nev=-2, ncv=-1, mpd=-1):
+        _chk(lib().EPSSetDimensions(self.h, C.c_int(nev), C.c_int(ncv), C.c_int(mpd)), "EPSSetDimensions")
+
+    def setTolerances(self, tol=-1.0, max_it=-1):
+        _chk(lib().EPSSetTolerances(self.h, C.c_double(tol), C.c_int(max_it)), "EPSSetTolerances")
+
+    def setKrylovSchurRestart(self, keep: float):
+        self._ensure_type()
+        _chk(lib().EPSKrylovSchurSetRestart(self.h, C.c_double(keep)), "EPSKrylovSchurSetRestart")
+
+    def setKrylovSchurLocking(self, lock: bool):
+        self._ensure_type()
+        _chk(lib().EPSKrylovSchurSetLocking(self.h, C.c_int(int(lock))), "EPSKrylovSchurSetLocking")
+
+    def _ensure_type(self):
+        t = C.c_char_p()
+        lib().EPSGetType.argtypes = [C.c_void_p, C.POINTER(C.c_char_p)]
+        _chk(lib().EPSGetType(self.h, C.byref(t)), "EPSGetType")
+        if not t.value:
+            _chk(lib().EPSSetType(self.h, b"krylovschur"), "EPSSetType")
+
+    def _bv(self):
+        bv = C.c_void_p()
+        _chk(lib().EPSGetBV(self.h, C.byref(bv)), "EPSGetBV")
+        return bv
+
+    def setBVPrecision(self, prec: int):
+        _chk(lib().BVHIPKSSetPrecision(self._bv(), C.c_int(prec)), "BVHIPKSSetPrecision")
+
+    def setOrthogonalization(self, refine=0, eta=0.7071):
+        _chk(lib().BVSetOrthogonalization(self._bv(), C.c_int(0), C.c_int(refine), C.c_double(eta), C.c_int(0)),
+             "BVSetOrthogonalization")
+
+    def setInitialSpace(self, vecs):
+        vs = [v if isinstance(v, Vec) else Vec.from_array(v) for v in vecs]
+        self._keep.append(vs)
+        arr = (C.c_void_p * len(vs))(*[v.h for v in vs])
+        _chk(lib().EPSSetInitialSpace(self.h, C.c_int(len(vs)), arr), "EPSSetInitialSpace")
+
+    def setDeflationSpace(self, vecs):
+        vs = [v if isinstance(v, Vec) else Vec.from_array(v) for v in vecs]
+        self._keep.append(vs)
+        arr = (C.c_void_p * len(vs))(*[v.h for v in vs])
+        _chk(lib().EPSSetDeflationSpace(self.h, C.c_int(len(vs)), arr), "EPSSetDeflationSpace")
+
+    # ---- solve and results
+    def setUp(self):
+        _chk(lib().EPSSetUp(self.h), "EPSSetUp")
+
+    def solve(self):
+        _chk(lib().EPSSolve(self.h), "EPSSolve")
+
+    def getConverged(self) -> int:
+        n = C.c_int()
+        _chk(lib().EPSGetConverged(self.h, C.byref(n)), "EPSGetConverged")
+        return n.value
+
+    def getIterationNumber(self) -> int:
+        n = C.c_int()
+        _chk(lib().EPSGetIterationNumber(self.h, C.byref(n)), "EPSGetIterationNumber")
+        return n.value
+
+    def getConvergedReason(self) -> int:
+        n = C.c_int()
+        _chk(lib().EPSGetConvergedReason(self.h, C.byref(n)), "EPSGetConvergedReason")
+        return n.value
+
+    def getEigenvalue(self, i: int):
+        re, im = C.c_double(), C.c_double()
+        _chk(lib().EPSGetEigenvalue(self.h, C.c_int(i), C.byref(re), C.byref(im)), "EPSGetEigenvalue")
+        return complex(re.value, im.value) if im.value != 0.0 else re.value
+
+    def getEigenvector(self, i: int, imag: bool = False):
+        vr = Vec(self.n)
+        vi = Vec(self.n) if imag else None
+        _chk(lib().EPSGetEigenvector(self.h, C.c_int(i), vr.h, vi.h if vi else None), "EPSGetEigenvector")
+        return (vr.array(), vi.array()) if imag else vr.array()
+
+    def getErrorEstimate(self, i: int) -> float:
+        r = C.c_double()
+        _chk(lib().EPSGetErrorEstimate(self.h, C.c_int(i), C.byref(r)), "EPSGetErrorEstimate")
+        return r.value
+
+    def computeError(self, i: int, etype: int = 1) -> float:
+        r = C.c_double()
+        _chk(lib().EPSComputeError(self.h, C.c_int(i), C.c_int(etype), C.byref(r)), "EPSComputeError")
+        return r.value
+
+    def stats(self) -> dict:
+        nm, gp = C.c_int64(), C.c_int64()
+        its = C.c_int()
+        tds, ts = C.c_double(), C.c_double()
+        _chk(lib().EPSGetSolveStats(self.h, C.byref(nm), C.byref(gp), C.byref(its), C.byref(tds), C.byref(ts)),
+             "EPSGetSolveStats")
+        return dict(nmatmult=nm.value, gs_passes=gp.value, its=its.value, t_ds=tds.value, t_solve=ts.value)
+
+    def eigenvalues(self):
+        k = self.getConverged()
+        return np.array([self.getEigenvalue(i) for i in range(k)])
+
+    def destroy(self):
+        if self.h:
+            lib().EPSDestroy(C.byref(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=True, v0=None, max_it=-1,
+          prec=F64, deflation=None, lock=True, keep=0.5, want_vectors=True):
+    """One-call convenience wrapper: returns a dict like the oracle's eps_solve."""
+    A = Mat(csr)
+    eps = EPS()
+    eps.setOperators(A)
+    eps.setProblemType(EPS.HEP if hermitian else EPS.NHEP)
+    eps.setDimensions(nev, ncv)
+    eps.setTolerances(tol, max_it)
+    eps.setWhichEigenpairs(which)
+    eps.setKrylovSchurLocking(lock)
+    eps.setKrylovSchurRestart(keep)
+    if prec != F64:
+        eps.setBVPrecision(prec)
+    if v0 is not None:
+        eps.setInitialSpace([v0])
+    if deflation is not None:
+        eps.setDeflationSpace([deflation[:, i] for i in range(deflation.shape[1])])
+    eps.solve()
+    k = eps.getConverged()
+    vals = [eps.getEigenvalue(i) for i in range(k)]
+    out = dict(nconv=k, its=eps.getIterationNumber(), reason=eps.getConvergedReason(),
+               eigr=np.array([np.real(v) for v in vals]), eigi=np.array([np.imag(v) for v in vals]),
+               errest=np.array([eps.getErrorEstimate(i) for i in range(k)]),
+               err=np.array([eps.computeError(i) for i in range(k)]))
+    if want_vectors:
+        X = np.zeros((A.n, k), order="F")
+        for i in range(k):
+            X[:, i] = eps.getEigenvector(i)
+        out["X"] = X
+    out.update(eps.stats())
+    eps.destroy()
+    A.destroy()
+    return out

@@ -1,0 +1,463 @@
+/*
+ * bv.c -- BV interface (src/sys/classes/bv/interface): registry and type
+ * selection (bvbasic.c:35-64, bvfunc.c:646-652), sizes, active columns, column
+ * views, orthogonalization entry points (bvorthog.c:285-427), Krylov
+ * factorizations (bvkrylov.c:56-226) and the in-place multiply (bvops.c:220-245).
+ * The numerical work is done by the type's ops table.
+ */
+#include "slepcimpl.h"
+
+static SlepcFunctionList BVList;
+
+PetscErrorCode BVRegister(const char name[], PetscErrorCode (*create)(BV))
+{
+  return SlepcFunctionListAdd(&BVList, name, (void *)create);
+}
+
+/* BVCreate defaults (bvfunc.c:158-224): CGS, refine if needed, eta = 0.7071 */
+PetscErrorCode BVCreate(BV *out)
+{
+  BV bv;
+  PetscCall(SlepcCheckInitialized());
+  PetscCall(PetscMallocZero(sizeof(struct _p_BV), &bv));
+  bv->orthog_type = BV_ORTHOG_CGS;
+  bv->orthog_ref = BV_ORTHOG_REFINE_IFNEEDED;
+  bv->orthog_eta = 0.7071;
+  bv->deftol = 10 * PETSC_MACHINE_EPSILON;
+  bv->ci[0] = bv->ci[1] = BV_NOCOL;
+  bv->prec = SLEPC_AMD_F64;
+  bv->ctx = SlepcAMDContext();
+  *out = bv;
+  return PETSC_SUCCESS;
+}
+
+static PetscErrorCode BVCreateData(BV bv)
+{
+  if (bv->ops.create && bv->m > 0 && bv->n > 0 && !bv->data) PetscCall(bv->ops.create(bv));
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode BVSetSizes(BV bv, PetscInt n, PetscInt N, PetscInt m)
+{
+  PetscCheck(bv, PETSC_ERR_ARG_NULL, "null BV");
+  PetscCheck(!bv->data, PETSC_ERR_ORDER, "sizes cannot be changed after storage is allocated");
+  PetscCheck(m > 0, PETSC_ERR_ARG_OUTOFRANGE, "number of columns %d must be positive", m);
+  bv->n = n >= 0 ? n : N;
+  bv->N = N >= 0 ? N : n;
+  bv->m = m;
+  bv->k = m;
+  bv->l = 0;
+  PetscCall(BVCreateData(bv));
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode BVSetSizesFromVec(BV bv, Vec t, PetscInt m)
+{
+  PetscCheck(t, PETSC_ERR_ARG_NULL, "null template vector");
+  return BVSetSizes(bv, t->n, t->n, m);
+}
+
+PetscErrorCode BVGetSizes(BV bv, PetscInt *n, PetscInt *N, PetscInt *m)
+{
+  PetscCheck(bv, PETSC_ERR_ARG_NULL, "null BV");
+  if (n) *n = bv->n;
+  if (N) *N = bv->N;
+  if (m) *m = bv->m;
+  return PETSC_SUCCESS;
+}
+
+/* BVSetType (bvbasic.c:35-64): creation deferred until sizes are set */
+PetscErrorCode BVSetType(BV bv, BVType type)
+{
+  PetscErrorCode (*create)(BV);
+  PetscCheck(bv && type, PETSC_ERR_ARG_NULL, "null argument");
+  if (!strcmp(bv->type, type)) return PETSC_SUCCESS;
+  create = (PetscErrorCode(*)(BV))SlepcFunctionListFind(&BVList, type);
+  PetscCheck(create, PETSC_ERR_ARG_WRONG, "unknown BV type %s", type);
+  if (bv->ops.destroy && bv->data) PetscCall(bv->ops.destroy(bv));
+  memset(&bv->ops, 0, sizeof(bv->ops));
+  strcpy(bv->type, type);
+  bv->ops.create = create;
+  PetscCall(BVCreateData(bv));
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode BVGetType(BV bv, BVType *type)
+{
+  PetscCheck(bv, PETSC_ERR_ARG_NULL, "null BV");
+  *type = bv->type;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode BVSetOrthogonalization(BV bv, BVOrthogType type, BVOrthogRefineType refine, PetscReal eta, int block)
+{
+  (void)block;
+  PetscCheck(bv, PETSC_ERR_ARG_NULL, "null BV");
+  PetscCheck(type == BV_ORTHOG_CGS, PETSC_ERR_SUP, "only classical Gram-Schmidt is implemented on the device");
+  bv->orthog_type = type;
+  bv->orthog_ref = refine;
+  if (eta != PETSC_DEFAULT && eta != PETSC_DETERMINE) {
+    PetscCheck(eta > 0.0 && eta <= 1.0, PETSC_ERR_ARG_OUTOFRANGE, "invalid eta %g", eta);
+    bv->orthog_eta = eta;
+  }
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode BVSetActiveColumns(BV bv, PetscInt l, PetscInt k)
+{
+  PetscCheck(bv, PETSC_ERR_ARG_NULL, "null BV");
+  PetscCheck(k >= 0 && k <= bv->m, PETSC_ERR_ARG_OUTOFRANGE, "bad k %d (m=%d)", k, bv->m);
+  PetscCheck(l >= 0 && l <= k, PETSC_ERR_ARG_OUTOFRANGE, "bad l %d (k=%d)", l, k);
+  bv->l = l;
+  bv->k = k;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode BVGetActiveColumns(BV bv, PetscInt *l, PetscInt *k)
+{
+  PetscCheck(bv, PETSC_ERR_ARG_NULL, "null BV");
+  if (l) *l = bv->l;
+  if (k) *k = bv->k;
+  return PETSC_SUCCESS;
+}
+
+static PetscErrorCode BVCheckReady(BV bv)
+{
+  PetscCheck(bv, PETSC_ERR_ARG_NULL, "null BV");
+  PetscCheck(bv->data, PETSC_ERR_ORDER, "BV needs a type and sizes first");
+  return PETSC_SUCCESS;
+}
+
+/* BVGetColumn (bvbasic.c:1116-1137): at most two columns lent at a time */
+PetscErrorCode BVGetColumn(BV bv, PetscInt j, Vec *v)
+{
+  PetscInt l;
+  PetscCall(BVCheckReady(bv));
+  PetscCheck(j >= -bv->nc && j < bv->m, PETSC_ERR_ARG_OUTOFRANGE, "column %d out of range", j);
+  PetscCheck(j != bv->ci[0] && j != bv->ci[1], PETSC_ERR_ARG_WRONG, "column %d already fetched", j);
+  PetscCheck(bv->ci[0] == BV_NOCOL || bv->ci[1] == BV_NOCOL, PETSC_ERR_SUP, "too many columns requested");
+  l = (bv->ci[0] == BV_NOCOL) ? 0 : 1;
+  bv->ci[l] = j;
+  PetscCall(bv->ops.getcolumn(bv, j, &bv->cv[l]));
+  *v = bv->cv[l];
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode BVRestoreColumn(BV bv, PetscInt j, Vec *v)
+{
+  PetscInt l;
+  PetscCall(BVCheckReady(bv));
+  PetscCheck(j == bv->ci[0] || j == bv->ci[1], PETSC_ERR_ARG_WRONG, "column %d has not been fetched", j);
+  l = (j == bv->ci[0]) ? 0 : 1;
+  PetscCheck(v && *v == bv->cv[l], PETSC_ERR_ARG_WRONG, "vector does not match the fetched column");
+  if (bv->ops.restorecolumn) PetscCall(bv->ops.restorecolumn(bv, j, &bv->cv[l]));
+  bv->ci[l] = BV_NOCOL;
+  *v = NULL;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode BVSetRandomColumn(BV bv, PetscInt j)
+{
+  PetscCall(BVCheckReady(bv));
+  PetscCheck(j >= 0 && j < bv->m, PETSC_ERR_ARG_OUTOFRANGE, "column %d out of range", j);
+  return bv->ops.setrandomcolumn(bv, j);
+}
+
+PetscErrorCode BVScaleColumn(BV bv, PetscInt j, PetscScalar alpha)
+{
+  PetscCall(BVCheckReady(bv));
+  PetscCheck(j >= 0 && j < bv->m, PETSC_ERR_ARG_OUTOFRANGE, "column %d out of range", j);
+  if (alpha == 1.0) return PETSC_SUCCESS;
+  return bv->ops.scale(bv, j, alpha);
+}
+
+PetscErrorCode BVNormColumn(BV bv, PetscInt j, NormType type, PetscReal *norm)
+{
+  PetscCall(BVCheckReady(bv));
+  PetscCheck(j >= -bv->nc && j < bv->m, PETSC_ERR_ARG_OUTOFRANGE, "column %d out of range", j);
+  return bv->ops.norm(bv, j, type, norm);
+}
+
+PetscErrorCode BVCopyColumn(BV bv, PetscInt j, PetscInt i)
+{
+  PetscCall(BVCheckReady(bv));
+  PetscCheck(j >= 0 && j < bv->m && i >= 0 && i < bv->m, PETSC_ERR_ARG_OUTOFRANGE, "columns %d,%d out of range", j, i);
+  if (j == i) return PETSC_SUCCESS;
+  return bv->ops.copycolumn(bv, j, i);
+}
+
+PetscErrorCode BVCopyVec(BV bv, PetscInt j, Vec w)
+{
+  Vec z;
+  PetscCall(BVGetColumn(bv, j, &z));
+  PetscErrorCode e = VecCopy(z, w);
+  PetscCall(BVRestoreColumn(bv, j, &z));
+  return e;
+}
+
+PetscErrorCode BVInsertVec(BV bv, PetscInt j, Vec w)
+{
+  Vec z;
+  PetscCall(BVGetColumn(bv, j, &z));
+  PetscErrorCode e = VecCopy(w, z);
+  PetscCall(BVRestoreColumn(bv, j, &z));
+  return e;
+}
+
+PetscErrorCode BVGetColumnValuesHost(BV bv, PetscInt j, PetscScalar *values)
+{
+  Vec z;
+  PetscCall(BVGetColumn(bv, j, &z));
+  PetscErrorCode e = VecGetValuesHost(z, values);
+  PetscCall(BVRestoreColumn(bv, j, &z));
+  return e;
+}
+
+PetscErrorCode BVSetColumnValuesHost(BV bv, PetscInt j, const PetscScalar *values)
+{
+  Vec z;
+  PetscCall(BVGetColumn(bv, j, &z));
+  PetscErrorCode e = VecSetValuesHost(z, values);
+  PetscCall(BVRestoreColumn(bv, j, &z));
+  return e;
+}
+
+/* BVInsertVecs (bvfunc.c:340-374) */
+PetscErrorCode BVInsertVecs(BV bv, PetscInt s, PetscInt *m, Vec *W, PetscBool orth)
+{
+  PetscInt ndep = 0;
+  PetscReal norm;
+  PetscBool lindep;
+  PetscCall(BVCheckReady(bv));
+  PetscCheck(m && *m > 0 && W, PETSC_ERR_ARG_OUTOFRANGE, "bad number of vectors");
+  PetscCheck(s >= 0 && s + *m <= bv->m, PETSC_ERR_ARG_OUTOFRANGE, "too many vectors for %d columns", bv->m);
+  for (PetscInt i = 0; i < *m; i++) {
+    PetscCheck(W[i] && W[i]->n == bv->n, PETSC_ERR_ARG_SIZ, "vector %d has the wrong size", i);
+    PetscCall(BVInsertVec(bv, s + i - ndep, W[i]));
+    if (orth) {
+      PetscCall(BVOrthogonalizeColumn(bv, s + i - ndep, NULL, &norm, &lindep));
+      if (norm == 0.0 || lindep) ndep++;
+      else PetscCall(BVScaleColumn(bv, s + i - ndep, 1.0 / norm));
+    }
+  }
+  *m -= ndep;
+  return PETSC_SUCCESS;
+}
+
+/* BVInsertConstraints (bvfunc.c:411-439): the storage is rebuilt with nc
+   leading constraint columns, which are inserted and orthonormalized. */
+PetscErrorCode BVInsertConstraints(BV bv, PetscInt *nc, Vec *C)
+{
+  PetscInt msave, ncin;
+  PetscCall(BVCheckReady(bv));
+  PetscCheck(nc && *nc >= 0, PETSC_ERR_ARG_OUTOFRANGE, "bad number of constraints");
+  PetscCheck(!bv->nc, PETSC_ERR_ARG_WRONGSTATE, "constraints already present in this BV object");
+  PetscCheck(bv->ci[0] == BV_NOCOL && bv->ci[1] == BV_NOCOL, PETSC_ERR_SUP, "cannot call BVInsertConstraints after BVGetColumn");
+  if (!*nc) return PETSC_SUCCESS;
+  ncin = *nc;
+  msave = bv->m;
+  PetscCall(bv->ops.destroy(bv));
+  bv->data = NULL;
+  bv->m = ncin + msave;
+  PetscCall(BVCreateData(bv));
+  bv->l = 0;
+  bv->k = bv->m;
+  PetscCall(BVInsertVecs(bv, 0, nc, C, PETSC_TRUE));
+  /* keep the storage: the first nc columns become the constraints */
+  bv->nc = *nc;
+  bv->m = msave;
+  bv->l = 0;
+  bv->k = msave;
+  bv->ci[0] = bv->ci[1] = BV_NOCOL;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode BVGetNumConstraints(BV bv, PetscInt *nc)
+{
+  PetscCheck(bv, PETSC_ERR_ARG_NULL, "null BV");
+  *nc = bv->nc;
+  return PETSC_SUCCESS;
+}
+
+/* BVDotVec (bvglobal.c): m = V(:,l:k)^T y */
+PetscErrorCode BVDotVec(BV bv, Vec y, PetscScalar m[])
+{
+  PetscCall(BVCheckReady(bv));
+  PetscCheck(y && y->n == bv->n, PETSC_ERR_ARG_SIZ, "vector size does not match");
+  return bv->ops.dotvec(bv, y, m);
+}
+
+/* BVMultVec (bvops.c): y = beta y + alpha V(:,l:k) q */
+PetscErrorCode BVMultVec(BV bv, PetscScalar alpha, PetscScalar beta, Vec y, PetscScalar q[])
+{
+  PetscCall(BVCheckReady(bv));
+  PetscCheck(y && y->n == bv->n, PETSC_ERR_ARG_SIZ, "vector size does not match");
+  return bv->ops.multvec(bv, alpha, beta, y, q);
+}
+
+/* BVMultInPlace (bvops.c:220-245) */
+PetscErrorCode BVMultInPlace(BV bv, Mat Q, PetscInt s, PetscInt e)
+{
+  PetscInt m, n;
+  PetscCall(BVCheckReady(bv));
+  PetscCheck(Q && !strcmp(Q->type, MATSEQDENSE), PETSC_ERR_ARG_WRONG, "Q must be a sequential dense matrix");
+  PetscCheck(s >= bv->l && s <= bv->m, PETSC_ERR_ARG_OUTOFRANGE, "Argument s has wrong value %d, should be between %d and %d", s, bv->l, bv->m);
+  PetscCheck(e >= bv->l && e <= bv->m, PETSC_ERR_ARG_OUTOFRANGE, "Argument e has wrong value %d, should be between %d and %d", e, bv->l, bv->m);
+  PetscCall(MatGetSize(Q, &m, &n));
+  PetscCheck(m >= bv->k, PETSC_ERR_ARG_SIZ, "Mat argument has %d rows, should have at least %d", m, bv->k);
+  PetscCheck(e <= n, PETSC_ERR_ARG_SIZ, "Mat argument only has %d columns, the requested value of e is larger: %d", n, e);
+  return bv->ops.multinplace(bv, Q, s, e);
+}
+
+/* BVOrthogonalizeColumn (bvorthog.c:285-320) */
+PetscErrorCode BVOrthogonalizeColumn(BV bv, PetscInt j, PetscScalar H[], PetscReal *norm, PetscBool *lindep)
+{
+  PetscReal nrm;
+  PetscBool lnd;
+  PetscCall(BVCheckReady(bv));
+  PetscCheck(j >= 0 && j < bv->m, PETSC_ERR_ARG_OUTOFRANGE, "Index j=%d but BV only has %d columns", j, bv->m);
+  PetscCheck(bv->ops.orthonormalizecolumn, PETSC_ERR_SUP, "BV type %s does not orthogonalize", bv->type);
+  PetscCall(bv->ops.orthonormalizecolumn(bv, j, PETSC_FALSE, PETSC_FALSE, PETSC_FALSE, &nrm, &lnd));
+  if (H) {   /* BV_StoreCoefficients: values l..j of column j (here l = 0 after the call) */
+    PetscCall(bv->ops.getbuffer(bv));
+    for (PetscInt i = 0; i < j; i++) H[i] = bv->buffer[(size_t)j * (bv->nc + bv->m) + bv->nc + i];
+  }
+  if (norm) *norm = nrm;
+  if (lindep) *lindep = lnd;
+  return PETSC_SUCCESS;
+}
+
+/* BV_OrthogonalizeColumn_Safe (bvimpl.h:446-465) */
+PetscErrorCode BVOrthogonalizeColumn_Safe(BV bv, PetscInt j, PetscReal *norm, PetscBool *lindep)
+{
+  PetscCall(bv->ops.orthonormalizecolumn(bv, j, PETSC_FALSE, PETSC_TRUE, PETSC_FALSE, NULL, NULL));
+  if (norm) *norm = 0.0;
+  if (lindep) *lindep = PETSC_TRUE;
+  return PETSC_SUCCESS;
+}
+
+/* BVOrthonormalizeColumn (bvorthog.c:380-427) */
+PetscErrorCode BVOrthonormalizeColumn(BV bv, PetscInt j, PetscBool replace, PetscReal *norm, PetscBool *lindep)
+{
+  PetscReal nrm;
+  PetscBool lnd;
+  PetscCall(BVCheckReady(bv));
+  PetscCheck(j >= 0 && j < bv->m, PETSC_ERR_ARG_OUTOFRANGE, "Index j=%d but BV only has %d columns", j, bv->m);
+  PetscCheck(bv->ops.orthonormalizecolumn, PETSC_ERR_SUP, "BV type %s does not orthogonalize", bv->type);
+  PetscCall(bv->ops.orthonormalizecolumn(bv, j, PETSC_TRUE, PETSC_FALSE, PETSC_FALSE, &nrm, &lnd));
+  if (replace && (nrm == 0.0 || lnd)) {
+    PetscCall(BVSetRandomColumn(bv, j));
+    PetscCall(bv->ops.orthonormalizecolumn(bv, j, PETSC_TRUE, PETSC_FALSE, PETSC_FALSE, &nrm, &lnd));
+    if (nrm == 0.0 || lnd) {
+      PetscCall(BVSetRandomColumn(bv, j));
+      PetscCall(bv->ops.orthonormalizecolumn(bv, j, PETSC_TRUE, PETSC_FALSE, PETSC_FALSE, &nrm, &lnd));
+    }
+  }
+  if (norm) *norm = nrm;
+  if (lindep) *lindep = lnd;
+  return PETSC_SUCCESS;
+}
+
+/* BVMatMultColumn (bvops.c:862-885) */
+PetscErrorCode BVMatMultColumn(BV bv, Mat A, PetscInt j)
+{
+  Vec vj, vj1;
+  PetscCall(BVCheckReady(bv));
+  PetscCheck(j >= 0 && j + 1 < bv->m, PETSC_ERR_ARG_OUTOFRANGE, "Result should go in index j+1=%d but BV only has %d columns", j + 1, bv->m);
+  PetscCall(BVGetColumn(bv, j, &vj));
+  PetscCall(BVGetColumn(bv, j + 1, &vj1));
+  PetscErrorCode e = MatMult(A, vj, vj1);
+  PetscCall(BVRestoreColumn(bv, j, &vj));
+  PetscCall(BVRestoreColumn(bv, j + 1, &vj1));
+  return e;
+}
+
+PetscErrorCode BVGetBufferHost(BV bv, const PetscScalar **a, PetscInt *ld)
+{
+  PetscCall(BVCheckReady(bv));
+  PetscCall(bv->ops.getbuffer(bv));
+  *a = bv->buffer;
+  if (ld) *ld = bv->nc + bv->m;
+  return PETSC_SUCCESS;
+}
+
+/* the expansion loop of BVMatArnoldi/BVMatLanczos; the type may fuse it */
+static PetscErrorCode BVMatKrylov_Private(BV V, Mat A, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown)
+{
+  PetscBool lindep = PETSC_FALSE;
+  if (V->ops.matkrylov) return V->ops.matkrylov(V, A, k, m, beta, breakdown);
+  for (PetscInt j = k; j < *m; j++) {
+    PetscCall(BVMatMultColumn(V, A, j));
+    if (j == V->N - 1) PetscCall(BVOrthogonalizeColumn_Safe(V, j + 1, beta, &lindep));
+    else PetscCall(BVOrthonormalizeColumn(V, j + 1, PETSC_FALSE, beta, &lindep));
+    if (lindep) { *m = j + 1; break; }
+  }
+  *breakdown = lindep;
+  PetscCall(V->ops.getbuffer(V));
+  return PETSC_SUCCESS;
+}
+
+/* BVMatArnoldi (bvkrylov.c:56-113) */
+PetscErrorCode BVMatArnoldi(BV V, Mat A, Mat H, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown)
+{
+  PetscBool bd = PETSC_FALSE;
+  PetscReal b = 0.0;
+  PetscCall(BVCheckReady(V));
+  PetscCheck(k >= 0 && k <= V->m, PETSC_ERR_ARG_OUTOFRANGE, "Argument k has wrong value %d, should be between 0 and %d", k, V->m);
+  PetscCheck(*m > 0 && *m <= V->m, PETSC_ERR_ARG_OUTOFRANGE, "Argument m has wrong value %d, should be between 1 and %d", *m, V->m);
+  PetscCheck(*m > k, PETSC_ERR_ARG_OUTOFRANGE, "Argument m should be at least equal to k+1");
+  if (H) {
+    PetscCheck(!strcmp(H->type, MATSEQDENSE), PETSC_ERR_ARG_WRONG, "H must be seqdense");
+    PetscCheck(H->m >= *m && H->n >= *m, PETSC_ERR_ARG_SIZ, "Matrix H is too small");
+  }
+  PetscCall(BVMatKrylov_Private(V, A, k, m, &b, &bd));
+  if (breakdown) *breakdown = bd;
+  if (beta) *beta = b;
+  if (H) {
+    Mat_SeqDense *hd = (Mat_SeqDense *)H->data;
+    PetscScalar *h = hd->v;
+    PetscInt ldh = hd->lda, ldb = V->nc + V->m;
+    const PetscScalar *a = V->buffer;
+    for (PetscInt j = k; j < *m - 1; j++) memcpy(h + (size_t)j * ldh, a + V->nc + (size_t)(j + 1) * ldb, sizeof(PetscScalar) * (size_t)(j + 2));
+    memcpy(h + (size_t)(*m - 1) * ldh, a + V->nc + (size_t)(*m) * ldb, sizeof(PetscScalar) * (size_t)(*m));
+    if (ldh > *m) h[(*m) + (size_t)(*m - 1) * ldh] = a[V->nc + (*m) + (size_t)(*m) * ldb];
+  }
+  return PETSC_SUCCESS;
+}
+
+/* BVMatLanczos (bvkrylov.c:165-226): T stores alpha in column 0, beta in column 1 */
+PetscErrorCode BVMatLanczos(BV V, Mat A, Mat T, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown)
+{
+  PetscBool bd = PETSC_FALSE;
+  PetscReal b = 0.0;
+  PetscCall(BVCheckReady(V));
+  PetscCheck(*m > k, PETSC_ERR_ARG_OUTOFRANGE, "Argument m should be at least equal to k+1");
+  if (T) {
+    PetscCheck(!strcmp(T->type, MATSEQDENSE), PETSC_ERR_ARG_WRONG, "T must be seqdense");
+    PetscCheck(T->m >= *m && T->n >= 2, PETSC_ERR_ARG_SIZ, "Matrix T is too small");
+  }
+  PetscCall(BVMatKrylov_Private(V, A, k, m, &b, &bd));
+  if (breakdown) *breakdown = bd;
+  if (beta) *beta = b;
+  if (T) {
+    Mat_SeqDense *td = (Mat_SeqDense *)T->data;
+    PetscReal *alpha = td->v, *betat = td->v + td->lda;
+    PetscInt ldb = V->nc + V->m;
+    const PetscScalar *a = V->buffer;
+    for (PetscInt j = k; j < *m; j++) {
+      alpha[j] = a[V->nc + j + (size_t)(j + 1) * ldb];
+      betat[j] = a[V->nc + j + 1 + (size_t)(j + 1) * ldb];
+    }
+  }
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode BVDestroy(BV *bv)
+{
+  if (!bv || !*bv) return PETSC_SUCCESS;
+  if ((*bv)->ops.destroy && (*bv)->data) PetscCall((*bv)->ops.destroy(*bv));
+  free((*bv)->buffer);
+  free(*bv);
+  *bv = NULL;
+  return PETSC_SUCCESS;
+}
