@@ -318,8 +318,9 @@ __global__ __launch_bounds__(KS_BLOCK) void k_multinplace_mfma(T *__restrict__ V
       }
     }
   }
-  // D layout of v_mfma_f64_16x16x4f64: lane l holds rows 4*(l/16)+q, column l%16
-  const int rowd = rbase + 4 * (lane >> 4);
+  // D layout of v_mfma_f64_16x16x4f64 (measured on gfx950): register q of lane l
+  // holds row l/16 + 4q, column l%16
+  const int rowd = rbase + (lane >> 4);
   const int cold = lane & 15;
 #pragma unroll
   for (int t = 0; t < NT; t++) {
@@ -327,7 +328,7 @@ __global__ __launch_bounds__(KS_BLOCK) void k_multinplace_mfma(T *__restrict__ V
     if (c < N) {
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const int row = rowd + q;
+        const int row = rowd + 4 * q;
         if (row < n) V[(size_t)row + (size_t)(s + c) * ld] = (T)acc[t][q];
       }
     }
