@@ -203,6 +203,179 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_update(const T *__restrict__ V,
   }
 }
 
+// ---------------------------------------------------------------- LDS-staged Gram-Schmidt
+// One CGS pass for column w against V(:,0..k), streamed in sub-chunks of R rows:
+//   phase A  the workgroup loads R rows x k columns with coalesced column
+//            segments into LDS (column-major, stride S = R + T) and, in the
+//            fused form, applies the update w <- w - V c of this pass; the
+//            vector (updated or not) becomes LDS column k;
+//   phase B  thread (column c, row class sub) accumulates column c . column k
+//            over rows sub, sub+T, ...  (c = k gives |w|^2), kept in registers
+//            across all sub-chunks of the workgroup.
+// A workgroup therefore reduces once, at the end, and the two-level finish
+// produces the k+1 dots.  FUSED=false is the first pass (c = V^T w, w^T w);
+// FUSED=true updates with this pass's c and returns the next pass's dots from
+// the updated vector, so a second DGKS pass costs one V read instead of two.
+template <typename T, int R, bool FUSED>
+__global__ __launch_bounds__(KS_BLOCK) void k_gs_lds(const T *__restrict__ V, long ld, int k, T *__restrict__ w,
+                                                    int n, int nblocks, int cpb, int tsub,
+                                                    double *__restrict__ cbuf, double *partial, double *gpartial,
+                                                    unsigned *counters, ks_status *st, double *hcol, int col,
+                                                    int pass, int refine, double eta, int do_scale, int safe,
+                                                    int *ran, double deftol) {
+  if (FUSED) {
+    if (!(st->active && st->col == col && !st->done && st->pass_done == pass - 1)) return;
+  } else {
+    if (!st->active) return;
+  }
+  if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = 1;
+  extern __shared__ double smem[];
+  __shared__ double fin[KS_NCOLP];
+  __shared__ int s_continue;
+  const int tid = threadIdx.x;
+  const int S = R + tsub;
+  const int G = KS_BLOCK / R;
+  const int r = tid % R, g = tid / R;
+  double *Vs = smem;                       // (k+1) columns x S
+  double *coef = Vs + (size_t)(k + 1) * S; // k
+  double *red = coef + k;                  // G x R
+  if (FUSED)
+    for (int c = tid; c < k; c += KS_BLOCK) coef[c] = cbuf[c];
+  const int cpt = KS_BLOCK / tsub;         // columns per phase-B sweep
+  const int sub = tid % tsub;
+  double dacc[2] = {0.0, 0.0};
+  __syncthreads();
+  for (int ch = 0; ch < cpb; ch++) {
+    const long r0 = ((long)blockIdx.x * cpb + ch) * R;
+    if (r0 >= n) break;
+    const long i = r0 + r;
+    const bool ok = i < n;
+    double acc = 0.0;
+    for (int c = g; c < k; c += G) {
+      const double v = ok ? (double)V[i + (size_t)c * ld] : 0.0;
+      Vs[(size_t)c * S + r] = v;
+      if (FUSED) acc += v * coef[c];
+    }
+    if (FUSED) {
+      red[g * R + r] = acc;
+      __syncthreads();
+      if (g == 0) {
+        double s = 0.0;
+        for (int q = 0; q < G; q++) s += red[q * R + r];
+        double wn = 0.0;
+        if (ok) {
+          const T t = (T)((double)w[i] - s);
+          w[i] = t;
+          wn = (double)t;
+        }
+        Vs[(size_t)k * S + r] = wn;
+      }
+    } else {
+      if (g == 0) Vs[(size_t)k * S + r] = ok ? (double)w[i] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int c = j * cpt + tid / tsub;
+      if (c <= k) {
+        const double *vc = Vs + (size_t)c * S, *vk = Vs + (size_t)k * S;
+        double s0 = 0.0, s1 = 0.0;
+        int q = sub;
+        for (; q + tsub < R; q += 2 * tsub) {
+          s0 += vc[q] * vk[q];
+          s1 += vc[q + tsub] * vk[q + tsub];
+        }
+        if (q < R) s0 += vc[q] * vk[q];
+        dacc[j] += s0 + s1;
+      }
+    }
+    __syncthreads();
+  }
+  const int ntot = k + 1;
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    double v = dacc[j];
+    for (int off = tsub >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int c = j * cpt + tid / tsub;
+    if (sub == 0 && c <= k) partial[(size_t)blockIdx.x * ntot + c] = v;
+  }
+  if (!ks_reduce_finish(partial, gpartial, counters, nblocks, ntot, fin)) return;
+  if (!FUSED) {
+    // first pass: BV_CleanCoefficients, c, |v| (BVDotColumnInc + BV_SquareRoot)
+    for (int c = tid; c <= k; c += KS_BLOCK) hcol[c] = 0.0;
+    __syncthreads();
+    for (int c = tid; c < k; c += KS_BLOCK) cbuf[c] = fin[c];
+    if (tid == 0) {
+      const double self = fin[k];
+      double sumsq = 0.0;
+      for (int c = 0; c < k; c++) sumsq += fin[c] * fin[c];
+      st->sumsq = sumsq;
+      if (self < -deftol) {
+        st->error = KS_ERR_INNER;
+        st->active = 0;
+      }
+      st->onrm = self < 0.0 ? 0.0 : sqrt(self);
+      st->col = col;
+      st->pass_done = 0;
+      st->done = 0;
+      st->lindep = 0;
+    }
+    return;
+  }
+  // fused pass: h += c of this pass, DGKS decision, next pass coefficients
+  for (int c = tid; c < k; c += KS_BLOCK) hcol[c] += cbuf[c];   // BV_AddCoefficients
+  __syncthreads();
+  if (tid == 0) {
+    const double exact = sqrt(fin[k]);
+    st->exact_sq = fin[k];
+    st->total_passes += 1;
+    const int l = ++st->pass_done;
+    const double onrm = st->onrm;
+    double nrm = 0.0;
+    int lindep = 0, finalize = 1, have_norm = 1;
+    if (safe) {
+      lindep = 1;
+      have_norm = 0;
+    } else if (refine == KS_REFINE_NEVER) {
+      nrm = exact;
+      lindep = !(nrm != 0.0);
+    } else {
+      if (refine == KS_REFINE_ALWAYS && l < 2) {
+        finalize = 0;
+      } else {
+        nrm = onrm * onrm - st->sumsq;             // estimate |v'| from |v| (bvorthog.c:124-127)
+        nrm = (nrm <= 0.0) ? exact : sqrt(nrm);
+        if (refine == KS_REFINE_IFNEEDED && l < 3 && nrm != 0.0 && fabs(nrm) < eta * fabs(onrm)) finalize = 0;
+        else lindep = !(nrm != 0.0 && fabs(nrm) >= eta * fabs(onrm));
+      }
+      st->nrm = nrm;
+    }
+    if (finalize) {
+      st->done = 1;
+      st->lindep = lindep;
+      st->nrm = nrm;
+      st->beta_last = nrm;
+      if (have_norm) hcol[k] = lindep ? 0.0 : nrm;   // BV_SetValue
+      st->scale = (do_scale && nrm != 1.0 && nrm != 0.0) ? 1.0 / nrm : 1.0;
+      if (lindep) {
+        st->active = 0;
+        st->breakdown_col = col;
+      }
+      s_continue = 0;
+    } else {
+      // next pass: its |v| and coefficients are the dots of the updated vector
+      double sumsq = 0.0;
+      for (int c = 0; c < k; c++) sumsq += fin[c] * fin[c];
+      st->sumsq = sumsq;
+      st->onrm = exact;
+      s_continue = 1;
+    }
+  }
+  __syncthreads();
+  if (s_continue)
+    for (int c = tid; c < k; c += KS_BLOCK) cbuf[c] = fin[c];
+}
+
 template <typename T>
 __global__ void k_scale_status(T *__restrict__ w, int n, const ks_status *__restrict__ st, int col, int *ran) {
   if (!(st->col == col && st->done)) return;
@@ -400,31 +573,69 @@ int split_checked(int n, int *rpt, int *nb) {
   return KS_OK;
 }
 
+// geometry of the LDS-staged Gram-Schmidt kernels for k columns
+struct gs_geom {
+  int R, tsub, nb, cpb;
+  size_t lds;
+};
+
+int gs_geometry(int n, int k, gs_geom *g) {
+  int t = 8;
+  while (t > 1 && (k + 1) * t > KS_BLOCK) t >>= 1;
+  g->tsub = t;
+  g->R = 0;
+  for (int R = 256; R >= 16; R >>= 1) {
+    const size_t lds = sizeof(double) * ((size_t)(k + 1) * (R + t) + k + KS_BLOCK);
+    if (lds <= 60 * 1024) {
+      g->R = R;
+      g->lds = lds;
+      break;
+    }
+  }
+  KS_CHECK(g->R, KS_ERR_RANGE, "%d columns do not fit the LDS tile", k);
+  const long chunks = (n + (long)g->R - 1) / g->R;
+  long nb = chunks < 2048 ? chunks : 2048;
+  g->cpb = (int)((chunks + nb - 1) / nb);
+  g->nb = (int)((chunks + g->cpb - 1) / g->cpb);
+  KS_CHECK(g->nb <= KS_MAXB, KS_ERR_RANGE, "vector length %d exceeds the reduction capacity", n);
+  return KS_OK;
+}
+
+template <typename T, bool FUSED>
+void launch_gs(ks_ctx ctx, const gs_geom &g, const T *V, long ld, int k, T *w, int n, double *cbuf, double *hcol,
+               int col, int pass, int refine, double eta, int do_scale, int safe, int *ran) {
+  const double deftol = 10.0 * 2.220446049250313e-16;
+  dim3 grid(g.nb), block(KS_BLOCK);
+#define KS_GS_CASE(RR)                                                                                            \
+  case RR:                                                                                                        \
+    hipLaunchKernelGGL((k_gs_lds<T, RR, FUSED>), grid, block, g.lds, ctx->stream, V, ld, k, w, n, g.nb, g.cpb,    \
+                       g.tsub, cbuf, ctx->d_partial, ctx->d_gpartial, ctx->d_counters, ctx->d_status, hcol, col,  \
+                       pass, refine, eta, do_scale, safe, ran, deftol);                                           \
+    break;
+  switch (g.R) { KS_GS_CASE(256) KS_GS_CASE(128) KS_GS_CASE(64) KS_GS_CASE(32) KS_GS_CASE(16) }
+#undef KS_GS_CASE
+}
+
 template <typename T>
 int orthonormalize_t(ks_ctx ctx, int n, T *V, long ld, int k, double *buf, int ldbuf, int hcol_idx, int col,
                      ks_refine refine, double eta, int do_scale, int safe) {
-  int rpt, nb, e;
-  if ((e = split_checked(n, &rpt, &nb))) return e;
+  gs_geom g;
+  int e;
+  if ((e = gs_geometry(n, k, &g))) return e;
   const double s = sizeof(T);
   T *w = V + (size_t)k * ld;
   double *cbuf = buf;                              // buffer column 0: scratch coefficients
   double *hcol = buf + (size_t)hcol_idx * ldbuf;   // buffer column hcol_idx: accumulated h
   const int passes = safe || refine == KS_REFINE_NEVER ? 1 : (refine == KS_REFINE_ALWAYS ? 2 : 3);
+  // first pass dots: c = V^T w and w^T w
+  int *ran = ks_prof_begin(ctx, KS_PROF_GS_DOT1, (k + 1.0) * n * s, 2.0 * (k + 1.0) * n, true);
+  launch_gs<T, false>(ctx, g, V, ld, k, w, n, cbuf, hcol, col, 0, refine, eta, do_scale, safe, ran);
+  ks_prof_end(ctx);
+  // each pass: update with its c and the next pass's dots in the same sweep
   for (int p = 1; p <= passes; p++) {
-    const bool first = p == 1;
-    int *ran = ks_prof_begin(ctx, first ? KS_PROF_GS_DOT1 : KS_PROF_GS_DOTR, (k + 1.0) * n * s, 2.0 * (k + 1.0) * n,
-                             true);
-    if (first) launch_dot<T, GS_FIRST>(ctx, rpt, nb, V, ld, k, w, n, cbuf, hcol, col, p, ran);
-    else launch_dot<T, GS_REPEAT>(ctx, rpt, nb, V, ld, k, w, n, cbuf, hcol, col, p, ran);
-    ks_prof_end(ctx);
-    ran = ks_prof_begin(ctx, first ? KS_PROF_GS_UPD1 : KS_PROF_GS_UPDR, (k + 2.0) * n * s, 2.0 * k * n + 2.0 * n,
-                        true);
-    if (first)
-      launch_upd<T, GS_FIRST>(ctx, rpt, nb, V, ld, k, w, n, cbuf, -1.0, 1.0, hcol, col, p, refine, eta, do_scale,
-                              safe, ran);
-    else
-      launch_upd<T, GS_REPEAT>(ctx, rpt, nb, V, ld, k, w, n, cbuf, -1.0, 1.0, hcol, col, p, refine, eta, do_scale,
-                               safe, ran);
+    ran = ks_prof_begin(ctx, p == 1 ? KS_PROF_GS_UPD1 : KS_PROF_GS_UPDR, (k + 2.0) * n * s,
+                        2.0 * k * n + 2.0 * (k + 1.0) * n, true);
+    launch_gs<T, true>(ctx, g, V, ld, k, w, n, cbuf, hcol, col, p, refine, eta, do_scale, safe, ran);
     ks_prof_end(ctx);
   }
   if (do_scale) {
