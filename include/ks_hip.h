@@ -98,6 +98,9 @@ int ks_csr_create_device(ks_csr *A, ks_ctx ctx, int nrows, int ncols, int64_t nn
                          const int32_t *d_rowptr, const int32_t *d_colidx, const void *d_val,
                          ks_prec vprec);
 int ks_csr_destroy(ks_csr A);
+/* New matrix with the same pattern and the values converted to `to` on the
+   device (FP32 copy of an FP64 operator for the mixed-precision phase). */
+int ks_csr_convert(ks_csr A, ks_ctx ctx, ks_prec to, ks_csr *out);
 /* nrows, nnz, max row length and the kernel variant chosen (0 stream, 1 vector) */
 int ks_csr_info(ks_csr A, int *nrows, int64_t *nnz, int *max_row, int *variant, int *nblocks);
 /* y = A x.  x, y device vectors in xprec.  When gate != 0 the product is

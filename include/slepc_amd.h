@@ -233,6 +233,12 @@ PetscErrorCode EPSSetInitialSpace(EPS eps, PetscInt n, Vec is[]);
 PetscErrorCode EPSSetDeflationSpace(EPS eps, PetscInt n, Vec v[]);
 PetscErrorCode EPSKrylovSchurSetRestart(EPS eps, PetscReal keep);
 PetscErrorCode EPSKrylovSchurSetLocking(EPS eps, PetscBool lock);
+/* Mixed-precision form (no reference counterpart): an FP32-basis Krylov-Schur
+   phase down to tol32 whose Ritz vectors seed the FP64 restarts; tol32 <= 0
+   turns it off.  The stats report the FP32 phase. */
+PetscErrorCode EPSKrylovSchurSetMixedPrecision(EPS eps, PetscReal tol32);
+PetscErrorCode EPSKrylovSchurGetMixedStats(EPS eps, PetscInt *its32, PetscInt *nconv32, int64_t *nmatmult32,
+                                           int64_t *passes32, PetscReal *t32);
 PetscErrorCode EPSSetBV(EPS eps, BV bv);
 PetscErrorCode EPSGetBV(EPS eps, BV *bv);
 PetscErrorCode EPSGetDS(EPS eps, DS *ds);

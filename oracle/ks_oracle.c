@@ -45,6 +45,17 @@ extern void   dgemm_(const char*, const char*, const int*, const int*, const int
                      const double*, const int*, const double*, const int*, const double*, double*,
                      const int*);
 extern double dnrm2_(const int*, const double*, const int*);
+extern int    MKL_Set_Threading_Layer(int);     /* MKL service (mkl_rt), by value: 3 = GNU OpenMP */
+
+/* MKL threads through the same GNU OpenMP runtime as the loops here; its Intel
+   runtime nested under a GNU OpenMP region gives wrong panel products when a
+   second libgomp (torch's) is the one in the process.  Set before the first
+   BLAS call of every entry point. */
+static void or_mkl_layer(void)
+{
+  static int done = 0;
+  if (!done) { MKL_Set_Threading_Layer(3); done = 1; }
+}
 extern void   dlartg_(double*, double*, double*, double*, double*);
 extern void   drot_(const int*, double*, const int*, double*, const int*, const double*, const double*);
 extern void   dsteqr_(const char*, const int*, double*, double*, double*, const int*, double*, int*);
@@ -239,6 +250,7 @@ static void bv_multinplace(obv *bv, const double *Q, int ldq, int s, int e)
 int or_multinplace(int n, int ld, double *V, int nc, int l, int k,
                    const double *Q, int ldq, int s, int e)
 {
+  or_mkl_layer();
   obv bv = {0};
   bv.n = n; bv.ld = ld; bv.nc = nc; bv.V = V; bv.l = l; bv.k = k;
   bv_multinplace(&bv, Q, ldq, s, e);
@@ -339,6 +351,7 @@ static int orthonormalize_column(obv *bv, int j, double *norm, int *lindep)
 int or_orthonormalize_column(int n, int ld, double *V, int nc, int j, int refine,
                              double eta, double *H, double *norm, int *lindep, int *npasses)
 {
+  or_mkl_layer();
   obv bv = {0};
   int m = j + 1;
   bv.n = n; bv.ld = ld; bv.nc = nc; bv.m = m; bv.V = V;
@@ -703,6 +716,7 @@ int or_ds_run(int type, int ld, int n, int l, int k, int compact, int extrarow,
               int state, double *A, double *T, int which, double *wr, double *wi,
               double *Q, int jvec, double *rnorm, double *X)
 {
+  or_mkl_layer();
   ods ds;
   double *wi_local = NULL;
   if (ds_alloc(&ds, type, ld)) return -1;
@@ -905,6 +919,7 @@ int or_eps_solve(int n, const int *rowptr, const int *colidx, const double *val,
                  double *eigr, double *eigi, double *errest, double *err, double *X,
                  or_info *info)
 {
+  or_mkl_layer();
   oeps eps;
   int e = 0, ncv, mpd, requested;
   double t0;

@@ -171,6 +171,28 @@ int ks_csr_create_device(ks_csr *out, ks_ctx ctx, int nrows, int ncols, int64_t 
   return KS_OK;
 }
 
+int ks_csr_convert(ks_csr A, ks_ctx ctx, ks_prec to, ks_csr *out) {
+  KS_CHECK(A && ctx && out, KS_ERR_ARG, "null argument");
+  ks_csr B = new ks_csr_s();
+  *B = *A;
+  B->vprec = to;
+  B->d_rowptr = B->d_colidx = B->d_blk = nullptr;
+  B->d_val = nullptr;
+  const size_t nz = A->nnz ? (size_t)A->nnz : 1;
+  KS_HIP(hipMalloc(&B->d_rowptr, sizeof(int32_t) * (A->nrows + 1)));
+  KS_HIP(hipMalloc(&B->d_colidx, sizeof(int32_t) * nz));
+  KS_HIP(hipMalloc(&B->d_blk, sizeof(int32_t) * (A->nblocks + 1)));
+  KS_HIP(hipMalloc(&B->d_val, (to == KS_F64 ? 8 : 4) * nz));
+  KS_HIP(hipMemcpyAsync(B->d_rowptr, A->d_rowptr, sizeof(int32_t) * (A->nrows + 1), hipMemcpyDeviceToDevice, ctx->stream));
+  KS_HIP(hipMemcpyAsync(B->d_colidx, A->d_colidx, sizeof(int32_t) * A->nnz, hipMemcpyDeviceToDevice, ctx->stream));
+  KS_HIP(hipMemcpyAsync(B->d_blk, A->d_blk, sizeof(int32_t) * (A->nblocks + 1), hipMemcpyDeviceToDevice, ctx->stream));
+  int e = ks_bv_convert(ctx, (int)A->nnz, A->vprec, A->d_val, to, B->d_val);
+  if (e) { ks_csr_destroy(B); return e; }
+  KS_HIP(hipStreamSynchronize(ctx->stream));
+  *out = B;
+  return KS_OK;
+}
+
 int ks_csr_destroy(ks_csr A) {
   if (!A) return KS_OK;
   (void)hipFree(A->d_rowptr);

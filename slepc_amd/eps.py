@@ -195,6 +195,19 @@ class EPS:
         self._ensure_type()
         _chk(lib().EPSKrylovSchurSetLocking(self.h, C.c_int(int(lock))), "EPSKrylovSchurSetLocking")
 
+    def setMixedPrecision(self, tol32: float):
+        """FP32-basis phase down to tol32, then the FP64 restarts (tol32 <= 0: off)."""
+        self._ensure_type()
+        _chk(lib().EPSKrylovSchurSetMixedPrecision(self.h, C.c_double(tol32)), "EPSKrylovSchurSetMixedPrecision")
+
+    def mixedStats(self) -> dict:
+        its, nc = C.c_int(), C.c_int()
+        nm, gp = C.c_int64(), C.c_int64()
+        t = C.c_double()
+        _chk(lib().EPSKrylovSchurGetMixedStats(self.h, C.byref(its), C.byref(nc), C.byref(nm), C.byref(gp),
+                                               C.byref(t)), "EPSKrylovSchurGetMixedStats")
+        return dict(its32=its.value, nconv32=nc.value, nmatmult32=nm.value, gs_passes32=gp.value, t32=t.value)
+
     def _ensure_type(self):
         t = C.c_char_p()
         lib().EPSGetType.argtypes = [C.c_void_p, C.POINTER(C.c_char_p)]
@@ -294,7 +307,7 @@ class EPS:
 
 
 def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=True, v0=None, max_it=-1,
-          prec=F64, deflation=None, lock=True, keep=0.5, want_vectors=True):
+          prec=F64, deflation=None, lock=True, keep=0.5, want_vectors=True, mixed_tol=0.0):
     """One-call convenience wrapper: returns a dict like the oracle's eps_solve."""
     A = Mat(csr)
     eps = EPS()
@@ -305,6 +318,8 @@ def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=Tru
     eps.setWhichEigenpairs(which)
     eps.setKrylovSchurLocking(lock)
     eps.setKrylovSchurRestart(keep)
+    if mixed_tol > 0:
+        eps.setMixedPrecision(mixed_tol)
     if prec != F64:
         eps.setBVPrecision(prec)
     if v0 is not None:
@@ -324,6 +339,8 @@ def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=Tru
             X[:, i] = eps.getEigenvector(i)
         out["X"] = X
     out.update(eps.stats())
+    if mixed_tol > 0:
+        out.update(eps.mixedStats())
     eps.destroy()
     A.destroy()
     return out

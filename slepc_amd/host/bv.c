@@ -381,19 +381,69 @@ PetscErrorCode BVGetBufferHost(BV bv, const PetscScalar **a, PetscInt *ld)
   return PETSC_SUCCESS;
 }
 
-/* the expansion loop of BVMatArnoldi/BVMatLanczos; the type may fuse it */
-static PetscErrorCode BVMatKrylov_Private(BV V, Mat A, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown)
+/* The expansion loop of BVMatArnoldi/BVMatLanczos (bvkrylov.c:88-97), split:
+   BVMatKrylovBegin enqueues steps k..m-1 (the type may keep them on the
+   device), BVMatKrylovEnd waits and returns the breakdown position and beta,
+   with the new coefficient columns in the host mirror of the buffer. */
+PetscErrorCode BVMatKrylovBegin(BV V, Mat A, PetscInt k, PetscInt m)
 {
   PetscBool lindep = PETSC_FALSE;
-  if (V->ops.matkrylov) return V->ops.matkrylov(V, A, k, m, beta, breakdown);
-  for (PetscInt j = k; j < *m; j++) {
+  PetscReal beta = 0.0;
+  PetscCall(BVCheckReady(V));
+  PetscCheck(k >= 0 && k <= V->m, PETSC_ERR_ARG_OUTOFRANGE, "Argument k has wrong value %d, should be between 0 and %d", k, V->m);
+  PetscCheck(m > 0 && m <= V->m, PETSC_ERR_ARG_OUTOFRANGE, "Argument m has wrong value %d, should be between 1 and %d", m, V->m);
+  PetscCheck(m > k, PETSC_ERR_ARG_OUTOFRANGE, "Argument m should be at least equal to k+1");
+  if (V->ops.matkrylovbegin) return V->ops.matkrylovbegin(V, A, k, m);
+  for (PetscInt j = k; j < m; j++) {
     PetscCall(BVMatMultColumn(V, A, j));
-    if (j == V->N - 1) PetscCall(BVOrthogonalizeColumn_Safe(V, j + 1, beta, &lindep));
-    else PetscCall(BVOrthonormalizeColumn(V, j + 1, PETSC_FALSE, beta, &lindep));
-    if (lindep) { *m = j + 1; break; }
+    if (j == V->N - 1) PetscCall(BVOrthogonalizeColumn_Safe(V, j + 1, &beta, &lindep));
+    else PetscCall(BVOrthonormalizeColumn(V, j + 1, PETSC_FALSE, &beta, &lindep));
+    if (lindep) { m = j + 1; break; }
   }
-  *breakdown = lindep;
+  V->kry_m = m;
+  V->kry_beta = beta;
+  V->kry_breakdown = lindep;
   PetscCall(V->ops.getbuffer(V));
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode BVMatKrylovEnd(BV V, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown)
+{
+  if (V->ops.matkrylovend) return V->ops.matkrylovend(V, k, m, beta, breakdown);
+  *m = V->kry_m;
+  *beta = V->kry_beta;
+  *breakdown = V->kry_breakdown;
+  return PETSC_SUCCESS;
+}
+
+/* H columns k..m-1 from the coefficient buffer (bvkrylov.c:100-109) */
+PetscErrorCode BVCopyHessenberg(BV V, Mat H, PetscInt k, PetscInt m)
+{
+  Mat_SeqDense *hd = (Mat_SeqDense *)H->data;
+  PetscScalar *h = hd->v;
+  PetscInt ldh = hd->lda, ldb = V->nc + V->m;
+  const PetscScalar *a = V->buffer;
+  PetscCheck(!strcmp(H->type, MATSEQDENSE), PETSC_ERR_ARG_WRONG, "H must be seqdense");
+  PetscCheck(H->m >= m && H->n >= m, PETSC_ERR_ARG_SIZ, "Matrix H is too small");
+  for (PetscInt j = k; j < m - 1; j++) memcpy(h + (size_t)j * ldh, a + V->nc + (size_t)(j + 1) * ldb, sizeof(PetscScalar) * (size_t)(j + 2));
+  memcpy(h + (size_t)(m - 1) * ldh, a + V->nc + (size_t)m * ldb, sizeof(PetscScalar) * (size_t)m);
+  if (ldh > m) h[m + (size_t)(m - 1) * ldh] = a[V->nc + m + (size_t)m * ldb];
+  return PETSC_SUCCESS;
+}
+
+/* alpha, beta of steps k..m-1 (bvkrylov.c:210-222): T stores alpha in column 0, beta in column 1 */
+PetscErrorCode BVCopyTridiagonal(BV V, Mat T, PetscInt k, PetscInt m)
+{
+  Mat_SeqDense *td = (Mat_SeqDense *)T->data;
+  PetscReal *alpha = td->v, *betat = td->v + td->lda;
+  PetscInt ldb = V->nc + V->m;
+  const PetscScalar *a = V->buffer;
+  PetscCheck(!strcmp(T->type, MATSEQDENSE), PETSC_ERR_ARG_WRONG, "T must be seqdense");
+  PetscCheck(T->m >= m && T->n >= 2, PETSC_ERR_ARG_SIZ, "Matrix T is too small");
+  for (PetscInt j = k; j < m; j++) {
+    alpha[j] = a[V->nc + j + (size_t)(j + 1) * ldb];
+    betat[j] = a[V->nc + j + 1 + (size_t)(j + 1) * ldb];
+  }
   return PETSC_SUCCESS;
 }
 
@@ -402,53 +452,24 @@ PetscErrorCode BVMatArnoldi(BV V, Mat A, Mat H, PetscInt k, PetscInt *m, PetscRe
 {
   PetscBool bd = PETSC_FALSE;
   PetscReal b = 0.0;
-  PetscCall(BVCheckReady(V));
-  PetscCheck(k >= 0 && k <= V->m, PETSC_ERR_ARG_OUTOFRANGE, "Argument k has wrong value %d, should be between 0 and %d", k, V->m);
-  PetscCheck(*m > 0 && *m <= V->m, PETSC_ERR_ARG_OUTOFRANGE, "Argument m has wrong value %d, should be between 1 and %d", *m, V->m);
-  PetscCheck(*m > k, PETSC_ERR_ARG_OUTOFRANGE, "Argument m should be at least equal to k+1");
-  if (H) {
-    PetscCheck(!strcmp(H->type, MATSEQDENSE), PETSC_ERR_ARG_WRONG, "H must be seqdense");
-    PetscCheck(H->m >= *m && H->n >= *m, PETSC_ERR_ARG_SIZ, "Matrix H is too small");
-  }
-  PetscCall(BVMatKrylov_Private(V, A, k, m, &b, &bd));
+  PetscCall(BVMatKrylovBegin(V, A, k, *m));
+  PetscCall(BVMatKrylovEnd(V, k, m, &b, &bd));
   if (breakdown) *breakdown = bd;
   if (beta) *beta = b;
-  if (H) {
-    Mat_SeqDense *hd = (Mat_SeqDense *)H->data;
-    PetscScalar *h = hd->v;
-    PetscInt ldh = hd->lda, ldb = V->nc + V->m;
-    const PetscScalar *a = V->buffer;
-    for (PetscInt j = k; j < *m - 1; j++) memcpy(h + (size_t)j * ldh, a + V->nc + (size_t)(j + 1) * ldb, sizeof(PetscScalar) * (size_t)(j + 2));
-    memcpy(h + (size_t)(*m - 1) * ldh, a + V->nc + (size_t)(*m) * ldb, sizeof(PetscScalar) * (size_t)(*m));
-    if (ldh > *m) h[(*m) + (size_t)(*m - 1) * ldh] = a[V->nc + (*m) + (size_t)(*m) * ldb];
-  }
+  if (H) PetscCall(BVCopyHessenberg(V, H, k, *m));
   return PETSC_SUCCESS;
 }
 
-/* BVMatLanczos (bvkrylov.c:165-226): T stores alpha in column 0, beta in column 1 */
+/* BVMatLanczos (bvkrylov.c:165-226) */
 PetscErrorCode BVMatLanczos(BV V, Mat A, Mat T, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown)
 {
   PetscBool bd = PETSC_FALSE;
   PetscReal b = 0.0;
-  PetscCall(BVCheckReady(V));
-  PetscCheck(*m > k, PETSC_ERR_ARG_OUTOFRANGE, "Argument m should be at least equal to k+1");
-  if (T) {
-    PetscCheck(!strcmp(T->type, MATSEQDENSE), PETSC_ERR_ARG_WRONG, "T must be seqdense");
-    PetscCheck(T->m >= *m && T->n >= 2, PETSC_ERR_ARG_SIZ, "Matrix T is too small");
-  }
-  PetscCall(BVMatKrylov_Private(V, A, k, m, &b, &bd));
+  PetscCall(BVMatKrylovBegin(V, A, k, *m));
+  PetscCall(BVMatKrylovEnd(V, k, m, &b, &bd));
   if (breakdown) *breakdown = bd;
   if (beta) *beta = b;
-  if (T) {
-    Mat_SeqDense *td = (Mat_SeqDense *)T->data;
-    PetscReal *alpha = td->v, *betat = td->v + td->lda;
-    PetscInt ldb = V->nc + V->m;
-    const PetscScalar *a = V->buffer;
-    for (PetscInt j = k; j < *m; j++) {
-      alpha[j] = a[V->nc + j + (size_t)(j + 1) * ldb];
-      betat[j] = a[V->nc + j + 1 + (size_t)(j + 1) * ldb];
-    }
-  }
+  if (T) PetscCall(BVCopyTridiagonal(V, T, k, *m));
   return PETSC_SUCCESS;
 }
 

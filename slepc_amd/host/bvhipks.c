@@ -11,7 +11,8 @@
  *   ops       dotvec/multvec (tall-skinny kernels), scale/norm/copy tails,
  *             multinplace (FP64 MFMA, no n x (e-s) temporary), random columns,
  *             orthonormalizecolumn (all CGS passes + DGKS + scaling on the device),
- *             matkrylov (a whole Arnoldi/Lanczos expansion with one host sync).
+ *             matkrylovbegin/end (a whole Arnoldi/Lanczos expansion enqueued
+ *             at once, one host sync when its results are collected).
  *
  * Storage precision is FP64 by default; BVHIPKSSetPrecision(bv, SLEPC_AMD_F32)
  * keeps V in FP32 (mixed-precision phase) while coefficients stay FP64.
@@ -158,16 +159,14 @@ static PetscErrorCode BVOrthonormalizeColumn_HIPKS(BV bv, PetscInt j, PetscBool 
    ordered: SpMV, the CGS passes, the DGKS decisions, lindep and the scaling
    stay on the device; the host waits once at the end for the status and the
    new coefficient columns. */
-static PetscErrorCode BVMatKrylov_HIPKS(BV V, Mat A, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown)
+static PetscErrorCode BVMatKrylovBegin_HIPKS(BV V, Mat A, PetscInt k, PetscInt m)
 {
-  BV_HIPKS *h = (BV_HIPKS *)V->data;
-  ks_status st;
   PetscBool gate = !strcmp(A->type, MATHIPCSR);
   Vec vj = NULL, vj1 = NULL;
   PetscErrorCode e = PETSC_SUCCESS;
   KSCall(ks_status_reset(V->ctx));
   if (gate) A->gate = PETSC_TRUE;
-  for (PetscInt j = k; j < *m && !e; j++) {
+  for (PetscInt j = k; j < m && !e; j++) {
     PetscCall(BVGetColumn(V, j, &vj));
     PetscCall(BVGetColumn(V, j + 1, &vj1));
     e = MatMult(A, vj, vj1);
@@ -178,7 +177,15 @@ static PetscErrorCode BVMatKrylov_HIPKS(BV V, Mat A, PetscInt k, PetscInt *m, Pe
     else e = BVOrthonormalizeColumn_HIPKS(V, j + 1, PETSC_TRUE, PETSC_FALSE, PETSC_TRUE, NULL, NULL);
   }
   A->gate = PETSC_FALSE;
-  PetscCall(e);
+  V->kry_m = m;
+  return e;
+}
+
+static PetscErrorCode BVMatKrylovEnd_HIPKS(BV V, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown)
+{
+  BV_HIPKS *h = (BV_HIPKS *)V->data;
+  ks_status st;
+  *m = V->kry_m;
   KSCall(ks_status_get(V->ctx, &st));
   PetscCall(StatusCheck(&st));
   V->gs_passes += st.total_passes;
@@ -242,7 +249,8 @@ PetscErrorCode BVCreate_HIPKS(BV bv)
   bv->ops.multinplace = BVMultInPlace_HIPKS;
   bv->ops.getbuffer = BVGetBuffer_HIPKS;
   bv->ops.orthonormalizecolumn = BVOrthonormalizeColumn_HIPKS;
-  bv->ops.matkrylov = BVMatKrylov_HIPKS;
+  bv->ops.matkrylovbegin = BVMatKrylovBegin_HIPKS;
+  bv->ops.matkrylovend = BVMatKrylovEnd_HIPKS;
   bv->ops.destroy = BVDestroy_HIPKS;
   bv->ops.create = BVCreate_HIPKS;
   if (bv->m > 0 && bv->n > 0 && !bv->data) PetscCall(BVAllocate_HIPKS(bv));

@@ -82,88 +82,241 @@ static PetscErrorCode EPSSetUp_KrylovSchur(EPS eps)
   return PETSC_SUCCESS;
 }
 
-/* EPSSolve_KrylovSchur_Default (krylovschur.c:227-337) */
-static PetscErrorCode EPSSolve_KrylovSchur_Default(EPS eps)
+/* EPSSolve_KrylovSchur_Default (krylovschur.c:227-337), written as a
+   resumable state machine so that several solves can be interleaved:
+     KSExpand      -- top of the restart loop: dimensions, then enqueue the
+                      nv-step Arnoldi/Lanczos expansion on the device;
+     KSAfterExpand -- wait for it, solve the projected problem, test
+                      convergence, truncate, enqueue the restart GEMM and the
+                      residual-vector copy, then expand again or finish.
+   EPSSolve runs them back to back; a serving loop polls the device between. */
+static PetscErrorCode KSExpand(EPS eps)
 {
   EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
-  PetscInt k = 0, l, nv, ld, nconv;
-  Mat U, Op, H, T;
+  Mat Op;
+  PetscInt nv;
+  eps->its++;
+  nv = PetscMin(eps->nconv + eps->mpd, eps->ncv);
+  PetscCall(DSSetDimensions(eps->ds, nv, eps->nconv, eps->nconv + ctx->l));
+  PetscCall(STGetOperator(eps->st, &Op));
+  eps->nmatmult += nv - (eps->nconv + ctx->l);
+  PetscCall(BVMatKrylovBegin(eps->V, Op, eps->nconv + ctx->l, nv));
+  PetscCall(STRestoreOperator(eps->st, &Op));
+  ctx->nv = nv;
+  ctx->phase = KS_PHASE_EXPANDING;
+  return PETSC_SUCCESS;
+}
+
+static PetscErrorCode KSAfterExpand(EPS eps)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  PetscInt k = 0, l = ctx->l, nv = ctx->nv, nconv;
+  Mat U, H, T;
   PetscReal beta = 0.0;
   PetscBool breakdown = PETSC_FALSE, hermitian = eps->ishermitian;
   double t0;
 
-  PetscCall(DSGetLeadingDimension(eps->ds, &ld));
-  /* get the starting Arnoldi vector */
-  PetscCall(EPSGetStartVector(eps, 0, NULL));
-  l = 0;
-
-  /* restart loop */
-  while (eps->reason == EPS_CONVERGED_ITERATING) {
-    eps->its++;
-
-    /* compute an nv-step Arnoldi factorization */
-    nv = PetscMin(eps->nconv + eps->mpd, eps->ncv);
-    PetscCall(DSSetDimensions(eps->ds, nv, eps->nconv, eps->nconv + l));
-    PetscCall(STGetOperator(eps->st, &Op));
-    eps->nmatmult += nv - (eps->nconv + l);
-    if (hermitian) {
-      PetscCall(DSGetMat(eps->ds, DS_MAT_T, &T));
-      PetscCall(BVMatLanczos(eps->V, Op, T, eps->nconv + l, &nv, &beta, &breakdown));
-      PetscCall(DSRestoreMat(eps->ds, DS_MAT_T, &T));
-    } else {
-      PetscCall(DSGetMat(eps->ds, DS_MAT_A, &H));
-      PetscCall(BVMatArnoldi(eps->V, Op, H, eps->nconv + l, &nv, &beta, &breakdown));
-      PetscCall(DSRestoreMat(eps->ds, DS_MAT_A, &H));
-    }
-    PetscCall(STRestoreOperator(eps->st, &Op));
-    PetscCall(DSSetDimensions(eps->ds, nv, eps->nconv, eps->nconv + l));
-    PetscCall(DSSetState(eps->ds, l ? DS_STATE_RAW : DS_STATE_INTERMEDIATE));
-    PetscCall(BVSetActiveColumns(eps->V, eps->nconv, nv));
-
-    /* solve the projected problem */
-    t0 = SlepcWallTime();
-    PetscCall(DSSolve(eps->ds, eps->eigr, eps->eigi));
-    PetscCall(DSSort(eps->ds, eps->eigr, eps->eigi, NULL, NULL, NULL));
-    PetscCall(DSUpdateExtraRow(eps->ds));
-
-    /* check convergence */
-    PetscCall(EPSKrylovConvergence(eps, PETSC_FALSE, eps->nconv, nv - eps->nconv, beta, 1.0, &k));
-    eps->t_ds += SlepcWallTime() - t0;
-    /* EPSStoppingBasic (epsdefault.c:290-302) */
-    if (k >= eps->nev) eps->reason = EPS_CONVERGED_TOL;
-    else if (eps->its >= eps->max_it) eps->reason = EPS_DIVERGED_ITS;
-    nconv = k;
-
-    /* update l */
-    if (eps->reason != EPS_CONVERGED_ITERATING || breakdown || k == nv) l = 0;
-    else {
-      l = PetscMax(1, (PetscInt)((nv - k) * ctx->keep));
-      if (!hermitian) PetscCall(DSGetTruncateSize(eps->ds, k, nv, &l));
-    }
-    if (!ctx->lock && l > 0) { l += k; k = 0; }   /* non-locking variant: reset no. of converged pairs */
-
-    if (eps->reason == EPS_CONVERGED_ITERATING) {
-      if (breakdown || k == nv) {
-        /* start a new Arnoldi factorization */
-        if (k < eps->nev) {
-          PetscCall(EPSGetStartVector(eps, k, &breakdown));
-          if (breakdown) eps->reason = EPS_DIVERGED_BREAKDOWN;
-        }
-      } else {
-        /* prepare the Rayleigh quotient for restart */
-        PetscCall(DSTruncate(eps->ds, k + l, PETSC_FALSE));
-      }
-    }
-    /* update the corresponding vectors V(:,idx) = V*Q(:,idx) */
-    PetscCall(DSGetMat(eps->ds, DS_MAT_Q, &U));
-    PetscCall(BVMultInPlace(eps->V, U, eps->nconv, k + l));
-    PetscCall(DSRestoreMat(eps->ds, DS_MAT_Q, &U));
-
-    if (eps->reason == EPS_CONVERGED_ITERATING && !breakdown) PetscCall(BVCopyColumn(eps->V, nv, k + l));
-    eps->nconv = k;
-    if (eps->monitor) PetscCall(eps->monitor(eps, eps->its, nconv, eps->eigr, eps->eigi, eps->errest, nv, eps->monitorctx));
+  PetscCall(BVMatKrylovEnd(eps->V, eps->nconv + l, &nv, &beta, &breakdown));
+  if (hermitian) {
+    PetscCall(DSGetMat(eps->ds, DS_MAT_T, &T));
+    PetscCall(BVCopyTridiagonal(eps->V, T, eps->nconv + l, nv));
+    PetscCall(DSRestoreMat(eps->ds, DS_MAT_T, &T));
+  } else {
+    PetscCall(DSGetMat(eps->ds, DS_MAT_A, &H));
+    PetscCall(BVCopyHessenberg(eps->V, H, eps->nconv + l, nv));
+    PetscCall(DSRestoreMat(eps->ds, DS_MAT_A, &H));
   }
-  PetscCall(DSTruncate(eps->ds, eps->nconv, PETSC_TRUE));
+  PetscCall(DSSetDimensions(eps->ds, nv, eps->nconv, eps->nconv + l));
+  PetscCall(DSSetState(eps->ds, l ? DS_STATE_RAW : DS_STATE_INTERMEDIATE));
+  PetscCall(BVSetActiveColumns(eps->V, eps->nconv, nv));
+
+  /* solve the projected problem */
+  t0 = SlepcWallTime();
+  PetscCall(DSSolve(eps->ds, eps->eigr, eps->eigi));
+  PetscCall(DSSort(eps->ds, eps->eigr, eps->eigi, NULL, NULL, NULL));
+  PetscCall(DSUpdateExtraRow(eps->ds));
+
+  /* check convergence */
+  PetscCall(EPSKrylovConvergence(eps, PETSC_FALSE, eps->nconv, nv - eps->nconv, beta, 1.0, &k));
+  eps->t_ds += SlepcWallTime() - t0;
+  /* EPSStoppingBasic (epsdefault.c:290-302) */
+  if (k >= eps->nev) eps->reason = EPS_CONVERGED_TOL;
+  else if (eps->its >= eps->max_it) eps->reason = EPS_DIVERGED_ITS;
+  nconv = k;
+
+  /* update l */
+  if (eps->reason != EPS_CONVERGED_ITERATING || breakdown || k == nv) l = 0;
+  else {
+    l = PetscMax(1, (PetscInt)((nv - k) * ctx->keep));
+    if (!hermitian) PetscCall(DSGetTruncateSize(eps->ds, k, nv, &l));
+  }
+  if (!ctx->lock && l > 0) { l += k; k = 0; }   /* non-locking variant: reset no. of converged pairs */
+
+  if (eps->reason == EPS_CONVERGED_ITERATING) {
+    if (breakdown || k == nv) {
+      /* start a new Arnoldi factorization */
+      if (k < eps->nev) {
+        PetscCall(EPSGetStartVector(eps, k, &breakdown));
+        if (breakdown) eps->reason = EPS_DIVERGED_BREAKDOWN;
+      }
+    } else {
+      /* prepare the Rayleigh quotient for restart */
+      PetscCall(DSTruncate(eps->ds, k + l, PETSC_FALSE));
+    }
+  }
+  /* update the corresponding vectors V(:,idx) = V*Q(:,idx) */
+  PetscCall(DSGetMat(eps->ds, DS_MAT_Q, &U));
+  PetscCall(BVMultInPlace(eps->V, U, eps->nconv, k + l));
+  PetscCall(DSRestoreMat(eps->ds, DS_MAT_Q, &U));
+
+  if (eps->reason == EPS_CONVERGED_ITERATING && !breakdown) PetscCall(BVCopyColumn(eps->V, nv, k + l));
+  eps->nconv = k;
+  ctx->l = l;
+  if (eps->monitor) PetscCall(eps->monitor(eps, eps->its, nconv, eps->eigr, eps->eigi, eps->errest, nv, eps->monitorctx));
+
+  if (eps->reason == EPS_CONVERGED_ITERATING) PetscCall(KSExpand(eps));
+  else {
+    PetscCall(DSTruncate(eps->ds, eps->nconv, PETSC_TRUE));
+    ctx->phase = KS_PHASE_DONE;
+  }
+  return PETSC_SUCCESS;
+}
+
+/* the start of the restart loop: starting vector, first expansion */
+static PetscErrorCode KSStart(EPS eps)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  PetscCall(EPSGetStartVector(eps, 0, NULL));
+  ctx->l = 0;
+  return KSExpand(eps);
+}
+
+/* Mixed-precision first phase: the same Krylov-Schur on an FP32 basis (FP32
+   SpMV on an FP32 copy of the operator values, FP32 Gram-Schmidt products with
+   FP64 DGKS scalars and an FP64 projected problem) down to ctx->mixed_tol.  Its
+   converged Ritz vectors are summed into the FP64 start vector of the main
+   solve (column 0 of V, used as the initial space), so the FP64 restarts start
+   from a vector whose unwanted components are already at the FP32 tolerance. */
+static PetscErrorCode EPSKrylovSchurMixedPhase(EPS eps)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data, *c32;
+  EPS e32;
+  BV bv32;
+  Vec v64 = NULL, acc = NULL, xr = NULL, xi = NULL, *defl = NULL;
+  PetscInt nconv32, nuse;
+  double t0 = SlepcWallTime();
+
+  PetscCall(EPSCreate(&e32));
+  PetscCall(EPSSetOperators(e32, eps->A, NULL));
+  PetscCall(EPSSetProblemType(e32, eps->problem_type));
+  PetscCall(EPSSetWhichEigenpairs(e32, eps->which));
+  PetscCall(EPSSetTarget(e32, eps->target));
+  PetscCall(EPSSetDimensions(e32, eps->nev, eps->ncv, eps->mpd));
+  PetscCall(EPSSetTolerances(e32, ctx->mixed_tol, eps->max_it));
+  PetscCall(EPSSetType(e32, EPSKRYLOVSCHUR));
+  c32 = (EPS_KRYLOVSCHUR *)e32->data;
+  c32->keep = ctx->keep;
+  c32->lock = ctx->lock;
+  PetscCall(EPSGetBV(e32, &bv32));
+  PetscCall(BVSetType(bv32, BVHIPKS));
+  PetscCall(BVHIPKSSetPrecision(bv32, SLEPC_AMD_F32));
+  bv32->orthog_ref = eps->V->orthog_ref;
+  bv32->orthog_eta = eps->V->orthog_eta;
+  bv32->rseed = eps->V->rseed;
+  /* same start: the FP64 initial vector inserted by EPSSetUp, rounded to FP32 */
+  if (eps->nini > 0) {
+    PetscCall(VecCreateHIP(eps->n, SLEPC_AMD_F32, &v64));
+    PetscCall(BVCopyVec(eps->V, 0, v64));
+    PetscCall(EPSSetInitialSpace(e32, 1, &v64));
+    PetscCall(VecDestroy(&v64));
+  }
+  /* same deflation space */
+  if (eps->nds > 0) {
+    PetscCall(PetscMallocZero(sizeof(Vec) * (size_t)eps->nds, &defl));
+    for (PetscInt i = 0; i < eps->nds; i++) {
+      PetscCall(VecCreateHIP(eps->n, SLEPC_AMD_F32, &defl[i]));
+      PetscCall(BVCopyVec(eps->V, i - eps->nds, defl[i]));
+    }
+    PetscCall(EPSSetDeflationSpace(e32, eps->nds, defl));
+    for (PetscInt i = 0; i < eps->nds; i++) PetscCall(VecDestroy(&defl[i]));
+    free(defl);
+  }
+  PetscCall(EPSSolve(e32));
+
+  nconv32 = e32->nconv;
+  ctx->its32 = e32->its;
+  ctx->nconv32 = nconv32;
+  ctx->nmatmult32 = e32->nmatmult;
+  ctx->passes32 = e32->V->gs_passes;
+  nuse = PetscMin(nconv32, eps->nev);
+  if (nuse > 0) {
+    PetscCall(VecCreateHIP(eps->n, SLEPC_AMD_F32, &acc));
+    PetscCall(VecCreateHIP(eps->n, SLEPC_AMD_F32, &xr));
+    PetscCall(VecCreateHIP(eps->n, SLEPC_AMD_F32, &xi));
+    for (PetscInt i = 0; i < nuse; i++) {
+      PetscScalar kr, ki;
+      PetscCall(EPSGetEigenpair(e32, i, &kr, &ki, xr, xi));
+      PetscCall(VecAXPY(acc, 1.0, xr));
+      if (ki != 0.0) PetscCall(VecAXPY(acc, 1.0, xi));
+    }
+    PetscCall(BVInsertVec(eps->V, 0, acc));   /* FP32 -> FP64 conversion */
+    eps->nini = 1;
+    PetscCall(VecDestroy(&acc));
+    PetscCall(VecDestroy(&xr));
+    PetscCall(VecDestroy(&xi));
+  }
+  PetscCall(EPSDestroy(&e32));
+  ctx->t32 = SlepcWallTime() - t0;
+  return PETSC_SUCCESS;
+}
+
+static PetscErrorCode EPSSolveBegin_KrylovSchur(EPS eps)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  ctx->its32 = ctx->nconv32 = 0;
+  ctx->nmatmult32 = ctx->passes32 = 0;
+  ctx->t32 = 0.0;
+  if (ctx->mixed_tol > 0.0 && ctx->mixed_tol > eps->tol && eps->V->prec == SLEPC_AMD_F64)
+    PetscCall(EPSKrylovSchurMixedPhase(eps));
+  return KSStart(eps);
+}
+
+/* one restart: collects the expansion in flight and enqueues the next one */
+static PetscErrorCode EPSSolveStep_KrylovSchur(EPS eps, PetscBool *done)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  if (ctx->phase == KS_PHASE_EXPANDING) PetscCall(KSAfterExpand(eps));
+  *done = ctx->phase == KS_PHASE_DONE;
+  return PETSC_SUCCESS;
+}
+
+static PetscErrorCode EPSSolve_KrylovSchur(EPS eps)
+{
+  PetscBool done = PETSC_FALSE;
+  PetscCall(EPSSolveBegin_KrylovSchur(eps));
+  while (!done) PetscCall(EPSSolveStep_KrylovSchur(eps, &done));
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode EPSKrylovSchurSetMixedPrecision(EPS eps, PetscReal tol32)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  PetscCheck(!strcmp(eps->type, EPSKRYLOVSCHUR), PETSC_ERR_ARG_WRONG, "not a Krylov-Schur solver");
+  ctx->mixed_tol = tol32 > 0.0 ? tol32 : 0.0;
+  eps->state = EPS_STATE_INITIAL;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode EPSKrylovSchurGetMixedStats(EPS eps, PetscInt *its32, PetscInt *nconv32, int64_t *nmatmult32,
+                                           int64_t *passes32, PetscReal *t32)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  PetscCheck(!strcmp(eps->type, EPSKRYLOVSCHUR), PETSC_ERR_ARG_WRONG, "not a Krylov-Schur solver");
+  if (its32) *its32 = ctx->its32;
+  if (nconv32) *nconv32 = ctx->nconv32;
+  if (nmatmult32) *nmatmult32 = ctx->nmatmult32;
+  if (passes32) *passes32 = ctx->passes32;
+  if (t32) *t32 = ctx->t32;
   return PETSC_SUCCESS;
 }
 
@@ -209,7 +362,9 @@ PetscErrorCode EPSCreate_KrylovSchur(EPS eps)
   ctx->lock = PETSC_TRUE;
   ctx->keep = 0.0;
   eps->ops.setup = EPSSetUp_KrylovSchur;
-  eps->ops.solve = EPSSolve_KrylovSchur_Default;
+  eps->ops.solve = EPSSolve_KrylovSchur;
+  eps->ops.solvebegin = EPSSolveBegin_KrylovSchur;
+  eps->ops.solvestep = EPSSolveStep_KrylovSchur;
   eps->ops.computevectors = EPSComputeVectors_KrylovSchur;
   eps->ops.destroy = NULL;
   return PETSC_SUCCESS;

@@ -135,12 +135,25 @@ PetscErrorCode MatDenseGetLDA(Mat A, PetscInt *lda)
 }
 
 /* ---------------------------------------------------------------- MATHIPCSR (device) */
+/* FP32 copy of FP64 values, built on the device the first time an FP32 vector
+   is multiplied (the mixed-precision phase reads half the matrix bytes) */
+static PetscErrorCode MatHIPCSRGetF32(Mat A, ks_csr *out)
+{
+  Mat_HIPCSR *h = (Mat_HIPCSR *)A->data;
+  if (h->vprec == SLEPC_AMD_F32) { *out = h->csr; return PETSC_SUCCESS; }
+  if (!h->csr32) KSCall(ks_csr_convert(h->csr, SlepcAMDContext(), KS_F32, &h->csr32));
+  *out = h->csr32;
+  return PETSC_SUCCESS;
+}
+
 static PetscErrorCode MatMult_HIPCSR(Mat A, Vec x, Vec y)
 {
   Mat_HIPCSR *h = (Mat_HIPCSR *)A->data;
+  ks_csr csr = h->csr;
   PetscCheck(h->csr, PETSC_ERR_ORDER, "MatSeqAIJSetPreallocationCSR must be called first");
   PetscCheck(x->prec == y->prec, PETSC_ERR_ARG_WRONG, "x and y precisions differ");
-  KSCall(ks_spmv(SlepcAMDContext(), h->csr, x->d, y->d, x->prec == SLEPC_AMD_F64 ? KS_F64 : KS_F32, A->gate));
+  if (x->prec == SLEPC_AMD_F32) PetscCall(MatHIPCSRGetF32(A, &csr));
+  KSCall(ks_spmv(SlepcAMDContext(), csr, x->d, y->d, x->prec == SLEPC_AMD_F64 ? KS_F64 : KS_F32, A->gate));
   return PETSC_SUCCESS;
 }
 
@@ -148,6 +161,7 @@ static PetscErrorCode MatDestroy_HIPCSR(Mat A)
 {
   Mat_HIPCSR *h = (Mat_HIPCSR *)A->data;
   if (h && h->csr) KSCall(ks_csr_destroy(h->csr));
+  if (h && h->csr32) KSCall(ks_csr_destroy(h->csr32));
   free(h);
   A->data = NULL;
   return PETSC_SUCCESS;
@@ -172,6 +186,7 @@ static PetscErrorCode MatHIPCSRSetHost(Mat A, const PetscInt i[], const PetscInt
   PetscCheck(A->m > 0 && A->n > 0, PETSC_ERR_ORDER, "MatSetSizes must be called first");
   h = (Mat_HIPCSR *)A->data;
   if (h->csr) { KSCall(ks_csr_destroy(h->csr)); h->csr = NULL; }
+  if (h->csr32) { KSCall(ks_csr_destroy(h->csr32)); h->csr32 = NULL; }
   KSCall(ks_csr_create(&h->csr, SlepcAMDContext(), A->m, A->n, (int64_t)i[A->m], i, j, v,
                        vprec == SLEPC_AMD_F64 ? KS_F64 : KS_F32));
   h->vprec = vprec;
@@ -196,6 +211,7 @@ PetscErrorCode MatHIPCSRSetDeviceArrays(Mat A, PetscInt nnz, const PetscInt *d_i
   PetscCheck(!strcmp(A->type, MATHIPCSR), PETSC_ERR_ARG_WRONG, "matrix type is %s, expected %s", A->type, MATHIPCSR);
   h = (Mat_HIPCSR *)A->data;
   if (h->csr) { KSCall(ks_csr_destroy(h->csr)); h->csr = NULL; }
+  if (h->csr32) { KSCall(ks_csr_destroy(h->csr32)); h->csr32 = NULL; }
   KSCall(ks_csr_create_device(&h->csr, SlepcAMDContext(), A->m, A->n, nnz, d_i, d_j, d_v,
                               vprec == SLEPC_AMD_F64 ? KS_F64 : KS_F32));
   h->vprec = vprec;

@@ -95,6 +95,7 @@ typedef struct {         /* MATSEQDENSE, host column-major */
 } Mat_SeqDense;
 typedef struct {         /* MATHIPCSR */
   ks_csr csr;
+  ks_csr csr32;          /* FP32 copy of FP64 values for FP32 vectors (built on demand) */
   SlepcAMDPrecision vprec;
 } Mat_HIPCSR;
 PetscErrorCode MatCreateSeqDenseView(PetscInt m, PetscInt n, PetscScalar *data, PetscInt lda, Mat *A);
@@ -119,8 +120,11 @@ typedef struct _BVOps {
      device (norm/lindep may be NULL); otherwise they are returned. */
   PetscErrorCode (*orthonormalizecolumn)(BV, PetscInt j, PetscBool scale, PetscBool safe, PetscBool async,
                                          PetscReal *norm, PetscBool *lindep);
-  /* Whole Arnoldi/Lanczos expansion k..m with one host synchronization (optional). */
-  PetscErrorCode (*matkrylov)(BV, Mat, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown);
+  /* Whole Arnoldi/Lanczos expansion k..m split in two (optional): begin enqueues
+     every step on the device without waiting, end waits once and returns the
+     breakdown position, beta and the new coefficient columns (host mirror). */
+  PetscErrorCode (*matkrylovbegin)(BV, Mat, PetscInt k, PetscInt m);
+  PetscErrorCode (*matkrylovend)(BV, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown);
 } BVOps;
 
 struct _p_BV {
@@ -140,8 +144,16 @@ struct _p_BV {
   SlepcAMDPrecision prec;
   ks_ctx ctx;
   int64_t gs_passes;         /* executed CGS passes */
+  /* results of an expansion done synchronously by BVMatKrylovBegin */
+  PetscInt kry_m;
+  PetscReal kry_beta;
+  PetscBool kry_breakdown;
 };
 PetscErrorCode BVOrthogonalizeColumn_Safe(BV bv, PetscInt j, PetscReal *norm, PetscBool *lindep);
+PetscErrorCode BVMatKrylovBegin(BV V, Mat A, PetscInt k, PetscInt m);
+PetscErrorCode BVMatKrylovEnd(BV V, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown);
+PetscErrorCode BVCopyHessenberg(BV V, Mat H, PetscInt k, PetscInt m);
+PetscErrorCode BVCopyTridiagonal(BV V, Mat T, PetscInt k, PetscInt m);
 
 /* ---------------------------------------------------------------- DS */
 typedef struct {
@@ -179,6 +191,8 @@ struct _p_ST {
 typedef struct {
   PetscErrorCode (*setup)(EPS);
   PetscErrorCode (*solve)(EPS);
+  PetscErrorCode (*solvebegin)(EPS);               /* optional resumable form of solve */
+  PetscErrorCode (*solvestep)(EPS, PetscBool *);
   PetscErrorCode (*computevectors)(EPS);
   PetscErrorCode (*destroy)(EPS);
 } EPSOps;
@@ -212,9 +226,19 @@ struct _p_EPS {
   PetscReal t_ds, t_solve;
 };
 
+typedef enum { KS_PHASE_IDLE = 0, KS_PHASE_EXPANDING = 1, KS_PHASE_DONE = 2 } KSPhase;
+
 typedef struct {            /* EPS_KRYLOVSCHUR (krylovschur.h:76-105), default variant fields */
   PetscReal keep;
   PetscBool lock;
+  /* restart-loop state kept between the halves of an iteration */
+  KSPhase phase;
+  PetscInt l, nv;
+  /* mixed-precision form: FP32 basis down to mixed_tol, then FP64 restarts */
+  PetscReal mixed_tol;       /* <= 0: off */
+  PetscInt  its32, nconv32;
+  int64_t   nmatmult32, passes32;
+  PetscReal t32;
 } EPS_KRYLOVSCHUR;
 
 PetscErrorCode EPSKrylovConvergence(EPS eps, PetscBool getall, PetscInt kini, PetscInt nits, PetscReal beta,

@@ -64,11 +64,18 @@ void *SlepcFunctionListFind(SlepcFunctionList *fl, const char *name)
   return NULL;
 }
 
+/* MKL service (mkl_rt), by value.  3 selects its GNU OpenMP threading layer:
+   the layer is fixed for the process at MKL's first call, and Intel-threaded
+   MKL called from inside a GNU OpenMP region (a host application's, or the
+   CPU checks') computes wrong results once a second libgomp is loaded. */
+extern int MKL_Set_Threading_Layer(int);
+
 /* SlepcInitialize: selects the device, creates the shared stream context and
    registers the built-in types (dlregisslepc.c registers the reference's). */
 PetscErrorCode SlepcInitialize(int device)
 {
   if (g_initialized) return PETSC_SUCCESS;
+  (void)MKL_Set_Threading_Layer(3);
   KSCall(ks_init(device));
   KSCall(ks_ctx_create(&g_ctx));
   PetscCall(MatRegister(MATHIPCSR, MatCreate_HIPCSR));

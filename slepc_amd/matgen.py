@@ -95,14 +95,17 @@ def varcoef_laplacian_3d(N: int, seed: int = 1):
 def convection_diffusion_2d(N: int, peclet: float = 10.0, seed: int = 2):
     """-div(k grad u) + w . grad u on the unit square, N x N interior cells,
     Dirichlet boundary, first-order upwind convection with constant velocity
-    w = (1, 1) scaled so that the grid Peclet number |w| h / (2 k_mean) equals
-    ``peclet``; k is uniform(0.5, 2) per cell (default_rng(seed)); face
-    diffusion is the harmonic mean of the two cells.  Row x + N*y."""
+    along the diagonal, |w| chosen so that the (global) Peclet number
+    |w| L / k_mean equals ``peclet`` (L = 1, k_mean = 1.25); k is uniform(0.5, 2)
+    per cell (default_rng(seed)); face diffusion is the harmonic mean of the two
+    cells.  Row x + N*y.  (A grid Peclet number of 10 instead would make the
+    operator so non-normal that Krylov Ritz values land in the pseudospectrum,
+    far from the eigenvalues, at any working precision.)"""
     rng = np.random.default_rng(seed)
     k = rng.uniform(0.5, 2.0, size=(N, N))          # [y, x]
     h = 1.0 / (N + 1)
     kmean = 1.25
-    wmag = peclet * 2.0 * kmean / h
+    wmag = peclet * kmean
     wx = wy = wmag / np.sqrt(2.0)
     # face diffusion coefficients (harmonic mean; boundary faces use the cell value)
     kx = np.empty((N, N + 1))
