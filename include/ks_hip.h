@@ -40,6 +40,7 @@ typedef enum { KS_REFINE_IFNEEDED = 0, KS_REFINE_NEVER = 1, KS_REFINE_ALWAYS = 2
 
 typedef struct ks_ctx_s *ks_ctx;
 typedef struct ks_csr_s *ks_csr;
+typedef struct ks_pool_s *ks_pool;
 
 /* Device-side state of the fused CGS+DGKS orthonormalization (the scalars of
    BVOrthogonalizeGS, src/sys/classes/bv/interface/bvorthog.c:145-217, kept on
@@ -83,6 +84,22 @@ int ks_host_alloc(void **ptr, size_t bytes);   /* pinned host memory */
 int ks_host_free(void *ptr);
 const char *ks_error_string(int code);
 int ks_mem_info(size_t *free_bytes, size_t *total_bytes);
+/* 1 in *idle when every operation enqueued on the context's stream has
+   finished (hipStreamQuery), 0 while work is pending; never waits. */
+int ks_ctx_query(ks_ctx ctx, int *idle);
+
+/* ---------------------------------------------------------------- pools
+ * No reference counterpart (PETSc allocates per object).  A pool is one device
+ * arena (plus an optional pinned host arena) with a first-fit allocator: while
+ * a pool is current (ks_pool_use), ks_malloc/ks_host_alloc and the CSR
+ * constructors carve from it, and ks_free/ks_host_free return pointers to the
+ * pool that holds them.  Serving admits and retires solves against it without
+ * hipMalloc/hipFree.  ks_pool_use returns the previously current pool (NULL =
+ * plain hipMalloc).  Destroying a pool with live blocks fails. */
+int ks_pool_create(ks_pool *pool, size_t dev_bytes, size_t host_bytes);
+int ks_pool_destroy(ks_pool pool);
+ks_pool ks_pool_use(ks_pool pool);
+int ks_pool_info(ks_pool pool, size_t *used, size_t *peak, size_t *size, size_t *largest_free);
 
 /* ---------------------------------------------------------------- CSR matrix
  * Replaces PETSc MatMult_SeqAIJHIPSPARSE / MatMult_SeqAIJ, called from

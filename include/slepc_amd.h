@@ -246,6 +246,13 @@ PetscErrorCode EPSGetST(EPS eps, ST *st);
 PetscErrorCode EPSMonitorSet(EPS eps, EPSMonitorFn fn, void *ctx);
 PetscErrorCode EPSSetUp(EPS eps);
 PetscErrorCode EPSSolve(EPS eps);
+/* EPSSolve in parts, for callers that interleave solves (no reference
+   counterpart; EPSSolve == Begin, Step until done, End).  EPSSolveReady
+   reports whether the next Step would find the device work it collects done. */
+PetscErrorCode EPSSolveBegin(EPS eps);
+PetscErrorCode EPSSolveReady(EPS eps, PetscBool *ready);
+PetscErrorCode EPSSolveStep(EPS eps, PetscBool *done);
+PetscErrorCode EPSSolveEnd(EPS eps);
 PetscErrorCode EPSGetConverged(EPS eps, PetscInt *nconv);
 PetscErrorCode EPSGetIterationNumber(EPS eps, PetscInt *its);
 PetscErrorCode EPSGetConvergedReason(EPS eps, EPSConvergedReason *reason);
@@ -265,6 +272,42 @@ PetscErrorCode MatCreate_HIPCSR(Mat A);
 
 /* the kernel-library context used by the objects (ks_ctx of ks_hip.h) */
 void *SlepcAMDGetDefaultContext(void);
+
+/* ---------------------------------------------------------------- serving
+ * No reference counterpart (SLEPc runs one EPSSolve at a time).  A server
+ * admits eigenproblems as they arrive, in arrival order, while a slot is free
+ * and their HBM (CSR + basis + work vectors) fits the budget; each admitted
+ * problem is an ordinary EPSKRYLOVSCHUR solve on its own stream, advanced one
+ * restart at a time by SlepcServerProgress, which never waits on the device.
+ * The CSR arrays and v0 must stay valid until the request finishes. */
+typedef struct {
+  PetscInt n;
+  const PetscInt *rowptr, *colidx;
+  const PetscScalar *val;
+  PetscInt nev, ncv, max_it;     /* max_it <= 0: the solver's default */
+  PetscReal tol;
+  EPSWhich which;                /* 0: largest magnitude */
+  PetscBool hermitian;
+  const PetscScalar *v0;         /* start vector (length n) or NULL */
+  PetscBool want_vectors;        /* keep the eigenvectors on the host */
+  double t_arrival;              /* on the SlepcServerClock clock; <= 0: submission time */
+} SlepcRequest;
+typedef struct {
+  PetscInt nconv, nret, ngood, its; /* converged, returned (<= nev), returned with residual <= tol, restarts */
+  EPSConvergedReason reason;
+  int64_t nmatmult, gs_passes;
+  double t_arrival, t_admit, t_done;
+} SlepcResult;
+typedef struct _p_SlepcServer *SlepcServer;
+PetscErrorCode SlepcServerCreate(size_t hbm_budget, PetscInt max_active, SlepcServer *srv);
+PetscErrorCode SlepcServerSubmit(SlepcServer srv, const SlepcRequest *req, PetscInt *id);
+PetscErrorCode SlepcServerProgress(SlepcServer srv, PetscBool *progressed, PetscInt *nactive, PetscInt *nqueued,
+                                   PetscInt *nfinished);
+PetscErrorCode SlepcServerGetResult(SlepcServer srv, PetscInt id, PetscBool *done, SlepcResult *res,
+                                    PetscScalar *eigr, PetscScalar *eigi, PetscReal *err, PetscScalar *X);
+PetscErrorCode SlepcServerGetStats(SlepcServer srv, size_t *peak_reserved, size_t *pool_peak, size_t *budget);
+PetscErrorCode SlepcServerDestroy(SlepcServer *srv);
+double SlepcServerClock(void);   /* monotonic seconds, the clock of t_arrival/t_admit/t_done */
 
 #ifdef __cplusplus
 }

@@ -251,10 +251,23 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_lds(const T *__restrict__ V, lo
     const long i = r0 + r;
     const bool ok = i < n;
     double acc = 0.0;
-    for (int c = g; c < k; c += G) {
-      const double v = ok ? (double)V[i + (size_t)c * ld] : 0.0;
-      Vs[(size_t)c * S + r] = v;
-      if (FUSED) acc += v * coef[c];
+    // KS_GS_BATCH independent column loads are issued before any is used, so
+    // each thread keeps that many HBM requests in flight
+    for (int c0 = g; c0 < k; c0 += G * KS_GS_BATCH) {
+      double v[KS_GS_BATCH];
+#pragma unroll
+      for (int u = 0; u < KS_GS_BATCH; u++) {
+        const int c = c0 + u * G;
+        v[u] = (ok && c < k) ? (double)V[i + (size_t)c * ld] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < KS_GS_BATCH; u++) {
+        const int c = c0 + u * G;
+        if (c < k) {
+          Vs[(size_t)c * S + r] = v[u];
+          if (FUSED) acc += v[u] * coef[c];
+        }
+      }
     }
     if (FUSED) {
       red[g * R + r] = acc;

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <vector>
 
 #include "../../include/ks_hip.h"
@@ -12,6 +13,9 @@
 #define KS_WAVES (KS_BLOCK / 64)
 #define KS_MAXB 8192                 // maximum blocks of one row-partitioned reduction
 #define KS_GROUP 32                  // blocks per first-level reduction group
+#ifndef KS_GS_BATCH
+#define KS_GS_BATCH 8                // column loads in flight per thread in the Gram-Schmidt sweep
+#endif
 #define KS_MAXG (KS_MAXB / KS_GROUP)
 #define KS_NCOLP (KS_MAX_COLS + 2)   // partial row width: columns + self dot + spare
 #define KS_SPMV_NNZ 1024             // products staged in LDS per SpMV row block
@@ -25,6 +29,12 @@
               __FILE__, __LINE__);                                                      \
       return KS_ERR_HIP;                                                                \
     }                                                                                   \
+  } while (0)
+
+#define KS_TRY(call)          \
+  do {                        \
+    int _e = (call);          \
+    if (_e) return _e;        \
   } while (0)
 
 #define KS_CHECK(cond, code, ...)                                                       \
@@ -58,6 +68,61 @@ struct ks_ctx_s {
   int *d_ran = nullptr;            // per-launch "did work" flags
   int ran_cap = 0, ran_used = 0;
   ks_prof_entry acc[KS_PROF_NCLASS] = {};
+};
+
+// first-fit arena over [base, base+size), blocks aligned to KS_POOL_ALIGN
+#define KS_POOL_ALIGN 256
+struct ks_arena {
+  char *base = nullptr;
+  size_t size = 0, used = 0, peak = 0;
+  std::map<size_t, size_t> free_;   // offset -> length, coalesced
+  std::map<size_t, size_t> taken;   // offset -> length
+  void init(char *b, size_t n) {
+    base = b; size = n; used = peak = 0;
+    free_.clear(); taken.clear();
+    if (n) free_[0] = n;
+  }
+  bool owns(const void *p) const { return base && (const char *)p >= base && (const char *)p < base + size; }
+  void *take(size_t n) {
+    n = (n + KS_POOL_ALIGN - 1) / KS_POOL_ALIGN * KS_POOL_ALIGN;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < n) continue;
+      size_t off = it->first, len = it->second;
+      free_.erase(it);
+      if (len > n) free_[off + n] = len - n;
+      taken[off] = n;
+      used += n;
+      if (used > peak) peak = used;
+      return base + off;
+    }
+    return nullptr;
+  }
+  bool give(const void *p) {
+    size_t off = (size_t)((const char *)p - base);
+    auto t = taken.find(off);
+    if (t == taken.end()) return false;
+    size_t len = t->second;
+    taken.erase(t);
+    used -= len;
+    auto nx = free_.lower_bound(off);
+    if (nx != free_.end() && off + len == nx->first) { len += nx->second; free_.erase(nx); }
+    auto nxt = free_.lower_bound(off);
+    if (nxt != free_.begin()) {
+      auto pv = std::prev(nxt);
+      if (pv->first + pv->second == off) { pv->second += len; return true; }
+    }
+    free_[off] = len;
+    return true;
+  }
+  size_t largest() const {
+    size_t m = 0;
+    for (auto &f : free_) m = f.second > m ? f.second : m;
+    return m;
+  }
+};
+
+struct ks_pool_s {
+  ks_arena dev, host;
 };
 
 struct ks_csr_s {

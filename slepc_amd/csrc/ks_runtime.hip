@@ -3,6 +3,8 @@
 #include "ks_internal.hpp"
 
 #include <cstring>
+#include <map>
+#include <vector>
 
 namespace {
 
@@ -39,6 +41,14 @@ __global__ void k_status_reset(ks_status *st) {
 }
 
 bool g_initialized = false;
+
+// Pools: a device arena (and optionally a pinned host arena) carved by a
+// first-fit allocator, so a serving loop admits and retires solves without
+// hipMalloc/hipFree (hipFree waits for the whole device).  ks_malloc draws from
+// the pool made current by ks_pool_use; ks_free returns a pointer to the pool
+// whose range holds it.
+std::vector<ks_pool> g_pools;
+ks_pool g_pool_current = nullptr;
 
 }  // namespace
 
@@ -127,6 +137,15 @@ int ks_malloc(void **ptr, size_t bytes) {
   KS_CHECK(ptr, KS_ERR_ARG, "null pointer");
   *ptr = nullptr;
   if (bytes == 0) bytes = 256;
+  if (g_pool_current) {
+    *ptr = g_pool_current->dev.take(bytes);
+    if (!*ptr) {
+      fprintf(stderr, "ks_hip: device pool exhausted (%zu bytes asked, %zu of %zu in use)\n", bytes,
+              g_pool_current->dev.used, g_pool_current->dev.size);
+      return KS_ERR_NOMEM;
+    }
+    return KS_OK;
+  }
   hipError_t e = hipMalloc(ptr, bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -137,7 +156,10 @@ int ks_malloc(void **ptr, size_t bytes) {
 }
 
 int ks_free(void *ptr) {
-  if (ptr) KS_HIP(hipFree(ptr));
+  if (!ptr) return KS_OK;
+  for (ks_pool p : g_pools)
+    if (p->dev.owns(ptr)) return p->dev.give(ptr) ? KS_OK : KS_ERR_ARG;
+  KS_HIP(hipFree(ptr));
   return KS_OK;
 }
 
@@ -159,13 +181,89 @@ int ks_memcpy_htod_async(ks_ctx c, void *dst, const void *src, size_t bytes) {
 int ks_host_alloc(void **ptr, size_t bytes) {
   KS_CHECK(ptr, KS_ERR_ARG, "null pointer");
   *ptr = nullptr;
+  if (g_pool_current && g_pool_current->host.size) {
+    *ptr = g_pool_current->host.take(bytes ? bytes : 64);
+    if (!*ptr) {
+      fprintf(stderr, "ks_hip: pinned host pool exhausted (%zu bytes asked)\n", bytes);
+      return KS_ERR_NOMEM;
+    }
+    return KS_OK;
+  }
   KS_HIP(hipHostMalloc(ptr, bytes ? bytes : 64, hipHostMallocDefault));
   return KS_OK;
 }
 
 int ks_host_free(void *ptr) {
-  if (ptr) KS_HIP(hipHostFree(ptr));
+  if (!ptr) return KS_OK;
+  for (ks_pool p : g_pools)
+    if (p->host.owns(ptr)) return p->host.give(ptr) ? KS_OK : KS_ERR_ARG;
+  KS_HIP(hipHostFree(ptr));
   return KS_OK;
+}
+
+/* ---------------------------------------------------------------- pools */
+int ks_pool_create(ks_pool *out, size_t dev_bytes, size_t host_bytes) {
+  KS_CHECK(out && dev_bytes > 0, KS_ERR_ARG, "bad pool arguments");
+  *out = nullptr;
+  auto *p = new ks_pool_s();
+  void *d = nullptr, *h = nullptr;
+  dev_bytes = (dev_bytes + KS_POOL_ALIGN - 1) / KS_POOL_ALIGN * KS_POOL_ALIGN;
+  host_bytes = (host_bytes + KS_POOL_ALIGN - 1) / KS_POOL_ALIGN * KS_POOL_ALIGN;
+  if (hipMalloc(&d, dev_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    delete p;
+    fprintf(stderr, "ks_hip: pool hipMalloc(%zu) failed\n", dev_bytes);
+    return KS_ERR_NOMEM;
+  }
+  if (host_bytes && hipHostMalloc(&h, host_bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(d);
+    delete p;
+    return KS_ERR_NOMEM;
+  }
+  p->dev.init((char *)d, dev_bytes);
+  p->host.init((char *)h, host_bytes);
+  g_pools.push_back(p);
+  *out = p;
+  return KS_OK;
+}
+
+int ks_pool_destroy(ks_pool p) {
+  if (!p) return KS_OK;
+  KS_CHECK(p->dev.used == 0 && p->host.used == 0, KS_ERR_ARG,
+           "pool destroyed with %zu device / %zu host bytes still allocated", p->dev.used, p->host.used);
+  if (g_pool_current == p) g_pool_current = nullptr;
+  for (size_t i = 0; i < g_pools.size(); i++)
+    if (g_pools[i] == p) { g_pools.erase(g_pools.begin() + (long)i); break; }
+  KS_HIP(hipDeviceSynchronize());
+  (void)hipFree(p->dev.base);
+  if (p->host.base) (void)hipHostFree(p->host.base);
+  delete p;
+  return KS_OK;
+}
+
+ks_pool ks_pool_use(ks_pool p) {
+  ks_pool prev = g_pool_current;
+  g_pool_current = p;
+  return prev;
+}
+
+int ks_pool_info(ks_pool p, size_t *used, size_t *peak, size_t *size, size_t *largest_free) {
+  KS_CHECK(p, KS_ERR_ARG, "null pool");
+  if (used) *used = p->dev.used;
+  if (peak) *peak = p->dev.peak;
+  if (size) *size = p->dev.size;
+  if (largest_free) *largest_free = p->dev.largest();
+  return KS_OK;
+}
+
+int ks_ctx_query(ks_ctx c, int *idle) {
+  KS_CHECK(c && idle, KS_ERR_ARG, "bad query arguments");
+  hipError_t e = hipStreamQuery(c->stream);
+  if (e == hipSuccess) { *idle = 1; return KS_OK; }
+  if (e == hipErrorNotReady) { (void)hipGetLastError(); *idle = 0; return KS_OK; }
+  fprintf(stderr, "ks_hip: hipStreamQuery: %s\n", hipGetErrorString(e));
+  return KS_ERR_HIP;
 }
 
 int ks_memcpy_dtoh(ks_ctx c, void *dst, const void *src, size_t bytes) {

@@ -94,7 +94,7 @@ int csr_finish(ks_csr A, ks_ctx ctx, const int32_t *h_rowptr) {
   A->max_row = mr;
   A->nblocks = (int)blocks.size() - 1;
   A->variant = mr > KS_SPMV_NNZ ? 1 : 0;
-  KS_HIP(hipMalloc(&A->d_blk, sizeof(int32_t) * blocks.size()));
+  KS_TRY(ks_malloc((void **)&A->d_blk, sizeof(int32_t) * blocks.size()));
   KS_HIP(hipMemcpyAsync(A->d_blk, blocks.data(), sizeof(int32_t) * blocks.size(), hipMemcpyHostToDevice, ctx->stream));
   KS_HIP(hipStreamSynchronize(ctx->stream));
   return KS_OK;
@@ -113,6 +113,13 @@ int csr_check_host(int nrows, int ncols, int64_t nnz, const int32_t *rowptr, con
 
 }  // namespace
 
+// allocation inside a CSR constructor: release what was built on failure
+#define KS_TRY_CSR(M, call)                 \
+  do {                                      \
+    int _e = (call);                        \
+    if (_e) { ks_csr_destroy(M); return _e; } \
+  } while (0)
+
 extern "C" {
 
 int ks_csr_create(ks_csr *out, ks_ctx ctx, int nrows, int ncols, int64_t nnz, const int32_t *rowptr,
@@ -127,9 +134,9 @@ int ks_csr_create(ks_csr *out, ks_ctx ctx, int nrows, int ncols, int64_t nnz, co
   A->nnz = nnz;
   A->vprec = vprec;
   size_t vs = vprec == KS_F64 ? 8 : 4;
-  KS_HIP(hipMalloc(&A->d_rowptr, sizeof(int32_t) * (nrows + 1)));
-  KS_HIP(hipMalloc(&A->d_colidx, sizeof(int32_t) * (nnz ? nnz : 1)));
-  KS_HIP(hipMalloc(&A->d_val, vs * (nnz ? nnz : 1)));
+  KS_TRY_CSR(A, ks_malloc((void **)&A->d_rowptr, sizeof(int32_t) * (nrows + 1)));
+  KS_TRY_CSR(A, ks_malloc((void **)&A->d_colidx, sizeof(int32_t) * (nnz ? nnz : 1)));
+  KS_TRY_CSR(A, ks_malloc((void **)&A->d_val, vs * (nnz ? nnz : 1)));
   KS_HIP(hipMemcpyAsync(A->d_rowptr, rowptr, sizeof(int32_t) * (nrows + 1), hipMemcpyHostToDevice, ctx->stream));
   if (nnz) {
     KS_HIP(hipMemcpyAsync(A->d_colidx, colidx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, ctx->stream));
@@ -157,9 +164,9 @@ int ks_csr_create_device(ks_csr *out, ks_ctx ctx, int nrows, int ncols, int64_t 
   A->nnz = nnz;
   A->vprec = vprec;
   size_t vs = vprec == KS_F64 ? 8 : 4;
-  KS_HIP(hipMalloc(&A->d_rowptr, sizeof(int32_t) * (nrows + 1)));
-  KS_HIP(hipMalloc(&A->d_colidx, sizeof(int32_t) * (nnz ? nnz : 1)));
-  KS_HIP(hipMalloc(&A->d_val, vs * (nnz ? nnz : 1)));
+  KS_TRY_CSR(A, ks_malloc((void **)&A->d_rowptr, sizeof(int32_t) * (nrows + 1)));
+  KS_TRY_CSR(A, ks_malloc((void **)&A->d_colidx, sizeof(int32_t) * (nnz ? nnz : 1)));
+  KS_TRY_CSR(A, ks_malloc((void **)&A->d_val, vs * (nnz ? nnz : 1)));
   KS_HIP(hipMemcpyAsync(A->d_rowptr, d_rowptr, sizeof(int32_t) * (nrows + 1), hipMemcpyDeviceToDevice, ctx->stream));
   if (nnz) {
     KS_HIP(hipMemcpyAsync(A->d_colidx, d_colidx, sizeof(int32_t) * nnz, hipMemcpyDeviceToDevice, ctx->stream));
@@ -179,10 +186,10 @@ int ks_csr_convert(ks_csr A, ks_ctx ctx, ks_prec to, ks_csr *out) {
   B->d_rowptr = B->d_colidx = B->d_blk = nullptr;
   B->d_val = nullptr;
   const size_t nz = A->nnz ? (size_t)A->nnz : 1;
-  KS_HIP(hipMalloc(&B->d_rowptr, sizeof(int32_t) * (A->nrows + 1)));
-  KS_HIP(hipMalloc(&B->d_colidx, sizeof(int32_t) * nz));
-  KS_HIP(hipMalloc(&B->d_blk, sizeof(int32_t) * (A->nblocks + 1)));
-  KS_HIP(hipMalloc(&B->d_val, (to == KS_F64 ? 8 : 4) * nz));
+  KS_TRY_CSR(B, ks_malloc((void **)&B->d_rowptr, sizeof(int32_t) * (A->nrows + 1)));
+  KS_TRY_CSR(B, ks_malloc((void **)&B->d_colidx, sizeof(int32_t) * nz));
+  KS_TRY_CSR(B, ks_malloc((void **)&B->d_blk, sizeof(int32_t) * (A->nblocks + 1)));
+  KS_TRY_CSR(B, ks_malloc((void **)&B->d_val, (to == KS_F64 ? 8 : 4) * nz));
   KS_HIP(hipMemcpyAsync(B->d_rowptr, A->d_rowptr, sizeof(int32_t) * (A->nrows + 1), hipMemcpyDeviceToDevice, ctx->stream));
   KS_HIP(hipMemcpyAsync(B->d_colidx, A->d_colidx, sizeof(int32_t) * A->nnz, hipMemcpyDeviceToDevice, ctx->stream));
   KS_HIP(hipMemcpyAsync(B->d_blk, A->d_blk, sizeof(int32_t) * (A->nblocks + 1), hipMemcpyDeviceToDevice, ctx->stream));
@@ -195,10 +202,10 @@ int ks_csr_convert(ks_csr A, ks_ctx ctx, ks_prec to, ks_csr *out) {
 
 int ks_csr_destroy(ks_csr A) {
   if (!A) return KS_OK;
-  (void)hipFree(A->d_rowptr);
-  (void)hipFree(A->d_colidx);
-  (void)hipFree(A->d_val);
-  (void)hipFree(A->d_blk);
+  (void)ks_free(A->d_rowptr);
+  (void)ks_free(A->d_colidx);
+  (void)ks_free(A->d_val);
+  (void)ks_free(A->d_blk);
   delete A;
   return KS_OK;
 }

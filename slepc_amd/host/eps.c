@@ -269,27 +269,65 @@ PetscErrorCode EPSSetUp(EPS eps)
   return PETSC_SUCCESS;
 }
 
-/* EPSSolve (epssolve.c:119-208) */
-PetscErrorCode EPSSolve(EPS eps)
+/* EPSSolve (epssolve.c:119-208), in three parts so a caller can interleave
+   several solves: EPSSolveBegin (set-up, start of the restart loop),
+   EPSSolveStep (one restart; EPSSolveReady says whether it would wait for the
+   device) and EPSSolveEnd (ordering of the converged pairs).  EPSSolve runs
+   them back to back. */
+PetscErrorCode EPSSolveBegin(EPS eps)
 {
   PetscInt i;
-  double t0;
   PetscCheck(eps, PETSC_ERR_ARG_NULL, "null EPS");
-  if (eps->state >= EPS_STATE_SOLVED) return PETSC_SUCCESS;
+  PetscCheck(eps->state < EPS_STATE_SOLVED, PETSC_ERR_ARG_WRONGSTATE, "EPS already solved; change its settings first");
   PetscCall(EPSSetUp(eps));
-  t0 = SlepcWallTime();
+  eps->t_begin = SlepcWallTime();
   eps->nconv = 0;
   eps->its = 0;
   eps->nmatmult = 0;
   eps->t_ds = 0.0;
   eps->V->gs_passes = 0;
+  eps->stepping = PETSC_FALSE;
   for (i = 0; i < eps->ncv; i++) {
     eps->eigr[i] = 0.0;
     eps->eigi[i] = 0.0;
     eps->errest[i] = 0.0;
     eps->perm[i] = i;
   }
-  PetscCall(eps->ops.solve(eps));
+  if (eps->ops.solvebegin) {
+    PetscCall(eps->ops.solvebegin(eps));
+    eps->stepping = PETSC_TRUE;
+  } else {
+    PetscCall(eps->ops.solve(eps));
+  }
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode EPSSolveReady(EPS eps, PetscBool *ready)
+{
+  int idle = 1;
+  PetscCheck(eps && ready, PETSC_ERR_ARG_NULL, "null argument");
+  if (eps->stepping) KSCall(ks_ctx_query(eps->V->ctx, &idle));
+  *ready = idle ? PETSC_TRUE : PETSC_FALSE;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode EPSSolveStep(EPS eps, PetscBool *done)
+{
+  PetscBool d = PETSC_TRUE;
+  PetscCheck(eps && done, PETSC_ERR_ARG_NULL, "null argument");
+  if (eps->stepping) {
+    PetscCall(eps->ops.solvestep(eps, &d));
+    if (d) eps->stepping = PETSC_FALSE;
+  }
+  *done = d;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode EPSSolveEnd(EPS eps)
+{
+  PetscInt i;
+  PetscCheck(eps, PETSC_ERR_ARG_NULL, "null EPS");
+  PetscCheck(!eps->stepping, PETSC_ERR_ARG_WRONGSTATE, "EPSSolveStep() has not finished");
   PetscCheck(eps->reason, PETSC_ERR_PLIB, "Internal error, solver returned without setting converged reason");
   eps->state = EPS_STATE_SOLVED;
   PetscCall(BVSetActiveColumns(eps->V, 0, eps->nconv));
@@ -306,12 +344,22 @@ PetscErrorCode EPSSolve(EPS eps)
     }
   }
   SlepcSortEigenvalues(&eps->sc, eps->nconv, eps->eigr, eps->eigi, eps->perm);
-  KSCall(ks_ctx_sync(SlepcAMDContext()));
-  eps->t_solve = SlepcWallTime() - t0;
+  KSCall(ks_ctx_sync(eps->V->ctx));
+  eps->t_solve = SlepcWallTime() - eps->t_begin;
   /* remove deflation and initial subspaces */
   if (eps->nds) eps->nds = 0;
   eps->nini = 0;
   return PETSC_SUCCESS;
+}
+
+PetscErrorCode EPSSolve(EPS eps)
+{
+  PetscBool done = PETSC_FALSE;
+  PetscCheck(eps, PETSC_ERR_ARG_NULL, "null EPS");
+  if (eps->state >= EPS_STATE_SOLVED) return PETSC_SUCCESS;
+  PetscCall(EPSSolveBegin(eps));
+  while (!done) PetscCall(EPSSolveStep(eps, &done));
+  return EPSSolveEnd(eps);
 }
 
 PetscErrorCode EPSComputeVectors(EPS eps)

@@ -1,0 +1,315 @@
+/*
+ * serve.c -- a request loop over the Krylov-Schur solver: eigenproblems of
+ * uneven size arrive while others run, are admitted against an HBM budget,
+ * and every admitted solve keeps its CSR matrix, Krylov basis and projected
+ * problem resident until it finishes.
+ *
+ * No reference counterpart: SLEPc solves one EPS at a time per process.  Each
+ * solve here is the unchanged EPSKRYLOVSCHUR on BVHIPKS/MATHIPCSR, driven in
+ * parts (EPSSolveBegin / EPSSolveReady / EPSSolveStep / EPSSolveEnd), so a
+ * request gives the same result alone and in the mix.
+ *
+ * Device side: one context (own HIP stream, own DGKS status, own reduction
+ * workspace) per slot, created once; all request memory comes from one device
+ * arena (ks_pool) sized to the budget, so admission and retirement never call
+ * hipMalloc/hipFree.  The expansions of all active requests run concurrently
+ * on their streams while the host solves the projected problems of the ones
+ * whose expansions have finished.
+ */
+#include "slepcimpl.h"
+
+typedef enum { REQ_QUEUED = 0, REQ_ACTIVE = 1, REQ_DONE = 2, REQ_FAILED = 3 } ReqState;
+
+typedef struct {
+  SlepcRequest in;
+  ReqState state;
+  size_t bytes;               /* reserved HBM */
+  int slot;
+  Mat A;
+  EPS eps;
+  SlepcResult res;
+  PetscScalar *eigr, *eigi;   /* nev each */
+  PetscReal *err;             /* relative residual of each returned pair */
+  PetscScalar *X;             /* n x nev eigenvectors (want_vectors) */
+} Request;
+
+struct _p_SlepcServer {
+  size_t budget, reserved, peak_reserved;
+  PetscInt max_active, nactive;
+  ks_ctx *slot_ctx;
+  PetscInt *slot_req;         /* request id per slot, -1 free */
+  ks_pool pool;
+  Request *req;
+  PetscInt nreq, capreq, next_queued;   /* FIFO admission cursor */
+  PetscInt nfinished;
+};
+
+/* HBM a request holds while active: CSR (values, indices, row pointers, row
+   blocks), the (ncv+1)-column basis, the coefficient buffer, the restart
+   matrix, the start vector and the residual-check work vectors; each block
+   rounded to the pool alignment. */
+static size_t RequestBytes(const SlepcRequest *r)
+{
+  const size_t al = 256;
+  size_t n = (size_t)r->n, nnz = (size_t)r->rowptr[r->n], ld = (n + 63) / 64 * 64;
+  size_t m = (size_t)r->ncv + 1, b = 0;
+#define ROUND(x) (((x) + al - 1) / al * al)
+  b += ROUND(4 * (n + 1)) + ROUND(4 * (nnz ? nnz : 1)) + ROUND(8 * (nnz ? nnz : 1)) + ROUND(4 * (n + 2));
+  b += ROUND(8 * ld * m) + ROUND(8 * m * m) + ROUND(8 * (m + 2)) + ROUND(8 * m * m);
+  b += 7 * ROUND(8 * n);
+#undef ROUND
+  return b + 64 * 1024;
+}
+
+PetscErrorCode SlepcServerCreate(size_t hbm_budget, PetscInt max_active, SlepcServer *out)
+{
+  SlepcServer s;
+  PetscCall(SlepcCheckInitialized());
+  PetscCheck(out && hbm_budget > 0 && max_active > 0, PETSC_ERR_ARG_OUTOFRANGE, "bad server arguments");
+  PetscCall(PetscMallocZero(sizeof(*s), &s));
+  s->budget = hbm_budget;
+  s->max_active = max_active;
+  PetscCall(PetscMallocZero(sizeof(ks_ctx) * (size_t)max_active, &s->slot_ctx));
+  PetscCall(PetscMallocZero(sizeof(PetscInt) * (size_t)max_active, &s->slot_req));
+  for (PetscInt i = 0; i < max_active; i++) {
+    KSCall(ks_ctx_create(&s->slot_ctx[i]));
+    s->slot_req[i] = -1;
+  }
+  /* the arena holds the budget plus per-slot pinned staging of the restart
+     matrices; 1/16 extra absorbs fragmentation */
+  KSCall(ks_pool_create(&s->pool, hbm_budget + hbm_budget / 16, (size_t)max_active * 4 * 1024 * 1024));
+  *out = s;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode SlepcServerSubmit(SlepcServer s, const SlepcRequest *r, PetscInt *id)
+{
+  Request *q;
+  PetscCheck(s && r && id, PETSC_ERR_ARG_NULL, "null argument");
+  PetscCheck(r->n > 0 && r->rowptr && r->colidx && r->val, PETSC_ERR_ARG_NULL, "request without a matrix");
+  PetscCheck(r->nev >= 1 && r->ncv > r->nev && r->ncv <= r->n && r->ncv + 1 <= KS_MAX_COLS, PETSC_ERR_ARG_OUTOFRANGE,
+             "request nev=%d ncv=%d n=%d not supported", (int)r->nev, (int)r->ncv, (int)r->n);
+  if (s->nreq == s->capreq) {
+    PetscInt cap = s->capreq ? 2 * s->capreq : 64;
+    Request *nr = (Request *)realloc(s->req, sizeof(Request) * (size_t)cap);
+    PetscCheck(nr, PETSC_ERR_MEM, "out of host memory");
+    memset(nr + s->capreq, 0, sizeof(Request) * (size_t)(cap - s->capreq));
+    s->req = nr;
+    s->capreq = cap;
+  }
+  q = &s->req[s->nreq];
+  memset(q, 0, sizeof(*q));
+  q->in = *r;
+  if (q->in.t_arrival <= 0.0) q->in.t_arrival = SlepcWallTime();
+  q->bytes = RequestBytes(r);
+  PetscCheck(q->bytes <= s->budget, PETSC_ERR_ARG_OUTOFRANGE,
+             "request needs %zu bytes of HBM, more than the whole budget (%zu)", q->bytes, s->budget);
+  q->state = REQ_QUEUED;
+  q->slot = -1;
+  q->res.t_arrival = q->in.t_arrival;
+  *id = s->nreq++;
+  return PETSC_SUCCESS;
+}
+
+/* builds the request's objects on its slot and starts the solve */
+static PetscErrorCode RequestAdmit(SlepcServer s, PetscInt id, int slot)
+{
+  Request *q = &s->req[id];
+  Vec v0;
+  BV V;
+  q->slot = slot;
+  q->res.t_admit = SlepcWallTime();
+  PetscCall(MatCreateHIPCSR(q->in.n, q->in.rowptr, q->in.colidx, q->in.val, &q->A));
+  PetscCall(EPSCreate(&q->eps));
+  PetscCall(EPSSetOperators(q->eps, q->A, NULL));
+  PetscCall(EPSSetProblemType(q->eps, q->in.hermitian ? EPS_HEP : EPS_NHEP));
+  PetscCall(EPSSetWhichEigenpairs(q->eps, q->in.which ? q->in.which : EPS_LARGEST_MAGNITUDE));
+  PetscCall(EPSSetDimensions(q->eps, q->in.nev, q->in.ncv, PETSC_DETERMINE));
+  PetscCall(EPSSetTolerances(q->eps, q->in.tol > 0 ? q->in.tol : 1e-10, q->in.max_it > 0 ? q->in.max_it : PETSC_DETERMINE));
+  PetscCall(EPSSetType(q->eps, EPSKRYLOVSCHUR));
+  PetscCall(EPSGetBV(q->eps, &V));
+  PetscCall(BVSetType(V, BVHIPKS));
+  if (q->in.v0) {
+    PetscCall(VecCreateHIP(q->in.n, SLEPC_AMD_F64, &v0));
+    PetscCall(VecSetValuesHost(v0, q->in.v0));
+    PetscCall(EPSSetInitialSpace(q->eps, 1, &v0));
+    PetscCall(VecDestroy(&v0));
+  }
+  PetscCall(EPSSolveBegin(q->eps));
+  q->state = REQ_ACTIVE;
+  s->slot_req[slot] = id;
+  s->nactive++;
+  return PETSC_SUCCESS;
+}
+
+/* collects eigenvalues, residuals (and vectors), then frees the device state */
+static PetscErrorCode RequestRetire(SlepcServer s, PetscInt id)
+{
+  Request *q = &s->req[id];
+  PetscInt nev = q->in.nev, nret;
+  Vec xr = NULL, xi = NULL;
+  PetscCall(EPSSolveEnd(q->eps));
+  q->res.nconv = q->eps->nconv;
+  q->res.its = q->eps->its;
+  q->res.reason = q->eps->reason;
+  q->res.nmatmult = q->eps->nmatmult;
+  q->res.gs_passes = q->eps->V->gs_passes;
+  nret = PetscMin(nev, q->eps->nconv);
+  q->res.nret = nret;
+  PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)nev, &q->eigr));
+  PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)nev, &q->eigi));
+  PetscCall(PetscMallocZero(sizeof(PetscReal) * (size_t)nev, &q->err));
+  if (q->in.want_vectors) {
+    PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)q->in.n * (size_t)nev, &q->X));
+    PetscCall(VecCreateHIP(q->in.n, SLEPC_AMD_F64, &xr));
+    PetscCall(VecCreateHIP(q->in.n, SLEPC_AMD_F64, &xi));
+  }
+  q->res.ngood = 0;
+  for (PetscInt i = 0; i < nret; i++) {
+    PetscCall(EPSGetEigenpair(q->eps, i, &q->eigr[i], &q->eigi[i], xr, xi));
+    PetscCall(EPSComputeError(q->eps, i, EPS_ERROR_RELATIVE, &q->err[i]));
+    if (q->err[i] <= q->eps->tol) q->res.ngood++;
+    if (xr) PetscCall(VecGetValuesHost(xr, q->X + (size_t)i * (size_t)q->in.n));
+  }
+  if (xr) PetscCall(VecDestroy(&xr));
+  if (xi) PetscCall(VecDestroy(&xi));
+  PetscCall(EPSDestroy(&q->eps));
+  PetscCall(MatDestroy(&q->A));
+  KSCall(ks_ctx_sync(s->slot_ctx[q->slot]));
+  q->res.t_done = SlepcWallTime();
+  q->state = REQ_DONE;
+  s->slot_req[q->slot] = -1;
+  q->slot = -1;
+  s->nactive--;
+  s->reserved -= q->bytes;
+  s->nfinished++;
+  /* the matrix belongs to the caller again */
+  q->in.rowptr = q->in.colidx = NULL;
+  q->in.val = NULL;
+  q->in.v0 = NULL;
+  return PETSC_SUCCESS;
+}
+
+/* One pass of the loop, never waiting on the device: admit queued requests in
+   arrival order while a slot and the budget allow, then advance every active
+   solve whose device work has finished by one restart, retiring the finished
+   ones.  *progressed says whether anything moved. */
+PetscErrorCode SlepcServerProgress(SlepcServer s, PetscBool *progressed, PetscInt *nactive, PetscInt *nqueued,
+                                   PetscInt *nfinished)
+{
+  ks_ctx prev_ctx;
+  ks_pool prev_pool;
+  PetscErrorCode ierr = PETSC_SUCCESS;
+  PetscBool moved = PETSC_FALSE;
+  PetscCheck(s, PETSC_ERR_ARG_NULL, "null server");
+  prev_pool = ks_pool_use(s->pool);
+  prev_ctx = SlepcAMDSetCurrentContext(NULL);
+  /* admission, FIFO: a large request at the head waits for memory rather than
+     being overtaken indefinitely */
+  while (s->next_queued < s->nreq && s->nactive < s->max_active) {
+    Request *q = &s->req[s->next_queued];
+    int slot = -1;
+    if (s->reserved + q->bytes > s->budget) break;
+    for (int i = 0; i < s->max_active; i++)
+      if (s->slot_req[i] < 0) { slot = i; break; }
+    if (slot < 0) break;
+    s->reserved += q->bytes;
+    if (s->reserved > s->peak_reserved) s->peak_reserved = s->reserved;
+    (void)SlepcAMDSetCurrentContext(s->slot_ctx[slot]);
+    ierr = RequestAdmit(s, s->next_queued, slot);
+    if (ierr) goto out;
+    s->next_queued++;
+    moved = PETSC_TRUE;
+  }
+  for (int i = 0; i < s->max_active; i++) {
+    PetscInt id = s->slot_req[i];
+    PetscBool ready, done;
+    if (id < 0) continue;
+    (void)SlepcAMDSetCurrentContext(s->slot_ctx[i]);
+    ierr = EPSSolveReady(s->req[id].eps, &ready);
+    if (ierr) goto out;
+    if (!ready) continue;
+    ierr = EPSSolveStep(s->req[id].eps, &done);
+    if (ierr) goto out;
+    moved = PETSC_TRUE;
+    if (done) {
+      ierr = RequestRetire(s, id);
+      if (ierr) goto out;
+    }
+  }
+out:
+  (void)SlepcAMDSetCurrentContext(prev_ctx);
+  (void)ks_pool_use(prev_pool);
+  if (ierr) return SlepcError(__LINE__, __func__, __FILE__, ierr, NULL);
+  if (progressed) *progressed = moved;
+  if (nactive) *nactive = s->nactive;
+  if (nqueued) *nqueued = s->nreq - s->next_queued;
+  if (nfinished) *nfinished = s->nfinished;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode SlepcServerGetResult(SlepcServer s, PetscInt id, PetscBool *done, SlepcResult *res,
+                                    PetscScalar *eigr, PetscScalar *eigi, PetscReal *err, PetscScalar *X)
+{
+  Request *q;
+  PetscCheck(s && id >= 0 && id < s->nreq, PETSC_ERR_ARG_OUTOFRANGE, "no request %d", (int)id);
+  q = &s->req[id];
+  if (done) *done = q->state == REQ_DONE ? PETSC_TRUE : PETSC_FALSE;
+  if (q->state != REQ_DONE) return PETSC_SUCCESS;
+  if (res) *res = q->res;
+  if (eigr) memcpy(eigr, q->eigr, sizeof(PetscScalar) * (size_t)q->res.nret);
+  if (eigi) memcpy(eigi, q->eigi, sizeof(PetscScalar) * (size_t)q->res.nret);
+  if (err) memcpy(err, q->err, sizeof(PetscReal) * (size_t)q->res.nret);
+  if (X) {
+    PetscCheck(q->X, PETSC_ERR_ARG_WRONG, "request %d was submitted without want_vectors", (int)id);
+    memcpy(X, q->X, sizeof(PetscScalar) * (size_t)q->in.n * (size_t)q->res.nret);
+  }
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode SlepcServerGetStats(SlepcServer s, size_t *peak_reserved, size_t *pool_peak, size_t *budget)
+{
+  PetscCheck(s, PETSC_ERR_ARG_NULL, "null server");
+  if (peak_reserved) *peak_reserved = s->peak_reserved;
+  if (budget) *budget = s->budget;
+  if (pool_peak) KSCall(ks_pool_info(s->pool, NULL, pool_peak, NULL, NULL));
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode SlepcServerDestroy(SlepcServer *ps)
+{
+  SlepcServer s;
+  if (!ps || !*ps) return PETSC_SUCCESS;
+  s = *ps;
+  {
+    ks_pool prev = ks_pool_use(s->pool);
+    for (int i = 0; i < s->max_active; i++) {
+      PetscInt id = s->slot_req[i];
+      if (id < 0) continue;
+      Request *q = &s->req[id];
+      ks_ctx prev_ctx = SlepcAMDSetCurrentContext(s->slot_ctx[i]);
+      KSCall(ks_ctx_sync(s->slot_ctx[i]));
+      PetscCall(EPSDestroy(&q->eps));
+      PetscCall(MatDestroy(&q->A));
+      (void)SlepcAMDSetCurrentContext(prev_ctx);
+      q->state = REQ_FAILED;
+    }
+    (void)ks_pool_use(prev);
+  }
+  for (PetscInt i = 0; i < s->nreq; i++) {
+    free(s->req[i].eigr);
+    free(s->req[i].eigi);
+    free(s->req[i].err);
+    free(s->req[i].X);
+  }
+  free(s->req);
+  KSCall(ks_pool_destroy(s->pool));
+  for (int i = 0; i < s->max_active; i++) KSCall(ks_ctx_destroy(s->slot_ctx[i]));
+  free(s->slot_ctx);
+  free(s->slot_req);
+  free(s);
+  *ps = NULL;
+  return PETSC_SUCCESS;
+}
+
+double SlepcServerClock(void) { return SlepcWallTime(); }

@@ -53,8 +53,9 @@ typedef struct {
 PetscErrorCode SlepcFunctionListAdd(SlepcFunctionList *fl, const char *name, void *create);
 void *SlepcFunctionListFind(SlepcFunctionList *fl, const char *name);
 
-/* the device context shared by the objects of this process */
+/* the device context objects bind to (default: the one made at initialization) */
 ks_ctx SlepcAMDContext(void);
+ks_ctx SlepcAMDSetCurrentContext(ks_ctx ctx);   /* NULL restores the default; returns the previous */
 PetscErrorCode SlepcCheckInitialized(void);
 
 /* ---------------------------------------------------------------- sorting criterion (slepcsc.c) */
@@ -224,6 +225,8 @@ struct _p_EPS {
   Vec work[5];
   int64_t nmatmult;
   PetscReal t_ds, t_solve;
+  double t_begin;
+  PetscBool stepping;        /* between EPSSolveBegin and the last EPSSolveStep */
 };
 
 typedef enum { KS_PHASE_IDLE = 0, KS_PHASE_EXPANDING = 1, KS_PHASE_DONE = 2 } KSPhase;

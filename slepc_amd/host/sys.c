@@ -108,7 +108,16 @@ PetscErrorCode SlepcCheckInitialized(void)
   return PETSC_SUCCESS;
 }
 
-ks_ctx SlepcAMDContext(void) { return g_ctx; }
+/* The context new objects bind to and Vec/Mat operations run on.  The serving
+   loop makes a request's own context current while it works on that request. */
+static ks_ctx g_current = NULL;
+ks_ctx SlepcAMDContext(void) { return g_current ? g_current : g_ctx; }
+ks_ctx SlepcAMDSetCurrentContext(ks_ctx ctx)
+{
+  ks_ctx prev = g_current;
+  g_current = ctx;
+  return prev;
+}
 void *SlepcAMDGetDefaultContext(void) { return (void *)g_ctx; }
 
 /* ---------------------------------------------------------------- sorting criteria */

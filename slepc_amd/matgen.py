@@ -13,6 +13,11 @@ given seed on any machine (numpy ``default_rng`` / PCG64).
                            cell's six face coefficients (boundary faces included,
                            which makes the matrix SPD).
 * ``convection_diffusion_2d`` -- nonsymmetric 5-point upwind convection-diffusion.
+* ``er_laplacian``      -- weighted Erdos-Renyi graph Laplacian D - W (serving stream).
+* ``rmat_shifted_normalized_laplacian`` -- 2I - (I - D^-1/2 A D^-1/2) of an R-MAT graph.
+* ``serving_stream``    -- the request parameters of a Poisson-arrival serving run.
+* ``uniform_vector``    -- U(-1,1) from the counter-based splitmix64 generator of the
+                           device (the same numbers as VecSetRandomSeeded).
 """
 from __future__ import annotations
 
@@ -164,3 +169,125 @@ def dense_to_csr(M: np.ndarray):
     np.cumsum(counts, out=rowptr[1:])
     cols = np.nonzero(mask)[1].astype(np.int32)
     return rowptr, cols, np.ascontiguousarray(M[mask], dtype=np.float64)
+
+
+def _csr_from_coo(n: int, r: np.ndarray, c: np.ndarray, v: np.ndarray):
+    """CSR with sorted, unique columns per row from COO triplets (no duplicates)."""
+    order = np.lexsort((c, r))
+    r, c, v = r[order], c[order], v[order]
+    counts = np.bincount(r, minlength=n).astype(np.int64)
+    rowptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(counts, out=rowptr[1:])
+    if rowptr[-1] > np.iinfo(np.int32).max:
+        raise ValueError("nnz exceeds 32-bit index range")
+    return rowptr.astype(np.int32), c.astype(np.int32), np.ascontiguousarray(v, dtype=np.float64)
+
+
+def er_laplacian(n: int, avg_degree: float = 8.0, seed: int = 0):
+    """Laplacian D - W of a weighted Erdos-Renyi graph G(n, p), p = avg_degree/(n-1),
+    edge weights uniform(0, 1].  The edge count is drawn Binomial(n(n-1)/2, p) and
+    that many distinct unordered pairs are drawn uniformly (which is G(n, p) in
+    distribution).  Every row stores its diagonal, also for isolated vertices."""
+    rng = np.random.default_rng(seed)
+    npairs = n * (n - 1) // 2
+    p = min(1.0, avg_degree / max(n - 1, 1))
+    m = int(rng.binomial(npairs, p))
+    keys = np.empty(0, dtype=np.int64)
+    while len(keys) < m:
+        need = m - len(keys)
+        a = rng.integers(0, n, size=int(need * 1.1) + 16, dtype=np.int64)
+        b = rng.integers(0, n, size=len(a), dtype=np.int64)
+        ok = a != b
+        lo, hi = np.minimum(a[ok], b[ok]), np.maximum(a[ok], b[ok])
+        keys = np.unique(np.concatenate([keys, lo * n + hi]))
+    keys = keys[rng.permutation(len(keys))[:m]]
+    i, j = keys // n, keys % n
+    w = 1.0 - rng.random(m)                      # uniform(0, 1]
+    deg = np.bincount(i, weights=w, minlength=n) + np.bincount(j, weights=w, minlength=n)
+    r = np.concatenate([i, j, np.arange(n, dtype=np.int64)])
+    c = np.concatenate([j, i, np.arange(n, dtype=np.int64)])
+    v = np.concatenate([-w, -w, deg])
+    return _csr_from_coo(n, r, c, v)
+
+
+def rmat_edges(scale: int, edge_factor: int = 16, abcd=(0.57, 0.19, 0.19, 0.05), seed: int = 5,
+               chunk: int = 1 << 24):
+    """Directed R-MAT edge list (Graph500 recursive quadrant choice, no noise):
+    2^scale vertices, edge_factor * 2^scale edges, as int64 (src, dst) arrays."""
+    rng = np.random.default_rng(seed)
+    a, b, c, _ = abcd
+    m = edge_factor << scale
+    src = np.empty(m, dtype=np.int64)
+    dst = np.empty(m, dtype=np.int64)
+    for s0 in range(0, m, chunk):
+        e = min(m, s0 + chunk)
+        u = np.zeros(e - s0, dtype=np.int64)
+        v = np.zeros(e - s0, dtype=np.int64)
+        for bit in range(scale):
+            x = rng.random(e - s0)
+            right = (x >= a) & (x < a + b) | (x >= a + b + c)   # quadrants b, d
+            down = x >= a + b                                  # quadrants c, d
+            u |= down.astype(np.int64) << bit
+            v |= right.astype(np.int64) << bit
+        src[s0:e], dst[s0:e] = u, v
+    return src, dst
+
+
+def rmat_shifted_normalized_laplacian(scale: int, edge_factor: int = 16, seed: int = 5,
+                                      abcd=(0.57, 0.19, 0.19, 0.05), dtype=np.float32):
+    """M = 2I - (I - D^-1/2 A D^-1/2) = I + D^-1/2 A D^-1/2 of the R-MAT graph
+    symmetrized (A = pattern of E + E^T), with self-loops and duplicate edges
+    dropped; isolated vertices keep M_ii = 1 (their Laplacian row is zero).
+    Eigenvalues lie in [0, 2]; the largest of M are the smallest of the
+    normalized Laplacian.  Values in ``dtype`` (float32 for the at-scale case)."""
+    n = 1 << scale
+    src, dst = rmat_edges(scale, edge_factor, abcd, seed)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    lo, hi = np.minimum(src, dst), np.maximum(src, dst)
+    del src, dst, keep
+    key = np.unique(lo * n + hi)
+    del lo, hi
+    i, j = key // n, key % n
+    del key
+    deg = (np.bincount(i, minlength=n) + np.bincount(j, minlength=n)).astype(np.float64)
+    dinv = np.where(deg > 0, 1.0 / np.sqrt(np.maximum(deg, 1.0)), 0.0)
+    w = dinv[i] * dinv[j]
+    r = np.concatenate([i, j, np.arange(n, dtype=np.int64)])
+    c = np.concatenate([j, i, np.arange(n, dtype=np.int64)])
+    del i, j
+    v = np.concatenate([w, w, np.ones(n)])
+    rowptr, colidx, val = _csr_from_coo(n, r, c, v)
+    return rowptr, colidx, val.astype(dtype)
+
+
+_SM_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _splitmix64(x: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        z = x + _SM_GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform_vector(seed: int, n: int) -> np.ndarray:
+    """u_i = 2 * (splitmix64(seed * golden + i) >> 11) * 2^-53 - 1: the device's
+    counter-based U(-1,1) stream (VecSetRandomSeeded / ks_bv_random)."""
+    with np.errstate(over="ignore"):
+        base = np.uint64(seed) * _SM_GOLDEN
+        x = base + np.arange(n, dtype=np.uint64)
+    return 2.0 * (_splitmix64(x) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53 - 1.0
+
+
+def serving_stream(nreq: int, qps: float, seed: int = 4, nmin: int = 1000, nmax: int = 200000):
+    """Request parameters of the serving workload: n = floor(10^U(log10 nmin, log10 nmax)),
+    nev uniform in {4..32}, ncv = 2 nev + 16, Poisson arrivals at ``qps`` (exponential
+    gaps), a graph seed and a start-vector seed per request."""
+    rng = np.random.default_rng(seed)
+    n = np.floor(10.0 ** rng.uniform(np.log10(nmin), np.log10(nmax), nreq)).astype(np.int64)
+    nev = rng.integers(4, 33, nreq)
+    t = np.cumsum(rng.exponential(1.0 / qps, nreq)) if qps > 0 else np.zeros(nreq)
+    return [dict(n=int(n[i]), nev=int(nev[i]), ncv=int(2 * nev[i] + 16), t_arrival=float(t[i]),
+                 graph_seed=1000 + i, v0_seed=2000 + i) for i in range(nreq)]

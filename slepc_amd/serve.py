@@ -1,0 +1,151 @@
+"""Python front-end of the serving loop of libslepc_amd.so (SlepcServer* in
+include/slepc_amd.h): eigenproblems arrive over time, are admitted against an
+HBM budget and solved concurrently, each by the unchanged Krylov-Schur solver
+on its own stream.  The loop itself runs in C; this module only submits
+requests at their arrival times, calls SlepcServerProgress and collects
+results."""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import eps as _eps
+from . import matgen
+
+_P = C.POINTER
+
+
+class Request(C.Structure):
+    _fields_ = [("n", C.c_int), ("rowptr", C.c_void_p), ("colidx", C.c_void_p), ("val", C.c_void_p),
+                ("nev", C.c_int), ("ncv", C.c_int), ("max_it", C.c_int), ("tol", C.c_double),
+                ("which", C.c_int), ("hermitian", C.c_int), ("v0", C.c_void_p), ("want_vectors", C.c_int),
+                ("t_arrival", C.c_double)]
+
+
+class Result(C.Structure):
+    _fields_ = [("nconv", C.c_int), ("nret", C.c_int), ("ngood", C.c_int), ("its", C.c_int),
+                ("reason", C.c_int), ("nmatmult", C.c_int64), ("gs_passes", C.c_int64),
+                ("t_arrival", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+def clock() -> float:
+    L = _eps.lib()
+    L.SlepcServerClock.restype = C.c_double
+    return L.SlepcServerClock()
+
+
+class Server:
+    """SlepcServer: ``budget`` bytes of HBM for resident requests, at most
+    ``max_active`` solves on the device at once."""
+
+    def __init__(self, budget: int, max_active: int = 8):
+        _eps.initialize()
+        self.h = C.c_void_p()
+        _eps._chk(_eps.lib().SlepcServerCreate(C.c_size_t(int(budget)), C.c_int(max_active), C.byref(self.h)),
+                  "SlepcServerCreate")
+        self._keep = {}
+        self._meta = {}
+
+    def submit(self, csr, nev: int, ncv: int, tol: float = 1e-10, which: int = 1, hermitian: bool = True,
+               v0=None, want_vectors: bool = False, t_arrival: float = 0.0, max_it: int = 0) -> int:
+        rowptr = np.ascontiguousarray(csr[0], dtype=np.int32)
+        colidx = np.ascontiguousarray(csr[1], dtype=np.int32)
+        val = np.ascontiguousarray(csr[2], dtype=np.float64)
+        n = len(rowptr) - 1
+        v0a = None if v0 is None else np.ascontiguousarray(v0, dtype=np.float64)
+        r = Request(n=n, rowptr=rowptr.ctypes.data, colidx=colidx.ctypes.data, val=val.ctypes.data, nev=nev,
+                    ncv=ncv, max_it=max_it, tol=tol, which=which, hermitian=int(hermitian),
+                    v0=None if v0a is None else v0a.ctypes.data, want_vectors=int(want_vectors),
+                    t_arrival=t_arrival)
+        rid = C.c_int()
+        _eps._chk(_eps.lib().SlepcServerSubmit(self.h, C.byref(r), C.byref(rid)), "SlepcServerSubmit")
+        self._keep[rid.value] = (rowptr, colidx, val, v0a)   # the C side reads them until the solve ends
+        self._meta[rid.value] = (n, nev, want_vectors)
+        return rid.value
+
+    def progress(self):
+        moved, na, nq, nf = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _eps._chk(_eps.lib().SlepcServerProgress(self.h, C.byref(moved), C.byref(na), C.byref(nq), C.byref(nf)),
+                  "SlepcServerProgress")
+        return bool(moved.value), na.value, nq.value, nf.value
+
+    def result(self, rid: int):
+        n, nev, want = self._meta[rid]
+        done = C.c_int()
+        res = Result()
+        eigr = np.zeros(nev)
+        eigi = np.zeros(nev)
+        err = np.zeros(nev)
+        X = np.zeros((nev, n)) if want else None
+        dp = lambda a: None if a is None else a.ctypes.data_as(_P(C.c_double))
+        _eps._chk(_eps.lib().SlepcServerGetResult(self.h, C.c_int(rid), C.byref(done), C.byref(res), dp(eigr),
+                                                  dp(eigi), dp(err), dp(X)), "SlepcServerGetResult")
+        if not done.value:
+            return None
+        self._keep.pop(rid, None)
+        k = res.nret
+        out = dict(nconv=res.nconv, nret=k, ngood=res.ngood, its=res.its, reason=res.reason,
+                   nmatmult=res.nmatmult, gs_passes=res.gs_passes, t_arrival=res.t_arrival,
+                   t_admit=res.t_admit, t_done=res.t_done, latency=res.t_done - res.t_arrival,
+                   eigr=eigr[:k], eigi=eigi[:k], err=err[:k])
+        if want:
+            out["X"] = X[:k].T.copy()
+        return out
+
+    def stats(self) -> dict:
+        a, b, c = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        _eps._chk(_eps.lib().SlepcServerGetStats(self.h, C.byref(a), C.byref(b), C.byref(c)), "SlepcServerGetStats")
+        return dict(peak_reserved=a.value, pool_peak=b.value, budget=c.value)
+
+    def close(self):
+        if self.h:
+            _eps._chk(_eps.lib().SlepcServerDestroy(C.byref(self.h)), "SlepcServerDestroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_stream(server: Server, requests, make_csr, tol: float = 1e-10, want_vectors: bool = False,
+               time_scale: float = 1.0, idle_sleep: float = 0.0):
+    """Feeds ``requests`` (dicts with n, nev, ncv, t_arrival, v0_seed, ...) to the
+    server at their arrival times (offsets in seconds from now, times
+    ``time_scale``) and runs the loop until every request has finished.
+    ``make_csr(req)`` builds a request's matrix; it is called ahead of the
+    arrival time, outside the latency.  Returns the results in request order
+    plus the wall time of the run."""
+    mats = [None] * len(requests)
+    nxt = 0
+    ids = [None] * len(requests)
+    # matrices and start vectors are prepared before the clock starts
+    for i, r in enumerate(requests):
+        mats[i] = (make_csr(r), matgen.uniform_vector(r["v0_seed"], r["n"]))
+    t0 = clock()
+    remaining = len(requests)
+    results = [None] * len(requests)
+    pending = set()
+    while remaining:
+        now = clock()
+        while nxt < len(requests) and t0 + time_scale * requests[nxt]["t_arrival"] <= now:
+            r = requests[nxt]
+            csr, v0 = mats[nxt]
+            ids[nxt] = server.submit(csr, r["nev"], r["ncv"], tol=tol, v0=v0, want_vectors=want_vectors,
+                                     t_arrival=t0 + time_scale * r["t_arrival"])
+            pending.add(nxt)
+            mats[nxt] = None
+            nxt += 1
+        moved, na, nq, nf = server.progress()
+        for i in list(pending):
+            res = server.result(ids[i])
+            if res is not None:
+                results[i] = res
+                pending.discard(i)
+                remaining -= 1
+        if not moved and idle_sleep > 0:
+            time.sleep(idle_sleep)
+    return results, clock() - t0

@@ -58,3 +58,51 @@ def test_convection_diffusion_nonsymmetric_and_normal_enough():
     # diffusion-dominated: rows are weakly diagonally dominant with positive diagonal
     off = np.abs(A - np.diag(np.diag(A))).sum(axis=1)
     assert np.all(np.diag(A) >= off - 1e-9 * np.diag(A))
+
+
+def test_er_laplacian_properties():
+    csr = M.er_laplacian(2000, 8.0, seed=3)
+    _check_csr(csr, 2000)
+    A = M.csr_to_dense(*csr)
+    assert np.allclose(A, A.T)
+    assert np.abs(A.sum(axis=1)).max() < 1e-12          # Laplacian rows sum to zero
+    off = A[~np.eye(2000, dtype=bool)]
+    w = -off[off != 0]
+    assert w.min() > 0 and w.max() <= 1.0               # weights in (0, 1]
+    ndeg = np.count_nonzero(A, axis=1) - 1
+    assert abs(ndeg.mean() - 8.0) < 0.5                  # average degree 8
+    assert np.linalg.eigvalsh(A).min() > -1e-10          # positive semidefinite
+    assert np.array_equal(M.er_laplacian(2000, 8.0, seed=3)[1], csr[1])
+
+
+def test_rmat_shifted_normalized_laplacian():
+    csr = M.rmat_shifted_normalized_laplacian(10, 16, seed=5, dtype=np.float64)
+    n = 1 << 10
+    _check_csr(csr, n)
+    A = M.csr_to_dense(*csr)
+    assert np.allclose(A, A.T)
+    assert np.allclose(np.diag(A), 1.0)
+    ev = np.linalg.eigvalsh(A)
+    assert ev.min() > -1e-12 and ev.max() < 2 + 1e-12    # 2I - L_norm has its spectrum in [0, 2]
+    # the top eigenvalue is 2 once per connected component with an edge: D^1/2 1 is its eigenvector
+    deg = np.count_nonzero(A, axis=1) - 1
+    u = np.sqrt(deg.astype(float))
+    assert np.allclose(A @ u, 2.0 * u)
+
+
+def test_uniform_vector_is_the_device_stream(oracle):
+    # the same counter-based generator as the oracle's and the device's
+    for seed in (0, 1, 123456789):
+        assert np.array_equal(M.uniform_vector(seed, 4097), oracle.random_uniform(seed, 4097))
+
+
+def test_serving_stream_parameters():
+    reqs = M.serving_stream(2000, qps=50.0, seed=4)
+    n = np.array([r["n"] for r in reqs])
+    nev = np.array([r["nev"] for r in reqs])
+    assert n.min() >= 1000 and n.max() <= 200000
+    assert abs(np.log10(n).mean() - (3 + np.log10(2e5)) / 2) < 0.05    # log-uniform
+    assert nev.min() == 4 and nev.max() == 32
+    assert all(r["ncv"] == 2 * r["nev"] + 16 for r in reqs)
+    gaps = np.diff([r["t_arrival"] for r in reqs])
+    assert abs(gaps.mean() - 1 / 50.0) < 0.002                          # Poisson at 50 QPS
