@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
-"""Benchmark: plain FP64 Krylov-Schur on one MI355X.
+"""Benchmark: Krylov-Schur on one MI355X.
 
-Workload (one "step" = one full EPSSolve): 3-D 7-point variable-coefficient
+Default workload, ``--workload fp64`` (one "step" = one full EPSSolve): 3-D 7-point variable-coefficient
 Laplacian on a 128^3 grid (n = 2,097,152, 14,581,760 nnz, CSR FP64/int32),
 lognormal(0,1) face coefficients from seed 1, diagonal = sum of the cell's face
 coefficients; nev = 16 largest magnitude, ncv = 48, tol = 1e-10, start vector
@@ -10,6 +10,18 @@ U(-1,1) from seed 0.  Inputs are resident in HBM before the timed region.
 value = converged eigenpairs/s, counting only pairs among the nev requested
 whose relative residual ||Ax - lambda x|| / |lambda|, recomputed on the device
 by EPSComputeError after the timed region, is <= 1e-10.
+
+Other forms, each printing its own JSON line:
+
+* ``--workload mixed`` -- nonsymmetric 2-D upwind convection-diffusion on a
+  2048^2 grid (Peclet 10, diffusion uniform(0.5,2) per cell, seed 2), nev 8
+  largest real part, ncv 40, tol 1e-10: the mixed-precision form (FP32 basis to
+  1e-5, then FP64 restarts) timed beside the plain FP64 solve of the same problem.
+* ``--workload serve`` -- a stream of weighted Erdos-Renyi graph Laplacians
+  (average degree 8, n log-uniform in [1e3, 2e5], nev uniform in {4..32},
+  ncv = 2 nev + 16, tol 1e-10) arriving by a Poisson process at ``--qps``,
+  served concurrently under an HBM budget, timed beside the same requests
+  solved one after another.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -64,6 +76,12 @@ def cpu_baseline(csr, n, nev, ncv, tol, gpu_its, budget_s):
 
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["fp64", "mixed", "serve"], default="fp64")
+    ap.add_argument("--requests", type=int, default=400, help="serve: number of requests")
+    ap.add_argument("--qps", type=float, default=200.0, help="serve: Poisson arrival rate")
+    ap.add_argument("--slots", type=int, default=8, help="serve: concurrent solves")
+    ap.add_argument("--budget-gb", type=float, default=64.0, help="serve: HBM admission budget")
+    ap.add_argument("--mixed-grid", type=int, default=2048)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -81,6 +99,11 @@ def main():
               f"(--gpus {args.gpus} requested).  Multi-GPU sharding is a design note, not a run mode.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "mixed":
+        return run_mixed(args)
+    if args.workload == "serve":
+        return run_serve(args)
 
     from slepc_amd import eps as E
     from slepc_amd import ks as K
@@ -216,6 +239,143 @@ def main():
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
     }
+    print(json.dumps(out), flush=True)
+
+
+def _base(args, value, unit, ms_per_step, steps, dtype, workload, extra):
+    out = {"metric": METRIC, "value": round(value, 4), "unit": unit, "n_gpus": 1, "steps": steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+           "config": {"workload": workload}}
+    out.update(extra)
+    return out
+
+
+def run_mixed(args):
+    """Mixed-precision Krylov-Schur vs plain FP64 on the convection-diffusion operator."""
+    from slepc_amd import eps as E
+    from slepc_amd import matgen as M
+    N = args.mixed_grid
+    t0 = time.time()
+    csr = M.convection_diffusion_2d(N, 10.0, seed=2)
+    n = N * N
+    log(f"[mixed] matrix {N}^2 n={n} nnz={int(csr[0][-1])} built in {time.time() - t0:.1f}s")
+    E.initialize(0)
+    A = E.Mat(csr)
+    v0 = E.Vec.random(n, 0)
+    nev, ncv, tol = 8, 40, 1e-10
+
+    def solve(mixed):
+        eps = E.EPS()
+        eps.setOperators(A)
+        eps.setProblemType(E.EPS.NHEP)
+        eps.setDimensions(nev, ncv)
+        eps.setTolerances(tol)
+        eps.setWhichEigenpairs(E.EPS.LARGEST_REAL)
+        if mixed:
+            eps.setMixedPrecision(1e-5)
+        eps.setInitialSpace([v0])
+        ts = time.perf_counter()
+        eps.solve()
+        dt = time.perf_counter() - ts
+        k = min(eps.getConverged(), nev)
+        errs = [eps.computeError(i) for i in range(k)]
+        vals = [eps.getEigenvalue(i) for i in range(k)]
+        st = eps.stats()
+        ms = eps.mixedStats() if mixed else {}
+        eps.destroy()
+        return dict(t=dt, verified=sum(e <= tol for e in errs), errs=errs, vals=vals, its=st["its"],
+                    nmatmult=st["nmatmult"], mixed=ms)
+
+    res = {}
+    for form in ("fp64", "mixed"):
+        for w in range(args.warmup):
+            r = solve(form == "mixed")
+            log(f"[mixed] warmup {form}: {r['t']:.3f}s its={r['its']}")
+        runs = [solve(form == "mixed") for _ in range(args.steps)]
+        for r in runs:
+            log(f"[mixed] {form}: {r['t']:.3f}s its={r['its']} verified={r['verified']} {r['mixed']}")
+        res[form] = runs
+    # per-pair error of the mixed form against the FP64 form (same problem, same start)
+    ref = res["fp64"][-1]["vals"]
+    got = res["mixed"][-1]["vals"]
+    pair_err = [abs(complex(g) - complex(r)) / abs(complex(r)) for g, r in zip(got, ref)]
+    rate = {f: sum(r["verified"] for r in res[f]) / sum(r["t"] for r in res[f]) for f in res}
+    out = _base(args, rate["mixed"], "eigenpairs/s", 1e3 * np.mean([r["t"] for r in res["mixed"]]), args.steps,
+                "f32+f64",
+                f"2-D upwind convection-diffusion {N}^2 (Peclet 10, diffusion U(0.5,2) seed 2), mixed-precision "
+                f"Krylov-Schur (FP32 basis to 1e-5, then FP64 to 1e-10), nev {nev} LR, ncv {ncv}",
+                {"forms": {f: {"eigenpairs_per_s": round(rate[f], 4),
+                               "solve_s": [round(r["t"], 4) for r in res[f]],
+                               "restarts": [r["its"] for r in res[f]],
+                               "spmv": [r["nmatmult"] for r in res[f]],
+                               "verified": [r["verified"] for r in res[f]]} for f in res},
+                 "mixed_phase": res["mixed"][-1]["mixed"],
+                 "speedup_vs_fp64": round(rate["mixed"] / rate["fp64"], 3) if rate["fp64"] else None,
+                 "pair_rel_error_vs_fp64": [float(f"{e:.3e}") for e in pair_err],
+                 "max_rel_residual": float(f"{max(res['mixed'][-1]['errs']):.3e}")})
+    out["config"].update({"n": n, "nnz": int(csr[0][-1]), "nev": nev, "ncv": ncv, "tol": tol, "which": "LR"})
+    print(json.dumps(out), flush=True)
+
+
+def _er_job(r):
+    from slepc_amd import matgen as M
+    return M.er_laplacian(r["n"], 8.0, r["graph_seed"])
+
+
+def run_serve(args):
+    """Poisson stream of ER-Laplacian eigenproblems: served concurrently vs one after another."""
+    from concurrent.futures import ProcessPoolExecutor
+    from slepc_amd import eps as E
+    from slepc_amd import matgen as M
+    from slepc_amd import serve as S
+    reqs = M.serving_stream(args.requests, args.qps, seed=4)
+    t0 = time.time()
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with ProcessPoolExecutor(workers) as ex:
+        mats = list(ex.map(_er_job, reqs, chunksize=8))
+    log(f"[serve] {len(reqs)} matrices (total nnz {sum(int(m[0][-1]) for m in mats)}) built in "
+        f"{time.time() - t0:.1f}s with {workers} workers")
+    E.initialize(0)
+    cache = {i: m for i, m in enumerate(mats)}
+    for i, r in enumerate(reqs):
+        r["_i"] = i
+    # served: concurrent, admitted against the budget
+    srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=args.slots)
+    results, wall = S.run_stream(srv, reqs, lambda r: cache[r["_i"]], tol=1e-10)
+    stats = srv.stats()
+    srv.close()
+    good = sum(r["ngood"] for r in results)
+    lat = np.array([r["latency"] for r in results]) * 1e3
+    log(f"[serve] served {len(reqs)} requests in {wall:.2f}s: {good} verified pairs, "
+        f"p50 {np.percentile(lat, 50):.1f} ms p99 {np.percentile(lat, 99):.1f} ms")
+    # the same requests, one EPSSolve after another on one stream (no arrivals: back to back)
+    ts = time.perf_counter()
+    seq_good = 0
+    seq_lat = []
+    for i, r in enumerate(reqs):
+        t1 = time.perf_counter()
+        out = E.solve(cache[i], nev=r["nev"], ncv=r["ncv"], tol=1e-10, v0=M.uniform_vector(r["v0_seed"], r["n"]),
+                      want_vectors=False)
+        seq_lat.append(time.perf_counter() - t1)
+        seq_good += int(np.sum(np.asarray(out["err"][:r["nev"]]) <= 1e-10))
+    seq_wall = time.perf_counter() - ts
+    log(f"[serve] sequential: {seq_wall:.2f}s, {seq_good} verified pairs")
+    rate = good / wall
+    seq_rate = seq_good / seq_wall
+    out = _base(args, rate, "eigenpairs/s", 1e3 * wall / len(reqs), len(reqs), "f64",
+                f"serving: {len(reqs)} weighted Erdos-Renyi Laplacians (degree 8, n log-uniform [1e3,2e5], "
+                f"nev U{{4..32}}, ncv 2nev+16, LM, tol 1e-10), Poisson arrivals at {args.qps:g} QPS, "
+                f"{args.slots} slots, {args.budget_gb:g} GiB HBM budget",
+                {"latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                                "p99": round(float(np.percentile(lat, 99)), 2)},
+                 "served": {"eigenpairs_per_s": round(rate, 4), "wall_s": round(wall, 3), "verified_pairs": good,
+                            "requests": len(reqs), "peak_reserved_bytes": stats["peak_reserved"]},
+                 "sequential": {"eigenpairs_per_s": round(seq_rate, 4), "wall_s": round(seq_wall, 3),
+                                "verified_pairs": seq_good,
+                                "latency_ms_p50": round(float(np.percentile(np.array(seq_lat) * 1e3, 50)), 2)},
+                 "speedup_vs_sequential": round(rate / seq_rate, 3) if seq_rate else None})
+    out["config"].update({"requests": len(reqs), "qps": args.qps, "slots": args.slots})
     print(json.dumps(out), flush=True)
 
 

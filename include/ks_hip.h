@@ -183,7 +183,7 @@ int ks_bv_multinplace(ks_ctx ctx, ks_prec prec, int n, void *V, int ld, int k,
  * When enabled, every kernel launch of the context is bracketed by HIP events
  * on the context stream; collect() accumulates per-class device time, the
  * algorithmic HBM bytes and flops of the launches that did work. */
-enum { KS_PROF_SPMV = 0, KS_PROF_GS_DOT1, KS_PROF_GS_UPD1, KS_PROF_GS_DOTR, KS_PROF_GS_UPDR,
+enum { KS_PROF_SPMV = 0, KS_PROF_GS_DOT1, KS_PROF_GS_UPD1, KS_PROF_GS_FIN, KS_PROF_GS_UPDR,
        KS_PROF_SCALE, KS_PROF_NORM, KS_PROF_AXPY, KS_PROF_COPY, KS_PROF_MULTINPLACE,
        KS_PROF_DOTVEC, KS_PROF_MULTVEC, KS_PROF_OTHER, KS_PROF_NCLASS };
 typedef struct {

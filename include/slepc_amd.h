@@ -52,6 +52,9 @@ typedef int    PetscBool;
 #define PETSC_ERR_CONV_FAILED       91
 #define PETSC_ERR_NOT_CONVERGED     PETSC_ERR_CONV_FAILED
 #define PETSC_ERR_GPU               97
+#define PETSC_ERR_FILE_OPEN         65
+#define PETSC_ERR_FILE_READ         66
+#define PETSC_ERR_FILE_UNEXPECTED   79
 
 typedef struct _p_Vec *Vec;
 typedef struct _p_Mat *Mat;
@@ -127,6 +130,12 @@ PetscErrorCode MatDenseGetArray(Mat A, PetscScalar **array);
 PetscErrorCode MatDenseRestoreArray(Mat A, PetscScalar **array);
 PetscErrorCode MatDenseGetLDA(Mat A, PetscInt *lda);
 PetscErrorCode MatCreateHIPCSR(PetscInt n, const PetscInt i[], const PetscInt j[], const PetscScalar v[], Mat *A);
+/* PETSc binary matrix files (MatLoad format, as read by src/eps/tutorials/ex4.c:43-50):
+   host CSR with sorted rows (free with MatFreePETScBinary), or straight into a MATHIPCSR */
+PetscErrorCode MatReadPETScBinary(const char *path, PetscInt *m, PetscInt *n, PetscInt **rowptr, PetscInt **colidx,
+                                  PetscScalar **val);
+void           MatFreePETScBinary(PetscInt *rowptr, PetscInt *colidx, PetscScalar *val);
+PetscErrorCode MatLoadPETScBinary(const char *path, Mat *A);
 
 /* ------------------------------------------------------------ BV */
 typedef enum { BV_ORTHOG_CGS = 0, BV_ORTHOG_MGS = 1 } BVOrthogType;

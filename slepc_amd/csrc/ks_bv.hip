@@ -203,6 +203,98 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_update(const T *__restrict__ V,
   }
 }
 
+// which launches of a column's pass sequence run: the first pass unless a
+// previous column broke down; a fused pass only while DGKS asks for it
+template <bool FUSED>
+__device__ __forceinline__ bool gs_pass_gate(const ks_status *st, int col, int pass) {
+  if (FUSED) return st->active && st->col == col && !st->done && st->pass_done == pass - 1;
+  return st->active != 0;
+}
+
+// Runs on the one workgroup that finished the k+1 sums in fin[]: the first pass
+// stores c and |v| (BVDotColumnInc + BV_SquareRoot); a fused pass adds its c
+// to h (BV_AddCoefficients) and takes the DGKS decision of bvorthog.c:91-217.
+template <bool FUSED>
+__device__ void gs_finish(const double *fin, int k, double *cbuf, double *hcol, ks_status *st, int col,
+                          int refine, double eta, int do_scale, int safe, double deftol) {
+  __shared__ int s_continue;
+  const int tid = threadIdx.x;
+  if (!FUSED) {
+    // first pass: BV_CleanCoefficients, c, |v| (BVDotColumnInc + BV_SquareRoot)
+    for (int c = tid; c <= k; c += blockDim.x) hcol[c] = 0.0;
+    __syncthreads();
+    for (int c = tid; c < k; c += blockDim.x) cbuf[c] = fin[c];
+    if (tid == 0) {
+      const double self = fin[k];
+      double sumsq = 0.0;
+      for (int c = 0; c < k; c++) sumsq += fin[c] * fin[c];
+      st->sumsq = sumsq;
+      if (self < -deftol) {
+        st->error = KS_ERR_INNER;
+        st->active = 0;
+      }
+      st->onrm = self < 0.0 ? 0.0 : sqrt(self);
+      st->col = col;
+      st->pass_done = 0;
+      st->done = 0;
+      st->lindep = 0;
+    }
+    return;
+  }
+  // fused pass: h += c of this pass, DGKS decision, next pass coefficients
+  for (int c = tid; c < k; c += blockDim.x) hcol[c] += cbuf[c];   // BV_AddCoefficients
+  __syncthreads();
+  if (tid == 0) {
+    const double exact = sqrt(fin[k]);
+    st->exact_sq = fin[k];
+    st->total_passes += 1;
+    const int l = ++st->pass_done;
+    const double onrm = st->onrm;
+    double nrm = 0.0;
+    int lindep = 0, finalize = 1, have_norm = 1;
+    if (safe) {
+      lindep = 1;
+      have_norm = 0;
+    } else if (refine == KS_REFINE_NEVER) {
+      nrm = exact;
+      lindep = !(nrm != 0.0);
+    } else {
+      if (refine == KS_REFINE_ALWAYS && l < 2) {
+        finalize = 0;
+      } else {
+        nrm = onrm * onrm - st->sumsq;             // estimate |v'| from |v| (bvorthog.c:124-127)
+        nrm = (nrm <= 0.0) ? exact : sqrt(nrm);
+        if (refine == KS_REFINE_IFNEEDED && l < 3 && nrm != 0.0 && fabs(nrm) < eta * fabs(onrm)) finalize = 0;
+        else lindep = !(nrm != 0.0 && fabs(nrm) >= eta * fabs(onrm));
+      }
+      st->nrm = nrm;
+    }
+    if (finalize) {
+      st->done = 1;
+      st->lindep = lindep;
+      st->nrm = nrm;
+      st->beta_last = nrm;
+      if (have_norm) hcol[k] = lindep ? 0.0 : nrm;   // BV_SetValue
+      st->scale = (do_scale && nrm != 1.0 && nrm != 0.0) ? 1.0 / nrm : 1.0;
+      if (lindep) {
+        st->active = 0;
+        st->breakdown_col = col;
+      }
+      s_continue = 0;
+    } else {
+      // next pass: its |v| and coefficients are the dots of the updated vector
+      double sumsq = 0.0;
+      for (int c = 0; c < k; c++) sumsq += fin[c] * fin[c];
+      st->sumsq = sumsq;
+      st->onrm = exact;
+      s_continue = 1;
+    }
+  }
+  __syncthreads();
+  if (s_continue)
+    for (int c = tid; c < k; c += blockDim.x) cbuf[c] = fin[c];
+}
+
 // ---------------------------------------------------------------- LDS-staged Gram-Schmidt
 // One CGS pass for column w against V(:,0..k), streamed in sub-chunks of R rows:
 //   phase A  the workgroup loads R rows x k columns with coalesced column
@@ -218,20 +310,12 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_update(const T *__restrict__ V,
 // the updated vector, so a second DGKS pass costs one V read instead of two.
 template <typename T, int R, bool FUSED>
 __global__ __launch_bounds__(KS_BLOCK) void k_gs_lds(const T *__restrict__ V, long ld, int k, T *__restrict__ w,
-                                                    int n, int nblocks, int cpb, int tsub,
-                                                    double *__restrict__ cbuf, double *partial, double *gpartial,
-                                                    unsigned *counters, ks_status *st, double *hcol, int col,
-                                                    int pass, int refine, double eta, int do_scale, int safe,
-                                                    int *ran, double deftol) {
-  if (FUSED) {
-    if (!(st->active && st->col == col && !st->done && st->pass_done == pass - 1)) return;
-  } else {
-    if (!st->active) return;
-  }
+                                                    int n, int cpb, int tsub, const double *__restrict__ cbuf,
+                                                    double *__restrict__ partial, const ks_status *st, int col,
+                                                    int pass, int *ran) {
+  if (!gs_pass_gate<FUSED>(st, col, pass)) return;
   if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = 1;
   extern __shared__ double smem[];
-  __shared__ double fin[KS_NCOLP];
-  __shared__ int s_continue;
   const int tid = threadIdx.x;
   const int S = R + tsub;
   const int G = KS_BLOCK / R;
@@ -312,81 +396,90 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_lds(const T *__restrict__ V, lo
     const int c = j * cpt + tid / tsub;
     if (sub == 0 && c <= k) partial[(size_t)blockIdx.x * ntot + c] = v;
   }
-  if (!ks_reduce_finish(partial, gpartial, counters, nblocks, ntot, fin)) return;
-  if (!FUSED) {
-    // first pass: BV_CleanCoefficients, c, |v| (BVDotColumnInc + BV_SquareRoot)
-    for (int c = tid; c <= k; c += KS_BLOCK) hcol[c] = 0.0;
-    __syncthreads();
-    for (int c = tid; c < k; c += KS_BLOCK) cbuf[c] = fin[c];
-    if (tid == 0) {
-      const double self = fin[k];
-      double sumsq = 0.0;
-      for (int c = 0; c < k; c++) sumsq += fin[c] * fin[c];
-      st->sumsq = sumsq;
-      if (self < -deftol) {
-        st->error = KS_ERR_INNER;
-        st->active = 0;
-      }
-      st->onrm = self < 0.0 ? 0.0 : sqrt(self);
-      st->col = col;
-      st->pass_done = 0;
-      st->done = 0;
-      st->lindep = 0;
-    }
-    return;
-  }
-  // fused pass: h += c of this pass, DGKS decision, next pass coefficients
-  for (int c = tid; c < k; c += KS_BLOCK) hcol[c] += cbuf[c];   // BV_AddCoefficients
+}
+
+// ---------------------------------------------------------------- register-resident Gram-Schmidt
+// The same pass for k <= KC columns with no LDS staging: a thread owns rows
+// tid, tid + 256*nb, ... and for each row issues all k column loads at once
+// (k requests in flight per thread), forms the update w_i - V(i,:) c of the
+// fused pass from those registers, then adds V(i,c) w_i into k+1 register
+// accumulators.  The workgroup reduces its accumulators once, at the end
+// (wave butterflies, then the four waves through LDS), into its partial row.
+template <typename T, int KC, bool FUSED>
+__global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, long ld, int k, T *__restrict__ w,
+                                                    int n, int nblocks, const double *__restrict__ cbuf,
+                                                    double *__restrict__ partial, const ks_status *st, int col,
+                                                    int pass, int *ran) {
+  if (!gs_pass_gate<FUSED>(st, col, pass)) return;
+  if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = 1;
+  __shared__ double coef[KC];
+  __shared__ double red[KS_WAVES][KC + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (FUSED)
+    for (int c = tid; c < KC; c += KS_BLOCK) coef[c] = c < k ? cbuf[c] : 0.0;
+  double acc[KC + 1];
+#pragma unroll
+  for (int c = 0; c <= KC; c++) acc[c] = 0.0;
   __syncthreads();
-  if (tid == 0) {
-    const double exact = sqrt(fin[k]);
-    st->exact_sq = fin[k];
-    st->total_passes += 1;
-    const int l = ++st->pass_done;
-    const double onrm = st->onrm;
-    double nrm = 0.0;
-    int lindep = 0, finalize = 1, have_norm = 1;
-    if (safe) {
-      lindep = 1;
-      have_norm = 0;
-    } else if (refine == KS_REFINE_NEVER) {
-      nrm = exact;
-      lindep = !(nrm != 0.0);
-    } else {
-      if (refine == KS_REFINE_ALWAYS && l < 2) {
-        finalize = 0;
-      } else {
-        nrm = onrm * onrm - st->sumsq;             // estimate |v'| from |v| (bvorthog.c:124-127)
-        nrm = (nrm <= 0.0) ? exact : sqrt(nrm);
-        if (refine == KS_REFINE_IFNEEDED && l < 3 && nrm != 0.0 && fabs(nrm) < eta * fabs(onrm)) finalize = 0;
-        else lindep = !(nrm != 0.0 && fabs(nrm) >= eta * fabs(onrm));
-      }
-      st->nrm = nrm;
+  // column c is read as (uniform column base) + (32-bit row byte offset), so each
+  // load is one scalar base pair plus one vector register (n * sizeof(T) < 4 GiB)
+  const unsigned stride = (unsigned)nblocks * KS_BLOCK;
+  for (unsigned i = blockIdx.x * KS_BLOCK + tid; i < (unsigned)n; i += stride) {
+    const unsigned off = i * (unsigned)sizeof(T);
+    double v[KC];
+#pragma unroll
+    for (int c = 0; c < KC; c++) {
+      const char *colbase = (const char *)(V + (size_t)c * ld);
+      v[c] = c < k ? (double)*(const T *)(colbase + off) : 0.0;
     }
-    if (finalize) {
-      st->done = 1;
-      st->lindep = lindep;
-      st->nrm = nrm;
-      st->beta_last = nrm;
-      if (have_norm) hcol[k] = lindep ? 0.0 : nrm;   // BV_SetValue
-      st->scale = (do_scale && nrm != 1.0 && nrm != 0.0) ? 1.0 / nrm : 1.0;
-      if (lindep) {
-        st->active = 0;
-        st->breakdown_col = col;
+    double wi = (double)w[i];
+    if (FUSED) {
+      double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+      for (int c = 0; c < KC; c += 2) {
+        s0 += v[c] * coef[c];
+        s1 += v[c + 1] * coef[c + 1];
       }
-      s_continue = 0;
-    } else {
-      // next pass: its |v| and coefficients are the dots of the updated vector
-      double sumsq = 0.0;
-      for (int c = 0; c < k; c++) sumsq += fin[c] * fin[c];
-      st->sumsq = sumsq;
-      st->onrm = exact;
-      s_continue = 1;
+      const T t = (T)(wi - (s0 + s1));
+      w[i] = t;
+      wi = (double)t;
     }
+#pragma unroll
+    for (int c = 0; c < KC; c++) acc[c] += v[c] * wi;
+    acc[KC] += wi * wi;
+  }
+  // workgroup reduction of k+1 sums; column k's slot carries |w|^2
+#pragma unroll
+  for (int c = 0; c <= KC; c++) {
+    double x = ks_wave_sum(acc[c]);
+    if (lane == 0) red[wave][c] = x;
   }
   __syncthreads();
-  if (s_continue)
-    for (int c = tid; c < k; c += KS_BLOCK) cbuf[c] = fin[c];
+  const int ntot = k + 1;
+  for (int c = tid; c < ntot; c += KS_BLOCK) {
+    const int src = c == k ? KC : c;
+    double x = 0.0;
+#pragma unroll
+    for (int q = 0; q < KS_WAVES; q++) x += red[q][src];
+    partial[(size_t)blockIdx.x * ntot + c] = x;
+  }
+}
+
+// The finish of a Gram-Schmidt pass, one workgroup of KS_FIN_THREADS: sums the
+// nb partial rows (fixed order: thread (c, p) adds blocks p, p+P, ... of
+// column c, then the P part sums are added in order) and takes the DGKS step.
+// A separate launch, so the sweep kernels need no fences or atomics: the
+// stream orders this workgroup after every partial row is written.
+template <bool FUSED>
+__global__ __launch_bounds__(KS_FIN_THREADS) void k_gs_fin(const double *__restrict__ partial, int nb, int k,
+                                                          double *cbuf, double *hcol, ks_status *st, int col,
+                                                          int pass, int refine, double eta, int do_scale, int safe,
+                                                          double deftol) {
+  if (!gs_pass_gate<FUSED>(st, col, pass)) return;
+  __shared__ double fin[KS_NCOLP];
+  __shared__ double part[KS_FIN_THREADS];
+  ks_sum_partials(partial, nb, k + 1, fin, part);
+  gs_finish<FUSED>(fin, k, cbuf, hcol, st, col, refine, eta, do_scale, safe, deftol);
 }
 
 template <typename T>
@@ -590,9 +683,21 @@ int split_checked(int n, int *rpt, int *nb) {
 struct gs_geom {
   int R, tsub, nb, cpb;
   size_t lds;
+  int kc;   // > 0: register-resident kernel for up to kc columns
 };
 
-int gs_geometry(int n, int k, gs_geom *g) {
+int gs_geometry(int n, int k, size_t esize, gs_geom *g) {
+  g->kc = 0;
+  if (k <= KS_GS_REGMAX && (uint64_t)n * esize < (1ull << 32)) {
+    g->kc = k <= 8 ? 8 : (k + 7) / 8 * 8;
+    const long blocks = (n + (long)KS_BLOCK - 1) / KS_BLOCK;
+    g->nb = (int)(blocks < KS_GS_REG_BLOCKS ? blocks : KS_GS_REG_BLOCKS);
+    g->cpb = 0;
+    g->R = 0;
+    g->tsub = 0;
+    g->lds = 0;
+    return KS_OK;
+  }
   int t = 8;
   while (t > 1 && (k + 1) * t > KS_BLOCK) t >>= 1;
   g->tsub = t;
@@ -615,18 +720,36 @@ int gs_geometry(int n, int k, gs_geom *g) {
 }
 
 template <typename T, bool FUSED>
-void launch_gs(ks_ctx ctx, const gs_geom &g, const T *V, long ld, int k, T *w, int n, double *cbuf, double *hcol,
-               int col, int pass, int refine, double eta, int do_scale, int safe, int *ran) {
-  const double deftol = 10.0 * 2.220446049250313e-16;
+void launch_gs(ks_ctx ctx, const gs_geom &g, const T *V, long ld, int k, T *w, int n, const double *cbuf, int col,
+               int pass, int *ran) {
   dim3 grid(g.nb), block(KS_BLOCK);
+  if (g.kc) {
+#define KS_GSR_CASE(KK)                                                                                           \
+  case KK:                                                                                                        \
+    hipLaunchKernelGGL((k_gs_reg<T, KK, FUSED>), grid, block, 0, ctx->stream, V, ld, k, w, n, g.nb, cbuf,          \
+                       ctx->d_partial, ctx->d_status, col, pass, ran);                                             \
+    break;
+    switch (g.kc) { KS_GSR_CASE(8) KS_GSR_CASE(16) KS_GSR_CASE(24) KS_GSR_CASE(32) KS_GSR_CASE(40) KS_GSR_CASE(48) }
+#undef KS_GSR_CASE
+    return;
+  }
 #define KS_GS_CASE(RR)                                                                                            \
   case RR:                                                                                                        \
-    hipLaunchKernelGGL((k_gs_lds<T, RR, FUSED>), grid, block, g.lds, ctx->stream, V, ld, k, w, n, g.nb, g.cpb,    \
-                       g.tsub, cbuf, ctx->d_partial, ctx->d_gpartial, ctx->d_counters, ctx->d_status, hcol, col,  \
-                       pass, refine, eta, do_scale, safe, ran, deftol);                                           \
+    hipLaunchKernelGGL((k_gs_lds<T, RR, FUSED>), grid, block, g.lds, ctx->stream, V, ld, k, w, n, g.cpb, g.tsub,  \
+                       cbuf, ctx->d_partial, ctx->d_status, col, pass, ran);                                       \
     break;
   switch (g.R) { KS_GS_CASE(256) KS_GS_CASE(128) KS_GS_CASE(64) KS_GS_CASE(32) KS_GS_CASE(16) }
 #undef KS_GS_CASE
+}
+
+template <bool FUSED>
+void launch_gs_fin(ks_ctx ctx, const gs_geom &g, int k, double *cbuf, double *hcol, int col, int pass, int refine,
+                   double eta, int do_scale, int safe) {
+  const double deftol = 10.0 * 2.220446049250313e-16;
+  ks_prof_begin(ctx, KS_PROF_GS_FIN, (double)g.nb * (k + 1) * 8.0, (double)g.nb * (k + 1), false);
+  hipLaunchKernelGGL((k_gs_fin<FUSED>), dim3(1), dim3(KS_FIN_THREADS), 0, ctx->stream, ctx->d_partial, g.nb, k, cbuf,
+                     hcol, ctx->d_status, col, pass, refine, eta, do_scale, safe, deftol);
+  ks_prof_end(ctx);
 }
 
 template <typename T>
@@ -634,7 +757,7 @@ int orthonormalize_t(ks_ctx ctx, int n, T *V, long ld, int k, double *buf, int l
                      ks_refine refine, double eta, int do_scale, int safe) {
   gs_geom g;
   int e;
-  if ((e = gs_geometry(n, k, &g))) return e;
+  if ((e = gs_geometry(n, k, sizeof(T), &g))) return e;
   const double s = sizeof(T);
   T *w = V + (size_t)k * ld;
   double *cbuf = buf;                              // buffer column 0: scratch coefficients
@@ -642,14 +765,16 @@ int orthonormalize_t(ks_ctx ctx, int n, T *V, long ld, int k, double *buf, int l
   const int passes = safe || refine == KS_REFINE_NEVER ? 1 : (refine == KS_REFINE_ALWAYS ? 2 : 3);
   // first pass dots: c = V^T w and w^T w
   int *ran = ks_prof_begin(ctx, KS_PROF_GS_DOT1, (k + 1.0) * n * s, 2.0 * (k + 1.0) * n, true);
-  launch_gs<T, false>(ctx, g, V, ld, k, w, n, cbuf, hcol, col, 0, refine, eta, do_scale, safe, ran);
+  launch_gs<T, false>(ctx, g, V, ld, k, w, n, cbuf, col, 0, ran);
   ks_prof_end(ctx);
+  launch_gs_fin<false>(ctx, g, k, cbuf, hcol, col, 0, refine, eta, do_scale, safe);
   // each pass: update with its c and the next pass's dots in the same sweep
   for (int p = 1; p <= passes; p++) {
     ran = ks_prof_begin(ctx, p == 1 ? KS_PROF_GS_UPD1 : KS_PROF_GS_UPDR, (k + 2.0) * n * s,
                         2.0 * k * n + 2.0 * (k + 1.0) * n, true);
-    launch_gs<T, true>(ctx, g, V, ld, k, w, n, cbuf, hcol, col, p, refine, eta, do_scale, safe, ran);
+    launch_gs<T, true>(ctx, g, V, ld, k, w, n, cbuf, col, p, ran);
     ks_prof_end(ctx);
+    launch_gs_fin<true>(ctx, g, k, cbuf, hcol, col, p, refine, eta, do_scale, safe);
   }
   if (do_scale) {
     int *ran = ks_prof_begin(ctx, KS_PROF_SCALE, 2.0 * n * s, (double)n, true);

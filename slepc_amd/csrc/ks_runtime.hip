@@ -323,7 +323,7 @@ int ks_status_get(ks_ctx c, ks_status *out) {
 
 // ---------------------------------------------------------------- profiling
 static const char *g_prof_names[KS_PROF_NCLASS] = {
-    "spmv", "gs_dot_first", "gs_update_first", "gs_dot_refine", "gs_update_refine", "scale",
+    "spmv", "gs_dot_first", "gs_update_first", "gs_finish", "gs_update_refine", "scale",
     "norm", "axpy", "copy", "multinplace", "dotvec", "multvec", "other"};
 
 const char *ks_prof_name(int cls) {

@@ -102,6 +102,26 @@ class Vec:
             pass
 
 
+def read_petsc_binary(path: str):
+    """(rowptr, colidx, val) of a matrix stored with PETSc's binary viewer, read by
+    MatReadPETScBinary (host only: no device needed)."""
+    L = lib()
+    m, n = C.c_int(), C.c_int()
+    rp, ci, va = C.POINTER(C.c_int)(), C.POINTER(C.c_int)(), C.POINTER(C.c_double)()
+    _chk(L.MatReadPETScBinary(os.fsencode(path), C.byref(m), C.byref(n), C.byref(rp), C.byref(ci), C.byref(va)),
+         "MatReadPETScBinary")
+    try:
+        rowptr = np.ctypeslib.as_array(rp, shape=(m.value + 1,)).astype(np.int32)
+        nnz = int(rowptr[-1])
+        colidx = np.ctypeslib.as_array(ci, shape=(max(nnz, 1),))[:nnz].astype(np.int32)
+        val = np.ctypeslib.as_array(va, shape=(max(nnz, 1),))[:nnz].copy()
+    finally:
+        L.MatFreePETScBinary(rp, ci, va)
+    if m.value != n.value:
+        raise SlepcError(f"{path}: {m.value} x {n.value} matrix is not square")
+    return rowptr, colidx, val
+
+
 class Mat:
     """MATHIPCSR matrix from host CSR arrays (32-bit indices)."""
 
@@ -125,6 +145,11 @@ class Mat:
         else:
             v = np.ascontiguousarray(val, dtype=np.float32)
             _chk(L.MatHIPCSRSetArraysF32(self.h, ip, jp, v.ctypes.data_as(C.c_void_p)), "MatHIPCSRSetArraysF32")
+
+    @classmethod
+    def load(cls, path: str):
+        """MatLoad of a PETSc binary matrix file into a device CSR matrix."""
+        return cls(read_petsc_binary(path))
 
     def mult(self, x: Vec, y: Vec):
         _chk(lib().MatMult(self.h, x.h, y.h), "MatMult")

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libks_hip.so")
 F64, F32 = 0, 1
 REFINE_IFNEEDED, REFINE_NEVER, REFINE_ALWAYS = 0, 1, 2
 KS_MAX_COLS = 320
-PROF_CLASSES = ["spmv", "gs_dot_first", "gs_update_first", "gs_dot_refine", "gs_update_refine", "scale",
+PROF_CLASSES = ["spmv", "gs_dot_first", "gs_update_first", "gs_finish", "gs_update_refine", "scale",
                 "norm", "axpy", "copy", "multinplace", "dotvec", "multvec", "other"]
 
 _lib = None

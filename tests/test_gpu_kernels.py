@@ -183,6 +183,51 @@ def test_orthonormalize_vs_oracle(ks, ctx, oracle):
     assert np.abs(r["Vg"][:, :j].T @ r["Vg"][:, j]).max() < 1e-14
 
 
+@pytest.mark.parametrize("j", [0, 1, 9, 24, 47, 48, 49, 96, 200])
+def test_orthonormalize_widths(ks, ctx, oracle, j):
+    # both Gram-Schmidt kernels: register-resident rows (<= 48 columns) and LDS tiles (more)
+    rng = np.random.default_rng(100 + j)
+    n = 30011
+    V = np.zeros((n, j + 1), order="F")
+    if j:
+        V[:, :j] = np.linalg.qr(rng.standard_normal((n, j)))[0]
+    V[:, j] = rng.standard_normal(n) + (V[:, :j] @ rng.standard_normal(j) if j else 0.0)
+    r = _orth_case(ks, ctx, oracle, V, j)
+    st = r["st"]
+    assert st.total_passes == r["passes"] and st.lindep == int(r["lindep"]) and st.done == 1
+    assert abs(st.nrm - r["nrm"]) <= 1e-12 * r["nrm"]
+    assert np.allclose(r["h"][:j + 1], r["H"], rtol=1e-10, atol=1e-12)
+    assert np.abs(r["Vg"][:, j] - r["Vo"][:, j]).max() < 1e-12
+    if j:
+        assert np.abs(r["Vg"][:, :j].T @ r["Vg"][:, j]).max() < 1e-14
+
+
+@pytest.mark.parametrize("j", [5, 40, 60])
+def test_orthonormalize_fp32(ks, ctx, oracle, j):
+    # FP32 basis: FP32 loads/stores, FP64 sums and DGKS scalars; checked against the
+    # FP64 oracle run on the same (FP32-representable) data
+    rng = np.random.default_rng(200 + j)
+    n = 20011
+    V = np.zeros((n, j + 1), order="F")
+    V[:, :j] = np.linalg.qr(rng.standard_normal((n, j)))[0]
+    V[:, j] = rng.standard_normal(n)
+    V = V.astype(np.float32).astype(np.float64)
+    Vo = np.asfortranarray(V.copy())
+    H, nrm, lindep, passes = oracle.orthonormalize_column(Vo, j)
+    dV = ks.DeviceArray.from_numpy(ctx, np.asfortranarray(V.astype(np.float32)))
+    buf = ks.DeviceArray(ctx, (j + 2, j + 2))
+    ctx.status_reset()
+    ks.orthonormalize_column(ctx, ks.F32, n, dV, n, j, buf, j + 2, j, j)
+    st = ctx.status()
+    Vg = dV.numpy().astype(np.float64)
+    h = buf.numpy()[:, j]
+    assert st.done == 1 and not st.lindep
+    assert abs(st.nrm - nrm) <= 1e-5 * nrm
+    assert np.allclose(h[:j + 1], H, rtol=1e-4, atol=1e-5 * nrm)
+    assert abs(np.linalg.norm(Vg[:, j]) - 1) < 1e-5
+    assert np.abs(Vg[:, :j].T @ Vg[:, j]).max() < 1e-5
+
+
 def test_dgks_second_pass_and_lindep(ks, ctx, oracle):
     rng = np.random.default_rng(5)
     n = 20000

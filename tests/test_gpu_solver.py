@@ -45,6 +45,24 @@ def test_ex2_golden(E):
     _check_pairs(M.laplacian_2d(72), r, 1e-8)
 
 
+def test_ex4_rdb200_golden(E, oracle):
+    # src/eps/tutorials/ex4.c on share/slepc/datafiles/matrices/rdb200.petsc (fixture copy),
+    # read with the PETSc-binary MatLoad reader; output/ex4_1.out: the 4 largest-magnitude
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rdb200.petsc")
+    A = E.Mat.load(path)
+    assert A.n == 200 and A.info()["nnz"] == 1120
+    A.destroy()
+    csr = E.read_petsc_binary(path)
+    v0 = oracle.random_uniform(0, 200)
+    r = E.solve(csr, nev=4, ncv=19, tol=1e-8, hermitian=False, v0=v0)
+    assert r["reason"] == 1
+    assert ", ".join("%.5f" % v for v in r["eigr"][:4]) == "-35.00752, -34.10419, -33.20131, -32.68111"
+    ro = oracle.eps_solve(csr, nev=4, ncv=19, tol=1e-8, hermitian=False, v0=v0)
+    assert np.allclose(r["eigr"][:4], ro["eigr"][:4], rtol=1e-10)
+    _check_pairs(csr, r, 1e-8)
+
+
 @pytest.mark.parametrize("lock", [True, False])
 def test_eps_test2_golden(E, lock):
     tol = max(1000 * EPS_MACH, 1e-9)
