@@ -203,6 +203,8 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_update(const T *__restrict__ V,
   }
 }
 
+static_assert(KS_GS_REGMAX <= 48, "the register-resident kernel is instantiated for at most 48 columns");
+
 // which launches of a column's pass sequence run: the first pass unless a
 // previous column broke down; a fused pass only while DGKS asks for it
 template <bool FUSED>
@@ -388,13 +390,12 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_lds(const T *__restrict__ V, lo
     }
     __syncthreads();
   }
-  const int ntot = k + 1;
 #pragma unroll
   for (int j = 0; j < 2; j++) {
     double v = dacc[j];
     for (int off = tsub >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     const int c = j * cpt + tid / tsub;
-    if (sub == 0 && c <= k) partial[(size_t)blockIdx.x * ntot + c] = v;
+    if (sub == 0 && c <= k) partial[(size_t)c * gridDim.x + blockIdx.x] = v;
   }
 }
 
@@ -407,7 +408,7 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_lds(const T *__restrict__ V, lo
 // (wave butterflies, then the four waves through LDS), into its partial row.
 template <typename T, int KC, bool FUSED>
 __global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, long ld, int k, T *__restrict__ w,
-                                                    int n, int nblocks, const double *__restrict__ cbuf,
+                                                    int n, int rpb, const double *__restrict__ cbuf,
                                                     double *__restrict__ partial, const ks_status *st, int col,
                                                     int pass, int *ran) {
   if (!gs_pass_gate<FUSED>(st, col, pass)) return;
@@ -421,10 +422,12 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, lo
 #pragma unroll
   for (int c = 0; c <= KC; c++) acc[c] = 0.0;
   __syncthreads();
-  // column c is read as (uniform column base) + (32-bit row byte offset), so each
-  // load is one scalar base pair plus one vector register (n * sizeof(T) < 4 GiB)
-  const unsigned stride = (unsigned)nblocks * KS_BLOCK;
-  for (unsigned i = blockIdx.x * KS_BLOCK + tid; i < (unsigned)n; i += stride) {
+  // each workgroup owns one contiguous row range (rpb rows, a multiple of 256),
+  // so a column's accesses stay inside a few pages while the workgroup runs;
+  // column c is read as (column base) + (32-bit row byte offset), n * sizeof(T) < 4 GiB
+  const unsigned r0 = blockIdx.x * (unsigned)rpb;
+  const unsigned r1 = min((unsigned)n, r0 + (unsigned)rpb);
+  for (unsigned i = r0 + tid; i < r1; i += KS_BLOCK) {
     const unsigned off = i * (unsigned)sizeof(T);
     double v[KC];
 #pragma unroll
@@ -461,24 +464,115 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, lo
     double x = 0.0;
 #pragma unroll
     for (int q = 0; q < KS_WAVES; q++) x += red[q][src];
-    partial[(size_t)blockIdx.x * ntot + c] = x;
+    partial[(size_t)c * gridDim.x + blockIdx.x] = x;
   }
 }
 
-// The finish of a Gram-Schmidt pass, one workgroup of KS_FIN_THREADS: sums the
-// nb partial rows (fixed order: thread (c, p) adds blocks p, p+P, ... of
-// column c, then the P part sums are added in order) and takes the DGKS step.
-// A separate launch, so the sweep kernels need no fences or atomics: the
-// stream orders this workgroup after every partial row is written.
-template <bool FUSED>
-__global__ __launch_bounds__(KS_FIN_THREADS) void k_gs_fin(const double *__restrict__ partial, int nb, int k,
-                                                          double *cbuf, double *hcol, ks_status *st, int col,
-                                                          int pass, int refine, double eta, int do_scale, int safe,
-                                                          double deftol) {
+// ---------------------------------------------------------------- column-split Gram-Schmidt
+// The four wavefronts of a workgroup split the k columns (wave q owns columns
+// q*CW .. q*CW+CW-1) and walk the same rows together, 64 rows per step (one per
+// lane).  Each thread keeps only CW values and CW accumulators, so occupancy
+// stays high while every wave still issues CW coalesced 512-byte column loads
+// per step.  In the fused pass the four partial sums of V(i,:) c meet in LDS
+// (double-buffered, one barrier per step) and are added in wave order, so
+// every wave forms the same updated w_i; wave 0 stores it and accumulates
+// |w|^2.  Each workgroup owns a contiguous range of rpb rows.
+template <typename T, int CW, bool FUSED>
+__global__ __launch_bounds__(KS_BLOCK) void k_gs_split(const T *__restrict__ V, long ld, int k, T *__restrict__ w,
+                                                      int n, int rpb, const double *__restrict__ cbuf,
+                                                      double *__restrict__ partial, const ks_status *st, int col,
+                                                      int pass, int *ran) {
   if (!gs_pass_gate<FUSED>(st, col, pass)) return;
+  if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = 1;
+  __shared__ double coef[KS_WAVES * CW];
+  __shared__ double red[2][KS_WAVES][64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c0 = wave * CW;
+  if (FUSED)
+    for (int c = tid; c < KS_WAVES * CW; c += KS_BLOCK) coef[c] = c < k ? cbuf[c] : 0.0;
+  __syncthreads();
+  double acc[CW];
+#pragma unroll
+  for (int u = 0; u < CW; u++) acc[u] = 0.0;
+  double self = 0.0;
+  const unsigned r0 = blockIdx.x * (unsigned)rpb;
+  const unsigned r1 = min((unsigned)n, r0 + (unsigned)rpb);
+  int buf = 0;
+  for (unsigned i0 = r0; i0 < r1; i0 += 64) {
+    const unsigned i = i0 + lane;
+    const bool ok = i < r1;
+    const unsigned off = i * (unsigned)sizeof(T);
+    double v[CW];
+#pragma unroll
+    for (int u = 0; u < CW; u++) {
+      const char *colbase = (const char *)(V + (size_t)(c0 + u) * ld);
+      v[u] = (ok && c0 + u < k) ? (double)*(const T *)(colbase + off) : 0.0;
+    }
+    double wi = ok ? (double)w[i] : 0.0;
+    if (FUSED) {
+      double s = 0.0;
+#pragma unroll
+      for (int u = 0; u < CW; u++) s += v[u] * coef[c0 + u];
+      red[buf][wave][lane] = s;
+      __syncthreads();
+      const double tot = ((red[buf][0][lane] + red[buf][1][lane]) + red[buf][2][lane]) + red[buf][3][lane];
+      const T t = (T)(wi - tot);
+      wi = ok ? (double)t : 0.0;
+      if (wave == 0 && ok) w[i] = t;
+      buf ^= 1;
+    }
+#pragma unroll
+    for (int u = 0; u < CW; u++) acc[u] += v[u] * wi;
+    if (wave == 0) self += wi * wi;
+  }
+#pragma unroll
+  for (int u = 0; u < CW; u++) {
+    const double x = ks_wave_sum(acc[u]);
+    if (lane == 0 && c0 + u < k) partial[(size_t)(c0 + u) * gridDim.x + blockIdx.x] = x;
+  }
+  if (wave == 0) {
+    const double x = ks_wave_sum(self);
+    if (lane == 0) partial[(size_t)k * gridDim.x + blockIdx.x] = x;
+  }
+}
+
+// The finish of a Gram-Schmidt pass: a separate launch, ordered by the stream
+// after every partial sum of the sweep is written (so the sweep kernels, which
+// write most of the data, need no fences or atomics).  The sweeps store their
+// partial sums column-major, partial[c * nb + b]; here each of the k+1
+// columns gets a workgroup that adds its nb values in a fixed order
+// (coalesced, eight loads in flight per thread, then the workgroup tree), and
+// the last workgroup to finish -- counted with one device-scope atomic after
+// a fence, cheap in this kernel since it has written only k+1 sums -- takes
+// the DGKS step on them.  The counter resets itself.
+template <bool FUSED>
+__global__ __launch_bounds__(KS_BLOCK) void k_gs_fin(const double *__restrict__ partial, int nb, int k,
+                                                    double *sums, unsigned *counter, double *cbuf, double *hcol,
+                                                    ks_status *st, int col, int pass, int refine, double eta,
+                                                    int do_scale, int safe, double deftol) {
+  if (!gs_pass_gate<FUSED>(st, col, pass)) return;
+  __shared__ double red[KS_WAVES];
   __shared__ double fin[KS_NCOLP];
-  __shared__ double part[KS_FIN_THREADS];
-  ks_sum_partials(partial, nb, k + 1, fin, part);
+  __shared__ unsigned ticket;
+  const double *p = partial + (size_t)blockIdx.x * nb;
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tail = 0.0;
+  int b = threadIdx.x;
+  for (; b + 7 * KS_BLOCK < nb; b += 8 * KS_BLOCK) {
+#pragma unroll
+    for (int u = 0; u < 8; u++) a[u] += p[b + u * KS_BLOCK];
+  }
+  for (; b < nb; b += KS_BLOCK) tail += p[b];
+  const double s = ks_block_sum((((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]))) + tail, red);
+  if (threadIdx.x == 0) sums[blockIdx.x] = s;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) ticket = atomicAdd(counter, 1u);
+  __syncthreads();
+  if (ticket != (unsigned)k) return;   // k+1 workgroups: the last one continues
+  __threadfence();
+  for (int c = threadIdx.x; c <= k; c += blockDim.x) fin[c] = __builtin_nontemporal_load(&sums[c]);
+  if (threadIdx.x == 0) *counter = 0u;
+  __syncthreads();
   gs_finish<FUSED>(fin, k, cbuf, hcol, st, col, refine, eta, do_scale, safe, deftol);
 }
 
@@ -681,18 +775,37 @@ int split_checked(int n, int *rpt, int *nb) {
 
 // geometry of the LDS-staged Gram-Schmidt kernels for k columns
 struct gs_geom {
-  int R, tsub, nb, cpb;
+  int R, tsub, nb, cpb;   // cpb: LDS kernel sub-chunks per workgroup; register kernel rows per workgroup
   size_t lds;
   int kc;   // > 0: register-resident kernel for up to kc columns
+  int cw;   // > 0: column-split kernel, cw columns per wavefront
 };
 
 int gs_geometry(int n, int k, size_t esize, gs_geom *g) {
   g->kc = 0;
+  g->cw = 0;
   if (k <= KS_GS_REGMAX && (uint64_t)n * esize < (1ull << 32)) {
     g->kc = k <= 8 ? 8 : (k + 7) / 8 * 8;
-    const long blocks = (n + (long)KS_BLOCK - 1) / KS_BLOCK;
-    g->nb = (int)(blocks < KS_GS_REG_BLOCKS ? blocks : KS_GS_REG_BLOCKS);
-    g->cpb = 0;
+    const long tiles = (n + (long)KS_BLOCK - 1) / KS_BLOCK;            // 256-row tiles
+    const long nb = tiles < KS_GS_REG_BLOCKS ? tiles : KS_GS_REG_BLOCKS;
+    const long tpb = (tiles + nb - 1) / nb;                             // tiles per workgroup
+    g->cpb = (int)(tpb * KS_BLOCK);                                     // rows per workgroup
+    g->nb = (int)((n + (long)g->cpb - 1) / g->cpb);
+    g->R = 0;
+    g->tsub = 0;
+    g->lds = 0;
+    return KS_OK;
+  }
+  if (k <= KS_GS_SPLITMAX && (uint64_t)n * esize < (1ull << 32)) {
+    const int per = (k + KS_WAVES - 1) / KS_WAVES;
+    static const int cws[] = {2, 4, 8, 12, 16, 24, 32};
+    for (int c : cws)
+      if (per <= c) { g->cw = c; break; }
+    const long tiles = (n + 63L) / 64;                                  // 64-row steps
+    const long nb = tiles < KS_GS_SPLIT_BLOCKS ? tiles : KS_GS_SPLIT_BLOCKS;
+    const long tpb = (tiles + nb - 1) / nb;
+    g->cpb = (int)(tpb * 64);                                           // rows per workgroup
+    g->nb = (int)((n + (long)g->cpb - 1) / g->cpb);
     g->R = 0;
     g->tsub = 0;
     g->lds = 0;
@@ -723,10 +836,21 @@ template <typename T, bool FUSED>
 void launch_gs(ks_ctx ctx, const gs_geom &g, const T *V, long ld, int k, T *w, int n, const double *cbuf, int col,
                int pass, int *ran) {
   dim3 grid(g.nb), block(KS_BLOCK);
+  if (g.cw) {
+#define KS_GSS_CASE(CC)                                                                                           \
+  case CC:                                                                                                        \
+    hipLaunchKernelGGL((k_gs_split<T, CC, FUSED>), grid, block, 0, ctx->stream, V, ld, k, w, n, g.cpb, cbuf,       \
+                       ctx->d_partial, ctx->d_status, col, pass, ran);                                             \
+    break;
+    switch (g.cw) { KS_GSS_CASE(2) KS_GSS_CASE(4) KS_GSS_CASE(8) KS_GSS_CASE(12) KS_GSS_CASE(16) KS_GSS_CASE(24)
+                    KS_GSS_CASE(32) }
+#undef KS_GSS_CASE
+    return;
+  }
   if (g.kc) {
 #define KS_GSR_CASE(KK)                                                                                           \
   case KK:                                                                                                        \
-    hipLaunchKernelGGL((k_gs_reg<T, KK, FUSED>), grid, block, 0, ctx->stream, V, ld, k, w, n, g.nb, cbuf,          \
+    hipLaunchKernelGGL((k_gs_reg<T, KK, FUSED>), grid, block, 0, ctx->stream, V, ld, k, w, n, g.cpb, cbuf,         \
                        ctx->d_partial, ctx->d_status, col, pass, ran);                                             \
     break;
     switch (g.kc) { KS_GSR_CASE(8) KS_GSR_CASE(16) KS_GSR_CASE(24) KS_GSR_CASE(32) KS_GSR_CASE(40) KS_GSR_CASE(48) }
@@ -747,8 +871,9 @@ void launch_gs_fin(ks_ctx ctx, const gs_geom &g, int k, double *cbuf, double *hc
                    double eta, int do_scale, int safe) {
   const double deftol = 10.0 * 2.220446049250313e-16;
   ks_prof_begin(ctx, KS_PROF_GS_FIN, (double)g.nb * (k + 1) * 8.0, (double)g.nb * (k + 1), false);
-  hipLaunchKernelGGL((k_gs_fin<FUSED>), dim3(1), dim3(KS_FIN_THREADS), 0, ctx->stream, ctx->d_partial, g.nb, k, cbuf,
-                     hcol, ctx->d_status, col, pass, refine, eta, do_scale, safe, deftol);
+  hipLaunchKernelGGL((k_gs_fin<FUSED>), dim3(k + 1), dim3(KS_BLOCK), 0, ctx->stream, ctx->d_partial, g.nb, k,
+                     ctx->d_gpartial, ctx->d_counters, cbuf, hcol, ctx->d_status, col, pass, refine, eta, do_scale,
+                     safe, deftol);
   ks_prof_end(ctx);
 }
 

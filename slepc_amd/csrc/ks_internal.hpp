@@ -13,19 +13,26 @@
 #define KS_WAVES (KS_BLOCK / 64)
 #define KS_MAXB 8192                 // maximum blocks of one row-partitioned reduction
 #define KS_GROUP 32                  // blocks per first-level reduction group
+#ifndef KS_GS_SPLITMAX
+#define KS_GS_SPLITMAX 128           // Gram-Schmidt against <= this many columns: column-split kernel
+#endif
+#ifndef KS_GS_SPLIT_BLOCKS
+#define KS_GS_SPLIT_BLOCKS 4096      // workgroups of the column-split Gram-Schmidt
+#endif
 #ifndef KS_GS_REGMAX
-#define KS_GS_REGMAX 48              // Gram-Schmidt against <= this many columns keeps rows in registers
+#define KS_GS_REGMAX 16              // Gram-Schmidt against <= this many columns: register-resident kernel
 #endif
 #ifndef KS_GS_REG_BLOCKS
-#define KS_GS_REG_BLOCKS 1024        // workgroups of the register-resident Gram-Schmidt
+#define KS_GS_REG_BLOCKS 512         // workgroups of the register-resident Gram-Schmidt
 #endif
-#define KS_FIN_THREADS 1024          // threads of the one-workgroup finishing kernels
 #ifndef KS_GS_BATCH
 #define KS_GS_BATCH 8                // column loads in flight per thread in the Gram-Schmidt sweep
 #endif
 #define KS_MAXG (KS_MAXB / KS_GROUP)
 #define KS_NCOLP (KS_MAX_COLS + 2)   // partial row width: columns + self dot + spare
-#define KS_SPMV_NNZ 1024             // products staged in LDS per SpMV row block
+#ifndef KS_SPMV_NNZ
+#define KS_SPMV_NNZ 2048             // products staged in LDS per SpMV row block (a multiple of KS_BLOCK)
+#endif
 #define KS_SPMV_ROWS 256             // rows per SpMV row block (one per thread)
 
 #define KS_HIP(call)                                                                    \
@@ -206,37 +213,6 @@ __device__ __forceinline__ bool ks_reduce_finish(double *partial, double *gparti
   if (threadIdx.x == 0) counters[KS_MAXG] = 0u;
   __syncthreads();
   return true;
-}
-
-// out[c] = sum_b partial[b * ncol + c] for c < ncol, by one workgroup in a
-// fixed order: thread t takes column t % ncol and the blocks p, p + P, ...
-// (p = t / ncol, P = threads / ncol), eight loads in flight at a time; the P
-// part sums are then added in order.  part: >= blockDim.x doubles of LDS.
-__device__ __forceinline__ void ks_sum_partials(const double *__restrict__ partial, int nb, int ncol,
-                                                double *out, double *part) {
-  const int nt = blockDim.x, t = threadIdx.x;
-  for (int c0 = 0; c0 < ncol; c0 += nt) {   // more columns than threads: column slabs
-    const int cols = min(nt, ncol - c0);
-    const int P = nt / cols;
-    const int c = c0 + t % cols, p = t / cols;
-    if (p < P) {
-      double a[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tail = 0.0;
-      int b = p;
-      for (; b + 7 * P < nb; b += 8 * P) {
-#pragma unroll
-        for (int u = 0; u < 8; u++) a[u] += partial[(size_t)(b + u * P) * ncol + c];
-      }
-      for (; b < nb; b += P) tail += partial[(size_t)b * ncol + c];
-      part[p * cols + (c - c0)] = (((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]))) + tail;
-    }
-    __syncthreads();
-    if (t < cols) {
-      double x = 0.0;
-      for (int q = 0; q < P; q++) x += part[q * cols + t];
-      out[c0 + t] = x;
-    }
-    __syncthreads();
-  }
 }
 
 // rows per thread (1,2,4,8,16) and block count for a row-partitioned reduction

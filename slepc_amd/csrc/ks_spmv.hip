@@ -43,7 +43,27 @@ __global__ __launch_bounds__(KS_BLOCK) void k_spmv_csr(const int32_t *__restrict
     if (tid == 0) y[r0] = (XT)tot;
     return;
   }
-  for (int p = p0 + tid; p < p1; p += KS_BLOCK) prod[p - p0] = (AT)val[p] * (AT)x[colidx[p]];
+  {
+    // all of this thread's column indices and values are loaded first, then all
+    // its x gathers: KS_SPMV_NNZ / 256 independent requests in flight per level
+    constexpr int PER = KS_SPMV_NNZ / KS_BLOCK;
+    int ci[PER];
+    AT va[PER], xv[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int p = p0 + tid + u * KS_BLOCK;
+      const bool ok = p < p1;
+      ci[u] = ok ? colidx[p] : 0;
+      va[u] = ok ? (AT)val[p] : (AT)0;
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) xv[u] = (p0 + tid + u * KS_BLOCK < p1) ? (AT)x[ci[u]] : (AT)0;
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int p = p0 + tid + u * KS_BLOCK;
+      if (p < p1) prod[p - p0] = va[u] * xv[u];
+    }
+  }
   __syncthreads();
   // threads per row: largest power of two with nrows * tpr <= KS_BLOCK, at most 64
   int tpr = 1;

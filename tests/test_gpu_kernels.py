@@ -183,7 +183,7 @@ def test_orthonormalize_vs_oracle(ks, ctx, oracle):
     assert np.abs(r["Vg"][:, :j].T @ r["Vg"][:, j]).max() < 1e-14
 
 
-@pytest.mark.parametrize("j", [0, 1, 9, 24, 47, 48, 49, 96, 200])
+@pytest.mark.parametrize("j", [0, 1, 9, 16, 17, 24, 48, 96, 128, 129, 200])
 def test_orthonormalize_widths(ks, ctx, oracle, j):
     # both Gram-Schmidt kernels: register-resident rows (<= 48 columns) and LDS tiles (more)
     rng = np.random.default_rng(100 + j)
@@ -202,7 +202,7 @@ def test_orthonormalize_widths(ks, ctx, oracle, j):
         assert np.abs(r["Vg"][:, :j].T @ r["Vg"][:, j]).max() < 1e-14
 
 
-@pytest.mark.parametrize("j", [5, 40, 60])
+@pytest.mark.parametrize("j", [5, 40, 60, 150])
 def test_orthonormalize_fp32(ks, ctx, oracle, j):
     # FP32 basis: FP32 loads/stores, FP64 sums and DGKS scalars; checked against the
     # FP64 oracle run on the same (FP32-representable) data
