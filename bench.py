@@ -265,7 +265,24 @@ def run_mixed(args):
     v0 = E.Vec.random(n, 0)
     nev, ncv, tol = 8, 40, 1e-10
 
+    from slepc_amd import ks as K
+    L = K.lib()
+    ctx = E.context_handle()
+
+    def prof_table():
+        K.check(L.ks_prof_collect(ctx), "ks_prof_collect")
+        tab = {}
+        for i, name in enumerate(K.PROF_CLASSES):
+            e = K.ProfEntry()
+            K.check(L.ks_prof_get(ctx, i, C_byref(e)), "ks_prof_get")
+            if e.launches and e.ms > 0:
+                tab[name] = {"launches": e.launches, "ms": round(e.ms, 3),
+                             "GBs": round(e.bytes / (e.ms * 1e-3) / 1e9, 1)}
+        return tab
+
     def solve(mixed):
+        K.check(L.ks_prof_reset(ctx), "ks_prof_reset")
+        K.check(L.ks_prof_enable(ctx, 1), "ks_prof_enable")
         eps = E.EPS()
         eps.setOperators(A)
         eps.setProblemType(E.EPS.NHEP)
@@ -278,6 +295,8 @@ def run_mixed(args):
         ts = time.perf_counter()
         eps.solve()
         dt = time.perf_counter() - ts
+        prof = prof_table()
+        K.check(L.ks_prof_enable(ctx, 0), "ks_prof_enable")
         k = min(eps.getConverged(), nev)
         errs = [eps.computeError(i) for i in range(k)]
         vals = [eps.getEigenvalue(i) for i in range(k)]
@@ -285,7 +304,7 @@ def run_mixed(args):
         ms = eps.mixedStats() if mixed else {}
         eps.destroy()
         return dict(t=dt, verified=sum(e <= tol for e in errs), errs=errs, vals=vals, its=st["its"],
-                    nmatmult=st["nmatmult"], mixed=ms)
+                    nmatmult=st["nmatmult"], mixed=ms, prof=prof)
 
     res = {}
     for form in ("fp64", "mixed"):
@@ -309,7 +328,8 @@ def run_mixed(args):
                                "solve_s": [round(r["t"], 4) for r in res[f]],
                                "restarts": [r["its"] for r in res[f]],
                                "spmv": [r["nmatmult"] for r in res[f]],
-                               "verified": [r["verified"] for r in res[f]]} for f in res},
+                               "verified": [r["verified"] for r in res[f]],
+                               "kernels": res[f][-1]["prof"]} for f in res},
                  "mixed_phase": res["mixed"][-1]["mixed"],
                  "speedup_vs_fp64": round(rate["mixed"] / rate["fp64"], 3) if rate["fp64"] else None,
                  "pair_rel_error_vs_fp64": [float(f"{e:.3e}") for e in pair_err],
@@ -340,8 +360,14 @@ def run_serve(args):
     cache = {i: m for i, m in enumerate(mats)}
     for i, r in enumerate(reqs):
         r["_i"] = i
-    # served: concurrent, admitted against the budget
-    srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=args.slots)
+    # served: concurrent, admitted against the budget; first with each request
+    # launching its own steps (streams only), then with grouped steps
+    srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=args.slots, grouped=False)
+    res_ng, wall_ng = S.run_stream(srv, reqs, lambda r: cache[r["_i"]], tol=1e-10)
+    srv.close()
+    good_ng = sum(r["ngood"] for r in res_ng)
+    log(f"[serve] streams only: {wall_ng:.2f}s, {good_ng} verified pairs")
+    srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=args.slots, grouped=True)
     results, wall = S.run_stream(srv, reqs, lambda r: cache[r["_i"]], tol=1e-10)
     stats = srv.stats()
     srv.close()
@@ -370,7 +396,13 @@ def run_serve(args):
                 {"latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
                                 "p99": round(float(np.percentile(lat, 99)), 2)},
                  "served": {"eigenpairs_per_s": round(rate, 4), "wall_s": round(wall, 3), "verified_pairs": good,
-                            "requests": len(reqs), "peak_reserved_bytes": stats["peak_reserved"]},
+                            "requests": len(reqs), "peak_reserved_bytes": stats["peak_reserved"],
+                            "grouped_steps": stats["group_steps"],
+                            "members_per_grouped_step": round(stats["group_member_steps"] / max(1, stats["group_steps"]), 2)},
+                 "served_streams_only": {"eigenpairs_per_s": round(good_ng / wall_ng, 4), "wall_s": round(wall_ng, 3),
+                                         "verified_pairs": good_ng,
+                                         "latency_ms_p50": round(float(np.percentile([r["latency"] * 1e3 for r in res_ng], 50)), 2),
+                                         "latency_ms_p99": round(float(np.percentile([r["latency"] * 1e3 for r in res_ng], 99)), 2)},
                  "sequential": {"eigenpairs_per_s": round(seq_rate, 4), "wall_s": round(seq_wall, 3),
                                 "verified_pairs": seq_good,
                                 "latency_ms_p50": round(float(np.percentile(np.array(seq_lat) * 1e3, 50)), 2)},

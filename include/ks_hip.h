@@ -41,6 +41,7 @@ typedef enum { KS_REFINE_IFNEEDED = 0, KS_REFINE_NEVER = 1, KS_REFINE_ALWAYS = 2
 typedef struct ks_ctx_s *ks_ctx;
 typedef struct ks_csr_s *ks_csr;
 typedef struct ks_pool_s *ks_pool;
+typedef struct ks_group_s *ks_group;
 
 /* Device-side state of the fused CGS+DGKS orthonormalization (the scalars of
    BVOrthogonalizeGS, src/sys/classes/bv/interface/bvorthog.c:145-217, kept on
@@ -198,6 +199,35 @@ int ks_prof_collect(ks_ctx ctx);
 int ks_prof_get(ks_ctx ctx, int cls, ks_prof_entry *out);
 int ks_prof_reset(ks_ctx ctx);
 const char *ks_prof_name(int cls);
+
+/* ---------------------------------------------------------------- grouped steps (serving)
+ * No reference counterpart (SLEPc runs one solve at a time).  One Arnoldi step
+ * for each of up to 16 independent FP64 solves with one launch per kernel
+ * kind on the group's own stream: member m computes column k of its basis as
+ * A * column k-1 (gated on its status like ks_spmv with gate), then the whole
+ * CGS+DGKS orthonormalization of column k against its first k columns and the
+ * scaling (as ks_bv_orthonormalize_column with hcol, col, scale = 1, safe = 0).
+ * Each member uses its own context's status and reduction workspace, so its
+ * results are its own; the member's stream must not run work meanwhile
+ * (order them with ks_group_wait_ctx / ks_ctx_wait_group).  Members need
+ * k <= 128 and n * 8 < 4 GiB. */
+typedef struct {
+  ks_ctx ctx;       /* the member's context (status, reduction workspace) */
+  ks_csr A;         /* FP64 values */
+  void *V;          /* FP64 basis, leading dimension ld */
+  int ld, n;
+  int k;            /* column written by the SpMV and orthonormalized */
+  double *buf;      /* coefficient buffer, leading dimension ldbuf */
+  int ldbuf, hcol, col;
+} ks_group_member;
+int ks_group_create(ks_group *group);
+int ks_group_destroy(ks_group group);
+int ks_group_step(ks_group group, int nmem, const ks_group_member *members, ks_refine refine, double eta);
+/* steps enqueued on the group that have not finished yet */
+int ks_group_inflight(ks_group group, int *steps);
+/* the group's stream waits for the work enqueued so far on ctx, and the reverse */
+int ks_group_wait_ctx(ks_group group, ks_ctx ctx);
+int ks_ctx_wait_group(ks_ctx ctx, ks_group group);
 
 #ifdef __cplusplus
 }

@@ -315,6 +315,10 @@ PetscErrorCode SlepcServerProgress(SlepcServer srv, PetscBool *progressed, Petsc
 PetscErrorCode SlepcServerGetResult(SlepcServer srv, PetscInt id, PetscBool *done, SlepcResult *res,
                                     PetscScalar *eigr, PetscScalar *eigi, PetscReal *err, PetscScalar *X);
 PetscErrorCode SlepcServerGetStats(SlepcServer srv, size_t *peak_reserved, size_t *pool_peak, size_t *budget);
+/* grouping (default on): the Arnoldi steps of all expanding requests go out as one
+   launch per kernel kind (ks_group_step); off: each request launches its own */
+PetscErrorCode SlepcServerSetGrouped(SlepcServer srv, PetscBool grouped);
+PetscErrorCode SlepcServerGetGroupStats(SlepcServer srv, int64_t *steps, int64_t *member_steps);
 PetscErrorCode SlepcServerDestroy(SlepcServer *srv);
 double SlepcServerClock(void);   /* monotonic seconds, the clock of t_arrival/t_admit/t_done */
 

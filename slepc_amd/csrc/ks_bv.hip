@@ -477,63 +477,101 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, lo
 // (double-buffered, one barrier per step) and are added in wave order, so
 // every wave forms the same updated w_i; wave 0 stores it and accumulates
 // |w|^2.  Each workgroup owns a contiguous range of rpb rows.
-template <typename T, int CW, bool FUSED>
+// workgroup b of nb: rows b*rpb .. (b+1)*rpb-1.  PAIR (FP32 data, even ld and
+// 8-byte aligned columns): each lane takes two adjacent rows with one 8-byte
+// load per column, so a wave still moves 512 contiguous bytes per column per
+// step, as with FP64 data.
+template <typename T, int CW, bool FUSED, bool PAIR>
+__device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restrict__ V, long ld, int k,
+                                               T *__restrict__ w, int n, int rpb, const double *__restrict__ cbuf,
+                                               double *__restrict__ partial) {
+  static_assert(!PAIR || sizeof(T) == 4, "row pairs are for FP32 data");
+  using AT = typename acc_t<T>::type;   // FP32 data: FP32 products and per-thread sums
+  constexpr int RPL = PAIR ? 2 : 1;     // rows per lane per step
+  constexpr int STEP = 64 * RPL;
+  __shared__ AT coef[KS_WAVES * CW];
+  __shared__ AT red[2][KS_WAVES][STEP];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c0 = wave * CW;
+  if (FUSED)
+    for (int c = tid; c < KS_WAVES * CW; c += KS_BLOCK) coef[c] = (AT)(c < k ? cbuf[c] : 0.0);
+  __syncthreads();
+  AT acc[CW];
+#pragma unroll
+  for (int u = 0; u < CW; u++) acc[u] = (AT)0;
+  AT self = (AT)0;
+  const unsigned r0 = (unsigned)b * (unsigned)rpb;
+  const unsigned r1 = min((unsigned)n, r0 + (unsigned)rpb);
+  int buf = 0;
+  for (unsigned i0 = r0; i0 < r1; i0 += STEP) {
+    const unsigned ia = i0 + RPL * lane;   // this lane's first row
+    T v[RPL][CW];
+    AT wi[RPL];
+#pragma unroll
+    for (int u = 0; u < CW; u++) {
+      const T *col = V + (size_t)(c0 + u) * ld;
+      const bool have = c0 + u < k;
+      if (PAIR) {
+        if (have && ia + 1 < r1) {
+          const float2 p2 = *(const float2 *)(col + ia);
+          v[0][u] = p2.x;
+          v[RPL - 1][u] = p2.y;
+        } else {
+          v[0][u] = (have && ia < r1) ? col[ia] : (T)0;
+          v[RPL - 1][u] = (T)0;
+        }
+      } else {
+        v[0][u] = (have && ia < r1) ? col[ia] : (T)0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < RPL; q++) wi[q] = ia + q < r1 ? (AT)w[ia + q] : (AT)0;
+    if (FUSED) {
+#pragma unroll
+      for (int q = 0; q < RPL; q++) {
+        AT s = (AT)0;
+#pragma unroll
+        for (int u = 0; u < CW; u++) s += (AT)v[q][u] * coef[c0 + u];
+        red[buf][wave][RPL * lane + q] = s;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < RPL; q++) {
+        const int r = RPL * lane + q;
+        const unsigned i = ia + q;
+        const AT tot = ((red[buf][0][r] + red[buf][1][r]) + red[buf][2][r]) + red[buf][3][r];
+        const T t = (T)(wi[q] - tot);
+        wi[q] = i < r1 ? (AT)t : (AT)0;
+        if (wave == 0 && i < r1) w[i] = t;
+      }
+      buf ^= 1;
+    }
+#pragma unroll
+    for (int q = 0; q < RPL; q++) {
+#pragma unroll
+      for (int u = 0; u < CW; u++) acc[u] += (AT)v[q][u] * wi[q];
+      if (wave == 0) self += wi[q] * wi[q];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < CW; u++) {
+    const double x = ks_wave_sum((double)acc[u]);   // lanes and workgroups combine in FP64
+    if (lane == 0 && c0 + u < k) partial[(size_t)(c0 + u) * nb + b] = x;
+  }
+  if (wave == 0) {
+    const double x = ks_wave_sum((double)self);
+    if (lane == 0) partial[(size_t)k * nb + b] = x;
+  }
+}
+
+template <typename T, int CW, bool FUSED, bool PAIR>
 __global__ __launch_bounds__(KS_BLOCK) void k_gs_split(const T *__restrict__ V, long ld, int k, T *__restrict__ w,
                                                       int n, int rpb, const double *__restrict__ cbuf,
                                                       double *__restrict__ partial, const ks_status *st, int col,
                                                       int pass, int *ran) {
   if (!gs_pass_gate<FUSED>(st, col, pass)) return;
   if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = 1;
-  __shared__ double coef[KS_WAVES * CW];
-  __shared__ double red[2][KS_WAVES][64];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int c0 = wave * CW;
-  if (FUSED)
-    for (int c = tid; c < KS_WAVES * CW; c += KS_BLOCK) coef[c] = c < k ? cbuf[c] : 0.0;
-  __syncthreads();
-  double acc[CW];
-#pragma unroll
-  for (int u = 0; u < CW; u++) acc[u] = 0.0;
-  double self = 0.0;
-  const unsigned r0 = blockIdx.x * (unsigned)rpb;
-  const unsigned r1 = min((unsigned)n, r0 + (unsigned)rpb);
-  int buf = 0;
-  for (unsigned i0 = r0; i0 < r1; i0 += 64) {
-    const unsigned i = i0 + lane;
-    const bool ok = i < r1;
-    const unsigned off = i * (unsigned)sizeof(T);
-    double v[CW];
-#pragma unroll
-    for (int u = 0; u < CW; u++) {
-      const char *colbase = (const char *)(V + (size_t)(c0 + u) * ld);
-      v[u] = (ok && c0 + u < k) ? (double)*(const T *)(colbase + off) : 0.0;
-    }
-    double wi = ok ? (double)w[i] : 0.0;
-    if (FUSED) {
-      double s = 0.0;
-#pragma unroll
-      for (int u = 0; u < CW; u++) s += v[u] * coef[c0 + u];
-      red[buf][wave][lane] = s;
-      __syncthreads();
-      const double tot = ((red[buf][0][lane] + red[buf][1][lane]) + red[buf][2][lane]) + red[buf][3][lane];
-      const T t = (T)(wi - tot);
-      wi = ok ? (double)t : 0.0;
-      if (wave == 0 && ok) w[i] = t;
-      buf ^= 1;
-    }
-#pragma unroll
-    for (int u = 0; u < CW; u++) acc[u] += v[u] * wi;
-    if (wave == 0) self += wi * wi;
-  }
-#pragma unroll
-  for (int u = 0; u < CW; u++) {
-    const double x = ks_wave_sum(acc[u]);
-    if (lane == 0 && c0 + u < k) partial[(size_t)(c0 + u) * gridDim.x + blockIdx.x] = x;
-  }
-  if (wave == 0) {
-    const double x = ks_wave_sum(self);
-    if (lane == 0) partial[(size_t)k * gridDim.x + blockIdx.x] = x;
-  }
+  gs_split_block<T, CW, FUSED, PAIR>(blockIdx.x, gridDim.x, V, ld, k, w, n, rpb, cbuf, partial);
 }
 
 // The finish of a Gram-Schmidt pass: a separate launch, ordered by the stream
@@ -546,24 +584,22 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_split(const T *__restrict__ V, 
 // a fence, cheap in this kernel since it has written only k+1 sums -- takes
 // the DGKS step on them.  The counter resets itself.
 template <bool FUSED>
-__global__ __launch_bounds__(KS_BLOCK) void k_gs_fin(const double *__restrict__ partial, int nb, int k,
-                                                    double *sums, unsigned *counter, double *cbuf, double *hcol,
-                                                    ks_status *st, int col, int pass, int refine, double eta,
-                                                    int do_scale, int safe, double deftol) {
-  if (!gs_pass_gate<FUSED>(st, col, pass)) return;
+__device__ __forceinline__ void gs_fin_block(int b, const double *__restrict__ partial, int nb, int k, double *sums,
+                                             unsigned *counter, double *cbuf, double *hcol, ks_status *st, int col,
+                                             int refine, double eta, int do_scale, int safe, double deftol) {
   __shared__ double red[KS_WAVES];
   __shared__ double fin[KS_NCOLP];
   __shared__ unsigned ticket;
-  const double *p = partial + (size_t)blockIdx.x * nb;
+  const double *p = partial + (size_t)b * nb;
   double a[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tail = 0.0;
-  int b = threadIdx.x;
-  for (; b + 7 * KS_BLOCK < nb; b += 8 * KS_BLOCK) {
+  int q = threadIdx.x;
+  for (; q + 7 * KS_BLOCK < nb; q += 8 * KS_BLOCK) {
 #pragma unroll
-    for (int u = 0; u < 8; u++) a[u] += p[b + u * KS_BLOCK];
+    for (int u = 0; u < 8; u++) a[u] += p[q + u * KS_BLOCK];
   }
-  for (; b < nb; b += KS_BLOCK) tail += p[b];
+  for (; q < nb; q += KS_BLOCK) tail += p[q];
   const double s = ks_block_sum((((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]))) + tail, red);
-  if (threadIdx.x == 0) sums[blockIdx.x] = s;
+  if (threadIdx.x == 0) sums[b] = s;
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) ticket = atomicAdd(counter, 1u);
@@ -576,15 +612,69 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_fin(const double *__restrict__ 
   gs_finish<FUSED>(fin, k, cbuf, hcol, st, col, refine, eta, do_scale, safe, deftol);
 }
 
+template <bool FUSED>
+__global__ __launch_bounds__(KS_BLOCK) void k_gs_fin(const double *__restrict__ partial, int nb, int k,
+                                                    double *sums, unsigned *counter, double *cbuf, double *hcol,
+                                                    ks_status *st, int col, int pass, int refine, double eta,
+                                                    int do_scale, int safe, double deftol) {
+  if (!gs_pass_gate<FUSED>(st, col, pass)) return;
+  gs_fin_block<FUSED>(blockIdx.x, partial, nb, k, sums, counter, cbuf, hcol, st, col, refine, eta, do_scale, safe,
+                      deftol);
+}
+
+template <typename T>
+__device__ __forceinline__ void scale_status_block(int b, int nblk, T *__restrict__ w, int n, double s) {
+  size_t i = (size_t)b * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)nblk * blockDim.x;
+  for (; i < (size_t)n; i += stride) w[i] = (T)((typename acc_t<T>::type)w[i] * (typename acc_t<T>::type)s);
+}
+
 template <typename T>
 __global__ void k_scale_status(T *__restrict__ w, int n, const ks_status *__restrict__ st, int col, int *ran) {
   if (!(st->col == col && st->done)) return;
   const double s = st->scale;
   if (s == 1.0) return;
   if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = 1;
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < (size_t)n; i += stride) w[i] = (T)((typename acc_t<T>::type)w[i] * (typename acc_t<T>::type)s);
+  scale_status_block<T>(blockIdx.x, gridDim.x, w, n, s);
+}
+
+// ---------------------------------------------------------------- grouped Gram-Schmidt (serving)
+// The sweeps, finishes and scalings of several independent solves, one launch
+// per kind: workgroup b belongs to the member whose prefix range holds it and
+// runs that member's workgroup with the member's own status (gates), column
+// count, rows and reduction workspace.  The member bodies are the single-solve
+// kernels' own.
+__device__ __forceinline__ int grp_member(const int *b0, int nmem, int b) {
+  int m = 0;
+  while (m + 1 < nmem && b >= b0[m + 1]) m++;
+  return m;
+}
+
+template <int CW, bool FUSED>
+__global__ __launch_bounds__(KS_BLOCK) void k_grp_gs(ks_grp_gs g, int pass) {
+  const int m = grp_member(g.b0_gs, g.nmem, blockIdx.x);
+  const ks_grp_gs_member &M = g.m[m];
+  if (!gs_pass_gate<FUSED>(M.st, M.col, pass)) return;
+  gs_split_block<double, CW, FUSED, false>((int)blockIdx.x - g.b0_gs[m], M.nb, M.V, M.ld, M.k, M.w, M.n, M.rpb,
+                                           M.cbuf, M.partial);
+}
+
+template <bool FUSED>
+__global__ __launch_bounds__(KS_BLOCK) void k_grp_fin(ks_grp_gs g, int pass, int refine, double eta, double deftol) {
+  const int m = grp_member(g.b0_fin, g.nmem, blockIdx.x);
+  const ks_grp_gs_member &M = g.m[m];
+  if (!gs_pass_gate<FUSED>(M.st, M.col, pass)) return;
+  gs_fin_block<FUSED>((int)blockIdx.x - g.b0_fin[m], M.partial, M.nb, M.k, M.sums, M.counter, M.cbuf, M.hcol, M.st,
+                      M.col, refine, eta, 1, 0, deftol);
+}
+
+__global__ void k_grp_scale(ks_grp_gs g) {
+  const int m = grp_member(g.b0_sc, g.nmem, blockIdx.x);
+  const ks_grp_gs_member &M = g.m[m];
+  if (!(M.st->col == M.col && M.st->done)) return;
+  const double s = M.st->scale;
+  if (s == 1.0) return;
+  scale_status_block<double>((int)blockIdx.x - g.b0_sc[m], g.b0_sc[m + 1] - g.b0_sc[m], M.w, M.n, s);
 }
 
 template <typename T>
@@ -779,11 +869,13 @@ struct gs_geom {
   size_t lds;
   int kc;   // > 0: register-resident kernel for up to kc columns
   int cw;   // > 0: column-split kernel, cw columns per wavefront
+  int pair; // column-split on FP32 data with two rows per lane (even ld, aligned)
 };
 
 int gs_geometry(int n, int k, size_t esize, gs_geom *g) {
   g->kc = 0;
   g->cw = 0;
+  g->pair = 0;
   if (k <= KS_GS_REGMAX && (uint64_t)n * esize < (1ull << 32)) {
     g->kc = k <= 8 ? 8 : (k + 7) / 8 * 8;
     const long tiles = (n + (long)KS_BLOCK - 1) / KS_BLOCK;            // 256-row tiles
@@ -801,10 +893,11 @@ int gs_geometry(int n, int k, size_t esize, gs_geom *g) {
     static const int cws[] = {2, 4, 8, 12, 16, 24, 32};
     for (int c : cws)
       if (per <= c) { g->cw = c; break; }
-    const long tiles = (n + 63L) / 64;                                  // 64-row steps
+    const long step = esize == 4 ? 128 : 64;                            // rows per workgroup step, even
+    const long tiles = (n + step - 1) / step;
     const long nb = tiles < KS_GS_SPLIT_BLOCKS ? tiles : KS_GS_SPLIT_BLOCKS;
     const long tpb = (tiles + nb - 1) / nb;
-    g->cpb = (int)(tpb * 64);                                           // rows per workgroup
+    g->cpb = (int)(tpb * step);                                         // rows per workgroup
     g->nb = (int)((n + (long)g->cpb - 1) / g->cpb);
     g->R = 0;
     g->tsub = 0;
@@ -839,8 +932,12 @@ void launch_gs(ks_ctx ctx, const gs_geom &g, const T *V, long ld, int k, T *w, i
   if (g.cw) {
 #define KS_GSS_CASE(CC)                                                                                           \
   case CC:                                                                                                        \
-    hipLaunchKernelGGL((k_gs_split<T, CC, FUSED>), grid, block, 0, ctx->stream, V, ld, k, w, n, g.cpb, cbuf,       \
-                       ctx->d_partial, ctx->d_status, col, pass, ran);                                             \
+    if (g.pair)                                                                                                   \
+      hipLaunchKernelGGL((k_gs_split<T, CC, FUSED, sizeof(T) == 4>), grid, block, 0, ctx->stream, V, ld, k, w, n,  \
+                         g.cpb, cbuf, ctx->d_partial, ctx->d_status, col, pass, ran);                             \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_gs_split<T, CC, FUSED, false>), grid, block, 0, ctx->stream, V, ld, k, w, n, g.cpb,    \
+                         cbuf, ctx->d_partial, ctx->d_status, col, pass, ran);                                    \
     break;
     switch (g.cw) { KS_GSS_CASE(2) KS_GSS_CASE(4) KS_GSS_CASE(8) KS_GSS_CASE(12) KS_GSS_CASE(16) KS_GSS_CASE(24)
                     KS_GSS_CASE(32) }
@@ -883,6 +980,7 @@ int orthonormalize_t(ks_ctx ctx, int n, T *V, long ld, int k, double *buf, int l
   gs_geom g;
   int e;
   if ((e = gs_geometry(n, k, sizeof(T), &g))) return e;
+  g.pair = g.cw && sizeof(T) == 4 && ld % 2 == 0 && ((uintptr_t)V % 8) == 0;
   const double s = sizeof(T);
   T *w = V + (size_t)k * ld;
   double *cbuf = buf;                              // buffer column 0: scratch coefficients
@@ -912,6 +1010,55 @@ int orthonormalize_t(ks_ctx ctx, int n, T *V, long ld, int k, double *buf, int l
 }
 
 }  // namespace
+
+int ks_ew_blocks(int n) { return ew_blocks(n); }
+
+int ks_gs_split_geometry(int n, int k, int *cw, int *rpb, int *nb) {
+  gs_geom g;
+  int e = gs_geometry(n, k, sizeof(double), &g);
+  if (e) return e;
+  if (!g.cw) {
+    // the grouped launches run the column-split sweep only; its geometry for this k
+    KS_CHECK(k <= KS_GS_SPLITMAX && (uint64_t)n * sizeof(double) < (1ull << 32), KS_ERR_RANGE,
+             "grouped Gram-Schmidt supports up to %d columns (asked %d)", KS_GS_SPLITMAX, k);
+    const int per = (k + KS_WAVES - 1) / KS_WAVES;
+    static const int cws[] = {2, 4, 8, 12, 16, 24, 32};
+    for (int c : cws)
+      if (per <= c) { g.cw = c; break; }
+    const long tiles = (n + 63L) / 64;
+    const long nbl = tiles < KS_GS_SPLIT_BLOCKS ? tiles : KS_GS_SPLIT_BLOCKS;
+    g.cpb = (int)(((tiles + nbl - 1) / nbl) * 64);
+    g.nb = (int)((n + (long)g.cpb - 1) / g.cpb);
+  }
+  *cw = g.cw;
+  *rpb = g.cpb;
+  *nb = g.nb;
+  return KS_OK;
+}
+
+void ks_group_launch_gs(hipStream_t stream, const ks_grp_gs &g, int cw, int refine, double eta) {
+  const double deftol = 10.0 * 2.220446049250313e-16;
+  const int passes = refine == KS_REFINE_NEVER ? 1 : (refine == KS_REFINE_ALWAYS ? 2 : 3);
+  const dim3 block(KS_BLOCK), ggs(g.b0_gs[g.nmem]), gfin(g.b0_fin[g.nmem]), gsc(g.b0_sc[g.nmem]);
+#define KS_GRP_SWEEP(FUSED, PASS)                                                                                 \
+  switch (cw) {                                                                                                   \
+    case 2: hipLaunchKernelGGL((k_grp_gs<2, FUSED>), ggs, block, 0, stream, g, PASS); break;                     \
+    case 4: hipLaunchKernelGGL((k_grp_gs<4, FUSED>), ggs, block, 0, stream, g, PASS); break;                     \
+    case 8: hipLaunchKernelGGL((k_grp_gs<8, FUSED>), ggs, block, 0, stream, g, PASS); break;                     \
+    case 12: hipLaunchKernelGGL((k_grp_gs<12, FUSED>), ggs, block, 0, stream, g, PASS); break;                   \
+    case 16: hipLaunchKernelGGL((k_grp_gs<16, FUSED>), ggs, block, 0, stream, g, PASS); break;                   \
+    case 24: hipLaunchKernelGGL((k_grp_gs<24, FUSED>), ggs, block, 0, stream, g, PASS); break;                   \
+    default: hipLaunchKernelGGL((k_grp_gs<32, FUSED>), ggs, block, 0, stream, g, PASS); break;                   \
+  }
+  KS_GRP_SWEEP(false, 0)
+  hipLaunchKernelGGL((k_grp_fin<false>), gfin, block, 0, stream, g, 0, refine, eta, deftol);
+  for (int p = 1; p <= passes; p++) {
+    KS_GRP_SWEEP(true, p)
+    hipLaunchKernelGGL((k_grp_fin<true>), gfin, block, 0, stream, g, p, refine, eta, deftol);
+  }
+#undef KS_GRP_SWEEP
+  hipLaunchKernelGGL(k_grp_scale, gsc, dim3(256), 0, stream, g);
+}
 
 extern "C" {
 

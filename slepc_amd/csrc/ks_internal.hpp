@@ -147,6 +147,41 @@ struct ks_csr_s {
   void *d_val = nullptr;
 };
 
+// ---------------------------------------------------------------- grouped launches (serving)
+#define KS_GROUP_MAX 16   // solves per grouped launch
+// members' SpMVs: workgroups b0[m] .. b0[m+1]-1 run member m's row blocks
+struct ks_grp_spmv {
+  int nmem;
+  int b0[KS_GROUP_MAX + 1];
+  const int32_t *blk[KS_GROUP_MAX], *rowptr[KS_GROUP_MAX], *colidx[KS_GROUP_MAX];
+  const double *val[KS_GROUP_MAX];
+  const double *x[KS_GROUP_MAX];
+  double *y[KS_GROUP_MAX];
+  const ks_status *st[KS_GROUP_MAX];
+};
+// members' Gram-Schmidt of one column each (FP64 basis)
+struct ks_grp_gs_member {
+  const double *V;
+  double *w, *cbuf, *hcol, *partial, *sums;
+  ks_status *st;
+  unsigned *counter;
+  long ld;
+  int n, k, rpb, nb, col;
+};
+struct ks_grp_gs {
+  int nmem;
+  int b0_gs[KS_GROUP_MAX + 1];     // sweep workgroups
+  int b0_fin[KS_GROUP_MAX + 1];    // finishing workgroups (k+1 per member)
+  int b0_sc[KS_GROUP_MAX + 1];     // scaling workgroups
+  ks_grp_gs_member m[KS_GROUP_MAX];
+};
+void ks_group_launch_spmv(hipStream_t stream, const ks_grp_spmv &g);
+void ks_group_launch_gs(hipStream_t stream, const ks_grp_gs &g, int cw, int refine, double eta);
+int ks_gs_split_geometry(int n, int k, int *cw, int *rpb, int *nb);   // KS_ERR_RANGE beyond the split kernel
+int ks_csr_blocks(ks_csr A, const int32_t **blk, const int32_t **rowptr, const int32_t **colidx, const void **val,
+                  int *nblocks, ks_prec *vprec);
+int ks_ew_blocks(int n);
+
 // ---------------------------------------------------------------- profiling hooks
 // begin() returns a pointer to the device ran-flag for gated kernels (or null).
 extern "C" int *ks_prof_begin(ks_ctx ctx, int cls, double bytes, double flops, bool gated);

@@ -19,20 +19,16 @@ namespace {
 template <typename XT> struct acc_of { using type = double; };
 template <> struct acc_of<float> { using type = float; };
 
+// one row block b of y = A x (the body of both the single and the grouped kernel)
 template <typename VT, typename XT>
-__global__ __launch_bounds__(KS_BLOCK) void k_spmv_csr(const int32_t *__restrict__ blk,
-                                                       const int32_t *__restrict__ rowptr,
-                                                       const int32_t *__restrict__ colidx,
-                                                       const VT *__restrict__ val,
-                                                       const XT *__restrict__ x,
-                                                       XT *__restrict__ y,
-                                                       const ks_status *__restrict__ st) {
+__device__ __forceinline__ void spmv_block(int b, const int32_t *__restrict__ blk, const int32_t *__restrict__ rowptr,
+                                           const int32_t *__restrict__ colidx, const VT *__restrict__ val,
+                                           const XT *__restrict__ x, XT *__restrict__ y) {
   using AT = typename acc_of<XT>::type;
-  if (st && !st->active) return;
   __shared__ AT prod[KS_SPMV_NNZ];
   __shared__ double red[KS_WAVES];
   const int tid = threadIdx.x;
-  const int r0 = blk[blockIdx.x], r1 = blk[blockIdx.x + 1];
+  const int r0 = blk[b], r1 = blk[b + 1];
   const int p0 = rowptr[r0], p1 = rowptr[r1];
   const int nrows = r1 - r0;
   if (nrows == 1 && p1 - p0 > KS_SPMV_NNZ) {
@@ -76,6 +72,28 @@ __global__ __launch_bounds__(KS_BLOCK) void k_spmv_csr(const int32_t *__restrict
     for (int off = tpr >> 1; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if (sub == 0) y[r0 + lr] = (XT)s;
   }
+}
+
+template <typename VT, typename XT>
+__global__ __launch_bounds__(KS_BLOCK) void k_spmv_csr(const int32_t *__restrict__ blk,
+                                                       const int32_t *__restrict__ rowptr,
+                                                       const int32_t *__restrict__ colidx,
+                                                       const VT *__restrict__ val,
+                                                       const XT *__restrict__ x,
+                                                       XT *__restrict__ y,
+                                                       const ks_status *__restrict__ st) {
+  if (st && !st->active) return;
+  spmv_block<VT, XT>(blockIdx.x, blk, rowptr, colidx, val, x, y);
+}
+
+// the SpMVs of several independent solves in one launch: workgroup b belongs to
+// the member m with b0[m] <= b < b0[m+1] and runs that member's row block b - b0[m]
+__global__ __launch_bounds__(KS_BLOCK) void k_grp_spmv(ks_grp_spmv g) {
+  int m = 0;
+  while (m + 1 < g.nmem && (int)blockIdx.x >= g.b0[m + 1]) m++;
+  if (!g.st[m]->active) return;
+  spmv_block<double, double>((int)blockIdx.x - g.b0[m], g.blk[m], g.rowptr[m], g.colidx[m], g.val[m], g.x[m],
+                             g.y[m]);
 }
 
 // host: row blocks from the row lengths
@@ -132,6 +150,22 @@ int csr_check_host(int nrows, int ncols, int64_t nnz, const int32_t *rowptr, con
 }
 
 }  // namespace
+
+void ks_group_launch_spmv(hipStream_t stream, const ks_grp_spmv &g) {
+  if (g.nmem > 0 && g.b0[g.nmem] > 0)
+    hipLaunchKernelGGL(k_grp_spmv, dim3(g.b0[g.nmem]), dim3(KS_BLOCK), 0, stream, g);
+}
+
+int ks_csr_blocks(ks_csr A, const int32_t **blk, const int32_t **rowptr, const int32_t **colidx, const void **val,
+                  int *nblocks, ks_prec *vprec) {
+  *blk = A->d_blk;
+  *rowptr = A->d_rowptr;
+  *colidx = A->d_colidx;
+  *val = A->d_val;
+  *nblocks = A->nblocks;
+  *vprec = A->vprec;
+  return KS_OK;
+}
 
 // allocation inside a CSR constructor: release what was built on failure
 #define KS_TRY_CSR(M, call)                 \

@@ -159,12 +159,32 @@ static PetscErrorCode BVOrthonormalizeColumn_HIPKS(BV bv, PetscInt j, PetscBool 
    ordered: SpMV, the CGS passes, the DGKS decisions, lindep and the scaling
    stay on the device; the host waits once at the end for the status and the
    new coefficient columns. */
+/* a grouped expansion leaves the steps to the caller that owns the group
+   (the serving loop): FP64 basis and matrix, at most 128 columns, and no step
+   that reaches the full-basis case */
+static PetscBool GroupEligible(BV V, Mat A, PetscInt m)
+{
+  Mat_HIPCSR *h;
+  if (!V->grouped || V->prec != SLEPC_AMD_F64 || strcmp(A->type, MATHIPCSR)) return PETSC_FALSE;
+  h = (Mat_HIPCSR *)A->data;
+  if (!h || !h->csr || h->vprec != SLEPC_AMD_F64) return PETSC_FALSE;
+  return (V->nc + m <= 128 && m <= V->N - 1) ? PETSC_TRUE : PETSC_FALSE;
+}
+
 static PetscErrorCode BVMatKrylovBegin_HIPKS(BV V, Mat A, PetscInt k, PetscInt m)
 {
   PetscBool gate = !strcmp(A->type, MATHIPCSR);
   Vec vj = NULL, vj1 = NULL;
   PetscErrorCode e = PETSC_SUCCESS;
   KSCall(ks_status_reset(V->ctx));
+  if (k < m && GroupEligible(V, A, m)) {
+    V->kry_csr = ((Mat_HIPCSR *)A->data)->csr;
+    V->kry_k = k;
+    V->kry_j = k;
+    V->kry_m = m;
+    V->kry_pending = PETSC_TRUE;
+    return PETSC_SUCCESS;
+  }
   if (gate) A->gate = PETSC_TRUE;
   for (PetscInt j = k; j < m && !e; j++) {
     PetscCall(BVGetColumn(V, j, &vj));
@@ -181,10 +201,38 @@ static PetscErrorCode BVMatKrylovBegin_HIPKS(BV V, Mat A, PetscInt k, PetscInt m
   return e;
 }
 
+/* the next step of a grouped expansion: column nc+j+1 = A column nc+j, then its
+   CGS+DGKS against the nc+j+1 columns before it and the scaling */
+PetscErrorCode BVHIPKSGroupMember(BV V, ks_group_member *mem)
+{
+  BV_HIPKS *h = (BV_HIPKS *)V->data;
+  PetscCheck(V->kry_pending, PETSC_ERR_ORDER, "no grouped expansion pending");
+  mem->ctx = V->ctx;
+  mem->A = V->kry_csr;
+  mem->V = h->dV;
+  mem->ld = V->ld;
+  mem->n = V->n;
+  mem->k = V->nc + V->kry_j + 1;
+  mem->buf = h->d_buf;
+  mem->ldbuf = h->ncols;
+  mem->hcol = V->kry_j + 1;
+  mem->col = V->kry_j + 1;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode BVHIPKSGroupAdvance(BV V, PetscBool *finished)
+{
+  V->kry_j++;
+  *finished = V->kry_j >= V->kry_m ? PETSC_TRUE : PETSC_FALSE;
+  if (*finished) V->kry_pending = PETSC_FALSE;
+  return PETSC_SUCCESS;
+}
+
 static PetscErrorCode BVMatKrylovEnd_HIPKS(BV V, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown)
 {
   BV_HIPKS *h = (BV_HIPKS *)V->data;
   ks_status st;
+  PetscCheck(!V->kry_pending, PETSC_ERR_ORDER, "grouped expansion not fully enqueued");
   *m = V->kry_m;
   KSCall(ks_status_get(V->ctx, &st));
   PetscCall(StatusCheck(&st));

@@ -306,7 +306,10 @@ PetscErrorCode EPSSolveReady(EPS eps, PetscBool *ready)
 {
   int idle = 1;
   PetscCheck(eps && ready, PETSC_ERR_ARG_NULL, "null argument");
-  if (eps->stepping) KSCall(ks_ctx_query(eps->V->ctx, &idle));
+  if (eps->stepping) {
+    if (eps->V->kry_pending) idle = 0;   /* grouped expansion still being enqueued by its owner */
+    else KSCall(ks_ctx_query(eps->V->ctx, &idle));
+  }
   *ready = idle ? PETSC_TRUE : PETSC_FALSE;
   return PETSC_SUCCESS;
 }

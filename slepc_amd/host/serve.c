@@ -18,11 +18,18 @@
  */
 #include "slepcimpl.h"
 
+#define KS_GROUP_MAX_MEMBERS 16   /* members of one grouped step (the kernel library's limit) */
+/* a kernel-library call inside SlepcServerProgress: on failure record it and leave by `out` */
+#define KSGO(x) do { int _k = (x); if (_k) { \
+      ierr = SlepcError(__LINE__, __func__, __FILE__, PETSC_ERR_GPU, "device library: %s (%d) in %s", \
+                        ks_error_string(_k), _k, #x); goto out; } } while (0)
+
 typedef enum { REQ_QUEUED = 0, REQ_ACTIVE = 1, REQ_DONE = 2, REQ_FAILED = 3 } ReqState;
 
 typedef struct {
   SlepcRequest in;
   ReqState state;
+  PetscBool joined;           /* its pending grouped expansion waits on its stream already */
   size_t bytes;               /* reserved HBM */
   int slot;
   Mat A;
@@ -39,6 +46,10 @@ struct _p_SlepcServer {
   ks_ctx *slot_ctx;
   PetscInt *slot_req;         /* request id per slot, -1 free */
   ks_pool pool;
+  ks_group group;             /* grouped Arnoldi steps of all expanding requests */
+  PetscBool grouped;
+  PetscInt group_depth;       /* grouped steps allowed in flight */
+  int64_t group_steps, group_members;
   Request *req;
   PetscInt nreq, capreq, next_queued;   /* FIFO admission cursor */
   PetscInt nfinished;
@@ -78,6 +89,9 @@ PetscErrorCode SlepcServerCreate(size_t hbm_budget, PetscInt max_active, SlepcSe
   /* the arena holds the budget plus per-slot pinned staging of the restart
      matrices; 1/16 extra absorbs fragmentation */
   KSCall(ks_pool_create(&s->pool, hbm_budget + hbm_budget / 16, (size_t)max_active * 4 * 1024 * 1024));
+  KSCall(ks_group_create(&s->group));
+  s->grouped = PETSC_TRUE;
+  s->group_depth = 16;
   *out = s;
   return PETSC_SUCCESS;
 }
@@ -129,6 +143,7 @@ static PetscErrorCode RequestAdmit(SlepcServer s, PetscInt id, int slot)
   PetscCall(EPSSetType(q->eps, EPSKRYLOVSCHUR));
   PetscCall(EPSGetBV(q->eps, &V));
   PetscCall(BVSetType(V, BVHIPKS));
+  V->grouped = s->grouped;
   if (q->in.v0) {
     PetscCall(VecCreateHIP(q->in.n, SLEPC_AMD_F64, &v0));
     PetscCall(VecSetValuesHost(v0, q->in.v0));
@@ -190,6 +205,57 @@ static PetscErrorCode RequestRetire(SlepcServer s, PetscInt id)
   return PETSC_SUCCESS;
 }
 
+/* Enqueue grouped Arnoldi steps -- one launch per kernel kind for the current
+   column of every request whose expansion is pending -- until group_depth
+   steps are in flight.  Called between the host's restart work so the device
+   keeps running while the projected problems are solved. */
+static PetscErrorCode GroupFeed(SlepcServer s, PetscBool *moved)
+{
+  PetscErrorCode ierr = PETSC_SUCCESS;
+  if (!s->grouped) return PETSC_SUCCESS;
+  for (;;) {
+    ks_group_member mem[KS_GROUP_MAX_MEMBERS];
+    PetscInt who[KS_GROUP_MAX_MEMBERS], nm = 0;
+    int inflight = 0, refine = -1;
+    double eta = 0.0;
+    KSGO(ks_group_inflight(s->group, &inflight));
+    if (inflight >= s->group_depth) break;
+    for (int i = 0; i < s->max_active && nm < KS_GROUP_MAX_MEMBERS; i++) {
+      PetscInt id = s->slot_req[i];
+      BV V;
+      if (id < 0) continue;
+      V = s->req[id].eps->V;
+      if (!V->kry_pending) continue;
+      if (refine < 0) { refine = (int)V->orthog_ref; eta = V->orthog_eta; }
+      if ((int)V->orthog_ref != refine || V->orthog_eta != eta) continue;   /* next round */
+      if (!s->req[id].joined) {
+        KSGO(ks_group_wait_ctx(s->group, s->slot_ctx[i]));   /* after its restart work */
+        s->req[id].joined = PETSC_TRUE;
+      }
+      ierr = BVHIPKSGroupMember(V, &mem[nm]);
+      if (ierr) goto out;
+      who[nm++] = id;
+    }
+    if (!nm) break;
+    KSGO(ks_group_step(s->group, (int)nm, mem, (ks_refine)refine, eta));
+    s->group_steps++;
+    s->group_members += nm;
+    *moved = PETSC_TRUE;
+    for (PetscInt t = 0; t < nm; t++) {
+      Request *q = &s->req[who[t]];
+      PetscBool fin;
+      ierr = BVHIPKSGroupAdvance(q->eps->V, &fin);
+      if (ierr) goto out;
+      if (fin) {   /* its stream picks up after the group's last step */
+        KSGO(ks_ctx_wait_group(s->slot_ctx[q->slot], s->group));
+        q->joined = PETSC_FALSE;
+      }
+    }
+  }
+out:
+  return ierr;
+}
+
 /* One pass of the loop, never waiting on the device: admit queued requests in
    arrival order while a slot and the budget allow, then advance every active
    solve whose device work has finished by one restart, retiring the finished
@@ -221,6 +287,8 @@ PetscErrorCode SlepcServerProgress(SlepcServer s, PetscBool *progressed, PetscIn
     s->next_queued++;
     moved = PETSC_TRUE;
   }
+  ierr = GroupFeed(s, &moved);
+  if (ierr) goto out;
   for (int i = 0; i < s->max_active; i++) {
     PetscInt id = s->slot_req[i];
     PetscBool ready, done;
@@ -232,6 +300,9 @@ PetscErrorCode SlepcServerProgress(SlepcServer s, PetscBool *progressed, PetscIn
     ierr = EPSSolveStep(s->req[id].eps, &done);
     if (ierr) goto out;
     moved = PETSC_TRUE;
+    /* keep the device fed while the host works through the restarts */
+    ierr = GroupFeed(s, &moved);
+    if (ierr) goto out;
     if (done) {
       ierr = RequestRetire(s, id);
       if (ierr) goto out;
@@ -264,6 +335,22 @@ PetscErrorCode SlepcServerGetResult(SlepcServer s, PetscInt id, PetscBool *done,
     PetscCheck(q->X, PETSC_ERR_ARG_WRONG, "request %d was submitted without want_vectors", (int)id);
     memcpy(X, q->X, sizeof(PetscScalar) * (size_t)q->in.n * (size_t)q->res.nret);
   }
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode SlepcServerSetGrouped(SlepcServer s, PetscBool grouped)
+{
+  PetscCheck(s, PETSC_ERR_ARG_NULL, "null server");
+  PetscCheck(s->nreq == 0, PETSC_ERR_ORDER, "set the grouping before the first request");
+  s->grouped = grouped;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode SlepcServerGetGroupStats(SlepcServer s, int64_t *steps, int64_t *member_steps)
+{
+  PetscCheck(s, PETSC_ERR_ARG_NULL, "null server");
+  if (steps) *steps = s->group_steps;
+  if (member_steps) *member_steps = s->group_members;
   return PETSC_SUCCESS;
 }
 
@@ -303,6 +390,7 @@ PetscErrorCode SlepcServerDestroy(SlepcServer *ps)
     free(s->req[i].X);
   }
   free(s->req);
+  KSCall(ks_group_destroy(s->group));
   KSCall(ks_pool_destroy(s->pool));
   for (int i = 0; i < s->max_active; i++) KSCall(ks_ctx_destroy(s->slot_ctx[i]));
   free(s->slot_ctx);

@@ -149,12 +149,19 @@ struct _p_BV {
   PetscInt kry_m;
   PetscReal kry_beta;
   PetscBool kry_breakdown;
+  /* grouped expansions (serving): the steps k..m-1 are enqueued by the server,
+     step kry_j next, while kry_pending */
+  PetscBool grouped, kry_pending;
+  PetscInt kry_k, kry_j;
+  ks_csr kry_csr;
 };
 PetscErrorCode BVOrthogonalizeColumn_Safe(BV bv, PetscInt j, PetscReal *norm, PetscBool *lindep);
 PetscErrorCode BVMatKrylovBegin(BV V, Mat A, PetscInt k, PetscInt m);
 PetscErrorCode BVMatKrylovEnd(BV V, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown);
 PetscErrorCode BVCopyHessenberg(BV V, Mat H, PetscInt k, PetscInt m);
 PetscErrorCode BVCopyTridiagonal(BV V, Mat T, PetscInt k, PetscInt m);
+PetscErrorCode BVHIPKSGroupMember(BV V, ks_group_member *mem);
+PetscErrorCode BVHIPKSGroupAdvance(BV V, PetscBool *finished);
 
 /* ---------------------------------------------------------------- DS */
 typedef struct {

@@ -40,11 +40,12 @@ class Server:
     """SlepcServer: ``budget`` bytes of HBM for resident requests, at most
     ``max_active`` solves on the device at once."""
 
-    def __init__(self, budget: int, max_active: int = 8):
+    def __init__(self, budget: int, max_active: int = 8, grouped: bool = True):
         _eps.initialize()
         self.h = C.c_void_p()
         _eps._chk(_eps.lib().SlepcServerCreate(C.c_size_t(int(budget)), C.c_int(max_active), C.byref(self.h)),
                   "SlepcServerCreate")
+        _eps._chk(_eps.lib().SlepcServerSetGrouped(self.h, C.c_int(int(grouped))), "SlepcServerSetGrouped")
         self._keep = {}
         self._meta = {}
 
@@ -97,7 +98,10 @@ class Server:
     def stats(self) -> dict:
         a, b, c = C.c_size_t(), C.c_size_t(), C.c_size_t()
         _eps._chk(_eps.lib().SlepcServerGetStats(self.h, C.byref(a), C.byref(b), C.byref(c)), "SlepcServerGetStats")
-        return dict(peak_reserved=a.value, pool_peak=b.value, budget=c.value)
+        gs, gm = C.c_int64(), C.c_int64()
+        _eps._chk(_eps.lib().SlepcServerGetGroupStats(self.h, C.byref(gs), C.byref(gm)), "SlepcServerGetGroupStats")
+        return dict(peak_reserved=a.value, pool_peak=b.value, budget=c.value, group_steps=gs.value,
+                    group_member_steps=gm.value)
 
     def close(self):
         if self.h:
@@ -129,6 +133,7 @@ def run_stream(server: Server, requests, make_csr, tol: float = 1e-10, want_vect
     remaining = len(requests)
     results = [None] * len(requests)
     pending = set()
+    seen_finished = 0
     while remaining:
         now = clock()
         while nxt < len(requests) and t0 + time_scale * requests[nxt]["t_arrival"] <= now:
@@ -140,12 +145,14 @@ def run_stream(server: Server, requests, make_csr, tol: float = 1e-10, want_vect
             mats[nxt] = None
             nxt += 1
         moved, na, nq, nf = server.progress()
-        for i in list(pending):
-            res = server.result(ids[i])
-            if res is not None:
-                results[i] = res
-                pending.discard(i)
-                remaining -= 1
+        if nf > seen_finished:   # collect only when something finished
+            seen_finished = nf
+            for i in list(pending):
+                res = server.result(ids[i])
+                if res is not None:
+                    results[i] = res
+                    pending.discard(i)
+                    remaining -= 1
         if not moved and idle_sleep > 0:
             time.sleep(idle_sleep)
     return results, clock() - t0

@@ -38,14 +38,21 @@ def _max_angle_sin(X, Y):
     return np.sqrt(max(0.0, 1.0 - s.min() ** 2))
 
 
-def test_serving_alone_vs_mix(E):
+@pytest.mark.parametrize("grouped", [True, False])
+def test_serving_alone_vs_mix(E, grouped):
+    # grouped: the Arnoldi steps of all expanding requests share launches;
+    # otherwise each request launches its own steps on its own stream
     from slepc_amd import serve as S
     reqs = _stream(10, seed=4, nmax=20000)
-    srv = S.Server(budget=2 << 30, max_active=4)
+    srv = S.Server(budget=2 << 30, max_active=4, grouped=grouped)
     mix, wall = S.run_stream(srv, reqs, _csr, tol=1e-10, want_vectors=True)
     st = srv.stats()
     srv.close()
     assert st["peak_reserved"] <= st["budget"]
+    if grouped:
+        assert st["group_steps"] > 0 and st["group_member_steps"] > st["group_steps"]   # steps were shared
+    else:
+        assert st["group_steps"] == 0
     for r, res in zip(reqs, mix):
         print(f"n={r['n']} nev={r['nev']} its={res['its']} ngood={res['ngood']} lat={1e3 * res['latency']:.1f}ms",
               flush=True)
