@@ -74,9 +74,23 @@ def cpu_baseline(csr, n, nev, ncv, tol, gpu_its, budget_s):
             "seconds_timed": t, "projected_solve_s": full}
 
 
+def _measured_traffic(kernel, grid):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    PMC profile of this workload (FETCH_SIZE / WRITE_SIZE, calibrated on a
+    streaming kernel of known size), or None when there is none for it."""
+    path = os.path.join(ROOT, "profiles", "ks_fp64_varcoef128_hbm_traffic.json")
+    if grid != 128 or not os.path.exists(path):
+        return None
+    k = {"gs_update_first": "gs_fused_sweep", "gs_dot_first": "gs_first_sweep", "spmv": "spmv",
+         "multinplace": "multinplace"}.get(kernel)
+    with open(path) as f:
+        prof = json.load(f)
+    return prof["kernels"][k]["traffic_bytes_per_launch"] if k in prof["kernels"] else None
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["fp64", "mixed", "serve"], default="fp64")
+    ap.add_argument("--workload", choices=["fp64", "mixed", "serve", "copycal"], default="fp64")
     ap.add_argument("--requests", type=int, default=400, help="serve: number of requests")
     ap.add_argument("--qps", type=float, default=200.0, help="serve: Poisson arrival rate")
     ap.add_argument("--slots", type=int, default=8, help="serve: concurrent solves")
@@ -104,6 +118,8 @@ def main():
         return run_mixed(args)
     if args.workload == "serve":
         return run_serve(args)
+    if args.workload == "copycal":
+        return run_copycal(args)
 
     from slepc_amd import eps as E
     from slepc_amd import ks as K
@@ -193,7 +209,7 @@ def main():
         dom = max(cands, key=lambda k: kernels[k]["total_ms"])
         dk = kernels[dom]
         roofline = {"bound": "hbm", "achieved": round(dk["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(dk["GBs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                    "frac": round(dk["GBs"] / HBM_PEAK_GBS, 4), "traffic": _measured_traffic(dom, N), "kernel": dom,
                     "avg_us": round(dk["avg_us"], 2), "bytes_per_launch": dk["bytes_per_launch"]}
         tot_b = sum(e["bytes"] for e in prof.values())
         tot_ms = sum(e["ms"] for e in prof.values())
@@ -338,6 +354,29 @@ def run_mixed(args):
     print(json.dumps(out), flush=True)
 
 
+def run_copycal(args):
+    """A streaming kernel of known traffic (k_scale: read and write 64 Mi doubles)
+    -- the calibration for the hardware counters FETCH_SIZE / WRITE_SIZE."""
+    from slepc_amd import ks as K
+    ctx = K.Context(0)
+    n = 64 << 20
+    x = K.DeviceArray(ctx, (n,))
+    K.random(ctx, K.F64, n, x, 1)
+    ctx.sync()
+    ctx.prof(True)
+    ctx.prof_reset()
+    for _ in range(max(1, args.steps)):
+        K.scale(ctx, K.F64, n, x, 1.0000001)
+    ctx.sync()
+    e = ctx.prof_get()["scale"]
+    ctx.prof(False)
+    us = 1e3 * e["ms"] / e["launches"]
+    out = {"workload": "copycal", "kernel": "k_scale<double>", "n": n, "bytes_read_per_launch": 8 * n,
+           "bytes_written_per_launch": 8 * n, "launches": e["launches"], "avg_us": round(us, 2),
+           "GBs": round(16.0 * n / (us * 1e-6) / 1e9, 1)}
+    print(json.dumps(out), flush=True)
+
+
 def _er_job(r):
     from slepc_amd import matgen as M
     return M.er_laplacian(r["n"], 8.0, r["graph_seed"])
@@ -398,6 +437,8 @@ def run_serve(args):
                  "served": {"eigenpairs_per_s": round(rate, 4), "wall_s": round(wall, 3), "verified_pairs": good,
                             "requests": len(reqs), "peak_reserved_bytes": stats["peak_reserved"],
                             "grouped_steps": stats["group_steps"],
+                            "host_ds_s": round(sum(r["t_host_ds"] for r in results), 3),
+                            "restarts": int(sum(r["its"] for r in results)),
                             "members_per_grouped_step": round(stats["group_member_steps"] / max(1, stats["group_steps"]), 2)},
                  "served_streams_only": {"eigenpairs_per_s": round(good_ng / wall_ng, 4), "wall_s": round(wall_ng, 3),
                                          "verified_pairs": good_ng,

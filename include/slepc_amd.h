@@ -306,6 +306,7 @@ typedef struct {
   EPSConvergedReason reason;
   int64_t nmatmult, gs_passes;
   double t_arrival, t_admit, t_done;
+  double t_host_ds;              /* host seconds in the projected problems of this request */
 } SlepcResult;
 typedef struct _p_SlepcServer *SlepcServer;
 PetscErrorCode SlepcServerCreate(size_t hbm_budget, PetscInt max_active, SlepcServer *srv);

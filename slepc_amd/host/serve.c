@@ -91,7 +91,7 @@ PetscErrorCode SlepcServerCreate(size_t hbm_budget, PetscInt max_active, SlepcSe
   KSCall(ks_pool_create(&s->pool, hbm_budget + hbm_budget / 16, (size_t)max_active * 4 * 1024 * 1024));
   KSCall(ks_group_create(&s->group));
   s->grouped = PETSC_TRUE;
-  s->group_depth = 16;
+  s->group_depth = 4;
   *out = s;
   return PETSC_SUCCESS;
 }
@@ -169,6 +169,7 @@ static PetscErrorCode RequestRetire(SlepcServer s, PetscInt id)
   q->res.reason = q->eps->reason;
   q->res.nmatmult = q->eps->nmatmult;
   q->res.gs_passes = q->eps->V->gs_passes;
+  q->res.t_host_ds = q->eps->t_ds;
   nret = PetscMin(nev, q->eps->nconv);
   q->res.nret = nret;
   PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)nev, &q->eigr));
@@ -206,10 +207,10 @@ static PetscErrorCode RequestRetire(SlepcServer s, PetscInt id)
 }
 
 /* Enqueue grouped Arnoldi steps -- one launch per kernel kind for the current
-   column of every request whose expansion is pending -- until group_depth
-   steps are in flight.  Called between the host's restart work so the device
+   column of every request whose expansion is pending -- until `depth` steps
+   are in flight.  Called between the host's restart work so the device
    keeps running while the projected problems are solved. */
-static PetscErrorCode GroupFeed(SlepcServer s, PetscBool *moved)
+static PetscErrorCode GroupFeed(SlepcServer s, PetscBool *moved, PetscInt depth)
 {
   PetscErrorCode ierr = PETSC_SUCCESS;
   if (!s->grouped) return PETSC_SUCCESS;
@@ -219,7 +220,7 @@ static PetscErrorCode GroupFeed(SlepcServer s, PetscBool *moved)
     int inflight = 0, refine = -1;
     double eta = 0.0;
     KSGO(ks_group_inflight(s->group, &inflight));
-    if (inflight >= s->group_depth) break;
+    if (inflight >= depth) break;
     for (int i = 0; i < s->max_active && nm < KS_GROUP_MAX_MEMBERS; i++) {
       PetscInt id = s->slot_req[i];
       BV V;
@@ -287,7 +288,7 @@ PetscErrorCode SlepcServerProgress(SlepcServer s, PetscBool *progressed, PetscIn
     s->next_queued++;
     moved = PETSC_TRUE;
   }
-  ierr = GroupFeed(s, &moved);
+  ierr = GroupFeed(s, &moved, s->group_depth);
   if (ierr) goto out;
   for (int i = 0; i < s->max_active; i++) {
     PetscInt id = s->slot_req[i];
@@ -300,8 +301,9 @@ PetscErrorCode SlepcServerProgress(SlepcServer s, PetscBool *progressed, PetscIn
     ierr = EPSSolveStep(s->req[id].eps, &done);
     if (ierr) goto out;
     moved = PETSC_TRUE;
-    /* keep the device fed while the host works through the restarts */
-    ierr = GroupFeed(s, &moved);
+    /* keep the device fed while the host works through the restarts, but
+       only when the group has run dry: fuller steps cost the host less */
+    ierr = GroupFeed(s, &moved, 1);
     if (ierr) goto out;
     if (done) {
       ierr = RequestRetire(s, id);

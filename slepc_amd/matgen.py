@@ -237,7 +237,8 @@ def rmat_shifted_normalized_laplacian(scale: int, edge_factor: int = 16, seed: i
                                       abcd=(0.57, 0.19, 0.19, 0.05), dtype=np.float32):
     """M = 2I - (I - D^-1/2 A D^-1/2) = I + D^-1/2 A D^-1/2 of the R-MAT graph
     symmetrized (A = pattern of E + E^T), with self-loops and duplicate edges
-    dropped; isolated vertices keep M_ii = 1 (their Laplacian row is zero).
+    dropped; isolated vertices get M_ii = 1 (normalized Laplacian L_ii = 1 for
+    them, the convention with D^-1/2 = 0 on isolated vertices).
     Eigenvalues lie in [0, 2]; the largest of M are the smallest of the
     normalized Laplacian.  Values in ``dtype`` (float32 for the at-scale case)."""
     n = 1 << scale

@@ -27,7 +27,8 @@ class Request(C.Structure):
 class Result(C.Structure):
     _fields_ = [("nconv", C.c_int), ("nret", C.c_int), ("ngood", C.c_int), ("its", C.c_int),
                 ("reason", C.c_int), ("nmatmult", C.c_int64), ("gs_passes", C.c_int64),
-                ("t_arrival", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+                ("t_arrival", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("t_host_ds", C.c_double)]
 
 
 def clock() -> float:
@@ -89,7 +90,7 @@ class Server:
         k = res.nret
         out = dict(nconv=res.nconv, nret=k, ngood=res.ngood, its=res.its, reason=res.reason,
                    nmatmult=res.nmatmult, gs_passes=res.gs_passes, t_arrival=res.t_arrival,
-                   t_admit=res.t_admit, t_done=res.t_done, latency=res.t_done - res.t_arrival,
+                   t_admit=res.t_admit, t_done=res.t_done, latency=res.t_done - res.t_arrival, t_host_ds=res.t_host_ds,
                    eigr=eigr[:k], eigi=eigi[:k], err=err[:k])
         if want:
             out["X"] = X[:k].T.copy()
