@@ -438,6 +438,7 @@ def run_serve(args):
                             "requests": len(reqs), "peak_reserved_bytes": stats["peak_reserved"],
                             "grouped_steps": stats["group_steps"],
                             "host_ds_s": round(sum(r["t_host_ds"] for r in results), 3),
+                            "host_loop_s": {k: round(v, 3) for k, v in stats["host_s"].items()},
                             "restarts": int(sum(r["its"] for r in results)),
                             "members_per_grouped_step": round(stats["group_member_steps"] / max(1, stats["group_steps"]), 2)},
                  "served_streams_only": {"eigenpairs_per_s": round(good_ng / wall_ng, 4), "wall_s": round(wall_ng, 3),

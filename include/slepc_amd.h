@@ -292,13 +292,15 @@ void *SlepcAMDGetDefaultContext(void);
 typedef struct {
   PetscInt n;
   const PetscInt *rowptr, *colidx;
-  const PetscScalar *val;
+  const PetscScalar *val;        /* FP64 values, or NULL with val32 */
+  const float *val32;            /* FP32 values (the operator is then stored in FP32) */
   PetscInt nev, ncv, max_it;     /* max_it <= 0: the solver's default */
   PetscReal tol;
   EPSWhich which;                /* 0: largest magnitude */
   PetscBool hermitian;
   const PetscScalar *v0;         /* start vector (length n) or NULL */
   PetscBool want_vectors;        /* keep the eigenvectors on the host */
+  PetscReal mixed_tol;           /* > tol: FP32-basis first phase to this tolerance */
   double t_arrival;              /* on the SlepcServerClock clock; <= 0: submission time */
 } SlepcRequest;
 typedef struct {
@@ -320,6 +322,8 @@ PetscErrorCode SlepcServerGetStats(SlepcServer srv, size_t *peak_reserved, size_
    launch per kernel kind (ks_group_step); off: each request launches its own */
 PetscErrorCode SlepcServerSetGrouped(SlepcServer srv, PetscBool grouped);
 PetscErrorCode SlepcServerGetGroupStats(SlepcServer srv, int64_t *steps, int64_t *member_steps);
+/* host seconds spent admitting, issuing grouped steps, restarting, retiring, polling */
+PetscErrorCode SlepcServerGetHostTimes(SlepcServer srv, double t[5]);
 PetscErrorCode SlepcServerDestroy(SlepcServer *srv);
 double SlepcServerClock(void);   /* monotonic seconds, the clock of t_arrival/t_admit/t_done */
 

@@ -333,8 +333,9 @@ class EPS:
 
 def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=True, v0=None, max_it=-1,
           prec=F64, deflation=None, lock=True, keep=0.5, want_vectors=True, mixed_tol=0.0):
-    """One-call convenience wrapper: returns a dict like the oracle's eps_solve."""
-    A = Mat(csr)
+    """One-call convenience wrapper: returns a dict like the oracle's eps_solve.
+    FP32 values in csr[2] keep the operator in FP32; ``prec`` is the basis precision."""
+    A = Mat(csr, prec=F32 if np.asarray(csr[2]).dtype == np.float32 else F64)
     eps = EPS()
     eps.setOperators(A)
     eps.setProblemType(EPS.HEP if hermitian else EPS.NHEP)

@@ -287,6 +287,7 @@ PetscErrorCode EPSSolveBegin(EPS eps)
   eps->t_ds = 0.0;
   eps->V->gs_passes = 0;
   eps->stepping = PETSC_FALSE;
+  eps->stepV = eps->V;
   for (i = 0; i < eps->ncv; i++) {
     eps->eigr[i] = 0.0;
     eps->eigi[i] = 0.0;
@@ -307,8 +308,9 @@ PetscErrorCode EPSSolveReady(EPS eps, PetscBool *ready)
   int idle = 1;
   PetscCheck(eps && ready, PETSC_ERR_ARG_NULL, "null argument");
   if (eps->stepping) {
-    if (eps->V->kry_pending) idle = 0;   /* grouped expansion still being enqueued by its owner */
-    else KSCall(ks_ctx_query(eps->V->ctx, &idle));
+    BV V = eps->stepV ? eps->stepV : eps->V;
+    if (V->kry_pending) idle = 0;   /* grouped expansion still being enqueued by its owner */
+    else KSCall(ks_ctx_query(V->ctx, &idle));
   }
   *ready = idle ? PETSC_TRUE : PETSC_FALSE;
   return PETSC_SUCCESS;

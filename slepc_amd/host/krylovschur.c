@@ -196,16 +196,17 @@ static PetscErrorCode KSStart(EPS eps)
    FP64 DGKS scalars and an FP64 projected problem) down to ctx->mixed_tol.  Its
    converged Ritz vectors are summed into the FP64 start vector of the main
    solve (column 0 of V, used as the initial space), so the FP64 restarts start
-   from a vector whose unwanted components are already at the FP32 tolerance. */
-static PetscErrorCode EPSKrylovSchurMixedPhase(EPS eps)
+   from a vector whose unwanted components are already at the FP32 tolerance.
+   The phase is itself stepped (MixedBegin, then EPSSolveStep of the FP32 solve
+   until it is done, then MixedEnd), so a serving loop never blocks on it. */
+static PetscErrorCode MixedBegin(EPS eps)
 {
   EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data, *c32;
   EPS e32;
   BV bv32;
-  Vec v64 = NULL, acc = NULL, xr = NULL, xi = NULL, *defl = NULL;
-  PetscInt nconv32, nuse;
-  double t0 = SlepcWallTime();
+  Vec v64 = NULL, *defl = NULL;
 
+  ctx->t32_start = SlepcWallTime();
   PetscCall(EPSCreate(&e32));
   PetscCall(EPSSetOperators(e32, eps->A, NULL));
   PetscCall(EPSSetProblemType(e32, eps->problem_type));
@@ -241,8 +242,21 @@ static PetscErrorCode EPSKrylovSchurMixedPhase(EPS eps)
     for (PetscInt i = 0; i < eps->nds; i++) PetscCall(VecDestroy(&defl[i]));
     free(defl);
   }
-  PetscCall(EPSSolve(e32));
+  PetscCall(EPSSolveBegin(e32));
+  ctx->e32 = e32;
+  ctx->phase = KS_PHASE_MIXED32;
+  eps->stepV = e32->V;
+  return PETSC_SUCCESS;
+}
 
+static PetscErrorCode MixedEnd(EPS eps)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  EPS e32 = ctx->e32;
+  Vec acc = NULL, xr = NULL, xi = NULL;
+  PetscInt nconv32, nuse;
+
+  PetscCall(EPSSolveEnd(e32));
   nconv32 = e32->nconv;
   ctx->its32 = e32->its;
   ctx->nconv32 = nconv32;
@@ -265,8 +279,9 @@ static PetscErrorCode EPSKrylovSchurMixedPhase(EPS eps)
     PetscCall(VecDestroy(&xr));
     PetscCall(VecDestroy(&xi));
   }
-  PetscCall(EPSDestroy(&e32));
-  ctx->t32 = SlepcWallTime() - t0;
+  PetscCall(EPSDestroy(&ctx->e32));
+  ctx->t32 = SlepcWallTime() - ctx->t32_start;
+  eps->stepV = eps->V;
   return PETSC_SUCCESS;
 }
 
@@ -276,8 +291,7 @@ static PetscErrorCode EPSSolveBegin_KrylovSchur(EPS eps)
   ctx->its32 = ctx->nconv32 = 0;
   ctx->nmatmult32 = ctx->passes32 = 0;
   ctx->t32 = 0.0;
-  if (ctx->mixed_tol > 0.0 && ctx->mixed_tol > eps->tol && eps->V->prec == SLEPC_AMD_F64)
-    PetscCall(EPSKrylovSchurMixedPhase(eps));
+  if (ctx->mixed_tol > 0.0 && ctx->mixed_tol > eps->tol && eps->V->prec == SLEPC_AMD_F64) return MixedBegin(eps);
   return KSStart(eps);
 }
 
@@ -285,7 +299,14 @@ static PetscErrorCode EPSSolveBegin_KrylovSchur(EPS eps)
 static PetscErrorCode EPSSolveStep_KrylovSchur(EPS eps, PetscBool *done)
 {
   EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
-  if (ctx->phase == KS_PHASE_EXPANDING) PetscCall(KSAfterExpand(eps));
+  if (ctx->phase == KS_PHASE_MIXED32) {
+    PetscBool d32;
+    PetscCall(EPSSolveStep(ctx->e32, &d32));
+    if (d32) {
+      PetscCall(MixedEnd(eps));
+      PetscCall(KSStart(eps));
+    }
+  } else if (ctx->phase == KS_PHASE_EXPANDING) PetscCall(KSAfterExpand(eps));
   *done = ctx->phase == KS_PHASE_DONE;
   return PETSC_SUCCESS;
 }
@@ -353,6 +374,14 @@ PetscErrorCode EPSKrylovSchurSetLocking(EPS eps, PetscBool lock)
   return PETSC_SUCCESS;
 }
 
+/* releases a first-phase FP32 solve abandoned mid-way (the data itself is freed by EPSDestroy) */
+static PetscErrorCode EPSDestroy_KrylovSchur(EPS eps)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  if (ctx && ctx->e32) PetscCall(EPSDestroy(&ctx->e32));
+  return PETSC_SUCCESS;
+}
+
 /* EPSCreate_KrylovSchur (krylovschur.c:1579-1627) */
 PetscErrorCode EPSCreate_KrylovSchur(EPS eps)
 {
@@ -366,6 +395,6 @@ PetscErrorCode EPSCreate_KrylovSchur(EPS eps)
   eps->ops.solvebegin = EPSSolveBegin_KrylovSchur;
   eps->ops.solvestep = EPSSolveStep_KrylovSchur;
   eps->ops.computevectors = EPSComputeVectors_KrylovSchur;
-  eps->ops.destroy = NULL;
+  eps->ops.destroy = EPSDestroy_KrylovSchur;
   return PETSC_SUCCESS;
 }

@@ -50,6 +50,7 @@ struct _p_SlepcServer {
   PetscBool grouped;
   PetscInt group_depth;       /* grouped steps allowed in flight */
   int64_t group_steps, group_members;
+  double t_admit, t_feed, t_step, t_retire, t_query;   /* host seconds in each part of the loop */
   Request *req;
   PetscInt nreq, capreq, next_queued;   /* FIFO admission cursor */
   PetscInt nfinished;
@@ -63,11 +64,16 @@ static size_t RequestBytes(const SlepcRequest *r)
 {
   const size_t al = 256;
   size_t n = (size_t)r->n, nnz = (size_t)r->rowptr[r->n], ld = (n + 63) / 64 * 64;
-  size_t m = (size_t)r->ncv + 1, b = 0;
+  size_t m = (size_t)r->ncv + 1, b = 0, vs = r->val32 ? 4 : 8;
+  int mixed = r->mixed_tol > 0.0 && r->mixed_tol > r->tol;
 #define ROUND(x) (((x) + al - 1) / al * al)
-  b += ROUND(4 * (n + 1)) + ROUND(4 * (nnz ? nnz : 1)) + ROUND(8 * (nnz ? nnz : 1)) + ROUND(4 * (n + 2));
+  b += ROUND(4 * (n + 1)) + ROUND(4 * (nnz ? nnz : 1)) + ROUND(vs * (nnz ? nnz : 1)) + ROUND(4 * (n + 2));
   b += ROUND(8 * ld * m) + ROUND(8 * m * m) + ROUND(8 * (m + 2)) + ROUND(8 * m * m);
   b += 7 * ROUND(8 * n);
+  if (mixed) {   /* the FP32 phase: its basis, buffers and vectors, and an FP32 copy of FP64 values */
+    b += ROUND(4 * ld * m) + ROUND(8 * m * m) + ROUND(8 * (m + 2)) + ROUND(8 * m * m) + 7 * ROUND(4 * n);
+    if (!r->val32) b += ROUND(4 * (n + 1)) + ROUND(4 * (nnz ? nnz : 1)) * 2 + ROUND(4 * (n + 2));
+  }
 #undef ROUND
   return b + 64 * 1024;
 }
@@ -100,7 +106,7 @@ PetscErrorCode SlepcServerSubmit(SlepcServer s, const SlepcRequest *r, PetscInt 
 {
   Request *q;
   PetscCheck(s && r && id, PETSC_ERR_ARG_NULL, "null argument");
-  PetscCheck(r->n > 0 && r->rowptr && r->colidx && r->val, PETSC_ERR_ARG_NULL, "request without a matrix");
+  PetscCheck(r->n > 0 && r->rowptr && r->colidx && (r->val || r->val32), PETSC_ERR_ARG_NULL, "request without a matrix");
   PetscCheck(r->nev >= 1 && r->ncv > r->nev && r->ncv <= r->n && r->ncv + 1 <= KS_MAX_COLS, PETSC_ERR_ARG_OUTOFRANGE,
              "request nev=%d ncv=%d n=%d not supported", (int)r->nev, (int)r->ncv, (int)r->n);
   if (s->nreq == s->capreq) {
@@ -133,7 +139,12 @@ static PetscErrorCode RequestAdmit(SlepcServer s, PetscInt id, int slot)
   BV V;
   q->slot = slot;
   q->res.t_admit = SlepcWallTime();
-  PetscCall(MatCreateHIPCSR(q->in.n, q->in.rowptr, q->in.colidx, q->in.val, &q->A));
+  if (q->in.val32) {
+    PetscCall(MatCreate(&q->A));
+    PetscCall(MatSetSizes(q->A, q->in.n, q->in.n, q->in.n, q->in.n));
+    PetscCall(MatSetType(q->A, MATHIPCSR));
+    PetscCall(MatHIPCSRSetArraysF32(q->A, q->in.rowptr, q->in.colidx, q->in.val32));
+  } else PetscCall(MatCreateHIPCSR(q->in.n, q->in.rowptr, q->in.colidx, q->in.val, &q->A));
   PetscCall(EPSCreate(&q->eps));
   PetscCall(EPSSetOperators(q->eps, q->A, NULL));
   PetscCall(EPSSetProblemType(q->eps, q->in.hermitian ? EPS_HEP : EPS_NHEP));
@@ -141,6 +152,7 @@ static PetscErrorCode RequestAdmit(SlepcServer s, PetscInt id, int slot)
   PetscCall(EPSSetDimensions(q->eps, q->in.nev, q->in.ncv, PETSC_DETERMINE));
   PetscCall(EPSSetTolerances(q->eps, q->in.tol > 0 ? q->in.tol : 1e-10, q->in.max_it > 0 ? q->in.max_it : PETSC_DETERMINE));
   PetscCall(EPSSetType(q->eps, EPSKRYLOVSCHUR));
+  if (q->in.mixed_tol > 0.0) PetscCall(EPSKrylovSchurSetMixedPrecision(q->eps, q->in.mixed_tol));
   PetscCall(EPSGetBV(q->eps, &V));
   PetscCall(BVSetType(V, BVHIPKS));
   V->grouped = s->grouped;
@@ -202,6 +214,7 @@ static PetscErrorCode RequestRetire(SlepcServer s, PetscInt id)
   /* the matrix belongs to the caller again */
   q->in.rowptr = q->in.colidx = NULL;
   q->in.val = NULL;
+  q->in.val32 = NULL;
   q->in.v0 = NULL;
   return PETSC_SUCCESS;
 }
@@ -268,6 +281,7 @@ PetscErrorCode SlepcServerProgress(SlepcServer s, PetscBool *progressed, PetscIn
   ks_pool prev_pool;
   PetscErrorCode ierr = PETSC_SUCCESS;
   PetscBool moved = PETSC_FALSE;
+  double t0;
   PetscCheck(s, PETSC_ERR_ARG_NULL, "null server");
   prev_pool = ks_pool_use(s->pool);
   prev_ctx = SlepcAMDSetCurrentContext(NULL);
@@ -283,30 +297,42 @@ PetscErrorCode SlepcServerProgress(SlepcServer s, PetscBool *progressed, PetscIn
     s->reserved += q->bytes;
     if (s->reserved > s->peak_reserved) s->peak_reserved = s->reserved;
     (void)SlepcAMDSetCurrentContext(s->slot_ctx[slot]);
+    t0 = SlepcWallTime();
     ierr = RequestAdmit(s, s->next_queued, slot);
+    s->t_admit += SlepcWallTime() - t0;
     if (ierr) goto out;
     s->next_queued++;
     moved = PETSC_TRUE;
   }
+  t0 = SlepcWallTime();
   ierr = GroupFeed(s, &moved, s->group_depth);
+  s->t_feed += SlepcWallTime() - t0;
   if (ierr) goto out;
   for (int i = 0; i < s->max_active; i++) {
     PetscInt id = s->slot_req[i];
     PetscBool ready, done;
     if (id < 0) continue;
     (void)SlepcAMDSetCurrentContext(s->slot_ctx[i]);
+    t0 = SlepcWallTime();
     ierr = EPSSolveReady(s->req[id].eps, &ready);
+    s->t_query += SlepcWallTime() - t0;
     if (ierr) goto out;
     if (!ready) continue;
+    t0 = SlepcWallTime();
     ierr = EPSSolveStep(s->req[id].eps, &done);
+    s->t_step += SlepcWallTime() - t0;
     if (ierr) goto out;
     moved = PETSC_TRUE;
     /* keep the device fed while the host works through the restarts, but
        only when the group has run dry: fuller steps cost the host less */
+    t0 = SlepcWallTime();
     ierr = GroupFeed(s, &moved, 1);
+    s->t_feed += SlepcWallTime() - t0;
     if (ierr) goto out;
     if (done) {
+      t0 = SlepcWallTime();
       ierr = RequestRetire(s, id);
+      s->t_retire += SlepcWallTime() - t0;
       if (ierr) goto out;
     }
   }
@@ -353,6 +379,17 @@ PetscErrorCode SlepcServerGetGroupStats(SlepcServer s, int64_t *steps, int64_t *
   PetscCheck(s, PETSC_ERR_ARG_NULL, "null server");
   if (steps) *steps = s->group_steps;
   if (member_steps) *member_steps = s->group_members;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode SlepcServerGetHostTimes(SlepcServer s, double t[5])
+{
+  PetscCheck(s && t, PETSC_ERR_ARG_NULL, "null argument");
+  t[0] = s->t_admit;
+  t[1] = s->t_feed;
+  t[2] = s->t_step;
+  t[3] = s->t_retire;
+  t[4] = s->t_query;
   return PETSC_SUCCESS;
 }
 

@@ -234,9 +234,10 @@ struct _p_EPS {
   PetscReal t_ds, t_solve;
   double t_begin;
   PetscBool stepping;        /* between EPSSolveBegin and the last EPSSolveStep */
+  BV stepV;                  /* the basis whose device work the next step collects */
 };
 
-typedef enum { KS_PHASE_IDLE = 0, KS_PHASE_EXPANDING = 1, KS_PHASE_DONE = 2 } KSPhase;
+typedef enum { KS_PHASE_IDLE = 0, KS_PHASE_EXPANDING = 1, KS_PHASE_DONE = 2, KS_PHASE_MIXED32 = 3 } KSPhase;
 
 typedef struct {            /* EPS_KRYLOVSCHUR (krylovschur.h:76-105), default variant fields */
   PetscReal keep;
@@ -249,6 +250,8 @@ typedef struct {            /* EPS_KRYLOVSCHUR (krylovschur.h:76-105), default v
   PetscInt  its32, nconv32;
   int64_t   nmatmult32, passes32;
   PetscReal t32;
+  EPS       e32;             /* the FP32 solve while the first phase runs */
+  double    t32_start;
 } EPS_KRYLOVSCHUR;
 
 PetscErrorCode EPSKrylovConvergence(EPS eps, PetscBool getall, PetscInt kini, PetscInt nits, PetscReal beta,

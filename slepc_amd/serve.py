@@ -19,9 +19,10 @@ _P = C.POINTER
 
 class Request(C.Structure):
     _fields_ = [("n", C.c_int), ("rowptr", C.c_void_p), ("colidx", C.c_void_p), ("val", C.c_void_p),
+                ("val32", C.c_void_p),
                 ("nev", C.c_int), ("ncv", C.c_int), ("max_it", C.c_int), ("tol", C.c_double),
                 ("which", C.c_int), ("hermitian", C.c_int), ("v0", C.c_void_p), ("want_vectors", C.c_int),
-                ("t_arrival", C.c_double)]
+                ("mixed_tol", C.c_double), ("t_arrival", C.c_double)]
 
 
 class Result(C.Structure):
@@ -51,16 +52,21 @@ class Server:
         self._meta = {}
 
     def submit(self, csr, nev: int, ncv: int, tol: float = 1e-10, which: int = 1, hermitian: bool = True,
-               v0=None, want_vectors: bool = False, t_arrival: float = 0.0, max_it: int = 0) -> int:
+               v0=None, want_vectors: bool = False, t_arrival: float = 0.0, max_it: int = 0,
+               mixed_tol: float = 0.0) -> int:
+        """FP32 values in csr[2] keep the operator in FP32; mixed_tol > tol adds the
+        FP32-basis first phase."""
         rowptr = np.ascontiguousarray(csr[0], dtype=np.int32)
         colidx = np.ascontiguousarray(csr[1], dtype=np.int32)
-        val = np.ascontiguousarray(csr[2], dtype=np.float64)
+        f32 = np.asarray(csr[2]).dtype == np.float32
+        val = np.ascontiguousarray(csr[2], dtype=np.float32 if f32 else np.float64)
         n = len(rowptr) - 1
         v0a = None if v0 is None else np.ascontiguousarray(v0, dtype=np.float64)
-        r = Request(n=n, rowptr=rowptr.ctypes.data, colidx=colidx.ctypes.data, val=val.ctypes.data, nev=nev,
+        r = Request(n=n, rowptr=rowptr.ctypes.data, colidx=colidx.ctypes.data,
+                    val=None if f32 else val.ctypes.data, val32=val.ctypes.data if f32 else None, nev=nev,
                     ncv=ncv, max_it=max_it, tol=tol, which=which, hermitian=int(hermitian),
                     v0=None if v0a is None else v0a.ctypes.data, want_vectors=int(want_vectors),
-                    t_arrival=t_arrival)
+                    mixed_tol=mixed_tol, t_arrival=t_arrival)
         rid = C.c_int()
         _eps._chk(_eps.lib().SlepcServerSubmit(self.h, C.byref(r), C.byref(rid)), "SlepcServerSubmit")
         self._keep[rid.value] = (rowptr, colidx, val, v0a)   # the C side reads them until the solve ends
@@ -101,8 +107,11 @@ class Server:
         _eps._chk(_eps.lib().SlepcServerGetStats(self.h, C.byref(a), C.byref(b), C.byref(c)), "SlepcServerGetStats")
         gs, gm = C.c_int64(), C.c_int64()
         _eps._chk(_eps.lib().SlepcServerGetGroupStats(self.h, C.byref(gs), C.byref(gm)), "SlepcServerGetGroupStats")
+        t = (C.c_double * 5)()
+        _eps._chk(_eps.lib().SlepcServerGetHostTimes(self.h, t), "SlepcServerGetHostTimes")
         return dict(peak_reserved=a.value, pool_peak=b.value, budget=c.value, group_steps=gs.value,
-                    group_member_steps=gm.value)
+                    group_member_steps=gm.value,
+                    host_s=dict(admit=t[0], grouped_steps=t[1], restarts=t[2], retire=t[3], poll=t[4]))
 
     def close(self):
         if self.h:
@@ -141,7 +150,8 @@ def run_stream(server: Server, requests, make_csr, tol: float = 1e-10, want_vect
             r = requests[nxt]
             csr, v0 = mats[nxt]
             ids[nxt] = server.submit(csr, r["nev"], r["ncv"], tol=tol, v0=v0, want_vectors=want_vectors,
-                                     t_arrival=t0 + time_scale * r["t_arrival"])
+                                     t_arrival=t0 + time_scale * r["t_arrival"], which=r.get("which", 1),
+                                     hermitian=r.get("hermitian", True), mixed_tol=r.get("mixed_tol", 0.0))
             pending.add(nxt)
             mats[nxt] = None
             nxt += 1

@@ -90,3 +90,34 @@ def test_serving_admission_under_tight_budget(E):
     srv.close()
     assert st["peak_reserved"] <= 96 << 20
     assert all(o["reason"] == 1 and o["ngood"] == r["nev"] for r, o in zip(reqs, res))
+
+
+def test_serving_mixed_fp32_operator_alongside_nonsymmetric(E):
+    # the at-scale serving case in small: a 2I - normalized-Laplacian of an R-MAT graph
+    # with FP32 values (largest, FP32 first phase to 1e-5, FP64 restarts to 1e-10) served
+    # next to nonsymmetric convection-diffusion requests (largest real part, same mixed
+    # form); each result agrees with the same request solved alone and with LAPACK
+    from slepc_amd import serve as S
+    rmat = M.rmat_shifted_normalized_laplacian(12, 16, seed=5)          # FP32 values
+    cds = [M.convection_diffusion_2d(N, 10.0, seed=2 + i) for i, N in enumerate((32, 40, 48))]
+    reqs = [dict(n=len(rmat[0]) - 1, nev=8, ncv=32, t_arrival=0.0, v0_seed=7, which=E.EPS.LARGEST_REAL,
+                 hermitian=True, mixed_tol=1e-5, _csr=rmat)]
+    for i, c in enumerate(cds):
+        reqs.append(dict(n=len(c[0]) - 1, nev=4, ncv=24, t_arrival=0.001 * (i + 1), v0_seed=8 + i,
+                         which=E.EPS.LARGEST_REAL, hermitian=False, mixed_tol=1e-5, _csr=c))
+    srv = S.Server(budget=1 << 30, max_active=4)
+    res, wall = S.run_stream(srv, reqs, lambda r: r["_csr"], tol=1e-10)
+    srv.close()
+    for r, out in zip(reqs, res):
+        assert out["reason"] == 1 and out["ngood"] == r["nev"], (r["n"], out)
+        alone = E.solve(r["_csr"], nev=r["nev"], ncv=r["ncv"], tol=1e-10, which=r["which"],
+                        hermitian=r["hermitian"], v0=M.uniform_vector(r["v0_seed"], r["n"]), mixed_tol=1e-5,
+                        want_vectors=False)
+        k = r["nev"]
+        a = np.asarray(alone["eigr"][:k]) + 1j * np.asarray(alone["eigi"][:k])
+        b = np.asarray(out["eigr"]) + 1j * np.asarray(out["eigi"])
+        assert np.allclose(np.sort_complex(a), np.sort_complex(b), rtol=1e-10, atol=0)
+        A = M.csr_to_dense(r["_csr"][0], r["_csr"][1], np.asarray(r["_csr"][2], dtype=np.float64))
+        ev = np.linalg.eigvalsh(A) if r["hermitian"] else np.linalg.eigvals(A)
+        ev = ev[np.argsort(-ev.real)][:k]
+        assert np.allclose(np.sort_complex(b), np.sort_complex(ev), rtol=1e-10, atol=0)
