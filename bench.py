@@ -47,10 +47,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(csr, n, nev, ncv, tol, gpu_its, budget_s):
+def cpu_baseline(csr, n, nev, ncv, tol, gpu_its, budget_s, restart_bytes=None):
     """The CPU restatement of the reference path (oracle/) on the host cores,
     timed on a bounded number of restarts of the same solve and scaled by the
-    restart count the full solve needs."""
+    restart count the full solve needs.  ``restart_bytes``: the algorithmic
+    bytes of one restart (the same formulas as the GPU kernels), for the host's
+    achieved GB/s."""
     from oracle import oracle as O
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     os.environ.setdefault("MKL_NUM_THREADS", str(cores))
@@ -71,7 +73,8 @@ def cpu_baseline(csr, n, nev, ncv, tol, gpu_its, budget_s):
             "sample": (f"first {r['its']} restarts of the same solve (same matrix, start vector, "
                        f"nev/ncv/tol) with the oracle's C restatement (OpenMP SpMV, MKL BLAS/LAPACK); "
                        f"{per_restart:.3f} s/restart x {gpu_its} restarts the full solve takes"),
-            "seconds_timed": t, "projected_solve_s": full}
+            "seconds_timed": t, "projected_solve_s": full,
+            "host_GBs": round(restart_bytes * r["its"] / t / 1e9, 2) if restart_bytes else None}
 
 
 def _measured_traffic(kernel, grid):
@@ -90,7 +93,10 @@ def _measured_traffic(kernel, grid):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["fp64", "mixed", "serve", "copycal"], default="fp64")
+    ap.add_argument("--workload", choices=["fp64", "mixed", "serve", "scale", "copycal"], default="fp64")
+    ap.add_argument("--rmat-scale", type=int, default=24, help="scale: R-MAT graph scale")
+    ap.add_argument("--cd-requests", type=int, default=8, help="scale: convection-diffusion requests served with it")
+    ap.add_argument("--no-sequential", action="store_true", help="scale: skip the one-after-another FP64 baseline")
     ap.add_argument("--requests", type=int, default=400, help="serve: number of requests")
     ap.add_argument("--qps", type=float, default=200.0, help="serve: Poisson arrival rate")
     ap.add_argument("--slots", type=int, default=8, help="serve: concurrent solves")
@@ -118,6 +124,8 @@ def main():
         return run_mixed(args)
     if args.workload == "serve":
         return run_serve(args)
+    if args.workload == "scale":
+        return run_scale(args)
     if args.workload == "copycal":
         return run_copycal(args)
 
@@ -158,9 +166,41 @@ def main():
             f"gs_passes={st['gs_passes']}")
 
     L = K.lib()
+
+    def collect():
+        K.check(L.ks_prof_collect(ctx), "ks_prof_collect")
+        out = {}
+        for i, name in enumerate(K.PROF_CLASSES):
+            e = K.ProfEntry()
+            K.check(L.ks_prof_get(ctx, i, C_byref(e)), "ks_prof_get")
+            out[name] = dict(launches=e.launches, skipped=e.skipped, ms=e.ms, bytes=e.bytes, flops=e.flops)
+        return out
+
+    # one solve with HIP events around every launch, outside the timed region:
+    # the per-kernel table.  Each event pair costs the stream a few microseconds,
+    # so the timed solves below carry events only around the dominant kernel.
+    kernels, prof_all, dom, t_prof = {}, {}, None, None
     if not args.no_prof:
         K.check(L.ks_prof_reset(ctx), "ks_prof_reset")
         K.check(L.ks_prof_enable(ctx, 1), "ks_prof_enable")
+        K.check(L.ks_ctx_sync(ctx), "sync")
+        t_prof = one_solve()
+        K.check(L.ks_ctx_sync(ctx), "sync")
+        prof_all = collect()
+        K.check(L.ks_prof_enable(ctx, 0), "ks_prof_enable")
+        for name, e in prof_all.items():
+            if e["launches"] and e["ms"] > 0:
+                kernels[name] = dict(launches=e["launches"], skipped=e["skipped"],
+                                     avg_us=1e3 * e["ms"] / e["launches"],
+                                     GBs=e["bytes"] / (e["ms"] * 1e-3) / 1e9,
+                                     TFLOPs=e["flops"] / (e["ms"] * 1e-3) / 1e12,
+                                     total_ms=e["ms"], bytes_per_launch=e["bytes"] / e["launches"])
+        # dominant kernel among those that run on every step (never gated off)
+        cands = [k for k in ("gs_update_first", "gs_dot_first", "spmv", "multinplace") if k in kernels]
+        dom = max(cands, key=lambda k: kernels[k]["total_ms"]) if cands else None
+        K.check(L.ks_prof_reset(ctx), "ks_prof_reset")
+        if dom is not None:
+            K.check(L.ks_prof_enable_classes(ctx, 1 << K.PROF_CLASSES.index(dom)), "ks_prof_enable_classes")
     K.check(L.ks_ctx_sync(ctx), "sync")
     times, its_list, passes = [], [], []
     t_begin = time.perf_counter()
@@ -174,14 +214,9 @@ def main():
             f"gs_passes={st['gs_passes']} t_ds={st['t_ds']:.3f}s")
     K.check(L.ks_ctx_sync(ctx), "sync")
     t_total = time.perf_counter() - t_begin
-
-    prof = {}
-    if not args.no_prof:
-        K.check(L.ks_prof_collect(ctx), "ks_prof_collect")
-        for i, name in enumerate(K.PROF_CLASSES):
-            e = K.ProfEntry()
-            K.check(L.ks_prof_get(ctx, i, C_byref(e)), "ks_prof_get")
-            prof[name] = dict(launches=e.launches, skipped=e.skipped, ms=e.ms, bytes=e.bytes, flops=e.flops)
+    prof_dom = None
+    if dom is not None:
+        prof_dom = collect()[dom]
         K.check(L.ks_prof_enable(ctx, 0), "ks_prof_enable")
 
     # verification outside the timed region (last solve)
@@ -195,26 +230,26 @@ def main():
     lat = np.array(times) * 1e3
 
     roofline = None
-    kernels = {}
-    if prof:
-        for name, e in prof.items():
-            if e["launches"] and e["ms"] > 0:
-                kernels[name] = dict(launches=e["launches"], skipped=e["skipped"],
-                                     avg_us=1e3 * e["ms"] / e["launches"],
-                                     GBs=e["bytes"] / (e["ms"] * 1e-3) / 1e9,
-                                     TFLOPs=e["flops"] / (e["ms"] * 1e-3) / 1e12,
-                                     total_ms=e["ms"], bytes_per_launch=e["bytes"] / e["launches"])
-        # dominant kernel among those that run on every step (never gated off)
-        cands = [k for k in ("gs_update_first", "gs_dot_first", "spmv", "multinplace") if k in kernels]
-        dom = max(cands, key=lambda k: kernels[k]["total_ms"])
-        dk = kernels[dom]
-        roofline = {"bound": "hbm", "achieved": round(dk["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(dk["GBs"] / HBM_PEAK_GBS, 4), "traffic": _measured_traffic(dom, N), "kernel": dom,
-                    "avg_us": round(dk["avg_us"], 2), "bytes_per_launch": dk["bytes_per_launch"]}
-        tot_b = sum(e["bytes"] for e in prof.values())
-        tot_ms = sum(e["ms"] for e in prof.values())
-        roofline["spmv_bv_aggregate_GBs"] = round(tot_b / (tot_ms * 1e-3) / 1e9, 1) if tot_ms else None
-        roofline["kernel_time_fraction_of_wall"] = round(tot_ms * 1e-3 / t_total, 3)
+    tot_b = None
+    if prof_dom is not None and prof_dom["launches"] and prof_dom["ms"] > 0:
+        # the dominant kernel's launches inside the timed region, timed by their own events
+        gbs = prof_dom["bytes"] / (prof_dom["ms"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": _measured_traffic(dom, N), "kernel": dom,
+                    "launches": prof_dom["launches"], "avg_us": round(1e3 * prof_dom["ms"] / prof_dom["launches"], 2),
+                    "bytes_per_launch": prof_dom["bytes"] / prof_dom["launches"]}
+        # the rest from the fully profiled solve
+        tot_b = sum(e["bytes"] for e in prof_all.values())
+        tot_ms = sum(e["ms"] for e in prof_all.values())
+        roofline["profiled_solve"] = {"spmv_bv_aggregate_GBs": round(tot_b / (tot_ms * 1e-3) / 1e9, 1),
+                                      "kernel_time_fraction_of_wall": round(tot_ms * 1e-3 / t_prof, 3),
+                                      "wall_ms_with_events_on_every_launch": round(1e3 * t_prof, 2)}
+        # algorithmic HBM bytes (the kernel byte formulas) per solve, per restart and per verified pair
+        med_its = float(np.median(its_list))
+        roofline["algorithmic_bytes"] = {"per_solve_GB": round(tot_b / 1e9, 3),
+                                         "per_restart_GB": round(tot_b / max(med_its, 1) / 1e9, 3),
+                                         "per_verified_pair_GB": round(tot_b / max(verified, 1) / 1e9, 3),
+                                         "per_solve_over_wall_GBs": round(tot_b / (float(np.median(times))) / 1e9, 1)}
         if "multinplace" in kernels:
             mk = kernels["multinplace"]
             roofline["multinplace"] = {"GBs": round(mk["GBs"], 1), "hbm_frac": round(mk["GBs"] / HBM_PEAK_GBS, 4),
@@ -224,7 +259,9 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(csr, n, args.nev, args.ncv, args.tol, int(np.median(its_list)), args.cpu_seconds)
+            per_restart_b = (tot_b / max(float(np.median(its_list)), 1)) if tot_b else None
+            cpu = cpu_baseline(csr, n, args.nev, args.ncv, args.tol, int(np.median(its_list)), args.cpu_seconds,
+                               per_restart_b)
             log(f"[bench] cpu baseline {cpu['value']:.4f} eigenpairs/s on {cpu['cores']} threads")
         except Exception as ex:   # the oracle is optional on the timing box; report why it is missing
             log(f"[bench] cpu baseline unavailable: {ex}")
@@ -296,9 +333,10 @@ def run_mixed(args):
                              "GBs": round(e.bytes / (e.ms * 1e-3) / 1e9, 1)}
         return tab
 
-    def solve(mixed):
+    def solve(mixed, profile=False):
+        # events around every launch only in untimed solves (they cost the stream a few us each)
         K.check(L.ks_prof_reset(ctx), "ks_prof_reset")
-        K.check(L.ks_prof_enable(ctx, 1), "ks_prof_enable")
+        K.check(L.ks_prof_enable(ctx, int(profile)), "ks_prof_enable")
         eps = E.EPS()
         eps.setOperators(A)
         eps.setProblemType(E.EPS.NHEP)
@@ -311,7 +349,7 @@ def run_mixed(args):
         ts = time.perf_counter()
         eps.solve()
         dt = time.perf_counter() - ts
-        prof = prof_table()
+        prof = prof_table() if profile else None
         K.check(L.ks_prof_enable(ctx, 0), "ks_prof_enable")
         k = min(eps.getConverged(), nev)
         errs = [eps.computeError(i) for i in range(k)]
@@ -322,9 +360,13 @@ def run_mixed(args):
         return dict(t=dt, verified=sum(e <= tol for e in errs), errs=errs, vals=vals, its=st["its"],
                     nmatmult=st["nmatmult"], mixed=ms, prof=prof)
 
-    res = {}
+    res, tables = {}, {}
     for form in ("fp64", "mixed"):
-        for w in range(args.warmup):
+        # the first untimed solve carries the per-kernel events (the table)
+        r = solve(form == "mixed", profile=True)
+        tables[form] = r["prof"]
+        log(f"[mixed] profiled {form}: {r['t']:.3f}s its={r['its']}")
+        for w in range(args.warmup - 1):
             r = solve(form == "mixed")
             log(f"[mixed] warmup {form}: {r['t']:.3f}s its={r['its']}")
         runs = [solve(form == "mixed") for _ in range(args.steps)]
@@ -345,7 +387,7 @@ def run_mixed(args):
                                "restarts": [r["its"] for r in res[f]],
                                "spmv": [r["nmatmult"] for r in res[f]],
                                "verified": [r["verified"] for r in res[f]],
-                               "kernels": res[f][-1]["prof"]} for f in res},
+                               "kernels_profiled_solve": tables[f]} for f in res},
                  "mixed_phase": res["mixed"][-1]["mixed"],
                  "speedup_vs_fp64": round(rate["mixed"] / rate["fp64"], 3) if rate["fp64"] else None,
                  "pair_rel_error_vs_fp64": [float(f"{e:.3e}") for e in pair_err],
@@ -450,6 +492,71 @@ def run_serve(args):
                                 "latency_ms_p50": round(float(np.percentile(np.array(seq_lat) * 1e3, 50)), 2)},
                  "speedup_vs_sequential": round(rate / seq_rate, 3) if seq_rate else None})
     out["config"].update({"requests": len(reqs), "qps": args.qps, "slots": args.slots})
+    print(json.dumps(out), flush=True)
+
+
+def run_scale(args):
+    """The at-scale mixed-precision serving case: the shifted normalized Laplacian
+    of an R-MAT graph (scale --rmat-scale, edge factor 16, FP32 values), nev 32
+    largest, ncv 96, FP32 phase to 1e-5 then FP64 restarts to 1e-10, served
+    together with 8 convection-diffusion requests of the mixed-precision shape
+    (2048^2, nev 8 largest real part, ncv 40); beside it the same nine requests
+    solved one after another in plain FP64."""
+    from slepc_amd import eps as E
+    from slepc_amd import matgen as M
+    from slepc_amd import serve as S
+    t0 = time.time()
+    rmat = M.rmat_shifted_normalized_laplacian(args.rmat_scale, 16, seed=5)
+    n5 = len(rmat[0]) - 1
+    log(f"[scale] R-MAT scale {args.rmat_scale}: n={n5} nnz={int(rmat[0][-1])} built in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    cds = [M.convection_diffusion_2d(args.mixed_grid, 10.0, seed=2 + i) for i in range(args.cd_requests)]
+    log(f"[scale] {len(cds)} convection-diffusion {args.mixed_grid}^2 matrices built in {time.time() - t0:.1f}s")
+    E.initialize(0)
+    reqs = [dict(n=n5, nev=32, ncv=96, t_arrival=0.0, v0_seed=7, which=E.EPS.LARGEST_REAL, hermitian=True,
+                 mixed_tol=1e-5, _csr=rmat, kind="rmat")]
+    for i, c in enumerate(cds):
+        reqs.append(dict(n=len(c[0]) - 1, nev=8, ncv=40, t_arrival=0.0, v0_seed=100 + i, which=E.EPS.LARGEST_REAL,
+                         hermitian=False, mixed_tol=1e-5, _csr=c, kind="convdiff"))
+    srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=len(reqs), grouped=True)
+    res, wall = S.run_stream(srv, reqs, lambda r: r["_csr"], tol=1e-10)
+    stats = srv.stats()
+    srv.close()
+    good = sum(r["ngood"] for r in res)
+    for r, o in zip(reqs, res):
+        log(f"[scale] served {r['kind']} n={r['n']}: its={o['its']} ngood={o['ngood']} latency {o['latency']:.2f}s")
+    log(f"[scale] served: {wall:.2f}s, {good} verified pairs")
+    seq = None
+    if not args.no_sequential:
+        ts = time.perf_counter()
+        seq_good, seq_t = 0, []
+        for r in reqs:
+            t1 = time.perf_counter()
+            out = E.solve(r["_csr"], nev=r["nev"], ncv=r["ncv"], tol=1e-10, which=r["which"], hermitian=r["hermitian"],
+                          v0=M.uniform_vector(r["v0_seed"], r["n"]), want_vectors=False)
+            seq_t.append(time.perf_counter() - t1)
+            seq_good += int(np.sum(np.asarray(out["err"][:r["nev"]]) <= 1e-10))
+            log(f"[scale] plain FP64 {r['kind']}: {seq_t[-1]:.2f}s its={out['its']}")
+        seq_wall = time.perf_counter() - ts
+        seq = {"eigenpairs_per_s": round(seq_good / seq_wall, 4), "wall_s": round(seq_wall, 3),
+               "verified_pairs": seq_good, "solve_s": [round(t, 3) for t in seq_t]}
+    rate = good / wall
+    out = _base(args, rate, "eigenpairs/s", 1e3 * wall, 1, "f32+f64",
+                f"mixed-precision serving at scale: 2I - normalized Laplacian of R-MAT scale {args.rmat_scale} "
+                f"(a,b,c,d = .57,.19,.19,.05, edge factor 16, FP32 values), nev 32 largest, ncv 96, FP32 phase to "
+                f"1e-5 then FP64 to 1e-10, served with {len(cds)} mixed-precision convection-diffusion "
+                f"{args.mixed_grid}^2 requests (nev 8 LR, ncv 40)",
+                {"latency_ms": {"p50": round(float(np.percentile([o["latency"] * 1e3 for o in res], 50)), 2),
+                                "p99": round(float(np.percentile([o["latency"] * 1e3 for o in res], 99)), 2)},
+                 "served": {"eigenpairs_per_s": round(rate, 4), "wall_s": round(wall, 3), "verified_pairs": good,
+                            "requests": [{"kind": r["kind"], "n": r["n"], "its": o["its"], "ngood": o["ngood"],
+                                          "nmatmult": o["nmatmult"], "latency_s": round(o["latency"], 3)}
+                                         for r, o in zip(reqs, res)],
+                            "peak_reserved_bytes": stats["peak_reserved"]},
+                 "sequential_plain_fp64": seq,
+                 "speedup_vs_sequential_fp64": round(rate / seq["eigenpairs_per_s"], 3) if seq else None})
+    out["config"].update({"rmat_scale": args.rmat_scale, "rmat_n": n5, "rmat_nnz": int(rmat[0][-1]),
+                          "convdiff_requests": len(cds), "convdiff_grid": args.mixed_grid})
     print(json.dumps(out), flush=True)
 
 

@@ -195,6 +195,8 @@ typedef struct {
   double  flops;
 } ks_prof_entry;
 int ks_prof_enable(ks_ctx ctx, int on);
+/* events only for the classes whose bit (1 << class) is set in mask; 0 turns profiling off */
+int ks_prof_enable_classes(ks_ctx ctx, unsigned mask);
 int ks_prof_collect(ks_ctx ctx);
 int ks_prof_get(ks_ctx ctx, int cls, ks_prof_entry *out);
 int ks_prof_reset(ks_ctx ctx);

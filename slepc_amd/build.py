@@ -61,6 +61,17 @@ def build_host():
     os.replace(tmp, HOST_LIB)
 
 
+GEN_LIB = os.path.join(HERE, "libks_gen.so")
+
+
+def build_gen():
+    """Workload generators (R-MAT graph Laplacian) with OpenMP."""
+    tmp = GEN_LIB + ".tmp"
+    _run([CC, "-O3", "-fopenmp", "-fPIC", "-shared", "-std=c11", "-Wall", "-o", tmp,
+          os.path.join(HERE, "gen", "rmat.c"), "-lm"])
+    os.replace(tmp, GEN_LIB)
+
+
 def build_oracle():
     if shutil.which("make"):
         _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
@@ -69,6 +80,7 @@ def build_oracle():
 def build_all(oracle: bool = True):
     build_kernels()
     build_host()
+    build_gen()
     if oracle:
         build_oracle()
 

@@ -12,7 +12,6 @@
 #include <cmath>
 
 #define KS_CT 8          // columns per register tile in the dot kernels
-#define KS_MIP_ROWS 64   // rows per workgroup of the restart GEMM (4 waves x 16)
 #define KS_MIP_KC 16     // k-chunk of Q staged in LDS
 
 namespace {
@@ -401,11 +400,15 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_lds(const T *__restrict__ V, lo
 
 // ---------------------------------------------------------------- register-resident Gram-Schmidt
 // The same pass for k <= KC columns with no LDS staging: a thread owns rows
-// tid, tid + 256*nb, ... and for each row issues all k column loads at once
-// (k requests in flight per thread), forms the update w_i - V(i,:) c of the
-// fused pass from those registers, then adds V(i,c) w_i into k+1 register
-// accumulators.  The workgroup reduces its accumulators once, at the end
-// (wave butterflies, then the four waves through LDS), into its partial row.
+// of its workgroup's contiguous range and for each row issues all KC column
+// loads at once (KC requests in flight per thread), forms the update
+// w_i - V(i,:) c of the fused pass from those registers, then adds V(i,c) w_i
+// into k+1 register accumulators.  Slots c >= k re-read column k-1 (a cached
+// line; no per-slot branch, which would make each load wait for the one
+// before it) and contribute nothing: their coefficients are zero and their
+// sums are not stored.  FP32 data: FP32 products and per-thread sums.  The
+// workgroup reduces its accumulators once, at the end (wave butterflies in
+// FP64, then the four waves through LDS), into its partial row.
 template <typename T, int KC, bool FUSED>
 __global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, long ld, int k, T *__restrict__ w,
                                                     int n, int rpb, const double *__restrict__ cbuf,
@@ -413,48 +416,50 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, lo
                                                     int pass, int *ran) {
   if (!gs_pass_gate<FUSED>(st, col, pass)) return;
   if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = 1;
-  __shared__ double coef[KC];
+  using AT = typename acc_t<T>::type;
+  __shared__ AT coef[KC];
   __shared__ double red[KS_WAVES][KC + 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (FUSED)
-    for (int c = tid; c < KC; c += KS_BLOCK) coef[c] = c < k ? cbuf[c] : 0.0;
-  double acc[KC + 1];
+    for (int c = tid; c < KC; c += KS_BLOCK) coef[c] = (AT)(c < k ? cbuf[c] : 0.0);
+  AT acc[KC + 1];
 #pragma unroll
-  for (int c = 0; c <= KC; c++) acc[c] = 0.0;
+  for (int c = 0; c <= KC; c++) acc[c] = (AT)0;
   __syncthreads();
   // each workgroup owns one contiguous row range (rpb rows, a multiple of 256),
   // so a column's accesses stay inside a few pages while the workgroup runs;
   // column c is read as (column base) + (32-bit row byte offset), n * sizeof(T) < 4 GiB
+  const int klast = k > 0 ? k - 1 : 0;   // k = 0: column 0 is w itself, a valid address
   const unsigned r0 = blockIdx.x * (unsigned)rpb;
   const unsigned r1 = min((unsigned)n, r0 + (unsigned)rpb);
   for (unsigned i = r0 + tid; i < r1; i += KS_BLOCK) {
     const unsigned off = i * (unsigned)sizeof(T);
-    double v[KC];
+    T v[KC];
 #pragma unroll
     for (int c = 0; c < KC; c++) {
-      const char *colbase = (const char *)(V + (size_t)c * ld);
-      v[c] = c < k ? (double)*(const T *)(colbase + off) : 0.0;
+      const char *colbase = (const char *)(V + (size_t)min(c, klast) * ld);
+      v[c] = *(const T *)(colbase + off);
     }
-    double wi = (double)w[i];
+    AT wi = (AT)w[i];
     if (FUSED) {
-      double s0 = 0.0, s1 = 0.0;
+      AT s0 = (AT)0, s1 = (AT)0;
 #pragma unroll
       for (int c = 0; c < KC; c += 2) {
-        s0 += v[c] * coef[c];
-        s1 += v[c + 1] * coef[c + 1];
+        s0 += (AT)v[c] * coef[c];
+        s1 += (AT)v[c + 1] * coef[c + 1];
       }
       const T t = (T)(wi - (s0 + s1));
       w[i] = t;
-      wi = (double)t;
+      wi = (AT)t;
     }
 #pragma unroll
-    for (int c = 0; c < KC; c++) acc[c] += v[c] * wi;
+    for (int c = 0; c < KC; c++) acc[c] += (AT)v[c] * wi;
     acc[KC] += wi * wi;
   }
   // workgroup reduction of k+1 sums; column k's slot carries |w|^2
 #pragma unroll
   for (int c = 0; c <= KC; c++) {
-    double x = ks_wave_sum(acc[c]);
+    double x = ks_wave_sum((double)acc[c]);
     if (lane == 0) red[wave][c] = x;
   }
   __syncthreads();
@@ -491,7 +496,8 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
   constexpr int STEP = 64 * RPL;
   __shared__ AT coef[KS_WAVES * CW];
   __shared__ AT red[2][KS_WAVES][STEP];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: column guards are scalar branches
   const int c0 = wave * CW;
   if (FUSED)
     for (int c = tid; c < KS_WAVES * CW; c += KS_BLOCK) coef[c] = (AT)(c < k ? cbuf[c] : 0.0);
@@ -507,25 +513,38 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
     const unsigned ia = i0 + RPL * lane;   // this lane's first row
     T v[RPL][CW];
     AT wi[RPL];
+    if (i0 + STEP <= r1) {
+      // every lane's rows exist (all steps but the last workgroup's last): no
+      // per-lane guards, so the CW loads are issued back to back
 #pragma unroll
-    for (int u = 0; u < CW; u++) {
-      const T *col = V + (size_t)(c0 + u) * ld;
-      const bool have = c0 + u < k;
-      if (PAIR) {
-        if (have && ia + 1 < r1) {
-          const float2 p2 = *(const float2 *)(col + ia);
-          v[0][u] = p2.x;
-          v[RPL - 1][u] = p2.y;
+      for (int u = 0; u < CW; u++) {
+        const T *col = V + (size_t)(c0 + u) * ld;
+        if (c0 + u < k) {
+          if (PAIR) {
+            const float2 p2 = *(const float2 *)(col + ia);
+            v[0][u] = p2.x;
+            v[RPL - 1][u] = p2.y;
+          } else {
+            v[0][u] = col[ia];
+          }
         } else {
-          v[0][u] = (have && ia < r1) ? col[ia] : (T)0;
-          v[RPL - 1][u] = (T)0;
-        }
-      } else {
-        v[0][u] = (have && ia < r1) ? col[ia] : (T)0;
-      }
-    }
 #pragma unroll
-    for (int q = 0; q < RPL; q++) wi[q] = ia + q < r1 ? (AT)w[ia + q] : (AT)0;
+          for (int q = 0; q < RPL; q++) v[q][u] = (T)0;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < RPL; q++) wi[q] = (AT)w[ia + q];
+    } else {
+#pragma unroll
+      for (int u = 0; u < CW; u++) {
+        const T *col = V + (size_t)(c0 + u) * ld;
+        const bool have = c0 + u < k;
+#pragma unroll
+        for (int q = 0; q < RPL; q++) v[q][u] = (have && ia + q < r1) ? col[ia + q] : (T)0;
+      }
+#pragma unroll
+      for (int q = 0; q < RPL; q++) wi[q] = ia + q < r1 ? (AT)w[ia + q] : (AT)0;
+    }
     if (FUSED) {
 #pragma unroll
       for (int q = 0; q < RPL; q++) {
@@ -722,47 +741,64 @@ __global__ void k_random(T *__restrict__ x, int n, uint64_t seed) {
 }
 
 template <typename T, int RPT>
-__global__ __launch_bounds__(KS_BLOCK) void k_norm(const T *__restrict__ x, int n, int nblocks, double *partial,
-                                                  double *gpartial, unsigned *counters, double *out) {
+__global__ __launch_bounds__(KS_BLOCK) void k_norm(const T *__restrict__ x, int n, double *partial) {
   __shared__ double red[KS_WAVES];
-  __shared__ double fin[2];
   const size_t row0 = (size_t)blockIdx.x * KS_BLOCK * RPT;
+  // rows of the last workgroup past n read row n-1 (a valid address) and add nothing
+  T v[RPT];
+#pragma unroll
+  for (int r = 0; r < RPT; r++) {
+    const size_t i = row0 + (size_t)r * KS_BLOCK + threadIdx.x;
+    v[r] = x[i < (size_t)n ? i : (size_t)n - 1];
+  }
   double s = 0.0;
 #pragma unroll
   for (int r = 0; r < RPT; r++) {
     const size_t i = row0 + (size_t)r * KS_BLOCK + threadIdx.x;
-    if (i < (size_t)n) {
-      const double v = (double)x[i];
-      s += v * v;
-    }
+    const double d = i < (size_t)n ? (double)v[r] : 0.0;
+    s += d * d;
   }
   const double bs = ks_block_sum(s, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = bs;
-  if (!ks_reduce_finish(partial, gpartial, counters, nblocks, 1, fin)) return;
-  if (threadIdx.x == 0) out[0] = sqrt(fin[0]);
+}
+
+// the norm's finish: one workgroup adds the per-workgroup sums in a fixed order
+// (a separate stream-ordered launch: no fences or atomics in the sweep)
+__global__ __launch_bounds__(KS_BLOCK) void k_norm_fin(const double *__restrict__ partial, int nb, double *out) {
+  __shared__ double red[KS_WAVES];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += KS_BLOCK) s += partial[b];
+  const double tot = ks_block_sum(s, red);
+  if (threadIdx.x == 0) out[0] = sqrt(tot);
 }
 
 // ---------------------------------------------------------------- restart GEMM
 // V(:, s:s+N) = V(:, 0:K) * Q(0:K, s:s+N) on v_mfma_f64_16x16x4f64.
-// Each wavefront owns 16 rows and all N output columns (NT 16-wide tiles kept in
-// the accumulators).  A fragments come straight from HBM (lane l: row l%16,
-// k = l/16), Q chunks are staged through LDS.  A wave reads every input column
-// of its rows before it stores any output, and rows are disjoint across waves,
-// so the update is in place without a temporary.
+// Each wavefront owns R tiles of 16 rows and all N output columns (R x NT
+// 16-wide tiles kept in the accumulators).  A fragments come straight from HBM
+// (lane l: row l%16, k = l/16), all R x KS_MIP_KC/4 of a k-chunk issued before
+// the first MFMA that uses them; Q chunks are staged through LDS once per
+// workgroup for its 64 R rows.  A wave reads every input column of its rows
+// before it stores any output, and rows are disjoint across waves, so the
+// update is in place without a temporary.  Lanes past the last row or column
+// read row n-1 / column K-1 (valid addresses, no per-lane branch); their
+// products meet zero rows of the staged Q or land in rows that are not stored.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-template <typename T, int NT>
+template <typename T, int NT, int R>
 __global__ __launch_bounds__(KS_BLOCK) void k_multinplace_mfma(T *__restrict__ V, long ld, int n, int K,
                                                               const double *__restrict__ Q, int ldq, int s,
                                                               int N) {
   __shared__ double sQ[KS_MIP_KC][NT * 16];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int rbase = blockIdx.x * KS_MIP_ROWS + wave * 16;
-  const int ra = rbase + (lane & 15);   // A-fragment row of this lane
-  const int kq = lane >> 4;             // k offset of this lane inside a 4-step
-  v4d acc[NT];
+  const int rbase = (blockIdx.x * KS_WAVES + wave) * (16 * R);   // this wave's first row
+  const int kq = lane >> 4;                                       // k offset of this lane inside a 4-step
+  const int nlast = n - 1;
+  v4d acc[R][NT];
 #pragma unroll
-  for (int t = 0; t < NT; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int r = 0; r < R; r++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[r][t] = v4d{0.0, 0.0, 0.0, 0.0};
   for (int k0 = 0; k0 < K; k0 += KS_MIP_KC) {
     __syncthreads();
     for (int idx = tid; idx < KS_MIP_KC * NT * 16; idx += KS_BLOCK) {
@@ -770,38 +806,54 @@ __global__ __launch_bounds__(KS_BLOCK) void k_multinplace_mfma(T *__restrict__ V
       sQ[kk][j] = (k0 + kk < K && j < N) ? Q[(size_t)(k0 + kk) + (size_t)(s + j) * ldq] : 0.0;
     }
     __syncthreads();
+    double a[R][KS_MIP_KC / 4];
 #pragma unroll
-    for (int kk = 0; kk < KS_MIP_KC; kk += 4) {
-      const int kc = k0 + kk + kq;
-      const double a = (ra < n && kc < K) ? (double)V[(size_t)ra + (size_t)kc * ld] : 0.0;
+    for (int r = 0; r < R; r++) {
+      const int row = min(rbase + r * 16 + (lane & 15), nlast);
+#pragma unroll
+      for (int kk = 0; kk < KS_MIP_KC / 4; kk++) {
+        const int kc = min(k0 + 4 * kk + kq, K - 1);
+        a[r][kk] = (double)V[(size_t)row + (size_t)kc * ld];
+      }
+    }
+#pragma unroll
+    for (int kk = 0; kk < KS_MIP_KC / 4; kk++) {
 #pragma unroll
       for (int t = 0; t < NT; t++) {
-        const double b = sQ[kk + kq][t * 16 + (lane & 15)];
-        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+        const double b = sQ[4 * kk + kq][t * 16 + (lane & 15)];
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r][kk], b, acc[r][t], 0, 0, 0);
       }
     }
   }
   // D layout of v_mfma_f64_16x16x4f64 (measured on gfx950): register q of lane l
   // holds row l/16 + 4q, column l%16
-  const int rowd = rbase + (lane >> 4);
   const int cold = lane & 15;
 #pragma unroll
-  for (int t = 0; t < NT; t++) {
-    const int c = t * 16 + cold;
-    if (c < N) {
+  for (int r = 0; r < R; r++) {
+    const int rowd = rbase + r * 16 + (lane >> 4);
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int row = rowd + 4 * q;
-        if (row < n) V[(size_t)row + (size_t)(s + c) * ld] = (T)acc[t][q];
+    for (int t = 0; t < NT; t++) {
+      const int c = t * 16 + cold;
+      if (c < N) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int row = rowd + 4 * q;
+          if (row < n) V[(size_t)row + (size_t)(s + c) * ld] = (T)acc[r][t][q];
+        }
       }
     }
   }
 }
 
+// row tiles per wavefront: more rows per staged Q chunk while the accumulators
+// (R * NT * 4 doubles per lane) stay within 64 registers' worth
 template <typename T, int NT>
 void launch_mip(ks_ctx ctx, T *V, long ld, int n, int K, const double *Q, int ldq, int s, int N) {
-  dim3 grid((n + KS_MIP_ROWS - 1) / KS_MIP_ROWS), block(KS_BLOCK);
-  hipLaunchKernelGGL((k_multinplace_mfma<T, NT>), grid, block, 0, ctx->stream, V, ld, n, K, Q, ldq, s, N);
+  constexpr int R = NT <= 2 ? 4 : (NT <= 4 ? 2 : 1);
+  const int rows = KS_WAVES * 16 * R;
+  dim3 grid((n + rows - 1) / rows), block(KS_BLOCK);
+  hipLaunchKernelGGL((k_multinplace_mfma<T, NT, R>), grid, block, 0, ctx->stream, V, ld, n, K, Q, ldq, s, N);
 }
 
 template <typename T>
@@ -1134,8 +1186,7 @@ int ks_bv_norm(ks_ctx ctx, ks_prec prec, int n, const void *x, double *nrm) {
   dim3 grid(nb), block(KS_BLOCK);
 #define KS_NORM_CASE(T, R)                                                                                       \
   case R:                                                                                                        \
-    hipLaunchKernelGGL((k_norm<T, R>), grid, block, 0, ctx->stream, (const T *)x, n, nb, ctx->d_partial,         \
-                       ctx->d_gpartial, ctx->d_counters, ctx->d_scalar);                                         \
+    hipLaunchKernelGGL((k_norm<T, R>), grid, block, 0, ctx->stream, (const T *)x, n, ctx->d_partial);            \
     break;
   if (prec == KS_F64) {
     switch (rpt) { KS_NORM_CASE(double, 1) KS_NORM_CASE(double, 2) KS_NORM_CASE(double, 4) KS_NORM_CASE(double, 8) KS_NORM_CASE(double, 16) }
@@ -1143,6 +1194,7 @@ int ks_bv_norm(ks_ctx ctx, ks_prec prec, int n, const void *x, double *nrm) {
     switch (rpt) { KS_NORM_CASE(float, 1) KS_NORM_CASE(float, 2) KS_NORM_CASE(float, 4) KS_NORM_CASE(float, 8) KS_NORM_CASE(float, 16) }
   }
 #undef KS_NORM_CASE
+  hipLaunchKernelGGL(k_norm_fin, dim3(1), block, 0, ctx->stream, (const double *)ctx->d_partial, nb, ctx->d_scalar);
   ks_prof_end(ctx);
   KS_HIP(hipGetLastError());
   KS_HIP(hipMemcpyAsync(ctx->h_scalar, ctx->d_scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

@@ -77,6 +77,8 @@ struct ks_ctx_s {
   double *h_scalar = nullptr;      // pinned host scratch
   // profiling
   bool prof = false;
+  unsigned prof_mask = ~0u;   // classes that get events while prof is on
+  bool prof_open = false;     // the last ks_prof_begin recorded its start event
   std::vector<ks_prof_rec> recs;
   std::vector<hipEvent_t> event_pool;
   int *d_ran = nullptr;            // per-launch "did work" flags

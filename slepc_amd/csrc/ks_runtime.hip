@@ -342,7 +342,8 @@ static hipEvent_t pool_get(ks_ctx c) {
 }
 
 int *ks_prof_begin(ks_ctx c, int cls, double bytes, double flops, bool gated) {
-  if (!c->prof) return nullptr;
+  c->prof_open = false;
+  if (!c->prof || !((c->prof_mask >> cls) & 1u)) return nullptr;
   if (c->recs.size() >= 65536 || (gated && c->ran_used >= c->ran_cap)) ks_prof_collect(c);
   ks_prof_rec r;
   r.cls = cls;
@@ -353,12 +354,14 @@ int *ks_prof_begin(ks_ctx c, int cls, double bytes, double flops, bool gated) {
   r.ev1 = pool_get(c);
   (void)hipEventRecord(r.ev0, c->stream);
   c->recs.push_back(r);
+  c->prof_open = true;
   return gated ? c->d_ran + r.ran_slot : nullptr;
 }
 
 void ks_prof_end(ks_ctx c) {
-  if (!c->prof || c->recs.empty()) return;
+  if (!c->prof_open) return;
   (void)hipEventRecord(c->recs.back().ev1, c->stream);
+  c->prof_open = false;
 }
 
 int ks_prof_enable(ks_ctx c, int on) {
@@ -371,6 +374,17 @@ int ks_prof_enable(ks_ctx c, int on) {
   }
   if (!on && c->prof) ks_prof_collect(c);
   c->prof = on != 0;
+  c->prof_mask = ~0u;
+  return KS_OK;
+}
+
+// events only around the launches of the classes in mask (bit cls): each
+// event pair costs the stream a few microseconds, so a timed run can keep the
+// one class it reports and leave the rest of the stream undisturbed
+int ks_prof_enable_classes(ks_ctx c, unsigned mask) {
+  int e = ks_prof_enable(c, mask != 0u);
+  if (e) return e;
+  c->prof_mask = mask;
   return KS_OK;
 }
 

@@ -39,26 +39,30 @@ __device__ __forceinline__ void spmv_block(int b, const int32_t *__restrict__ bl
     if (tid == 0) y[r0] = (XT)tot;
     return;
   }
-  {
+  const int cnt = p1 - p0;
+  if (cnt > 0) {
     // all of this thread's column indices and values are loaded first, then all
-    // its x gathers: KS_SPMV_NNZ / 256 independent requests in flight per level
+    // its x gathers: KS_SPMV_NNZ / 256 independent requests in flight per level.
+    // No per-lane guards on the loads (a guarded load compiles to a branch that
+    // waits for it): a lane past the block's last entry re-reads that entry,
+    // which sits in a cache line the wave fetches anyway, and its product is
+    // not read.
     constexpr int PER = KS_SPMV_NNZ / KS_BLOCK;
+    const int last = p1 - 1;
     int ci[PER];
-    AT va[PER], xv[PER];
+    VT va[PER];
+    AT xv[PER];
 #pragma unroll
     for (int u = 0; u < PER; u++) {
-      const int p = p0 + tid + u * KS_BLOCK;
-      const bool ok = p < p1;
-      ci[u] = ok ? colidx[p] : 0;
-      va[u] = ok ? (AT)val[p] : (AT)0;
+      const int p = min(p0 + tid + u * KS_BLOCK, last);
+      ci[u] = colidx[p];
+      va[u] = val[p];
     }
 #pragma unroll
-    for (int u = 0; u < PER; u++) xv[u] = (p0 + tid + u * KS_BLOCK < p1) ? (AT)x[ci[u]] : (AT)0;
+    for (int u = 0; u < PER; u++) xv[u] = (AT)x[ci[u]];
+    // every slot is written (slots past the block's entries are never read)
 #pragma unroll
-    for (int u = 0; u < PER; u++) {
-      const int p = p0 + tid + u * KS_BLOCK;
-      if (p < p1) prod[p - p0] = va[u] * xv[u];
-    }
+    for (int u = 0; u < PER; u++) prod[tid + u * KS_BLOCK] = (AT)va[u] * xv[u];
   }
   __syncthreads();
   // threads per row: largest power of two with nrows * tpr <= KS_BLOCK, at most 64

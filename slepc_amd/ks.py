@@ -67,7 +67,7 @@ def lib():
             "ks_bv_random": [vp, i32, i32, vp, C.c_uint64],
             "ks_bv_convert": [vp, i32, i32, vp, i32, vp],
             "ks_bv_multinplace": [vp, i32, i32, vp, i32, i32, vp, i32, i32, i32],
-            "ks_prof_enable": [vp, i32], "ks_prof_collect": [vp],
+            "ks_prof_enable": [vp, i32], "ks_prof_enable_classes": [vp, C.c_uint], "ks_prof_collect": [vp],
             "ks_prof_get": [vp, i32, C.POINTER(ProfEntry)], "ks_prof_reset": [vp],
             "ks_mem_info": [C.POINTER(sz), C.POINTER(sz)],
         }

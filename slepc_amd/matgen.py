@@ -210,37 +210,48 @@ def er_laplacian(n: int, avg_degree: float = 8.0, seed: int = 0):
     return _csr_from_coo(n, r, c, v)
 
 
-def rmat_edges(scale: int, edge_factor: int = 16, abcd=(0.57, 0.19, 0.19, 0.05), seed: int = 5,
-               chunk: int = 1 << 24):
-    """Directed R-MAT edge list (Graph500 recursive quadrant choice, no noise):
-    2^scale vertices, edge_factor * 2^scale edges, as int64 (src, dst) arrays."""
-    rng = np.random.default_rng(seed)
+def _rmat_thresholds(abcd):
     a, b, c, _ = abcd
+    return (np.uint64(int(round(a * 65536.0))), np.uint64(int(round((a + b) * 65536.0))),
+            np.uint64(int(round((a + b + c) * 65536.0))))
+
+
+def rmat_edges(scale: int, edge_factor: int = 16, abcd=(0.57, 0.19, 0.19, 0.05), seed: int = 5,
+               chunk: int = 1 << 20):
+    """Directed R-MAT edge list (Graph500 recursive quadrant choice, no noise):
+    2^scale vertices, edge_factor * 2^scale edges, as int64 (src, dst) arrays.
+    Counter-based randomness: edge e draws r_g = splitmix64(seed * golden + e*G + g),
+    g < G = ceil(scale/4); level l = 4g + t uses the 16-bit fraction
+    (r_g >> 16t) & 0xffff against round(65536 a), round(65536 (a+b)),
+    round(65536 (a+b+c)) -- the same stream as slepc_amd/gen/rmat.c."""
+    TA, TB, TC = _rmat_thresholds(abcd)
+    G = (scale + 3) // 4
     m = edge_factor << scale
     src = np.empty(m, dtype=np.int64)
     dst = np.empty(m, dtype=np.int64)
-    for s0 in range(0, m, chunk):
-        e = min(m, s0 + chunk)
-        u = np.zeros(e - s0, dtype=np.int64)
-        v = np.zeros(e - s0, dtype=np.int64)
-        for bit in range(scale):
-            x = rng.random(e - s0)
-            right = (x >= a) & (x < a + b) | (x >= a + b + c)   # quadrants b, d
-            down = x >= a + b                                  # quadrants c, d
-            u |= down.astype(np.int64) << bit
-            v |= right.astype(np.int64) << bit
-        src[s0:e], dst[s0:e] = u, v
+    with np.errstate(over="ignore"):
+        base = np.uint64(seed) * _SM_GOLDEN
+        for s0 in range(0, m, chunk):
+            e1 = min(m, s0 + chunk)
+            e = np.arange(s0, e1, dtype=np.uint64)
+            u = np.zeros(e1 - s0, dtype=np.uint64)
+            v = np.zeros(e1 - s0, dtype=np.uint64)
+            for g in range(G):
+                r = _splitmix64(base + e * np.uint64(G) + np.uint64(g))
+                for t in range(4):
+                    lvl = 4 * g + t
+                    if lvl >= scale:
+                        break
+                    x = (r >> np.uint64(16 * t)) & np.uint64(0xFFFF)
+                    right = ((x >= TA) & (x < TB)) | (x >= TC)
+                    down = x >= TB
+                    u |= down.astype(np.uint64) << np.uint64(lvl)
+                    v |= right.astype(np.uint64) << np.uint64(lvl)
+            src[s0:e1], dst[s0:e1] = u.astype(np.int64), v.astype(np.int64)
     return src, dst
 
 
-def rmat_shifted_normalized_laplacian(scale: int, edge_factor: int = 16, seed: int = 5,
-                                      abcd=(0.57, 0.19, 0.19, 0.05), dtype=np.float32):
-    """M = 2I - (I - D^-1/2 A D^-1/2) = I + D^-1/2 A D^-1/2 of the R-MAT graph
-    symmetrized (A = pattern of E + E^T), with self-loops and duplicate edges
-    dropped; isolated vertices get M_ii = 1 (normalized Laplacian L_ii = 1 for
-    them, the convention with D^-1/2 = 0 on isolated vertices).
-    Eigenvalues lie in [0, 2]; the largest of M are the smallest of the
-    normalized Laplacian.  Values in ``dtype`` (float32 for the at-scale case)."""
+def _rmat_laplacian_numpy(scale, edge_factor, seed, abcd):
     n = 1 << scale
     src, dst = rmat_edges(scale, edge_factor, abcd, seed)
     keep = src != dst
@@ -258,7 +269,55 @@ def rmat_shifted_normalized_laplacian(scale: int, edge_factor: int = 16, seed: i
     c = np.concatenate([j, i, np.arange(n, dtype=np.int64)])
     del i, j
     v = np.concatenate([w, w, np.ones(n)])
-    rowptr, colidx, val = _csr_from_coo(n, r, c, v)
+    return _csr_from_coo(n, r, c, v)   # FP64 values
+
+
+def _rmat_laplacian_native(scale, edge_factor, seed, abcd):
+    """The same matrix from slepc_amd/libks_gen.so (OpenMP); None when it is not built."""
+    import ctypes as C
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libks_gen.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    L.ks_gen_rmat_laplacian.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                                        C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                        C.POINTER(C.c_int64)]
+    L.ks_gen_free.argtypes = [C.c_void_p]
+    rp, ci, va, nnz = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_int64()
+    a, b, c, _ = abcd
+    e = L.ks_gen_rmat_laplacian(scale, edge_factor, C.c_uint64(seed), a, b, c, C.byref(rp), C.byref(ci),
+                                C.byref(va), C.byref(nnz))
+    if e != 0:
+        raise MemoryError(f"ks_gen_rmat_laplacian failed ({e})")
+    n = 1 << scale
+    try:
+        rowptr = np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_int32)), shape=(n + 1,)).copy()
+        colidx = np.ctypeslib.as_array(C.cast(ci, C.POINTER(C.c_int32)), shape=(nnz.value,)).copy()
+        val = np.ctypeslib.as_array(C.cast(va, C.POINTER(C.c_float)), shape=(nnz.value,)).copy()
+    finally:
+        for p in (rp, ci, va):
+            L.ks_gen_free(p)
+    return rowptr, colidx, val
+
+
+def rmat_shifted_normalized_laplacian(scale: int, edge_factor: int = 16, seed: int = 5,
+                                      abcd=(0.57, 0.19, 0.19, 0.05), dtype=np.float32, native: bool = True):
+    """M = 2I - (I - D^-1/2 A D^-1/2) = I + D^-1/2 A D^-1/2 of the R-MAT graph
+    symmetrized (A = pattern of E + E^T), with self-loops and duplicate edges
+    dropped; isolated vertices get M_ii = 1 (normalized Laplacian L_ii = 1 for
+    them, the convention with D^-1/2 = 0 on isolated vertices).  Off-diagonal
+    values are d_i^-1/2 d_j^-1/2 formed in FP64 and rounded once to FP32.
+    Eigenvalues lie in [0, 2]; the largest of M are the smallest of the
+    normalized Laplacian.  ``native``: an FP32 matrix comes from the OpenMP
+    generator in libks_gen.so when that is built (the same matrix, much faster
+    at scale); FP64 values are formed in numpy and not rounded."""
+    out = None
+    if native and np.dtype(dtype) == np.float32:
+        out = _rmat_laplacian_native(scale, edge_factor, seed, abcd)
+    if out is None:
+        out = _rmat_laplacian_numpy(scale, edge_factor, seed, abcd)
+    rowptr, colidx, val = out
     return rowptr, colidx, val.astype(dtype)
 
 

@@ -90,6 +90,33 @@ def test_rmat_shifted_normalized_laplacian():
     assert np.allclose(A @ u, 2.0 * u)
 
 
+def test_rmat_native_generator_matches_numpy_restatement():
+    # the OpenMP generator (slepc_amd/gen/rmat.c) and the numpy restatement of the
+    # same counter-based steps build the same CSR, bit for bit
+    import os
+    from slepc_amd import build
+    if not os.path.exists(build.GEN_LIB):
+        build.build_gen()
+    for scale in (8, 11):
+        a = M._rmat_laplacian_native(scale, 16, 5, (0.57, 0.19, 0.19, 0.05))
+        b = M._rmat_laplacian_numpy(scale, 16, 5, (0.57, 0.19, 0.19, 0.05))
+        b = (b[0], b[1], b[2].astype(np.float32))    # rounded once to FP32, as the generator does
+        assert a is not None
+        for x, y in zip(a, b):
+            assert x.dtype == y.dtype and np.array_equal(x, y)
+
+
+def test_rmat_quadrant_frequencies():
+    # level choices follow (a, b, c, d) = (0.57, 0.19, 0.19, 0.05): the top bit of
+    # (src, dst) is (down, right) of the first level
+    src, dst = M.rmat_edges(12, 16, seed=3)
+    top = 1 << 11
+    d, r = (src & 1) > 0, (dst & 1) > 0
+    freq = [np.mean(~d & ~r), np.mean(~d & r), np.mean(d & ~r), np.mean(d & r)]
+    assert np.allclose(freq, [0.57, 0.19, 0.19, 0.05], atol=0.01), freq
+    assert src.max() < 2 * top and dst.max() < 2 * top
+
+
 def test_uniform_vector_is_the_device_stream(oracle):
     # the same counter-based generator as the oracle's and the device's
     for seed in (0, 1, 123456789):
