@@ -81,6 +81,8 @@ int ks_memset_zero(ks_ctx ctx, void *dst, size_t bytes);   /* fill kernel, not h
 /* stream-ordered host->device copy that returns without waiting; src must stay
    untouched until the stream passes this point (use pinned memory) */
 int ks_memcpy_htod_async(ks_ctx ctx, void *dst, const void *src, size_t bytes);
+/* stream-ordered device-to-host copy without a wait (dst: pinned host memory) */
+int ks_memcpy_dtoh_async(ks_ctx ctx, void *dst, const void *src, size_t bytes);
 int ks_host_alloc(void **ptr, size_t bytes);   /* pinned host memory */
 int ks_host_free(void *ptr);
 const char *ks_error_string(int code);
@@ -156,6 +158,10 @@ int ks_bv_orthonormalize_column(ks_ctx ctx, ks_prec prec, int n, void *V, int ld
 int ks_status_reset(ks_ctx ctx);
 /* Copy the status to the host (synchronizes the stream). */
 int ks_status_get(ks_ctx ctx, ks_status *out);
+/* post: enqueue a copy of the status to pinned host memory; read: wait for the
+   stream, then return the posted copy (one wait instead of a copy round trip) */
+int ks_status_post(ks_ctx ctx);
+int ks_status_read(ks_ctx ctx, ks_status *out);
 
 /* x *= alpha (alpha==0 zeroes).  Replaces BVScale_BLAS_Private (bvblas.c:266-278). */
 int ks_bv_scale(ks_ctx ctx, ks_prec prec, int n, void *x, double alpha);

@@ -572,10 +572,18 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
       if (wave == 0) self += wi[q] * wi[q];
     }
   }
+  // lanes and workgroups combine in FP64; all CW sums of the wave by recursive halving
+  double t[CW];
 #pragma unroll
-  for (int u = 0; u < CW; u++) {
-    const double x = ks_wave_sum((double)acc[u]);   // lanes and workgroups combine in FP64
-    if (lane == 0 && c0 + u < k) partial[(size_t)(c0 + u) * nb + b] = x;
+  for (int u = 0; u < CW; u++) t[u] = (double)acc[u];
+  using MS = ks_multi_sum<CW>;
+  MS::run(t, lane);
+  if (MS::leader(lane)) {
+#pragma unroll
+    for (int i = 0; i < MS::R; i++) {
+      const int c = c0 + MS::index(lane, i);
+      if (c < k) partial[(size_t)c * nb + b] = t[i];
+    }
   }
   if (wave == 0) {
     const double x = ks_wave_sum((double)self);

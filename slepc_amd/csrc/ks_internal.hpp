@@ -75,6 +75,7 @@ struct ks_ctx_s {
   unsigned *d_counters = nullptr;  // [KS_MAXG + 1], zero between launches
   double *d_scalar = nullptr;      // small device scratch (norm results)
   double *h_scalar = nullptr;      // pinned host scratch
+  ks_status *h_status = nullptr;   // pinned mirror of d_status, filled by ks_status_post
   // profiling
   bool prof = false;
   unsigned prof_mask = ~0u;   // classes that get events while prof is on
@@ -195,6 +196,57 @@ __device__ __forceinline__ double ks_wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+
+// Wave sums of M values per lane by recursive halving: at each step the lanes
+// of an xor pair keep opposite halves of their values and add the partner's
+// copy of the half they keep, so M values cost about M + 3 log2(64) shuffles
+// instead of 6 M.  While M stays even the halving continues (offsets 32, 16,
+// ...); the values left are then summed over the remaining lane bits.  On
+// return v[i] of lane l holds the wave total of value ks_multi_sum<M>::index(l, i).
+// Fixed pairing and order, so the result does not depend on timing.
+template <int M, int OFF>
+struct ks_multi_halve {
+  // every index below is a compile-time constant, so v stays in registers
+  __device__ static __forceinline__ void run(double *v, int lane) {
+    if constexpr (M % 2 == 0 && OFF >= 1) {
+      constexpr int half = M / 2;
+      const bool up = (lane & OFF) != 0;
+#pragma unroll
+      for (int i = 0; i < half; i++) {
+        const double keep = up ? v[half + i] : v[i];
+        const double give = up ? v[i] : v[half + i];
+        v[i] = keep + __shfl_xor(give, OFF, 64);
+      }
+      ks_multi_halve<half, OFF / 2>::run(v, lane);
+    } else {
+#pragma unroll
+      for (int off = OFF; off > 0; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < M; i++) v[i] += __shfl_xor(v[i], off, 64);
+    }
+  }
+};
+
+template <int M>
+struct ks_multi_sum {
+  static constexpr int halvings(int m, int off) { return (m % 2 == 0 && off >= 1) ? 1 + halvings(m / 2, off / 2) : 0; }
+  static constexpr int H = halvings(M, 32);    // halving steps
+  static constexpr int R = M >> H;             // values left per lane
+  __device__ static __forceinline__ void run(double *v, int lane) { ks_multi_halve<M, 32>::run(v, lane); }
+  // which of the M values lane's v[i] holds after run()
+  __device__ static __forceinline__ int index(int lane, int i) {
+    int base = 0, m = M;
+#pragma unroll
+    for (int s = 0; s < H; s++) {
+      const int half = m >> 1;
+      if (lane & (32 >> s)) base += half;
+      m = half;
+    }
+    return base + i;
+  }
+  // the lane that stores: lowest lane of its group of equal totals
+  __device__ static __forceinline__ bool leader(int lane) { return (lane & ((64 >> H) - 1)) == 0; }
+};
 
 // Block-wide sum of one double; result valid in thread 0.  sh: >= KS_WAVES doubles.
 __device__ __forceinline__ double ks_block_sum(double v, double *sh) {

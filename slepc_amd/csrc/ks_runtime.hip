@@ -99,6 +99,7 @@ int ks_ctx_create(ks_ctx *out) {
   KS_HIP(hipMalloc(&c->d_counters, sizeof(unsigned) * (KS_MAXG + 1)));
   KS_HIP(hipMalloc(&c->d_scalar, sizeof(double) * 64));
   KS_HIP(hipHostMalloc(&c->h_scalar, sizeof(double) * 64, hipHostMallocDefault));
+  KS_HIP(hipHostMalloc((void **)&c->h_status, sizeof(ks_status), hipHostMallocDefault));
   int e = ks_memset_zero(c, c->d_counters, sizeof(unsigned) * (KS_MAXG + 1));
   if (e) return e;
   hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(64), 0, c->stream, c->d_status);
@@ -120,6 +121,7 @@ int ks_ctx_destroy(ks_ctx c) {
   (void)hipFree(c->d_scalar);
   if (c->d_ran) (void)hipFree(c->d_ran);
   (void)hipHostFree(c->h_scalar);
+  (void)hipHostFree(c->h_status);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return KS_OK;
@@ -274,6 +276,13 @@ int ks_memcpy_dtoh(ks_ctx c, void *dst, const void *src, size_t bytes) {
   return KS_OK;
 }
 
+int ks_memcpy_dtoh_async(ks_ctx c, void *dst, const void *src, size_t bytes) {
+  KS_CHECK(c && (dst || !bytes) && (src || !bytes), KS_ERR_ARG, "bad memcpy arguments");
+  if (!bytes) return KS_OK;
+  KS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  return KS_OK;
+}
+
 int ks_memcpy_dtod(ks_ctx c, void *dst, const void *src, size_t bytes) {
   KS_CHECK(c && (dst || !bytes) && (src || !bytes), KS_ERR_ARG, "bad memcpy arguments");
   if (!bytes) return KS_OK;
@@ -318,6 +327,21 @@ int ks_status_get(ks_ctx c, ks_status *out) {
   KS_HIP(hipMemcpyAsync(c->h_scalar, c->d_status, sizeof(ks_status), hipMemcpyDeviceToHost, c->stream));
   KS_HIP(hipStreamSynchronize(c->stream));
   memcpy(out, c->h_scalar, sizeof(ks_status));
+  return KS_OK;
+}
+
+// the status copied to pinned host memory in stream order (no wait here); a
+// later ks_status_read waits for the stream and returns that copy
+int ks_status_post(ks_ctx c) {
+  KS_CHECK(c, KS_ERR_ARG, "null context");
+  KS_HIP(hipMemcpyAsync(c->h_status, c->d_status, sizeof(ks_status), hipMemcpyDeviceToHost, c->stream));
+  return KS_OK;
+}
+
+int ks_status_read(ks_ctx c, ks_status *out) {
+  KS_CHECK(c && out, KS_ERR_ARG, "null argument");
+  KS_HIP(hipStreamSynchronize(c->stream));
+  memcpy(out, c->h_status, sizeof(ks_status));
   return KS_OK;
 }
 

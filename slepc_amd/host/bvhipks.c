@@ -171,15 +171,30 @@ static PetscBool GroupEligible(BV V, Mat A, PetscInt m)
   return (V->nc + m <= 128 && m <= V->N - 1) ? PETSC_TRUE : PETSC_FALSE;
 }
 
+/* the end of an expansion: its status and coefficient columns k+1..m go to
+   pinned host memory in stream order, so BVMatKrylovEnd waits once for the
+   stream instead of making two copy round trips through a busy device */
+PetscErrorCode BVHIPKSPostExpansion(BV V)
+{
+  BV_HIPKS *h = (BV_HIPKS *)V->data;
+  const PetscInt k = V->kry_k, m = V->kry_m;
+  KSCall(ks_status_post(V->ctx));
+  KSCall(ks_memcpy_dtoh_async(V->ctx, V->buffer + (size_t)(k + 1) * h->ncols, h->d_buf + (size_t)(k + 1) * h->ncols,
+                              sizeof(double) * (size_t)h->ncols * (size_t)(m - k)));
+  V->kry_posted = PETSC_TRUE;
+  return PETSC_SUCCESS;
+}
+
 static PetscErrorCode BVMatKrylovBegin_HIPKS(BV V, Mat A, PetscInt k, PetscInt m)
 {
   PetscBool gate = !strcmp(A->type, MATHIPCSR);
   Vec vj = NULL, vj1 = NULL;
   PetscErrorCode e = PETSC_SUCCESS;
   KSCall(ks_status_reset(V->ctx));
+  V->kry_posted = PETSC_FALSE;
+  V->kry_k = k;
   if (k < m && GroupEligible(V, A, m)) {
     V->kry_csr = ((Mat_HIPCSR *)A->data)->csr;
-    V->kry_k = k;
     V->kry_j = k;
     V->kry_m = m;
     V->kry_pending = PETSC_TRUE;
@@ -198,6 +213,7 @@ static PetscErrorCode BVMatKrylovBegin_HIPKS(BV V, Mat A, PetscInt k, PetscInt m
   }
   A->gate = PETSC_FALSE;
   V->kry_m = m;
+  if (!e && k < m) e = BVHIPKSPostExpansion(V);
   return e;
 }
 
@@ -234,7 +250,8 @@ static PetscErrorCode BVMatKrylovEnd_HIPKS(BV V, PetscInt k, PetscInt *m, PetscR
   ks_status st;
   PetscCheck(!V->kry_pending, PETSC_ERR_ORDER, "grouped expansion not fully enqueued");
   *m = V->kry_m;
-  KSCall(ks_status_get(V->ctx, &st));
+  if (V->kry_posted) KSCall(ks_status_read(V->ctx, &st));
+  else KSCall(ks_status_get(V->ctx, &st));
   PetscCall(StatusCheck(&st));
   V->gs_passes += st.total_passes;
   if (st.breakdown_col >= 0) {
@@ -242,9 +259,11 @@ static PetscErrorCode BVMatKrylovEnd_HIPKS(BV V, PetscInt k, PetscInt *m, PetscR
     *breakdown = PETSC_TRUE;
   } else *breakdown = PETSC_FALSE;
   *beta = st.beta_last;
-  /* new coefficient columns k+1..m into the host mirror */
-  KSCall(ks_memcpy_dtoh(V->ctx, V->buffer + (size_t)(k + 1) * h->ncols, h->d_buf + (size_t)(k + 1) * h->ncols,
-                        sizeof(double) * (size_t)h->ncols * (size_t)(*m - k)));
+  /* new coefficient columns k+1..m into the host mirror (already there when posted) */
+  if (!V->kry_posted || k != V->kry_k)
+    KSCall(ks_memcpy_dtoh(V->ctx, V->buffer + (size_t)(k + 1) * h->ncols, h->d_buf + (size_t)(k + 1) * h->ncols,
+                          sizeof(double) * (size_t)h->ncols * (size_t)(*m - k)));
+  V->kry_posted = PETSC_FALSE;
   return PETSC_SUCCESS;
 }
 
@@ -261,7 +280,7 @@ static PetscErrorCode BVDestroy_HIPKS(BV bv)
   KSCall(ks_host_free(h->h_Q));
   free(h);
   bv->data = NULL;
-  free(bv->buffer);
+  KSCall(ks_host_free(bv->buffer));
   bv->buffer = NULL;
   return PETSC_SUCCESS;
 }
@@ -280,7 +299,10 @@ static PetscErrorCode BVAllocate_HIPKS(BV bv)
   KSCall(ks_memset_zero(bv->ctx, h->d_buf, sizeof(double) * (size_t)h->ncols * (size_t)bv->m));
   KSCall(ks_malloc((void **)&h->d_q, sizeof(double) * (size_t)(h->ncols + 2)));
   free(bv->buffer);
-  PetscCall(PetscMallocZero(sizeof(double) * (size_t)h->ncols * (size_t)bv->m, &bv->buffer));
+  bv->buffer = NULL;
+  /* pinned, so the expansion's coefficient columns can be copied into it in stream order */
+  KSCall(ks_host_alloc((void **)&bv->buffer, sizeof(double) * (size_t)h->ncols * (size_t)bv->m));
+  memset(bv->buffer, 0, sizeof(double) * (size_t)h->ncols * (size_t)bv->m);
   return PETSC_SUCCESS;
 }
 

@@ -263,6 +263,8 @@ static PetscErrorCode GroupFeed(SlepcServer s, PetscBool *moved, PetscInt depth)
       if (fin) {   /* its stream picks up after the group's last step */
         KSGO(ks_ctx_wait_group(s->slot_ctx[q->slot], s->group));
         q->joined = PETSC_FALSE;
+        ierr = BVHIPKSPostExpansion(q->eps->V);
+        if (ierr) goto out;
       }
     }
   }

@@ -154,6 +154,9 @@ struct _p_BV {
   PetscBool grouped, kry_pending;
   PetscInt kry_k, kry_j;
   ks_csr kry_csr;
+  /* the status and the new coefficient columns were copied to pinned host
+     memory in stream order at the end of the expansion */
+  PetscBool kry_posted;
 };
 PetscErrorCode BVOrthogonalizeColumn_Safe(BV bv, PetscInt j, PetscReal *norm, PetscBool *lindep);
 PetscErrorCode BVMatKrylovBegin(BV V, Mat A, PetscInt k, PetscInt m);
@@ -162,6 +165,7 @@ PetscErrorCode BVCopyHessenberg(BV V, Mat H, PetscInt k, PetscInt m);
 PetscErrorCode BVCopyTridiagonal(BV V, Mat T, PetscInt k, PetscInt m);
 PetscErrorCode BVHIPKSGroupMember(BV V, ks_group_member *mem);
 PetscErrorCode BVHIPKSGroupAdvance(BV V, PetscBool *finished);
+PetscErrorCode BVHIPKSPostExpansion(BV V);
 
 /* ---------------------------------------------------------------- DS */
 typedef struct {
