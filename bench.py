@@ -526,6 +526,13 @@ def run_scale(args):
     for r, o in zip(reqs, res):
         log(f"[scale] served {r['kind']} n={r['n']}: its={o['its']} ngood={o['ngood']} latency {o['latency']:.2f}s")
     log(f"[scale] served: {wall:.2f}s, {good} verified pairs")
+    # the same nine requests served in plain FP64 (concurrency without the FP32 phase)
+    plain = [dict(r, mixed_tol=0.0) for r in reqs]
+    srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=len(plain), grouped=True)
+    res64, wall64 = S.run_stream(srv, plain, lambda r: r["_csr"], tol=1e-10)
+    srv.close()
+    good64 = sum(r["ngood"] for r in res64)
+    log(f"[scale] served plain FP64: {wall64:.2f}s, {good64} verified pairs")
     seq = None
     if not args.no_sequential:
         ts = time.perf_counter()
@@ -553,6 +560,9 @@ def run_scale(args):
                                           "nmatmult": o["nmatmult"], "latency_s": round(o["latency"], 3)}
                                          for r, o in zip(reqs, res)],
                             "peak_reserved_bytes": stats["peak_reserved"]},
+                 "served_plain_fp64": {"eigenpairs_per_s": round(good64 / wall64, 4), "wall_s": round(wall64, 3),
+                                       "verified_pairs": good64,
+                                       "its": [o["its"] for o in res64]},
                  "sequential_plain_fp64": seq,
                  "speedup_vs_sequential_fp64": round(rate / seq["eigenpairs_per_s"], 3) if seq else None})
     out["config"].update({"rmat_scale": args.rmat_scale, "rmat_n": n5, "rmat_nnz": int(rmat[0][-1]),

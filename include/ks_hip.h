@@ -42,6 +42,7 @@ typedef struct ks_ctx_s *ks_ctx;
 typedef struct ks_csr_s *ks_csr;
 typedef struct ks_pool_s *ks_pool;
 typedef struct ks_group_s *ks_group;
+typedef struct ks_graph_s *ks_graph;
 
 /* Device-side state of the fused CGS+DGKS orthonormalization (the scalars of
    BVOrthogonalizeGS, src/sys/classes/bv/interface/bvorthog.c:145-217, kept on
@@ -236,6 +237,21 @@ int ks_group_inflight(ks_group group, int *steps);
 /* the group's stream waits for the work enqueued so far on ctx, and the reverse */
 int ks_group_wait_ctx(ks_group group, ks_ctx ctx);
 int ks_ctx_wait_group(ks_ctx ctx, ks_group group);
+
+/* ---------------------------------------------------------------- captured launch sequences
+ * Everything enqueued on the context's stream between ks_graph_begin and
+ * ks_graph_end is recorded (hipStreamBeginCapture) instead of run;
+ * ks_graph_launch replays it on the context stream with one call.  A graph
+ * keeps the arguments it was recorded with, so it is valid only while every
+ * buffer it touches stays allocated where it was; it records no memset nodes
+ * (fills are kernels).  ks_graph_begin fails with KS_ERR_ARG while the
+ * context's per-kernel events are on.  No counterpart in the reference: the
+ * replay of one Arnoldi step (SpMV + CGS/DGKS + scaling) for small problems,
+ * where launch overhead rivals the kernels. */
+int ks_graph_begin(ks_ctx ctx);
+int ks_graph_end(ks_ctx ctx, ks_graph *graph);   /* *graph = NULL when the capture failed */
+int ks_graph_launch(ks_ctx ctx, ks_graph graph);
+int ks_graph_destroy(ks_graph graph);
 
 #ifdef __cplusplus
 }

@@ -149,6 +149,11 @@ PetscErrorCode BVSetType(BV bv, BVType type);
 PetscErrorCode BVGetType(BV bv, BVType *type);
 PetscErrorCode BVHIPKSSetPrecision(BV bv, SlepcAMDPrecision prec);
 PetscErrorCode BVHIPKSGetPrecision(BV bv, SlepcAMDPrecision *prec);
+/* capture each Arnoldi step (SpMV, CGS/DGKS passes, scaling) of an expansion
+   once per column as a hipGraph and replay it in later restarts; meant for
+   small problems, where launch overhead rivals the kernels (no counterpart in
+   the reference) */
+PetscErrorCode BVHIPKSSetStepGraphs(BV bv, PetscBool flg);
 PetscErrorCode BVSetOrthogonalization(BV bv, BVOrthogType type, BVOrthogRefineType refine, PetscReal eta, int block);
 PetscErrorCode BVSetActiveColumns(BV bv, PetscInt l, PetscInt k);
 PetscErrorCode BVGetActiveColumns(BV bv, PetscInt *l, PetscInt *k);

@@ -80,6 +80,7 @@ struct ks_ctx_s {
   bool prof = false;
   unsigned prof_mask = ~0u;   // classes that get events while prof is on
   bool prof_open = false;     // the last ks_prof_begin recorded its start event
+  bool capturing = false;     // between ks_graph_begin and ks_graph_end
   std::vector<ks_prof_rec> recs;
   std::vector<hipEvent_t> event_pool;
   int *d_ran = nullptr;            // per-launch "did work" flags

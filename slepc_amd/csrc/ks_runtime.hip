@@ -345,6 +345,61 @@ int ks_status_read(ks_ctx c, ks_status *out) {
   return KS_OK;
 }
 
+// ---------------------------------------------------------------- captured launch sequences
+struct ks_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int ks_graph_begin(ks_ctx c) {
+  KS_CHECK(c, KS_ERR_ARG, "null context");
+  KS_CHECK(!c->capturing, KS_ERR_ARG, "capture already open on this context");
+  KS_CHECK(!c->prof, KS_ERR_ARG, "no capture while per-kernel events are on");
+  KS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  return KS_OK;
+}
+
+int ks_graph_end(ks_ctx c, ks_graph *out) {
+  KS_CHECK(c && out, KS_ERR_ARG, "null argument");
+  KS_CHECK(c->capturing, KS_ERR_ARG, "no capture open on this context");
+  *out = nullptr;
+  c->capturing = false;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (e != hipSuccess || !g) {
+    (void)hipGetLastError();
+    if (g) (void)hipGraphDestroy(g);
+    fprintf(stderr, "ks_hip: hipStreamEndCapture: %s\n", hipGetErrorString(e));
+    return KS_ERR_HIP;
+  }
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipGraphDestroy(g);
+    fprintf(stderr, "ks_hip: hipGraphInstantiate: %s\n", hipGetErrorString(e));
+    return KS_ERR_HIP;
+  }
+  *out = new ks_graph_s{g, x};
+  return KS_OK;
+}
+
+int ks_graph_launch(ks_ctx c, ks_graph g) {
+  KS_CHECK(c && g, KS_ERR_ARG, "null argument");
+  KS_CHECK(!c->capturing, KS_ERR_ARG, "graph launch inside a capture");
+  KS_HIP(hipGraphLaunch(g->exec, c->stream));
+  return KS_OK;
+}
+
+int ks_graph_destroy(ks_graph g) {
+  if (!g) return KS_OK;
+  (void)hipGraphExecDestroy(g->exec);
+  (void)hipGraphDestroy(g->graph);
+  delete g;
+  return KS_OK;
+}
+
 // ---------------------------------------------------------------- profiling
 static const char *g_prof_names[KS_PROF_NCLASS] = {
     "spmv", "gs_dot_first", "gs_update_first", "gs_finish", "gs_update_refine", "scale",
@@ -367,7 +422,7 @@ static hipEvent_t pool_get(ks_ctx c) {
 
 int *ks_prof_begin(ks_ctx c, int cls, double bytes, double flops, bool gated) {
   c->prof_open = false;
-  if (!c->prof || !((c->prof_mask >> cls) & 1u)) return nullptr;
+  if (!c->prof || c->capturing || !((c->prof_mask >> cls) & 1u)) return nullptr;
   if (c->recs.size() >= 65536 || (gated && c->ran_used >= c->ran_cap)) ks_prof_collect(c);
   ks_prof_rec r;
   r.cls = cls;
@@ -390,6 +445,7 @@ void ks_prof_end(ks_ctx c) {
 
 int ks_prof_enable(ks_ctx c, int on) {
   KS_CHECK(c, KS_ERR_ARG, "null context");
+  KS_CHECK(!(on && c->capturing), KS_ERR_ARG, "no per-kernel events inside a capture");
   if (on && !c->d_ran) {
     c->ran_cap = 65536;
     KS_HIP(hipMalloc(&c->d_ran, sizeof(int) * c->ran_cap));

@@ -248,6 +248,10 @@ class EPS:
     def setBVPrecision(self, prec: int):
         _chk(lib().BVHIPKSSetPrecision(self._bv(), C.c_int(prec)), "BVHIPKSSetPrecision")
 
+    def setStepGraphs(self, flg: bool = True):
+        """Replay each Arnoldi step from a hipGraph captured once per column."""
+        _chk(lib().BVHIPKSSetStepGraphs(self._bv(), C.c_int(int(flg))), "BVHIPKSSetStepGraphs")
+
     def setOrthogonalization(self, refine=0, eta=0.7071):
         _chk(lib().BVSetOrthogonalization(self._bv(), C.c_int(0), C.c_int(refine), C.c_double(eta), C.c_int(0)),
              "BVSetOrthogonalization")
@@ -332,7 +336,7 @@ class EPS:
 
 
 def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=True, v0=None, max_it=-1,
-          prec=F64, deflation=None, lock=True, keep=0.5, want_vectors=True, mixed_tol=0.0):
+          prec=F64, deflation=None, lock=True, keep=0.5, want_vectors=True, mixed_tol=0.0, step_graphs=False):
     """One-call convenience wrapper: returns a dict like the oracle's eps_solve.
     FP32 values in csr[2] keep the operator in FP32; ``prec`` is the basis precision."""
     A = Mat(csr, prec=F32 if np.asarray(csr[2]).dtype == np.float32 else F64)
@@ -348,6 +352,8 @@ def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=Tru
         eps.setMixedPrecision(mixed_tol)
     if prec != F64:
         eps.setBVPrecision(prec)
+    if step_graphs:
+        eps.setStepGraphs(True)
     if v0 is not None:
         eps.setInitialSpace([v0])
     if deflation is not None:

@@ -27,7 +27,22 @@ typedef struct {
   double *h_Q;           /* pinned host staging of Q */
   size_t Qcap;           /* elements of d_Q / h_Q */
   PetscInt ncols;        /* nc + m columns allocated */
+  /* captured Arnoldi steps: step[j] makes column j+1 from column j; valid for
+     the matrix step_csr and the orthogonalization settings they were made with */
+  ks_graph *step;
+  ks_csr step_csr;
+  int step_refine;
+  double step_eta;
+  PetscBool step_failed;   /* a capture failed: steps run as plain launches */
 } BV_HIPKS;
+
+static void StepGraphsClear(BV_HIPKS *h, PetscInt m)
+{
+  if (!h->step) return;
+  for (PetscInt j = 0; j < m; j++) (void)ks_graph_destroy(h->step[j]);
+  free(h->step);
+  h->step = NULL;
+}
 
 static ks_prec kp(BV bv) { return bv->prec == SLEPC_AMD_F64 ? KS_F64 : KS_F32; }
 static size_t esz(BV bv) { return SlepcPrecSize(bv->prec); }
@@ -185,10 +200,62 @@ PetscErrorCode BVHIPKSPostExpansion(BV V)
   return PETSC_SUCCESS;
 }
 
+/* one Arnoldi step: column j+1 = A column j, then its CGS+DGKS and scaling */
+static PetscErrorCode ArnoldiStep(BV V, Mat A, PetscInt j)
+{
+  Vec vj = NULL, vj1 = NULL;
+  PetscErrorCode e;
+  PetscCall(BVGetColumn(V, j, &vj));
+  PetscCall(BVGetColumn(V, j + 1, &vj1));
+  e = MatMult(A, vj, vj1);
+  PetscCall(BVRestoreColumn(V, j, &vj));
+  PetscCall(BVRestoreColumn(V, j + 1, &vj1));
+  if (e) return e;
+  if (j == V->N - 1) return BVOrthonormalizeColumn_HIPKS(V, j + 1, PETSC_FALSE, PETSC_TRUE, PETSC_TRUE, NULL, NULL);
+  return BVOrthonormalizeColumn_HIPKS(V, j + 1, PETSC_TRUE, PETSC_FALSE, PETSC_TRUE, NULL, NULL);
+}
+
+/* the same step replayed from a graph captured the first time column j is
+   expanded (with this matrix and these orthogonalization settings); the
+   step's launches depend on j only, and the status gates make the replay
+   follow each restart's own DGKS decisions */
+static PetscErrorCode ArnoldiStepGraph(BV V, Mat A, ks_csr csr, PetscInt j, PetscBool *done)
+{
+  BV_HIPKS *h = (BV_HIPKS *)V->data;
+  ks_graph g = NULL;
+  PetscErrorCode e;
+  int ce;
+  *done = PETSC_FALSE;
+  if (h->step && (h->step_csr != csr || h->step_refine != (int)V->orthog_ref || h->step_eta != V->orthog_eta))
+    StepGraphsClear(h, V->m);
+  if (!h->step) {
+    h->step = (ks_graph *)calloc((size_t)V->m, sizeof(ks_graph));
+    if (!h->step) return PETSC_SUCCESS;
+    h->step_csr = csr;
+    h->step_refine = (int)V->orthog_ref;
+    h->step_eta = V->orthog_eta;
+  }
+  if (!h->step[j]) {
+    if (ks_graph_begin(V->ctx)) { h->step_failed = PETSC_TRUE; return PETSC_SUCCESS; }   /* events on: plain launches */
+    e = ArnoldiStep(V, A, j);
+    ce = ks_graph_end(V->ctx, &g);
+    if (e || ce || !g) {   /* nothing of the step ran: it is redone with plain launches */
+      (void)ks_graph_destroy(g);
+      h->step_failed = PETSC_TRUE;
+      return PETSC_SUCCESS;
+    }
+    h->step[j] = g;
+  }
+  KSCall(ks_graph_launch(V->ctx, h->step[j]));
+  *done = PETSC_TRUE;
+  return PETSC_SUCCESS;
+}
+
 static PetscErrorCode BVMatKrylovBegin_HIPKS(BV V, Mat A, PetscInt k, PetscInt m)
 {
   PetscBool gate = !strcmp(A->type, MATHIPCSR);
-  Vec vj = NULL, vj1 = NULL;
+  BV_HIPKS *h = (BV_HIPKS *)V->data;
+  ks_csr csr = NULL;
   PetscErrorCode e = PETSC_SUCCESS;
   KSCall(ks_status_reset(V->ctx));
   V->kry_posted = PETSC_FALSE;
@@ -201,15 +268,16 @@ static PetscErrorCode BVMatKrylovBegin_HIPKS(BV V, Mat A, PetscInt k, PetscInt m
     return PETSC_SUCCESS;
   }
   if (gate) A->gate = PETSC_TRUE;
+  /* graphs: a hipcsr matrix whose SpMV runs on this basis' stream, its FP32
+     copy made beforehand (no allocation inside a capture) */
+  if (V->step_graphs && !h->step_failed && gate && SlepcAMDContext() == V->ctx) {
+    if (V->prec == SLEPC_AMD_F32) e = MatHIPCSRGetF32(A, &csr);
+    else csr = ((Mat_HIPCSR *)A->data)->csr;
+  }
   for (PetscInt j = k; j < m && !e; j++) {
-    PetscCall(BVGetColumn(V, j, &vj));
-    PetscCall(BVGetColumn(V, j + 1, &vj1));
-    e = MatMult(A, vj, vj1);
-    PetscCall(BVRestoreColumn(V, j, &vj));
-    PetscCall(BVRestoreColumn(V, j + 1, &vj1));
-    if (e) break;
-    if (j == V->N - 1) e = BVOrthonormalizeColumn_HIPKS(V, j + 1, PETSC_FALSE, PETSC_TRUE, PETSC_TRUE, NULL, NULL);
-    else e = BVOrthonormalizeColumn_HIPKS(V, j + 1, PETSC_TRUE, PETSC_FALSE, PETSC_TRUE, NULL, NULL);
+    PetscBool done = PETSC_FALSE;
+    if (csr && !h->step_failed && j != V->N - 1) e = ArnoldiStepGraph(V, A, csr, j, &done);
+    if (!e && !done) e = ArnoldiStep(V, A, j);
   }
   A->gate = PETSC_FALSE;
   V->kry_m = m;
@@ -272,6 +340,7 @@ static PetscErrorCode BVDestroy_HIPKS(BV bv)
   BV_HIPKS *h = (BV_HIPKS *)bv->data;
   if (!h) return PETSC_SUCCESS;
   KSCall(ks_ctx_sync(bv->ctx));
+  StepGraphsClear(h, bv->m);
   for (int l = 0; l < 2; l++) if (bv->cv[l]) PetscCall(VecDestroy(&bv->cv[l]));
   KSCall(ks_free(h->dV));
   KSCall(ks_free(h->d_buf));
@@ -332,6 +401,14 @@ PetscErrorCode BVHIPKSSetPrecision(BV bv, SlepcAMDPrecision prec)
   PetscCheck(bv, PETSC_ERR_ARG_NULL, "null BV");
   PetscCheck(!bv->data, PETSC_ERR_ORDER, "precision must be set before the storage is allocated");
   bv->prec = prec;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode BVHIPKSSetStepGraphs(BV bv, PetscBool flg)
+{
+  PetscCheck(bv, PETSC_ERR_ARG_NULL, "null BV");
+  bv->step_graphs = flg;
+  if (!flg && bv->data) StepGraphsClear((BV_HIPKS *)bv->data, bv->m);
   return PETSC_SUCCESS;
 }
 

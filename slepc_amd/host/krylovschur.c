@@ -224,6 +224,7 @@ static PetscErrorCode MixedBegin(EPS eps)
   bv32->orthog_ref = eps->V->orthog_ref;
   bv32->orthog_eta = eps->V->orthog_eta;
   bv32->rseed = eps->V->rseed;
+  bv32->step_graphs = eps->V->step_graphs;
   /* same start: the FP64 initial vector inserted by EPSSetUp, rounded to FP32 */
   if (eps->nini > 0) {
     PetscCall(VecCreateHIP(eps->n, SLEPC_AMD_F32, &v64));

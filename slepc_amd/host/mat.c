@@ -137,7 +137,7 @@ PetscErrorCode MatDenseGetLDA(Mat A, PetscInt *lda)
 /* ---------------------------------------------------------------- MATHIPCSR (device) */
 /* FP32 copy of FP64 values, built on the device the first time an FP32 vector
    is multiplied (the mixed-precision phase reads half the matrix bytes) */
-static PetscErrorCode MatHIPCSRGetF32(Mat A, ks_csr *out)
+PetscErrorCode MatHIPCSRGetF32(Mat A, ks_csr *out)
 {
   Mat_HIPCSR *h = (Mat_HIPCSR *)A->data;
   if (h->vprec == SLEPC_AMD_F32) { *out = h->csr; return PETSC_SUCCESS; }

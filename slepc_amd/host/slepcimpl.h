@@ -99,6 +99,8 @@ typedef struct {         /* MATHIPCSR */
   ks_csr csr32;          /* FP32 copy of FP64 values for FP32 vectors (built on demand) */
   SlepcAMDPrecision vprec;
 } Mat_HIPCSR;
+/* the FP32-valued copy of a hipcsr matrix, made on first use */
+PetscErrorCode MatHIPCSRGetF32(Mat A, ks_csr *out);
 PetscErrorCode MatCreateSeqDenseView(PetscInt m, PetscInt n, PetscScalar *data, PetscInt lda, Mat *A);
 
 /* ---------------------------------------------------------------- BV */
@@ -157,6 +159,8 @@ struct _p_BV {
   /* the status and the new coefficient columns were copied to pinned host
      memory in stream order at the end of the expansion */
   PetscBool kry_posted;
+  /* replay each Arnoldi step of an expansion from a captured graph */
+  PetscBool step_graphs;
 };
 PetscErrorCode BVOrthogonalizeColumn_Safe(BV bv, PetscInt j, PetscReal *norm, PetscBool *lindep);
 PetscErrorCode BVMatKrylovBegin(BV V, Mat A, PetscInt k, PetscInt m);

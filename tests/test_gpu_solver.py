@@ -193,3 +193,38 @@ def test_full_basis_safe_path(E):
     closed = np.sort(2 - 2 * np.cos(np.arange(1, n + 1) * np.pi / (n + 1)))[::-1]
     assert r["nconv"] >= 4
     assert np.allclose(r["eigr"][:4], closed[:4], rtol=1e-10)
+
+
+@pytest.mark.parametrize("mixed", [0.0, 1e-5])
+def test_step_graphs_replay_the_same_solve(E, mixed):
+    # each Arnoldi step captured once per column as a hipGraph and replayed in the
+    # later restarts runs the same kernels with the same arguments: the solve is
+    # identical to the one with plain launches (restarts, passes, eigenvalues to
+    # the last bit), FP64 and mixed precision (FP32 basis phase) alike
+    csr = M.laplacian_2d(100)
+    v0 = M.uniform_vector(0, 10000)
+    a = E.solve(csr, nev=10, ncv=32, tol=1e-10, v0=v0, want_vectors=False, mixed_tol=mixed)
+    b = E.solve(csr, nev=10, ncv=32, tol=1e-10, v0=v0, want_vectors=False, mixed_tol=mixed, step_graphs=True)
+    assert a["its"] > 2 and a["its"] == b["its"] and a["gs_passes"] == b["gs_passes"]
+    assert np.array_equal(a["eigr"], b["eigr"])
+    assert (b["err"][:10] <= 1e-10).all()
+    # the same solver object given another matrix of the same shape (2A) must not
+    # replay the first matrix's graphs: its eigenvalues are exactly twice a's
+    A1, A2 = E.Mat(csr), E.Mat((csr[0], csr[1], 2.0 * csr[2]))
+    eps = E.EPS()
+    eps.setProblemType(E.EPS.HEP)
+    eps.setDimensions(10, 32)
+    eps.setTolerances(1e-10)
+    eps.setWhichEigenpairs(E.EPS.LARGEST_MAGNITUDE)
+    if mixed > 0:
+        eps.setMixedPrecision(mixed)
+    vals = []
+    for A in (A1, A2):
+        eps.setOperators(A)
+        eps.setStepGraphs(True)
+        eps.setInitialSpace([v0])
+        eps.solve()
+        vals.append(np.array([np.real(eps.getEigenvalue(i)) for i in range(10)]))
+    eps.destroy()
+    assert np.array_equal(vals[0], a["eigr"][:10])
+    assert np.allclose(vals[1], 2.0 * a["eigr"][:10], rtol=1e-12, atol=0)
