@@ -486,19 +486,20 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, lo
 // 8-byte aligned columns): each lane takes two adjacent rows with one 8-byte
 // load per column, so a wave still moves 512 contiguous bytes per column per
 // step, as with FP64 data.
-template <typename T, int CW, bool FUSED, bool PAIR>
+template <typename T, int CW, bool FUSED, int RPL, bool PAIR>
 __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restrict__ V, long ld, int k,
                                                T *__restrict__ w, int n, int rpb, const double *__restrict__ cbuf,
                                                double *__restrict__ partial) {
-  static_assert(!PAIR || sizeof(T) == 4, "row pairs are for FP32 data");
+  static_assert(!PAIR || (sizeof(T) == 4 && RPL == 2), "row pairs are two adjacent FP32 rows");
   using AT = typename acc_t<T>::type;   // FP32 data: FP32 products and per-thread sums
-  constexpr int RPL = PAIR ? 2 : 1;     // rows per lane per step
-  constexpr int STEP = 64 * RPL;
+  constexpr int STEP = 64 * RPL;        // rows per step; each lane takes RPL of them
   __shared__ AT coef[KS_WAVES * CW];
   __shared__ AT red[2][KS_WAVES][STEP];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: column guards are scalar branches
   const int c0 = wave * CW;
+  // this lane's q-th row of a step: adjacent pairs (one 8-byte load) or 64 apart
+  auto loff = [&](int q) { return PAIR ? RPL * lane + q : 64 * q + lane; };
   if (FUSED)
     for (int c = tid; c < KS_WAVES * CW; c += KS_BLOCK) coef[c] = (AT)(c < k ? cbuf[c] : 0.0);
   __syncthreads();
@@ -510,22 +511,22 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
   const unsigned r1 = min((unsigned)n, r0 + (unsigned)rpb);
   int buf = 0;
   for (unsigned i0 = r0; i0 < r1; i0 += STEP) {
-    const unsigned ia = i0 + RPL * lane;   // this lane's first row
     T v[RPL][CW];
     AT wi[RPL];
     if (i0 + STEP <= r1) {
       // every lane's rows exist (all steps but the last workgroup's last): no
-      // per-lane guards, so the CW loads are issued back to back
+      // per-lane guards, so the CW * RPL loads are issued back to back
 #pragma unroll
       for (int u = 0; u < CW; u++) {
         const T *col = V + (size_t)(c0 + u) * ld;
         if (c0 + u < k) {
           if (PAIR) {
-            const float2 p2 = *(const float2 *)(col + ia);
+            const float2 p2 = *(const float2 *)(col + i0 + loff(0));
             v[0][u] = p2.x;
             v[RPL - 1][u] = p2.y;
           } else {
-            v[0][u] = col[ia];
+#pragma unroll
+            for (int q = 0; q < RPL; q++) v[q][u] = col[i0 + loff(q)];
           }
         } else {
 #pragma unroll
@@ -533,17 +534,17 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
         }
       }
 #pragma unroll
-      for (int q = 0; q < RPL; q++) wi[q] = (AT)w[ia + q];
+      for (int q = 0; q < RPL; q++) wi[q] = (AT)w[i0 + loff(q)];
     } else {
 #pragma unroll
       for (int u = 0; u < CW; u++) {
         const T *col = V + (size_t)(c0 + u) * ld;
         const bool have = c0 + u < k;
 #pragma unroll
-        for (int q = 0; q < RPL; q++) v[q][u] = (have && ia + q < r1) ? col[ia + q] : (T)0;
+        for (int q = 0; q < RPL; q++) v[q][u] = (have && i0 + loff(q) < r1) ? col[i0 + loff(q)] : (T)0;
       }
 #pragma unroll
-      for (int q = 0; q < RPL; q++) wi[q] = ia + q < r1 ? (AT)w[ia + q] : (AT)0;
+      for (int q = 0; q < RPL; q++) wi[q] = i0 + loff(q) < r1 ? (AT)w[i0 + loff(q)] : (AT)0;
     }
     if (FUSED) {
 #pragma unroll
@@ -551,13 +552,13 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
         AT s = (AT)0;
 #pragma unroll
         for (int u = 0; u < CW; u++) s += (AT)v[q][u] * coef[c0 + u];
-        red[buf][wave][RPL * lane + q] = s;
+        red[buf][wave][loff(q)] = s;
       }
       __syncthreads();
 #pragma unroll
       for (int q = 0; q < RPL; q++) {
-        const int r = RPL * lane + q;
-        const unsigned i = ia + q;
+        const int r = loff(q);
+        const unsigned i = i0 + r;
         const AT tot = ((red[buf][0][r] + red[buf][1][r]) + red[buf][2][r]) + red[buf][3][r];
         const T t = (T)(wi[q] - tot);
         wi[q] = i < r1 ? (AT)t : (AT)0;
@@ -591,14 +592,14 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
   }
 }
 
-template <typename T, int CW, bool FUSED, bool PAIR>
+template <typename T, int CW, bool FUSED, int RPL, bool PAIR>
 __global__ __launch_bounds__(KS_BLOCK) void k_gs_split(const T *__restrict__ V, long ld, int k, T *__restrict__ w,
                                                       int n, int rpb, const double *__restrict__ cbuf,
                                                       double *__restrict__ partial, const ks_status *st, int col,
                                                       int pass, int *ran) {
   if (!gs_pass_gate<FUSED>(st, col, pass)) return;
   if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = 1;
-  gs_split_block<T, CW, FUSED, PAIR>(blockIdx.x, gridDim.x, V, ld, k, w, n, rpb, cbuf, partial);
+  gs_split_block<T, CW, FUSED, RPL, PAIR>(blockIdx.x, gridDim.x, V, ld, k, w, n, rpb, cbuf, partial);
 }
 
 // The finish of a Gram-Schmidt pass: a separate launch, ordered by the stream
@@ -682,7 +683,7 @@ __global__ __launch_bounds__(KS_BLOCK) void k_grp_gs(ks_grp_gs g, int pass) {
   const int m = grp_member(g.b0_gs, g.nmem, blockIdx.x);
   const ks_grp_gs_member &M = g.m[m];
   if (!gs_pass_gate<FUSED>(M.st, M.col, pass)) return;
-  gs_split_block<double, CW, FUSED, false>((int)blockIdx.x - g.b0_gs[m], M.nb, M.V, M.ld, M.k, M.w, M.n, M.rpb,
+  gs_split_block<double, CW, FUSED, 1, false>((int)blockIdx.x - g.b0_gs[m], M.nb, M.V, M.ld, M.k, M.w, M.n, M.rpb,
                                            M.cbuf, M.partial);
 }
 
@@ -930,12 +931,14 @@ struct gs_geom {
   int kc;   // > 0: register-resident kernel for up to kc columns
   int cw;   // > 0: column-split kernel, cw columns per wavefront
   int pair; // column-split on FP32 data with two rows per lane (even ld, aligned)
+  int rpl;  // column-split on FP64 data: rows per lane per step (2: two loads 64 rows apart)
 };
 
 int gs_geometry(int n, int k, size_t esize, gs_geom *g) {
   g->kc = 0;
   g->cw = 0;
   g->pair = 0;
+  g->rpl = 1;
   if (k <= KS_GS_REGMAX && (uint64_t)n * esize < (1ull << 32)) {
     g->kc = k <= 8 ? 8 : (k + 7) / 8 * 8;
     const long tiles = (n + (long)KS_BLOCK - 1) / KS_BLOCK;            // 256-row tiles
@@ -953,7 +956,10 @@ int gs_geometry(int n, int k, size_t esize, gs_geom *g) {
     static const int cws[] = {2, 4, 8, 12, 16, 24, 32};
     for (int c : cws)
       if (per <= c) { g->cw = c; break; }
-    const long step = esize == 4 ? 128 : 64;                            // rows per workgroup step, even
+    // FP64 with few columns per wave: two rows per lane per step, so a wave
+    // keeps 2 CW loads in flight (FP32 takes two rows per lane as a pair)
+    if (esize == 8 && g->cw <= KS_GS_RPL2_MAXCW) g->rpl = 2;
+    const long step = esize == 4 ? 128 : 64 * g->rpl;                   // rows per workgroup step, even
     const long tiles = (n + step - 1) / step;
     const long nb = tiles < KS_GS_SPLIT_BLOCKS ? tiles : KS_GS_SPLIT_BLOCKS;
     const long tpb = (tiles + nb - 1) / nb;
@@ -993,10 +999,13 @@ void launch_gs(ks_ctx ctx, const gs_geom &g, const T *V, long ld, int k, T *w, i
 #define KS_GSS_CASE(CC)                                                                                           \
   case CC:                                                                                                        \
     if (g.pair)                                                                                                   \
-      hipLaunchKernelGGL((k_gs_split<T, CC, FUSED, sizeof(T) == 4>), grid, block, 0, ctx->stream, V, ld, k, w, n,  \
-                         g.cpb, cbuf, ctx->d_partial, ctx->d_status, col, pass, ran);                             \
+      hipLaunchKernelGGL((k_gs_split<T, CC, FUSED, 2, sizeof(T) == 4>), grid, block, 0, ctx->stream, V, ld, k, w,  \
+                         n, g.cpb, cbuf, ctx->d_partial, ctx->d_status, col, pass, ran);                          \
+    else if (sizeof(T) == 8 && g.rpl == 2)                                                                        \
+      hipLaunchKernelGGL((k_gs_split<T, CC, FUSED, 2, false>), grid, block, 0, ctx->stream, V, ld, k, w, n, g.cpb, \
+                         cbuf, ctx->d_partial, ctx->d_status, col, pass, ran);                                    \
     else                                                                                                          \
-      hipLaunchKernelGGL((k_gs_split<T, CC, FUSED, false>), grid, block, 0, ctx->stream, V, ld, k, w, n, g.cpb,    \
+      hipLaunchKernelGGL((k_gs_split<T, CC, FUSED, 1, false>), grid, block, 0, ctx->stream, V, ld, k, w, n, g.cpb, \
                          cbuf, ctx->d_partial, ctx->d_status, col, pass, ran);                                    \
     break;
     switch (g.cw) { KS_GSS_CASE(2) KS_GSS_CASE(4) KS_GSS_CASE(8) KS_GSS_CASE(12) KS_GSS_CASE(16) KS_GSS_CASE(24)

@@ -19,6 +19,9 @@
 #ifndef KS_GS_SPLIT_BLOCKS
 #define KS_GS_SPLIT_BLOCKS 4096      // workgroups of the column-split Gram-Schmidt
 #endif
+#ifndef KS_GS_RPL2_MAXCW
+#define KS_GS_RPL2_MAXCW 12          // FP64 column-split sweeps with <= this many columns per wave: 2 rows per lane
+#endif
 #ifndef KS_GS_REGMAX
 #define KS_GS_REGMAX 16              // Gram-Schmidt against <= this many columns: register-resident kernel
 #endif
