@@ -17,7 +17,7 @@
 #define KS_GS_SPLITMAX 128           // Gram-Schmidt against <= this many columns: column-split kernel
 #endif
 #ifndef KS_GS_SPLIT_BLOCKS
-#define KS_GS_SPLIT_BLOCKS 4096      // workgroups of the column-split Gram-Schmidt
+#define KS_GS_SPLIT_BLOCKS 6144      // workgroups of the column-split Gram-Schmidt
 #endif
 #ifndef KS_GS_RPL2_MAXCW
 #define KS_GS_RPL2_MAXCW 12          // FP64 column-split sweeps with <= this many columns per wave: 2 rows per lane
