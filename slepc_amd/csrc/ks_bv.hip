@@ -441,6 +441,9 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, lo
       v[c] = *(const T *)(colbase + off);
     }
     AT wi = (AT)w[i];
+    // keep the KC + 1 loads ahead of every use: left alone the scheduler
+    // interleaves them with the accumulations and only a few stay in flight
+    __builtin_amdgcn_sched_barrier(0);
     if (FUSED) {
       AT s0 = (AT)0, s1 = (AT)0;
 #pragma unroll
@@ -535,6 +538,7 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
       }
 #pragma unroll
       for (int q = 0; q < RPL; q++) wi[q] = (AT)w[i0 + loff(q)];
+      __builtin_amdgcn_sched_barrier(0);   // all loads of the step before any use
     } else {
 #pragma unroll
       for (int u = 0; u < CW; u++) {
@@ -611,23 +615,31 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_split(const T *__restrict__ V, 
 // the last workgroup to finish -- counted with one device-scope atomic after
 // a fence, cheap in this kernel since it has written only k+1 sums -- takes
 // the DGKS step on them.  The counter resets itself.
-template <bool FUSED>
+template <bool FUSED, int NT>
 __device__ __forceinline__ void gs_fin_block(int b, const double *__restrict__ partial, int nb, int k, double *sums,
                                              unsigned *counter, double *cbuf, double *hcol, ks_status *st, int col,
                                              int refine, double eta, int do_scale, int safe, double deftol) {
-  __shared__ double red[KS_WAVES];
+  constexpr int NW = NT / 64;
+  __shared__ double red[NW];
   __shared__ double fin[KS_NCOLP];
   __shared__ unsigned ticket;
   const double *p = partial + (size_t)b * nb;
   double a[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tail = 0.0;
   int q = threadIdx.x;
-  for (; q + 7 * KS_BLOCK < nb; q += 8 * KS_BLOCK) {
+  for (; q + 7 * NT < nb; q += 8 * NT) {
 #pragma unroll
-    for (int u = 0; u < 8; u++) a[u] += p[q + u * KS_BLOCK];
+    for (int u = 0; u < 8; u++) a[u] += p[q + u * NT];
   }
-  for (; q < nb; q += KS_BLOCK) tail += p[q];
-  const double s = ks_block_sum((((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]))) + tail, red);
-  if (threadIdx.x == 0) sums[b] = s;
+  for (; q < nb; q += NT) tail += p[q];
+  double v = ks_wave_sum((((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]))) + tail);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) s += red[w];
+    sums[b] = s;
+  }
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) ticket = atomicAdd(counter, 1u);
@@ -640,14 +652,17 @@ __device__ __forceinline__ void gs_fin_block(int b, const double *__restrict__ p
   gs_finish<FUSED>(fin, k, cbuf, hcol, st, col, refine, eta, do_scale, safe, deftol);
 }
 
+// single solves: 1024 threads per column, so the (up to 8192) partial sums of
+// a column are read in one round of eight loads per thread
+#define KS_FIN_BLOCK 1024
 template <bool FUSED>
-__global__ __launch_bounds__(KS_BLOCK) void k_gs_fin(const double *__restrict__ partial, int nb, int k,
-                                                    double *sums, unsigned *counter, double *cbuf, double *hcol,
-                                                    ks_status *st, int col, int pass, int refine, double eta,
-                                                    int do_scale, int safe, double deftol) {
+__global__ __launch_bounds__(KS_FIN_BLOCK) void k_gs_fin(const double *__restrict__ partial, int nb, int k,
+                                                        double *sums, unsigned *counter, double *cbuf, double *hcol,
+                                                        ks_status *st, int col, int pass, int refine, double eta,
+                                                        int do_scale, int safe, double deftol) {
   if (!gs_pass_gate<FUSED>(st, col, pass)) return;
-  gs_fin_block<FUSED>(blockIdx.x, partial, nb, k, sums, counter, cbuf, hcol, st, col, refine, eta, do_scale, safe,
-                      deftol);
+  gs_fin_block<FUSED, KS_FIN_BLOCK>(blockIdx.x, partial, nb, k, sums, counter, cbuf, hcol, st, col, refine, eta,
+                                    do_scale, safe, deftol);
 }
 
 template <typename T>
@@ -692,7 +707,7 @@ __global__ __launch_bounds__(KS_BLOCK) void k_grp_fin(ks_grp_gs g, int pass, int
   const int m = grp_member(g.b0_fin, g.nmem, blockIdx.x);
   const ks_grp_gs_member &M = g.m[m];
   if (!gs_pass_gate<FUSED>(M.st, M.col, pass)) return;
-  gs_fin_block<FUSED>((int)blockIdx.x - g.b0_fin[m], M.partial, M.nb, M.k, M.sums, M.counter, M.cbuf, M.hcol, M.st,
+  gs_fin_block<FUSED, KS_BLOCK>((int)blockIdx.x - g.b0_fin[m], M.partial, M.nb, M.k, M.sums, M.counter, M.cbuf, M.hcol, M.st,
                       M.col, refine, eta, 1, 0, deftol);
 }
 
@@ -1037,7 +1052,7 @@ void launch_gs_fin(ks_ctx ctx, const gs_geom &g, int k, double *cbuf, double *hc
                    double eta, int do_scale, int safe) {
   const double deftol = 10.0 * 2.220446049250313e-16;
   ks_prof_begin(ctx, KS_PROF_GS_FIN, (double)g.nb * (k + 1) * 8.0, (double)g.nb * (k + 1), false);
-  hipLaunchKernelGGL((k_gs_fin<FUSED>), dim3(k + 1), dim3(KS_BLOCK), 0, ctx->stream, ctx->d_partial, g.nb, k,
+  hipLaunchKernelGGL((k_gs_fin<FUSED>), dim3(k + 1), dim3(KS_FIN_BLOCK), 0, ctx->stream, ctx->d_partial, g.nb, k,
                      ctx->d_gpartial, ctx->d_counters, cbuf, hcol, ctx->d_status, col, pass, refine, eta, do_scale,
                      safe, deftol);
   ks_prof_end(ctx);
