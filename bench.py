@@ -444,12 +444,17 @@ def run_serve(args):
     # served: concurrent, admitted against the budget; first with each request
     # launching its own steps (streams only), then with grouped steps
     srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=args.slots, grouped=False)
-    res_ng, wall_ng = S.run_stream(srv, reqs, lambda r: cache[r["_i"]], tol=1e-10)
+    res_ng, wall_ng = S.run_stream_native(srv, reqs, lambda r: cache[r["_i"]], tol=1e-10)
     srv.close()
     good_ng = sum(r["ngood"] for r in res_ng)
     log(f"[serve] streams only: {wall_ng:.2f}s, {good_ng} verified pairs")
+    # the same grouped serving driven from Python (arrivals and progress calls through ctypes)
     srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=args.slots, grouped=True)
-    results, wall = S.run_stream(srv, reqs, lambda r: cache[r["_i"]], tol=1e-10)
+    res_py, wall_py = S.run_stream(srv, reqs, lambda r: cache[r["_i"]], tol=1e-10)
+    srv.close()
+    log(f"[serve] grouped, Python-driven loop: {wall_py:.2f}s")
+    srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=args.slots, grouped=True)
+    results, wall = S.run_stream_native(srv, reqs, lambda r: cache[r["_i"]], tol=1e-10)
     stats = srv.stats()
     srv.close()
     good = sum(r["ngood"] for r in results)
@@ -483,6 +488,9 @@ def run_serve(args):
                             "host_loop_s": {k: round(v, 3) for k, v in stats["host_s"].items()},
                             "restarts": int(sum(r["its"] for r in results)),
                             "members_per_grouped_step": round(stats["group_member_steps"] / max(1, stats["group_steps"]), 2)},
+                 "served_python_driven": {"eigenpairs_per_s": round(sum(r["ngood"] for r in res_py) / wall_py, 4),
+                                          "wall_s": round(wall_py, 3),
+                                          "latency_ms_p50": round(float(np.percentile([r["latency"] * 1e3 for r in res_py], 50)), 2)},
                  "served_streams_only": {"eigenpairs_per_s": round(good_ng / wall_ng, 4), "wall_s": round(wall_ng, 3),
                                          "verified_pairs": good_ng,
                                          "latency_ms_p50": round(float(np.percentile([r["latency"] * 1e3 for r in res_ng], 50)), 2),
@@ -519,7 +527,7 @@ def run_scale(args):
         reqs.append(dict(n=len(c[0]) - 1, nev=8, ncv=40, t_arrival=0.0, v0_seed=100 + i, which=E.EPS.LARGEST_REAL,
                          hermitian=False, mixed_tol=1e-5, _csr=c, kind="convdiff"))
     srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=len(reqs), grouped=True)
-    res, wall = S.run_stream(srv, reqs, lambda r: r["_csr"], tol=1e-10)
+    res, wall = S.run_stream_native(srv, reqs, lambda r: r["_csr"], tol=1e-10)
     stats = srv.stats()
     srv.close()
     good = sum(r["ngood"] for r in res)
@@ -529,7 +537,7 @@ def run_scale(args):
     # the same nine requests served in plain FP64 (concurrency without the FP32 phase)
     plain = [dict(r, mixed_tol=0.0) for r in reqs]
     srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=len(plain), grouped=True)
-    res64, wall64 = S.run_stream(srv, plain, lambda r: r["_csr"], tol=1e-10)
+    res64, wall64 = S.run_stream_native(srv, plain, lambda r: r["_csr"], tol=1e-10)
     srv.close()
     good64 = sum(r["ngood"] for r in res64)
     log(f"[scale] served plain FP64: {wall64:.2f}s, {good64} verified pairs")

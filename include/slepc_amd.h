@@ -329,6 +329,11 @@ PetscErrorCode SlepcServerSetGrouped(SlepcServer srv, PetscBool grouped);
 PetscErrorCode SlepcServerGetGroupStats(SlepcServer srv, int64_t *steps, int64_t *member_steps);
 /* host seconds spent admitting, issuing grouped steps, restarting, retiring, polling */
 PetscErrorCode SlepcServerGetHostTimes(SlepcServer srv, double t[5]);
+/* serve a whole stream from C: req[i].t_arrival is an offset (seconds, ascending)
+   from the call; results into res[i] and eigr[i]/eigi[i]/err[i] (nev each), X[i]
+   (n x nev or NULL); *wall = seconds until the last result */
+PetscErrorCode SlepcServerRun(SlepcServer srv, PetscInt nreq, const SlepcRequest *req, SlepcResult *res,
+                              PetscScalar **eigr, PetscScalar **eigi, PetscReal **err, PetscScalar **X, double *wall);
 PetscErrorCode SlepcServerDestroy(SlepcServer *srv);
 double SlepcServerClock(void);   /* monotonic seconds, the clock of t_arrival/t_admit/t_done */
 

@@ -652,9 +652,9 @@ __device__ __forceinline__ void gs_fin_block(int b, const double *__restrict__ p
   gs_finish<FUSED>(fin, k, cbuf, hcol, st, col, refine, eta, do_scale, safe, deftol);
 }
 
-// single solves: 1024 threads per column, so the (up to 8192) partial sums of
-// a column are read in one round of eight loads per thread
-#define KS_FIN_BLOCK 1024
+// one 256-thread workgroup per column (1024-thread workgroups, which read a
+// column's partial sums in one round, measured 1.5-2.5x slower end to end)
+#define KS_FIN_BLOCK 256
 template <bool FUSED>
 __global__ __launch_bounds__(KS_FIN_BLOCK) void k_gs_fin(const double *__restrict__ partial, int nb, int k,
                                                         double *sums, unsigned *counter, double *cbuf, double *hcol,

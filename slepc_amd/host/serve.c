@@ -368,6 +368,55 @@ PetscErrorCode SlepcServerGetResult(SlepcServer s, PetscInt id, PetscBool *done,
   return PETSC_SUCCESS;
 }
 
+/* A whole stream served from C: request i is submitted once SlepcServerClock()
+   passes the call's start plus req[i].t_arrival (an offset in seconds; the
+   offsets ascend), the loop runs until every request has finished, and
+   result i lands in res[i] and the arrays eigr[i], eigi[i], err[i] (nev
+   entries each) and X[i] (n x nev, or NULL).  The latency clock of each
+   request starts at its arrival. */
+PetscErrorCode SlepcServerRun(SlepcServer s, PetscInt nreq, const SlepcRequest *req, SlepcResult *res,
+                              PetscScalar **eigr, PetscScalar **eigi, PetscReal **err, PetscScalar **X, double *wall)
+{
+  PetscErrorCode ierr = PETSC_SUCCESS;
+  PetscInt *ids = NULL, next = 0, ndone = 0, seen = 0;
+  PetscBool *got = NULL;
+  double t0;
+  PetscCheck(s && (nreq == 0 || (req && res && eigr && eigi && err)), PETSC_ERR_ARG_NULL, "null argument");
+  for (PetscInt i = 1; i < nreq; i++)
+    PetscCheck(req[i].t_arrival >= req[i - 1].t_arrival, PETSC_ERR_ARG_WRONG, "arrival offsets must ascend");
+  PetscCall(PetscMallocZero(sizeof(PetscInt) * (size_t)(nreq ? nreq : 1), &ids));
+  ierr = PetscMallocZero(sizeof(PetscBool) * (size_t)(nreq ? nreq : 1), &got);
+  if (ierr) { free(ids); return ierr; }
+  t0 = SlepcServerClock();
+  while (ndone < nreq) {
+    const double now = SlepcServerClock();
+    PetscInt nf = 0;
+    while (next < nreq && t0 + req[next].t_arrival <= now) {
+      SlepcRequest r = req[next];
+      r.t_arrival = t0 + req[next].t_arrival;
+      ierr = SlepcServerSubmit(s, &r, &ids[next]);
+      if (ierr) goto out;
+      next++;
+    }
+    ierr = SlepcServerProgress(s, NULL, NULL, NULL, &nf);
+    if (ierr) goto out;
+    if (nf == seen) continue;
+    seen = nf;
+    for (PetscInt i = 0; i < next; i++) {
+      PetscBool d = PETSC_FALSE;
+      if (got[i]) continue;
+      ierr = SlepcServerGetResult(s, ids[i], &d, &res[i], eigr[i], eigi[i], err[i], X ? X[i] : NULL);
+      if (ierr) goto out;
+      if (d) { got[i] = PETSC_TRUE; ndone++; }
+    }
+  }
+out:
+  if (wall) *wall = SlepcServerClock() - t0;
+  free(ids);
+  free(got);
+  return ierr;
+}
+
 PetscErrorCode SlepcServerSetGrouped(SlepcServer s, PetscBool grouped)
 {
   PetscCheck(s, PETSC_ERR_ARG_NULL, "null server");

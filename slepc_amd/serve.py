@@ -167,3 +167,53 @@ def run_stream(server: Server, requests, make_csr, tol: float = 1e-10, want_vect
         if not moved and idle_sleep > 0:
             time.sleep(idle_sleep)
     return results, clock() - t0
+
+
+def run_stream_native(server: Server, requests, make_csr, tol: float = 1e-10, want_vectors: bool = False,
+                      time_scale: float = 1.0):
+    """The same stream as run_stream, driven by SlepcServerRun: submission at the
+    arrival times, the progress loop and result collection all run in C, so no
+    Python work sits between the server's steps.  Returns the results in request
+    order and the wall time."""
+    n = len(requests)
+    keep, outs = [], []
+    reqs = (Request * max(n, 1))()
+    res = (Result * max(n, 1))()
+    P = C.POINTER(C.c_double)
+    er, ei, ee = (P * max(n, 1))(), (P * max(n, 1))(), (P * max(n, 1))()
+    xs = (P * max(n, 1))() if want_vectors else None
+    for i, r in enumerate(requests):
+        csr = make_csr(r)
+        rowptr = np.ascontiguousarray(csr[0], dtype=np.int32)
+        colidx = np.ascontiguousarray(csr[1], dtype=np.int32)
+        f32 = np.asarray(csr[2]).dtype == np.float32
+        val = np.ascontiguousarray(csr[2], dtype=np.float32 if f32 else np.float64)
+        v0 = np.ascontiguousarray(matgen.uniform_vector(r["v0_seed"], r["n"]), dtype=np.float64)
+        keep.append((rowptr, colidx, val, v0))
+        reqs[i] = Request(n=len(rowptr) - 1, rowptr=rowptr.ctypes.data, colidx=colidx.ctypes.data,
+                          val=None if f32 else val.ctypes.data, val32=val.ctypes.data if f32 else None,
+                          nev=r["nev"], ncv=r["ncv"], max_it=0, tol=tol, which=r.get("which", 1),
+                          hermitian=int(r.get("hermitian", True)), v0=v0.ctypes.data, want_vectors=int(want_vectors),
+                          mixed_tol=r.get("mixed_tol", 0.0), t_arrival=time_scale * r["t_arrival"])
+        o = dict(eigr=np.zeros(r["nev"]), eigi=np.zeros(r["nev"]), err=np.zeros(r["nev"]))
+        if want_vectors:
+            o["X"] = np.zeros((r["nev"], r["n"]))
+            xs[i] = o["X"].ctypes.data_as(P)
+        er[i], ei[i], ee[i] = (o[k].ctypes.data_as(P) for k in ("eigr", "eigi", "err"))
+        outs.append(o)
+    wall = C.c_double()
+    _eps._chk(_eps.lib().SlepcServerRun(server.h, C.c_int(n), reqs, res, er, ei, ee, xs, C.byref(wall)),
+              "SlepcServerRun")
+    results = []
+    for i in range(n):
+        q, o = res[i], outs[i]
+        k = q.nret
+        out = dict(nconv=q.nconv, nret=k, ngood=q.ngood, its=q.its, reason=q.reason, nmatmult=q.nmatmult,
+                   gs_passes=q.gs_passes, t_arrival=q.t_arrival, t_admit=q.t_admit, t_done=q.t_done,
+                   latency=q.t_done - q.t_arrival, t_host_ds=q.t_host_ds,
+                   eigr=o["eigr"][:k], eigi=o["eigi"][:k], err=o["err"][:k])
+        if want_vectors:
+            out["X"] = o["X"][:k].T.copy()
+        results.append(out)
+    return results, wall.value
+

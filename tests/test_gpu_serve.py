@@ -38,14 +38,16 @@ def _max_angle_sin(X, Y):
     return np.sqrt(max(0.0, 1.0 - s.min() ** 2))
 
 
-@pytest.mark.parametrize("grouped", [True, False])
-def test_serving_alone_vs_mix(E, grouped):
+@pytest.mark.parametrize("grouped,driver", [(True, "python"), (False, "python"), (True, "native")])
+def test_serving_alone_vs_mix(E, grouped, driver):
     # grouped: the Arnoldi steps of all expanding requests share launches;
-    # otherwise each request launches its own steps on its own stream
+    # otherwise each request launches its own steps on its own stream.
+    # driver: arrivals and the progress loop from Python, or all in C (SlepcServerRun)
     from slepc_amd import serve as S
     reqs = _stream(10, seed=4, nmax=20000)
     srv = S.Server(budget=2 << 30, max_active=4, grouped=grouped)
-    mix, wall = S.run_stream(srv, reqs, _csr, tol=1e-10, want_vectors=True)
+    run = S.run_stream if driver == "python" else S.run_stream_native
+    mix, wall = run(srv, reqs, _csr, tol=1e-10, want_vectors=True)
     st = srv.stats()
     srv.close()
     assert st["peak_reserved"] <= st["budget"]
