@@ -184,6 +184,12 @@ int ks_bv_convert(ks_ctx ctx, int n, ks_prec px, const void *x, ks_prec py, void
    Replaces BVMultInPlace_BLAS_Private (bvblas.c:74-106) / BVMultInPlace_BLAS_HIP
    (src/sys/classes/bv/impls/hip/bvhip.hip.cpp:65-112) without the n x (e-s)
    temporary.  Requires k <= KS_MAX_COLS and e <= KS_MAX_COLS. */
+/* G = Y^T X for tall-skinny X (n x p, leading dimension ldx) and Y (n x q):
+   G(i,j) = y_i . x_j into the device array d_G (q x p, leading dimension ldg),
+   stream ordered.  The matrix form of BVDot (src/sys/classes/bv/interface/
+   bvglobal.c:86-120, BVDot_BLAS_Private bvblas.c:199-233). */
+int ks_bv_gram(ks_ctx ctx, ks_prec prec, int n, const void *X, int ldx, int p, const void *Y, int ldy, int q,
+               double *d_G, int ldg);
 int ks_bv_multinplace(ks_ctx ctx, ks_prec prec, int n, void *V, int ld, int k,
                       const double *d_Q, int ldq, int s, int e);
 

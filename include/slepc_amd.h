@@ -47,6 +47,7 @@ typedef int    PetscBool;
 #define PETSC_ERR_ARG_WRONG         62
 #define PETSC_ERR_ARG_OUTOFRANGE    63
 #define PETSC_ERR_ARG_WRONGSTATE    73
+#define PETSC_ERR_ARG_INCOMP        75
 #define PETSC_ERR_ARG_NULL          85
 #define PETSC_ERR_USER_INPUT        90
 #define PETSC_ERR_CONV_FAILED       91
@@ -140,6 +141,9 @@ PetscErrorCode MatLoadPETScBinary(const char *path, Mat *A);
 /* ------------------------------------------------------------ BV */
 typedef enum { BV_ORTHOG_CGS = 0, BV_ORTHOG_MGS = 1 } BVOrthogType;
 typedef enum { BV_ORTHOG_REFINE_IFNEEDED = 0, BV_ORTHOG_REFINE_NEVER = 1, BV_ORTHOG_REFINE_ALWAYS = 2 } BVOrthogRefineType;
+/* BVOrthogBlockType (include/slepcbv.h:84-88); TSQR and TSQRCHOL are not implemented on the device */
+typedef enum { BV_ORTHOG_BLOCK_GS = 0, BV_ORTHOG_BLOCK_CHOL = 1, BV_ORTHOG_BLOCK_TSQR = 2, BV_ORTHOG_BLOCK_TSQRCHOL = 3,
+               BV_ORTHOG_BLOCK_SVQB = 4 } BVOrthogBlockType;
 PetscErrorCode BVRegister(const char name[], PetscErrorCode (*create)(BV));
 PetscErrorCode BVCreate(BV *bv);
 PetscErrorCode BVSetSizes(BV bv, PetscInt n, PetscInt N, PetscInt m);
@@ -171,6 +175,14 @@ PetscErrorCode BVGetNumConstraints(BV bv, PetscInt *nc);
 PetscErrorCode BVDotVec(BV bv, Vec y, PetscScalar m[]);
 PetscErrorCode BVMultVec(BV bv, PetscScalar alpha, PetscScalar beta, Vec y, PetscScalar q[]);
 PetscErrorCode BVMultInPlace(BV bv, Mat Q, PetscInt s, PetscInt e);
+/* M = Y^T X over the active columns (bvglobal.c:86-120): rows ly..ky-1, columns lx..kx-1 of M */
+PetscErrorCode BVDot(BV X, BV Y, Mat M);
+/* QR of the active columns (bvorthog.c:726-767, methods bvorthog.c:489-663): leading columns 0..l-1 are kept
+   (assumed orthonormal), columns l..k-1 become orthonormal, R (k x k or NULL) gets
+   the columns l..k-1 of the factor; block types GS, CHOL and SVQB */
+PetscErrorCode BVOrthogonalize(BV V, Mat R);
+/* M = Y^T A X over the active columns (bvglobal.c:1093-1160, the non-symmetric path) */
+PetscErrorCode BVMatProject(BV X, Mat A, BV Y, Mat M);
 PetscErrorCode BVOrthogonalizeColumn(BV bv, PetscInt j, PetscScalar H[], PetscReal *norm, PetscBool *lindep);
 PetscErrorCode BVOrthonormalizeColumn(BV bv, PetscInt j, PetscBool replace, PetscReal *norm, PetscBool *lindep);
 PetscErrorCode BVMatMultColumn(BV bv, Mat A, PetscInt j);

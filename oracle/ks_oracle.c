@@ -45,6 +45,10 @@ extern void   dgemm_(const char*, const char*, const int*, const int*, const int
                      const double*, const int*, const double*, const int*, const double*, double*,
                      const int*);
 extern double dnrm2_(const int*, const double*, const int*);
+extern void   dpotrf_(const char*, const int*, double*, const int*, int*);
+extern void   dtrtri_(const char*, const char*, const int*, double*, const int*, int*);
+extern void   dsyev_(const char*, const char*, const int*, double*, const int*, double*, double*, const int*,
+                     int*);
 extern int    MKL_Set_Threading_Layer(int);     /* MKL service (mkl_rt), by value: 3 = GNU OpenMP */
 
 /* MKL threads through the same GNU OpenMP runtime as the loops here; its Intel
@@ -363,6 +367,128 @@ int or_orthonormalize_column(int n, int ld, double *V, int nc, int j, int refine
   if (npasses) *npasses = (int)bv.passes;
   free(bv.buffer);
   return e;
+}
+
+/* ------------------------------------------------------------------ block orthogonalization */
+/* BVDot_BLAS_Private (bvblas.c:199-233): M(ly:ky, lx:kx) = Y(:, ly:ky)^T X(:, lx:kx) as one gemm */
+static void bv_dot(const double *X, int ldx, int lx, int kx, const double *Y, int ldy, int ly, int ky, int n,
+                   double *M, int ldm)
+{
+  int m = ky - ly, k = kx - lx;
+  double one = 1.0, zero = 0.0;
+  if (m <= 0 || k <= 0) return;
+  if (n == 0) {
+    for (int j = 0; j < k; j++) for (int i = 0; i < m; i++) M[(ly + i) + (size_t)(lx + j) * ldm] = 0.0;
+    return;
+  }
+  dgemm_("C", "N", &m, &k, &n, &one, Y + (size_t)ly * ldy, &ldy, X + (size_t)lx * ldx, &ldx, &zero,
+         M + (size_t)lx * ldm + ly, &ldm);
+}
+
+int or_bv_dot(int n, const double *X, int ldx, int lx, int kx, const double *Y, int ldy, int ly, int ky,
+              double *M, int ldm)
+{
+  or_mkl_layer();
+  bv_dot(X, ldx, lx, kx, Y, ldy, ly, ky, n, M, ldm);
+  return 0;
+}
+
+/* BVOrthogonalize (bvorthog.c:726-767) on V (n x k, leading dimension ld, no
+   constraints): columns 0..l-1 kept, l..k-1 orthonormalized; R (ldr x k) gets
+   columns l..k-1 of the factor.  block: 0 GS (bvorthog.c:507-547), 1 CHOL
+   (bvorthog.c:581-596 with BVMatCholInv_LAPACK_Private bvlapack.c:136-202),
+   4 SVQB (bvorthog.c:646-663 with BVMatSVQB_LAPACK_Private bvlapack.c:261-345);
+   the leading columns by block Gram-Schmidt first (bvorthog.c:489-502). */
+int or_bv_orthogonalize(int n, int ld, double *V, int l, int k, int block, double *R, int ldr)
+{
+  or_mkl_layer();
+  if (l >= k) return 0;
+  if (block == 0) {
+    obv bv = {0};
+    bv.n = n; bv.ld = ld; bv.nc = 0; bv.m = k; bv.V = V;
+    bv.refine = 0; bv.eta = 0.7071;
+    bv.buffer = (double*)calloc((size_t)k * k, sizeof(double));
+    if (!bv.buffer) return -1;
+    for (int j = l; j < k; j++) {
+      double norm = 0.0;
+      int e = orthogonalize_column(&bv, j, &norm, NULL);
+      if (e || norm == 0.0) { free(bv.buffer); return e ? e : -2; }
+      if (R) {
+        for (int i = 0; i < j; i++) R[i + (size_t)j * ldr] = bv.buffer[(size_t)j * k + i];
+        R[j + (size_t)j * ldr] = norm;
+      }
+      bv_scalecol(&bv, j, 1.0 / norm);
+    }
+    free(bv.buffer);
+    return 0;
+  }
+  if (block != 1 && block != 4) return -3;
+  const int nn = k - l;
+  double *W = (double*)calloc((size_t)k * k, sizeof(double));     /* the buffer as a k x k matrix */
+  double *S = (double*)calloc((size_t)k * k, sizeof(double));
+  if (!W || !S) { free(W); free(S); return -1; }
+  obv bv = {0};
+  bv.n = n; bv.ld = ld; bv.nc = 0; bv.m = k; bv.V = V;
+  if (l > 0) {   /* R12 = V1^T V2, V2 -= V1 R12 */
+    bv_dot(V, ld, l, k, V, ld, 0, l, n, W, k);
+    for (int j = l; j < k; j++) {
+      for (int i = 0; i < l; i++) S[i + (size_t)j * k] = -W[i + (size_t)j * k];
+      S[j + (size_t)j * k] = 1.0;
+    }
+    bv.l = 0; bv.k = k;
+    bv_multinplace(&bv, S, k, l, k);
+    memset(S, 0, sizeof(double) * (size_t)k * k);
+  }
+  bv_dot(V, ld, l, k, V, ld, l, k, n, W, k);   /* W(l:k, l:k) = V2^T V2 */
+  int nn_ = nn, ldk = k, info = 0;
+  double *g = W + (size_t)l * k + l;
+  if (block == 1) {
+    for (int j = l; j < k; j++) memcpy(S + (size_t)j * k + l, W + (size_t)j * k + l, sizeof(double) * nn);
+    dpotrf_("U", &nn_, g, &ldk, &info);
+    if (info) {
+      for (int j = l; j < k; j++) {
+        memcpy(W + (size_t)j * k + l, S + (size_t)j * k + l, sizeof(double) * nn);
+        W[j + (size_t)j * k] += 50.0 * DBL_EPSILON;
+      }
+      dpotrf_("U", &nn_, g, &ldk, &info);
+    }
+    if (info) { free(W); free(S); return -4; }
+    memset(S, 0, sizeof(double) * (size_t)k * k);
+    for (int j = l; j < k; j++) memcpy(S + (size_t)j * k + l, W + (size_t)j * k + l, sizeof(double) * nn);
+    dtrtri_("U", "N", &nn_, S + (size_t)l * k + l, &ldk, &info);
+    if (info) { free(W); free(S); return -5; }
+    for (int j = l; j < k; j++)
+      for (int i = j + 1; i < k; i++) { W[i + (size_t)j * k] = 0.0; S[i + (size_t)j * k] = 0.0; }
+  } else {
+    double *D = (double*)malloc(sizeof(double) * nn), *eig = (double*)malloc(sizeof(double) * nn), a = 0.0;
+    int lwork = -1;
+    for (int i = l; i < k; i++) D[i - l] = 1.0 / sqrt(W[i + (size_t)i * k]);
+    for (int i = l; i < k; i++) for (int j = l; j < k; j++) S[i + (size_t)j * k] = W[i + (size_t)j * k] * D[i - l];
+    for (int j = l; j < k; j++) for (int i = l; i < k; i++) S[i + (size_t)j * k] *= D[j - l];
+    dsyev_("V", "L", &nn_, S + (size_t)l * k + l, &ldk, eig, &a, &lwork, &info);
+    lwork = (int)a;
+    double *work = (double*)malloc(sizeof(double) * (size_t)(lwork > 1 ? lwork : 1));
+    dsyev_("V", "L", &nn_, S + (size_t)l * k + l, &ldk, eig, work, &lwork, &info);
+    free(work);
+    if (info) { free(D); free(eig); free(W); free(S); return -6; }
+    for (int i = l; i < k; i++) for (int j = l; j < k; j++) W[i + (size_t)j * k] = S[j + (size_t)i * k];   /* R = U' */
+    for (int i = l; i < k; i++) for (int j = l; j < k; j++) S[i + (size_t)j * k] *= D[i - l];
+    for (int j = l; j < k; j++) for (int i = l; i < k; i++) S[i + (size_t)j * k] /= sqrt(eig[j - l]);
+    for (int i = l; i < k; i++) for (int j = l; j < k; j++) W[i + (size_t)j * k] *= sqrt(eig[i - l]);
+    for (int j = l; j < k; j++) for (int i = l; i < k; i++) W[i + (size_t)j * k] /= D[j - l];
+    free(D);
+    free(eig);
+  }
+  bv.l = l; bv.k = k;
+  bv_multinplace(&bv, S, k, l, k);   /* V2 = V2 S */
+  if (R)   /* BV_StoreCoeffsBlock_Default (bvorthog.c:567-579) */
+    for (int j = l; j < k; j++) {
+      const int rows = block == 1 ? j + 1 : k;
+      for (int i = 0; i < rows; i++) R[i + (size_t)j * ldr] = W[i + (size_t)j * k];
+    }
+  free(W);
+  free(S);
+  return 0;
 }
 
 /* ------------------------------------------------------------------ DS */

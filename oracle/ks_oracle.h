@@ -67,6 +67,11 @@ int or_orthonormalize_column(int n, int ld, double *V, int nc, int j, int refine
 
 /* BVMultInPlace: V(:,s:e-1) = V(:,l:k-1) * Q(l:k-1,s:e-1), columns counted
    after nc constraints. */
+/* M(ly:ky, lx:kx) = Y(:, ly:ky)^T X(:, lx:kx) (BVDot) */
+int or_bv_dot(int n, const double *X, int ldx, int lx, int kx, const double *Y, int ldy, int ly, int ky,
+              double *M, int ldm);
+/* BVOrthogonalize of V(:, l:k) with block type 0 (GS), 1 (CHOL) or 4 (SVQB); R: columns l..k-1 */
+int or_bv_orthogonalize(int n, int ld, double *V, int l, int k, int block, double *R, int ldr);
 int or_multinplace(int n, int ld, double *V, int nc, int l, int k,
                    const double *Q, int ldq, int s, int e);
 

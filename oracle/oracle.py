@@ -99,6 +99,31 @@ def multinplace(V: np.ndarray, Q: np.ndarray, l: int, k: int, s: int, e: int, nc
     assert rc == 0
 
 
+BLOCK_GS, BLOCK_CHOL, BLOCK_SVQB = 0, 1, 4
+
+
+def bv_orthogonalize(V: np.ndarray, l: int = 0, block: int = BLOCK_GS):
+    """BVOrthogonalize of columns l.. of V (n, k), in place; returns R (k, k)
+    with columns l.. filled."""
+    assert V.flags.f_contiguous and V.dtype == np.float64
+    n, k = V.shape
+    R = np.zeros((k, k), order="F")
+    rc = lib().or_bv_orthogonalize(C.c_int(n), C.c_int(n), _p(V), C.c_int(l), C.c_int(k), C.c_int(block),
+                                   _p(R), C.c_int(k))
+    if rc:
+        raise RuntimeError(f"or_bv_orthogonalize failed ({rc})")
+    return R
+
+
+def bv_dot(X: np.ndarray, Y: np.ndarray, M: np.ndarray, lx: int = 0, ly: int = 0):
+    """M(ly:, lx:) = Y(:, ly:)^T X(:, lx:) in place (BVDot with leading columns)."""
+    assert X.flags.f_contiguous and Y.flags.f_contiguous and M.flags.f_contiguous
+    n = X.shape[0]
+    rc = lib().or_bv_dot(C.c_int(n), _p(X), C.c_int(n), C.c_int(lx), C.c_int(X.shape[1]), _p(Y), C.c_int(n),
+                         C.c_int(ly), C.c_int(Y.shape[1]), _p(M), C.c_int(M.shape[0]))
+    assert rc == 0
+
+
 def ds_run(dstype: str, ld: int, n: int, l: int = 0, k: int = 0, compact: bool = False,
            extrarow: bool = False, state: int = 0, A=None, T=None, which: int = LARGEST_MAGNITUDE,
            jvec: int = -1):

@@ -895,6 +895,107 @@ int dispatch_mip(ks_ctx ctx, T *V, long ld, int n, int K, const double *Q, int l
   return KS_OK;
 }
 
+// ---------------------------------------------------------------- block dot (BVDot)
+// G = Y^T X for tall-skinny X (n x p) and Y (n x q), one 64 x 64 block of G
+// (rows i0.., columns j0..) per launch.  Workgroup b owns a contiguous range of
+// rows; per 32-row chunk it stages the 32 x 64 slices of X and Y in LDS with
+// coalesced column segments (rows past n and columns past p or q read a valid
+// element and are zeroed by a select, not a branch), and thread (ti, tj) of a
+// 16 x 16 grid accumulates the 4 x 4 entries G(i0+4ti.., j0+4tj..).  Partial
+// blocks go to partial[b][64*64]; k_gram_fin adds them in workgroup order.
+#define KS_GRAM_ROWS 32
+#define KS_GRAM_MAXB 512
+template <typename T>
+__global__ __launch_bounds__(KS_BLOCK) void k_gram(const T *__restrict__ X, long ldx, int p, const T *__restrict__ Y,
+                                                  long ldy, int q, int n, int rpb, int i0, int j0,
+                                                  double *__restrict__ partial) {
+  __shared__ double sX[KS_GRAM_ROWS][64];
+  __shared__ double sY[KS_GRAM_ROWS][64];
+  const int tid = threadIdx.x;
+  const int ti = tid >> 4, tj = tid & 15;
+  const int lr = tid & (KS_GRAM_ROWS - 1), lc = tid / KS_GRAM_ROWS;   // loader: row, first column
+  const long r0 = (long)blockIdx.x * rpb;
+  const long r1 = min((long)n, r0 + rpb);
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[a][c] = 0.0;
+  for (long rb = r0; rb < r1; rb += KS_GRAM_ROWS) {
+    const long row = rb + lr;
+    const bool rok = row < r1;
+    const long rowc = rok ? row : r1 - 1;
+    double vx[8], vy[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int c = lc + 8 * u;
+      const int cx = min(j0 + c, p - 1), cy = min(i0 + c, q - 1);
+      vx[u] = (double)X[(size_t)cx * ldx + rowc];
+      vy[u] = (double)Y[(size_t)cy * ldy + rowc];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    __syncthreads();   // the previous chunk's reads are done
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int c = lc + 8 * u;
+      sX[lr][c] = (rok && j0 + c < p) ? vx[u] : 0.0;
+      sY[lr][c] = (rok && i0 + c < q) ? vy[u] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int r = 0; r < KS_GRAM_ROWS; r++) {
+      double a[4], bb[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        a[u] = sY[r][4 * ti + u];
+        bb[u] = sX[r][4 * tj + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[u][v] += a[u] * bb[v];
+    }
+  }
+  double *out = partial + (size_t)blockIdx.x * 4096;
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) out[(4 * ti + u) * 64 + 4 * tj + v] = acc[u][v];
+}
+
+// G(i0 + i, j0 + j) = sum over the nb partial blocks, in workgroup order
+__global__ __launch_bounds__(KS_BLOCK) void k_gram_fin(const double *__restrict__ partial, int nb, int qb, int pb,
+                                                      int i0, int j0, double *__restrict__ G, int ldg) {
+  const int e = blockIdx.x * KS_BLOCK + threadIdx.x;   // entry of the 64 x 64 block
+  if (e >= 4096) return;
+  const int i = e >> 6, j = e & 63;
+  double s = 0.0;
+  for (int b = 0; b < nb; b++) s += partial[(size_t)b * 4096 + e];
+  if (i < qb && j < pb) G[(size_t)(i0 + i) + (size_t)(j0 + j) * ldg] = s;
+}
+
+template <typename T>
+int gram_t(ks_ctx ctx, int n, const T *X, long ldx, int p, const T *Y, long ldy, int q, double *Gdev, int ldg) {
+  int nb = 0, rpb = 0;   // n == 0: no partial blocks, the finisher writes zeros
+  if (n > 0) {
+    nb = (n + KS_GRAM_ROWS - 1) / KS_GRAM_ROWS;
+    if (nb > KS_GRAM_MAXB) nb = KS_GRAM_MAXB;
+    rpb = ((n + nb - 1) / nb + KS_GRAM_ROWS - 1) / KS_GRAM_ROWS * KS_GRAM_ROWS;
+    nb = (n + rpb - 1) / rpb;
+  }
+  static_assert((size_t)KS_GRAM_MAXB * 4096 <= (size_t)KS_MAXB * KS_NCOLP, "Gram partials exceed the workspace");
+  for (int i0 = 0; i0 < q; i0 += 64)
+    for (int j0 = 0; j0 < p; j0 += 64) {
+      if (nb > 0)
+        hipLaunchKernelGGL((k_gram<T>), dim3(nb), dim3(KS_BLOCK), 0, ctx->stream, X, ldx, p, Y, ldy, q, n, rpb, i0,
+                           j0, ctx->d_partial);
+      hipLaunchKernelGGL(k_gram_fin, dim3(4096 / KS_BLOCK), dim3(KS_BLOCK), 0, ctx->stream,
+                         (const double *)ctx->d_partial, nb, min(64, q - i0), min(64, p - j0), i0, j0, Gdev, ldg);
+    }
+  KS_HIP(hipGetLastError());
+  return KS_OK;
+}
+
 // ---------------------------------------------------------------- launch helpers
 inline int ew_blocks(int n) {
   int b = (n + 255) / 256;
@@ -1288,6 +1389,21 @@ int ks_bv_convert(ks_ctx ctx, int n, ks_prec px, const void *x, ks_prec py, void
   ks_prof_end(ctx);
   KS_HIP(hipGetLastError());
   return KS_OK;
+}
+
+int ks_bv_gram(ks_ctx ctx, ks_prec prec, int n, const void *X, int ldx, int p, const void *Y, int ldy, int q,
+               double *d_G, int ldg) {
+  KS_CHECK(ctx && X && Y && d_G, KS_ERR_ARG, "null gram argument");
+  KS_CHECK(n > 0 && p >= 0 && q >= 0 && ldx >= n && ldy >= n && ldg >= q, KS_ERR_ARG,
+           "bad gram sizes n=%d p=%d q=%d ldx=%d ldy=%d ldg=%d", n, p, q, ldx, ldy, ldg);
+  KS_CHECK(p <= KS_MAX_COLS && q <= KS_MAX_COLS, KS_ERR_RANGE, "gram supports at most %d columns", KS_MAX_COLS);
+  if (p == 0 || q == 0) return KS_OK;
+  const double sz = prec == KS_F64 ? 8.0 : 4.0;
+  ks_prof_begin(ctx, KS_PROF_OTHER, (double)n * (p + q) * sz, 2.0 * n * (double)p * q, false);
+  int err = prec == KS_F64 ? gram_t<double>(ctx, n, (const double *)X, ldx, p, (const double *)Y, ldy, q, d_G, ldg)
+                           : gram_t<float>(ctx, n, (const float *)X, ldx, p, (const float *)Y, ldy, q, d_G, ldg);
+  ks_prof_end(ctx);
+  return err;
 }
 
 int ks_bv_multinplace(ks_ctx ctx, ks_prec prec, int n, void *V, int ld, int k, const double *d_Q, int ldq, int s,

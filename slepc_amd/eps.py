@@ -102,6 +102,95 @@ class Vec:
             pass
 
 
+class BV:
+    """A BVHIPKS basis of m columns of length n (BVCreate/BVSetSizes/BVSetType),
+    with host-side column access for tests and small drivers."""
+    GS, CHOL, TSQR, TSQRCHOL, SVQB = 0, 1, 2, 3, 4   # BVOrthogBlockType
+
+    def __init__(self, n: int, m: int, prec: int = F64):
+        initialize()
+        self.n, self.m = n, m
+        self.h = C.c_void_p()
+        _chk(lib().BVCreate(C.byref(self.h)), "BVCreate")
+        _chk(lib().BVSetSizes(self.h, C.c_int(n), C.c_int(n), C.c_int(m)), "BVSetSizes")
+        if prec != F64:
+            _chk(lib().BVHIPKSSetPrecision(self.h, C.c_int(prec)), "BVHIPKSSetPrecision")
+        _chk(lib().BVSetType(self.h, b"hipks"), "BVSetType")
+
+    def set_columns(self, X, start: int = 0):
+        X = np.asarray(X, dtype=np.float64)
+        for j in range(X.shape[1]):
+            col = np.ascontiguousarray(X[:, j])
+            _chk(lib().BVSetColumnValuesHost(self.h, C.c_int(start + j), col.ctypes.data_as(C.POINTER(C.c_double))),
+                 "BVSetColumnValuesHost")
+
+    def get_columns(self, start: int = 0, stop: int | None = None) -> np.ndarray:
+        stop = self.m if stop is None else stop
+        out = np.zeros((self.n, stop - start), order="F")
+        for j in range(start, stop):
+            col = np.zeros(self.n)
+            _chk(lib().BVGetColumnValuesHost(self.h, C.c_int(j), col.ctypes.data_as(C.POINTER(C.c_double))),
+                 "BVGetColumnValuesHost")
+            out[:, j - start] = col
+        return out
+
+    def set_active(self, l: int, k: int):
+        _chk(lib().BVSetActiveColumns(self.h, C.c_int(l), C.c_int(k)), "BVSetActiveColumns")
+
+    def set_block(self, block: int):
+        _chk(lib().BVSetOrthogonalization(self.h, C.c_int(0), C.c_int(0), C.c_double(0.7071), C.c_int(block)),
+             "BVSetOrthogonalization")
+
+    @staticmethod
+    def _dense(m: int, n: int, M=None):
+        arr = np.zeros((m, n), order="F") if M is None else np.asfortranarray(M, dtype=np.float64)
+        h = C.c_void_p()
+        _chk(lib().MatCreateSeqDense(C.c_int(m), C.c_int(n), arr.ctypes.data_as(C.POINTER(C.c_double)), C.byref(h)),
+             "MatCreateSeqDense")
+        return arr, h
+
+    def orthogonalize(self, want_r: bool = True) -> np.ndarray | None:
+        """BVOrthogonalize on the active columns; returns R (m x m) or None."""
+        if not want_r:
+            _chk(lib().BVOrthogonalize(self.h, None), "BVOrthogonalize")
+            return None
+        R, h = self._dense(self.m, self.m)
+        try:
+            _chk(lib().BVOrthogonalize(self.h, h), "BVOrthogonalize")
+        finally:
+            lib().MatDestroy(C.byref(h))
+        return R
+
+    def dot(self, Y: "BV", M=None) -> np.ndarray:
+        """BVDot(self, Y, M): M = Y^T self over the active columns (other entries kept)."""
+        M, h = self._dense(Y.m, self.m, M)
+        try:
+            _chk(lib().BVDot(self.h, Y.h, h), "BVDot")
+        finally:
+            lib().MatDestroy(C.byref(h))
+        return M
+
+    def mat_project(self, A, Y: "BV", M=None) -> np.ndarray:
+        """BVMatProject(self, A, Y, M): M = Y^T A self (A a Mat or None)."""
+        M, h = self._dense(Y.m, self.m, M)
+        try:
+            _chk(lib().BVMatProject(self.h, A.h if A is not None else None, Y.h, h), "BVMatProject")
+        finally:
+            lib().MatDestroy(C.byref(h))
+        return M
+
+    def destroy(self):
+        if self.h:
+            lib().BVDestroy(C.byref(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 def read_petsc_binary(path: str):
     """(rowptr, colidx, val) of a matrix stored with PETSc's binary viewer, read by
     MatReadPETScBinary (host only: no device needed)."""

@@ -7,6 +7,12 @@
  */
 #include "slepcimpl.h"
 
+/* LAPACK, Fortran ABI, 32-bit integers */
+extern void dpotrf_(const char *, const int *, double *, const int *, int *);
+extern void dtrtri_(const char *, const char *, const int *, double *, const int *, int *);
+extern void dsyev_(const char *, const char *, const int *, double *, const int *, double *, double *, const int *,
+                   int *);
+
 static SlepcFunctionList BVList;
 
 PetscErrorCode BVRegister(const char name[], PetscErrorCode (*create)(BV))
@@ -91,8 +97,9 @@ PetscErrorCode BVGetType(BV bv, BVType *type)
 
 PetscErrorCode BVSetOrthogonalization(BV bv, BVOrthogType type, BVOrthogRefineType refine, PetscReal eta, int block)
 {
-  (void)block;
   PetscCheck(bv, PETSC_ERR_ARG_NULL, "null BV");
+  PetscCheck(block >= BV_ORTHOG_BLOCK_GS && block <= BV_ORTHOG_BLOCK_SVQB, PETSC_ERR_ARG_OUTOFRANGE, "invalid block type %d", block);
+  bv->orthog_block = (BVOrthogBlockType)block;
   PetscCheck(type == BV_ORTHOG_CGS, PETSC_ERR_SUP, "only classical Gram-Schmidt is implemented on the device");
   bv->orthog_type = type;
   bv->orthog_ref = refine;
@@ -307,6 +314,240 @@ PetscErrorCode BVMultInPlace(BV bv, Mat Q, PetscInt s, PetscInt e)
   PetscCheck(m >= bv->k, PETSC_ERR_ARG_SIZ, "Mat argument has %d rows, should have at least %d", m, bv->k);
   PetscCheck(e <= n, PETSC_ERR_ARG_SIZ, "Mat argument only has %d columns, the requested value of e is larger: %d", n, e);
   return bv->ops.multinplace(bv, Q, s, e);
+}
+
+/* BVDot (bvglobal.c:86-120): M(ly:ky, lx:kx) = Y(:, ly:ky)^T X(:, lx:kx) */
+PetscErrorCode BVDot(BV X, BV Y, Mat M)
+{
+  PetscInt m, n;
+  PetscCall(BVCheckReady(X));
+  PetscCall(BVCheckReady(Y));
+  PetscCheck(M && !strcmp(M->type, MATSEQDENSE), PETSC_ERR_ARG_WRONG, "M must be a sequential dense matrix");
+  PetscCall(MatGetSize(M, &m, &n));
+  PetscCheck(m >= Y->k, PETSC_ERR_ARG_SIZ, "Mat argument has %d rows, should have at least %d", m, Y->k);
+  PetscCheck(n >= X->k, PETSC_ERR_ARG_SIZ, "Mat argument has %d columns, should have at least %d", n, X->k);
+  PetscCheck(X->n == Y->n, PETSC_ERR_ARG_INCOMP, "Mismatching local dimension X %d, Y %d", X->n, Y->n);
+  PetscCheck(!strcmp(X->type, Y->type) && X->prec == Y->prec, PETSC_ERR_ARG_INCOMP, "X and Y must have the same type and precision");
+  if (X->l == X->k || Y->l == Y->k) return PETSC_SUCCESS;
+  PetscCheck(X->ops.gram, PETSC_ERR_SUP, "BV type %s has no block dot", X->type);
+  {
+    Mat_SeqDense *d = (Mat_SeqDense *)M->data;
+    return X->ops.gram(X, X->l, X->k, Y, Y->l, Y->k, d->v + (size_t)X->l * d->lda + Y->l, d->lda);
+  }
+}
+
+static PetscErrorCode DenseAt(Mat A, PetscScalar **a, PetscInt *lda)
+{
+  Mat_SeqDense *d = (Mat_SeqDense *)A->data;
+  *a = d->v;
+  *lda = d->lda;
+  return PETSC_SUCCESS;
+}
+
+/* Block Gram-Schmidt against the leading columns (bvorthog.c:489-502):
+   R12 = V(:,0:l)^T V(:,l:k), V(:,l:k) -= V(:,0:l) R12; R12 goes to rows 0..l-1
+   of the columns l..k-1 of W (k x k, column-major, leading dimension ldw) */
+static PetscErrorCode OrthoBlockGS(BV V, PetscScalar *W, PetscInt ldw)
+{
+  const PetscInt l = V->l, k = V->k;
+  Mat Q;
+  PetscScalar *q;
+  PetscInt ldq;
+  PetscErrorCode e;
+  PetscCall(V->ops.gram(V, l, k, V, 0, l, W + l * ldw, ldw));
+  /* V(:, l:k) = V(:, 0:k) [-R12; I] */
+  PetscCall(MatCreateSeqDense(k, k, NULL, &Q));
+  PetscCall(DenseAt(Q, &q, &ldq));
+  for (PetscInt j = l; j < k; j++) {
+    for (PetscInt i = 0; i < l; i++) q[i + j * ldq] = -W[i + j * ldw];
+    q[j + j * ldq] = 1.0;
+  }
+  V->l = 0;
+  e = BVMultInPlace(V, Q, l, k);
+  V->l = l;
+  MatDestroy(&Q);
+  return e;
+}
+
+/* Cholesky QR (bvorthog.c:581-596, BVMatCholInv_LAPACK_Private bvlapack.c:136-202):
+   G = V2^T V2, upper Cholesky G = R^T R (retried once on G + 50 eps I), V2 = V2 R^-1 */
+static PetscErrorCode OrthoChol(BV V, PetscScalar *W, PetscInt ldw)
+{
+  const PetscInt l = V->l, k = V->k, n = k - l;
+  int n_ = (int)n, ld_ = (int)ldw, info = 0;
+  Mat S;
+  PetscScalar *g, *sv, *save;
+  PetscInt lds;
+  PetscCall(V->ops.gram(V, l, k, V, l, k, W + l * ldw + l, ldw));   /* W(l:k, l:k) = V2^T V2 */
+  g = W + l * ldw + l;
+  PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)n * (size_t)n, &save));
+  for (PetscInt j = 0; j < n; j++) memcpy(save + j * n, g + j * ldw, sizeof(PetscScalar) * (size_t)n);
+  dpotrf_("U", &n_, g, &ld_, &info);
+  if (info) {   /* retry on the diagonally perturbed matrix */
+    for (PetscInt j = 0; j < n; j++) {
+      memcpy(g + j * ldw, save + j * n, sizeof(PetscScalar) * (size_t)n);
+      g[j + j * ldw] += 50.0 * PETSC_MACHINE_EPSILON;
+    }
+    dpotrf_("U", &n_, g, &ld_, &info);
+  }
+  free(save);
+  PetscCheck(info == 0, PETSC_ERR_LIB, "Error in LAPACK subroutine potrf: info=%d", info);
+  for (PetscInt j = 0; j < n; j++)   /* zero below the diagonal */
+    for (PetscInt i = j + 1; i < n; i++) g[i + j * ldw] = 0.0;
+  /* S(l:k, l:k) = R^-1 */
+  PetscCall(MatCreateSeqDense(k, k, NULL, &S));
+  PetscCall(DenseAt(S, &sv, &lds));
+  for (PetscInt j = 0; j < n; j++) memcpy(sv + (l + j) * lds + l, g + j * ldw, sizeof(PetscScalar) * (size_t)n);
+  {
+    int lds_ = (int)lds;
+    dtrtri_("U", "N", &n_, sv + l * lds + l, &lds_, &info);
+  }
+  if (info) { MatDestroy(&S); SETERRQ(PETSC_ERR_LIB, "Error in LAPACK subroutine trtri: info=%d", info); }
+  {
+    PetscErrorCode e = BVMultInPlace(V, S, l, k);
+    MatDestroy(&S);
+    PetscCall(e);
+  }
+  return PETSC_SUCCESS;
+}
+
+/* SVQB (bvorthog.c:646-663, BVMatSVQB_LAPACK_Private bvlapack.c:261-345):
+   G = V2^T V2, D = diag(G)^-1/2, D G D = U L U^T, V2 = V2 D U L^-1/2, R = L^1/2 U^T D^-1 */
+static PetscErrorCode OrthoSVQB(BV V, PetscScalar *W, PetscInt ldw)
+{
+  const PetscInt l = V->l, k = V->k, n = k - l;
+  int n_ = (int)n, info = 0, lwork = -1;
+  Mat S;
+  PetscScalar *sv, *D, *eig, *work, *U, a = 0.0;
+  PetscInt lds;
+  PetscCall(V->ops.gram(V, l, k, V, l, k, W + l * ldw + l, ldw));   /* W(l:k, l:k) = V2^T V2 */
+  PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)n * (size_t)(n + 2), &U));
+  D = U + (size_t)n * n;
+  eig = D + n;
+  for (PetscInt i = 0; i < n; i++) D[i] = 1.0 / sqrt(W[(l + i) + (l + i) * ldw]);
+  for (PetscInt j = 0; j < n; j++)
+    for (PetscInt i = 0; i < n; i++) U[i + j * n] = W[(l + i) + (l + j) * ldw] * D[i] * D[j];
+  dsyev_("V", "L", &n_, U, &n_, eig, &a, &lwork, &info);
+  lwork = (int)a;
+  PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)(lwork > 1 ? lwork : 1), &work));
+  dsyev_("V", "L", &n_, U, &n_, eig, work, &lwork, &info);
+  free(work);
+  if (info) { free(U); SETERRQ(PETSC_ERR_LIB, "Error in LAPACK subroutine syev: info=%d", info); }
+  /* S(l:k, l:k) = D U L^-1/2;  W(l:k, l:k) = L^1/2 U^T D^-1 */
+  PetscCall(MatCreateSeqDense(k, k, NULL, &S));
+  PetscCall(DenseAt(S, &sv, &lds));
+  for (PetscInt j = 0; j < n; j++)
+    for (PetscInt i = 0; i < n; i++) {
+      sv[(l + i) + (l + j) * lds] = D[i] * U[i + j * n] / sqrt(eig[j]);
+      W[(l + i) + (l + j) * ldw] = sqrt(eig[i]) * U[j + i * n] / D[j];
+    }
+  free(U);
+  {
+    PetscErrorCode e = BVMultInPlace(V, S, l, k);
+    MatDestroy(&S);
+    PetscCall(e);
+  }
+  return PETSC_SUCCESS;
+}
+
+/* BVOrthogonalize (bvorthog.c:726-767) */
+PetscErrorCode BVOrthogonalize(BV V, Mat R)
+{
+  PetscInt m, n, l, k, ldw;
+  PetscScalar *W = NULL;
+  PetscErrorCode e = PETSC_SUCCESS;
+  PetscCall(BVCheckReady(V));
+  l = V->l;
+  k = V->k;
+  if (R) {
+    PetscCheck(!strcmp(R->type, MATSEQDENSE), PETSC_ERR_ARG_WRONG, "R must be a sequential dense matrix");
+    PetscCall(MatGetSize(R, &m, &n));
+    PetscCheck(m == n, PETSC_ERR_ARG_SIZ, "Mat argument is not square, it has %d rows and %d columns", m, n);
+    PetscCheck(n >= k, PETSC_ERR_ARG_SIZ, "Mat size %d is smaller than the number of BV active columns %d", n, k);
+  }
+  PetscCheck(!V->nc, PETSC_ERR_SUP, "Not implemented for BV with constraints, use BVOrthogonalizeColumn() instead");
+  if (l >= k) return PETSC_SUCCESS;
+  if (V->orthog_block == BV_ORTHOG_BLOCK_GS) {   /* column by column (bvorthog.c:507-547) */
+    PetscScalar *r = NULL, *h = NULL;
+    PetscInt ldr = 0;
+    if (R) PetscCall(DenseAt(R, &r, &ldr));
+    PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)(k + 1), &h));
+    for (PetscInt j = l; j < k && !e; j++) {
+      PetscReal norm = 0.0;
+      e = BVOrthogonalizeColumn(V, j, h, &norm, NULL);
+      if (e) break;
+      if (norm == 0.0) { e = PETSC_ERR_CONV_FAILED; fprintf(stderr, "Breakdown in BVOrthogonalize due to a linearly dependent column\n"); break; }
+      if (r) {
+        for (PetscInt i = 0; i < j; i++) r[i + j * ldr] = h[i];
+        r[j + j * ldr] = norm;
+      }
+      e = BVScaleColumn(V, j, 1.0 / norm);
+    }
+    free(h);
+    return e;
+  }
+  PetscCheck(V->orthog_block == BV_ORTHOG_BLOCK_CHOL || V->orthog_block == BV_ORTHOG_BLOCK_SVQB, PETSC_ERR_SUP,
+             "block orthogonalization %d is not implemented on the device (GS, CHOL and SVQB are)", (int)V->orthog_block);
+  PetscCheck(V->ops.gram, PETSC_ERR_SUP, "BV type %s has no block dot", V->type);
+  ldw = k;
+  PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)ldw * (size_t)k, &W));
+  if (l > 0) e = OrthoBlockGS(V, W, ldw);
+  if (!e) e = V->orthog_block == BV_ORTHOG_BLOCK_CHOL ? OrthoChol(V, W, ldw) : OrthoSVQB(V, W, ldw);
+  if (!e && R) {   /* BV_StoreCoeffsBlock_Default: columns l..k-1, rows 0..j (CHOL) or 0..k-1 (SVQB) */
+    PetscScalar *r;
+    PetscInt ldr;
+    PetscCall(DenseAt(R, &r, &ldr));
+    for (PetscInt j = l; j < k; j++) {
+      const PetscInt rows = V->orthog_block == BV_ORTHOG_BLOCK_CHOL ? j + 1 : k;
+      for (PetscInt i = 0; i < rows; i++) r[i + j * ldr] = W[i + j * ldw];
+    }
+  }
+  free(W);
+  return e;
+}
+
+/* BVMatProject (bvglobal.c:1093-1160): every entry of M(0:ky, 0:kx) but the leading
+   ly x lx block, M = Y^T A X (A = NULL: X holds A X already) */
+PetscErrorCode BVMatProject(BV X, Mat A, BV Y, Mat M)
+{
+  PetscInt m, n, ldm, lx, kx, ly, ky;
+  PetscScalar *mv, *g = NULL;
+  BV W = NULL;
+  PetscErrorCode e = PETSC_SUCCESS;
+  PetscCall(BVCheckReady(X));
+  PetscCall(BVCheckReady(Y));
+  PetscCheck(M && !strcmp(M->type, MATSEQDENSE), PETSC_ERR_ARG_WRONG, "M must be a sequential dense matrix");
+  PetscCall(MatGetSize(M, &m, &n));
+  PetscCheck(m >= Y->k, PETSC_ERR_ARG_SIZ, "Matrix M has %d rows, should have at least %d", m, Y->k);
+  PetscCheck(n >= X->k, PETSC_ERR_ARG_SIZ, "Matrix M has %d columns, should have at least %d", n, X->k);
+  PetscCheck(X->n == Y->n, PETSC_ERR_ARG_INCOMP, "Mismatching local dimension X %d, Y %d", X->n, Y->n);
+  PetscCheck(X->ops.gram, PETSC_ERR_SUP, "BV type %s has no block dot", X->type);
+  lx = X->l; kx = X->k; ly = Y->l; ky = Y->k;
+  if (!kx || !ky) return PETSC_SUCCESS;
+  PetscCall(DenseAt(M, &mv, &ldm));
+  PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)ky * (size_t)kx, &g));
+  if (A) {   /* W = A X(:, 0:kx), one product per column */
+    e = BVCreate(&W);
+    if (!e) e = BVSetSizes(W, X->n, X->N, kx);
+    if (!e) e = BVHIPKSSetPrecision(W, X->prec);
+    if (!e) e = BVSetType(W, X->type);
+    for (PetscInt j = 0; j < kx && !e; j++) {
+      Vec x = NULL, w = NULL;
+      e = BVGetColumn(X, j, &x);
+      if (!e) e = BVGetColumn(W, j, &w);
+      if (!e) e = MatMult(A, x, w);
+      if (w) BVRestoreColumn(W, j, &w);
+      if (x) BVRestoreColumn(X, j, &x);
+    }
+  }
+  if (!e) e = (A ? W : X)->ops.gram(A ? W : X, 0, kx, Y, 0, ky, g, ky);
+  if (!e)
+    for (PetscInt j = 0; j < kx; j++)
+      for (PetscInt i = 0; i < ky; i++)
+        if (i >= ly || j >= lx) mv[i + j * ldm] = g[i + j * ky];
+  if (W) BVDestroy(&W);
+  free(g);
+  return e;
 }
 
 /* BVOrthogonalizeColumn (bvorthog.c:285-320) */

@@ -26,6 +26,8 @@ typedef struct {
   double *d_Q;           /* ldq x ldq staging of the dense Q of BVMultInPlace */
   double *h_Q;           /* pinned host staging of Q */
   size_t Qcap;           /* elements of d_Q / h_Q */
+  double *d_G;           /* block dot result */
+  size_t Gcap;
   PetscInt ncols;        /* nc + m columns allocated */
   /* captured Arnoldi steps: step[j] makes column j+1 from column j; valid for
      the matrix step_csr and the orthogonalization settings they were made with */
@@ -132,6 +134,38 @@ static PetscErrorCode BVMultInPlace_HIPKS(BV bv, Mat Q, PetscInt s, PetscInt e)
   KSCall(ks_memcpy_htod_async(bv->ctx, h->d_Q, h->h_Q, sizeof(double) * need));
   KSCall(ks_bv_multinplace(bv->ctx, kp(bv), bv->n, colptr(bv, bv->l), bv->ld, K,
                            h->d_Q + (size_t)bv->l * ldq + bv->l, ldq, s - bv->l, e - bv->l));
+  return PETSC_SUCCESS;
+}
+
+/* G(0:ye-ys, 0:xe-xs) = Y(:, ys:ye)^T X(:, xs:xe) (the block dot, ks_bv_gram),
+   through a device array of the basis X, copied into host memory */
+static PetscErrorCode BVGram_HIPKS(BV X, PetscInt xs, PetscInt xe, BV Y, PetscInt ys, PetscInt ye, PetscScalar *G,
+                                   PetscInt ldg)
+{
+  BV_HIPKS *h = (BV_HIPKS *)X->data;
+  const PetscInt p = xe - xs, q = ye - ys;
+  size_t need = (size_t)p * (size_t)q;
+  double *tmp;
+  PetscCheck(!strcmp(Y->type, X->type) && Y->prec == X->prec && Y->n == X->n, PETSC_ERR_ARG_INCOMP,
+             "block dot of bases of different type, precision or length");
+  PetscCheck(xs >= -X->nc && xe <= X->m && ys >= -Y->nc && ye <= Y->m, PETSC_ERR_ARG_OUTOFRANGE, "column range out of bounds");
+  if (p <= 0 || q <= 0) return PETSC_SUCCESS;
+  if (need > h->Gcap) {
+    KSCall(ks_ctx_sync(X->ctx));
+    KSCall(ks_free(h->d_G));
+    h->d_G = NULL;
+    KSCall(ks_malloc((void **)&h->d_G, sizeof(double) * need));
+    h->Gcap = need;
+  }
+  /* Y's columns live in Y's storage; both bases are on X's stream */
+  KSCall(ks_bv_gram(X->ctx, kp(X), X->n, colptr(X, xs), X->ld, p, colptr(Y, ys), Y->ld, q, h->d_G, q));
+  PetscCall(PetscMallocZero(sizeof(double) * need, &tmp));
+  {
+    int e = ks_memcpy_dtoh(X->ctx, tmp, h->d_G, sizeof(double) * need);
+    if (e) { free(tmp); KSCall(e); }
+  }
+  for (PetscInt j = 0; j < p; j++) memcpy(G + (size_t)j * ldg, tmp + (size_t)j * q, sizeof(double) * (size_t)q);
+  free(tmp);
   return PETSC_SUCCESS;
 }
 
@@ -346,6 +380,7 @@ static PetscErrorCode BVDestroy_HIPKS(BV bv)
   KSCall(ks_free(h->d_buf));
   KSCall(ks_free(h->d_q));
   KSCall(ks_free(h->d_Q));
+  KSCall(ks_free(h->d_G));
   KSCall(ks_host_free(h->h_Q));
   free(h);
   bv->data = NULL;
@@ -386,6 +421,7 @@ PetscErrorCode BVCreate_HIPKS(BV bv)
   bv->ops.dotvec = BVDotVec_HIPKS;
   bv->ops.multvec = BVMultVec_HIPKS;
   bv->ops.multinplace = BVMultInPlace_HIPKS;
+  bv->ops.gram = BVGram_HIPKS;
   bv->ops.getbuffer = BVGetBuffer_HIPKS;
   bv->ops.orthonormalizecolumn = BVOrthonormalizeColumn_HIPKS;
   bv->ops.matkrylovbegin = BVMatKrylovBegin_HIPKS;

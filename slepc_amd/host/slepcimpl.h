@@ -128,6 +128,9 @@ typedef struct _BVOps {
      breakdown position, beta and the new coefficient columns (host mirror). */
   PetscErrorCode (*matkrylovbegin)(BV, Mat, PetscInt k, PetscInt m);
   PetscErrorCode (*matkrylovend)(BV, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown);
+  /* G(0:ye-ys, 0:xe-xs) = Y(:, ys:ye)^T X(:, xs:xe) into host memory (leading
+     dimension ldg): the block dot under BVDot, BVOrthogonalize, BVMatProject */
+  PetscErrorCode (*gram)(BV X, PetscInt xs, PetscInt xe, BV Y, PetscInt ys, PetscInt ye, PetscScalar *G, PetscInt ldg);
 } BVOps;
 
 struct _p_BV {
@@ -138,6 +141,7 @@ struct _p_BV {
   PetscInt l, k, nc, ld;     /* active columns, constraints, leading dimension */
   BVOrthogType orthog_type;
   BVOrthogRefineType orthog_ref;
+  BVOrthogBlockType orthog_block;   /* method of BVOrthogonalize */
   PetscReal orthog_eta;
   PetscReal deftol;
   PetscScalar *buffer;       /* host mirror of the (nc+m) x m coefficient buffer */
