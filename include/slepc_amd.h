@@ -46,6 +46,7 @@ typedef int    PetscBool;
 #define PETSC_ERR_ARG_SIZ           60
 #define PETSC_ERR_ARG_WRONG         62
 #define PETSC_ERR_ARG_OUTOFRANGE    63
+#define PETSC_ERR_FP                72
 #define PETSC_ERR_ARG_WRONGSTATE    73
 #define PETSC_ERR_ARG_INCOMP        75
 #define PETSC_ERR_ARG_NULL          85
@@ -175,6 +176,11 @@ PetscErrorCode BVGetNumConstraints(BV bv, PetscInt *nc);
 PetscErrorCode BVDotVec(BV bv, Vec y, PetscScalar m[]);
 PetscErrorCode BVMultVec(BV bv, PetscScalar alpha, PetscScalar beta, Vec y, PetscScalar q[]);
 PetscErrorCode BVMultInPlace(BV bv, Mat Q, PetscInt s, PetscInt e);
+/* column j = beta * column j + alpha * V(:, l:j) q (bvops.c:165-192); q has j - l entries */
+PetscErrorCode BVMultColumn(BV bv, PetscScalar alpha, PetscScalar beta, PetscInt j, PetscScalar q[]);
+/* scale the active columns to unit norm; columns i, i+1 with eigi[i] != 0 share one
+   norm (bvglobal.c:829-849, bvlapack.c:88-130) */
+PetscErrorCode BVNormalize(BV bv, PetscScalar eigi[]);
 /* M = Y^T X over the active columns (bvglobal.c:86-120): rows ly..ky-1, columns lx..kx-1 of M */
 PetscErrorCode BVDot(BV X, BV Y, Mat M);
 /* QR of the active columns (bvorthog.c:726-767, methods bvorthog.c:489-663): leading columns 0..l-1 are kept
@@ -220,6 +226,11 @@ PetscErrorCode DSUpdateExtraRow(DS ds);
 PetscErrorCode DSVectors(DS ds, DSMatType mat, PetscInt *j, PetscReal *rnorm);
 PetscErrorCode DSGetTruncateSize(DS ds, PetscInt l, PetscInt n, PetscInt *k);
 PetscErrorCode DSTruncate(DS ds, PetscInt n, PetscBool trim);
+/* A + g b^T with g = (A - tau I)^-T b, b = beta e_n (harmonic Rayleigh-Ritz), or with
+   recover the translation undone on the restarted decomposition; gamma = sqrt(1 + g^T g)
+   (dsops.c:1050-1066, DSNHEP: dsnhep.c:466-535) */
+PetscErrorCode DSTranslateHarmonic(DS ds, PetscScalar tau, PetscReal beta, PetscBool recover, PetscScalar *g,
+                                   PetscReal *gamma);
 /* sorting criterion: which (EPSWhich values) and target */
 PetscErrorCode DSSetSortCriterion(DS ds, int which, PetscScalar target);
 PetscErrorCode DSDestroy(DS *ds);
@@ -240,6 +251,9 @@ typedef enum { EPS_LARGEST_MAGNITUDE = 1, EPS_SMALLEST_MAGNITUDE = 2, EPS_LARGES
 typedef enum { EPS_CONVERGED_TOL = 1, EPS_CONVERGED_USER = 3, EPS_DIVERGED_ITS = -1,
                EPS_DIVERGED_BREAKDOWN = -2, EPS_CONVERGED_ITERATING = 0 } EPSConvergedReason;
 typedef enum { EPS_ERROR_ABSOLUTE = 0, EPS_ERROR_RELATIVE = 1, EPS_ERROR_BACKWARD = 2 } EPSErrorType;
+/* include/slepceps.h:94-100; Krylov-Schur supports EPS_RITZ and EPS_HARMONIC */
+typedef enum { EPS_RITZ = 0, EPS_HARMONIC = 1, EPS_HARMONIC_RELATIVE = 2, EPS_HARMONIC_RIGHT = 3,
+               EPS_HARMONIC_LARGEST = 4, EPS_REFINED = 5, EPS_REFINED_HARMONIC = 6 } EPSExtraction;
 typedef PetscErrorCode (*EPSMonitorFn)(EPS, PetscInt its, PetscInt nconv, PetscScalar *eigr,
                                        PetscScalar *eigi, PetscReal *errest, PetscInt nest, void *ctx);
 
@@ -251,6 +265,10 @@ PetscErrorCode EPSSetOperators(EPS eps, Mat A, Mat B);
 PetscErrorCode EPSSetProblemType(EPS eps, EPSProblemType type);
 PetscErrorCode EPSSetWhichEigenpairs(EPS eps, EPSWhich which);
 PetscErrorCode EPSSetTarget(EPS eps, PetscScalar target);
+PetscErrorCode EPSGetTarget(EPS eps, PetscScalar *target);
+/* EPSSetExtraction (epsopts.c): harmonic extraction targets interior eigenvalues near the target */
+PetscErrorCode EPSSetExtraction(EPS eps, EPSExtraction extr);
+PetscErrorCode EPSGetExtraction(EPS eps, EPSExtraction *extr);
 PetscErrorCode EPSSetDimensions(EPS eps, PetscInt nev, PetscInt ncv, PetscInt mpd);
 PetscErrorCode EPSGetDimensions(EPS eps, PetscInt *nev, PetscInt *ncv, PetscInt *mpd);
 PetscErrorCode EPSSetTolerances(EPS eps, PetscReal tol, PetscInt maxits);

@@ -49,6 +49,9 @@ extern void   dpotrf_(const char*, const int*, double*, const int*, int*);
 extern void   dtrtri_(const char*, const char*, const int*, double*, const int*, int*);
 extern void   dsyev_(const char*, const char*, const int*, double*, const int*, double*, double*, const int*,
                      int*);
+extern void   dgetrf_(const int*, const int*, double*, const int*, int*, int*);
+extern void   dgetrs_(const char*, const int*, const int*, const double*, const int*, const int*, double*,
+                      const int*, int*);
 extern int    MKL_Set_Threading_Layer(int);     /* MKL service (mkl_rt), by value: 3 = GNU OpenMP */
 
 /* MKL threads through the same GNU OpenMP runtime as the loops here; its Intel
@@ -838,6 +841,68 @@ static void ds_truncate(ods *ds, int n, int trim)
   ds->state = trim ? DS_RAW : DS_TRUNCATED;
 }
 
+/* DSTranslateHarmonic_NHEP (dsnhep.c:466-535); g has ld entries */
+static int ds_translate_harmonic(ods *ds, double tau, double beta, int recover, double *g, double *gammaout)
+{
+  int n = ds->n, ld = ds->ld, one = 1, info = 0;
+  double *A = ds->A, gamma;
+  if (!recover) {
+    double *B = (double*)malloc(sizeof(double) * (size_t)ld * ld);
+    int *ipiv = (int*)malloc(sizeof(int) * (size_t)ld);
+    if (!B || !ipiv) { free(B); free(ipiv); return -55; }
+    memcpy(B, A, sizeof(double) * (size_t)ld * ld);
+    memset(g, 0, sizeof(double) * (size_t)n);
+    g[n - 1] = beta;                                                /* b = beta e_n */
+    for (int i = 0; i < n; i++) B[i + (size_t)i * ld] -= tau;
+    dgetrf_(&n, &n, B, &ld, ipiv, &info);
+    if (!info) dgetrs_("C", &n, &one, B, &ld, ipiv, g, &ld, &info);  /* g = (A - tau I)^-T b */
+    free(B); free(ipiv);
+    if (info) OR_ERR(-81, "harmonic translation: LAPACK info %d", info);
+    for (int i = 0; i < n; i++) A[i + (size_t)(n - 1) * ld] += g[i] * beta;
+  } else {
+    int ncol = ds->l + ds->k;
+    double *ghat = (double*)calloc((size_t)ld, sizeof(double));
+    if (!ghat) return -55;
+    for (int j = 0; j < ncol; j++) {                                 /* ghat = -Q(:,idx)^T g */
+      double t = 0.0;
+      for (int i = 0; i < n; i++) t += ds->Q[i + (size_t)j * ld] * g[i];
+      ghat[j] = -t;
+    }
+    for (int i = 0; i < ncol; i++)                                  /* A += ghat b^T Q(:,l:l+k) */
+      for (int j = ds->l; j < ds->l + ds->k; j++)
+        A[i + (size_t)j * ld] += ghat[i] * ds->Q[n - 1 + (size_t)j * ld] * beta;
+    for (int i = 0; i < n; i++) {                                   /* g~ = g + Q(:,idx) ghat */
+      double t = 0.0;
+      for (int j = 0; j < ncol; j++) t += ds->Q[i + (size_t)j * ld] * ghat[j];
+      g[i] += t;
+    }
+    free(ghat);
+  }
+  gamma = hypot(1.0, dnrm2_(&n, g, &one));
+  if (gammaout) *gammaout = gamma;
+  if (recover && ds->extrarow)
+    for (int j = ds->l; j < ds->l + ds->k; j++) A[ds->n + (size_t)j * ld] *= gamma;
+  ds->state = DS_RAW;
+  return 0;
+}
+
+/* the translation alone, on an (n+1) x n Hessenberg H with H(n, n-1) = beta (ld >= n+1):
+   A gets H + g b^T, g and gamma are returned */
+int or_ds_translate_harmonic(int ld, int n, double *A, double tau, double beta, double *g, double *gamma)
+{
+  or_mkl_layer();
+  ods ds;
+  int e;
+  if (ds_alloc(&ds, DS_NHEP, ld)) return -55;
+  ds.extrarow = 1;
+  ds_setdims(&ds, n, 0, 0);
+  memcpy(ds.A, A, sizeof(double) * (size_t)ld * ld);
+  e = ds_translate_harmonic(&ds, tau, beta, 0, g, gamma);
+  if (!e) memcpy(A, ds.A, sizeof(double) * (size_t)ld * ld);
+  ds_free(&ds);
+  return e;
+}
+
 int or_ds_run(int type, int ld, int n, int l, int k, int compact, int extrarow,
               int state, double *A, double *T, int which, double *wr, double *wi,
               double *Q, int jvec, double *rnorm, double *X)
@@ -871,6 +936,7 @@ int or_ds_run(int type, int ld, int n, int l, int k, int compact, int extrarow,
 /* ------------------------------------------------------------------ EPS + Krylov-Schur */
 typedef struct {
   int n, nev, ncv, mpd, max_it, which, hermitian, lock, nconv, its, reason;
+  int harmonic, dsherm;           /* dsherm: Lanczos + DSHEP (Hermitian and Ritz extraction) */
   double tol, target, keep;
   const int *rowptr, *colidx;
   const double *val;
@@ -908,7 +974,7 @@ static int mat_krylov(oeps *eps, int k, int *m, double *beta, int *breakdown)
     if (lindep) { *m = j + 1; break; }
   }
   *breakdown = lindep;
-  if (eps->hermitian) {
+  if (eps->dsherm) {
     double *alpha = ds->T, *betat = ds->T + ld;
     for (int j = k; j < *m; j++) {
       alpha[j] = a[bv->nc + j + (size_t)(j + 1) * ldb];
@@ -940,7 +1006,7 @@ static int get_start_vector(oeps *eps, int i, int have_initial, int *breakdown)
 }
 
 /* EPSKrylovConvergence (epskrylov.c:207-295), STSHIFT sigma=0, trivial RG, relative test */
-static int krylov_convergence(oeps *eps, int kini, int nits, double beta, int *kout)
+static int krylov_convergence(oeps *eps, int kini, int nits, double beta, double corrf, int *kout)
 {
   int k, marker = -1;
   double resnorm;
@@ -948,7 +1014,7 @@ static int krylov_convergence(oeps *eps, int kini, int nits, double beta, int *k
     double re = eps->eigr[k], im = eps->eigi[k], w;
     int newk = k;
     OR_CALL(ds_vectors(&eps->ds, &newk, &resnorm));
-    resnorm *= beta;                                               /* corrf = 1 */
+    resnorm *= beta * corrf;                                       /* corrf: gamma of harmonic KS */
     w = hypot(re, im);                                             /* EPSConvergedRelative */
     eps->errest[k] = (w != 0.0) ? resnorm / w : DBL_MAX;
     if (marker == -1 && eps->errest[k] >= eps->tol) marker = k;
@@ -960,11 +1026,11 @@ static int krylov_convergence(oeps *eps, int kini, int nits, double beta, int *k
   return 0;
 }
 
-/* EPSSolve_KrylovSchur_Default (krylovschur.c:227-337), Ritz extraction */
-static int ks_solve(oeps *eps, int have_initial, int restart_cap)
+/* EPSSolve_KrylovSchur_Default (krylovschur.c:227-337), Ritz or harmonic extraction */
+static int ks_loop(oeps *eps, int have_initial, int restart_cap, double *g)
 {
   int k = 0, l = 0, nv, nconv, breakdown = 0;
-  double beta = 0.0, t0;
+  double beta = 0.0, t0, gamma = 1.0;
   obv *V = &eps->V;
   ods *ds = &eps->ds;
 
@@ -980,10 +1046,11 @@ static int ks_solve(oeps *eps, int have_initial, int restart_cap)
     V->l = eps->nconv; V->k = nv;                                  /* BVSetActiveColumns */
 
     t0 = wtime();
+    if (eps->harmonic) OR_CALL(ds_translate_harmonic(ds, eps->target, beta, 0, g, &gamma));
     OR_CALL(ds_solve(ds, eps->eigr, eps->eigi));
     OR_CALL(ds_sort(ds, eps->eigr, eps->eigi));
     ds_update_extra_row(ds);
-    OR_CALL(krylov_convergence(eps, eps->nconv, nv - eps->nconv, beta, &k));
+    OR_CALL(krylov_convergence(eps, eps->nconv, nv - eps->nconv, beta, gamma, &k));
     eps->t_ds += wtime() - t0;
 
     /* EPSStoppingBasic (epsdefault.c:290-302) */
@@ -996,7 +1063,7 @@ static int ks_solve(oeps *eps, int have_initial, int restart_cap)
     if (eps->reason != OR_CONVERGED_ITERATING || breakdown || k == nv) l = 0;
     else {
       l = (int)((nv - k) * eps->keep); if (l < 1) l = 1;
-      if (!eps->hermitian) ds_truncate_size(ds, k, nv, &l);
+      if (!eps->dsherm) ds_truncate_size(ds, k, nv, &l);
     }
     if (!eps->lock && l > 0) { l += k; k = 0; }
 
@@ -1006,7 +1073,18 @@ static int ks_solve(oeps *eps, int have_initial, int restart_cap)
           OR_CALL(get_start_vector(eps, k, have_initial, &breakdown));
           if (breakdown) eps->reason = OR_DIVERGED_BREAKDOWN;
         }
-      } else ds_truncate(ds, k + l, 0);
+      } else {
+        if (eps->harmonic) {                                       /* krylovschur.c:310-319 */
+          ds_setdims(ds, nv, k, l);
+          OR_CALL(ds_translate_harmonic(ds, 0.0, beta, 1, g, &gamma));
+          V->l = 0; V->k = nv;                                     /* BVMultColumn(V,-1,1,nv,g) */
+          bv_multvec(V, -1.0, 1.0, COL(V, nv), g);
+          bv_scalecol(V, nv, 1.0 / gamma);
+          V->l = eps->nconv; V->k = nv;
+          ds_setdims(ds, nv, k, nv);
+        }
+        ds_truncate(ds, k + l, 0);
+      }
     }
     /* BVMultInPlace(V, Q, nconv, k+l) */
     bv_multinplace(V, ds->Q, ds->ld, eps->nconv, k + l);
@@ -1016,6 +1094,16 @@ static int ks_solve(oeps *eps, int have_initial, int restart_cap)
   }
   ds_truncate(ds, eps->nconv, 1);
   return 0;
+}
+
+static int ks_solve(oeps *eps, int have_initial, int restart_cap)
+{
+  double *g = NULL;                                                /* translation vector, harmonic only */
+  int e;
+  if (eps->harmonic && !(g = (double*)calloc((size_t)eps->ds.ld, sizeof(double)))) return -55;
+  e = ks_loop(eps, have_initial, restart_cap, g);
+  free(g);
+  return e;
 }
 
 /* EPSComputeResidualNorm_Private (epssolve.c:666-718), standard problem */
@@ -1055,6 +1143,9 @@ int or_eps_solve(int n, const int *rowptr, const int *colidx, const double *val,
   eps.n = n; eps.rowptr = rowptr; eps.colidx = colidx; eps.val = val;
   eps.nev = opts->nev; eps.which = opts->which ? opts->which : OR_LARGEST_MAGNITUDE;
   eps.target = opts->target; eps.hermitian = opts->hermitian;
+  eps.harmonic = opts->extraction == 1;
+  if (opts->extraction != 0 && opts->extraction != 1) OR_ERR(-56, "Unsupported extraction type");
+  eps.dsherm = eps.hermitian && !eps.harmonic;
   eps.tol = opts->tol > 0 ? opts->tol : 1e-8;
   eps.keep = opts->keep > 0 ? opts->keep : 0.5;
   eps.lock = opts->lock;
@@ -1086,8 +1177,8 @@ int or_eps_solve(int n, const int *rowptr, const int *colidx, const double *val,
   eps.errest = (double*)calloc(requested, sizeof(double));
   eps.perm = (int*)calloc(requested, sizeof(int));
   if (!eps.V.V || !eps.V.buffer || !eps.eigr || !eps.eigi || !eps.errest || !eps.perm) { e = -55; goto done; }
-  if ((e = ds_alloc(&eps.ds, eps.hermitian ? DS_HEP : DS_NHEP, ncv + 1))) goto done;
-  eps.ds.compact = eps.hermitian; eps.ds.extrarow = 1; eps.ds.sc = eps.sc;
+  if ((e = ds_alloc(&eps.ds, eps.dsherm ? DS_HEP : DS_NHEP, ncv + 1))) goto done;
+  eps.ds.compact = eps.dsherm; eps.ds.extrarow = 1; eps.ds.sc = eps.sc;
 
   /* deflation space: BVInsertConstraints (orthonormalized, bvfunc.c:411) */
   if (eps.V.nc) {
@@ -1120,11 +1211,20 @@ int or_eps_solve(int n, const int *rowptr, const int *colidx, const double *val,
   t0 = wtime();
   if ((e = ks_solve(&eps, v0 != NULL, opts->restart_cap))) goto done;
 
-  /* EPSComputeVectors_Schur (epsdefault.c:105-169) for NHEP; Hermitian: no-op */
-  if (!eps.hermitian && eps.nconv > 0) {
+  /* EPSComputeVectors_Schur (epsdefault.c:105-169) for NHEP; Hermitian with DSHEP: no-op;
+     Hermitian with harmonic extraction: EPSComputeVectors_Hermitian (epsdefault.c:53-72) */
+  if (!eps.dsherm && eps.nconv > 0) {
     if ((e = ds_vectors(&eps.ds, NULL, NULL))) goto done;
     eps.V.l = 0; eps.V.k = eps.nconv;
-    bv_multinplace(&eps.V, eps.ds.X, eps.ds.ld, 0, eps.nconv);
+    bv_multinplace(&eps.V, eps.ds.X, eps.ds.ld, 0, eps.hermitian ? eps.ds.n : eps.nconv);
+    if (eps.hermitian)                                             /* BVNormalize(V, eigi) */
+      for (int j = 0; j < eps.nconv; j++) {
+        int one = 1, pair = eps.eigi[j] != 0.0 && j + 1 < eps.nconv;
+        double nr = dnrm2_(&n, COL(&eps.V, j), &one);
+        if (pair) nr = hypot(nr, dnrm2_(&n, COL(&eps.V, j + 1), &one));
+        bv_scalecol(&eps.V, j, 1.0 / nr);
+        if (pair) bv_scalecol(&eps.V, ++j, 1.0 / nr);
+      }
   }
   /* reorder conjugate pairs (epssolve.c:161-175) */
   for (int i = 0; i < eps.nconv - 1; i++) {

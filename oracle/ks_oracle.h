@@ -40,6 +40,7 @@ typedef struct {
   int    refine;                 /* OR_REFINE_* */
   int    nthreads;               /* OpenMP threads for SpMV/BLAS; <= 0: leave as is */
   int    restart_cap;            /* > 0: stop after this many restarts (bounded timing sample) */
+  int    extraction;             /* 0 EPS_RITZ, 1 EPS_HARMONIC (translation by target) */
 } or_opts;
 
 typedef struct {
@@ -67,6 +68,8 @@ int or_orthonormalize_column(int n, int ld, double *V, int nc, int j, int refine
 
 /* BVMultInPlace: V(:,s:e-1) = V(:,l:k-1) * Q(l:k-1,s:e-1), columns counted
    after nc constraints. */
+/* DSTranslateHarmonic (translate) on an n x n projected matrix with extra row */
+int or_ds_translate_harmonic(int ld, int n, double *A, double tau, double beta, double *g, double *gamma);
 /* M(ly:ky, lx:kx) = Y(:, ly:ky)^T X(:, lx:kx) (BVDot) */
 int or_bv_dot(int n, const double *X, int ldx, int lx, int kx, const double *Y, int ldy, int ly, int ky,
               double *M, int ldm);

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "libks_oracle.so")
 LARGEST_MAGNITUDE, SMALLEST_MAGNITUDE, LARGEST_REAL, SMALLEST_REAL = 1, 2, 3, 4
 LARGEST_IMAGINARY, SMALLEST_IMAGINARY, TARGET_MAGNITUDE, TARGET_REAL = 5, 6, 7, 8
 REFINE_IFNEEDED, REFINE_NEVER, REFINE_ALWAYS = 0, 1, 2
+RITZ, HARMONIC = 0, 1
 
 
 class Opts(C.Structure):
@@ -25,7 +26,7 @@ class Opts(C.Structure):
                 ("tol", C.c_double), ("which", C.c_int), ("target", C.c_double),
                 ("hermitian", C.c_int), ("keep", C.c_double), ("lock", C.c_int),
                 ("eta", C.c_double), ("refine", C.c_int), ("nthreads", C.c_int),
-                ("restart_cap", C.c_int)]
+                ("restart_cap", C.c_int), ("extraction", C.c_int)]
 
 
 class Info(C.Structure):
@@ -99,6 +100,22 @@ def multinplace(V: np.ndarray, Q: np.ndarray, l: int, k: int, s: int, e: int, nc
     assert rc == 0
 
 
+def ds_translate_harmonic(H: np.ndarray, tau: float, beta: float):
+    """DSTranslateHarmonic of the leading n x n block of H ((n+1) x n, Hessenberg):
+    returns (H + g b^T, g, gamma)."""
+    n = H.shape[1]
+    ld = n + 1
+    A = np.zeros((ld, ld), order="F")
+    A[:n + 1, :n] = H
+    g = np.zeros(ld)
+    gamma = C.c_double()
+    rc = lib().or_ds_translate_harmonic(C.c_int(ld), C.c_int(n), _p(A), C.c_double(tau), C.c_double(beta), _p(g),
+                                        C.byref(gamma))
+    if rc:
+        raise RuntimeError(f"or_ds_translate_harmonic failed ({rc})")
+    return A[:n, :n].copy(), g[:n].copy(), gamma.value
+
+
 BLOCK_GS, BLOCK_CHOL, BLOCK_SVQB = 0, 1, 4
 
 
@@ -149,12 +166,12 @@ def eps_solve(csr, nev: int, ncv: int = 0, mpd: int = 0, tol: float = 1e-8,
               which: int = LARGEST_MAGNITUDE, hermitian: bool = True, target: float = 0.0,
               v0=None, deflation=None, max_it: int = 0, keep: float = 0.5, lock: bool = True,
               eta: float = 0.7071, refine: int = REFINE_IFNEEDED, nthreads: int = 0,
-              restart_cap: int = 0, want_vectors: bool = True):
+              restart_cap: int = 0, want_vectors: bool = True, extraction: int = 0):
     rowptr, colidx, val = _csr_args(csr)
     n = len(rowptr) - 1
     o = Opts(nev=nev, ncv=ncv, mpd=mpd, max_it=max_it, tol=tol, which=which, target=target,
              hermitian=int(hermitian), keep=keep, lock=int(lock), eta=eta, refine=refine,
-             nthreads=nthreads, restart_cap=restart_cap)
+             nthreads=nthreads, restart_cap=restart_cap, extraction=extraction)
     ncv_eff = ncv if ncv > 0 else max(2 * nev, nev + 15)
     ncv_eff = min(ncv_eff, n) if ncv_eff > 0 else n
     size = max(ncv_eff, nev) + 2

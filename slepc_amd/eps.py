@@ -272,6 +272,8 @@ class EPS:
     # EPSErrorType
     ERROR_ABSOLUTE, ERROR_RELATIVE = 0, 1
     REFINE_IFNEEDED, REFINE_NEVER, REFINE_ALWAYS = 0, 1, 2
+    # EPSExtraction (Krylov-Schur: RITZ, HARMONIC)
+    RITZ, HARMONIC = 0, 1
 
     def __init__(self):
         initialize()
@@ -294,6 +296,9 @@ class EPS:
 
     def setTarget(self, t: float):
         _chk(lib().EPSSetTarget(self.h, C.c_double(t)), "EPSSetTarget")
+
+    def setExtraction(self, extr: int):
+        _chk(lib().EPSSetExtraction(self.h, C.c_int(extr)), "EPSSetExtraction")
 
     def setDimensions(self, nev=-2, ncv=-1, mpd=-1):
         _chk(lib().EPSSetDimensions(self.h, C.c_int(nev), C.c_int(ncv), C.c_int(mpd)), "EPSSetDimensions")
@@ -425,7 +430,8 @@ class EPS:
 
 
 def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=True, v0=None, max_it=-1,
-          prec=F64, deflation=None, lock=True, keep=0.5, want_vectors=True, mixed_tol=0.0, step_graphs=False):
+          prec=F64, deflation=None, lock=True, keep=0.5, want_vectors=True, mixed_tol=0.0, step_graphs=False,
+          target=None, extraction=0):
     """One-call convenience wrapper: returns a dict like the oracle's eps_solve.
     FP32 values in csr[2] keep the operator in FP32; ``prec`` is the basis precision."""
     A = Mat(csr, prec=F32 if np.asarray(csr[2]).dtype == np.float32 else F64)
@@ -435,6 +441,10 @@ def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=Tru
     eps.setDimensions(nev, ncv)
     eps.setTolerances(tol, max_it)
     eps.setWhichEigenpairs(which)
+    if target is not None:
+        eps.setTarget(target)
+    if extraction:
+        eps.setExtraction(extraction)
     eps.setKrylovSchurLocking(lock)
     eps.setKrylovSchurRestart(keep)
     if mixed_tol > 0:

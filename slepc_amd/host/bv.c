@@ -302,6 +302,44 @@ PetscErrorCode BVMultVec(BV bv, PetscScalar alpha, PetscScalar beta, Vec y, Pets
   return bv->ops.multvec(bv, alpha, beta, y, q);
 }
 
+/* BVMultColumn (bvops.c:165-192): column j = beta * column j + alpha * V(:, l:j) q,
+   with the active columns set to l..j for the product */
+PetscErrorCode BVMultColumn(BV bv, PetscScalar alpha, PetscScalar beta, PetscInt j, PetscScalar q[])
+{
+  PetscInt ksave;
+  Vec y = NULL;
+  PetscErrorCode e;
+  PetscCall(BVCheckReady(bv));
+  PetscCheck(j >= 0 && j < bv->m, PETSC_ERR_ARG_OUTOFRANGE, "Index j=%d but BV only has %d columns", j, bv->m);
+  PetscCheck(q || j <= bv->l, PETSC_ERR_ARG_NULL, "coefficients are required");
+  ksave = bv->k;
+  bv->k = j;
+  e = BVGetColumn(bv, j, &y);
+  if (!e) e = bv->ops.multvec(bv, alpha, beta, y, q);
+  if (y) BVRestoreColumn(bv, j, &y);
+  bv->k = ksave;
+  return e;
+}
+
+/* BVNormalize (bvglobal.c:829-849) as BVNormalize_LAPACK_Private (bvlapack.c:88-130):
+   active columns to unit norm, a conjugate pair (eigi[i] != 0) by the Frobenius
+   norm of its two columns */
+PetscErrorCode BVNormalize(BV bv, PetscScalar eigi[])
+{
+  PetscCall(BVCheckReady(bv));
+  for (PetscInt j = bv->l; j < bv->k; j++) {
+    PetscReal nr, ni = 0.0;
+    PetscBool pair = eigi && eigi[j] != 0.0 && j + 1 < bv->k;
+    PetscCall(BVNormColumn(bv, j, NORM_2, &nr));
+    if (pair) PetscCall(BVNormColumn(bv, j + 1, NORM_2, &ni));
+    nr = SlepcAbsEigenvalue(nr, ni);
+    PetscCheck(nr > 0.0, PETSC_ERR_FP, "zero column %d cannot be normalized", j);
+    PetscCall(BVScaleColumn(bv, j, 1.0 / nr));
+    if (pair) PetscCall(BVScaleColumn(bv, ++j, 1.0 / nr));
+  }
+  return PETSC_SUCCESS;
+}
+
 /* BVMultInPlace (bvops.c:220-245) */
 PetscErrorCode BVMultInPlace(BV bv, Mat Q, PetscInt s, PetscInt e)
 {

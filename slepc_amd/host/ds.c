@@ -18,6 +18,9 @@
 extern void   dgemv_(const char *, const int *, const int *, const double *, const double *, const int *,
                      const double *, const int *, const double *, double *, const int *);
 extern double dnrm2_(const int *, const double *, const int *);
+extern void   dgetrf_(const int *, const int *, double *, const int *, int *, int *);
+extern void   dgetrs_(const char *, const int *, const int *, const double *, const int *, const int *, double *,
+                      const int *, int *);
 extern void   dlartg_(double *, double *, double *, double *, double *);
 extern void   drot_(const int *, double *, const int *, double *, const int *, const double *, const double *);
 extern void   dsteqr_(const char *, const int *, double *, double *, double *, const int *, double *, int *);
@@ -268,6 +271,19 @@ PetscErrorCode DSTruncate(DS ds, PetscInt n, PetscBool trim)
   DS_SEQ_END;
   PetscCall(ierr);
   ds->state = trim ? DS_STATE_RAW : DS_STATE_TRUNCATED;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode DSTranslateHarmonic(DS ds, PetscScalar tau, PetscReal beta, PetscBool recover, PetscScalar *g,
+                                   PetscReal *gamma)
+{
+  PetscErrorCode ierr;
+  PetscCheck(ds->ops.transharm, PETSC_ERR_SUP, "DS type %s has no harmonic translation", ds->type);
+  DS_SEQ_BEGIN;
+  ierr = ds->ops.transharm(ds, tau, beta, recover, g, gamma);
+  DS_SEQ_END;
+  PetscCall(ierr);
+  ds->state = DS_STATE_RAW;
   return PETSC_SUCCESS;
 }
 
@@ -655,8 +671,51 @@ static PetscErrorCode DSTruncate_NHEP(DS ds, PetscInt n, PetscBool trim)
   return PETSC_SUCCESS;
 }
 
+/* DSTranslateHarmonic_NHEP (dsnhep.c:466-535).  Translate: g = (A - tau I)^-T b
+   with b = beta e_n, A(:, n-1) += beta g.  Recover (after the sort, with l = the
+   converged count and k = the kept count): ghat = -Q(:, 0:l+k)^T g,
+   A(0:l+k, l:l+k) += ghat Q(n-1, l:l+k) beta, g += Q(:, 0:l+k) ghat.  Both
+   return gamma = sqrt(1 + g^T g); recover scales the extra row by it. */
+static PetscErrorCode DSTranslateHarmonic_NHEP(DS ds, PetscScalar tau, PetscReal beta, PetscBool recover,
+                                               PetscScalar *gin, PetscReal *gammaout)
+{
+  int n = ds->n, ld = ds->ld, one = 1, info;
+  PetscScalar *A = ds->mat[DS_MAT_A], *g = gin ? gin : ds->work;
+  PetscReal gamma = 1.0;
+  if (!recover) {
+    double *B = NULL;
+    int *ipiv = ds->iwork;
+    PetscCall(PetscMallocZero(sizeof(double) * (size_t)ld * (size_t)ld, &B));
+    memcpy(B, A, sizeof(double) * (size_t)ld * (size_t)ld);
+    for (int i = 0; i < n; i++) g[i] = 0.0;
+    g[n - 1] = beta;
+    for (int i = 0; i < n; i++) B[i + (size_t)i * ld] -= tau;
+    dgetrf_(&n, &n, B, &ld, ipiv, &info);
+    if (info) { free(B); LAPACK_CHECK("getrf", info); }
+    dgetrs_("C", &n, &one, B, &ld, ipiv, g, &ld, &info);
+    free(B);
+    LAPACK_CHECK("getrs", info);
+    for (int i = 0; i < n; i++) A[i + (size_t)(n - 1) * ld] += g[i] * beta;
+  } else {
+    const PetscScalar *Q = ds->mat[DS_MAT_Q];
+    double *ghat = ds->work + 2 * (size_t)ld, dmone = -1.0, done = 1.0, dzero = 0.0;
+    int ncol = ds->l + ds->k;
+    dgemv_("C", &n, &ncol, &dmone, Q, &ld, g, &one, &dzero, ghat, &one);
+    for (PetscInt i = 0; i < ds->l + ds->k; i++)
+      for (PetscInt j = ds->l; j < ds->l + ds->k; j++)
+        A[i + (size_t)j * ld] += ghat[i] * Q[n - 1 + (size_t)j * ld] * beta;
+    dgemv_("N", &n, &ncol, &done, Q, &ld, ghat, &one, &done, g, &one);
+  }
+  if (gammaout || (recover && ds->extrarow)) gamma = SlepcAbsEigenvalue(1.0, dnrm2_(&n, g, &one));
+  if (gammaout) *gammaout = gamma;
+  if (recover && ds->extrarow)
+    for (PetscInt j = ds->l; j < ds->l + ds->k; j++) A[ds->n + (size_t)j * ld] *= gamma;
+  return PETSC_SUCCESS;
+}
+
 PetscErrorCode DSCreate_NHEP(DS ds)
 {
+  ds->ops.transharm = DSTranslateHarmonic_NHEP;
   ds->ops.solve = DSSolve_NHEP;
   ds->ops.sort = DSSort_NHEP;
   ds->ops.update = DSUpdateExtraRow_NHEP;

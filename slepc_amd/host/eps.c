@@ -94,7 +94,29 @@ PetscErrorCode EPSSetWhichEigenpairs(EPS eps, EPSWhich which)
 
 PetscErrorCode EPSSetTarget(EPS eps, PetscScalar target)
 {
+  if (eps->target != target) eps->state = EPS_STATE_INITIAL;
   eps->target = target;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode EPSGetTarget(EPS eps, PetscScalar *target)
+{
+  *target = eps->target;
+  return PETSC_SUCCESS;
+}
+
+/* EPSSetExtraction (epsopts.c:984-991); the solver's setup rejects what it cannot do */
+PetscErrorCode EPSSetExtraction(EPS eps, EPSExtraction extr)
+{
+  PetscCheck(extr >= EPS_RITZ && extr <= EPS_REFINED_HARMONIC, PETSC_ERR_ARG_OUTOFRANGE, "invalid extraction type");
+  if (eps->extraction != extr) eps->state = EPS_STATE_INITIAL;
+  eps->extraction = extr;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode EPSGetExtraction(EPS eps, EPSExtraction *extr)
+{
+  *extr = eps->extraction;
   return PETSC_SUCCESS;
 }
 

@@ -1,8 +1,9 @@
 /*
  * krylovschur.c -- EPSKRYLOVSCHUR, default variant
  * (src/eps/impls/krylov/krylovschur/krylovschur.c:93-194 setup, :227-337 solve,
- * :1579-1627 create) with Ritz extraction: Lanczos + DSHEP for Hermitian
- * problems, Arnoldi + DSNHEP otherwise.  The restart loop, the truncation and
+ * :1579-1627 create) with Ritz extraction (Lanczos + DSHEP for Hermitian
+ * problems, Arnoldi + DSNHEP otherwise) or harmonic extraction (Arnoldi +
+ * DSNHEP with the translated projected matrix, krylovschur.c:270-319).  The restart loop, the truncation and
  * the locking logic are the reference's; the n-length work goes to the BV and
  * Mat types (BVHIPKS and MATHIPCSR on the MI355X).
  */
@@ -68,9 +69,17 @@ static PetscErrorCode EPSSetUp_KrylovSchur(EPS eps)
   if (eps->max_it == PETSC_DETERMINE) eps->max_it = PetscMax(100, 2 * eps->n / eps->ncv);
   if (!eps->which) eps->which = EPS_LARGEST_MAGNITUDE;   /* EPSSetWhichEigenpairs_Default with STSHIFT */
   PetscCheck(ctx->lock || eps->mpd >= eps->ncv, PETSC_ERR_SUP, "Should not use mpd parameter in non-locking variant");
+  PetscCheck(eps->extraction == EPS_RITZ || eps->extraction == EPS_HARMONIC, PETSC_ERR_SUP, "Unsupported extraction type");
+  ctx->harmonic = eps->extraction == EPS_HARMONIC;
+  PetscCheck(!ctx->harmonic || ctx->mixed_tol <= 0.0, PETSC_ERR_SUP,
+             "Harmonic extraction is not available in the mixed-precision form");
   if (!ctx->keep) ctx->keep = 0.5;
   PetscCall(EPSAllocateSolution(eps, 1));
-  if (eps->ishermitian) {
+  free(ctx->g);
+  ctx->g = NULL;
+  if (ctx->harmonic) PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)(eps->ncv + 1), &ctx->g));
+  /* harmonic extraction runs the default (Arnoldi + DSNHEP) variant for Hermitian problems too */
+  if (eps->ishermitian && !ctx->harmonic) {
     PetscCall(DSSetType(eps->ds, DSHEP));
     PetscCall(DSSetCompact(eps->ds, PETSC_TRUE));
   } else {
@@ -113,7 +122,8 @@ static PetscErrorCode KSAfterExpand(EPS eps)
   PetscInt k = 0, l = ctx->l, nv = ctx->nv, nconv;
   Mat U, H, T;
   PetscReal beta = 0.0;
-  PetscBool breakdown = PETSC_FALSE, hermitian = eps->ishermitian;
+  PetscReal gamma = 1.0;
+  PetscBool breakdown = PETSC_FALSE, hermitian = eps->ishermitian && !ctx->harmonic;
   double t0;
 
   PetscCall(BVMatKrylovEnd(eps->V, eps->nconv + l, &nv, &beta, &breakdown));
@@ -130,14 +140,17 @@ static PetscErrorCode KSAfterExpand(EPS eps)
   PetscCall(DSSetState(eps->ds, l ? DS_STATE_RAW : DS_STATE_INTERMEDIATE));
   PetscCall(BVSetActiveColumns(eps->V, eps->nconv, nv));
 
-  /* solve the projected problem */
+  /* translation of the Krylov decomposition for harmonic extraction */
   t0 = SlepcWallTime();
+  if (ctx->harmonic) PetscCall(DSTranslateHarmonic(eps->ds, eps->target, beta, PETSC_FALSE, ctx->g, &gamma));
+
+  /* solve the projected problem */
   PetscCall(DSSolve(eps->ds, eps->eigr, eps->eigi));
   PetscCall(DSSort(eps->ds, eps->eigr, eps->eigi, NULL, NULL, NULL));
   PetscCall(DSUpdateExtraRow(eps->ds));
 
   /* check convergence */
-  PetscCall(EPSKrylovConvergence(eps, PETSC_FALSE, eps->nconv, nv - eps->nconv, beta, 1.0, &k));
+  PetscCall(EPSKrylovConvergence(eps, PETSC_FALSE, eps->nconv, nv - eps->nconv, beta, gamma, &k));
   eps->t_ds += SlepcWallTime() - t0;
   /* EPSStoppingBasic (epsdefault.c:290-302) */
   if (k >= eps->nev) eps->reason = EPS_CONVERGED_TOL;
@@ -160,6 +173,16 @@ static PetscErrorCode KSAfterExpand(EPS eps)
         if (breakdown) eps->reason = EPS_DIVERGED_BREAKDOWN;
       }
     } else {
+      if (ctx->harmonic) {
+        /* undo the translation: gamma u^ = u - V g~ (krylovschur.c:310-319) */
+        PetscCall(DSSetDimensions(eps->ds, nv, k, l));
+        PetscCall(DSTranslateHarmonic(eps->ds, 0.0, beta, PETSC_TRUE, ctx->g, &gamma));
+        PetscCall(BVSetActiveColumns(eps->V, 0, nv));
+        PetscCall(BVMultColumn(eps->V, -1.0, 1.0, nv, ctx->g));
+        PetscCall(BVScaleColumn(eps->V, nv, 1.0 / gamma));
+        PetscCall(BVSetActiveColumns(eps->V, eps->nconv, nv));
+        PetscCall(DSSetDimensions(eps->ds, nv, k, nv));
+      }
       /* prepare the Rayleigh quotient for restart */
       PetscCall(DSTruncate(eps->ds, k + l, PETSC_FALSE));
     }
@@ -342,16 +365,26 @@ PetscErrorCode EPSKrylovSchurGetMixedStats(EPS eps, PetscInt *its32, PetscInt *n
   return PETSC_SUCCESS;
 }
 
-/* EPSComputeVectors_Schur (epsdefault.c:105-169) / _Hermitian (epsdefault.c:28-50) */
+/* EPSComputeVectors_Schur (epsdefault.c:105-169), which for Hermitian problems
+   is EPSComputeVectors_Hermitian (epsdefault.c:53-72) */
 static PetscErrorCode EPSComputeVectors_KrylovSchur(EPS eps)
 {
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
   Mat Z;
-  if (eps->ishermitian) return PETSC_SUCCESS;   /* Ritz vectors are already in V */
+  PetscInt n;
+  if (eps->ishermitian && !ctx->harmonic) return PETSC_SUCCESS;   /* DSHEP: Ritz vectors are already in V */
   if (!eps->nconv) return PETSC_SUCCESS;
   PetscCall(DSVectors(eps->ds, DS_MAT_X, NULL, NULL));
   PetscCall(DSGetMat(eps->ds, DS_MAT_X, &Z));
-  PetscCall(BVMultInPlace(eps->V, Z, 0, eps->nconv));
-  PetscCall(DSRestoreMat(eps->ds, DS_MAT_X, &Z));
+  if (eps->ishermitian) {
+    PetscCall(DSGetDimensions(eps->ds, &n, NULL, NULL, NULL));
+    PetscCall(BVMultInPlace(eps->V, Z, 0, n));
+    PetscCall(DSRestoreMat(eps->ds, DS_MAT_X, &Z));
+    PetscCall(BVNormalize(eps->V, eps->eigi));
+  } else {
+    PetscCall(BVMultInPlace(eps->V, Z, 0, eps->nconv));
+    PetscCall(DSRestoreMat(eps->ds, DS_MAT_X, &Z));
+  }
   return PETSC_SUCCESS;
 }
 
@@ -380,6 +413,10 @@ static PetscErrorCode EPSDestroy_KrylovSchur(EPS eps)
 {
   EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
   if (ctx && ctx->e32) PetscCall(EPSDestroy(&ctx->e32));
+  if (ctx) {
+    free(ctx->g);
+    ctx->g = NULL;
+  }
   return PETSC_SUCCESS;
 }
 

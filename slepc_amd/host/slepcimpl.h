@@ -183,6 +183,7 @@ typedef struct {
   PetscErrorCode (*vectors)(DS, DSMatType, PetscInt *, PetscReal *);
   PetscErrorCode (*truncate)(DS, PetscInt, PetscBool);
   PetscErrorCode (*gettruncatesize)(DS, PetscInt, PetscInt, PetscInt *);
+  PetscErrorCode (*transharm)(DS, PetscScalar, PetscReal, PetscBool, PetscScalar *, PetscReal *);
 } DSOps;
 struct _p_DS {
   char type[SLEPC_NAME_LEN];
@@ -230,6 +231,7 @@ struct _p_EPS {
   PetscReal tol;
   EPSWhich which;
   PetscScalar target;
+  EPSExtraction extraction;
   EPSProblemType problem_type;
   PetscBool ishermitian, isgeneralized;
   EPSConvergedReason reason;
@@ -264,6 +266,9 @@ typedef struct {            /* EPS_KRYLOVSCHUR (krylovschur.h:76-105), default v
   PetscReal t32;
   EPS       e32;             /* the FP32 solve while the first phase runs */
   double    t32_start;
+  /* harmonic extraction: the translation vector g (ncv + 1 entries) */
+  PetscBool harmonic;
+  PetscScalar *g;
 } EPS_KRYLOVSCHUR;
 
 PetscErrorCode EPSKrylovConvergence(EPS eps, PetscBool getall, PetscInt kini, PetscInt nits, PetscReal beta,
