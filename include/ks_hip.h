@@ -122,6 +122,11 @@ int ks_csr_destroy(ks_csr A);
 /* New matrix with the same pattern and the values converted to `to` on the
    device (FP32 copy of an FP64 operator for the mixed-precision phase). */
 int ks_csr_convert(ks_csr A, ks_ctx ctx, ks_prec to, ks_csr *out);
+/* New matrix A^T (ncols x nrows), built on the device: each row lists its entries in
+   increasing column order.  Replaces MatCreateHermitianTranspose of the operator in
+   two-sided Krylov-Schur (src/eps/impls/krylov/krylovschur/ks-twosided.c:138) by an
+   explicit matrix, so products with A^T use the same SpMV kernel. */
+int ks_csr_transpose(ks_csr A, ks_ctx ctx, ks_csr *out);
 /* nrows, nnz, max row length and the kernel variant chosen (0 stream, 1 vector) */
 int ks_csr_info(ks_csr A, int *nrows, int64_t *nnz, int *max_row, int *variant, int *nblocks);
 /* y = A x.  x, y device vectors in xprec.  When gate != 0 the product is

@@ -76,6 +76,7 @@ typedef const char *EPSType;
 #define BVHIPKS        "hipks"
 #define DSHEP          "hep"
 #define DSNHEP         "nhep"
+#define DSNHEPTS       "nhepts"
 #define STSHIFT        "shift"
 #define EPSKRYLOVSCHUR "krylovschur"
 
@@ -116,6 +117,10 @@ PetscErrorCode MatSetSizes(Mat A, PetscInt m, PetscInt n, PetscInt M, PetscInt N
 PetscErrorCode MatSetType(Mat A, MatType type);
 PetscErrorCode MatGetType(Mat A, MatType *type);
 PetscErrorCode MatGetSize(Mat A, PetscInt *M, PetscInt *N);
+/* y = A^T x (for MATHIPCSR with an explicit transpose formed on the device on first use) */
+PetscErrorCode MatMultTranspose(Mat A, Vec x, Vec y);
+/* a new MATHIPCSR holding A^T, formed on the device (the left operator of two-sided Krylov-Schur) */
+PetscErrorCode MatHIPCSRCreateTranspose(Mat A, Mat *AT);
 PetscErrorCode MatMult(Mat A, Vec x, Vec y);
 PetscErrorCode MatDestroy(Mat *A);
 /* CSR arrays on the host (copied to the device); row pointers and column
@@ -204,7 +209,8 @@ PetscErrorCode BVCreate_HIPKS(BV bv);
 
 /* ------------------------------------------------------------ DS */
 typedef enum { DS_STATE_RAW = 0, DS_STATE_INTERMEDIATE = 1, DS_STATE_CONDENSED = 2, DS_STATE_TRUNCATED = 3 } DSStateType;
-typedef enum { DS_MAT_A = 0, DS_MAT_T = 3, DS_MAT_Q = 5, DS_MAT_X = 7, DS_NUM_MAT = 12 } DSMatType;
+typedef enum { DS_MAT_A = 0, DS_MAT_B = 1, DS_MAT_T = 3, DS_MAT_Q = 5, DS_MAT_Z = 6, DS_MAT_X = 7, DS_MAT_Y = 8,
+               DS_NUM_MAT = 12 } DSMatType;
 PetscErrorCode DSRegister(const char name[], PetscErrorCode (*create)(DS));
 PetscErrorCode DSCreate(DS *ds);
 PetscErrorCode DSSetType(DS ds, DSType type);
@@ -302,6 +308,11 @@ PetscErrorCode EPSGetIterationNumber(EPS eps, PetscInt *its);
 PetscErrorCode EPSGetConvergedReason(EPS eps, EPSConvergedReason *reason);
 PetscErrorCode EPSGetEigenvalue(EPS eps, PetscInt i, PetscScalar *eigr, PetscScalar *eigi);
 PetscErrorCode EPSGetEigenvector(EPS eps, PetscInt i, Vec Vr, Vec Vi);
+/* left eigenvector (two-sided solves, epssolve.c:567-611) */
+PetscErrorCode EPSGetLeftEigenvector(EPS eps, PetscInt i, Vec Wr, Vec Wi);
+/* compute left eigenvectors as well, by two-sided Krylov-Schur (ks-twosided.c) */
+PetscErrorCode EPSSetTwoSided(EPS eps, PetscBool twosided);
+PetscErrorCode EPSGetTwoSided(EPS eps, PetscBool *twosided);
 PetscErrorCode EPSGetEigenpair(EPS eps, PetscInt i, PetscScalar *eigr, PetscScalar *eigi, Vec Vr, Vec Vi);
 PetscErrorCode EPSGetErrorEstimate(EPS eps, PetscInt i, PetscReal *errest);
 PetscErrorCode EPSComputeError(EPS eps, PetscInt i, EPSErrorType type, PetscReal *error);

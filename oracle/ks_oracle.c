@@ -496,14 +496,24 @@ int or_bv_orthogonalize(int n, int ld, double *V, int l, int k, int block, doubl
 
 /* ------------------------------------------------------------------ DS */
 enum { DS_RAW = 0, DS_INTERMEDIATE = 1, DS_CONDENSED = 2, DS_TRUNCATED = 3 };
-enum { DS_HEP = 1, DS_NHEP = 2 };
+enum { DS_HEP = 1, DS_NHEP = 2, DS_NHEPTS = 3 };
 
 typedef struct {
   int type, ld, n, l, k, t, state, compact, extrarow;
   double *A, *T, *Q, *X, *work;
+  double *B, *Z, *Y, *wr2, *wi2;        /* DSNHEPTS: the problem from the transpose */
   int *perm, *iwork;
   or_sc sc;
 } ods;
+
+/* DSNHEPTS runs the DSNHEP routines on (B, Z, Y) by exchanging them with (A, Q, X) */
+static void ds_swap_sides(ods *ds)
+{
+  double *t;
+  t = ds->A; ds->A = ds->B; ds->B = t;
+  t = ds->Q; ds->Q = ds->Z; ds->Z = t;
+  t = ds->X; ds->X = ds->Y; ds->Y = t;
+}
 
 static int ds_alloc(ods *ds, int type, int ld)
 {
@@ -517,12 +527,21 @@ static int ds_alloc(ods *ds, int type, int ld)
   ds->perm = (int*)calloc((size_t)ld, sizeof(int));
   ds->iwork = (int*)calloc((size_t)ld, sizeof(int));
   if (!ds->A || !ds->T || !ds->Q || !ds->X || !ds->work || !ds->perm || !ds->iwork) return -1;
+  if (type == DS_NHEPTS) {
+    ds->B = (double*)calloc((size_t)ld * ld, sizeof(double));
+    ds->Z = (double*)calloc((size_t)ld * ld, sizeof(double));
+    ds->Y = (double*)calloc((size_t)ld * ld, sizeof(double));
+    ds->wr2 = (double*)calloc((size_t)ld, sizeof(double));
+    ds->wi2 = (double*)calloc((size_t)ld, sizeof(double));
+    if (!ds->B || !ds->Z || !ds->Y || !ds->wr2 || !ds->wi2) return -1;
+  }
   return 0;
 }
 
 static void ds_free(ods *ds)
 {
   free(ds->A); free(ds->T); free(ds->Q); free(ds->X); free(ds->work); free(ds->perm); free(ds->iwork);
+  free(ds->B); free(ds->Z); free(ds->Y); free(ds->wr2); free(ds->wi2);
 }
 
 /* DSSetDimensions (dsops.c:130-160) */
@@ -783,10 +802,83 @@ static void ds_vectors_hep(ods *ds, int *j, double *rnorm)
   }
 }
 
+/* DSSortWithPermutation_NHEP_Private (dsutil.c:177-236) on (A, Q) */
+static int ds_sort_permutation_nhep(ods *ds, int *perm, double *wr, double *wi)
+{
+  int n = ds->n, ld = ds->ld, info;
+  double *T = ds->A;
+  for (int i = ds->l; i < n - 1; i++) {
+    int pos = perm[i];
+    int inc = (pos < n - 1 && T[pos + 1 + (size_t)pos * ld] != 0.0) ? 2 : 1;
+    if (pos != i) {
+      if (!((pos == 0 || T[pos + (size_t)(pos - 1) * ld] == 0.0 || perm[i + 1] == pos - 1) &&
+            (T[pos + 1 + (size_t)pos * ld] == 0.0 || perm[i + 1] == pos + 1)))
+        OR_ERR(-72, "Invalid permutation due to a 2x2 block at position %d", pos);
+      int ifst = pos + 1, ilst = i + 1;
+      dtrexc_("V", &n, T, &ld, ds->Q, &ld, &ifst, &ilst, ds->work, &info);
+      if (info) OR_ERR(-76, "trexc info %d", info);
+      for (int j = i + 1; j < n; j++) if (perm[j] >= i && perm[j] < pos) perm[j] += inc;
+      perm[i] = i;
+      if (inc == 2) perm[i + 1] = i + 1;
+    }
+    if (inc == 2) i++;
+  }
+  for (int j = ds->l; j < n; j++) {
+    wr[j] = T[j + (size_t)j * ld];
+    if (j < n - 1 && T[j + 1 + (size_t)j * ld] != 0.0) {
+      wi[j] = sqrt(fabs(T[j + 1 + (size_t)j * ld])) * sqrt(fabs(T[j + (size_t)(j + 1) * ld]));
+      wr[j + 1] = wr[j]; wi[j + 1] = -wi[j]; j++;
+    } else wi[j] = 0.0;
+  }
+  return 0;
+}
+
+/* DSSort_NHEPTS (dsnhepts.c:188-224): both sorted by the criterion, then B's blocks
+   permuted so that its eigenvalue i is the one nearest A's eigenvalue i */
+static int ds_sort_nhepts(ods *ds, double *wr, double *wi)
+{
+  int n = ds->n, cont = 0, id = 0, e = 0;
+  int *p = (int*)malloc(sizeof(int) * (size_t)ds->ld), *idx = (int*)malloc(sizeof(int) * (size_t)ds->ld),
+      *idx2 = (int*)malloc(sizeof(int) * (size_t)ds->ld);
+  double *wr2 = ds->wr2, *wi2 = ds->wi2;
+  if (!p || !idx || !idx2) { free(p); free(idx); free(idx2); return -55; }
+  e = ds_sort_nhep(ds, wr, wi);
+  if (!e) { ds_swap_sides(ds); e = ds_sort_nhep(ds, wr2, wi2); ds_swap_sides(ds); }
+  if (!e) {
+    for (int i = 0; i < n; i++) {
+      if (hypot(wr2[i] - wr[i], wi2[i] - wi[i]) > sqrt(DBL_EPSILON)) { idx2[cont] = i; idx[cont++] = i; }
+      p[i] = -1;
+    }
+    if (cont) {
+      for (int i = 0; i < cont; i++) {
+        double t = DBL_MAX, sd;
+        for (int j = 0; j < cont; j++)
+          if (idx2[j] != -1 && (sd = hypot(wr2[idx[j]] - wr[idx[i]], wi2[idx[j]] - wi[idx[i]])) < t) { id = j; t = sd; }
+        p[idx[i]] = idx[id];
+        idx2[id] = -1;
+      }
+      for (int i = 0; i < n; i++) if (p[i] == -1) p[i] = i;
+      ds_swap_sides(ds);
+      e = ds_sort_permutation_nhep(ds, p, wr2, wi2);
+      ds_swap_sides(ds);
+    }
+  }
+  free(p); free(idx); free(idx2);
+  return e;
+}
+
 static int ds_solve(ods *ds, double *wr, double *wi)
 {
   if (ds->state >= DS_CONDENSED) return 0;
-  OR_CALL(ds->type == DS_HEP ? ds_solve_hep(ds, wr, wi) : ds_solve_nhep(ds, wr, wi));
+  if (ds->type == DS_HEP) OR_CALL(ds_solve_hep(ds, wr, wi));
+  else OR_CALL(ds_solve_nhep(ds, wr, wi));
+  if (ds->type == DS_NHEPTS) {                                     /* DSSolve_NHEPTS (dsnhepts.c:241-252) */
+    int e;
+    ds_swap_sides(ds);
+    e = ds_solve_nhep(ds, ds->wr2, ds->wi2);
+    ds_swap_sides(ds);
+    if (e) return e;
+  }
   ds->state = DS_CONDENSED;
   return 0;
 }
@@ -795,6 +887,7 @@ static int ds_sort(ods *ds, double *wr, double *wi)
 {
   for (int i = 0; i < ds->n; i++) ds->perm[i] = i;
   if (ds->type == DS_HEP) { ds_sort_hep(ds, wr); return 0; }
+  if (ds->type == DS_NHEPTS) return ds_sort_nhepts(ds, wr, wi);
   return ds_sort_nhep(ds, wr, wi);
 }
 
@@ -802,6 +895,7 @@ static void ds_update_extra_row(ods *ds)
 {
   if (ds->type == DS_HEP) ds_update_extra_row_hep(ds);
   else ds_update_extra_row_nhep(ds);
+  if (ds->type == DS_NHEPTS) { ds_swap_sides(ds); ds_update_extra_row_nhep(ds); ds_swap_sides(ds); }
 }
 
 static int ds_vectors(ods *ds, int *j, double *rnorm)
@@ -810,29 +904,46 @@ static int ds_vectors(ods *ds, int *j, double *rnorm)
   return j ? ds_vectors_nhep_some(ds, j, rnorm) : ds_vectors_nhep_all(ds);
 }
 
-/* DSGetTruncateSize_Default (dsops.c:329-344) */
+/* DS_MAT_Y of DSNHEPTS: right eigenvectors of B (dsnhepts.c:162-186) */
+static int ds_vectors_left(ods *ds, int *j, double *rnorm)
+{
+  int e;
+  ds_swap_sides(ds);
+  e = j ? ds_vectors_nhep_some(ds, j, rnorm) : ds_vectors_nhep_all(ds);
+  ds_swap_sides(ds);
+  return e;
+}
+
+/* DSGetTruncateSize_Default (dsops.c:329-344); DSNHEPTS looks at B too (dsnhepts.c:349-367) */
 static void ds_truncate_size(ods *ds, int l, int n, int *k)
 {
   if (ds->type == DS_HEP) return;
   double val = ds->A[(l + *k) + (size_t)(l + *k - 1) * ds->ld];
+  if (ds->type == DS_NHEPTS && val == 0.0) val = ds->B[(l + *k) + (size_t)(l + *k - 1) * ds->ld];
   if (val != 0.0) {
     if (l + *k < n - 1) (*k)++;
     else (*k)--;
   }
 }
 
-/* DSTruncate_HEP / DSTruncate_NHEP (dshep.c:643-670, dsnhep.c:394-425) */
+/* DSTruncate_HEP / DSTruncate_NHEP / DSTruncate_NHEPTS (dshep.c:643-670, dsnhep.c:394-425,
+   dsnhepts.c:369-398) */
 static void ds_truncate(ods *ds, int n, int trim)
 {
   int ld = ds->ld, l = ds->l;
-  int fullA = (ds->type == DS_NHEP || !ds->compact) && ds->extrarow;
+  int fullA = (ds->type != DS_HEP || !ds->compact) && ds->extrarow;
+  double *B = ds->type == DS_NHEPTS ? ds->B : NULL;
   if (trim) {
-    if (fullA) for (int i = l; i < ds->n; i++) ds->A[ds->n + (size_t)i * ld] = 0.0;
+    if (fullA)
+      for (int i = l; i < ds->n; i++) { ds->A[ds->n + (size_t)i * ld] = 0.0; if (B) B[ds->n + (size_t)i * ld] = 0.0; }
     ds->l = 0; ds->k = 0; ds->n = n; ds->t = n;
   } else {
     if (fullA && ds->k == ds->n) {
-      for (int i = l; i < n; i++) ds->A[n + (size_t)i * ld] = ds->A[ds->n + (size_t)i * ld];
-      for (int i = l; i < ds->n; i++) ds->A[ds->n + (size_t)i * ld] = 0.0;
+      for (int i = l; i < n; i++) {
+        ds->A[n + (size_t)i * ld] = ds->A[ds->n + (size_t)i * ld];
+        if (B) B[n + (size_t)i * ld] = B[ds->n + (size_t)i * ld];
+      }
+      for (int i = l; i < ds->n; i++) { ds->A[ds->n + (size_t)i * ld] = 0.0; if (B) B[ds->n + (size_t)i * ld] = 0.0; }
     }
     ds->k = ds->extrarow ? n : 0;
     ds->t = ds->n;
@@ -937,10 +1048,13 @@ int or_ds_run(int type, int ld, int n, int l, int k, int compact, int extrarow,
 typedef struct {
   int n, nev, ncv, mpd, max_it, which, hermitian, lock, nconv, its, reason;
   int harmonic, dsherm;           /* dsherm: Lanczos + DSHEP (Hermitian and Ritz extraction) */
+  int twosided;
   double tol, target, keep;
   const int *rowptr, *colidx;
   const double *val;
-  obv V;
+  int *trowptr, *tcolidx;         /* A^T (two-sided) */
+  double *tval, *M;               /* M = W^T V, ncv x ncv (two-sided) */
+  obv V, W;
   ods ds;
   double *eigr, *eigi, *errest;
   int *perm;
@@ -955,15 +1069,22 @@ static void op_mult(oeps *eps, const double *x, double *y)
   eps->nmatmult++;
 }
 
-/* BVMatArnoldi / BVMatLanczos (bvkrylov.c:56-226) */
-static int mat_krylov(oeps *eps, int k, int *m, double *beta, int *breakdown)
+static void op_mult_transpose(oeps *eps, const double *x, double *y)
 {
-  obv *bv = &eps->V;
+  or_spmv(eps->n, eps->trowptr, eps->tcolidx, eps->tval, x, y);
+  eps->nmatmult++;
+}
+
+/* BVMatArnoldi / BVMatLanczos (bvkrylov.c:56-226) on basis bv with A (or A^T), the
+   Hessenberg columns into H (or the tridiagonal of the DS) */
+static int mat_krylov_on(oeps *eps, obv *bv, int transpose, double *H, int k, int *m, double *beta, int *breakdown)
+{
   ods *ds = &eps->ds;
   int lindep = 0, ldb = bv->nc + bv->m, ld = ds->ld;
   const double *a = bv->buffer;
   for (int j = k; j < *m; j++) {
-    op_mult(eps, COL(bv, j), COL(bv, j + 1));                     /* BVMatMultColumn */
+    if (transpose) op_mult_transpose(eps, COL(bv, j), COL(bv, j + 1));
+    else op_mult(eps, COL(bv, j), COL(bv, j + 1));               /* BVMatMultColumn */
     if (j == eps->n - 1) {                                         /* BV_OrthogonalizeColumn_Safe */
       int save = bv->refine;
       bv->refine = OR_REFINE_NEVER;
@@ -981,12 +1102,17 @@ static int mat_krylov(oeps *eps, int k, int *m, double *beta, int *breakdown)
       betat[j] = a[bv->nc + j + 1 + (size_t)(j + 1) * ldb];
     }
   } else {
-    double *h = ds->A;
+    double *h = H;
     for (int j = k; j < *m - 1; j++) memcpy(h + (size_t)j * ld, a + bv->nc + (size_t)(j + 1) * ldb, sizeof(double) * (j + 2));
     memcpy(h + (size_t)(*m - 1) * ld, a + bv->nc + (size_t)(*m) * ldb, sizeof(double) * (*m));
     if (ld > *m) h[*m + (size_t)(*m - 1) * ld] = a[bv->nc + *m + (size_t)(*m) * ldb];
   }
   return 0;
+}
+
+static int mat_krylov(oeps *eps, int k, int *m, double *beta, int *breakdown)
+{
+  return mat_krylov_on(eps, &eps->V, 0, eps->ds.A, k, m, beta, breakdown);
 }
 
 /* EPSGetStartVector (epssolve.c:841-873) */
@@ -1006,7 +1132,7 @@ static int get_start_vector(oeps *eps, int i, int have_initial, int *breakdown)
 }
 
 /* EPSKrylovConvergence (epskrylov.c:207-295), STSHIFT sigma=0, trivial RG, relative test */
-static int krylov_convergence(oeps *eps, int kini, int nits, double beta, double corrf, int *kout)
+static int krylov_convergence(oeps *eps, int kini, int nits, double beta, double betat, double corrf, int *kout)
 {
   int k, marker = -1;
   double resnorm;
@@ -1018,6 +1144,15 @@ static int krylov_convergence(oeps *eps, int kini, int nits, double beta, double
     w = hypot(re, im);                                             /* EPSConvergedRelative */
     eps->errest[k] = (w != 0.0) ? resnorm / w : DBL_MAX;
     if (marker == -1 && eps->errest[k] >= eps->tol) marker = k;
+    if (eps->twosided) {                                           /* epskrylov.c:269-276 */
+      int newk2 = k;
+      double lerrest;
+      OR_CALL(ds_vectors_left(&eps->ds, &newk2, &resnorm));
+      resnorm *= betat;
+      lerrest = (w != 0.0) ? resnorm / w : DBL_MAX;
+      if (lerrest > eps->errest[k]) eps->errest[k] = lerrest;
+      if (marker == -1 && lerrest >= eps->tol) marker = k;
+    }
     if (newk == k + 1) { eps->errest[k + 1] = eps->errest[k]; k++; }
     if (marker != -1) break;
   }
@@ -1050,7 +1185,7 @@ static int ks_loop(oeps *eps, int have_initial, int restart_cap, double *g)
     OR_CALL(ds_solve(ds, eps->eigr, eps->eigi));
     OR_CALL(ds_sort(ds, eps->eigr, eps->eigi));
     ds_update_extra_row(ds);
-    OR_CALL(krylov_convergence(eps, eps->nconv, nv - eps->nconv, beta, gamma, &k));
+    OR_CALL(krylov_convergence(eps, eps->nconv, nv - eps->nconv, beta, 0.0, gamma, &k));
     eps->t_ds += wtime() - t0;
 
     /* EPSStoppingBasic (epsdefault.c:290-302) */
@@ -1096,10 +1231,171 @@ static int ks_loop(oeps *eps, int have_initial, int restart_cap, double *g)
   return 0;
 }
 
+/* ------------------------------------------------------------------ two-sided variant */
+/* EPSGetLeftStartVector (epssolve.c:879-900): always a random column (no left initial space) */
+static int get_left_start_vector(oeps *eps, int i, int *breakdown)
+{
+  double norm;
+  int lindep = 0;
+  bv_randomcol(&eps->W, i);
+  OR_CALL(orthogonalize_column(&eps->W, i, &norm, &lindep));
+  if (breakdown) *breakdown = lindep;
+  else if (lindep || norm == 0.0) OR_ERR(-80, "Unable to generate more left start vectors");
+  bv_scalecol(&eps->W, i, 1.0 / norm);
+  return 0;
+}
+
+/* BVMatProject(V, NULL, W, M) (bvglobal.c:1093-1160): M(i,j) = W(:,i)^T V(:,j) for
+   i, j < nv outside the leading (l x l) block, l = the active offset of both */
+static void mat_project(oeps *eps, int l, int nv)
+{
+  int n = eps->n, ncv = eps->ncv;
+  for (int j = 0; j < nv; j++)
+    for (int i = 0; i < nv; i++) {
+      if (i < l && j < l) continue;
+      double t = 0.0;
+      const double *w = COL(&eps->W, i), *v = COL(&eps->V, j);
+      for (int r = 0; r < n; r++) t += w[r] * v[r];
+      eps->M[i + (size_t)j * ncv] = t;
+    }
+}
+
+/* EPSTwoSidedRQUpdate1 (ks-twosided.c:25-77) */
+static int rq_update1(oeps *eps, int nv, double beta, double betat)
+{
+  int ncv = eps->ncv, n_ = nv, one = 1, info = 0, ld = eps->ds.ld;
+  int *p = (int*)malloc(sizeof(int) * (size_t)(nv > 0 ? nv : 1));
+  double *A = (double*)malloc(sizeof(double) * (size_t)ncv * ncv), *w = (double*)calloc((size_t)ncv, sizeof(double));
+  obv *V = &eps->V, *W = &eps->W;
+  int vl = V->l, vk = V->k;
+  if (!p || !A || !w) { free(p); free(A); free(w); return -55; }
+  V->l = 0; V->k = nv; W->l = 0; W->k = nv;
+  bv_dotvec(W, COL(V, nv), w);                                     /* w = W^T u */
+  memcpy(A, eps->M, sizeof(double) * (size_t)ncv * ncv);
+  dgetrf_(&n_, &n_, A, &ncv, p, &info);
+  if (!info) dgetrs_("N", &n_, &one, A, &ncv, p, w, &ncv, &info);
+  if (!info) {
+    bv_multvec(V, -1.0, 1.0, COL(V, nv), w);                       /* u -= V M^-1 W^T u */
+    for (int i = 0; i < nv; i++) eps->ds.A[(size_t)(nv - 1) * ld + i] += beta * w[i];
+    bv_dotvec(V, COL(W, nv), w);                                   /* w = V^T v */
+    dgetrs_("C", &n_, &one, A, &ncv, p, w, &ncv, &info);
+  }
+  if (!info) {
+    bv_multvec(W, -1.0, 1.0, COL(W, nv), w);
+    for (int i = 0; i < nv; i++) eps->ds.B[(size_t)(nv - 1) * ld + i] += betat * w[i];
+  }
+  V->l = vl; V->k = vk; W->l = vl; W->k = vk;
+  free(p); free(A); free(w);
+  if (info) OR_ERR(-76, "getrf/getrs info %d", info);
+  return 0;
+}
+
+/* EPSTwoSidedRQUpdate2 (ks-twosided.c:79-136) */
+static int rq_update2(oeps *eps, int k)
+{
+  int ncv = eps->ncv, ld = eps->ds.ld, n_, ld_ = ld;
+  obv *V = &eps->V, *W = &eps->W;
+  int l = V->l, nv = V->k;
+  double norm, zero = 0.0, sone = 1.0;
+  double *w = (double*)calloc((size_t)ncv * ncv, sizeof(double));
+  if (!w) return -55;
+  for (int side = 0; side < 2; side++) {
+    obv *X = side ? W : V;
+    int ldb = X->nc + X->m;
+    double *A = side ? eps->ds.B : eps->ds.A, *c;
+    X->l = 0; X->k = nv;
+    OR_CALL(orthogonalize_column(X, k, &norm, NULL));              /* coefficients in buffer column k */
+    c = X->buffer + (size_t)k * ldb + X->nc;
+    bv_scalecol(X, k, 1.0 / norm);
+    for (int j = l; j < k; j++) {
+      for (int i = 0; i < k; i++) A[i + (size_t)j * ld] += c[i] * A[k + (size_t)j * ld];
+      A[k + (size_t)j * ld] *= norm;
+    }
+    X->l = l; X->k = nv;
+  }
+  n_ = nv;
+  dgemm_("N", "N", &n_, &n_, &n_, &sone, eps->M, &ncv, eps->ds.Q, &ld_, &zero, w, &ncv);
+  dgemm_("C", "N", &n_, &n_, &n_, &sone, eps->ds.Z, &ld_, w, &ncv, &zero, eps->M, &ncv);
+  free(w);
+  return 0;
+}
+
+/* EPSSolve_KrylovSchur_TwoSided (ks-twosided.c:138-241) */
+static int ks_loop_twosided(oeps *eps, int have_initial, int restart_cap)
+{
+  int k = 0, l = 0, nv, nvt, breakdown = 0, breakdownt = 0, breakdownl = 0;
+  double beta = 0.0, betat = 0.0, norm, norm2, t0;
+  int one = 1;
+  obv *V = &eps->V, *W = &eps->W;
+  ods *ds = &eps->ds;
+  OR_CALL(get_start_vector(eps, 0, have_initial, NULL));
+  OR_CALL(get_left_start_vector(eps, 0, NULL));
+  while (eps->reason == OR_CONVERGED_ITERATING) {
+    eps->its++;
+    nv = eps->nconv + eps->mpd < eps->ncv ? eps->nconv + eps->mpd : eps->ncv;
+    ds_setdims(ds, nv, eps->nconv, eps->nconv + l);
+    OR_CALL(mat_krylov_on(eps, V, 0, ds->A, eps->nconv + l, &nv, &beta, &breakdown));
+    nvt = nv;
+    ds_setdims(ds, nv, eps->nconv, eps->nconv + l);
+    OR_CALL(mat_krylov_on(eps, W, 1, ds->B, eps->nconv + l, &nvt, &betat, &breakdownt));
+    if (nvt < nv) nv = nvt;
+    ds_setdims(ds, nv, eps->nconv, eps->nconv + l);
+    ds->state = l == 0 ? DS_INTERMEDIATE : DS_RAW;
+    breakdown = breakdown || breakdownt;
+
+    V->l = eps->nconv + l; V->k = nv; W->l = eps->nconv + l; W->k = nv;
+    mat_project(eps, eps->nconv + l, nv);
+    OR_CALL(rq_update1(eps, nv, beta, betat));
+
+    t0 = wtime();
+    OR_CALL(ds_solve(ds, eps->eigr, eps->eigi));
+    OR_CALL(ds_sort(ds, eps->eigr, eps->eigi));
+    ds_update_extra_row(ds);
+    norm = dnrm2_(&eps->n, COL(V, nv), &one);
+    norm2 = dnrm2_(&eps->n, COL(W, nv), &one);
+    OR_CALL(krylov_convergence(eps, eps->nconv, nv - eps->nconv, beta * norm, betat * norm2, 1.0, &k));
+    eps->t_ds += wtime() - t0;
+    if (k >= eps->nev) eps->reason = OR_CONVERGED_TOL;
+    else if (eps->its >= eps->max_it) eps->reason = OR_DIVERGED_ITS;
+    if (restart_cap > 0 && eps->its >= restart_cap && eps->reason == OR_CONVERGED_ITERATING)
+      eps->reason = OR_DIVERGED_ITS;
+
+    if (eps->reason != OR_CONVERGED_ITERATING || breakdown || k == nv) l = 0;
+    else {
+      l = (int)((nv - k) * eps->keep); if (l < 1) l = 1;
+      ds_truncate_size(ds, k, nv, &l);
+    }
+    if (!eps->lock && l > 0) { l += k; k = 0; }
+
+    V->l = eps->nconv; V->k = nv; W->l = eps->nconv; W->k = nv;
+    bv_multinplace(V, ds->Q, ds->ld, eps->nconv, k + l);
+    bv_multinplace(W, ds->Z, ds->ld, eps->nconv, k + l);
+    if (eps->reason == OR_CONVERGED_ITERATING && !breakdown) { bv_copycol(V, nv, k + l); bv_copycol(W, nv, k + l); }
+    if (eps->reason == OR_CONVERGED_ITERATING) {
+      if (breakdown || k == nv) {
+        if (k < eps->nev) {
+          OR_CALL(get_start_vector(eps, k, have_initial, &breakdown));
+          OR_CALL(get_left_start_vector(eps, k, &breakdownl));
+          if (breakdown || breakdownl) eps->reason = OR_DIVERGED_BREAKDOWN;
+        }
+      } else {
+        int dsn = ds->n, dsk = ds->k;
+        ds_setdims(ds, dsn, k, dsk);
+        ds_truncate(ds, k + l, 0);
+      }
+      OR_CALL(rq_update2(eps, k + l));
+    }
+    eps->nconv = k;
+  }
+  ds_truncate(ds, eps->nconv, 1);
+  return 0;
+}
+
 static int ks_solve(oeps *eps, int have_initial, int restart_cap)
 {
   double *g = NULL;                                                /* translation vector, harmonic only */
   int e;
+  if (eps->twosided) return ks_loop_twosided(eps, have_initial, restart_cap);
   if (eps->harmonic && !(g = (double*)calloc((size_t)eps->ds.ld, sizeof(double)))) return -55;
   e = ks_loop(eps, have_initial, restart_cap, g);
   free(g);
@@ -1128,10 +1424,39 @@ static double residual_norm(oeps *eps, double kr, double ki, const double *xr, c
   }
 }
 
+/* A^T by a stable counting sort (rows of A^T in increasing column order) */
+static int transpose_csr(int n, const int *rp, const int *ci, const double *v, int **trp, int **tci, double **tv)
+{
+  long nnz = rp[n];
+  int *fill = (int*)calloc((size_t)n + 1, sizeof(int));
+  *trp = (int*)calloc((size_t)n + 1, sizeof(int));
+  *tci = (int*)malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+  *tv = (double*)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+  if (!fill || !*trp || !*tci || !*tv) { free(fill); return -55; }
+  for (long p = 0; p < nnz; p++) (*trp)[ci[p] + 1]++;
+  for (int j = 0; j < n; j++) (*trp)[j + 1] += (*trp)[j];
+  for (int i = 0; i < n; i++)
+    for (int p = rp[i]; p < rp[i + 1]; p++) {
+      int q = (*trp)[ci[p]] + fill[ci[p]]++;
+      (*tci)[q] = i;
+      (*tv)[q] = v[p];
+    }
+  free(fill);
+  return 0;
+}
+
 int or_eps_solve(int n, const int *rowptr, const int *colidx, const double *val,
                  const or_opts *opts, const double *v0, int nds, const double *defl,
                  double *eigr, double *eigi, double *errest, double *err, double *X,
                  or_info *info)
+{
+  return or_eps_solve_twosided(n, rowptr, colidx, val, opts, v0, nds, defl, eigr, eigi, errest, err, X, NULL, info);
+}
+
+int or_eps_solve_twosided(int n, const int *rowptr, const int *colidx, const double *val,
+                          const or_opts *opts, const double *v0, int nds, const double *defl,
+                          double *eigr, double *eigi, double *errest, double *err, double *X, double *Y,
+                          or_info *info)
 {
   or_mkl_layer();
   oeps eps;
@@ -1145,6 +1470,9 @@ int or_eps_solve(int n, const int *rowptr, const int *colidx, const double *val,
   eps.target = opts->target; eps.hermitian = opts->hermitian;
   eps.harmonic = opts->extraction == 1;
   if (opts->extraction != 0 && opts->extraction != 1) OR_ERR(-56, "Unsupported extraction type");
+  eps.twosided = opts->twosided != 0;
+  if (eps.twosided && eps.hermitian) OR_ERR(-56, "Two-sided methods are not intended for symmetric problems");
+  if (eps.twosided) eps.harmonic = 0;                              /* the variant takes precedence */
   eps.dsherm = eps.hermitian && !eps.harmonic;
   eps.tol = opts->tol > 0 ? opts->tol : 1e-8;
   eps.keep = opts->keep > 0 ? opts->keep : 0.5;
@@ -1177,7 +1505,16 @@ int or_eps_solve(int n, const int *rowptr, const int *colidx, const double *val,
   eps.errest = (double*)calloc(requested, sizeof(double));
   eps.perm = (int*)calloc(requested, sizeof(int));
   if (!eps.V.V || !eps.V.buffer || !eps.eigr || !eps.eigi || !eps.errest || !eps.perm) { e = -55; goto done; }
-  if ((e = ds_alloc(&eps.ds, eps.dsherm ? DS_HEP : DS_NHEP, ncv + 1))) goto done;
+  if ((e = ds_alloc(&eps.ds, eps.twosided ? DS_NHEPTS : (eps.dsherm ? DS_HEP : DS_NHEP), ncv + 1))) goto done;
+  if (eps.twosided) {
+    eps.W = eps.V;                                                 /* same sizes and settings, own storage */
+    eps.W.nc = 0; eps.W.seed = eps.V.seed + 1000000007ULL; eps.W.rcount = 0; eps.W.passes = 0;
+    eps.W.V = (double*)calloc((size_t)n * requested, sizeof(double));
+    eps.W.buffer = (double*)calloc((size_t)requested * requested, sizeof(double));
+    eps.M = (double*)calloc((size_t)ncv * ncv, sizeof(double));
+    if (!eps.W.V || !eps.W.buffer || !eps.M) { e = -55; goto done; }
+    if ((e = transpose_csr(n, rowptr, colidx, val, &eps.trowptr, &eps.tcolidx, &eps.tval))) goto done;
+  }
   eps.ds.compact = eps.dsherm; eps.ds.extrarow = 1; eps.ds.sc = eps.sc;
 
   /* deflation space: BVInsertConstraints (orthonormalized, bvfunc.c:411) */
@@ -1217,6 +1554,11 @@ int or_eps_solve(int n, const int *rowptr, const int *colidx, const double *val,
     if ((e = ds_vectors(&eps.ds, NULL, NULL))) goto done;
     eps.V.l = 0; eps.V.k = eps.nconv;
     bv_multinplace(&eps.V, eps.ds.X, eps.ds.ld, 0, eps.hermitian ? eps.ds.n : eps.nconv);
+    if (eps.twosided) {                                            /* W = W * Y (epsdefault.c:141-148) */
+      if ((e = ds_vectors_left(&eps.ds, NULL, NULL))) goto done;
+      eps.W.l = 0; eps.W.k = eps.nconv;
+      bv_multinplace(&eps.W, eps.ds.Y, eps.ds.ld, 0, eps.nconv);
+    }
     if (eps.hermitian)                                             /* BVNormalize(V, eigi) */
       for (int j = 0; j < eps.nconv; j++) {
         int one = 1, pair = eps.eigi[j] != 0.0 && j + 1 < eps.nconv;
@@ -1255,6 +1597,10 @@ int or_eps_solve(int n, const int *rowptr, const int *colidx, const double *val,
         else if (ki > 0) memcpy(X + (size_t)i * n, xr, sizeof(double) * (size_t)n);
         else memcpy(X + (size_t)i * n, xr, sizeof(double) * (size_t)n);
       }
+      if (Y && eps.twosided) {                                     /* EPSGetLeftEigenvector, real part */
+        const double *yr = COL(&eps.W, ki < 0 ? k - 1 : k);
+        memcpy(Y + (size_t)i * n, yr, sizeof(double) * (size_t)n);
+      }
       if (err) {
         double r;
         if (ki == 0) r = residual_norm(&eps, kr, 0.0, xr, NULL, u, NULL);
@@ -1275,6 +1621,7 @@ int or_eps_solve(int n, const int *rowptr, const int *colidx, const double *val,
     info->nmatmult = eps.nmatmult; info->gs_passes = eps.V.passes; info->t_ds = eps.t_ds;
   }
 done:
+  if (eps.twosided) { free(eps.W.V); free(eps.W.buffer); free(eps.M); free(eps.trowptr); free(eps.tcolidx); free(eps.tval); }
   free(eps.V.V); free(eps.V.buffer); free(eps.eigr); free(eps.eigi); free(eps.errest); free(eps.perm);
   if (eps.ds.A) ds_free(&eps.ds);
   return e;

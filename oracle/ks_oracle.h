@@ -41,6 +41,7 @@ typedef struct {
   int    nthreads;               /* OpenMP threads for SpMV/BLAS; <= 0: leave as is */
   int    restart_cap;            /* > 0: stop after this many restarts (bounded timing sample) */
   int    extraction;             /* 0 EPS_RITZ, 1 EPS_HARMONIC (translation by target) */
+  int    twosided;               /* 1: two-sided variant (left eigenvectors too) */
 } or_opts;
 
 typedef struct {
@@ -98,6 +99,11 @@ int or_eps_solve(int n, const int *rowptr, const int *colidx, const double *val,
                  const or_opts *opts, const double *v0, int nds, const double *defl,
                  double *eigr, double *eigi, double *errest, double *err, double *X,
                  or_info *info);
+/* the same with the left eigenvectors of a two-sided solve (real parts) in Y (n x nconv) */
+int or_eps_solve_twosided(int n, const int *rowptr, const int *colidx, const double *val,
+                          const or_opts *opts, const double *v0, int nds, const double *defl,
+                          double *eigr, double *eigi, double *errest, double *err, double *X, double *Y,
+                          or_info *info);
 
 #ifdef __cplusplus
 }

@@ -26,7 +26,7 @@ class Opts(C.Structure):
                 ("tol", C.c_double), ("which", C.c_int), ("target", C.c_double),
                 ("hermitian", C.c_int), ("keep", C.c_double), ("lock", C.c_int),
                 ("eta", C.c_double), ("refine", C.c_int), ("nthreads", C.c_int),
-                ("restart_cap", C.c_int), ("extraction", C.c_int)]
+                ("restart_cap", C.c_int), ("extraction", C.c_int), ("twosided", C.c_int)]
 
 
 class Info(C.Structure):
@@ -166,12 +166,12 @@ def eps_solve(csr, nev: int, ncv: int = 0, mpd: int = 0, tol: float = 1e-8,
               which: int = LARGEST_MAGNITUDE, hermitian: bool = True, target: float = 0.0,
               v0=None, deflation=None, max_it: int = 0, keep: float = 0.5, lock: bool = True,
               eta: float = 0.7071, refine: int = REFINE_IFNEEDED, nthreads: int = 0,
-              restart_cap: int = 0, want_vectors: bool = True, extraction: int = 0):
+              restart_cap: int = 0, want_vectors: bool = True, extraction: int = 0, twosided: bool = False):
     rowptr, colidx, val = _csr_args(csr)
     n = len(rowptr) - 1
     o = Opts(nev=nev, ncv=ncv, mpd=mpd, max_it=max_it, tol=tol, which=which, target=target,
              hermitian=int(hermitian), keep=keep, lock=int(lock), eta=eta, refine=refine,
-             nthreads=nthreads, restart_cap=restart_cap, extraction=extraction)
+             nthreads=nthreads, restart_cap=restart_cap, extraction=extraction, twosided=int(twosided))
     ncv_eff = ncv if ncv > 0 else max(2 * nev, nev + 15)
     ncv_eff = min(ncv_eff, n) if ncv_eff > 0 else n
     size = max(ncv_eff, nev) + 2
@@ -187,13 +187,17 @@ def eps_solve(csr, nev: int, ncv: int = 0, mpd: int = 0, tol: float = 1e-8,
         defl = np.asfortranarray(deflation, dtype=np.float64)
         nds = defl.shape[1]
     info = Info()
-    rc = lib().or_eps_solve(C.c_int(n), _p(rowptr, C.c_int), _p(colidx, C.c_int), _p(val),
-                            C.byref(o), _p(v0a), C.c_int(nds), _p(defl), _p(eigr), _p(eigi),
-                            _p(errest), _p(err), _p(X), C.byref(info))
+    Y = np.zeros((n, size), order="F") if (want_vectors and twosided) else None
+    rc = lib().or_eps_solve_twosided(C.c_int(n), _p(rowptr, C.c_int), _p(colidx, C.c_int), _p(val),
+                                     C.byref(o), _p(v0a), C.c_int(nds), _p(defl), _p(eigr), _p(eigi),
+                                     _p(errest), _p(err), _p(X), _p(Y), C.byref(info))
     if rc:
         raise RuntimeError(f"or_eps_solve failed ({rc})")
     k = info.nconv
-    return dict(nconv=k, its=info.its, reason=info.reason, eigr=eigr[:k].copy(),
-                eigi=eigi[:k].copy(), errest=errest[:k].copy(), err=err[:k].copy(),
-                X=None if X is None else X[:, :k].copy(), nmatmult=info.nmatmult,
-                gs_passes=info.gs_passes, t_solve=info.t_solve, t_ds=info.t_ds)
+    out = dict(nconv=k, its=info.its, reason=info.reason, eigr=eigr[:k].copy(),
+               eigi=eigi[:k].copy(), errest=errest[:k].copy(), err=err[:k].copy(),
+               X=None if X is None else X[:, :k].copy(), nmatmult=info.nmatmult,
+               gs_passes=info.gs_passes, t_solve=info.t_solve, t_ds=info.t_ds)
+    if Y is not None:
+        out["Y"] = Y[:, :k].copy()
+    return out

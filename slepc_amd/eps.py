@@ -243,6 +243,9 @@ class Mat:
     def mult(self, x: Vec, y: Vec):
         _chk(lib().MatMult(self.h, x.h, y.h), "MatMult")
 
+    def multTranspose(self, x: Vec, y: Vec):
+        _chk(lib().MatMultTranspose(self.h, x.h, y.h), "MatMultTranspose")
+
     def info(self):
         nnz, mr, var, nb = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         _chk(lib().MatHIPCSRGetInfo(self.h, C.byref(nnz), C.byref(mr), C.byref(var), C.byref(nb)),
@@ -299,6 +302,9 @@ class EPS:
 
     def setExtraction(self, extr: int):
         _chk(lib().EPSSetExtraction(self.h, C.c_int(extr)), "EPSSetExtraction")
+
+    def setTwoSided(self, flg: bool = True):
+        _chk(lib().EPSSetTwoSided(self.h, C.c_int(int(flg))), "EPSSetTwoSided")
 
     def setDimensions(self, nev=-2, ncv=-1, mpd=-1):
         _chk(lib().EPSSetDimensions(self.h, C.c_int(nev), C.c_int(ncv), C.c_int(mpd)), "EPSSetDimensions")
@@ -395,6 +401,12 @@ class EPS:
         _chk(lib().EPSGetEigenvector(self.h, C.c_int(i), vr.h, vi.h if vi else None), "EPSGetEigenvector")
         return (vr.array(), vi.array()) if imag else vr.array()
 
+    def getLeftEigenvector(self, i: int, imag: bool = False):
+        wr = Vec(self.n)
+        wi = Vec(self.n) if imag else None
+        _chk(lib().EPSGetLeftEigenvector(self.h, C.c_int(i), wr.h, wi.h if wi else None), "EPSGetLeftEigenvector")
+        return (wr.array(), wi.array()) if imag else wr.array()
+
     def getErrorEstimate(self, i: int) -> float:
         r = C.c_double()
         _chk(lib().EPSGetErrorEstimate(self.h, C.c_int(i), C.byref(r)), "EPSGetErrorEstimate")
@@ -431,7 +443,7 @@ class EPS:
 
 def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=True, v0=None, max_it=-1,
           prec=F64, deflation=None, lock=True, keep=0.5, want_vectors=True, mixed_tol=0.0, step_graphs=False,
-          target=None, extraction=0):
+          target=None, extraction=0, twosided=False):
     """One-call convenience wrapper: returns a dict like the oracle's eps_solve.
     FP32 values in csr[2] keep the operator in FP32; ``prec`` is the basis precision."""
     A = Mat(csr, prec=F32 if np.asarray(csr[2]).dtype == np.float32 else F64)
@@ -445,6 +457,8 @@ def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=Tru
         eps.setTarget(target)
     if extraction:
         eps.setExtraction(extraction)
+    if twosided:
+        eps.setTwoSided(True)
     eps.setKrylovSchurLocking(lock)
     eps.setKrylovSchurRestart(keep)
     if mixed_tol > 0:
@@ -469,6 +483,11 @@ def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=Tru
         for i in range(k):
             X[:, i] = eps.getEigenvector(i)
         out["X"] = X
+        if twosided:
+            Y = np.zeros((A.n, k), order="F")
+            for i in range(k):
+                Y[:, i] = eps.getLeftEigenvector(i)
+            out["Y"] = Y
     out.update(eps.stats())
     if mixed_tol > 0:
         out.update(eps.mixedStats())

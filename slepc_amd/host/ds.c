@@ -14,6 +14,8 @@
  */
 #include "slepcimpl.h"
 
+#include <float.h>
+
 /* LAPACK/BLAS, Fortran ABI, 32-bit integers */
 extern void   dgemv_(const char *, const int *, const int *, const double *, const double *, const int *,
                      const double *, const int *, const double *, double *, const int *);
@@ -74,6 +76,8 @@ static void DSFreeMats(DS ds)
   PetscFree(ds->rwork);
   PetscFree(ds->iwork);
   PetscFree(ds->perm);
+  PetscFree(ds->wr2);
+  PetscFree(ds->wi2);
 }
 
 PetscErrorCode DSAllocate(DS ds, PetscInt ld)
@@ -87,6 +91,13 @@ PetscErrorCode DSAllocate(DS ds, PetscInt ld)
   PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)ld * 3, &ds->mat[DS_MAT_T]));
   PetscCall(PetscMallocZero(sizeof(PetscScalar) * sq, &ds->mat[DS_MAT_Q]));
   PetscCall(PetscMallocZero(sizeof(PetscScalar) * sq, &ds->mat[DS_MAT_X]));
+  if (!strcmp(ds->type, DSNHEPTS)) {   /* the second problem of the two-sided variant */
+    PetscCall(PetscMallocZero(sizeof(PetscScalar) * sq, &ds->mat[DS_MAT_B]));
+    PetscCall(PetscMallocZero(sizeof(PetscScalar) * sq, &ds->mat[DS_MAT_Z]));
+    PetscCall(PetscMallocZero(sizeof(PetscScalar) * sq, &ds->mat[DS_MAT_Y]));
+    PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)ld, &ds->wr2));
+    PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)ld, &ds->wi2));
+  }
   PetscCall(PetscMallocZero(sizeof(PetscScalar) * (sq + 8 * (size_t)ld), &ds->work));
   PetscCall(PetscMallocZero(sizeof(PetscReal) * (2 * (size_t)ld + 2), &ds->rwork));
   PetscCall(PetscMallocZero(sizeof(int) * (size_t)ld, &ds->iwork));
@@ -184,7 +195,7 @@ PetscErrorCode DSGetMat(DS ds, DSMatType m, Mat *A)
   PetscInt rows, cols;
   PetscCheck(m >= 0 && m < DS_NUM_MAT && ds->mat[m], PETSC_ERR_ARG_WRONG, "matrix %d not available", (int)m);
   if (ds->state == DS_STATE_TRUNCATED && m >= DS_MAT_Q) rows = ds->t;
-  else rows = (m == DS_MAT_A && ds->extrarow) ? ds->n + 1 : ds->n;
+  else rows = ((m == DS_MAT_A || m == DS_MAT_B) && ds->extrarow) ? ds->n + 1 : ds->n;
   cols = (m == DS_MAT_T) ? 3 : ds->n;
   PetscCheck(rows && cols, PETSC_ERR_ORDER, "Must call DSSetDimensions() first");
   if (ds->matview[m]) PetscCall(MatDestroy(&ds->matview[m]));
@@ -248,7 +259,7 @@ PetscErrorCode DSUpdateExtraRow(DS ds)
 PetscErrorCode DSVectors(DS ds, DSMatType mat, PetscInt *j, PetscReal *rnorm)
 {
   PetscErrorCode ierr;
-  PetscCheck(mat == DS_MAT_X, PETSC_ERR_SUP, "only right vectors (DS_MAT_X) are implemented");
+  PetscCheck(mat == DS_MAT_X || mat == DS_MAT_Y, PETSC_ERR_SUP, "only eigenvectors (DS_MAT_X, DS_MAT_Y) are implemented");
   PetscCheck(!rnorm || j, PETSC_ERR_ORDER, "Must give a value of j");
   DS_SEQ_BEGIN;
   ierr = ds->ops.vectors(ds, mat, j, rnorm);
@@ -476,12 +487,15 @@ PetscErrorCode DSCreate_HEP(DS ds)
 }
 
 /* ================================================================ DSNHEP */
+/* The real-Schur routines take the matrix pair they work on, so that DSNHEPTS
+   can run them on (A, Q) and on (B, Z). */
+
 /* DSSolve_NHEP_Private (dsutil.c:21-87): Hessenberg reduction when the state is
    RAW, then the real Schur form with dhseqr */
-static PetscErrorCode DSSolve_NHEP(DS ds, PetscScalar *wr, PetscScalar *wi)
+static PetscErrorCode NHEPSolve_Private(DS ds, DSMatType mA, DSMatType mQ, PetscScalar *wr, PetscScalar *wi)
 {
   int n = ds->n, ld = ds->ld, ilo = ds->l + 1, lwork = 6 * ld, info;
-  PetscScalar *A = ds->mat[DS_MAT_A], *Q = ds->mat[DS_MAT_Q], *tau = ds->work, *work = ds->work + ld;
+  PetscScalar *A = ds->mat[mA], *Q = ds->mat[mQ], *tau = ds->work, *work = ds->work + ld;
   PetscCheck(wi, PETSC_ERR_ARG_NULL, "DSNHEP needs the imaginary parts array");
   memset(Q, 0, sizeof(PetscScalar) * (size_t)ld * (size_t)ld);
   for (int i = 0; i < n; i++) Q[i + (size_t)i * ld] = 1.0;
@@ -518,13 +532,26 @@ static PetscErrorCode DSSolve_NHEP(DS ds, PetscScalar *wr, PetscScalar *wi)
   return PETSC_SUCCESS;
 }
 
+/* eigenvalues from the diagonal blocks of a real Schur form, rows j0..n-1 */
+static void SchurEigenvalues(DS ds, const PetscScalar *T, PetscInt j0, PetscScalar *wr, PetscScalar *wi)
+{
+  PetscInt n = ds->n, ld = ds->ld;
+  for (PetscInt j = j0; j < n; j++) {
+    wr[j] = T[j + (size_t)j * ld];
+    if (j < n - 1 && T[j + 1 + (size_t)j * ld] != 0.0) {
+      wi[j] = sqrt(fabs(T[j + 1 + (size_t)j * ld])) * sqrt(fabs(T[j + (size_t)(j + 1) * ld]));
+      wr[j + 1] = wr[j];
+      wi[j + 1] = -wi[j];
+      j++;
+    } else wi[j] = 0.0;
+  }
+}
+
 /* DSSort_NHEP_Total (dsutil.c:93-171): selection sort with dtrexc swaps */
-static PetscErrorCode DSSort_NHEP(DS ds, PetscScalar *wr, PetscScalar *wi, PetscScalar *rr, PetscScalar *ri, PetscInt *kk)
+static PetscErrorCode NHEPSortTotal_Private(DS ds, DSMatType mT, DSMatType mQ, PetscScalar *wr, PetscScalar *wi)
 {
   int n = ds->n, ld = ds->ld, info;
-  PetscScalar *T = ds->mat[DS_MAT_A], *Q = ds->mat[DS_MAT_Q];
-  (void)ri; (void)kk;
-  PetscCheck(!rr || rr == wr, PETSC_ERR_SUP, "arbitrary selection is not implemented");
+  PetscScalar *T = ds->mat[mT], *Q = ds->mat[mQ];
   for (int i = ds->l; i < n - 1; i++) {
     PetscScalar re = wr[i], im = wi[i];
     int pos = 0, j = i + 1;
@@ -541,41 +568,61 @@ static PetscErrorCode DSSort_NHEP(DS ds, PetscScalar *wr, PetscScalar *wi, Petsc
       int ifst = pos + 1, ilst = i + 1;
       dtrexc_("V", &n, T, &ld, Q, &ld, &ifst, &ilst, ds->work, &info);
       LAPACK_CHECK("trexc", info);
-      for (j = i; j < n; j++) {
-        wr[j] = T[j + (size_t)j * ld];
-        if (j < n - 1 && T[j + 1 + (size_t)j * ld] != 0.0) {
-          wi[j] = sqrt(fabs(T[j + 1 + (size_t)j * ld])) * sqrt(fabs(T[j + (size_t)(j + 1) * ld]));
-          wr[j + 1] = wr[j];
-          wi[j + 1] = -wi[j];
-          j++;
-        } else wi[j] = 0.0;
-      }
+      SchurEigenvalues(ds, T, i, wr, wi);
     }
     if (wi[i] != 0) i++;
   }
   return PETSC_SUCCESS;
 }
 
-/* DSUpdateExtraRow_NHEP (dsnhep.c:318-340): extra row <- extra row * Q */
-static PetscErrorCode DSUpdateExtraRow_NHEP(DS ds)
+/* DSSortWithPermutation_NHEP_Private (dsutil.c:177-236): move the block at perm[i]
+   to position i, for i = l, l+1, ... */
+static PetscErrorCode NHEPSortPermutation_Private(DS ds, PetscInt *perm, DSMatType mT, DSMatType mQ, PetscScalar *wr,
+                                                  PetscScalar *wi)
 {
-  int n = ds->n, ld = ds->ld, inc = 1;
-  double one = 1.0, zero = 0.0, *x = ds->work, *y = ds->work + ld;
-  PetscScalar *A = ds->mat[DS_MAT_A];
-  for (int i = 0; i < n; i++) x[i] = A[n + (size_t)i * ld];
-  dgemv_("C", &n, &n, &one, ds->mat[DS_MAT_Q], &ld, x, &inc, &zero, y, &inc);
-  for (int i = 0; i < n; i++) A[n + (size_t)i * ld] = y[i];
-  ds->k = n;
+  int n = ds->n, ld = ds->ld, info;
+  PetscScalar *T = ds->mat[mT], *Q = ds->mat[mQ];
+  for (PetscInt i = ds->l; i < n - 1; i++) {
+    PetscInt pos = perm[i];
+    int inc = (pos < n - 1 && T[pos + 1 + (size_t)pos * ld] != 0.0) ? 2 : 1;
+    if (pos != i) {
+      PetscCheck((pos == 0 || T[pos + (size_t)(pos - 1) * ld] == 0.0 || perm[i + 1] == pos - 1) &&
+                     (T[pos + 1 + (size_t)pos * ld] == 0.0 || perm[i + 1] == pos + 1),
+                 PETSC_ERR_FP, "Invalid permutation due to a 2x2 block at position %d", pos);
+      int ifst = pos + 1, ilst = i + 1;
+      dtrexc_("V", &n, T, &ld, Q, &ld, &ifst, &ilst, ds->work, &info);
+      LAPACK_CHECK("trexc", info);
+      for (PetscInt j = i + 1; j < n; j++)
+        if (perm[j] >= i && perm[j] < pos) perm[j] += inc;
+      perm[i] = i;
+      if (inc == 2) perm[i + 1] = i + 1;
+    }
+    if (inc == 2) i++;
+  }
+  SchurEigenvalues(ds, T, ds->l, wr, wi);
   return PETSC_SUCCESS;
 }
 
-/* DSVectors_NHEP_Eigen_Some (dsnhep.c:101-167) */
-static PetscErrorCode DSVectors_NHEP_Some(DS ds, PetscInt *k, PetscReal *rnorm)
+/* DSUpdateExtraRow_NHEP (dsnhep.c:318-340): extra row <- extra row * Q */
+static void NHEPUpdateExtraRow_Private(DS ds, DSMatType mA, DSMatType mQ)
+{
+  int n = ds->n, ld = ds->ld, inc = 1;
+  double one = 1.0, zero = 0.0, *x = ds->work, *y = ds->work + ld;
+  PetscScalar *A = ds->mat[mA];
+  for (int i = 0; i < n; i++) x[i] = A[n + (size_t)i * ld];
+  dgemv_("C", &n, &n, &one, ds->mat[mQ], &ld, x, &inc, &zero, y, &inc);
+  for (int i = 0; i < n; i++) A[n + (size_t)i * ld] = y[i];
+}
+
+/* DSVectors_NHEP_Eigen_Some (dsnhep.c:101-167): eigenvector k of the Schur form
+   mA, back-transformed with mQ into column k of mX */
+static PetscErrorCode NHEPVectorsSome_Private(DS ds, DSMatType mA, DSMatType mQ, DSMatType mX, PetscInt *k,
+                                              PetscReal *rnorm)
 {
   int n = ds->n, ld = ds->ld, mm, mout, info, inc = 1, cols = 1, zero = 0, iscomplex = 0;
   double sone = 1.0, szero = 0.0, norm, done = 1.0;
   int *select = ds->iwork;
-  PetscScalar *A = ds->mat[DS_MAT_A], *Y = ds->mat[DS_MAT_X] + (size_t)(*k) * ld;
+  PetscScalar *A = ds->mat[mA], *Y = ds->mat[mX] + (size_t)(*k) * ld;
   for (int i = 0; i < n; i++) select[i] = 0;
   select[*k] = 1;
   if (*k < n - 1 && A[*k + 1 + (size_t)(*k) * ld] != 0.0) iscomplex = 1;
@@ -587,8 +634,8 @@ static PetscErrorCode DSVectors_NHEP_Some(DS ds, PetscInt *k, PetscReal *rnorm)
   if (ds->state >= DS_STATE_CONDENSED) {
     double *w = ds->work;
     memcpy(w, Y, sizeof(double) * (size_t)mout * (size_t)ld);
-    dgemv_("N", &n, &n, &sone, ds->mat[DS_MAT_Q], &ld, w, &inc, &szero, Y, &inc);
-    if (iscomplex) dgemv_("N", &n, &n, &sone, ds->mat[DS_MAT_Q], &ld, w + ld, &inc, &szero, Y + ld, &inc);
+    dgemv_("N", &n, &n, &sone, ds->mat[mQ], &ld, w, &inc, &szero, Y, &inc);
+    if (iscomplex) dgemv_("N", &n, &n, &sone, ds->mat[mQ], &ld, w + ld, &inc, &szero, Y + ld, &inc);
     norm = dnrm2_(&n, Y, &inc);
     if (iscomplex) {
       norm = SlepcAbsEigenvalue(norm, dnrm2_(&n, Y + ld, &inc));
@@ -603,15 +650,15 @@ static PetscErrorCode DSVectors_NHEP_Some(DS ds, PetscInt *k, PetscReal *rnorm)
 }
 
 /* DSVectors_NHEP_Eigen_All (dsnhep.c:168-226) */
-static PetscErrorCode DSVectors_NHEP_All(DS ds)
+static PetscErrorCode NHEPVectorsAll_Private(DS ds, DSMatType mA, DSMatType mQ, DSMatType mX)
 {
   int n = ds->n, ld = ds->ld, mout, info, inc = 1, cols, zero = 0;
   double norm, done = 1.0;
-  PetscScalar *A = ds->mat[DS_MAT_A], *X = ds->mat[DS_MAT_X];
+  PetscScalar *A = ds->mat[mA], *X = ds->mat[mX];
   const char *back = "A";
   if (ds->state >= DS_STATE_CONDENSED) {
     back = "B";
-    memcpy(X, ds->mat[DS_MAT_Q], sizeof(PetscScalar) * (size_t)ld * (size_t)ld);
+    memcpy(X, ds->mat[mQ], sizeof(PetscScalar) * (size_t)ld * (size_t)ld);
   }
   dtrevc_("R", back, NULL, &n, A, &ld, NULL, &ld, X, &ld, &n, &mout, ds->work, &info);
   LAPACK_CHECK("trevc", info);
@@ -630,11 +677,30 @@ static PetscErrorCode DSVectors_NHEP_All(DS ds)
   return PETSC_SUCCESS;
 }
 
+static PetscErrorCode DSSolve_NHEP(DS ds, PetscScalar *wr, PetscScalar *wi)
+{
+  return NHEPSolve_Private(ds, DS_MAT_A, DS_MAT_Q, wr, wi);
+}
+
+static PetscErrorCode DSSort_NHEP(DS ds, PetscScalar *wr, PetscScalar *wi, PetscScalar *rr, PetscScalar *ri, PetscInt *kk)
+{
+  (void)ri; (void)kk;
+  PetscCheck(!rr || rr == wr, PETSC_ERR_SUP, "arbitrary selection is not implemented");
+  return NHEPSortTotal_Private(ds, DS_MAT_A, DS_MAT_Q, wr, wi);
+}
+
+static PetscErrorCode DSUpdateExtraRow_NHEP(DS ds)
+{
+  NHEPUpdateExtraRow_Private(ds, DS_MAT_A, DS_MAT_Q);
+  ds->k = ds->n;
+  return PETSC_SUCCESS;
+}
+
 static PetscErrorCode DSVectors_NHEP(DS ds, DSMatType mat, PetscInt *j, PetscReal *rnorm)
 {
-  (void)mat;
-  if (j) return DSVectors_NHEP_Some(ds, j, rnorm);
-  return DSVectors_NHEP_All(ds);
+  PetscCheck(mat == DS_MAT_X, PETSC_ERR_SUP, "DSNHEP computes right vectors only");
+  if (j) return NHEPVectorsSome_Private(ds, DS_MAT_A, DS_MAT_Q, DS_MAT_X, j, rnorm);
+  return NHEPVectorsAll_Private(ds, DS_MAT_A, DS_MAT_Q, DS_MAT_X);
 }
 
 /* DSGetTruncateSize_Default (dsops.c:329-344): do not split a 2x2 block */
@@ -648,27 +714,43 @@ static PetscErrorCode DSGetTruncateSize_NHEP(DS ds, PetscInt l, PetscInt n, Pets
   return PETSC_SUCCESS;
 }
 
-/* DSTruncate_NHEP (dsnhep.c:394-425) */
-static PetscErrorCode DSTruncate_NHEP(DS ds, PetscInt n, PetscBool trim)
+/* DSTruncate_NHEP (dsnhep.c:394-425); DSTruncate_NHEPTS (dsnhepts.c:369-398) does the
+   same to B */
+static PetscErrorCode TruncateNHEP_Private(DS ds, PetscInt n, PetscBool trim, PetscBool withB)
 {
   PetscInt ld = ds->ld, l = ds->l;
-  PetscScalar *A = ds->mat[DS_MAT_A];
+  PetscScalar *A = ds->mat[DS_MAT_A], *B = withB ? ds->mat[DS_MAT_B] : NULL;
   if (trim) {
-    if (ds->extrarow) for (PetscInt i = l; i < ds->n; i++) A[ds->n + (size_t)i * ld] = 0.0;
+    if (ds->extrarow)
+      for (PetscInt i = l; i < ds->n; i++) {
+        A[ds->n + (size_t)i * ld] = 0.0;
+        if (B) B[ds->n + (size_t)i * ld] = 0.0;
+      }
     ds->l = 0;
     ds->k = 0;
     ds->n = n;
     ds->t = n;
   } else {
     if (ds->extrarow && ds->k == ds->n) {
-      for (PetscInt i = l; i < n; i++) A[n + (size_t)i * ld] = A[ds->n + (size_t)i * ld];
-      for (PetscInt i = l; i < ds->n; i++) A[ds->n + (size_t)i * ld] = 0.0;
+      for (PetscInt i = l; i < n; i++) {
+        A[n + (size_t)i * ld] = A[ds->n + (size_t)i * ld];
+        if (B) B[n + (size_t)i * ld] = B[ds->n + (size_t)i * ld];
+      }
+      for (PetscInt i = l; i < ds->n; i++) {
+        A[ds->n + (size_t)i * ld] = 0.0;
+        if (B) B[ds->n + (size_t)i * ld] = 0.0;
+      }
     }
     ds->k = ds->extrarow ? n : 0;
     ds->t = ds->n;
     ds->n = n;
   }
   return PETSC_SUCCESS;
+}
+
+static PetscErrorCode DSTruncate_NHEP(DS ds, PetscInt n, PetscBool trim)
+{
+  return TruncateNHEP_Private(ds, n, trim, PETSC_FALSE);
 }
 
 /* DSTranslateHarmonic_NHEP (dsnhep.c:466-535).  Translate: g = (A - tau I)^-T b
@@ -722,5 +804,104 @@ PetscErrorCode DSCreate_NHEP(DS ds)
   ds->ops.vectors = DSVectors_NHEP;
   ds->ops.truncate = DSTruncate_NHEP;
   ds->ops.gettruncatesize = DSGetTruncateSize_NHEP;
+  return PETSC_SUCCESS;
+}
+
+/* ================================================================ DSNHEPTS */
+/* DSNHEPTS (dsnhepts.c): two related problems, A X = X Lambda from the Arnoldi
+   process on the operator and B Y = Y Lambda' from the one on its transpose.
+   After the solve A and B hold real Schur forms A Q = Q T, B Z = Z S; the sort
+   orders A's eigenvalues by the criterion, B's the same way, then permutes B's
+   blocks so that B's eigenvalue i is the one nearest A's eigenvalue i. */
+
+/* DSSolve_NHEPTS (dsnhepts.c:241-252) */
+static PetscErrorCode DSSolve_NHEPTS(DS ds, PetscScalar *wr, PetscScalar *wi)
+{
+  PetscCall(NHEPSolve_Private(ds, DS_MAT_A, DS_MAT_Q, wr, wi));
+  return NHEPSolve_Private(ds, DS_MAT_B, DS_MAT_Z, ds->wr2, ds->wi2);
+}
+
+/* DSSort_NHEPTS (dsnhepts.c:188-224) */
+static PetscErrorCode DSSort_NHEPTS(DS ds, PetscScalar *wr, PetscScalar *wi, PetscScalar *rr, PetscScalar *ri, PetscInt *kk)
+{
+  PetscInt n = ds->n, cont = 0, id = 0, *p = NULL, *idx = NULL, *idx2 = NULL;
+  const PetscReal tol = sqrt(DBL_EPSILON);
+  PetscScalar *wr2 = ds->wr2, *wi2 = ds->wi2;
+  PetscErrorCode e;
+  (void)ri; (void)kk;
+  PetscCheck(!rr || rr == wr, PETSC_ERR_SUP, "arbitrary selection is not implemented");
+  PetscCall(NHEPSortTotal_Private(ds, DS_MAT_A, DS_MAT_Q, wr, wi));
+  PetscCall(NHEPSortTotal_Private(ds, DS_MAT_B, DS_MAT_Z, wr2, wi2));
+  PetscCall(PetscMallocZero(sizeof(PetscInt) * (size_t)ds->ld, &p));
+  PetscCall(PetscMallocZero(sizeof(PetscInt) * (size_t)ds->ld, &idx));
+  PetscCall(PetscMallocZero(sizeof(PetscInt) * (size_t)ds->ld, &idx2));
+  for (PetscInt i = 0; i < n; i++) {
+    if (SlepcAbsEigenvalue(wr2[i] - wr[i], wi2[i] - wi[i]) > tol) { idx2[cont] = i; idx[cont++] = i; }
+    p[i] = -1;
+  }
+  e = PETSC_SUCCESS;
+  if (cont) {
+    for (PetscInt i = 0; i < cont; i++) {
+      PetscReal t = DBL_MAX, s;
+      for (PetscInt j = 0; j < cont; j++)
+        if (idx2[j] != -1 && (s = SlepcAbsEigenvalue(wr2[idx[j]] - wr[idx[i]], wi2[idx[j]] - wi[idx[i]])) < t) {
+          id = j;
+          t = s;
+        }
+      p[idx[i]] = idx[id];
+      idx2[id] = -1;
+    }
+    for (PetscInt i = 0; i < n; i++)
+      if (p[i] == -1) p[i] = i;
+    e = NHEPSortPermutation_Private(ds, p, DS_MAT_B, DS_MAT_Z, wr2, wi2);
+  }
+  free(p);
+  free(idx);
+  free(idx2);
+  return e;
+}
+
+/* DSUpdateExtraRow_NHEPTS (dsnhepts.c:226-239) */
+static PetscErrorCode DSUpdateExtraRow_NHEPTS(DS ds)
+{
+  NHEPUpdateExtraRow_Private(ds, DS_MAT_A, DS_MAT_Q);
+  NHEPUpdateExtraRow_Private(ds, DS_MAT_B, DS_MAT_Z);
+  ds->k = ds->n;
+  return PETSC_SUCCESS;
+}
+
+/* DSVectors_NHEPTS (dsnhepts.c:162-186): X from (A, Q), Y from (B, Z) */
+static PetscErrorCode DSVectors_NHEPTS(DS ds, DSMatType mat, PetscInt *j, PetscReal *rnorm)
+{
+  const PetscBool left = mat == DS_MAT_Y;
+  PetscCheck(mat == DS_MAT_X || mat == DS_MAT_Y, PETSC_ERR_ARG_OUTOFRANGE, "Invalid mat parameter");
+  if (j) return NHEPVectorsSome_Private(ds, left ? DS_MAT_B : DS_MAT_A, left ? DS_MAT_Z : DS_MAT_Q, mat, j, rnorm);
+  return NHEPVectorsAll_Private(ds, left ? DS_MAT_B : DS_MAT_A, left ? DS_MAT_Z : DS_MAT_Q, mat);
+}
+
+/* DSGetTruncateSize_NHEPTS (dsnhepts.c:349-367) */
+static PetscErrorCode DSGetTruncateSize_NHEPTS(DS ds, PetscInt l, PetscInt n, PetscInt *k)
+{
+  const size_t at = (size_t)(l + *k) + (size_t)(l + *k - 1) * ds->ld;
+  if (ds->mat[DS_MAT_A][at] != 0.0 || ds->mat[DS_MAT_B][at] != 0.0) {
+    if (l + *k < n - 1) (*k)++;
+    else (*k)--;
+  }
+  return PETSC_SUCCESS;
+}
+
+static PetscErrorCode DSTruncate_NHEPTS(DS ds, PetscInt n, PetscBool trim)
+{
+  return TruncateNHEP_Private(ds, n, trim, PETSC_TRUE);
+}
+
+PetscErrorCode DSCreate_NHEPTS(DS ds)
+{
+  ds->ops.solve = DSSolve_NHEPTS;
+  ds->ops.sort = DSSort_NHEPTS;
+  ds->ops.update = DSUpdateExtraRow_NHEPTS;
+  ds->ops.vectors = DSVectors_NHEPTS;
+  ds->ops.truncate = DSTruncate_NHEPTS;
+  ds->ops.gettruncatesize = DSGetTruncateSize_NHEPTS;
   return PETSC_SUCCESS;
 }

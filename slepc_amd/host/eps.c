@@ -451,6 +451,49 @@ PetscErrorCode EPSGetEigenvector(EPS eps, PetscInt i, Vec Vr, Vec Vi)
   return PETSC_SUCCESS;
 }
 
+/* EPSGetLeftEigenvector (epssolve.c:567-611): the left eigenvector from W, for
+   solves with EPSSetTwoSided */
+PetscErrorCode EPSGetLeftEigenvector(EPS eps, PetscInt i, Vec Wr, Vec Wi)
+{
+  PetscInt k;
+  PetscScalar ei;
+  PetscCheck(eps->state >= EPS_STATE_SOLVED, PETSC_ERR_ARG_WRONGSTATE, "Must call EPSSolve() first");
+  PetscCheck(i >= 0 && i < eps->nconv, PETSC_ERR_ARG_OUTOFRANGE, "Argument 2 out of range");
+  PetscCheck(eps->twosided && eps->W, PETSC_ERR_ARG_WRONGSTATE, "Must request left vectors with EPSSetTwoSided");
+  PetscCall(EPSComputeVectors(eps));
+  k = eps->perm[i];
+  ei = eps->eigi[k];
+  if (ei > 0.0) {
+    if (Wr) PetscCall(BVCopyVec(eps->W, k, Wr));
+    if (Wi) PetscCall(BVCopyVec(eps->W, k + 1, Wi));
+  } else if (ei < 0.0) {
+    if (Wr) PetscCall(BVCopyVec(eps->W, k - 1, Wr));
+    if (Wi) {
+      PetscCall(BVCopyVec(eps->W, k, Wi));
+      PetscCall(VecScale(Wi, -1.0));
+    }
+  } else {
+    if (Wr) PetscCall(BVCopyVec(eps->W, k, Wr));
+    if (Wi) PetscCall(VecSet(Wi, 0.0));
+  }
+  return PETSC_SUCCESS;
+}
+
+/* EPSSetTwoSided (epsopts.c): also compute left eigenvectors, by the two-sided
+   Krylov-Schur variant (ks-twosided.c) */
+PetscErrorCode EPSSetTwoSided(EPS eps, PetscBool twosided)
+{
+  if (eps->twosided != twosided) eps->state = EPS_STATE_INITIAL;
+  eps->twosided = twosided ? PETSC_TRUE : PETSC_FALSE;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode EPSGetTwoSided(EPS eps, PetscBool *twosided)
+{
+  *twosided = eps->twosided;
+  return PETSC_SUCCESS;
+}
+
 PetscErrorCode EPSGetEigenpair(EPS eps, PetscInt i, PetscScalar *eigr, PetscScalar *eigi, Vec Vr, Vec Vi)
 {
   PetscCall(EPSGetEigenvector(eps, i, Vr, Vi));
@@ -548,10 +591,10 @@ PetscErrorCode EPSGetStartVector(EPS eps, PetscInt i, PetscBool *breakdown)
 /* EPSKrylovConvergence (epskrylov.c:207-295) for STSHIFT sigma=0, trivial
    region and the relative convergence test EPSConvergedRelative (epsdefault.c:224-232) */
 PetscErrorCode EPSKrylovConvergence(EPS eps, PetscBool getall, PetscInt kini, PetscInt nits, PetscReal beta,
-                                    PetscReal corrf, PetscInt *kout)
+                                    PetscReal betat, PetscReal corrf, PetscInt *kout)
 {
-  PetscInt k, newk, marker = -1;
-  PetscReal resnorm, w;
+  PetscInt k, newk, newk2, marker = -1;
+  PetscReal resnorm, w, lerrest;
   for (k = kini; k < kini + nits; k++) {
     PetscScalar re = eps->eigr[k], im = eps->eigi[k];
     newk = k;
@@ -560,6 +603,14 @@ PetscErrorCode EPSKrylovConvergence(EPS eps, PetscBool getall, PetscInt kini, Pe
     w = SlepcAbsEigenvalue(re, im);
     eps->errest[k] = (w != 0.0) ? resnorm / w : PETSC_MAX_REAL;
     if (marker == -1 && eps->errest[k] >= eps->tol) marker = k;
+    if (eps->twosided) {   /* the left residual must converge too (epskrylov.c:269-276) */
+      newk2 = k;
+      PetscCall(DSVectors(eps->ds, DS_MAT_Y, &newk2, &resnorm));
+      resnorm *= betat;
+      lerrest = (w != 0.0) ? resnorm / w : PETSC_MAX_REAL;
+      eps->errest[k] = PetscMax(eps->errest[k], lerrest);
+      if (marker == -1 && lerrest >= eps->tol) marker = k;
+    }
     if (newk == k + 1) {
       eps->errest[k + 1] = eps->errest[k];
       k++;
@@ -593,6 +644,7 @@ PetscErrorCode EPSDestroy(EPS *eps)
   EPSFreeBasis(&e->IS, &e->nini);
   EPSFreeBasis(&e->defl, &e->nds);
   PetscCall(BVDestroy(&e->V));
+  PetscCall(BVDestroy(&e->W));
   PetscCall(DSDestroy(&e->ds));
   PetscCall(STDestroy(&e->st));
   free(e->eigr);

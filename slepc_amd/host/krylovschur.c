@@ -9,6 +9,15 @@
  */
 #include "slepcimpl.h"
 
+extern void dgetrf_(const int *, const int *, double *, const int *, int *, int *);
+extern void dgetrs_(const char *, const int *, const int *, const double *, const int *, const int *, double *,
+                    const int *, int *);
+extern void dgemm_(const char *, const char *, const int *, const int *, const int *, const double *, const double *,
+                   const int *, const double *, const int *, const double *, double *, const int *);
+
+/* the left basis draws its random columns from a stream of its own */
+#define KS_LEFT_SEED_OFFSET 1000000007ULL
+
 /* EPSSetDimensions_Default (epssetup.c:648-671) */
 static PetscErrorCode EPSSetDimensions_Default(EPS eps, PetscInt nev, PetscInt *ncv, PetscInt *mpd)
 {
@@ -57,6 +66,17 @@ static PetscErrorCode EPSAllocateSolution(EPS eps, PetscInt extra)
     V->ci[0] = V->ci[1] = BV_NOCOL;
     PetscCall(V->ops.create(V));
   }
+  /* W: a basis like V for the two-sided variant (epssetup.c:726-731) */
+  if (eps->W && (!eps->twosided || eps->W->m != requested || eps->W->n != eps->n || eps->W->prec != V->prec))
+    PetscCall(BVDestroy(&eps->W));
+  if (eps->twosided && !eps->W) {
+    PetscCall(BVCreate(&eps->W));
+    PetscCall(BVSetSizes(eps->W, eps->n, eps->n, requested));
+    if (V->prec != SLEPC_AMD_F64) PetscCall(BVHIPKSSetPrecision(eps->W, V->prec));
+    PetscCall(BVSetOrthogonalization(eps->W, V->orthog_type, V->orthog_ref, V->orthog_eta, V->orthog_block));
+    PetscCall(BVSetType(eps->W, V->type));
+    eps->W->rseed = V->rseed + KS_LEFT_SEED_OFFSET;
+  }
   return PETSC_SUCCESS;
 }
 
@@ -70,7 +90,11 @@ static PetscErrorCode EPSSetUp_KrylovSchur(EPS eps)
   if (!eps->which) eps->which = EPS_LARGEST_MAGNITUDE;   /* EPSSetWhichEigenpairs_Default with STSHIFT */
   PetscCheck(ctx->lock || eps->mpd >= eps->ncv, PETSC_ERR_SUP, "Should not use mpd parameter in non-locking variant");
   PetscCheck(eps->extraction == EPS_RITZ || eps->extraction == EPS_HARMONIC, PETSC_ERR_SUP, "Unsupported extraction type");
-  ctx->harmonic = eps->extraction == EPS_HARMONIC;
+  PetscCheck(!eps->twosided || !eps->ishermitian, PETSC_ERR_SUP, "Two-sided methods are not intended for symmetric problems");
+  PetscCheck(!eps->twosided || ctx->mixed_tol <= 0.0, PETSC_ERR_SUP,
+             "The two-sided variant is not available in the mixed-precision form");
+  /* the two-sided variant takes precedence over the extraction (krylovschur.c:143-151) */
+  ctx->harmonic = eps->extraction == EPS_HARMONIC && !eps->twosided;
   PetscCheck(!ctx->harmonic || ctx->mixed_tol <= 0.0, PETSC_ERR_SUP,
              "Harmonic extraction is not available in the mixed-precision form");
   if (!ctx->keep) ctx->keep = 0.5;
@@ -78,8 +102,19 @@ static PetscErrorCode EPSSetUp_KrylovSchur(EPS eps)
   free(ctx->g);
   ctx->g = NULL;
   if (ctx->harmonic) PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)(eps->ncv + 1), &ctx->g));
-  /* harmonic extraction runs the default (Arnoldi + DSNHEP) variant for Hermitian problems too */
-  if (eps->ishermitian && !ctx->harmonic) {
+  PetscCall(MatDestroy(&ctx->AT));
+  free(ctx->M);
+  ctx->M = NULL;
+  if (eps->twosided) {
+    Mat Op;
+    PetscCall(STGetOperator(eps->st, &Op));
+    PetscCall(MatHIPCSRCreateTranspose(Op, &ctx->AT));
+    PetscCall(STRestoreOperator(eps->st, &Op));
+    PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)eps->ncv * (size_t)eps->ncv, &ctx->M));
+    PetscCall(DSSetType(eps->ds, DSNHEPTS));
+    PetscCall(DSSetCompact(eps->ds, PETSC_FALSE));
+  } else if (eps->ishermitian && !ctx->harmonic) {
+    /* harmonic extraction runs the default (Arnoldi + DSNHEP) variant for Hermitian problems too */
     PetscCall(DSSetType(eps->ds, DSHEP));
     PetscCall(DSSetCompact(eps->ds, PETSC_TRUE));
   } else {
@@ -150,7 +185,7 @@ static PetscErrorCode KSAfterExpand(EPS eps)
   PetscCall(DSUpdateExtraRow(eps->ds));
 
   /* check convergence */
-  PetscCall(EPSKrylovConvergence(eps, PETSC_FALSE, eps->nconv, nv - eps->nconv, beta, gamma, &k));
+  PetscCall(EPSKrylovConvergence(eps, PETSC_FALSE, eps->nconv, nv - eps->nconv, beta, 0.0, gamma, &k));
   eps->t_ds += SlepcWallTime() - t0;
   /* EPSStoppingBasic (epsdefault.c:290-302) */
   if (k >= eps->nev) eps->reason = EPS_CONVERGED_TOL;
@@ -309,12 +344,228 @@ static PetscErrorCode MixedEnd(EPS eps)
   return PETSC_SUCCESS;
 }
 
+/* ------------------------------------------------------------ two-sided variant */
+/* EPSGetLeftStartVector (epssolve.c:879-900): a random column of W, orthonormalized */
+PetscErrorCode EPSGetLeftStartVector(EPS eps, PetscInt i, PetscBool *breakdown)
+{
+  PetscReal norm;
+  PetscBool lindep;
+  PetscCall(BVSetRandomColumn(eps->W, i));
+  PetscCall(BVOrthogonalizeColumn(eps->W, i, NULL, &norm, &lindep));
+  if (breakdown) *breakdown = lindep;
+  else if (lindep || norm == 0.0) {
+    PetscCheck(i, PETSC_ERR_PLIB, "Left initial vector is zero");
+    SETERRQ(PETSC_ERR_CONV_FAILED, "Unable to generate more left start vectors");
+  }
+  PetscCall(BVScaleColumn(eps->W, i, 1.0 / norm));
+  return PETSC_SUCCESS;
+}
+
+/* one Arnoldi expansion k..nv of V with Op into the DS matrix mat (BVMatArnoldi) */
+static PetscErrorCode TwoSidedExpand(EPS eps, BV V, Mat Op, DSMatType mat, PetscInt k, PetscInt *nv, PetscReal *beta,
+                                     PetscBool *breakdown)
+{
+  Mat H;
+  eps->nmatmult += *nv - k;
+  PetscCall(BVMatKrylovBegin(V, Op, k, *nv));
+  PetscCall(BVMatKrylovEnd(V, k, nv, beta, breakdown));
+  PetscCall(DSGetMat(eps->ds, mat, &H));
+  PetscCall(BVCopyHessenberg(V, H, k, *nv));
+  PetscCall(DSRestoreMat(eps->ds, mat, &H));
+  return PETSC_SUCCESS;
+}
+
+/* EPSTwoSidedRQUpdate1 (ks-twosided.c:25-77): oblique correction of the two residual
+   vectors, u -= V M^-1 W^T u and w -= W M^-T V^T w, folded into S and T */
+static PetscErrorCode TwoSidedRQUpdate1(EPS eps, PetscInt nv, PetscReal beta, PetscReal betat)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  PetscInt ld, l, nnv, ncv = eps->ncv;
+  int n_ = (int)nv, ncv_ = (int)ncv, one = 1, info = 0, *p = NULL;
+  PetscScalar *A = NULL, *w = NULL, *S;
+  Vec u = NULL;
+  PetscErrorCode e = PETSC_SUCCESS;
+  PetscCall(DSGetLeadingDimension(eps->ds, &ld));
+  PetscCall(PetscMallocZero(sizeof(int) * (size_t)(nv > 0 ? nv : 1), &p));
+  PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)ncv * (size_t)ncv, &A));
+  PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)ncv, &w));
+  PetscCall(BVGetActiveColumns(eps->V, &l, &nnv));
+  PetscCall(BVSetActiveColumns(eps->V, 0, nv));
+  PetscCall(BVSetActiveColumns(eps->W, 0, nv));
+  e = BVGetColumn(eps->V, nv, &u);
+  if (!e) e = BVDotVec(eps->W, u, w);
+  if (u) BVRestoreColumn(eps->V, nv, &u);
+  if (!e) {
+    memcpy(A, ctx->M, sizeof(PetscScalar) * (size_t)ncv * (size_t)ncv);
+    dgetrf_(&n_, &n_, A, &ncv_, p, &info);
+    if (!info) dgetrs_("N", &n_, &one, A, &ncv_, p, w, &ncv_, &info);
+    if (info) e = PETSC_ERR_LIB;
+  }
+  if (!e) e = BVMultColumn(eps->V, -1.0, 1.0, nv, w);
+  if (!e) {
+    S = eps->ds->mat[DS_MAT_A];
+    for (PetscInt i = 0; i < nv; i++) S[(size_t)(nv - 1) * ld + i] += beta * w[i];
+    u = NULL;
+    e = BVGetColumn(eps->W, nv, &u);
+    if (!e) e = BVDotVec(eps->V, u, w);
+    if (u) BVRestoreColumn(eps->W, nv, &u);
+  }
+  if (!e) {
+    dgetrs_("C", &n_, &one, A, &ncv_, p, w, &ncv_, &info);
+    if (info) e = PETSC_ERR_LIB;
+  }
+  if (!e) e = BVMultColumn(eps->W, -1.0, 1.0, nv, w);
+  if (!e) {
+    S = eps->ds->mat[DS_MAT_B];
+    for (PetscInt i = 0; i < nv; i++) S[(size_t)(nv - 1) * ld + i] += betat * w[i];
+  }
+  free(p);
+  free(A);
+  free(w);
+  PetscCall(BVSetActiveColumns(eps->V, l, nnv));
+  PetscCall(BVSetActiveColumns(eps->W, l, nnv));
+  PetscCheck(!e || e != PETSC_ERR_LIB, PETSC_ERR_LIB, "LAPACK getrf/getrs failed on the projection matrix (info %d)", info);
+  return e;
+}
+
+/* EPSTwoSidedRQUpdate2 (ks-twosided.c:79-136): re-orthogonalize the kept residual
+   columns k of V and W, fold the coefficients into S and T, and M = Z^T M Q */
+static PetscErrorCode TwoSidedRQUpdate2(EPS eps, PetscInt k)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  PetscInt ld, l, nv, ncv = eps->ncv;
+  PetscReal norm;
+  PetscScalar *c = NULL, *w = NULL, *A, zero = 0.0, sone = 1.0;
+  int n_, ncv_ = (int)ncv, ld_;
+  PetscCall(DSGetLeadingDimension(eps->ds, &ld));
+  PetscCall(BVGetActiveColumns(eps->V, &l, &nv));
+  PetscCall(BVSetActiveColumns(eps->V, 0, nv));
+  PetscCall(BVSetActiveColumns(eps->W, 0, nv));
+  PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)ncv * (size_t)ncv, &w));
+  PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)(ncv + 1), &c));
+  for (int side = 0; side < 2; side++) {
+    BV X = side ? eps->W : eps->V;
+    PetscCall(BVOrthogonalizeColumn(X, k, c, &norm, NULL));
+    PetscCall(BVScaleColumn(X, k, 1.0 / norm));
+    A = eps->ds->mat[side ? DS_MAT_B : DS_MAT_A];
+    for (PetscInt j = l; j < k; j++) {
+      for (PetscInt i = 0; i < k; i++) A[i + (size_t)j * ld] += c[i] * A[k + (size_t)j * ld];
+      A[k + (size_t)j * ld] *= norm;
+    }
+  }
+  n_ = (int)nv;
+  ld_ = (int)ld;
+  dgemm_("N", "N", &n_, &n_, &n_, &sone, ctx->M, &ncv_, eps->ds->mat[DS_MAT_Q], &ld_, &zero, w, &ncv_);
+  dgemm_("C", "N", &n_, &n_, &n_, &sone, eps->ds->mat[DS_MAT_Z], &ld_, w, &ncv_, &zero, ctx->M, &ncv_);
+  free(w);
+  free(c);
+  PetscCall(BVSetActiveColumns(eps->V, l, nv));
+  PetscCall(BVSetActiveColumns(eps->W, l, nv));
+  return PETSC_SUCCESS;
+}
+
+/* EPSSolve_KrylovSchur_TwoSided (ks-twosided.c:138-241) */
+static PetscErrorCode EPSSolve_KrylovSchur_TwoSided(EPS eps)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  Mat Op, U, Mm;
+  PetscReal norm, norm2, beta = 0.0, betat = 0.0;
+  PetscInt l, nv, nvt, k = 0, dsn, dsk;
+  PetscBool breakdownt = PETSC_FALSE, breakdown = PETSC_FALSE, breakdownl = PETSC_FALSE;
+  double t0;
+
+  PetscCall(EPSGetStartVector(eps, 0, NULL));
+  PetscCall(EPSGetLeftStartVector(eps, 0, NULL));
+  l = 0;
+  memset(ctx->M, 0, sizeof(PetscScalar) * (size_t)eps->ncv * (size_t)eps->ncv);
+  PetscCall(MatCreateSeqDenseView(eps->ncv, eps->ncv, ctx->M, eps->ncv, &Mm));
+  PetscCall(STGetOperator(eps->st, &Op));
+  while (eps->reason == EPS_CONVERGED_ITERATING) {
+    eps->its++;
+    /* nv-step Arnoldi factorizations for Op and Op^T */
+    nv = PetscMin(eps->nconv + eps->mpd, eps->ncv);
+    PetscCall(DSSetDimensions(eps->ds, nv, eps->nconv, eps->nconv + l));
+    PetscCall(TwoSidedExpand(eps, eps->V, Op, DS_MAT_A, eps->nconv + l, &nv, &beta, &breakdown));
+    nvt = nv;
+    PetscCall(DSSetDimensions(eps->ds, nv, eps->nconv, eps->nconv + l));
+    PetscCall(TwoSidedExpand(eps, eps->W, ctx->AT, DS_MAT_B, eps->nconv + l, &nvt, &betat, &breakdownt));
+    nv = PetscMin(nv, nvt);
+    PetscCall(DSSetDimensions(eps->ds, nv, eps->nconv, eps->nconv + l));
+    PetscCall(DSSetState(eps->ds, l == 0 ? DS_STATE_INTERMEDIATE : DS_STATE_RAW));
+    breakdown = (breakdown || breakdownt) ? PETSC_TRUE : PETSC_FALSE;
+
+    /* update M, modify the Rayleigh quotients S and T */
+    PetscCall(BVSetActiveColumns(eps->V, eps->nconv + l, nv));
+    PetscCall(BVSetActiveColumns(eps->W, eps->nconv + l, nv));
+    PetscCall(BVMatProject(eps->V, NULL, eps->W, Mm));
+    PetscCall(TwoSidedRQUpdate1(eps, nv, beta, betat));
+
+    /* solve the projected problem */
+    t0 = SlepcWallTime();
+    PetscCall(DSSolve(eps->ds, eps->eigr, eps->eigi));
+    PetscCall(DSSort(eps->ds, eps->eigr, eps->eigi, NULL, NULL, NULL));
+    PetscCall(DSUpdateExtraRow(eps->ds));
+
+    /* check convergence */
+    PetscCall(BVNormColumn(eps->V, nv, NORM_2, &norm));
+    PetscCall(BVNormColumn(eps->W, nv, NORM_2, &norm2));
+    PetscCall(EPSKrylovConvergence(eps, PETSC_FALSE, eps->nconv, nv - eps->nconv, beta * norm, betat * norm2, 1.0, &k));
+    eps->t_ds += SlepcWallTime() - t0;
+    if (k >= eps->nev) eps->reason = EPS_CONVERGED_TOL;
+    else if (eps->its >= eps->max_it) eps->reason = EPS_DIVERGED_ITS;
+
+    /* update l */
+    if (eps->reason != EPS_CONVERGED_ITERATING || breakdown || k == nv) l = 0;
+    else {
+      l = PetscMax(1, (PetscInt)((nv - k) * ctx->keep));
+      PetscCall(DSGetTruncateSize(eps->ds, k, nv, &l));
+    }
+    if (!ctx->lock && l > 0) { l += k; k = 0; }
+
+    /* V(:,idx) = V*Q(:,idx), W(:,idx) = W*Z(:,idx) */
+    PetscCall(BVSetActiveColumns(eps->V, eps->nconv, nv));
+    PetscCall(BVSetActiveColumns(eps->W, eps->nconv, nv));
+    PetscCall(DSGetMat(eps->ds, DS_MAT_Q, &U));
+    PetscCall(BVMultInPlace(eps->V, U, eps->nconv, k + l));
+    PetscCall(DSRestoreMat(eps->ds, DS_MAT_Q, &U));
+    PetscCall(DSGetMat(eps->ds, DS_MAT_Z, &U));
+    PetscCall(BVMultInPlace(eps->W, U, eps->nconv, k + l));
+    PetscCall(DSRestoreMat(eps->ds, DS_MAT_Z, &U));
+    if (eps->reason == EPS_CONVERGED_ITERATING && !breakdown) {
+      PetscCall(BVCopyColumn(eps->V, nv, k + l));
+      PetscCall(BVCopyColumn(eps->W, nv, k + l));
+    }
+    if (eps->reason == EPS_CONVERGED_ITERATING) {
+      if (breakdown || k == nv) {
+        if (k < eps->nev) {
+          PetscCall(EPSGetStartVector(eps, k, &breakdown));
+          PetscCall(EPSGetLeftStartVector(eps, k, &breakdownl));
+          if (breakdown || breakdownl) eps->reason = EPS_DIVERGED_BREAKDOWN;
+        }
+      } else {
+        PetscCall(DSGetDimensions(eps->ds, &dsn, NULL, &dsk, NULL));
+        PetscCall(DSSetDimensions(eps->ds, dsn, k, dsk));
+        PetscCall(DSTruncate(eps->ds, k + l, PETSC_FALSE));
+      }
+      PetscCall(TwoSidedRQUpdate2(eps, k + l));
+    }
+    eps->nconv = k;
+    if (eps->monitor) PetscCall(eps->monitor(eps, eps->its, k, eps->eigr, eps->eigi, eps->errest, nv, eps->monitorctx));
+  }
+  PetscCall(STRestoreOperator(eps->st, &Op));
+  PetscCall(MatDestroy(&Mm));
+  PetscCall(DSTruncate(eps->ds, eps->nconv, PETSC_TRUE));
+  ctx->phase = KS_PHASE_DONE;
+  return PETSC_SUCCESS;
+}
+
 static PetscErrorCode EPSSolveBegin_KrylovSchur(EPS eps)
 {
   EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
   ctx->its32 = ctx->nconv32 = 0;
   ctx->nmatmult32 = ctx->passes32 = 0;
   ctx->t32 = 0.0;
+  if (eps->twosided) return EPSSolve_KrylovSchur_TwoSided(eps);   /* runs to the end */
   if (ctx->mixed_tol > 0.0 && ctx->mixed_tol > eps->tol && eps->V->prec == SLEPC_AMD_F64) return MixedBegin(eps);
   return KSStart(eps);
 }
@@ -385,6 +636,13 @@ static PetscErrorCode EPSComputeVectors_KrylovSchur(EPS eps)
     PetscCall(BVMultInPlace(eps->V, Z, 0, eps->nconv));
     PetscCall(DSRestoreMat(eps->ds, DS_MAT_X, &Z));
   }
+  if (eps->twosided) {   /* left eigenvectors: W = W * Y (epsdefault.c:141-148) */
+    PetscCall(DSVectors(eps->ds, DS_MAT_Y, NULL, NULL));
+    PetscCall(BVSetActiveColumns(eps->W, 0, eps->nconv));
+    PetscCall(DSGetMat(eps->ds, DS_MAT_Y, &Z));
+    PetscCall(BVMultInPlace(eps->W, Z, 0, eps->nconv));
+    PetscCall(DSRestoreMat(eps->ds, DS_MAT_Y, &Z));
+  }
   return PETSC_SUCCESS;
 }
 
@@ -416,6 +674,9 @@ static PetscErrorCode EPSDestroy_KrylovSchur(EPS eps)
   if (ctx) {
     free(ctx->g);
     ctx->g = NULL;
+    free(ctx->M);
+    ctx->M = NULL;
+    PetscCall(MatDestroy(&ctx->AT));
   }
   return PETSC_SUCCESS;
 }

@@ -70,6 +70,16 @@ PetscErrorCode MatMult(Mat A, Vec x, Vec y)
   return A->ops.mult(A, x, y);
 }
 
+PetscErrorCode MatMultTranspose(Mat A, Vec x, Vec y)
+{
+  PetscCheck(A && x && y, PETSC_ERR_ARG_NULL, "null argument");
+  PetscCheck(A->ops.multtranspose, PETSC_ERR_SUP, "Mat type %s has no MatMultTranspose", A->type);
+  PetscCheck(x->n == A->m && y->n == A->n, PETSC_ERR_ARG_SIZ, "MatMultTranspose sizes: A %dx%d, x %d, y %d", A->m,
+             A->n, x->n, y->n);
+  PetscCheck(x != y, PETSC_ERR_ARG_WRONG, "x and y must be different vectors");
+  return A->ops.multtranspose(A, x, y);
+}
+
 PetscErrorCode MatDestroy(Mat *A)
 {
   if (!A || !*A) return PETSC_SUCCESS;
@@ -157,11 +167,42 @@ static PetscErrorCode MatMult_HIPCSR(Mat A, Vec x, Vec y)
   return PETSC_SUCCESS;
 }
 
+/* y = A^T x with the explicit transpose, formed on the device the first time */
+static PetscErrorCode MatMultTranspose_HIPCSR(Mat A, Vec x, Vec y)
+{
+  Mat_HIPCSR *h = (Mat_HIPCSR *)A->data;
+  PetscCheck(h->csr, PETSC_ERR_ORDER, "MatSeqAIJSetPreallocationCSR must be called first");
+  PetscCheck(x->prec == y->prec && x->prec == h->vprec, PETSC_ERR_SUP,
+             "MatMultTranspose needs vectors in the precision of the matrix values");
+  if (!h->csrT) KSCall(ks_csr_transpose(h->csr, SlepcAMDContext(), &h->csrT));
+  KSCall(ks_spmv(SlepcAMDContext(), h->csrT, x->d, y->d, x->prec == SLEPC_AMD_F64 ? KS_F64 : KS_F32, 0));
+  return PETSC_SUCCESS;
+}
+
+/* A new MATHIPCSR holding A^T, formed on the device.  The two-sided Krylov-Schur
+   runs its left Arnoldi process on it; the reference wraps the operator in
+   MatCreateHermitianTranspose instead (ks-twosided.c:138). */
+PetscErrorCode MatHIPCSRCreateTranspose(Mat A, Mat *AT)
+{
+  Mat_HIPCSR *h, *t;
+  PetscCheck(A && AT && !strcmp(A->type, MATHIPCSR), PETSC_ERR_ARG_WRONG, "not a %s matrix", MATHIPCSR);
+  h = (Mat_HIPCSR *)A->data;
+  PetscCheck(h->csr, PETSC_ERR_ORDER, "matrix has no values");
+  PetscCall(MatCreate(AT));
+  PetscCall(MatSetSizes(*AT, A->n, A->m, A->n, A->m));
+  PetscCall(MatSetType(*AT, MATHIPCSR));
+  t = (Mat_HIPCSR *)(*AT)->data;
+  KSCall(ks_csr_transpose(h->csr, SlepcAMDContext(), &t->csr));
+  t->vprec = h->vprec;
+  return PETSC_SUCCESS;
+}
+
 static PetscErrorCode MatDestroy_HIPCSR(Mat A)
 {
   Mat_HIPCSR *h = (Mat_HIPCSR *)A->data;
   if (h && h->csr) KSCall(ks_csr_destroy(h->csr));
   if (h && h->csr32) KSCall(ks_csr_destroy(h->csr32));
+  if (h && h->csrT) KSCall(ks_csr_destroy(h->csrT));
   free(h);
   A->data = NULL;
   return PETSC_SUCCESS;
@@ -173,6 +214,7 @@ PetscErrorCode MatCreate_HIPCSR(Mat A)
   PetscCall(PetscMallocZero(sizeof(Mat_HIPCSR), &h));
   A->data = h;
   A->ops.mult = MatMult_HIPCSR;
+  A->ops.multtranspose = MatMultTranspose_HIPCSR;
   A->ops.destroy = MatDestroy_HIPCSR;
   return PETSC_SUCCESS;
 }
@@ -187,6 +229,7 @@ static PetscErrorCode MatHIPCSRSetHost(Mat A, const PetscInt i[], const PetscInt
   h = (Mat_HIPCSR *)A->data;
   if (h->csr) { KSCall(ks_csr_destroy(h->csr)); h->csr = NULL; }
   if (h->csr32) { KSCall(ks_csr_destroy(h->csr32)); h->csr32 = NULL; }
+  if (h->csrT) { KSCall(ks_csr_destroy(h->csrT)); h->csrT = NULL; }
   KSCall(ks_csr_create(&h->csr, SlepcAMDContext(), A->m, A->n, (int64_t)i[A->m], i, j, v,
                        vprec == SLEPC_AMD_F64 ? KS_F64 : KS_F32));
   h->vprec = vprec;
@@ -212,6 +255,7 @@ PetscErrorCode MatHIPCSRSetDeviceArrays(Mat A, PetscInt nnz, const PetscInt *d_i
   h = (Mat_HIPCSR *)A->data;
   if (h->csr) { KSCall(ks_csr_destroy(h->csr)); h->csr = NULL; }
   if (h->csr32) { KSCall(ks_csr_destroy(h->csr32)); h->csr32 = NULL; }
+  if (h->csrT) { KSCall(ks_csr_destroy(h->csrT)); h->csrT = NULL; }
   KSCall(ks_csr_create_device(&h->csr, SlepcAMDContext(), A->m, A->n, nnz, d_i, d_j, d_v,
                               vprec == SLEPC_AMD_F64 ? KS_F64 : KS_F32));
   h->vprec = vprec;

@@ -80,6 +80,7 @@ PetscErrorCode VecCreateHIPWithArray(PetscInt n, SlepcAMDPrecision prec, void *d
 /* ---------------------------------------------------------------- Mat */
 typedef struct {
   PetscErrorCode (*mult)(Mat, Vec, Vec);
+  PetscErrorCode (*multtranspose)(Mat, Vec, Vec);
   PetscErrorCode (*destroy)(Mat);
 } MatOps;
 struct _p_Mat {
@@ -97,6 +98,7 @@ typedef struct {         /* MATSEQDENSE, host column-major */
 typedef struct {         /* MATHIPCSR */
   ks_csr csr;
   ks_csr csr32;          /* FP32 copy of FP64 values for FP32 vectors (built on demand) */
+  ks_csr csrT;           /* A^T for MatMultTranspose (built on the device on demand) */
   SlepcAMDPrecision vprec;
 } Mat_HIPCSR;
 /* the FP32-valued copy of a hipcsr matrix, made on first use */
@@ -198,9 +200,11 @@ struct _p_DS {
   PetscInt *perm;
   SlepcSC sc;
   Mat matview[DS_NUM_MAT];
+  PetscScalar *wr2, *wi2;         /* DSNHEPTS: eigenvalues of B */
 };
 PetscErrorCode DSCreate_HEP(DS ds);
 PetscErrorCode DSCreate_NHEP(DS ds);
+PetscErrorCode DSCreate_NHEPTS(DS ds);
 
 /* ---------------------------------------------------------------- ST */
 struct _p_ST {
@@ -226,12 +230,14 @@ struct _p_EPS {
   Mat A;
   ST st;
   BV V;
+  BV W;                      /* left basis of the two-sided variant */
   DS ds;
   PetscInt n, nev, ncv, mpd, max_it, nconv, its, nini, nds;
   PetscReal tol;
   EPSWhich which;
   PetscScalar target;
   EPSExtraction extraction;
+  PetscBool twosided;
   EPSProblemType problem_type;
   PetscBool ishermitian, isgeneralized;
   EPSConvergedReason reason;
@@ -269,10 +275,14 @@ typedef struct {            /* EPS_KRYLOVSCHUR (krylovschur.h:76-105), default v
   /* harmonic extraction: the translation vector g (ncv + 1 entries) */
   PetscBool harmonic;
   PetscScalar *g;
+  /* two-sided variant: A^T and the oblique projection M = W^T V (ncv x ncv) */
+  Mat AT;
+  PetscScalar *M;
 } EPS_KRYLOVSCHUR;
 
 PetscErrorCode EPSKrylovConvergence(EPS eps, PetscBool getall, PetscInt kini, PetscInt nits, PetscReal beta,
-                                    PetscReal corrf, PetscInt *kout);
+                                    PetscReal betat, PetscReal corrf, PetscInt *kout);
+PetscErrorCode EPSGetLeftStartVector(EPS eps, PetscInt i, PetscBool *breakdown);
 PetscErrorCode EPSGetStartVector(EPS eps, PetscInt i, PetscBool *breakdown);
 PetscErrorCode EPSComputeVectors(EPS eps);
 PetscErrorCode EPSMatMult(EPS eps, Mat A, Vec x, Vec y);

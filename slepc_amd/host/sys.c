@@ -82,6 +82,7 @@ PetscErrorCode SlepcInitialize(int device)
   PetscCall(BVRegister(BVHIPKS, BVCreate_HIPKS));
   PetscCall(DSRegister(DSHEP, DSCreate_HEP));
   PetscCall(DSRegister(DSNHEP, DSCreate_NHEP));
+  PetscCall(DSRegister(DSNHEPTS, DSCreate_NHEPTS));
   PetscCall(EPSRegister(EPSKRYLOVSCHUR, EPSCreate_KrylovSchur));
   g_initialized = PETSC_TRUE;
   return PETSC_SUCCESS;

@@ -154,6 +154,63 @@ def random_sparse_symmetric(n: int, density: float, seed: int):
     return dense_to_csr(M)
 
 
+def markov_model(m: int):
+    """Random walk on a triangular grid with m points a side (N = m(m+1)/2), the
+    transpose of a stochastic matrix: the test matrix of the reference's
+    src/eps/tutorials/ex5.c:139-170 (MatMarkovModel), built row by row the same way."""
+    cst = 0.5 / (m - 1)
+    N = m * (m + 1) // 2
+    rows = [dict() for _ in range(N)]
+    ix = 0
+    for i in range(1, m + 1):
+        jmax = m - i + 1
+        for j in range(1, jmax + 1):
+            ix += 1
+            r = rows[ix - 1]
+            if j != jmax:
+                pd = cst * (i + j - 1)
+                r[ix] = 2 * pd if i == 1 else pd                     # north
+                r[ix + jmax - 1] = 2 * pd if j == 1 else pd          # east
+            pu = 0.5 - cst * (i + j - 3)
+            if j > 1:
+                r[ix - 2] = pu                                       # south
+            if i > 1:
+                r[ix - jmax - 2] = pu                                # west
+    return _rows_to_csr(rows)
+
+
+def brusselator(n: int, alpha: float = 2.0, beta: float = 5.45, delta1: float = 0.008, delta2: float = 0.004,
+                L: float = 0.51302):
+    """The Brusselator wave model of src/eps/tutorials/ex9.c:20-35, 92-121 as an
+    explicit 2n x 2n matrix [tau1 T + (beta-1) I, alpha^2 I; -beta I, tau2 T - alpha^2 I]
+    with T = tridiag(1, -2, 1), h = 1/(n+1), tau_i = delta_i / (h L)^2."""
+    h = 1.0 / (n + 1)
+    tau1 = delta1 / ((h * L) ** 2)
+    tau2 = delta2 / ((h * L) ** 2)
+    rows = [dict() for _ in range(2 * n)]
+    for i in range(n):
+        r1, r2 = rows[i], rows[n + i]
+        r1[i] = -2.0 * tau1 + beta - 1.0
+        r2[n + i] = -2.0 * tau2 - alpha * alpha
+        if i > 0:
+            r1[i - 1] = tau1
+            r2[n + i - 1] = tau2
+        if i < n - 1:
+            r1[i + 1] = tau1
+            r2[n + i + 1] = tau2
+        r1[n + i] = alpha * alpha
+        r2[i] = -beta
+    return _rows_to_csr(rows)
+
+
+def _rows_to_csr(rows):
+    rowptr = np.zeros(len(rows) + 1, dtype=np.int32)
+    rowptr[1:] = np.cumsum([len(r) for r in rows])
+    col = np.array([c for r in rows for c in sorted(r)], dtype=np.int32)
+    val = np.array([r[c] for r in rows for c in sorted(r)], dtype=np.float64)
+    return rowptr, col, val
+
+
 def random_sparse_nonsymmetric(n: int, density: float, seed: int):
     rng = np.random.default_rng(seed)
     M = (rng.random((n, n)) < density) * rng.standard_normal((n, n))
