@@ -127,6 +127,16 @@ int ks_csr_convert(ks_csr A, ks_ctx ctx, ks_prec to, ks_csr *out);
    two-sided Krylov-Schur (src/eps/impls/krylov/krylovschur/ks-twosided.c:138) by an
    explicit matrix, so products with A^T use the same SpMV kernel. */
 int ks_csr_transpose(ks_csr A, ks_ctx ctx, ks_csr *out);
+/* The inner solve of shift-and-invert: (A - sigma I) x = b by preconditioned
+   conjugate gradients (KSPCG's recurrences, preconditioned residual norm,
+   rnorm <= max(rtol |B b|, abstol), at most maxits iterations) with
+   B = diag(A - sigma I)^-1 from ks_pcg_jacobi (jacobi = 0: B = I).  Replaces
+   STApply_Sinvert's KSPSolve (src/sys/classes/st/impls/sinvert/sinvert.c:16-30).
+   d_work holds 4n + 8 doubles; reason > 0 converged, -3 iteration limit,
+   -8 p^T (A - sigma I) p <= 0 (not positive definite). */
+int ks_pcg_jacobi(ks_ctx ctx, ks_csr A, double sigma, int jacobi, double *d_dinv);
+int ks_pcg_solve(ks_ctx ctx, ks_csr A, double sigma, const double *d_dinv, const double *d_b, double *d_x,
+                 double rtol, double abstol, int maxits, double *d_work, int *its, double *rnorm, int *reason);
 /* nrows, nnz, max row length and the kernel variant chosen (0 stream, 1 vector) */
 int ks_csr_info(ks_csr A, int *nrows, int64_t *nnz, int *max_row, int *variant, int *nblocks);
 /* y = A x.  x, y device vectors in xprec.  When gate != 0 the product is

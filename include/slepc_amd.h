@@ -242,7 +242,31 @@ PetscErrorCode DSSetSortCriterion(DS ds, int which, PetscScalar target);
 PetscErrorCode DSDestroy(DS *ds);
 
 /* ------------------------------------------------------------ ST */
+typedef const char *STType;
+typedef const char *KSPType;
+typedef const char *PCType;
+typedef struct _p_KSP *KSP;
+typedef struct _p_PC *PC;
+#define STSHIFT   "shift"
+#define STSINVERT "sinvert"
+#define KSPCG     "cg"
+#define PCJACOBI  "jacobi"
+#define PCNONE    "none"
 PetscErrorCode STCreate(ST *st);
+PetscErrorCode STSetType(ST st, STType type);
+PetscErrorCode STGetType(ST st, STType *type);
+PetscErrorCode STSetShift(ST st, PetscScalar shift);
+PetscErrorCode STGetShift(ST st, PetscScalar *shift);
+PetscErrorCode STSetDefaultShift(ST st, PetscScalar defaultshift);
+/* STSINVERT's inner solver: conjugate gradients on the device, Jacobi or no
+   preconditioner (the reference's KSP/PC objects, reduced to what runs here) */
+PetscErrorCode STGetKSP(ST st, KSP *ksp);
+PetscErrorCode KSPSetType(KSP ksp, KSPType type);
+PetscErrorCode KSPGetPC(KSP ksp, PC *pc);
+PetscErrorCode PCSetType(PC pc, PCType type);
+PetscErrorCode KSPSetTolerances(KSP ksp, PetscReal rtol, PetscReal abstol, PetscReal dtol, PetscInt maxits);
+PetscErrorCode KSPGetTotalIterations(KSP ksp, PetscInt *its);
+PetscErrorCode STBackTransform(ST st, PetscInt n, PetscScalar *eigr, PetscScalar *eigi);
 PetscErrorCode STSetMatrices(ST st, PetscInt n, Mat A[]);
 PetscErrorCode STGetOperator(ST st, Mat *Op);
 PetscErrorCode STRestoreOperator(ST st, Mat *Op);

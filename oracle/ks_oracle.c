@@ -117,11 +117,16 @@ int or_spmv(int n, const int *rowptr, const int *colidx, const double *val,
 }
 
 /* ------------------------------------------------------------------ sorting (slepcsc.c) */
-typedef struct { int which; double target; } or_sc;
+typedef struct { int which; double target; int map; double sigma; } or_sc;   /* map: compare sigma + 1/theta */
 
 /* SlepcCompare* (slepcsc.c:152-190): result < 0 means a goes first */
 static int sc_compare(const or_sc *sc, double ar, double ai, double br, double bi)
 {
+  if (sc->map) {                                                   /* SlepcMap_ST (slepcsc.c:118-126) */
+    double na = ar * ar + ai * ai, nb = br * br + bi * bi;
+    if (!ai) ar = 1.0 / ar + sc->sigma; else { ar = ar / na + sc->sigma; ai = -ai / na; }
+    if (!bi) br = 1.0 / br + sc->sigma; else { br = br / nb + sc->sigma; bi = -bi / nb; }
+  }
   double a = 0, b = 0;
   switch (sc->which) {
   case OR_LARGEST_MAGNITUDE:  a = hypot(ar, ai); b = hypot(br, bi); return a < b ? 1 : (a > b ? -1 : 0);
@@ -1049,6 +1054,10 @@ typedef struct {
   int n, nev, ncv, mpd, max_it, which, hermitian, lock, nconv, its, reason;
   int harmonic, dsherm;           /* dsherm: Lanczos + DSHEP (Hermitian and Ritz extraction) */
   int twosided;
+  int sinvert, pc_jacobi, ksp_maxits;   /* shift-and-invert: (A - sigma I)^-1 by PCG */
+  double sigma, ksp_rtol, *dinv, *kwork;
+  long ksp_its;
+  int ksp_failed;
   double tol, target, keep;
   const int *rowptr, *colidx;
   const double *val;
@@ -1063,9 +1072,39 @@ typedef struct {
   double t_ds;
 } oeps;
 
+/* The preconditioned conjugate gradient method in PETSc's KSPCG form (Hestenes-Stiefel
+   recurrences; convergence on the preconditioned residual norm |B r| <= max(rtol |B b|,
+   abstol), abstol 1e-50) with B = diag(A - sigma I)^-1 (PCJACOBI: zero entries -> 1) or
+   B = I.  PETSc is not part of the reference tree: this restates its published
+   algorithm; the reference calls it from STApply_Sinvert (sinvert.c:16-30). */
+static int pcg_solve(oeps *eps, const double *b, double *x)
+{
+  int n = eps->n;
+  double *r = eps->kwork, *z = r + n, *p = z + n, *q = p + n, rz = 0.0, rzold = 0.0, zz = 0.0;
+  for (int i = 0; i < n; i++) { x[i] = 0.0; r[i] = b[i]; z[i] = eps->dinv[i] * r[i]; rz += z[i] * r[i]; zz += z[i] * z[i]; }
+  double rnorm0 = sqrt(zz), ttol = fmax(eps->ksp_rtol * rnorm0, 1e-50);
+  if (rnorm0 <= ttol) return 0;
+  for (int it = 0; it < eps->ksp_maxits; it++) {
+    double beta = it ? rz / rzold : 0.0, pq = 0.0, a;
+    for (int i = 0; i < n; i++) p[i] = it ? z[i] + beta * p[i] : z[i];
+    or_spmv(n, eps->rowptr, eps->colidx, eps->val, p, q);
+    for (int i = 0; i < n; i++) { q[i] -= eps->sigma * p[i]; pq += p[i] * q[i]; }
+    a = rz / pq; rzold = rz; rz = 0.0; zz = 0.0;
+    for (int i = 0; i < n; i++) {
+      x[i] += a * p[i]; r[i] -= a * q[i]; z[i] = eps->dinv[i] * r[i]; rz += z[i] * r[i]; zz += z[i] * z[i];
+    }
+    eps->ksp_its++;
+    if (!(pq > 0.0)) OR_ERR(-91, "KSP did not converge: p^T (A - sigma I) p <= 0");
+    if (sqrt(zz) <= ttol) return 0;
+  }
+  OR_ERR(-91, "KSP did not converge in %d iterations", eps->ksp_maxits);
+}
+
 static void op_mult(oeps *eps, const double *x, double *y)
 {
-  or_spmv(eps->n, eps->rowptr, eps->colidx, eps->val, x, y);
+  if (eps->sinvert) {
+    if (pcg_solve(eps, x, y)) eps->ksp_failed = 1;
+  } else or_spmv(eps->n, eps->rowptr, eps->colidx, eps->val, x, y);
   eps->nmatmult++;
 }
 
@@ -1085,6 +1124,7 @@ static int mat_krylov_on(oeps *eps, obv *bv, int transpose, double *H, int k, in
   for (int j = k; j < *m; j++) {
     if (transpose) op_mult_transpose(eps, COL(bv, j), COL(bv, j + 1));
     else op_mult(eps, COL(bv, j), COL(bv, j + 1));               /* BVMatMultColumn */
+    if (eps->ksp_failed) OR_ERR(-91, "the inner solve of shift-and-invert did not converge");
     if (j == eps->n - 1) {                                         /* BV_OrthogonalizeColumn_Safe */
       int save = bv->refine;
       bv->refine = OR_REFINE_NEVER;
@@ -1403,20 +1443,27 @@ static int ks_solve(oeps *eps, int have_initial, int restart_cap)
 }
 
 /* EPSComputeResidualNorm_Private (epssolve.c:666-718), standard problem */
+/* y = A x for residuals (the matrix itself, also under shift-and-invert) */
+static void a_mult(oeps *eps, const double *x, double *y)
+{
+  or_spmv(eps->n, eps->rowptr, eps->colidx, eps->val, x, y);
+  eps->nmatmult++;
+}
+
 static double residual_norm(oeps *eps, double kr, double ki, const double *xr, const double *xi,
                             double *u, double *w)
 {
   int n = eps->n, one = 1;
   if (ki == 0 || fabs(ki) < fabs(kr * MACH_EPS)) {
-    op_mult(eps, xr, u);
+    a_mult(eps, xr, u);
     if (fabs(kr) > MACH_EPS) for (int i = 0; i < n; i++) u[i] -= kr * xr[i];
     return dnrm2_(&n, u, &one);
   } else {
     double nr, ni;
-    op_mult(eps, xr, u);
+    a_mult(eps, xr, u);
     if (hypot(kr, ki) > MACH_EPS) for (int i = 0; i < n; i++) u[i] += -kr * xr[i] + ki * xi[i];
     nr = dnrm2_(&n, u, &one);
-    op_mult(eps, xi, u);
+    a_mult(eps, xi, u);
     if (hypot(kr, ki) > MACH_EPS) for (int i = 0; i < n; i++) u[i] += -kr * xi[i] - ki * xr[i];
     ni = dnrm2_(&n, u, &one);
     (void)w;
@@ -1466,7 +1513,9 @@ int or_eps_solve_twosided(int n, const int *rowptr, const int *colidx, const dou
   if (opts->nthreads > 0) omp_set_num_threads(opts->nthreads);
 
   eps.n = n; eps.rowptr = rowptr; eps.colidx = colidx; eps.val = val;
-  eps.nev = opts->nev; eps.which = opts->which ? opts->which : OR_LARGEST_MAGNITUDE;
+  eps.sinvert = opts->st_sinvert != 0;
+  eps.nev = opts->nev;
+  eps.which = opts->which ? opts->which : (eps.sinvert ? OR_TARGET_MAGNITUDE : OR_LARGEST_MAGNITUDE);
   eps.target = opts->target; eps.hermitian = opts->hermitian;
   eps.harmonic = opts->extraction == 1;
   if (opts->extraction != 0 && opts->extraction != 1) OR_ERR(-56, "Unsupported extraction type");
@@ -1478,6 +1527,15 @@ int or_eps_solve_twosided(int n, const int *rowptr, const int *colidx, const dou
   eps.keep = opts->keep > 0 ? opts->keep : 0.5;
   eps.lock = opts->lock;
   eps.sc.which = eps.which; eps.sc.target = eps.target;
+  if (eps.sinvert) {                                               /* STSINVERT, shift = target */
+    if (eps.which != OR_TARGET_MAGNITUDE && eps.which != OR_TARGET_REAL)
+      OR_ERR(-90, "Shift-and-invert requires a target 'which'");
+    if (opts->extraction != 0 || opts->twosided) OR_ERR(-56, "shift-and-invert runs with Ritz extraction, one-sided");
+    eps.sigma = opts->target;
+    eps.ksp_rtol = opts->ksp_rtol > 0 ? opts->ksp_rtol : 1e-8;
+    eps.ksp_maxits = 10000;
+    eps.pc_jacobi = !opts->pc_none;
+  }
 
   /* EPSSetUp (epssetup.c:344-345) and EPSSetDimensions_Default (epssetup.c:648-671) */
   if (eps.nev > n) eps.nev = n;
@@ -1516,6 +1574,17 @@ int or_eps_solve_twosided(int n, const int *rowptr, const int *colidx, const dou
     if ((e = transpose_csr(n, rowptr, colidx, val, &eps.trowptr, &eps.tcolidx, &eps.tval))) goto done;
   }
   eps.ds.compact = eps.dsherm; eps.ds.extrarow = 1; eps.ds.sc = eps.sc;
+  if (eps.sinvert) {
+    eps.ds.sc.map = 1; eps.ds.sc.sigma = eps.sigma;
+    eps.dinv = (double*)malloc(sizeof(double) * (size_t)n);
+    eps.kwork = (double*)malloc(sizeof(double) * 4 * (size_t)n);
+    if (!eps.dinv || !eps.kwork) { e = -55; goto done; }
+    for (int i = 0; i < n; i++) {                                  /* PCJACOBI on A - sigma I */
+      double d = 0.0;
+      if (eps.pc_jacobi) { for (int q = rowptr[i]; q < rowptr[i + 1]; q++) if (colidx[q] == i) d += val[q]; d -= eps.sigma; }
+      eps.dinv[i] = d == 0.0 ? 1.0 : 1.0 / d;
+    }
+  }
 
   /* deflation space: BVInsertConstraints (orthonormalized, bvfunc.c:411) */
   if (eps.V.nc) {
@@ -1568,6 +1637,13 @@ int or_eps_solve_twosided(int n, const int *rowptr, const int *colidx, const dou
         if (pair) bv_scalecol(&eps.V, ++j, 1.0 / nr);
       }
   }
+  /* EPSComputeValues (epssolve.c:27-59): STBackTransform_Sinvert, lambda = sigma + 1/theta */
+  if (eps.sinvert)
+    for (int j = 0; j < eps.nconv; j++) {
+      double re = eps.eigr[j], im = eps.eigi[j], nrm = re * re + im * im;
+      if (!im) eps.eigr[j] = 1.0 / re + eps.sigma;
+      else { eps.eigr[j] = re / nrm + eps.sigma; eps.eigi[j] = -im / nrm; }
+    }
   /* reorder conjugate pairs (epssolve.c:161-175) */
   for (int i = 0; i < eps.nconv - 1; i++) {
     if (eps.eigi[i] != 0) {
@@ -1617,10 +1693,12 @@ int or_eps_solve_twosided(int n, const int *rowptr, const int *colidx, const dou
     free(u); (void)zero;
   }
   if (info) {
+    info->ksp_its = eps.ksp_its;
     info->nconv = eps.nconv; info->its = eps.its; info->reason = eps.reason;
     info->nmatmult = eps.nmatmult; info->gs_passes = eps.V.passes; info->t_ds = eps.t_ds;
   }
 done:
+  free(eps.dinv); free(eps.kwork);
   if (eps.twosided) { free(eps.W.V); free(eps.W.buffer); free(eps.M); free(eps.trowptr); free(eps.tcolidx); free(eps.tval); }
   free(eps.V.V); free(eps.V.buffer); free(eps.eigr); free(eps.eigi); free(eps.errest); free(eps.perm);
   if (eps.ds.A) ds_free(&eps.ds);

@@ -42,6 +42,9 @@ typedef struct {
   int    restart_cap;            /* > 0: stop after this many restarts (bounded timing sample) */
   int    extraction;             /* 0 EPS_RITZ, 1 EPS_HARMONIC (translation by target) */
   int    twosided;               /* 1: two-sided variant (left eigenvectors too) */
+  int    st_sinvert;             /* 1: STSINVERT with shift = target, inner PCG */
+  int    pc_none;                /* 1: PCG without the Jacobi preconditioner */
+  double ksp_rtol;               /* <= 0 means 1e-8 */
 } or_opts;
 
 typedef struct {
@@ -50,6 +53,7 @@ typedef struct {
   long   gs_passes;              /* executed CGS passes (DGKS included) */
   double t_solve;                /* wall seconds of the restart loop + vectors */
   double t_ds;                   /* wall seconds spent in DS (projected problem) */
+  long   ksp_its;                /* inner PCG iterations (shift-and-invert) */
 } or_info;
 
 /* y = A x for CSR (PETSc MatMult_SeqAIJ semantics). */

@@ -26,13 +26,14 @@ class Opts(C.Structure):
                 ("tol", C.c_double), ("which", C.c_int), ("target", C.c_double),
                 ("hermitian", C.c_int), ("keep", C.c_double), ("lock", C.c_int),
                 ("eta", C.c_double), ("refine", C.c_int), ("nthreads", C.c_int),
-                ("restart_cap", C.c_int), ("extraction", C.c_int), ("twosided", C.c_int)]
+                ("restart_cap", C.c_int), ("extraction", C.c_int), ("twosided", C.c_int),
+                ("st_sinvert", C.c_int), ("pc_none", C.c_int), ("ksp_rtol", C.c_double)]
 
 
 class Info(C.Structure):
     _fields_ = [("nconv", C.c_int), ("its", C.c_int), ("reason", C.c_int),
                 ("nmatmult", C.c_long), ("gs_passes", C.c_long),
-                ("t_solve", C.c_double), ("t_ds", C.c_double)]
+                ("t_solve", C.c_double), ("t_ds", C.c_double), ("ksp_its", C.c_long)]
 
 
 _lib = None
@@ -166,12 +167,14 @@ def eps_solve(csr, nev: int, ncv: int = 0, mpd: int = 0, tol: float = 1e-8,
               which: int = LARGEST_MAGNITUDE, hermitian: bool = True, target: float = 0.0,
               v0=None, deflation=None, max_it: int = 0, keep: float = 0.5, lock: bool = True,
               eta: float = 0.7071, refine: int = REFINE_IFNEEDED, nthreads: int = 0,
-              restart_cap: int = 0, want_vectors: bool = True, extraction: int = 0, twosided: bool = False):
+              restart_cap: int = 0, want_vectors: bool = True, extraction: int = 0, twosided: bool = False,
+              sinvert: bool = False, pc_none: bool = False, ksp_rtol: float = 0.0):
     rowptr, colidx, val = _csr_args(csr)
     n = len(rowptr) - 1
     o = Opts(nev=nev, ncv=ncv, mpd=mpd, max_it=max_it, tol=tol, which=which, target=target,
              hermitian=int(hermitian), keep=keep, lock=int(lock), eta=eta, refine=refine,
-             nthreads=nthreads, restart_cap=restart_cap, extraction=extraction, twosided=int(twosided))
+             nthreads=nthreads, restart_cap=restart_cap, extraction=extraction, twosided=int(twosided),
+             st_sinvert=int(sinvert), pc_none=int(pc_none), ksp_rtol=ksp_rtol)
     ncv_eff = ncv if ncv > 0 else max(2 * nev, nev + 15)
     ncv_eff = min(ncv_eff, n) if ncv_eff > 0 else n
     size = max(ncv_eff, nev) + 2
@@ -197,7 +200,7 @@ def eps_solve(csr, nev: int, ncv: int = 0, mpd: int = 0, tol: float = 1e-8,
     out = dict(nconv=k, its=info.its, reason=info.reason, eigr=eigr[:k].copy(),
                eigi=eigi[:k].copy(), errest=errest[:k].copy(), err=err[:k].copy(),
                X=None if X is None else X[:, :k].copy(), nmatmult=info.nmatmult,
-               gs_passes=info.gs_passes, t_solve=info.t_solve, t_ds=info.t_ds)
+               gs_passes=info.gs_passes, t_solve=info.t_solve, t_ds=info.t_ds, ksp_its=info.ksp_its)
     if Y is not None:
         out["Y"] = Y[:, :k].copy()
     return out

@@ -303,6 +303,31 @@ class EPS:
     def setExtraction(self, extr: int):
         _chk(lib().EPSSetExtraction(self.h, C.c_int(extr)), "EPSSetExtraction")
 
+    def setST(self, st_type: str = "shift", shift=None, ksp_rtol=None, ksp_maxits=None, pc_type=None):
+        """STSetType (+ STSetShift) and the inner solver of shift-and-invert
+        (STGetKSP -> KSPSetTolerances, KSPGetPC -> PCSetType)."""
+        st = C.c_void_p()
+        _chk(lib().EPSGetST(self.h, C.byref(st)), "EPSGetST")
+        _chk(lib().STSetType(st, st_type.encode()), "STSetType")
+        if shift is not None:
+            _chk(lib().STSetShift(st, C.c_double(shift)), "STSetShift")
+        if st_type == "sinvert":
+            ksp, pc = C.c_void_p(), C.c_void_p()
+            _chk(lib().STGetKSP(st, C.byref(ksp)), "STGetKSP")
+            _chk(lib().KSPSetTolerances(ksp, C.c_double(-1.0 if ksp_rtol is None else ksp_rtol), C.c_double(-1.0),
+                                        C.c_double(-1.0), C.c_int(0 if ksp_maxits is None else ksp_maxits)),
+                 "KSPSetTolerances")
+            if pc_type is not None:
+                _chk(lib().KSPGetPC(ksp, C.byref(pc)), "KSPGetPC")
+                _chk(lib().PCSetType(pc, pc_type.encode()), "PCSetType")
+
+    def kspIterations(self) -> int:
+        st, ksp, its = C.c_void_p(), C.c_void_p(), C.c_int()
+        _chk(lib().EPSGetST(self.h, C.byref(st)), "EPSGetST")
+        _chk(lib().STGetKSP(st, C.byref(ksp)), "STGetKSP")
+        _chk(lib().KSPGetTotalIterations(ksp, C.byref(its)), "KSPGetTotalIterations")
+        return its.value
+
     def setTwoSided(self, flg: bool = True):
         _chk(lib().EPSSetTwoSided(self.h, C.c_int(int(flg))), "EPSSetTwoSided")
 
@@ -443,7 +468,7 @@ class EPS:
 
 def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=True, v0=None, max_it=-1,
           prec=F64, deflation=None, lock=True, keep=0.5, want_vectors=True, mixed_tol=0.0, step_graphs=False,
-          target=None, extraction=0, twosided=False):
+          target=None, extraction=0, twosided=False, st=None):
     """One-call convenience wrapper: returns a dict like the oracle's eps_solve.
     FP32 values in csr[2] keep the operator in FP32; ``prec`` is the basis precision."""
     A = Mat(csr, prec=F32 if np.asarray(csr[2]).dtype == np.float32 else F64)
@@ -459,6 +484,8 @@ def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=Tru
         eps.setExtraction(extraction)
     if twosided:
         eps.setTwoSided(True)
+    if st is not None:            # dict(st_type="sinvert", shift=..., ksp_rtol=..., pc_type="jacobi")
+        eps.setST(**st)
     eps.setKrylovSchurLocking(lock)
     eps.setKrylovSchurRestart(keep)
     if mixed_tol > 0:
@@ -489,6 +516,8 @@ def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=Tru
                 Y[:, i] = eps.getLeftEigenvector(i)
             out["Y"] = Y
     out.update(eps.stats())
+    if st is not None and st.get("st_type") == "sinvert":
+        out["ksp_its"] = eps.kspIterations()
     if mixed_tol > 0:
         out.update(eps.mixedStats())
     eps.destroy()

@@ -92,10 +92,12 @@ PetscErrorCode EPSSetWhichEigenpairs(EPS eps, EPSWhich which)
   return PETSC_SUCCESS;
 }
 
+/* EPSSetTarget (epsbasic.c:378-386): also the ST's default shift */
 PetscErrorCode EPSSetTarget(EPS eps, PetscScalar target)
 {
   if (eps->target != target) eps->state = EPS_STATE_INITIAL;
   eps->target = target;
+  PetscCall(STSetDefaultShift(eps->st, target));
   return PETSC_SUCCESS;
 }
 
@@ -270,7 +272,17 @@ PetscErrorCode EPSSetUp(EPS eps)
   /* EPSSetUpSort_Default (epssetup.c:223-240): same criterion for EPS and DS */
   eps->sc.which = eps->which;
   eps->sc.target = eps->target;
+  eps->sc.map = PETSC_FALSE;
   PetscCall(DSSetSortCriterion(eps->ds, eps->which, eps->target));
+  if (STIsSinvert(eps->st)) {
+    /* EPSCheckCompatibleST (epssetup.c:116-120) and the DS criterion on sigma + 1/theta */
+    PetscCheck(eps->which == EPS_TARGET_MAGNITUDE || eps->which == EPS_TARGET_REAL, PETSC_ERR_USER_INPUT,
+               "Shift-and-invert requires a target 'which' (see EPSSetWhichEigenpairs), for instance -st_type "
+               "sinvert -eps_target 0 -eps_target_magnitude");
+    PetscCall(STSetUp(eps->st));
+    eps->ds->sc.map = PETSC_TRUE;
+    eps->ds->sc.sigma = eps->st->sigma;
+  } else eps->ds->sc.map = PETSC_FALSE;
   /* deflation and initial vectors */
   if (eps->nds < 0) {
     k = -eps->nds;
@@ -358,6 +370,8 @@ PetscErrorCode EPSSolveEnd(EPS eps)
   PetscCheck(eps->reason, PETSC_ERR_PLIB, "Internal error, solver returned without setting converged reason");
   eps->state = EPS_STATE_SOLVED;
   PetscCall(BVSetActiveColumns(eps->V, 0, eps->nconv));
+  /* EPSComputeValues (epssolve.c:27-59): back-transform theta to lambda */
+  PetscCall(STBackTransform(eps->st, eps->nconv, eps->eigr, eps->eigi));
   /* reorder conjugate eigenvalues (positive imaginary first) */
   for (i = 0; i < eps->nconv - 1; i++) {
     if (eps->eigi[i] != 0) {

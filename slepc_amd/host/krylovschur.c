@@ -87,7 +87,13 @@ static PetscErrorCode EPSSetUp_KrylovSchur(EPS eps)
   PetscCall(EPSSetDimensions_Default(eps, eps->nev, &eps->ncv, &eps->mpd));
   PetscCheck(eps->ncv <= eps->nev + eps->mpd, PETSC_ERR_USER_INPUT, "The value of ncv must not be larger than nev+mpd");
   if (eps->max_it == PETSC_DETERMINE) eps->max_it = PetscMax(100, 2 * eps->n / eps->ncv);
-  if (!eps->which) eps->which = EPS_LARGEST_MAGNITUDE;   /* EPSSetWhichEigenpairs_Default with STSHIFT */
+  /* EPSSetWhichEigenpairs_Default (epsdefault.c:210-219) */
+  if (!eps->which) eps->which = STIsSinvert(eps->st) ? EPS_TARGET_MAGNITUDE : EPS_LARGEST_MAGNITUDE;
+  if (STIsSinvert(eps->st)) {
+    PetscCheck(eps->extraction == EPS_RITZ && !eps->twosided && ctx->mixed_tol <= 0.0, PETSC_ERR_SUP,
+               "shift-and-invert runs with Ritz extraction, one-sided, in FP64");
+    PetscCheck(!eps->V || eps->V->prec == SLEPC_AMD_F64, PETSC_ERR_SUP, "shift-and-invert needs an FP64 basis");
+  }
   PetscCheck(ctx->lock || eps->mpd >= eps->ncv, PETSC_ERR_SUP, "Should not use mpd parameter in non-locking variant");
   PetscCheck(eps->extraction == EPS_RITZ || eps->extraction == EPS_HARMONIC, PETSC_ERR_SUP, "Unsupported extraction type");
   PetscCheck(!eps->twosided || !eps->ishermitian, PETSC_ERR_SUP, "Two-sided methods are not intended for symmetric problems");

@@ -62,6 +62,8 @@ PetscErrorCode SlepcCheckInitialized(void);
 typedef struct {
   int which;
   PetscScalar target;
+  PetscBool map;             /* compare sigma + 1/theta (SlepcMap_ST of a shift-and-invert ST) */
+  PetscScalar sigma;
 } SlepcSC;
 int SlepcSCCompare(const SlepcSC *sc, PetscScalar ar, PetscScalar ai, PetscScalar br, PetscScalar bi);
 void SlepcSortEigenvalues(const SlepcSC *sc, PetscInt n, const PetscScalar *eigr, const PetscScalar *eigi,
@@ -207,10 +209,27 @@ PetscErrorCode DSCreate_NHEP(DS ds);
 PetscErrorCode DSCreate_NHEPTS(DS ds);
 
 /* ---------------------------------------------------------------- ST */
-struct _p_ST {
-  Mat A;
-  PetscScalar sigma;
+struct _p_PC {
+  char type[SLEPC_NAME_LEN];
 };
+struct _p_KSP {              /* the inner solver of STSINVERT (device PCG) */
+  char type[SLEPC_NAME_LEN];
+  struct _p_PC pc;
+  PetscReal rtol, abstol;
+  PetscInt maxits, n;
+  double *d_dinv, *d_work;   /* Jacobi diagonal, r z p q + scalars */
+  int64_t its_total, nsolves;
+};
+struct _p_ST {
+  char type[SLEPC_NAME_LEN];
+  Mat A;
+  PetscScalar sigma, defsigma;
+  PetscBool sigma_set, setup;
+  Mat P;                     /* STSINVERT: the operator (A - sigma I)^-1 */
+  KSP ksp;
+};
+PetscErrorCode STSetUp(ST st);
+PetscBool STIsSinvert(ST st);
 
 /* ---------------------------------------------------------------- EPS */
 typedef struct {

@@ -126,6 +126,13 @@ void *SlepcAMDGetDefaultContext(void) { return (void *)g_ctx; }
 int SlepcSCCompare(const SlepcSC *sc, PetscScalar ar, PetscScalar ai, PetscScalar br, PetscScalar bi)
 {
   PetscReal a, b;
+  if (sc->map) {   /* SlepcMap_ST (slepcsc.c:118-126): compare the back-transformed values */
+    PetscScalar na = ar * ar + ai * ai, nb = br * br + bi * bi;
+    if (!ai) ar = 1.0 / ar + sc->sigma;
+    else { ar = ar / na + sc->sigma; ai = -ai / na; }
+    if (!bi) br = 1.0 / br + sc->sigma;
+    else { br = br / nb + sc->sigma; bi = -bi / nb; }
+  }
   switch (sc->which) {
   case EPS_LARGEST_MAGNITUDE:  a = SlepcAbsEigenvalue(ar, ai); b = SlepcAbsEigenvalue(br, bi); return a < b ? 1 : (a > b ? -1 : 0);
   case EPS_SMALLEST_MAGNITUDE: a = SlepcAbsEigenvalue(ar, ai); b = SlepcAbsEigenvalue(br, bi); return a > b ? 1 : (a < b ? -1 : 0);

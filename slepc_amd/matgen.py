@@ -211,6 +211,23 @@ def _rows_to_csr(rows):
     return rowptr, col, val
 
 
+def grid_graph_laplacian(n: int, m: int | None = None):
+    """Laplacian L(G) of the n x m grid graph: the degree on the diagonal, -1 per
+    edge (src/eps/tutorials/ex11.c:40-60); singular, the constant vector spans
+    its null space."""
+    m = n if m is None else m
+    rows = [dict() for _ in range(n * m)]
+    for II in range(n * m):
+        i, j = divmod(II, n)
+        w = 0.0
+        for cond, c in ((i > 0, II - n), (i < m - 1, II + n), (j > 0, II - 1), (j < n - 1, II + 1)):
+            if cond:
+                rows[II][c] = -1.0
+                w += 1.0
+        rows[II][II] = w
+    return _rows_to_csr(rows)
+
+
 def random_sparse_nonsymmetric(n: int, density: float, seed: int):
     rng = np.random.default_rng(seed)
     M = (rng.random((n, n)) < density) * rng.standard_normal((n, n))

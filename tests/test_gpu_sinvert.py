@@ -1,0 +1,70 @@
+"""Shift-and-invert on the MI355X (STSINVERT; the inner solves by the device
+PCG of ks_pcg_solve, through STGetKSP/KSPSetTolerances/PCSetType): the
+reference's ex11 golden with -st_type sinvert -st_ksp_type cg -st_pc_type
+jacobi, the device against the oracle (restarts, inner iterations,
+eigenvalues), closed forms near a shift at device size, and the error paths
+(a non-target 'which', an indefinite shifted matrix for CG -- forced from the
+host: every kernel exits normally and the host reports the failed solve)."""
+import numpy as np
+import pytest
+
+from slepc_amd import matgen as M
+
+pytestmark = pytest.mark.gpu
+
+EX11 = "0.09789, 0.09789, 0.19577, 0.38197"
+
+
+@pytest.fixture(scope="module")
+def E():
+    from slepc_amd import eps as E
+    E.initialize(0)
+    return E
+
+
+@pytest.mark.parametrize("pc", ["jacobi", "none"])
+def test_ex11_sinvert_golden_vs_oracle(E, oracle, pc):
+    csr = M.grid_graph_laplacian(10)
+    ones = np.ones((100, 1))
+    r = E.solve(csr, nev=4, ncv=20, which=E.EPS.TARGET_MAGNITUDE, target=0.0, deflation=ones,
+                st=dict(st_type="sinvert", pc_type=pc))
+    o = oracle.eps_solve(csr, nev=4, ncv=20, which=oracle.TARGET_MAGNITUDE, target=0.0, deflation=ones, sinvert=True,
+                         pc_none=(pc == "none"))
+    print(f"pc={pc} its={r['its']}/{o['its']} ksp_its={r['ksp_its']}/{o['ksp_its']}", flush=True)
+    assert r["reason"] == 1 and ", ".join("%.5f" % v for v in r["eigr"][:4]) == EX11
+    assert r["its"] == o["its"]
+    assert abs(r["ksp_its"] - o["ksp_its"]) <= 0.02 * o["ksp_its"] + 2
+    assert np.allclose(r["eigr"][:4], o["eigr"][:4], rtol=1e-7, atol=0)
+    assert (r["err"][:4] < 1e-6).all()
+
+
+def test_sinvert_closed_form_at_device_size(E):
+    # 256 x 255 5-point Laplacian (65,280 rows, simple eigenvalues): the four nearest to a
+    # shift below the spectrum, A - sigma I positive definite for CG
+    N, Mm, sigma = 256, 255, -0.01
+    csr = M.laplacian_2d(N, Mm)
+    n = len(csr[0]) - 1
+    r = E.solve(csr, nev=4, ncv=20, tol=1e-10, which=E.EPS.TARGET_MAGNITUDE, target=sigma,
+                v0=M.uniform_vector(2, n), want_vectors=False, st=dict(st_type="sinvert", ksp_rtol=1e-12))
+    closed = np.sort(M.laplacian_2d_eigenvalues(N, Mm))
+    near = closed[np.argsort(np.abs(closed - sigma))][:4]
+    print(f"n={n} its={r['its']} ksp_its={r['ksp_its']} eig={r['eigr'][:4]}", flush=True)
+    assert r["reason"] == 1
+    assert np.allclose(r["eigr"][:4], near, rtol=1e-9, atol=0)
+    # tol bounds the residual for theta = 1/(lambda - sigma); for A it grows by about
+    # |lambda - sigma| |A - sigma I| / |lambda| (|A| <= 8)
+    amp = np.abs(near - sigma) * (8 - sigma) / np.abs(near)
+    assert (r["err"][:4] < 100 * 1e-10 * amp).all(), (r["err"][:4], amp)   # margin for the close pair
+
+
+def test_sinvert_requires_target_which(E):
+    with pytest.raises(E.SlepcError):
+        E.solve(M.laplacian_1d(50), nev=2, which=E.EPS.LARGEST_MAGNITUDE, st=dict(st_type="sinvert"))
+
+
+def test_sinvert_indefinite_shift_reports_failed_solve(E):
+    # A - 2 I of the 1-D Laplacian is indefinite: CG stops with p^T (A - sigma I) p <= 0 (or
+    # runs out of iterations) and the solve fails with an error instead of a wrong answer
+    with pytest.raises(E.SlepcError):
+        E.solve(M.laplacian_1d(200), nev=2, which=E.EPS.TARGET_MAGNITUDE, target=2.0,
+                st=dict(st_type="sinvert", ksp_maxits=500))
