@@ -524,12 +524,12 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
         const T *col = V + (size_t)(c0 + u) * ld;
         if (c0 + u < k) {
           if (PAIR) {
-            const float2 p2 = *(const float2 *)(col + i0 + loff(0));
+            const float2 p2 = ks_stream_load2((const float *)(col + i0 + loff(0)));
             v[0][u] = p2.x;
             v[RPL - 1][u] = p2.y;
           } else {
 #pragma unroll
-            for (int q = 0; q < RPL; q++) v[q][u] = col[i0 + loff(q)];
+            for (int q = 0; q < RPL; q++) v[q][u] = ks_stream_load(col + i0 + loff(q));
           }
         } else {
 #pragma unroll
