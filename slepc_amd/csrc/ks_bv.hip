@@ -837,7 +837,7 @@ __global__ __launch_bounds__(KS_BLOCK) void k_multinplace_mfma(T *__restrict__ V
 #pragma unroll
       for (int kk = 0; kk < KS_MIP_KC / 4; kk++) {
         const int kc = min(k0 + 4 * kk + kq, K - 1);
-        a[r][kk] = (double)V[(size_t)row + (size_t)kc * ld];
+        a[r][kk] = (double)ks_stream_load(V + (size_t)row + (size_t)kc * ld);   // V is read once
       }
     }
 #pragma unroll
