@@ -63,6 +63,9 @@ typedef struct {
   double  scale;           /* 1/nrm to apply to the finalized column, or 1 */
   double  sumsq;           /* sum of squares of the current pass coefficients */
   double  exact_sq;        /* exact |q|^2 from the update pass */
+  double  prescale;        /* 1/nrm written by the final pass's sweep itself ... */
+  int32_t prescale_col;    /* ... for this column (-1: none) ... */
+  int32_t prescale_pass;   /* ... and this pass */
 } ks_status;
 
 /* ---------------------------------------------------------------- runtime */

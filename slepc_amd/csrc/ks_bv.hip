@@ -215,9 +215,33 @@ __device__ __forceinline__ bool gs_pass_gate(const ks_status *st, int col, int p
 // Runs on the one workgroup that finished the k+1 sums in fin[]: the first pass
 // stores c and |v| (BVDotColumnInc + BV_SquareRoot); a fused pass adds its c
 // to h (BV_AddCoefficients) and takes the DGKS decision of bvorthog.c:91-217.
+// The outcome of the next pass is fixed before it runs: its |v| and
+// coefficients (st->onrm, st->sumsq) are known here, and its finish decides from
+// them alone unless the estimate |v|^2 - sum c^2 is not positive.  When that
+// pass will be the last (and not lindep) and its sweep can apply a factor
+// (prescale_ok: the split sweeps), it writes the normalized vector itself, the
+// same rounding as the separate scaling, and the scaling launch finds nothing
+// to do.
+__device__ inline void gs_predict_last(ks_status *st, int col, int refine, double eta, int do_scale, int safe,
+                                       int prescale_ok) {
+  st->prescale_col = -1;
+  if (!prescale_ok || !do_scale || safe || refine == KS_REFINE_NEVER) return;
+  const int lp = st->pass_done + 1;
+  if (refine == KS_REFINE_ALWAYS && lp < 2) return;
+  const double onrm = st->onrm;
+  double nrm = onrm * onrm - st->sumsq;
+  if (!(nrm > 0.0)) return;
+  nrm = sqrt(nrm);
+  if (refine == KS_REFINE_IFNEEDED && lp < 3 && fabs(nrm) < eta * fabs(onrm)) return;   // another pass follows
+  if (!(fabs(nrm) >= eta * fabs(onrm)) || nrm == 1.0) return;                          // lindep, or nothing to scale
+  st->prescale = 1.0 / nrm;
+  st->prescale_col = col;
+  st->prescale_pass = lp;
+}
+
 template <bool FUSED>
 __device__ void gs_finish(const double *fin, int k, double *cbuf, double *hcol, ks_status *st, int col,
-                          int refine, double eta, int do_scale, int safe, double deftol) {
+                          int refine, double eta, int do_scale, int safe, double deftol, int prescale_ok) {
   __shared__ int s_continue;
   const int tid = threadIdx.x;
   if (!FUSED) {
@@ -239,6 +263,8 @@ __device__ void gs_finish(const double *fin, int k, double *cbuf, double *hcol, 
       st->pass_done = 0;
       st->done = 0;
       st->lindep = 0;
+      if (!st->error) gs_predict_last(st, col, refine, eta, do_scale, safe, prescale_ok);
+      else st->prescale_col = -1;
     }
     return;
   }
@@ -277,6 +303,7 @@ __device__ void gs_finish(const double *fin, int k, double *cbuf, double *hcol, 
       st->beta_last = nrm;
       if (have_norm) hcol[k] = lindep ? 0.0 : nrm;   // BV_SetValue
       st->scale = (do_scale && nrm != 1.0 && nrm != 0.0) ? 1.0 / nrm : 1.0;
+      if (st->prescale_col == col && st->prescale_pass == l) st->scale = 1.0;   // the sweep applied it
       if (lindep) {
         st->active = 0;
         st->breakdown_col = col;
@@ -288,6 +315,7 @@ __device__ void gs_finish(const double *fin, int k, double *cbuf, double *hcol, 
       for (int c = 0; c < k; c++) sumsq += fin[c] * fin[c];
       st->sumsq = sumsq;
       st->onrm = exact;
+      gs_predict_last(st, col, refine, eta, do_scale, safe, prescale_ok);
       s_continue = 1;
     }
   }
@@ -489,10 +517,16 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, lo
 // 8-byte aligned columns): each lane takes two adjacent rows with one 8-byte
 // load per column, so a wave still moves 512 contiguous bytes per column per
 // step, as with FP64 data.
+// the factor a fused split sweep applies to the vector it writes (see gs_predict_last)
+template <bool FUSED>
+__device__ __forceinline__ double gs_prescale(const ks_status *st, int col, int pass) {
+  return (FUSED && st->prescale_col == col && st->prescale_pass == pass) ? st->prescale : 1.0;
+}
+
 template <typename T, int CW, bool FUSED, int RPL, bool PAIR>
 __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restrict__ V, long ld, int k,
                                                T *__restrict__ w, int n, int rpb, const double *__restrict__ cbuf,
-                                               double *__restrict__ partial) {
+                                               double *__restrict__ partial, double wscale) {
   static_assert(!PAIR || (sizeof(T) == 4 && RPL == 2), "row pairs are two adjacent FP32 rows");
   using AT = typename acc_t<T>::type;   // FP32 data: FP32 products and per-thread sums
   constexpr int STEP = 64 * RPL;        // rows per step; each lane takes RPL of them
@@ -566,7 +600,8 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
         const AT tot = ((red[buf][0][r] + red[buf][1][r]) + red[buf][2][r]) + red[buf][3][r];
         const T t = (T)(wi[q] - tot);
         wi[q] = i < r1 ? (AT)t : (AT)0;
-        if (wave == 0 && i < r1) w[i] = t;
+        // the final pass of a column stores it normalized, rounded as k_scale_status would
+        if (wave == 0 && i < r1) w[i] = wscale == 1.0 ? t : (T)((AT)t * (AT)wscale);
       }
       buf ^= 1;
     }
@@ -603,7 +638,8 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_split(const T *__restrict__ V, 
                                                       int pass, int *ran) {
   if (!gs_pass_gate<FUSED>(st, col, pass)) return;
   if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = 1;
-  gs_split_block<T, CW, FUSED, RPL, PAIR>(blockIdx.x, gridDim.x, V, ld, k, w, n, rpb, cbuf, partial);
+  gs_split_block<T, CW, FUSED, RPL, PAIR>(blockIdx.x, gridDim.x, V, ld, k, w, n, rpb, cbuf, partial,
+                                          gs_prescale<FUSED>(st, col, pass));
 }
 
 // The finish of a Gram-Schmidt pass: a separate launch, ordered by the stream
@@ -618,7 +654,8 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_split(const T *__restrict__ V, 
 template <bool FUSED, int NT>
 __device__ __forceinline__ void gs_fin_block(int b, const double *__restrict__ partial, int nb, int k, double *sums,
                                              unsigned *counter, double *cbuf, double *hcol, ks_status *st, int col,
-                                             int refine, double eta, int do_scale, int safe, double deftol) {
+                                             int refine, double eta, int do_scale, int safe, double deftol,
+                                             int prescale_ok) {
   constexpr int NW = NT / 64;
   __shared__ double red[NW];
   __shared__ double fin[KS_NCOLP];
@@ -649,7 +686,7 @@ __device__ __forceinline__ void gs_fin_block(int b, const double *__restrict__ p
   for (int c = threadIdx.x; c <= k; c += blockDim.x) fin[c] = __builtin_nontemporal_load(&sums[c]);
   if (threadIdx.x == 0) *counter = 0u;
   __syncthreads();
-  gs_finish<FUSED>(fin, k, cbuf, hcol, st, col, refine, eta, do_scale, safe, deftol);
+  gs_finish<FUSED>(fin, k, cbuf, hcol, st, col, refine, eta, do_scale, safe, deftol, prescale_ok);
 }
 
 // one 256-thread workgroup per column (1024-thread workgroups, which read a
@@ -659,10 +696,10 @@ template <bool FUSED>
 __global__ __launch_bounds__(KS_FIN_BLOCK) void k_gs_fin(const double *__restrict__ partial, int nb, int k,
                                                         double *sums, unsigned *counter, double *cbuf, double *hcol,
                                                         ks_status *st, int col, int pass, int refine, double eta,
-                                                        int do_scale, int safe, double deftol) {
+                                                        int do_scale, int safe, double deftol, int prescale_ok) {
   if (!gs_pass_gate<FUSED>(st, col, pass)) return;
   gs_fin_block<FUSED, KS_FIN_BLOCK>(blockIdx.x, partial, nb, k, sums, counter, cbuf, hcol, st, col, refine, eta,
-                                    do_scale, safe, deftol);
+                                    do_scale, safe, deftol, prescale_ok);
 }
 
 template <typename T>
@@ -699,7 +736,7 @@ __global__ __launch_bounds__(KS_BLOCK) void k_grp_gs(ks_grp_gs g, int pass) {
   const ks_grp_gs_member &M = g.m[m];
   if (!gs_pass_gate<FUSED>(M.st, M.col, pass)) return;
   gs_split_block<double, CW, FUSED, 1, false>((int)blockIdx.x - g.b0_gs[m], M.nb, M.V, M.ld, M.k, M.w, M.n, M.rpb,
-                                           M.cbuf, M.partial);
+                                           M.cbuf, M.partial, gs_prescale<FUSED>(M.st, M.col, pass));
 }
 
 template <bool FUSED>
@@ -708,7 +745,7 @@ __global__ __launch_bounds__(KS_BLOCK) void k_grp_fin(ks_grp_gs g, int pass, int
   const ks_grp_gs_member &M = g.m[m];
   if (!gs_pass_gate<FUSED>(M.st, M.col, pass)) return;
   gs_fin_block<FUSED, KS_BLOCK>((int)blockIdx.x - g.b0_fin[m], M.partial, M.nb, M.k, M.sums, M.counter, M.cbuf, M.hcol, M.st,
-                      M.col, refine, eta, 1, 0, deftol);
+                      M.col, refine, eta, 1, 0, deftol, 1);
 }
 
 __global__ void k_grp_scale(ks_grp_gs g) {
@@ -1155,7 +1192,7 @@ void launch_gs_fin(ks_ctx ctx, const gs_geom &g, int k, double *cbuf, double *hc
   ks_prof_begin(ctx, KS_PROF_GS_FIN, (double)g.nb * (k + 1) * 8.0, (double)g.nb * (k + 1), false);
   hipLaunchKernelGGL((k_gs_fin<FUSED>), dim3(k + 1), dim3(KS_FIN_BLOCK), 0, ctx->stream, ctx->d_partial, g.nb, k,
                      ctx->d_gpartial, ctx->d_counters, cbuf, hcol, ctx->d_status, col, pass, refine, eta, do_scale,
-                     safe, deftol);
+                     safe, deftol, g.cw != 0);
   ks_prof_end(ctx);
 }
 

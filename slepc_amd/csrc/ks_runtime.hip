@@ -37,6 +37,9 @@ __global__ void k_status_reset(ks_status *st) {
     st->scale = 1.0;
     st->sumsq = 0.0;
     st->exact_sq = 0.0;
+    st->prescale = 1.0;
+    st->prescale_col = -1;
+    st->prescale_pass = 0;
   }
 }
 

@@ -32,7 +32,8 @@ class Status(C.Structure):
                 ("pass_done", C.c_int32), ("done", C.c_int32), ("lindep", C.c_int32),
                 ("error", C.c_int32), ("need_norm", C.c_int32), ("total_passes", C.c_int64),
                 ("onrm", C.c_double), ("nrm", C.c_double), ("beta_last", C.c_double),
-                ("scale", C.c_double), ("sumsq", C.c_double), ("exact_sq", C.c_double)]
+                ("scale", C.c_double), ("sumsq", C.c_double), ("exact_sq", C.c_double),
+                ("prescale", C.c_double), ("prescale_col", C.c_int32), ("prescale_pass", C.c_int32)]
 
 
 class ProfEntry(C.Structure):
