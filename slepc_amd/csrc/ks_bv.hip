@@ -437,6 +437,12 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_lds(const T *__restrict__ V, lo
 // sums are not stored.  FP32 data: FP32 products and per-thread sums.  The
 // workgroup reduces its accumulators once, at the end (wave butterflies in
 // FP64, then the four waves through LDS), into its partial row.
+// the factor a fused sweep (split or register) applies to the vector it writes (see gs_predict_last)
+template <bool FUSED>
+__device__ __forceinline__ double gs_prescale(const ks_status *st, int col, int pass) {
+  return (FUSED && st->prescale_col == col && st->prescale_pass == pass) ? st->prescale : 1.0;
+}
+
 template <typename T, int KC, bool FUSED>
 __global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, long ld, int k, T *__restrict__ w,
                                                     int n, int rpb, const double *__restrict__ cbuf,
@@ -444,6 +450,7 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, lo
                                                     int pass, int *ran) {
   if (!gs_pass_gate<FUSED>(st, col, pass)) return;
   if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = 1;
+  const double wscale = gs_prescale<FUSED>(st, col, pass);
   using AT = typename acc_t<T>::type;
   __shared__ AT coef[KC];
   __shared__ double red[KS_WAVES][KC + 1];
@@ -480,7 +487,7 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, lo
         s1 += (AT)v[c + 1] * coef[c + 1];
       }
       const T t = (T)(wi - (s0 + s1));
-      w[i] = t;
+      w[i] = wscale == 1.0 ? t : (T)((AT)t * (AT)wscale);   // the final pass stores it normalized
       wi = (AT)t;
     }
 #pragma unroll
@@ -517,12 +524,6 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, lo
 // 8-byte aligned columns): each lane takes two adjacent rows with one 8-byte
 // load per column, so a wave still moves 512 contiguous bytes per column per
 // step, as with FP64 data.
-// the factor a fused split sweep applies to the vector it writes (see gs_predict_last)
-template <bool FUSED>
-__device__ __forceinline__ double gs_prescale(const ks_status *st, int col, int pass) {
-  return (FUSED && st->prescale_col == col && st->prescale_pass == pass) ? st->prescale : 1.0;
-}
-
 template <typename T, int CW, bool FUSED, int RPL, bool PAIR>
 __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restrict__ V, long ld, int k,
                                                T *__restrict__ w, int n, int rpb, const double *__restrict__ cbuf,
@@ -1192,7 +1193,7 @@ void launch_gs_fin(ks_ctx ctx, const gs_geom &g, int k, double *cbuf, double *hc
   ks_prof_begin(ctx, KS_PROF_GS_FIN, (double)g.nb * (k + 1) * 8.0, (double)g.nb * (k + 1), false);
   hipLaunchKernelGGL((k_gs_fin<FUSED>), dim3(k + 1), dim3(KS_FIN_BLOCK), 0, ctx->stream, ctx->d_partial, g.nb, k,
                      ctx->d_gpartial, ctx->d_counters, cbuf, hcol, ctx->d_status, col, pass, refine, eta, do_scale,
-                     safe, deftol, g.cw != 0);
+                     safe, deftol, g.cw != 0 || g.kc != 0);
   ks_prof_end(ctx);
 }
 
