@@ -662,14 +662,24 @@ __device__ __forceinline__ void gs_fin_block(int b, const double *__restrict__ p
   __shared__ double fin[KS_NCOLP];
   __shared__ unsigned ticket;
   const double *p = partial + (size_t)b * nb;
-  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tail = 0.0;
-  int q = threadIdx.x;
-  for (; q + 7 * NT < nb; q += 8 * NT) {
+  // FIN_U predicated loads per thread issued together: one memory round trip
+  // covers a column of up to FIN_U * NT partials (6144 with 256 threads)
+  constexpr int FIN_U = 24;
+  double a[FIN_U];
 #pragma unroll
-    for (int u = 0; u < 8; u++) a[u] += p[q + u * NT];
+  for (int u = 0; u < FIN_U; u++) a[u] = 0.0;
+  for (int q0 = threadIdx.x; q0 < nb; q0 += FIN_U * NT) {
+#pragma unroll
+    for (int u = 0; u < FIN_U; u++) {
+      const int q = q0 + u * NT;
+      a[u] += q < nb ? p[q] : 0.0;
+    }
   }
-  for (; q < nb; q += NT) tail += p[q];
-  double v = ks_wave_sum((((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]))) + tail);
+#pragma unroll
+  for (int h = FIN_U / 2; h >= 3; h /= 2)   // 24 -> 12 -> 6 -> 3, pairwise in a fixed order
+#pragma unroll
+    for (int u = 0; u < h; u++) a[u] += a[u + h];
+  double v = ks_wave_sum((a[0] + a[1]) + a[2]);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
