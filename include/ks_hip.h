@@ -173,6 +173,19 @@ int ks_bv_multvec(ks_ctx ctx, ks_prec prec, int n, double alpha, const void *V, 
 int ks_bv_orthonormalize_column(ks_ctx ctx, ks_prec prec, int n, void *V, int ld, int k,
                                 double *d_buf, int ldbuf, int hcol, int col, ks_refine refine,
                                 double eta, int scale, int safe);
+/* The same orthonormalization without a context and without allocating: no
+   counterpart in the reference.  For callers that cannot allocate device
+   memory (code under HIP graph capture, serving loops, harnesses that forbid
+   hipMalloc).  The status, the finishing counter and the partial sums live in
+   one file-scope device array of the library (6.3 MB per GPU, held until the
+   process ends), so k <= 128 (KS_ERR_RANGE beyond).  Work is enqueued on
+   `stream` (a hipStream_t; NULL = the null stream); the status of the column
+   (ks_status: norm, lindep, passes) is copied to d_status_out (device memory,
+   may be NULL) in stream order.  Every call on a GPU shares that GPU's array:
+   calls that overlap on one GPU (other streams, other host threads) race. */
+int ks_bv_orthonormalize_column_noalloc(void *stream, ks_prec prec, int n, void *V, int ld, int k,
+                                        double *d_buf, int ldbuf, int hcol, int col, ks_refine refine,
+                                        double eta, int scale, int safe, ks_status *d_status_out);
 /* Reset the status for a new expansion (active=1, breakdown_col=-1). */
 int ks_status_reset(ks_ctx ctx);
 /* Copy the status to the host (synchronizes the stream). */

@@ -10,6 +10,7 @@
 #include "ks_internal.hpp"
 
 #include <cmath>
+#include <mutex>
 
 #define KS_CT 8          // columns per register tile in the dot kernels
 #define KS_MIP_KC 16     // k-chunk of Q staged in LDS
@@ -1242,6 +1243,79 @@ int orthonormalize_t(ks_ctx ctx, int n, T *V, long ld, int k, double *buf, int l
   return KS_OK;
 }
 
+// ---------------------------------------------------------------- allocation-free orthonormalization
+// ks_bv_orthonormalize_column_noalloc runs the kernels of
+// ks_bv_orthonormalize_column with their workspace in the file-scope array
+// `scratch` instead of a context's: the DGKS status, the finishing counter,
+// the k+1 column sums and the (k+1) x nb partial sums of one sweep.
+#define KS_NOALLOC_MAXK KS_GS_SPLITMAX                 // columns orthogonalized against, at most
+#define KS_NOALLOC_STATUS 0                            // offsets in doubles, 256-byte aligned
+#define KS_NOALLOC_COUNTER 32
+#define KS_NOALLOC_SUMS 64
+#define KS_NOALLOC_PARTIAL 448
+#define KS_NOALLOC_PARTIAL_CAP ((size_t)(KS_NOALLOC_MAXK + 1) * KS_GS_SPLIT_BLOCKS)
+#define KS_NOALLOC_DOUBLES (KS_NOALLOC_PARTIAL + KS_NOALLOC_PARTIAL_CAP)
+static_assert(sizeof(ks_status) <= 32 * sizeof(double), "status slot too small");
+static_assert(KS_NOALLOC_SUMS + KS_NCOLP <= KS_NOALLOC_PARTIAL, "column-sum slot too small");
+
+// the status of a fresh column and a zero finishing counter (fill kernel, no memset node)
+__global__ void k_noalloc_reset(ks_status *st, unsigned *counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->active = 1;
+    st->breakdown_col = -1;
+    st->col = -1;
+    st->pass_done = 0;
+    st->done = 0;
+    st->lindep = 0;
+    st->error = 0;
+    st->need_norm = 0;
+    st->total_passes = 0;
+    st->onrm = 0.0;
+    st->nrm = 0.0;
+    st->beta_last = 0.0;
+    st->scale = 1.0;
+    st->sumsq = 0.0;
+    st->exact_sq = 0.0;
+    st->prescale = 1.0;
+    st->prescale_col = -1;
+    st->prescale_pass = 0;
+    *counter = 0u;
+  }
+}
+
+__global__ void k_noalloc_status_out(const ks_status *st, ks_status *out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = *st;
+}
+
+}  // namespace
+
+// 6.3 MB per device, held until the process ends.  Host code never passes the
+// array's name to a kernel: it uses the device address that hipGetSymbolAddress
+// returns for the current device.
+static __device__ __align__(256) double scratch[KS_NOALLOC_DOUBLES];
+
+namespace {
+
+#define KS_NOALLOC_MAXDEV 64
+std::mutex g_noalloc_mu;
+double *g_noalloc_ptr[KS_NOALLOC_MAXDEV];
+
+// device address of `scratch` on the current device, looked up once per device
+int noalloc_scratch(double **out) {
+  int dev = -1;
+  KS_HIP(hipGetDevice(&dev));
+  KS_CHECK(dev >= 0 && dev < KS_NOALLOC_MAXDEV, KS_ERR_ARG, "device %d beyond the scratch table", dev);
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void *p = nullptr;
+    KS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    KS_CHECK(p, KS_ERR_HIP, "no device address for the scratch array on device %d", dev);
+    g_noalloc_ptr[dev] = (double *)p;
+  }
+  *out = g_noalloc_ptr[dev];
+  return KS_OK;
+}
+
 }  // namespace
 
 int ks_ew_blocks(int n) { return ew_blocks(n); }
@@ -1468,6 +1542,41 @@ int ks_bv_multinplace(ks_ctx ctx, ks_prec prec, int n, void *V, int ld, int k, c
                            : dispatch_mip<float>(ctx, (float *)V, ld, n, k, d_Q, ldq, s, N);
   ks_prof_end(ctx);
   if (err) return err;
+  KS_HIP(hipGetLastError());
+  return KS_OK;
+}
+
+int ks_bv_orthonormalize_column_noalloc(void *stream, ks_prec prec, int n, void *V, int ld, int k, double *d_buf,
+                                        int ldbuf, int hcol, int col, ks_refine refine, double eta, int scale,
+                                        int safe, ks_status *d_status_out) {
+  KS_CHECK(V && d_buf, KS_ERR_ARG, "null orthonormalize argument");
+  KS_CHECK(n > 0 && ld >= n && k >= 0 && ldbuf > k && hcol >= 0, KS_ERR_ARG,
+           "bad orthonormalize sizes n=%d ld=%d k=%d ldbuf=%d", n, ld, k, ldbuf);
+  KS_CHECK(refine >= KS_REFINE_IFNEEDED && refine <= KS_REFINE_ALWAYS, KS_ERR_ARG, "bad refinement type");
+  KS_CHECK(k <= KS_NOALLOC_MAXK, KS_ERR_RANGE,
+           "the allocation-free orthonormalization takes at most %d columns (asked %d); use "
+           "ks_bv_orthonormalize_column", KS_NOALLOC_MAXK, k);
+  gs_geom g;
+  KS_TRY(gs_geometry(n, k, prec == KS_F64 ? sizeof(double) : sizeof(float), &g));
+  KS_CHECK((size_t)(k + 1) * (size_t)g.nb <= KS_NOALLOC_PARTIAL_CAP, KS_ERR_RANGE,
+           "%d x %d partial sums exceed the scratch array", k + 1, g.nb);
+  double *base = nullptr;
+  KS_TRY(noalloc_scratch(&base));
+  // a context view of the scratch: no events, no allocation
+  ks_ctx_s c;
+  c.stream = (hipStream_t)stream;
+  c.d_status = (ks_status *)(base + KS_NOALLOC_STATUS);
+  c.d_counters = (unsigned *)(base + KS_NOALLOC_COUNTER);
+  c.d_gpartial = base + KS_NOALLOC_SUMS;
+  c.d_partial = base + KS_NOALLOC_PARTIAL;
+  hipLaunchKernelGGL(k_noalloc_reset, dim3(1), dim3(64), 0, c.stream, c.d_status, c.d_counters);
+  int err = prec == KS_F64 ? orthonormalize_t<double>(&c, n, (double *)V, ld, k, d_buf, ldbuf, hcol, col, refine,
+                                                      eta, scale, safe)
+                           : orthonormalize_t<float>(&c, n, (float *)V, ld, k, d_buf, ldbuf, hcol, col, refine,
+                                                     eta, scale, safe);
+  if (err) return err;
+  if (d_status_out)
+    hipLaunchKernelGGL(k_noalloc_status_out, dim3(1), dim3(64), 0, c.stream, c.d_status, d_status_out);
   KS_HIP(hipGetLastError());
   return KS_OK;
 }

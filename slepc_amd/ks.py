@@ -62,6 +62,8 @@ def lib():
             "ks_bv_multvec": [vp, i32, i32, dbl, vp, i32, i32, vp, dbl, vp],
             "ks_bv_orthonormalize_column": [vp, i32, i32, vp, i32, i32, vp, i32, i32, i32, i32, dbl,
                                             i32, i32],
+            "ks_bv_orthonormalize_column_noalloc": [vp, i32, i32, vp, i32, i32, vp, i32, i32, i32, i32, dbl,
+                                                    i32, i32, vp],
             "ks_status_reset": [vp], "ks_status_get": [vp, C.POINTER(Status)],
             "ks_bv_scale": [vp, i32, i32, vp, dbl], "ks_bv_norm": [vp, i32, i32, vp, C.POINTER(dbl)],
             "ks_bv_axpy": [vp, i32, i32, dbl, vp, vp], "ks_bv_copy": [vp, i32, i32, vp, vp],
@@ -235,6 +237,16 @@ def orthonormalize_column(ctx, prec, n, V, ld, k, buf, ldbuf, hcol, col, refine=
                           eta=0.7071, scale=1, safe=0):
     check(lib().ks_bv_orthonormalize_column(ctx.h, prec, n, V.ptr, ld, k, buf.ptr, ldbuf, hcol, col, refine,
                                             eta, scale, safe), "ks_bv_orthonormalize_column")
+
+
+def orthonormalize_column_noalloc(stream, prec, n, V, ld, k, buf, ldbuf, hcol, col, refine=REFINE_IFNEEDED,
+                                  eta=0.7071, scale=1, safe=0, status_out=None):
+    """The allocation-free variant: `stream` is a raw hipStream_t (ks_ctx_stream of a
+    context), the workspace is the library's own device array."""
+    check(lib().ks_bv_orthonormalize_column_noalloc(stream, prec, n, V.ptr, ld, k, buf.ptr, ldbuf, hcol, col,
+                                                    refine, eta, scale, safe,
+                                                    status_out.ptr if status_out is not None else None),
+          "ks_bv_orthonormalize_column_noalloc")
 
 
 def multinplace(ctx, prec, n, V, ld, k, Q, ldq, s, e, voff=0, qoff=0):
