@@ -66,6 +66,8 @@ typedef struct {
   double  prescale;        /* 1/nrm written by the final pass's sweep itself ... */
   int32_t prescale_col;    /* ... for this column (-1: none) ... */
   int32_t prescale_pass;   /* ... and this pass */
+  int32_t halt_col;        /* a lean orthonormalization stopped the expansion at this column (-1: none) ... */
+  int32_t halt_need;       /* ... and needs 1: the third DGKS pass, 2: the scaling (ks_bv_orthonormalize_resume) */
 } ks_status;
 
 /* ---------------------------------------------------------------- runtime */
@@ -135,11 +137,20 @@ int ks_csr_transpose(ks_csr A, ks_ctx ctx, ks_csr *out);
    rnorm <= max(rtol |B b|, abstol), at most maxits iterations) with
    B = diag(A - sigma I)^-1 from ks_pcg_jacobi (jacobi = 0: B = I).  Replaces
    STApply_Sinvert's KSPSolve (src/sys/classes/st/impls/sinvert/sinvert.c:16-30).
-   d_work holds 4n + 8 doubles; reason > 0 converged, -3 iteration limit,
-   -8 p^T (A - sigma I) p <= 0 (not positive definite). */
+   The convergence tests run on the device; the host reads the solver state
+   once per 8 iterations.  d_work holds 4n + 64 doubles; reason > 0 converged,
+   -3 iteration limit, -8 p^T (A - sigma I) p <= 0 (not positive definite). */
 int ks_pcg_jacobi(ks_ctx ctx, ks_csr A, double sigma, int jacobi, double *d_dinv);
 int ks_pcg_solve(ks_ctx ctx, ks_csr A, double sigma, const double *d_dinv, const double *d_b, double *d_x,
                  double rtol, double abstol, int maxits, double *d_work, int *its, double *rnorm, int *reason);
+/* The same solve by BiCGStab (KSPBCGS's recurrences with the left
+   preconditioner B, PETSc src/ksp/ksp/impls/bcgs/bcgs.c), which needs no
+   definiteness: the inner solver of shift-and-invert at a target inside the
+   spectrum.  The iteration ends early when |s| <= the threshold.  d_work holds
+   6n + 64 doubles; reason > 0 converged, -3 iteration limit, -5 breakdown
+   ((rhat, v) = 0, (t, t) = 0, rho = 0 or omega = 0). */
+int ks_bcgs_solve(ks_ctx ctx, ks_csr A, double sigma, const double *d_dinv, const double *d_b, double *d_x,
+                  double rtol, double abstol, int maxits, double *d_work, int *its, double *rnorm, int *reason);
 /* nrows, nnz, max row length and the kernel variant chosen (0 stream, 1 vector) */
 int ks_csr_info(ks_csr A, int *nrows, int64_t *nnz, int *max_row, int *variant, int *nblocks);
 /* y = A x.  x, y device vectors in xprec.  When gate != 0 the product is
@@ -186,6 +197,25 @@ int ks_bv_orthonormalize_column(ks_ctx ctx, ks_prec prec, int n, void *V, int ld
 int ks_bv_orthonormalize_column_noalloc(void *stream, ks_prec prec, int n, void *V, int ld, int k,
                                         double *d_buf, int ldbuf, int hcol, int col, ks_refine refine,
                                         double eta, int scale, int safe, ks_status *d_status_out);
+/* The same orthonormalization with only the launches a column normally needs:
+   the first-pass dots and the DGKS passes that IFNEEDED takes unless a third is
+   due (two), or ALWAYS's two, each with its finish, and no separate scaling --
+   the last pass's sweep stores the normalized column.  When the column needs
+   more (a third pass, or a scaling the sweep could not apply) the finish stops
+   the expansion on the device instead: status active = 0, halt_col = col,
+   halt_need = 1 or 2, so every later gated launch does nothing.  The caller
+   then runs ks_bv_orthonormalize_resume for that column and enqueues the rest
+   of the expansion again; the result is bit for bit that of
+   ks_bv_orthonormalize_column.  Other refinement types, safe mode, scale = 0
+   and the LDS sweeps (k > 128) fall back to the full sequence. */
+int ks_bv_orthonormalize_column_lean(ks_ctx ctx, ks_prec prec, int n, void *V, int ld, int k, double *d_buf,
+                                     int ldbuf, int hcol, int col, ks_refine refine, double eta, int scale,
+                                     int safe);
+/* Completes a column stopped by ks_bv_orthonormalize_column_lean (the status
+   names it): re-activates the status and enqueues the remaining pass and the
+   scaling, each gated on the device as in ks_bv_orthonormalize_column. */
+int ks_bv_orthonormalize_resume(ks_ctx ctx, ks_prec prec, int n, void *V, int ld, int k, double *d_buf, int ldbuf,
+                                int hcol, int col, ks_refine refine, double eta);
 /* Reset the status for a new expansion (active=1, breakdown_col=-1). */
 int ks_status_reset(ks_ctx ctx);
 /* Copy the status to the host (synchronizes the stream). */

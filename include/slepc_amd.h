@@ -71,6 +71,8 @@ typedef const char *DSType;
 typedef const char *STType;
 typedef const char *EPSType;
 
+#define VECHIP         "hip"       /* device vectors, FP64 */
+#define VECHIPF32      "hipf32"    /* device vectors, FP32 (the mixed-precision basis) */
 #define MATHIPCSR      "hipcsr"
 #define MATSEQDENSE    "seqdense"
 #define BVHIPKS        "hipks"
@@ -87,6 +89,10 @@ typedef enum { NORM_1 = 0, NORM_2 = 1, NORM_FROBENIUS = 2, NORM_INFINITY = 3 } N
 
 /* ------------------------------------------------------------ system */
 PetscErrorCode SlepcInitialize(int device);
+/* the options database: "-name value" pairs (PETSC_OPTIONS is read by SlepcInitialize) */
+PetscErrorCode PetscOptionsInsertString(const char in_str[]);
+PetscErrorCode PetscOptionsSetValue(const char name[], const char value[]);
+PetscErrorCode PetscOptionsClearValue(const char name[]);
 PetscErrorCode SlepcFinalize(void);
 PetscErrorCode SlepcInitialized(PetscBool *flg);
 /* last error message recorded by the library */
@@ -164,7 +170,23 @@ PetscErrorCode BVHIPKSGetPrecision(BV bv, SlepcAMDPrecision *prec);
    small problems, where launch overhead rivals the kernels (no counterpart in
    the reference) */
 PetscErrorCode BVHIPKSSetStepGraphs(BV bv, PetscBool flg);
-PetscErrorCode BVSetOrthogonalization(BV bv, BVOrthogType type, BVOrthogRefineType refine, PetscReal eta, int block);
+PetscErrorCode BVSetOrthogonalization(BV bv, BVOrthogType type, BVOrthogRefineType refine, PetscReal eta,
+                                      BVOrthogBlockType block);
+PetscErrorCode BVGetOrthogonalization(BV bv, BVOrthogType *type, BVOrthogRefineType *refine, PetscReal *eta,
+                                      BVOrthogBlockType *block);
+/* BVSetVecType (bvbasic.c): VECHIP (FP64 columns, the default) or VECHIPF32 (FP32
+   columns, the mixed-precision basis); before the storage is allocated */
+PetscErrorCode BVSetVecType(BV bv, VecType vtype);
+PetscErrorCode BVGetVecType(BV bv, VecType *vtype);
+/* BVSetFromOptions (bvbasic.c): -bv_type, -bv_orthog_type cgs, -bv_orthog_refine
+   never|ifneeded|always, -bv_orthog_eta, -bv_orthog_block gs|chol|tsqr|tsqrchol|svqb,
+   and the type's own options (BVHIPKS: -bv_hipks_krylov 0|1) */
+PetscErrorCode BVSetFromOptions(BV bv);
+/* BVHIPKS: turn the device-resident Krylov expansion on (default) or off; off, the
+   interface runs the reference's per-step loop over the gramschmidt op
+   (also -bv_hipks_krylov 0 in the options database) */
+PetscErrorCode BVHIPKSSetKrylovExtension(BV bv, PetscBool flg);
+PetscErrorCode BVHIPKSGetKrylovExtension(BV bv, PetscBool *flg);
 PetscErrorCode BVSetActiveColumns(BV bv, PetscInt l, PetscInt k);
 PetscErrorCode BVGetActiveColumns(BV bv, PetscInt *l, PetscInt *k);
 PetscErrorCode BVGetColumn(BV bv, PetscInt j, Vec *v);
@@ -196,6 +218,8 @@ PetscErrorCode BVOrthogonalize(BV V, Mat R);
 PetscErrorCode BVMatProject(BV X, Mat A, BV Y, Mat M);
 PetscErrorCode BVOrthogonalizeColumn(BV bv, PetscInt j, PetscScalar H[], PetscReal *norm, PetscBool *lindep);
 PetscErrorCode BVOrthonormalizeColumn(BV bv, PetscInt j, PetscBool replace, PetscReal *norm, PetscBool *lindep);
+/* v against the active columns and the constraints (bvorthog.c:236-266); H gets k-l coefficients */
+PetscErrorCode BVOrthogonalizeVec(BV bv, Vec v, PetscScalar H[], PetscReal *norm, PetscBool *lindep);
 PetscErrorCode BVMatMultColumn(BV bv, Mat A, PetscInt j);
 PetscErrorCode BVMatArnoldi(BV bv, Mat A, Mat H, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown);
 PetscErrorCode BVMatLanczos(BV bv, Mat A, Mat T, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown);
@@ -250,6 +274,7 @@ typedef struct _p_PC *PC;
 #define STSHIFT   "shift"
 #define STSINVERT "sinvert"
 #define KSPCG     "cg"
+#define KSPBCGS   "bcgs"
 #define PCJACOBI  "jacobi"
 #define PCNONE    "none"
 PetscErrorCode STCreate(ST *st);
@@ -258,8 +283,11 @@ PetscErrorCode STGetType(ST st, STType *type);
 PetscErrorCode STSetShift(ST st, PetscScalar shift);
 PetscErrorCode STGetShift(ST st, PetscScalar *shift);
 PetscErrorCode STSetDefaultShift(ST st, PetscScalar defaultshift);
-/* STSINVERT's inner solver: conjugate gradients on the device, Jacobi or no
-   preconditioner (the reference's KSP/PC objects, reduced to what runs here) */
+/* STSINVERT's inner solver on the device: conjugate gradients (KSPCG, the
+   default, for a positive definite A - sigma I) or BiCGStab (KSPBCGS, for a
+   shift inside the spectrum), Jacobi or no preconditioner (the reference's
+   KSP/PC objects, reduced to what runs here).  The option -st_ksp_type
+   selects the solver, as KSPSetFromOptions does. */
 PetscErrorCode STGetKSP(ST st, KSP *ksp);
 PetscErrorCode KSPSetType(KSP ksp, KSPType type);
 PetscErrorCode KSPGetPC(KSP ksp, PC *pc);
@@ -311,6 +339,9 @@ PetscErrorCode EPSKrylovSchurSetLocking(EPS eps, PetscBool lock);
    phase down to tol32 whose Ritz vectors seed the FP64 restarts; tol32 <= 0
    turns it off.  The stats report the FP32 phase. */
 PetscErrorCode EPSKrylovSchurSetMixedPrecision(EPS eps, PetscReal tol32);
+/* capture each Arnoldi step of the basis' expansions as a hipGraph and replay it
+   (BVHIPKSSetStepGraphs on the solver's basis, applied at set-up) */
+PetscErrorCode EPSKrylovSchurSetStepGraphs(EPS eps, PetscBool flg);
 PetscErrorCode EPSKrylovSchurGetMixedStats(EPS eps, PetscInt *its32, PetscInt *nconv32, int64_t *nmatmult32,
                                            int64_t *passes32, PetscReal *t32);
 PetscErrorCode EPSSetBV(EPS eps, BV bv);
@@ -379,6 +410,7 @@ typedef struct {
   int64_t nmatmult, gs_passes;
   double t_arrival, t_admit, t_done;
   double t_host_ds;              /* host seconds in the projected problems of this request */
+  PetscErrorCode error;          /* nonzero: the request failed (at admission) with this code, no pairs */
 } SlepcResult;
 typedef struct _p_SlepcServer *SlepcServer;
 PetscErrorCode SlepcServerCreate(size_t hbm_budget, PetscInt max_active, SlepcServer *srv);

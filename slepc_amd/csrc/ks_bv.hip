@@ -240,9 +240,13 @@ __device__ inline void gs_predict_last(ks_status *st, int col, int refine, doubl
   st->prescale_pass = lp;
 }
 
+// halt_pass > 0 (the lean sequence): the finish of that pass stops the
+// expansion when the column needs a launch the lean sequence left out -- a
+// further DGKS pass, or a scaling its last sweep did not apply.
 template <bool FUSED>
 __device__ void gs_finish(const double *fin, int k, double *cbuf, double *hcol, ks_status *st, int col,
-                          int refine, double eta, int do_scale, int safe, double deftol, int prescale_ok) {
+                          int refine, double eta, int do_scale, int safe, double deftol, int prescale_ok,
+                          int halt_pass) {
   __shared__ int s_continue;
   const int tid = threadIdx.x;
   if (!FUSED) {
@@ -308,6 +312,10 @@ __device__ void gs_finish(const double *fin, int k, double *cbuf, double *hcol, 
       if (lindep) {
         st->active = 0;
         st->breakdown_col = col;
+      } else if (halt_pass > 0 && st->scale != 1.0) {
+        st->active = 0;
+        st->halt_col = col;
+        st->halt_need = 2;
       }
       s_continue = 0;
     } else {
@@ -317,6 +325,11 @@ __device__ void gs_finish(const double *fin, int k, double *cbuf, double *hcol, 
       st->sumsq = sumsq;
       st->onrm = exact;
       gs_predict_last(st, col, refine, eta, do_scale, safe, prescale_ok);
+      if (halt_pass == l) {
+        st->active = 0;
+        st->halt_col = col;
+        st->halt_need = 1;
+      }
       s_continue = 1;
     }
   }
@@ -657,7 +670,7 @@ template <bool FUSED, int NT>
 __device__ __forceinline__ void gs_fin_block(int b, const double *__restrict__ partial, int nb, int k, double *sums,
                                              unsigned *counter, double *cbuf, double *hcol, ks_status *st, int col,
                                              int refine, double eta, int do_scale, int safe, double deftol,
-                                             int prescale_ok) {
+                                             int prescale_ok, int halt_pass) {
   constexpr int NW = NT / 64;
   __shared__ double red[NW];
   __shared__ double fin[KS_NCOLP];
@@ -698,7 +711,7 @@ __device__ __forceinline__ void gs_fin_block(int b, const double *__restrict__ p
   for (int c = threadIdx.x; c <= k; c += blockDim.x) fin[c] = __builtin_nontemporal_load(&sums[c]);
   if (threadIdx.x == 0) *counter = 0u;
   __syncthreads();
-  gs_finish<FUSED>(fin, k, cbuf, hcol, st, col, refine, eta, do_scale, safe, deftol, prescale_ok);
+  gs_finish<FUSED>(fin, k, cbuf, hcol, st, col, refine, eta, do_scale, safe, deftol, prescale_ok, halt_pass);
 }
 
 // one 256-thread workgroup per column (1024-thread workgroups, which read a
@@ -708,10 +721,11 @@ template <bool FUSED>
 __global__ __launch_bounds__(KS_FIN_BLOCK) void k_gs_fin(const double *__restrict__ partial, int nb, int k,
                                                         double *sums, unsigned *counter, double *cbuf, double *hcol,
                                                         ks_status *st, int col, int pass, int refine, double eta,
-                                                        int do_scale, int safe, double deftol, int prescale_ok) {
+                                                        int do_scale, int safe, double deftol, int prescale_ok,
+                                                        int halt_pass) {
   if (!gs_pass_gate<FUSED>(st, col, pass)) return;
   gs_fin_block<FUSED, KS_FIN_BLOCK>(blockIdx.x, partial, nb, k, sums, counter, cbuf, hcol, st, col, refine, eta,
-                                    do_scale, safe, deftol, prescale_ok);
+                                    do_scale, safe, deftol, prescale_ok, halt_pass);
 }
 
 template <typename T>
@@ -719,6 +733,15 @@ __device__ __forceinline__ void scale_status_block(int b, int nblk, T *__restric
   size_t i = (size_t)b * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)nblk * blockDim.x;
   for (; i < (size_t)n; i += stride) w[i] = (T)((typename acc_t<T>::type)w[i] * (typename acc_t<T>::type)s);
+}
+
+// the expansion stopped by a lean orthonormalization goes on (ks_bv_orthonormalize_resume)
+__global__ void k_status_resume(ks_status *st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && st->halt_col >= 0) {
+    st->active = 1;
+    st->halt_col = -1;
+    st->halt_need = 0;
+  }
 }
 
 template <typename T>
@@ -757,7 +780,7 @@ __global__ __launch_bounds__(KS_BLOCK) void k_grp_fin(ks_grp_gs g, int pass, int
   const ks_grp_gs_member &M = g.m[m];
   if (!gs_pass_gate<FUSED>(M.st, M.col, pass)) return;
   gs_fin_block<FUSED, KS_BLOCK>((int)blockIdx.x - g.b0_fin[m], M.partial, M.nb, M.k, M.sums, M.counter, M.cbuf, M.hcol, M.st,
-                      M.col, refine, eta, 1, 0, deftol, 1);
+                      M.col, refine, eta, 1, 0, deftol, 1, 0);
 }
 
 __global__ void k_grp_scale(ks_grp_gs g) {
@@ -1199,18 +1222,23 @@ void launch_gs(ks_ctx ctx, const gs_geom &g, const T *V, long ld, int k, T *w, i
 
 template <bool FUSED>
 void launch_gs_fin(ks_ctx ctx, const gs_geom &g, int k, double *cbuf, double *hcol, int col, int pass, int refine,
-                   double eta, int do_scale, int safe) {
+                   double eta, int do_scale, int safe, int halt_pass) {
   const double deftol = 10.0 * 2.220446049250313e-16;
   ks_prof_begin(ctx, KS_PROF_GS_FIN, (double)g.nb * (k + 1) * 8.0, (double)g.nb * (k + 1), false);
   hipLaunchKernelGGL((k_gs_fin<FUSED>), dim3(k + 1), dim3(KS_FIN_BLOCK), 0, ctx->stream, ctx->d_partial, g.nb, k,
                      ctx->d_gpartial, ctx->d_counters, cbuf, hcol, ctx->d_status, col, pass, refine, eta, do_scale,
-                     safe, deftol, g.cw != 0 || g.kc != 0);
+                     safe, deftol, g.cw != 0 || g.kc != 0, halt_pass);
   ks_prof_end(ctx);
 }
 
+// mode 0: the whole sequence (first-pass dots, every DGKS pass the refinement
+// allows, the scaling), each launch gated on the device status.  mode 1 (lean):
+// the passes a column normally needs and no scaling launch; the finish stops
+// the expansion when the column needs more (see gs_finish).  mode 2 (resume):
+// the launches mode 1 left out, after re-activating the status.
 template <typename T>
 int orthonormalize_t(ks_ctx ctx, int n, T *V, long ld, int k, double *buf, int ldbuf, int hcol_idx, int col,
-                     ks_refine refine, double eta, int do_scale, int safe) {
+                     ks_refine refine, double eta, int do_scale, int safe, int mode = 0) {
   gs_geom g;
   int e;
   if ((e = gs_geometry(n, k, sizeof(T), &g))) return e;
@@ -1220,20 +1248,30 @@ int orthonormalize_t(ks_ctx ctx, int n, T *V, long ld, int k, double *buf, int l
   double *cbuf = buf;                              // buffer column 0: scratch coefficients
   double *hcol = buf + (size_t)hcol_idx * ldbuf;   // buffer column hcol_idx: accumulated h
   const int passes = safe || refine == KS_REFINE_NEVER ? 1 : (refine == KS_REFINE_ALWAYS ? 2 : 3);
-  // first pass dots: c = V^T w and w^T w
-  int *ran = ks_prof_begin(ctx, KS_PROF_GS_DOT1, (k + 1.0) * n * s, 2.0 * (k + 1.0) * n, true);
-  launch_gs<T, false>(ctx, g, V, ld, k, w, n, cbuf, col, 0, ran);
-  ks_prof_end(ctx);
-  launch_gs_fin<false>(ctx, g, k, cbuf, hcol, col, 0, refine, eta, do_scale, safe);
+  // the lean sequence needs the sweeps that store the normalized column
+  const bool lean_ok = !safe && do_scale && refine != KS_REFINE_NEVER && (g.cw || g.kc);
+  if (mode && !lean_ok) mode = mode == 1 ? 0 : -1;   // full sequence, or nothing left to resume
+  if (mode < 0) return KS_OK;
+  const int halt_pass = mode == 1 ? 2 : 0;
+  const int p_first = mode == 2 ? 3 : 1, p_last = mode == 1 ? 2 : passes;
+  int *ran;
+  if (mode == 2) hipLaunchKernelGGL(k_status_resume, dim3(1), dim3(64), 0, ctx->stream, ctx->d_status);
+  if (mode != 2) {
+    // first pass dots: c = V^T w and w^T w
+    ran = ks_prof_begin(ctx, KS_PROF_GS_DOT1, (k + 1.0) * n * s, 2.0 * (k + 1.0) * n, true);
+    launch_gs<T, false>(ctx, g, V, ld, k, w, n, cbuf, col, 0, ran);
+    ks_prof_end(ctx);
+    launch_gs_fin<false>(ctx, g, k, cbuf, hcol, col, 0, refine, eta, do_scale, safe, 0);
+  }
   // each pass: update with its c and the next pass's dots in the same sweep
-  for (int p = 1; p <= passes; p++) {
+  for (int p = p_first; p <= p_last; p++) {
     ran = ks_prof_begin(ctx, p == 1 ? KS_PROF_GS_UPD1 : KS_PROF_GS_UPDR, (k + 2.0) * n * s,
                         2.0 * k * n + 2.0 * (k + 1.0) * n, true);
     launch_gs<T, true>(ctx, g, V, ld, k, w, n, cbuf, col, p, ran);
     ks_prof_end(ctx);
-    launch_gs_fin<true>(ctx, g, k, cbuf, hcol, col, p, refine, eta, do_scale, safe);
+    launch_gs_fin<true>(ctx, g, k, cbuf, hcol, col, p, refine, eta, do_scale, safe, halt_pass);
   }
-  if (do_scale) {
+  if (do_scale && mode != 1) {
     int *ran = ks_prof_begin(ctx, KS_PROF_SCALE, 2.0 * n * s, (double)n, true);
     hipLaunchKernelGGL((k_scale_status<T>), dim3(ew_blocks(n)), dim3(256), 0, ctx->stream, w, n, ctx->d_status, col,
                        ran);
@@ -1279,6 +1317,8 @@ __global__ void k_noalloc_reset(ks_status *st, unsigned *counter) {
     st->prescale = 1.0;
     st->prescale_col = -1;
     st->prescale_pass = 0;
+    st->halt_col = -1;
+    st->halt_need = 0;
     *counter = 0u;
   }
 }
@@ -1378,6 +1418,28 @@ int ks_bv_orthonormalize_column(ks_ctx ctx, ks_prec prec, int n, void *V, int ld
   if (prec == KS_F64)
     return orthonormalize_t<double>(ctx, n, (double *)V, ld, k, d_buf, ldbuf, hcol, col, refine, eta, scale, safe);
   return orthonormalize_t<float>(ctx, n, (float *)V, ld, k, d_buf, ldbuf, hcol, col, refine, eta, scale, safe);
+}
+
+int ks_bv_orthonormalize_column_lean(ks_ctx ctx, ks_prec prec, int n, void *V, int ld, int k, double *d_buf,
+                                     int ldbuf, int hcol, int col, ks_refine refine, double eta, int scale, int safe) {
+  KS_CHECK(ctx && V && d_buf, KS_ERR_ARG, "null orthonormalize argument");
+  KS_CHECK(n > 0 && ld >= n && k >= 0 && k < KS_MAX_COLS && ldbuf > k && hcol >= 0, KS_ERR_ARG,
+           "bad orthonormalize sizes n=%d ld=%d k=%d ldbuf=%d", n, ld, k, ldbuf);
+  KS_CHECK(refine >= KS_REFINE_IFNEEDED && refine <= KS_REFINE_ALWAYS, KS_ERR_ARG, "bad refinement type");
+  if (prec == KS_F64)
+    return orthonormalize_t<double>(ctx, n, (double *)V, ld, k, d_buf, ldbuf, hcol, col, refine, eta, scale, safe, 1);
+  return orthonormalize_t<float>(ctx, n, (float *)V, ld, k, d_buf, ldbuf, hcol, col, refine, eta, scale, safe, 1);
+}
+
+int ks_bv_orthonormalize_resume(ks_ctx ctx, ks_prec prec, int n, void *V, int ld, int k, double *d_buf, int ldbuf,
+                                int hcol, int col, ks_refine refine, double eta) {
+  KS_CHECK(ctx && V && d_buf, KS_ERR_ARG, "null orthonormalize argument");
+  KS_CHECK(n > 0 && ld >= n && k >= 0 && k < KS_MAX_COLS && ldbuf > k && hcol >= 0, KS_ERR_ARG,
+           "bad orthonormalize sizes n=%d ld=%d k=%d ldbuf=%d", n, ld, k, ldbuf);
+  KS_CHECK(refine >= KS_REFINE_IFNEEDED && refine <= KS_REFINE_ALWAYS, KS_ERR_ARG, "bad refinement type");
+  if (prec == KS_F64)
+    return orthonormalize_t<double>(ctx, n, (double *)V, ld, k, d_buf, ldbuf, hcol, col, refine, eta, 1, 0, 2);
+  return orthonormalize_t<float>(ctx, n, (float *)V, ld, k, d_buf, ldbuf, hcol, col, refine, eta, 1, 0, 2);
 }
 
 int ks_bv_dotvec(ks_ctx ctx, ks_prec prec, int n, const void *V, int ld, int k, const void *y, double *d_q) {

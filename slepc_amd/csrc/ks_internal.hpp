@@ -202,6 +202,9 @@ int ks_gs_split_geometry(int n, int k, int *cw, int *rpb, int *nb);   // KS_ERR_
 int ks_csr_blocks(ks_csr A, const int32_t **blk, const int32_t **rowptr, const int32_t **colidx, const void **val,
                   int *nblocks, ks_prec *vprec);
 int ks_ew_blocks(int n);
+// y = A x, skipped on the device when st is given and st->active is 0 (the
+// inner solvers gate their products on a status of their own)
+int ks_spmv_gated(ks_ctx ctx, ks_csr A, const void *x, void *y, ks_prec xprec, const ks_status *st);
 
 // ---------------------------------------------------------------- profiling hooks
 // begin() returns a pointer to the device ran-flag for gated kernels (or null).

@@ -40,6 +40,8 @@ __global__ void k_status_reset(ks_status *st) {
     st->prescale = 1.0;
     st->prescale_col = -1;
     st->prescale_pass = 0;
+    st->halt_col = -1;
+    st->halt_need = 0;
   }
 }
 

@@ -280,11 +280,18 @@ int ks_csr_info(ks_csr A, int *nrows, int64_t *nnz, int *max_row, int *variant, 
 
 int ks_spmv(ks_ctx ctx, ks_csr A, const void *x, void *y, ks_prec xprec, int gate) {
   KS_CHECK(ctx && A && x && y, KS_ERR_ARG, "null SpMV argument");
+  return ks_spmv_gated(ctx, A, x, y, xprec, gate ? ctx->d_status : nullptr);
+}
+
+}  // extern "C"
+
+// y = A x, skipped on the device when st is given and st->active is 0
+int ks_spmv_gated(ks_ctx ctx, ks_csr A, const void *x, void *y, ks_prec xprec, const ks_status *st) {
+  KS_CHECK(ctx && A && x && y, KS_ERR_ARG, "null SpMV argument");
   KS_CHECK(A->nrows == A->ncols || x != y, KS_ERR_ARG, "in-place SpMV");
   const double vs = A->vprec == KS_F64 ? 8.0 : 4.0, xs = xprec == KS_F64 ? 8.0 : 4.0;
   const double bytes = (double)A->nnz * (vs + 4.0) + (A->nrows + 1.0) * 4.0 + A->ncols * xs + A->nrows * xs;
   ks_prof_begin(ctx, KS_PROF_SPMV, bytes, 2.0 * (double)A->nnz, false);
-  const ks_status *st = gate ? ctx->d_status : nullptr;
   dim3 grid(A->nblocks), block(KS_BLOCK);
   if (A->vprec == KS_F64 && xprec == KS_F64)
     hipLaunchKernelGGL((k_spmv_csr<double, double>), grid, block, 0, ctx->stream, A->d_blk, A->d_rowptr,
@@ -302,5 +309,3 @@ int ks_spmv(ks_ctx ctx, ks_csr A, const void *x, void *y, ks_prec xprec, int gat
   KS_HIP(hipGetLastError());
   return KS_OK;
 }
-
-}  // extern "C"

@@ -50,6 +50,17 @@ def initialize(device: int = 0):
         _initialized = True
 
 
+def krylov_extension() -> bool:
+    """Whether BVHIPKS offers its device Krylov expansion (the default; the
+    options database's -bv_hipks_krylov 0, from PETSC_OPTIONS, turns it off)."""
+    opts = os.environ.get("PETSC_OPTIONS", "").split()
+    for i, o in enumerate(opts):
+        if o == "-bv_hipks_krylov":
+            v = opts[i + 1] if i + 1 < len(opts) and not opts[i + 1].startswith("-") else "1"
+            return v.lower() not in ("0", "false", "no")
+    return True
+
+
 def context_handle() -> C.c_void_p:
     initialize()
     return C.c_void_p(lib().SlepcAMDGetDefaultContext())
@@ -303,9 +314,11 @@ class EPS:
     def setExtraction(self, extr: int):
         _chk(lib().EPSSetExtraction(self.h, C.c_int(extr)), "EPSSetExtraction")
 
-    def setST(self, st_type: str = "shift", shift=None, ksp_rtol=None, ksp_maxits=None, pc_type=None):
+    def setST(self, st_type: str = "shift", shift=None, ksp_rtol=None, ksp_maxits=None, pc_type=None,
+              ksp_type=None):
         """STSetType (+ STSetShift) and the inner solver of shift-and-invert
-        (STGetKSP -> KSPSetTolerances, KSPGetPC -> PCSetType)."""
+        (STGetKSP -> KSPSetType ("cg" or "bcgs"), KSPSetTolerances, KSPGetPC ->
+        PCSetType)."""
         st = C.c_void_p()
         _chk(lib().EPSGetST(self.h, C.byref(st)), "EPSGetST")
         _chk(lib().STSetType(st, st_type.encode()), "STSetType")
@@ -314,6 +327,8 @@ class EPS:
         if st_type == "sinvert":
             ksp, pc = C.c_void_p(), C.c_void_p()
             _chk(lib().STGetKSP(st, C.byref(ksp)), "STGetKSP")
+            if ksp_type is not None:
+                _chk(lib().KSPSetType(ksp, ksp_type.encode()), "KSPSetType")
             _chk(lib().KSPSetTolerances(ksp, C.c_double(-1.0 if ksp_rtol is None else ksp_rtol), C.c_double(-1.0),
                                         C.c_double(-1.0), C.c_int(0 if ksp_maxits is None else ksp_maxits)),
                  "KSPSetTolerances")
@@ -375,7 +390,8 @@ class EPS:
 
     def setStepGraphs(self, flg: bool = True):
         """Replay each Arnoldi step from a hipGraph captured once per column."""
-        _chk(lib().BVHIPKSSetStepGraphs(self._bv(), C.c_int(int(flg))), "BVHIPKSSetStepGraphs")
+        self._ensure_type()
+        _chk(lib().EPSKrylovSchurSetStepGraphs(self.h, C.c_int(int(flg))), "EPSKrylovSchurSetStepGraphs")
 
     def setOrthogonalization(self, refine=0, eta=0.7071):
         _chk(lib().BVSetOrthogonalization(self._bv(), C.c_int(0), C.c_int(refine), C.c_double(eta), C.c_int(0)),
@@ -484,7 +500,7 @@ def solve(csr, nev, ncv=-1, tol=1e-8, which=EPS.LARGEST_MAGNITUDE, hermitian=Tru
         eps.setExtraction(extraction)
     if twosided:
         eps.setTwoSided(True)
-    if st is not None:            # dict(st_type="sinvert", shift=..., ksp_rtol=..., pc_type="jacobi")
+    if st is not None:            # dict(st_type="sinvert", shift=..., ksp_type="cg", ksp_rtol=..., pc_type="jacobi")
         eps.setST(**st)
     eps.setKrylovSchurLocking(lock)
     eps.setKrylovSchurRestart(keep)

@@ -303,6 +303,14 @@ PetscErrorCode EPSSetUp(EPS eps)
   return PETSC_SUCCESS;
 }
 
+/* the stream the basis' work runs on */
+static ks_ctx BasisContext(BV V)
+{
+  ks_ctx c = NULL;
+  if (V && V->data && !strcmp(BVTypeName(V), BVHIPKS) && !BVHIPKSGetContext(V, &c)) return c;
+  return SlepcAMDContext();
+}
+
 /* EPSSolve (epssolve.c:119-208), in three parts so a caller can interleave
    several solves: EPSSolveBegin (set-up, start of the restart loop),
    EPSSolveStep (one restart; EPSSolveReady says whether it would wait for the
@@ -319,7 +327,7 @@ PetscErrorCode EPSSolveBegin(EPS eps)
   eps->its = 0;
   eps->nmatmult = 0;
   eps->t_ds = 0.0;
-  eps->V->gs_passes = 0;
+  PetscCall(BVHIPKSResetPasses(eps->V));
   eps->stepping = PETSC_FALSE;
   eps->stepV = eps->V;
   for (i = 0; i < eps->ncv; i++) {
@@ -343,8 +351,10 @@ PetscErrorCode EPSSolveReady(EPS eps, PetscBool *ready)
   PetscCheck(eps && ready, PETSC_ERR_ARG_NULL, "null argument");
   if (eps->stepping) {
     BV V = eps->stepV ? eps->stepV : eps->V;
-    if (V->kry_pending) idle = 0;   /* grouped expansion still being enqueued by its owner */
-    else KSCall(ks_ctx_query(V->ctx, &idle));
+    PetscBool pending;
+    PetscCall(BVHIPKSGroupPending(V, &pending));
+    if (pending) idle = 0;   /* grouped expansion still being enqueued by its owner */
+    else KSCall(ks_ctx_query(BasisContext(V), &idle));
   }
   *ready = idle ? PETSC_TRUE : PETSC_FALSE;
   return PETSC_SUCCESS;
@@ -385,7 +395,7 @@ PetscErrorCode EPSSolveEnd(EPS eps)
     }
   }
   SlepcSortEigenvalues(&eps->sc, eps->nconv, eps->eigr, eps->eigi, eps->perm);
-  KSCall(ks_ctx_sync(eps->V->ctx));
+  KSCall(ks_ctx_sync(BasisContext(eps->V)));
   eps->t_solve = SlepcWallTime() - eps->t_begin;
   /* remove deflation and initial subspaces */
   if (eps->nds) eps->nds = 0;
@@ -526,9 +536,9 @@ PetscErrorCode EPSGetErrorEstimate(EPS eps, PetscInt i, PetscReal *errest)
 static PetscErrorCode EPSSetWorkVecs(EPS eps, PetscInt nw)
 {
   for (PetscInt i = 0; i < nw; i++) {
-    if (eps->work[i] && (eps->work[i]->n != eps->n || eps->work[i]->prec != eps->V->prec))
+    if (eps->work[i] && (eps->work[i]->n != eps->n || eps->work[i]->prec != BVPrecision(eps->V)))
       PetscCall(VecDestroy(&eps->work[i]));
-    if (!eps->work[i]) PetscCall(VecCreateHIP(eps->n, eps->V->prec, &eps->work[i]));
+    if (!eps->work[i]) PetscCall(VecCreateHIP(eps->n, BVPrecision(eps->V), &eps->work[i]));
   }
   return PETSC_SUCCESS;
 }
@@ -640,7 +650,7 @@ PetscErrorCode EPSGetSolveStats(EPS eps, int64_t *nmatmult, int64_t *gs_passes, 
                                 PetscReal *t_solve)
 {
   if (nmatmult) *nmatmult = eps->nmatmult;
-  if (gs_passes) *gs_passes = eps->V ? eps->V->gs_passes : 0;
+  if (gs_passes) PetscCall(BVHIPKSGetPasses(eps->V, gs_passes));
   if (its) *its = eps->its;
   if (t_ds) *t_ds = eps->t_ds;
   if (t_solve) *t_solve = eps->t_solve;

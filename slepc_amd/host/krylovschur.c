@@ -39,8 +39,10 @@ static PetscErrorCode EPSSetDimensions_Default(EPS eps, PetscInt nev, PetscInt *
 /* EPSAllocateSolution (epssetup.c:692-732), extra = 1 */
 static PetscErrorCode EPSAllocateSolution(EPS eps, PetscInt extra)
 {
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
   PetscInt requested = eps->ncv + extra;
   BV V;
+  uint64_t seed;
   free(eps->eigr);
   free(eps->eigi);
   free(eps->errest);
@@ -50,32 +52,25 @@ static PetscErrorCode EPSAllocateSolution(EPS eps, PetscInt extra)
   PetscCall(PetscMallocZero(sizeof(PetscReal) * (size_t)requested, &eps->errest));
   PetscCall(PetscMallocZero(sizeof(PetscInt) * (size_t)requested, &eps->perm));
   PetscCall(EPSGetBV(eps, &V));
-  if (V->data && (V->m != requested || V->n != eps->n || V->nc)) {
-    /* BVResize without copy: rebuild the storage for the new sizes */
-    PetscCall(V->ops.destroy(V));
-    V->data = NULL;
-    V->nc = 0;
-  }
-  if (!V->type[0]) PetscCall(BVSetType(V, BVHIPKS));
-  if (!V->data) {
-    V->n = eps->n;
-    V->N = eps->n;
-    V->m = requested;
-    V->l = 0;
-    V->k = requested;
-    V->ci[0] = V->ci[1] = BV_NOCOL;
-    PetscCall(V->ops.create(V));
+  if (!BVTypeName(V)[0]) PetscCall(BVSetType(V, BVHIPKS));
+  /* BVResize without copy: the storage is rebuilt for the new sizes */
+  if (!V->data || V->m != requested || V->n != eps->n || V->nc) PetscCall(BVResizeDiscard_Private(V, eps->n, requested));
+  /* the device-path settings of this solver on its basis */
+  if (!strcmp(BVTypeName(V), BVHIPKS)) {
+    PetscCall(BVHIPKSSetStepGraphs(V, ctx->step_graphs));
+    PetscCall(BVHIPKSSetGrouped(V, ctx->grouped));
   }
   /* W: a basis like V for the two-sided variant (epssetup.c:726-731) */
-  if (eps->W && (!eps->twosided || eps->W->m != requested || eps->W->n != eps->n || eps->W->prec != V->prec))
+  if (eps->W && (!eps->twosided || eps->W->m != requested || eps->W->n != eps->n || strcmp(eps->W->vtype, V->vtype)))
     PetscCall(BVDestroy(&eps->W));
   if (eps->twosided && !eps->W) {
     PetscCall(BVCreate(&eps->W));
-    PetscCall(BVSetSizes(eps->W, eps->n, eps->n, requested));
-    if (V->prec != SLEPC_AMD_F64) PetscCall(BVHIPKSSetPrecision(eps->W, V->prec));
+    PetscCall(BVSetVecType(eps->W, V->vtype));
     PetscCall(BVSetOrthogonalization(eps->W, V->orthog_type, V->orthog_ref, V->orthog_eta, V->orthog_block));
-    PetscCall(BVSetType(eps->W, V->type));
-    eps->W->rseed = V->rseed + KS_LEFT_SEED_OFFSET;
+    PetscCall(BVGetRandomSeed(V, &seed));
+    PetscCall(BVSetRandomSeed(eps->W, seed + KS_LEFT_SEED_OFFSET));
+    PetscCall(BVSetSizes(eps->W, eps->n, eps->n, requested));
+    PetscCall(BVSetType(eps->W, BVTypeName(V)));
   }
   return PETSC_SUCCESS;
 }
@@ -92,7 +87,7 @@ static PetscErrorCode EPSSetUp_KrylovSchur(EPS eps)
   if (STIsSinvert(eps->st)) {
     PetscCheck(eps->extraction == EPS_RITZ && !eps->twosided && ctx->mixed_tol <= 0.0, PETSC_ERR_SUP,
                "shift-and-invert runs with Ritz extraction, one-sided, in FP64");
-    PetscCheck(!eps->V || eps->V->prec == SLEPC_AMD_F64, PETSC_ERR_SUP, "shift-and-invert needs an FP64 basis");
+    PetscCheck(!eps->V || BVPrecision(eps->V) == SLEPC_AMD_F64, PETSC_ERR_SUP, "shift-and-invert needs an FP64 basis");
   }
   PetscCheck(ctx->lock || eps->mpd >= eps->ncv, PETSC_ERR_SUP, "Should not use mpd parameter in non-locking variant");
   PetscCheck(eps->extraction == EPS_RITZ || eps->extraction == EPS_HARMONIC, PETSC_ERR_SUP, "Unsupported extraction type");
@@ -150,10 +145,93 @@ static PetscErrorCode KSExpand(EPS eps)
   PetscCall(DSSetDimensions(eps->ds, nv, eps->nconv, eps->nconv + ctx->l));
   PetscCall(STGetOperator(eps->st, &Op));
   eps->nmatmult += nv - (eps->nconv + ctx->l);
-  PetscCall(BVMatKrylovBegin(eps->V, Op, eps->nconv + ctx->l, nv));
+  PetscCall(BVGetKrylovOps(eps->V, &ctx->kry_ext));
+  if (ctx->kry_ext) PetscCall(ctx->kry_ext->matkrylovbegin(eps->V, Op, eps->nconv + ctx->l, nv));
+  else {   /* the reference's per-step loop, finished here; KSAfterExpand collects its results */
+    ctx->kry_m = nv;
+    PetscCall(BVMatKrylov(eps->V, Op, eps->nconv + ctx->l, &ctx->kry_m, &ctx->kry_beta, &ctx->kry_breakdown));
+  }
   PetscCall(STRestoreOperator(eps->st, &Op));
   ctx->nv = nv;
   ctx->phase = KS_PHASE_EXPANDING;
+  return PETSC_SUCCESS;
+}
+
+/* The hand-over of the mixed-precision form.  At the first phase's last
+   restart the kept columns satisfy the Krylov-Schur relation
+       A V_m = V_m B + v_{m+1} b^T      (B: m x m Schur form, b: the extra row)
+   and span(V_m) is the Krylov space of one vector: reduced to Arnoldi form
+   (Stewart, "A Krylov-Schur algorithm for large eigenproblems", SIAM J. Matrix
+   Anal. Appl. 23 (2001), Sec. 2), A V_m Q = V_m Q H + beta v_{m+1} e_m^T with H
+   upper Hessenberg, and that vector is u = V_m Q e_1.  One reflector takes b to
+   beta e_m, then for rows r = m-1..2 a reflector on the leading r indices
+   zeroes B(r, 0:r-2) without touching index m-1.  q1 = Q e_1 is stored; the
+   FP64 phase restarts from u, so its Krylov space holds the whole retained
+   subspace, not only the converged vectors. */
+static void HouseApply(PetscInt m, PetscInt r, const PetscScalar *v, PetscScalar *B, PetscScalar *Q)
+{
+  /* P = I - 2 v v^T / (v^T v) on indices 0..r-1: B = P B P, Q = Q P */
+  PetscScalar vv = 0.0;
+  for (PetscInt i = 0; i < r; i++) vv += v[i] * v[i];
+  if (vv == 0.0) return;
+  for (PetscInt j = 0; j < m; j++) {   /* B(0:r, j) = P B(0:r, j) */
+    PetscScalar t = 0.0;
+    for (PetscInt i = 0; i < r; i++) t += v[i] * B[i + j * m];
+    t *= 2.0 / vv;
+    for (PetscInt i = 0; i < r; i++) B[i + j * m] -= t * v[i];
+  }
+  for (PetscInt i = 0; i < m; i++) {   /* B(i, 0:r) = B(i, 0:r) P, Q(i, 0:r) = Q(i, 0:r) P */
+    PetscScalar t = 0.0, u = 0.0;
+    for (PetscInt j = 0; j < r; j++) t += B[i + j * m] * v[j], u += Q[i + j * m] * v[j];
+    t *= 2.0 / vv;
+    u *= 2.0 / vv;
+    for (PetscInt j = 0; j < r; j++) B[i + j * m] -= t * v[j], Q[i + j * m] -= u * v[j];
+  }
+}
+
+/* v = x + sign(x_p) |x| e_p (the reflector taking x to a multiple of e_p) */
+static void HouseVector(PetscInt r, const PetscScalar *x, PetscInt p, PetscScalar *v)
+{
+  PetscScalar nx = 0.0;
+  for (PetscInt i = 0; i < r; i++) v[i] = x[i], nx += x[i] * x[i];
+  nx = sqrt(nx);
+  if (nx == 0.0) return;
+  v[p] += x[p] >= 0.0 ? nx : -nx;
+}
+
+static PetscErrorCode KSHandoverVector(EPS eps, PetscInt m, PetscInt nv)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  DS ds = eps->ds;
+  const PetscInt ld = ds->ld;
+  PetscScalar *B = NULL, *Q = NULL, *v = NULL, *x = NULL;
+  PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)m * (size_t)m * 2 + sizeof(PetscScalar) * 2 * (size_t)(m + 1), &B));
+  Q = B + (size_t)m * m;
+  v = Q + (size_t)m * m;
+  x = v + m + 1;
+  if (eps->ishermitian) {   /* DSHEP: B = diag(theta), b = the updated arrow (column 1 of T) */
+    const PetscReal *e = ds->mat[DS_MAT_T] + ld;
+    for (PetscInt i = 0; i < m; i++) B[i + i * m] = eps->eigr[i], x[i] = e[i];
+  } else {                  /* DSNHEP: B = the leading Schur block, b = row nv of A */
+    const PetscScalar *A = ds->mat[DS_MAT_A];
+    for (PetscInt j = 0; j < m; j++) {
+      for (PetscInt i = 0; i < m; i++) B[i + j * m] = A[i + (size_t)j * ld];
+      x[j] = A[nv + (size_t)j * ld];
+    }
+  }
+  for (PetscInt i = 0; i < m; i++) Q[i + i * m] = 1.0;
+  HouseVector(m, x, m - 1, v);
+  HouseApply(m, m, v, B, Q);
+  for (PetscInt r = m - 1; r >= 2; r--) {
+    for (PetscInt j = 0; j < r; j++) x[j] = B[r + j * m];
+    HouseVector(r, x, r - 1, v);
+    HouseApply(m, r, v, B, Q);
+  }
+  free(ctx->ho_q1);
+  PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)m, &ctx->ho_q1));
+  for (PetscInt i = 0; i < m; i++) ctx->ho_q1[i] = Q[i];
+  ctx->ho_m = m;
+  free(B);
   return PETSC_SUCCESS;
 }
 
@@ -164,10 +242,15 @@ static PetscErrorCode KSAfterExpand(EPS eps)
   Mat U, H, T;
   PetscReal beta = 0.0;
   PetscReal gamma = 1.0;
-  PetscBool breakdown = PETSC_FALSE, hermitian = eps->ishermitian && !ctx->harmonic;
+  PetscBool breakdown = PETSC_FALSE, hermitian = eps->ishermitian && !ctx->harmonic, handover;
   double t0;
 
-  PetscCall(BVMatKrylovEnd(eps->V, eps->nconv + l, &nv, &beta, &breakdown));
+  if (ctx->kry_ext) PetscCall(ctx->kry_ext->matkrylovend(eps->V, eps->nconv + l, &nv, &beta, &breakdown));
+  else {
+    nv = ctx->kry_m;
+    beta = ctx->kry_beta;
+    breakdown = ctx->kry_breakdown;
+  }
   if (hermitian) {
     PetscCall(DSGetMat(eps->ds, DS_MAT_T, &T));
     PetscCall(BVCopyTridiagonal(eps->V, T, eps->nconv + l, nv));
@@ -197,16 +280,23 @@ static PetscErrorCode KSAfterExpand(EPS eps)
   if (k >= eps->nev) eps->reason = EPS_CONVERGED_TOL;
   else if (eps->its >= eps->max_it) eps->reason = EPS_DIVERGED_ITS;
   nconv = k;
+  /* the first phase of the mixed form restarts once more at convergence and
+     hands its kept subspace over (KSHandoverVector) */
+  handover = ctx->handover && eps->reason == EPS_CONVERGED_TOL && !breakdown && k < nv;
 
   /* update l */
-  if (eps->reason != EPS_CONVERGED_ITERATING || breakdown || k == nv) l = 0;
+  if ((eps->reason != EPS_CONVERGED_ITERATING && !handover) || breakdown || k == nv) l = 0;
   else {
     l = PetscMax(1, (PetscInt)((nv - k) * ctx->keep));
     if (!hermitian) PetscCall(DSGetTruncateSize(eps->ds, k, nv, &l));
   }
   if (!ctx->lock && l > 0) { l += k; k = 0; }   /* non-locking variant: reset no. of converged pairs */
+  if (handover) {
+    PetscCall(KSHandoverVector(eps, k + l, nv));
+    ctx->ho_nconv = nconv;
+  }
 
-  if (eps->reason == EPS_CONVERGED_ITERATING) {
+  if (eps->reason == EPS_CONVERGED_ITERATING || handover) {
     if (breakdown || k == nv) {
       /* start a new Arnoldi factorization */
       if (k < eps->nev) {
@@ -233,7 +323,7 @@ static PetscErrorCode KSAfterExpand(EPS eps)
   PetscCall(BVMultInPlace(eps->V, U, eps->nconv, k + l));
   PetscCall(DSRestoreMat(eps->ds, DS_MAT_Q, &U));
 
-  if (eps->reason == EPS_CONVERGED_ITERATING && !breakdown) PetscCall(BVCopyColumn(eps->V, nv, k + l));
+  if ((eps->reason == EPS_CONVERGED_ITERATING || handover) && !breakdown) PetscCall(BVCopyColumn(eps->V, nv, k + l));
   eps->nconv = k;
   ctx->l = l;
   if (eps->monitor) PetscCall(eps->monitor(eps, eps->its, nconv, eps->eigr, eps->eigi, eps->errest, nv, eps->monitorctx));
@@ -257,10 +347,12 @@ static PetscErrorCode KSStart(EPS eps)
 
 /* Mixed-precision first phase: the same Krylov-Schur on an FP32 basis (FP32
    SpMV on an FP32 copy of the operator values, FP32 Gram-Schmidt products with
-   FP64 DGKS scalars and an FP64 projected problem) down to ctx->mixed_tol.  Its
-   converged Ritz vectors are summed into the FP64 start vector of the main
-   solve (column 0 of V, used as the initial space), so the FP64 restarts start
-   from a vector whose unwanted components are already at the FP32 tolerance.
+   FP64 DGKS scalars and an FP64 projected problem) down to ctx->mixed_tol.  At
+   convergence it restarts once more and its kept Krylov-Schur decomposition,
+   reduced to Arnoldi form, gives the one vector u whose Krylov space is the
+   kept subspace; u becomes the FP64 start vector of the main solve (column 0
+   of V, used as the initial space), so the FP64 restarts continue from the
+   whole retained subspace at the FP32 accuracy.
    The phase is itself stepped (MixedBegin, then EPSSolveStep of the FP32 solve
    until it is done, then MixedEnd), so a serving loop never blocks on it. */
 static PetscErrorCode MixedBegin(EPS eps)
@@ -282,13 +374,30 @@ static PetscErrorCode MixedBegin(EPS eps)
   c32 = (EPS_KRYLOVSCHUR *)e32->data;
   c32->keep = ctx->keep;
   c32->lock = ctx->lock;
+  {
+    /* the start vector the FP64 phase gets: "sum" of the converged Ritz vectors,
+       or "arnoldi", the start of the Arnoldi form of the last restart's kept
+       subspace (non-locking, so that decomposition couples every kept column) */
+    char ho[16] = "sum";
+    PetscBool set;
+    PetscCall(PetscOptionsGetString("-eps_krylovschur_mixed_handover", ho, sizeof(ho), &set));
+    PetscCheck(!strcmp(ho, "sum") || !strcmp(ho, "arnoldi"), PETSC_ERR_ARG_WRONG, "unknown hand-over %s", ho);
+    if (!strcmp(ho, "arnoldi")) {
+      c32->lock = PETSC_FALSE;
+      c32->handover = PETSC_TRUE;
+    }
+  }
   PetscCall(EPSGetBV(e32, &bv32));
   PetscCall(BVSetType(bv32, BVHIPKS));
   PetscCall(BVHIPKSSetPrecision(bv32, SLEPC_AMD_F32));
   bv32->orthog_ref = eps->V->orthog_ref;
   bv32->orthog_eta = eps->V->orthog_eta;
-  bv32->rseed = eps->V->rseed;
-  bv32->step_graphs = eps->V->step_graphs;
+  {
+    uint64_t seed;
+    PetscCall(BVGetRandomSeed(eps->V, &seed));
+    PetscCall(BVSetRandomSeed(bv32, seed));
+  }
+  c32->step_graphs = ctx->step_graphs;
   /* same start: the FP64 initial vector inserted by EPSSetUp, rounded to FP32 */
   if (eps->nini > 0) {
     PetscCall(VecCreateHIP(eps->n, SLEPC_AMD_F32, &v64));
@@ -316,34 +425,41 @@ static PetscErrorCode MixedBegin(EPS eps)
 
 static PetscErrorCode MixedEnd(EPS eps)
 {
-  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data, *c32;
   EPS e32 = ctx->e32;
-  Vec acc = NULL, xr = NULL, xi = NULL;
-  PetscInt nconv32, nuse;
+  Vec u = NULL, xr = NULL, xi = NULL;
+  PetscInt nuse;
 
   PetscCall(EPSSolveEnd(e32));
-  nconv32 = e32->nconv;
+  c32 = (EPS_KRYLOVSCHUR *)e32->data;
   ctx->its32 = e32->its;
-  ctx->nconv32 = nconv32;
+  ctx->nconv32 = c32->ho_q1 ? c32->ho_nconv : e32->nconv;
   ctx->nmatmult32 = e32->nmatmult;
-  ctx->passes32 = e32->V->gs_passes;
-  nuse = PetscMin(nconv32, eps->nev);
-  if (nuse > 0) {
-    PetscCall(VecCreateHIP(eps->n, SLEPC_AMD_F32, &acc));
+  PetscCall(BVHIPKSGetPasses(e32->V, &ctx->passes32));
+  nuse = PetscMin(e32->nconv, eps->nev);
+  if (c32->ho_q1) {   /* u = V32(:, 0:m) q1, the start of the Arnoldi form, in FP64 */
+    PetscCall(BVSetActiveColumns(e32->V, 0, c32->ho_m));
+    PetscCall(VecCreateHIP(eps->n, SLEPC_AMD_F32, &u));
+    PetscCall(BVMultVec(e32->V, 1.0, 0.0, u, c32->ho_q1));
+    PetscCall(BVInsertVec(eps->V, 0, u));   /* FP32 -> FP64 conversion */
+    eps->nini = 1;
+  } else if (nuse > 0) {   /* the sum of the converged Ritz vectors, a conjugate pair by both parts */
+    PetscCall(VecCreateHIP(eps->n, SLEPC_AMD_F32, &u));
     PetscCall(VecCreateHIP(eps->n, SLEPC_AMD_F32, &xr));
     PetscCall(VecCreateHIP(eps->n, SLEPC_AMD_F32, &xi));
     for (PetscInt i = 0; i < nuse; i++) {
       PetscScalar kr, ki;
       PetscCall(EPSGetEigenpair(e32, i, &kr, &ki, xr, xi));
-      PetscCall(VecAXPY(acc, 1.0, xr));
-      if (ki != 0.0) PetscCall(VecAXPY(acc, 1.0, xi));
+      PetscCall(VecAXPY(u, 1.0, xr));
+      if (ki > 0.0) PetscCall(VecAXPY(u, 1.0, xi));   /* the pair's real and imaginary parts once */
+      if (ki != 0.0) i++;
     }
-    PetscCall(BVInsertVec(eps->V, 0, acc));   /* FP32 -> FP64 conversion */
+    PetscCall(BVInsertVec(eps->V, 0, u));   /* FP32 -> FP64 conversion */
     eps->nini = 1;
-    PetscCall(VecDestroy(&acc));
-    PetscCall(VecDestroy(&xr));
-    PetscCall(VecDestroy(&xi));
   }
+  PetscCall(VecDestroy(&u));
+  PetscCall(VecDestroy(&xr));
+  PetscCall(VecDestroy(&xi));
   PetscCall(EPSDestroy(&ctx->e32));
   ctx->t32 = SlepcWallTime() - ctx->t32_start;
   eps->stepV = eps->V;
@@ -373,8 +489,7 @@ static PetscErrorCode TwoSidedExpand(EPS eps, BV V, Mat Op, DSMatType mat, Petsc
 {
   Mat H;
   eps->nmatmult += *nv - k;
-  PetscCall(BVMatKrylovBegin(V, Op, k, *nv));
-  PetscCall(BVMatKrylovEnd(V, k, nv, beta, breakdown));
+  PetscCall(BVMatKrylov(V, Op, k, nv, beta, breakdown));
   PetscCall(DSGetMat(eps->ds, mat, &H));
   PetscCall(BVCopyHessenberg(V, H, k, *nv));
   PetscCall(DSRestoreMat(eps->ds, mat, &H));
@@ -572,7 +687,7 @@ static PetscErrorCode EPSSolveBegin_KrylovSchur(EPS eps)
   ctx->nmatmult32 = ctx->passes32 = 0;
   ctx->t32 = 0.0;
   if (eps->twosided) return EPSSolve_KrylovSchur_TwoSided(eps);   /* runs to the end */
-  if (ctx->mixed_tol > 0.0 && ctx->mixed_tol > eps->tol && eps->V->prec == SLEPC_AMD_F64) return MixedBegin(eps);
+  if (ctx->mixed_tol > 0.0 && ctx->mixed_tol > eps->tol && BVPrecision(eps->V) == SLEPC_AMD_F64) return MixedBegin(eps);
   return KSStart(eps);
 }
 
@@ -606,6 +721,25 @@ PetscErrorCode EPSKrylovSchurSetMixedPrecision(EPS eps, PetscReal tol32)
   PetscCheck(!strcmp(eps->type, EPSKRYLOVSCHUR), PETSC_ERR_ARG_WRONG, "not a Krylov-Schur solver");
   ctx->mixed_tol = tol32 > 0.0 ? tol32 : 0.0;
   eps->state = EPS_STATE_INITIAL;
+  return PETSC_SUCCESS;
+}
+
+/* replay each Arnoldi step from a captured graph (the basis' setting, applied at set-up) */
+PetscErrorCode EPSKrylovSchurSetStepGraphs(EPS eps, PetscBool flg)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  PetscCheck(!strcmp(eps->type, EPSKRYLOVSCHUR), PETSC_ERR_ARG_WRONG, "not a Krylov-Schur solver");
+  ctx->step_graphs = flg;
+  if (eps->V && eps->V->data && !strcmp(BVTypeName(eps->V), BVHIPKS)) PetscCall(BVHIPKSSetStepGraphs(eps->V, flg));
+  return PETSC_SUCCESS;
+}
+
+/* the serving loop's grouped expansions (applied to the basis at set-up) */
+PetscErrorCode EPSKrylovSchurSetGrouped_Private(EPS eps, PetscBool flg)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  PetscCheck(!strcmp(eps->type, EPSKRYLOVSCHUR), PETSC_ERR_ARG_WRONG, "not a Krylov-Schur solver");
+  ctx->grouped = flg;
   return PETSC_SUCCESS;
 }
 
@@ -680,6 +814,8 @@ static PetscErrorCode EPSDestroy_KrylovSchur(EPS eps)
   if (ctx) {
     free(ctx->g);
     ctx->g = NULL;
+    free(ctx->ho_q1);
+    ctx->ho_q1 = NULL;
     free(ctx->M);
     ctx->M = NULL;
     PetscCall(MatDestroy(&ctx->AT));

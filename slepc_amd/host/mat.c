@@ -18,6 +18,7 @@ PetscErrorCode MatCreate(Mat *A)
 {
   PetscCall(SlepcCheckInitialized());
   PetscCall(PetscMallocZero(sizeof(struct _p_Mat), A));
+  PetscCall(PetscHeaderInit(&(*A)->hdr, "Mat"));
   return PETSC_SUCCESS;
 }
 
@@ -34,13 +35,13 @@ PetscErrorCode MatSetType(Mat A, MatType type)
 {
   PetscErrorCode (*create)(Mat);
   PetscCheck(A && type, PETSC_ERR_ARG_NULL, "null argument");
-  if (!strcmp(A->type, type)) return PETSC_SUCCESS;
+  if (!strcmp(A->hdr.type_name, type)) return PETSC_SUCCESS;
   create = (PetscErrorCode(*)(Mat))SlepcFunctionListFind(&MatList, type);
   if (!create && !strcmp(type, MATSEQDENSE)) SETERRQ(PETSC_ERR_ARG_WRONG, "use MatCreateSeqDense for %s", type);
   PetscCheck(create, PETSC_ERR_ARG_WRONG, "unknown Mat type %s", type);
-  if (A->ops.destroy) PetscCall(A->ops.destroy(A));
-  memset(&A->ops, 0, sizeof(A->ops));
-  strcpy(A->type, type);
+  if (A->ops->destroy) PetscCall(A->ops->destroy(A));
+  memset(A->ops, 0, sizeof(struct _MatOps));
+  snprintf(A->hdr.type_name, sizeof(A->hdr.type_name), "%s", type);
   PetscCall(create(A));
   return PETSC_SUCCESS;
 }
@@ -48,7 +49,7 @@ PetscErrorCode MatSetType(Mat A, MatType type)
 PetscErrorCode MatGetType(Mat A, MatType *type)
 {
   PetscCheck(A, PETSC_ERR_ARG_NULL, "null matrix");
-  *type = A->type;
+  *type = A->hdr.type_name;
   return PETSC_SUCCESS;
 }
 
@@ -63,27 +64,27 @@ PetscErrorCode MatGetSize(Mat A, PetscInt *M, PetscInt *N)
 PetscErrorCode MatMult(Mat A, Vec x, Vec y)
 {
   PetscCheck(A && x && y, PETSC_ERR_ARG_NULL, "null argument");
-  PetscCheck(A->ops.mult, PETSC_ERR_SUP, "Mat type %s has no MatMult", A->type);
+  PetscCheck(A->ops->mult, PETSC_ERR_SUP, "Mat type %s has no MatMult", A->hdr.type_name);
   PetscCheck(x->n == A->n && y->n == A->m, PETSC_ERR_ARG_SIZ, "MatMult sizes: A %dx%d, x %d, y %d", A->m, A->n,
              x->n, y->n);
   PetscCheck(x != y, PETSC_ERR_ARG_WRONG, "x and y must be different vectors");
-  return A->ops.mult(A, x, y);
+  return A->ops->mult(A, x, y);
 }
 
 PetscErrorCode MatMultTranspose(Mat A, Vec x, Vec y)
 {
   PetscCheck(A && x && y, PETSC_ERR_ARG_NULL, "null argument");
-  PetscCheck(A->ops.multtranspose, PETSC_ERR_SUP, "Mat type %s has no MatMultTranspose", A->type);
+  PetscCheck(A->ops->multtranspose, PETSC_ERR_SUP, "Mat type %s has no MatMultTranspose", A->hdr.type_name);
   PetscCheck(x->n == A->m && y->n == A->n, PETSC_ERR_ARG_SIZ, "MatMultTranspose sizes: A %dx%d, x %d, y %d", A->m,
              A->n, x->n, y->n);
   PetscCheck(x != y, PETSC_ERR_ARG_WRONG, "x and y must be different vectors");
-  return A->ops.multtranspose(A, x, y);
+  return A->ops->multtranspose(A, x, y);
 }
 
 PetscErrorCode MatDestroy(Mat *A)
 {
   if (!A || !*A) return PETSC_SUCCESS;
-  if ((*A)->ops.destroy) PetscCall((*A)->ops.destroy(*A));
+  if ((*A)->ops->destroy) PetscCall((*A)->ops->destroy(*A));
   free(*A);
   *A = NULL;
   return PETSC_SUCCESS;
@@ -106,7 +107,7 @@ PetscErrorCode MatCreateSeqDenseView(PetscInt m, PetscInt n, PetscScalar *data, 
   PetscCall(PetscMallocZero(sizeof(Mat_SeqDense), &d));
   (*A)->m = m;
   (*A)->n = n;
-  strcpy((*A)->type, MATSEQDENSE);
+  snprintf((*A)->hdr.type_name, sizeof((*A)->hdr.type_name), "%s", MATSEQDENSE);
   d->lda = lda;
   if (data) d->v = data;
   else {
@@ -114,7 +115,7 @@ PetscErrorCode MatCreateSeqDenseView(PetscInt m, PetscInt n, PetscScalar *data, 
     d->owns = PETSC_TRUE;
   }
   (*A)->data = d;
-  (*A)->ops.destroy = MatDestroy_SeqDense;
+  (*A)->ops->destroy = MatDestroy_SeqDense;
   return PETSC_SUCCESS;
 }
 
@@ -125,7 +126,7 @@ PetscErrorCode MatCreateSeqDense(PetscInt m, PetscInt n, PetscScalar *data, Mat 
 
 PetscErrorCode MatDenseGetArray(Mat A, PetscScalar **array)
 {
-  PetscCheck(A && !strcmp(A->type, MATSEQDENSE), PETSC_ERR_ARG_WRONG, "not a seqdense matrix");
+  PetscCheck(A && !strcmp(A->hdr.type_name, MATSEQDENSE), PETSC_ERR_ARG_WRONG, "not a seqdense matrix");
   *array = ((Mat_SeqDense *)A->data)->v;
   return PETSC_SUCCESS;
 }
@@ -139,7 +140,7 @@ PetscErrorCode MatDenseRestoreArray(Mat A, PetscScalar **array)
 
 PetscErrorCode MatDenseGetLDA(Mat A, PetscInt *lda)
 {
-  PetscCheck(A && !strcmp(A->type, MATSEQDENSE), PETSC_ERR_ARG_WRONG, "not a seqdense matrix");
+  PetscCheck(A && !strcmp(A->hdr.type_name, MATSEQDENSE), PETSC_ERR_ARG_WRONG, "not a seqdense matrix");
   *lda = ((Mat_SeqDense *)A->data)->lda;
   return PETSC_SUCCESS;
 }
@@ -163,7 +164,7 @@ static PetscErrorCode MatMult_HIPCSR(Mat A, Vec x, Vec y)
   PetscCheck(h->csr, PETSC_ERR_ORDER, "MatSeqAIJSetPreallocationCSR must be called first");
   PetscCheck(x->prec == y->prec, PETSC_ERR_ARG_WRONG, "x and y precisions differ");
   if (x->prec == SLEPC_AMD_F32) PetscCall(MatHIPCSRGetF32(A, &csr));
-  KSCall(ks_spmv(SlepcAMDContext(), csr, x->d, y->d, x->prec == SLEPC_AMD_F64 ? KS_F64 : KS_F32, A->gate));
+  KSCall(ks_spmv(SlepcAMDContext(), csr, x->d, y->d, x->prec == SLEPC_AMD_F64 ? KS_F64 : KS_F32, ((Mat_HIPCSR *)A->data)->gate));
   return PETSC_SUCCESS;
 }
 
@@ -185,7 +186,7 @@ static PetscErrorCode MatMultTranspose_HIPCSR(Mat A, Vec x, Vec y)
 PetscErrorCode MatHIPCSRCreateTranspose(Mat A, Mat *AT)
 {
   Mat_HIPCSR *h, *t;
-  PetscCheck(A && AT && !strcmp(A->type, MATHIPCSR), PETSC_ERR_ARG_WRONG, "not a %s matrix", MATHIPCSR);
+  PetscCheck(A && AT && !strcmp(A->hdr.type_name, MATHIPCSR), PETSC_ERR_ARG_WRONG, "not a %s matrix", MATHIPCSR);
   h = (Mat_HIPCSR *)A->data;
   PetscCheck(h->csr, PETSC_ERR_ORDER, "matrix has no values");
   PetscCall(MatCreate(AT));
@@ -213,9 +214,9 @@ PetscErrorCode MatCreate_HIPCSR(Mat A)
   Mat_HIPCSR *h;
   PetscCall(PetscMallocZero(sizeof(Mat_HIPCSR), &h));
   A->data = h;
-  A->ops.mult = MatMult_HIPCSR;
-  A->ops.multtranspose = MatMultTranspose_HIPCSR;
-  A->ops.destroy = MatDestroy_HIPCSR;
+  A->ops->mult = MatMult_HIPCSR;
+  A->ops->multtranspose = MatMultTranspose_HIPCSR;
+  A->ops->destroy = MatDestroy_HIPCSR;
   return PETSC_SUCCESS;
 }
 
@@ -224,7 +225,7 @@ static PetscErrorCode MatHIPCSRSetHost(Mat A, const PetscInt i[], const PetscInt
 {
   Mat_HIPCSR *h;
   PetscCheck(A && i, PETSC_ERR_ARG_NULL, "null argument");
-  PetscCheck(!strcmp(A->type, MATHIPCSR), PETSC_ERR_ARG_WRONG, "matrix type is %s, expected %s", A->type, MATHIPCSR);
+  PetscCheck(!strcmp(A->hdr.type_name, MATHIPCSR), PETSC_ERR_ARG_WRONG, "matrix type is %s, expected %s", A->hdr.type_name, MATHIPCSR);
   PetscCheck(A->m > 0 && A->n > 0, PETSC_ERR_ORDER, "MatSetSizes must be called first");
   h = (Mat_HIPCSR *)A->data;
   if (h->csr) { KSCall(ks_csr_destroy(h->csr)); h->csr = NULL; }
@@ -251,7 +252,7 @@ PetscErrorCode MatHIPCSRSetDeviceArrays(Mat A, PetscInt nnz, const PetscInt *d_i
 {
   Mat_HIPCSR *h;
   PetscCheck(A && d_i, PETSC_ERR_ARG_NULL, "null argument");
-  PetscCheck(!strcmp(A->type, MATHIPCSR), PETSC_ERR_ARG_WRONG, "matrix type is %s, expected %s", A->type, MATHIPCSR);
+  PetscCheck(!strcmp(A->hdr.type_name, MATHIPCSR), PETSC_ERR_ARG_WRONG, "matrix type is %s, expected %s", A->hdr.type_name, MATHIPCSR);
   h = (Mat_HIPCSR *)A->data;
   if (h->csr) { KSCall(ks_csr_destroy(h->csr)); h->csr = NULL; }
   if (h->csr32) { KSCall(ks_csr_destroy(h->csr32)); h->csr32 = NULL; }
@@ -267,7 +268,7 @@ PetscErrorCode MatHIPCSRGetInfo(Mat A, PetscInt *nnz, PetscInt *max_row, PetscIn
   Mat_HIPCSR *h;
   int64_t z;
   int mr, var, nb;
-  PetscCheck(A && !strcmp(A->type, MATHIPCSR), PETSC_ERR_ARG_WRONG, "not a %s matrix", MATHIPCSR);
+  PetscCheck(A && !strcmp(A->hdr.type_name, MATHIPCSR), PETSC_ERR_ARG_WRONG, "not a %s matrix", MATHIPCSR);
   h = (Mat_HIPCSR *)A->data;
   PetscCheck(h->csr, PETSC_ERR_ORDER, "matrix has no values");
   KSCall(ks_csr_info(h->csr, NULL, &z, &mr, &var, &nb));

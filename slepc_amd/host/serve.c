@@ -155,7 +155,7 @@ static PetscErrorCode RequestAdmit(SlepcServer s, PetscInt id, int slot)
   if (q->in.mixed_tol > 0.0) PetscCall(EPSKrylovSchurSetMixedPrecision(q->eps, q->in.mixed_tol));
   PetscCall(EPSGetBV(q->eps, &V));
   PetscCall(BVSetType(V, BVHIPKS));
-  V->grouped = s->grouped;
+  PetscCall(EPSKrylovSchurSetGrouped_Private(q->eps, s->grouped));
   if (q->in.v0) {
     PetscCall(VecCreateHIP(q->in.n, SLEPC_AMD_F64, &v0));
     PetscCall(VecSetValuesHost(v0, q->in.v0));
@@ -166,6 +166,33 @@ static PetscErrorCode RequestAdmit(SlepcServer s, PetscInt id, int slot)
   q->state = REQ_ACTIVE;
   s->slot_req[slot] = id;
   s->nactive++;
+  return PETSC_SUCCESS;
+}
+
+/* a request whose admission failed: whatever RequestAdmit built is destroyed,
+   its reserved bytes go back to the budget and it finishes with the error code
+   in its result, so the queue moves on */
+static PetscErrorCode RequestAbort(SlepcServer s, PetscInt id, PetscErrorCode code)
+{
+  Request *q = &s->req[id];
+  if (q->slot >= 0) KSCall(ks_ctx_sync(s->slot_ctx[q->slot]));
+  PetscCall(EPSDestroy(&q->eps));
+  PetscCall(MatDestroy(&q->A));
+  if (q->slot >= 0 && s->slot_req[q->slot] == id) {
+    s->slot_req[q->slot] = -1;
+    s->nactive--;
+  }
+  q->slot = -1;
+  s->reserved -= q->bytes;
+  q->state = REQ_FAILED;
+  q->res.error = code;
+  q->res.reason = EPS_DIVERGED_BREAKDOWN;
+  q->res.t_done = SlepcWallTime();
+  s->nfinished++;
+  q->in.rowptr = q->in.colidx = NULL;
+  q->in.val = NULL;
+  q->in.val32 = NULL;
+  q->in.v0 = NULL;
   return PETSC_SUCCESS;
 }
 
@@ -180,7 +207,7 @@ static PetscErrorCode RequestRetire(SlepcServer s, PetscInt id)
   q->res.its = q->eps->its;
   q->res.reason = q->eps->reason;
   q->res.nmatmult = q->eps->nmatmult;
-  q->res.gs_passes = q->eps->V->gs_passes;
+  PetscCall(BVHIPKSGetPasses(q->eps->V, &q->res.gs_passes));
   q->res.t_host_ds = q->eps->t_ds;
   nret = PetscMin(nev, q->eps->nconv);
   q->res.nret = nret;
@@ -238,8 +265,11 @@ static PetscErrorCode GroupFeed(SlepcServer s, PetscBool *moved, PetscInt depth)
       PetscInt id = s->slot_req[i];
       BV V;
       if (id < 0) continue;
+      PetscBool pending;
       V = s->req[id].eps->V;
-      if (!V->kry_pending) continue;
+      ierr = BVHIPKSGroupPending(V, &pending);
+      if (ierr) goto out;
+      if (!pending) continue;
       if (refine < 0) { refine = (int)V->orthog_ref; eta = V->orthog_eta; }
       if ((int)V->orthog_ref != refine || V->orthog_eta != eta) continue;   /* next round */
       if (!s->req[id].joined) {
@@ -302,7 +332,10 @@ PetscErrorCode SlepcServerProgress(SlepcServer s, PetscBool *progressed, PetscIn
     t0 = SlepcWallTime();
     ierr = RequestAdmit(s, s->next_queued, slot);
     s->t_admit += SlepcWallTime() - t0;
-    if (ierr) goto out;
+    if (ierr) {   /* the request fails alone: its objects and its budget are released */
+      ierr = RequestAbort(s, s->next_queued, ierr);
+      if (ierr) goto out;
+    }
     s->next_queued++;
     moved = PETSC_TRUE;
   }
@@ -355,7 +388,11 @@ PetscErrorCode SlepcServerGetResult(SlepcServer s, PetscInt id, PetscBool *done,
   Request *q;
   PetscCheck(s && id >= 0 && id < s->nreq, PETSC_ERR_ARG_OUTOFRANGE, "no request %d", (int)id);
   q = &s->req[id];
-  if (done) *done = q->state == REQ_DONE ? PETSC_TRUE : PETSC_FALSE;
+  if (done) *done = (q->state == REQ_DONE || q->state == REQ_FAILED) ? PETSC_TRUE : PETSC_FALSE;
+  if (q->state == REQ_FAILED) {   /* no pairs: the error code says why */
+    if (res) *res = q->res;
+    return PETSC_SUCCESS;
+  }
   if (q->state != REQ_DONE) return PETSC_SUCCESS;
   if (res) *res = q->res;
   if (eigr) memcpy(eigr, q->eigr, sizeof(PetscScalar) * (size_t)q->res.nret);

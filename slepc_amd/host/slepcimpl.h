@@ -69,28 +69,93 @@ int SlepcSCCompare(const SlepcSC *sc, PetscScalar ar, PetscScalar ai, PetscScala
 void SlepcSortEigenvalues(const SlepcSC *sc, PetscInt n, const PetscScalar *eigr, const PetscScalar *eigi,
                           PetscInt *perm);
 
+/* ---------------------------------------------------------------- objects
+ * PETSc's object header (petsc/private/petscimpl.h, outside the reference):
+ * class and type names, a state counter, and the composed functions through
+ * which a type offers methods beyond its ops table
+ * (PetscObjectComposeFunction / PetscObjectQueryFunction). */
+typedef void (*PetscVoidFn)(void);
+typedef int64_t PetscObjectId;
+typedef int64_t PetscObjectState;
+#define SLEPC_MAX_COMPOSED 8
+typedef struct _p_PetscObject {
+  char class_name[SLEPC_NAME_LEN];
+  char type_name[SLEPC_NAME_LEN];
+  PetscObjectId id;
+  PetscObjectState state;
+  int nfn;
+  char fname[SLEPC_MAX_COMPOSED][64];
+  PetscVoidFn fn[SLEPC_MAX_COMPOSED];
+} _p_PetscObject;
+typedef struct _p_PetscObject *PetscObject;
+#define PETSCHEADER(ObjectOps) \
+  _p_PetscObject hdr;          \
+  ObjectOps ops[1]
+PetscErrorCode PetscHeaderInit(PetscObject obj, const char *class_name);
+PetscErrorCode PetscObjectComposeFunction_Private(PetscObject obj, const char name[], PetscVoidFn fn);
+PetscErrorCode PetscObjectQueryFunction_Private(PetscObject obj, const char name[], PetscVoidFn *fn);
+#define PetscObjectComposeFunction(obj, name, f) \
+  PetscObjectComposeFunction_Private((PetscObject)(obj), (name), (PetscVoidFn)(f))
+#define PetscObjectQueryFunction(obj, name, fptr) \
+  PetscObjectQueryFunction_Private((PetscObject)(obj), (name), (PetscVoidFn *)(fptr))
+
+/* the opaque PETSc types the reference's BV layout names but this path never uses */
+typedef struct _p_PetscLayout *PetscLayout;
+typedef struct _p_IS *IS;
+typedef struct _p_PetscViewer *PetscViewer;
+typedef struct _p_PetscOptionItems PetscOptionItems;
+
+/* PetscRandom: the counter generator of ks_bv_random; every draw advances the
+   counter, draw c of seed s uses the stream s + 1000003 c */
+struct _p_PetscRandom {
+  uint64_t seed, count;
+};
+typedef struct _p_PetscRandom *PetscRandom;
+PetscErrorCode PetscRandomCreate(PetscRandom *r);
+PetscErrorCode PetscRandomDestroy(PetscRandom *r);
+
+/* ---------------------------------------------------------------- options database
+ * PETSc's, reduced to what the types here read: "-name value" pairs from the
+ * PETSC_OPTIONS environment variable (read at SlepcInitialize) and from
+ * PetscOptionsInsertString. */
+PetscErrorCode PetscOptionsGetBool(const char name[], PetscBool *flg, PetscBool *set);
+PetscErrorCode PetscOptionsGetString(const char name[], char *value, size_t len, PetscBool *set);
+
 /* ---------------------------------------------------------------- Vec */
+/* where the current values of a vector with a host copy are (PetscOffloadMask) */
+typedef enum { PETSC_OFFLOAD_UNALLOCATED = 0, PETSC_OFFLOAD_GPU = 1, PETSC_OFFLOAD_BOTH = 3 } PetscOffloadMask;
 struct _p_Vec {
   PetscInt n;
   SlepcAMDPrecision prec;
   void *d;               /* device storage */
   PetscBool owns;        /* device storage allocated by this Vec */
+  PetscScalar *h;        /* pinned host copy (FP64 vectors), made by the first VecGetArray */
+  PetscOffloadMask mask;
 };
 size_t SlepcPrecSize(SlepcAMDPrecision p);
 PetscErrorCode VecCreateHIPWithArray(PetscInt n, SlepcAMDPrecision prec, void *d, Vec *v);
+/* host access (FP64 vectors): the device values are copied to the host copy when
+   they are newer; VecRestoreArray copies the host values back */
+PetscErrorCode VecGetArray(Vec v, PetscScalar **a);
+PetscErrorCode VecRestoreArray(Vec v, PetscScalar **a);
+PetscErrorCode VecGetArrayRead(Vec v, const PetscScalar **a);
+PetscErrorCode VecRestoreArrayRead(Vec v, const PetscScalar **a);
+PetscErrorCode VecHIPGetArrayRead(Vec v, const void **d);
+/* the host copy, made current by a copy the caller enqueued and waited for itself */
+PetscErrorCode VecHIPGetHostCopy(Vec v, PetscScalar **h);
+PetscErrorCode VecHIPSetHostCurrent(Vec v);
+PetscErrorCode VecSetRandom(Vec v, PetscRandom r);
 
 /* ---------------------------------------------------------------- Mat */
-typedef struct {
+struct _MatOps {        /* the members of PETSc's struct _MatOps this path uses */
   PetscErrorCode (*mult)(Mat, Vec, Vec);
   PetscErrorCode (*multtranspose)(Mat, Vec, Vec);
   PetscErrorCode (*destroy)(Mat);
-} MatOps;
+};
 struct _p_Mat {
-  char type[SLEPC_NAME_LEN];
-  MatOps ops;
+  PETSCHEADER(struct _MatOps);
   PetscInt m, n;
   void *data;
-  PetscBool gate;        /* hipcsr: skip the product once an expansion broke down */
 };
 typedef struct {         /* MATSEQDENSE, host column-major */
   PetscScalar *v;
@@ -102,82 +167,161 @@ typedef struct {         /* MATHIPCSR */
   ks_csr csr32;          /* FP32 copy of FP64 values for FP32 vectors (built on demand) */
   ks_csr csrT;           /* A^T for MatMultTranspose (built on the device on demand) */
   SlepcAMDPrecision vprec;
+  PetscBool gate;        /* skip the product once an expansion broke down */
 } Mat_HIPCSR;
+#define MatTypeIs(A, t) (!strcmp((A)->hdr.type_name, (t)))
 /* the FP32-valued copy of a hipcsr matrix, made on first use */
 PetscErrorCode MatHIPCSRGetF32(Mat A, ks_csr *out);
 PetscErrorCode MatCreateSeqDenseView(PetscInt m, PetscInt n, PetscScalar *data, PetscInt lda, Mat *A);
 
-/* ---------------------------------------------------------------- BV */
-typedef struct _BVOps {
+/* ---------------------------------------------------------------- BV
+ * struct _BVOps and struct _p_BV have exactly the members of the reference's
+ * include/slepc/private/bvimpl.h:25-61 and :63-113, in the same order: a BV type
+ * written against them (BVHIPKS) drops into an unmodified SLEPc.  The type's own
+ * state lives behind bv->data; what it offers beyond the ops table goes through
+ * composed functions. */
+typedef enum { BV_MATMULT_VECS = 0, BV_MATMULT_MAT = 1, BV_MATMULT_MAT_SAVE = 2 } BVMatMultType;
+
+typedef struct _BVOps *BVOps;
+
+struct _BVOps {
+  PetscErrorCode (*mult)(BV, PetscScalar, PetscScalar, BV, Mat);
   PetscErrorCode (*multvec)(BV, PetscScalar, PetscScalar, Vec, PetscScalar *);
   PetscErrorCode (*multinplace)(BV, Mat, PetscInt, PetscInt);
+  PetscErrorCode (*multinplacetrans)(BV, Mat, PetscInt, PetscInt);
+  PetscErrorCode (*dot)(BV, BV, Mat);
   PetscErrorCode (*dotvec)(BV, Vec, PetscScalar *);
+  PetscErrorCode (*dotvec_local)(BV, Vec, PetscScalar *);
+  PetscErrorCode (*dotvec_begin)(BV, Vec, PetscScalar *);
+  PetscErrorCode (*dotvec_end)(BV, Vec, PetscScalar *);
   PetscErrorCode (*scale)(BV, PetscInt, PetscScalar);
   PetscErrorCode (*norm)(BV, PetscInt, NormType, PetscReal *);
+  PetscErrorCode (*norm_local)(BV, PetscInt, NormType, PetscReal *);
+  PetscErrorCode (*norm_begin)(BV, PetscInt, NormType, PetscReal *);
+  PetscErrorCode (*norm_end)(BV, PetscInt, NormType, PetscReal *);
+  PetscErrorCode (*normalize)(BV, PetscScalar *);
+  PetscErrorCode (*matmult)(BV, Mat, BV);
+  PetscErrorCode (*copy)(BV, BV);
   PetscErrorCode (*copycolumn)(BV, PetscInt, PetscInt);
+  PetscErrorCode (*resize)(BV, PetscInt, PetscBool);
   PetscErrorCode (*getcolumn)(BV, PetscInt, Vec *);
   PetscErrorCode (*restorecolumn)(BV, PetscInt, Vec *);
-  PetscErrorCode (*setrandomcolumn)(BV, PetscInt);
-  PetscErrorCode (*getbuffer)(BV);       /* refresh bv->buffer (host mirror) */
+  PetscErrorCode (*getarray)(BV, PetscScalar **);
+  PetscErrorCode (*restorearray)(BV, PetscScalar **);
+  PetscErrorCode (*getarrayread)(BV, const PetscScalar **);
+  PetscErrorCode (*restorearrayread)(BV, const PetscScalar **);
+  PetscErrorCode (*restoresplit)(BV, BV *, BV *);
+  PetscErrorCode (*restoresplitrows)(BV, IS, IS, BV *, BV *);
+  PetscErrorCode (*gramschmidt)(BV, PetscInt, Vec, PetscBool *, PetscScalar *, PetscScalar *, PetscReal *, PetscReal *);
+  PetscErrorCode (*getmat)(BV, Mat *);
+  PetscErrorCode (*restoremat)(BV, Mat *);
+  PetscErrorCode (*duplicate)(BV, BV);
+  PetscErrorCode (*create)(BV);
+  PetscErrorCode (*setfromoptions)(BV, PetscOptionItems *);
+  PetscErrorCode (*view)(BV, PetscViewer);
   PetscErrorCode (*destroy)(BV);
-  PetscErrorCode (*create)(BV);          /* deferred until sizes are known */
-  /* Whole-column orthonormalization (all CGS passes, DGKS decisions, lindep and
-     the scaling) done inside the type.  Optional: the interface falls back to
-     the pass-by-pass loop when it is NULL.  `async` leaves the outputs on the
-     device (norm/lindep may be NULL); otherwise they are returned. */
-  PetscErrorCode (*orthonormalizecolumn)(BV, PetscInt j, PetscBool scale, PetscBool safe, PetscBool async,
-                                         PetscReal *norm, PetscBool *lindep);
-  /* Whole Arnoldi/Lanczos expansion k..m split in two (optional): begin enqueues
-     every step on the device without waiting, end waits once and returns the
-     breakdown position, beta and the new coefficient columns (host mirror). */
-  PetscErrorCode (*matkrylovbegin)(BV, Mat, PetscInt k, PetscInt m);
-  PetscErrorCode (*matkrylovend)(BV, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown);
-  /* G(0:ye-ys, 0:xe-xs) = Y(:, ys:ye)^T X(:, xs:xe) into host memory (leading
-     dimension ldg): the block dot under BVDot, BVOrthogonalize, BVMatProject */
-  PetscErrorCode (*gram)(BV X, PetscInt xs, PetscInt xe, BV Y, PetscInt ys, PetscInt ye, PetscScalar *G, PetscInt ldg);
-} BVOps;
+};
 
 struct _p_BV {
-  char type[SLEPC_NAME_LEN];
-  BVOps ops;
-  void *data;
-  PetscInt n, N, m;          /* local/global rows, columns (without constraints) */
-  PetscInt l, k, nc, ld;     /* active columns, constraints, leading dimension */
-  BVOrthogType orthog_type;
-  BVOrthogRefineType orthog_ref;
-  BVOrthogBlockType orthog_block;   /* method of BVOrthogonalize */
-  PetscReal orthog_eta;
-  PetscReal deftol;
-  PetscScalar *buffer;       /* host mirror of the (nc+m) x m coefficient buffer */
-  Vec cv[2];                 /* column views */
-  PetscInt ci[2];
-  unsigned long long rseed, rcount;
-  SlepcAMDPrecision prec;
-  ks_ctx ctx;
-  int64_t gs_passes;         /* executed CGS passes */
-  /* results of an expansion done synchronously by BVMatKrylovBegin */
-  PetscInt kry_m;
-  PetscReal kry_beta;
-  PetscBool kry_breakdown;
-  /* grouped expansions (serving): the steps k..m-1 are enqueued by the server,
-     step kry_j next, while kry_pending */
-  PetscBool grouped, kry_pending;
-  PetscInt kry_k, kry_j;
-  ks_csr kry_csr;
-  /* the status and the new coefficient columns were copied to pinned host
-     memory in stream order at the end of the expansion */
-  PetscBool kry_posted;
-  /* replay each Arnoldi step of an expansion from a captured graph */
-  PetscBool step_graphs;
+  PETSCHEADER(struct _BVOps);
+  /*------------------------- User parameters --------------------------*/
+  PetscLayout        map;          /* layout of columns */
+  VecType            vtype;        /* vector type */
+  PetscInt           n, N;         /* dimensions of vectors (local, global) */
+  PetscInt           m;            /* number of vectors */
+  PetscInt           l;            /* number of leading columns */
+  PetscInt           k;            /* number of active columns */
+  PetscInt           nc;           /* number of constraints */
+  PetscInt           ld;           /* leading dimension */
+  BVOrthogType       orthog_type;  /* the method of vector orthogonalization */
+  BVOrthogRefineType orthog_ref;   /* refinement method */
+  PetscReal          orthog_eta;   /* refinement threshold */
+  BVOrthogBlockType  orthog_block; /* the method of block orthogonalization */
+  Mat                matrix;       /* inner product matrix */
+  PetscBool          indef;        /* matrix is indefinite */
+  BVMatMultType      vmm;          /* version of matmult operation */
+  PetscBool          rrandom;      /* reproducible random vectors */
+  PetscReal          deftol;       /* tolerance for BV_SafeSqrt */
+
+  /*---------------------- Cached data and workspace -------------------*/
+  Vec                buffer;       /* buffer vector used in orthogonalization */
+  Mat                Abuffer;      /* auxiliary seqdense matrix that wraps the buffer */
+  Vec                Bx;           /* result of matrix times a vector x */
+  PetscObjectId      xid;          /* object id of vector x */
+  PetscObjectState   xstate;       /* state of vector x */
+  Vec                cv[2];        /* column vectors obtained with BVGetColumn() */
+  PetscInt           ci[2];        /* column indices of obtained vectors */
+  PetscObjectState   st[2];        /* state of obtained vectors */
+  PetscObjectId      id[2];        /* object id of obtained vectors */
+  PetscScalar        *h, *c;       /* orthogonalization coefficients */
+  Vec                omega;        /* signature matrix values for indefinite case */
+  PetscBool          defersfo;     /* deferred call to setfromoptions */
+  BV                 cached;       /* cached BV to store result of matrix times BV */
+  PetscObjectState   bvstate;      /* state of BV when BVApplyMatrixBV() was called */
+  BV                 L, R;         /* BV objects obtained with BVGetSplit/Rows() */
+  PetscObjectState   lstate, rstate; /* state of L and R when BVGetSplit/Rows() was called */
+  PetscInt           lsplit;       /* value of l when BVGetSplit() was called (-1 if BVGetSplitRows()) */
+  PetscInt           issplit;      /* !=0 if BV is from split (1=left, 2=right, -1=top, -2=bottom) */
+  BV                 splitparent;  /* my parent if I am a split BV */
+  PetscRandom        rand;         /* random number generator */
+  Mat                Acreate;      /* matrix given at BVCreateFromMat() */
+  Mat                Aget;         /* matrix returned for BVGetMat() */
+  PetscBool          cuda;         /* true if NVIDIA GPU must be used */
+  PetscBool          hip;          /* true if AMD GPU must be used */
+  PetscBool          sfocalled;    /* setfromoptions has been called */
+  PetscScalar        *work;
+  PetscInt           lwork;
+  void               *data;
 };
+#define BVTypeName(bv) (((PetscObject)(bv))->type_name)
+
+/* BVGetBufferVec (bvbasic.c:755-786): the (nc+m) x m coefficient buffer, made on first use */
+PetscErrorCode BVGetBufferVec(BV bv, Vec *buffer);
+PetscErrorCode BV_CleanCoefficients(BV bv, PetscInt j, PetscScalar *h);
+PetscErrorCode BV_SetValue(BV bv, PetscInt j, PetscInt k, PetscScalar *h, PetscScalar value);
+PetscErrorCode BV_StoreCoefficients(BV bv, PetscInt j, PetscScalar *h, PetscScalar *dest);
 PetscErrorCode BVOrthogonalizeColumn_Safe(BV bv, PetscInt j, PetscReal *norm, PetscBool *lindep);
 PetscErrorCode BVMatKrylovBegin(BV V, Mat A, PetscInt k, PetscInt m);
 PetscErrorCode BVMatKrylovEnd(BV V, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown);
 PetscErrorCode BVCopyHessenberg(BV V, Mat H, PetscInt k, PetscInt m);
 PetscErrorCode BVCopyTridiagonal(BV V, Mat T, PetscInt k, PetscInt m);
+PetscErrorCode BVMatKrylov(BV V, Mat A, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown);
+PetscErrorCode BVResizeDiscard_Private(BV bv, PetscInt n, PetscInt m);
+PetscErrorCode BVSetRandomSeed(BV bv, uint64_t seed);
+PetscErrorCode BVGetRandomSeed(BV bv, uint64_t *seed);
+
+/* The one hook through which a BV type offers a device-resident Krylov
+ * expansion (no reference counterpart).  A type composes
+ * "BVGetKrylovOps_C" -> PetscErrorCode f(BV, const BVKrylovOps **); a NULL table
+ * means the type (or this object) leaves orthonormalization to the per-pass
+ * gramschmidt op and the expansion to the reference's per-step loop.
+ *   orthonormalizecolumn  all CGS passes, the DGKS decisions, lindep and the
+ *                         1/norm scaling of one column on the device
+ *                         (BVOrthogonalizeGS bvorthog.c:145-217 + the scaling of
+ *                         BVOrthonormalizeColumn bvorthog.c:416-422);
+ *   matkrylovbegin/end    the steps k..m-1 of BVMatArnoldi/BVMatLanczos
+ *                         (bvkrylov.c:88-97) enqueued at once, then one wait
+ *                         for the breakdown position, beta and the new
+ *                         coefficient columns (in the buffer's host copy). */
+typedef struct {
+  PetscErrorCode (*orthonormalizecolumn)(BV, PetscInt j, PetscBool scale, PetscBool safe, PetscReal *norm,
+                                         PetscBool *lindep);
+  PetscErrorCode (*matkrylovbegin)(BV, Mat, PetscInt k, PetscInt m);
+  PetscErrorCode (*matkrylovend)(BV, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown);
+} BVKrylovOps;
+PetscErrorCode BVGetKrylovOps(BV bv, const BVKrylovOps **ops);
+
+/* BVHIPKS functions used by the solver and the serving loop */
+PetscErrorCode BVHIPKSGetContext(BV bv, ks_ctx *ctx);
+PetscErrorCode BVHIPKSGetPasses(BV bv, int64_t *passes);
+PetscErrorCode BVHIPKSResetPasses(BV bv);
+PetscErrorCode BVHIPKSSetGrouped(BV bv, PetscBool grouped);
+PetscErrorCode BVHIPKSGroupPending(BV bv, PetscBool *pending);
 PetscErrorCode BVHIPKSGroupMember(BV V, ks_group_member *mem);
 PetscErrorCode BVHIPKSGroupAdvance(BV V, PetscBool *finished);
 PetscErrorCode BVHIPKSPostExpansion(BV V);
+PetscErrorCode BVHIPKSGetStepGraphs(BV bv, PetscBool *flg);
+SlepcAMDPrecision BVPrecision(BV bv);
 
 /* ---------------------------------------------------------------- DS */
 typedef struct {
@@ -212,12 +356,13 @@ PetscErrorCode DSCreate_NHEPTS(DS ds);
 struct _p_PC {
   char type[SLEPC_NAME_LEN];
 };
-struct _p_KSP {              /* the inner solver of STSINVERT (device PCG) */
+struct _p_KSP {              /* the inner solver of STSINVERT (device CG or BiCGStab) */
   char type[SLEPC_NAME_LEN];
   struct _p_PC pc;
   PetscReal rtol, abstol;
   PetscInt maxits, n;
-  double *d_dinv, *d_work;   /* Jacobi diagonal, r z p q + scalars */
+  double *d_dinv, *d_work;   /* Jacobi diagonal, the solver's vectors + its state */
+  size_t nwork;              /* doubles in d_work */
   int64_t its_total, nsolves;
 };
 struct _p_ST {
@@ -297,7 +442,21 @@ typedef struct {            /* EPS_KRYLOVSCHUR (krylovschur.h:76-105), default v
   /* two-sided variant: A^T and the oblique projection M = W^T V (ncv x ncv) */
   Mat AT;
   PetscScalar *M;
+  /* the expansion in flight: through the basis' Krylov extension, or (NULL)
+     already run by the per-step loop, with its results here */
+  const BVKrylovOps *kry_ext;
+  PetscInt kry_m;
+  PetscReal kry_beta;
+  PetscBool kry_breakdown;
+  /* device-path settings applied to the basis at set-up */
+  PetscBool step_graphs, grouped;
+  /* first phase of the mixed form: hand the kept subspace over at convergence
+     as q1, the start of its Arnoldi form in the kept columns 0..ho_m-1 */
+  PetscBool handover;
+  PetscInt ho_m, ho_nconv;
+  PetscScalar *ho_q1;
 } EPS_KRYLOVSCHUR;
+PetscErrorCode EPSKrylovSchurSetGrouped_Private(EPS eps, PetscBool flg);
 
 PetscErrorCode EPSKrylovConvergence(EPS eps, PetscBool getall, PetscInt kini, PetscInt nits, PetscReal beta,
                                     PetscReal betat, PetscReal corrf, PetscInt *kout);

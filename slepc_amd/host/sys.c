@@ -3,6 +3,7 @@
  * sorting criteria (src/sys/slepcsc.c:40-190).
  */
 #include <stdarg.h>
+#include <strings.h>
 #include <time.h>
 
 #include "slepcimpl.h"
@@ -64,6 +65,140 @@ void *SlepcFunctionListFind(SlepcFunctionList *fl, const char *name)
   return NULL;
 }
 
+/* ---------------------------------------------------------------- objects */
+static PetscObjectId g_next_id = 1;
+
+PetscErrorCode PetscHeaderInit(PetscObject obj, const char *class_name)
+{
+  memset(obj, 0, sizeof(*obj));
+  snprintf(obj->class_name, sizeof(obj->class_name), "%s", class_name);
+  obj->id = g_next_id++;
+  return PETSC_SUCCESS;
+}
+
+/* a NULL function removes the entry */
+PetscErrorCode PetscObjectComposeFunction_Private(PetscObject obj, const char name[], PetscVoidFn fn)
+{
+  PetscCheck(obj && name, PETSC_ERR_ARG_NULL, "null argument");
+  PetscCheck(strlen(name) < sizeof(obj->fname[0]), PETSC_ERR_ARG_WRONG, "composed name too long: %s", name);
+  for (int i = 0; i < obj->nfn; i++) {
+    if (strcmp(obj->fname[i], name)) continue;
+    if (fn) obj->fn[i] = fn;
+    else {
+      obj->nfn--;
+      memmove(obj->fname[i], obj->fname[obj->nfn], sizeof(obj->fname[0]));
+      obj->fn[i] = obj->fn[obj->nfn];
+    }
+    return PETSC_SUCCESS;
+  }
+  if (!fn) return PETSC_SUCCESS;
+  PetscCheck(obj->nfn < SLEPC_MAX_COMPOSED, PETSC_ERR_SUP, "too many composed functions");
+  strcpy(obj->fname[obj->nfn], name);
+  obj->fn[obj->nfn++] = fn;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode PetscObjectQueryFunction_Private(PetscObject obj, const char name[], PetscVoidFn *fn)
+{
+  PetscCheck(obj && name && fn, PETSC_ERR_ARG_NULL, "null argument");
+  *fn = NULL;
+  for (int i = 0; i < obj->nfn; i++)
+    if (!strcmp(obj->fname[i], name)) { *fn = obj->fn[i]; break; }
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode PetscRandomCreate(PetscRandom *r)
+{
+  PetscCall(PetscMallocZero(sizeof(struct _p_PetscRandom), r));
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode PetscRandomDestroy(PetscRandom *r)
+{
+  if (r && *r) PetscFree(*r);
+  return PETSC_SUCCESS;
+}
+
+/* ---------------------------------------------------------------- options database */
+#define SLEPC_MAX_OPTIONS 128
+static struct {
+  char name[64];
+  char value[128];
+} g_opt[SLEPC_MAX_OPTIONS];
+static int g_nopt = 0;
+
+PetscErrorCode PetscOptionsSetValue(const char name[], const char value[])
+{
+  int i;
+  PetscCheck(name && name[0] == '-', PETSC_ERR_ARG_WRONG, "option names start with '-'");
+  PetscCheck(strlen(name) < sizeof(g_opt[0].name) && (!value || strlen(value) < sizeof(g_opt[0].value)),
+             PETSC_ERR_ARG_WRONG, "option %s too long", name);
+  for (i = 0; i < g_nopt; i++)
+    if (!strcmp(g_opt[i].name, name)) break;
+  if (i == g_nopt) {
+    PetscCheck(g_nopt < SLEPC_MAX_OPTIONS, PETSC_ERR_SUP, "options database full");
+    g_nopt++;
+  }
+  strcpy(g_opt[i].name, name);
+  snprintf(g_opt[i].value, sizeof(g_opt[i].value), "%s", value ? value : "");
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode PetscOptionsClearValue(const char name[])
+{
+  for (int i = 0; i < g_nopt; i++) {
+    if (strcmp(g_opt[i].name, name)) continue;
+    g_opt[i] = g_opt[--g_nopt];
+    return PETSC_SUCCESS;
+  }
+  return PETSC_SUCCESS;
+}
+
+/* "-a 1 -b -c x": a name followed by a word not starting with '-' takes it as value */
+PetscErrorCode PetscOptionsInsertString(const char in_str[])
+{
+  char buf[2048], *tok[256];
+  int nt = 0;
+  if (!in_str) return PETSC_SUCCESS;
+  PetscCheck(strlen(in_str) < sizeof(buf), PETSC_ERR_ARG_WRONG, "options string too long");
+  strcpy(buf, in_str);
+  for (char *t = strtok(buf, " \t\n"); t && nt < 256; t = strtok(NULL, " \t\n")) tok[nt++] = t;
+  for (int i = 0; i < nt; i++) {
+    PetscCheck(tok[i][0] == '-', PETSC_ERR_ARG_WRONG, "unexpected option value %s", tok[i]);
+    if (i + 1 < nt && tok[i + 1][0] != '-') {
+      PetscCall(PetscOptionsSetValue(tok[i], tok[i + 1]));
+      i++;
+    } else PetscCall(PetscOptionsSetValue(tok[i], NULL));
+  }
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode PetscOptionsGetString(const char name[], char *value, size_t len, PetscBool *set)
+{
+  *set = PETSC_FALSE;
+  for (int i = 0; i < g_nopt; i++) {
+    if (strcmp(g_opt[i].name, name)) continue;
+    snprintf(value, len, "%s", g_opt[i].value);
+    *set = PETSC_TRUE;
+    break;
+  }
+  return PETSC_SUCCESS;
+}
+
+/* PetscOptionsGetBool: an empty value means true; 1/0, true/false, yes/no */
+PetscErrorCode PetscOptionsGetBool(const char name[], PetscBool *flg, PetscBool *set)
+{
+  char v[128];
+  PetscBool s;
+  PetscCall(PetscOptionsGetString(name, v, sizeof(v), &s));
+  if (set) *set = s;
+  if (!s) return PETSC_SUCCESS;
+  if (!v[0] || !strcmp(v, "1") || !strcasecmp(v, "true") || !strcasecmp(v, "yes")) *flg = PETSC_TRUE;
+  else if (!strcmp(v, "0") || !strcasecmp(v, "false") || !strcasecmp(v, "no")) *flg = PETSC_FALSE;
+  else SETERRQ(PETSC_ERR_ARG_WRONG, "option %s: %s is not a boolean", name, v);
+  return PETSC_SUCCESS;
+}
+
 /* MKL service (mkl_rt), by value.  3 selects its GNU OpenMP threading layer:
    the layer is fixed for the process at MKL's first call, and Intel-threaded
    MKL called from inside a GNU OpenMP region (a host application's, or the
@@ -76,6 +211,7 @@ PetscErrorCode SlepcInitialize(int device)
 {
   if (g_initialized) return PETSC_SUCCESS;
   (void)MKL_Set_Threading_Layer(3);
+  PetscCall(PetscOptionsInsertString(getenv("PETSC_OPTIONS")));
   KSCall(ks_init(device));
   KSCall(ks_ctx_create(&g_ctx));
   PetscCall(MatRegister(MATHIPCSR, MatCreate_HIPCSR));

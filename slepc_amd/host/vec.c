@@ -47,6 +47,7 @@ PetscErrorCode VecDestroy(Vec *v)
 {
   if (!v || !*v) return PETSC_SUCCESS;
   if ((*v)->owns) KSCall(ks_free((*v)->d));
+  if ((*v)->h) KSCall(ks_host_free((*v)->h));
   free(*v);
   *v = NULL;
   return PETSC_SUCCESS;
@@ -59,17 +60,109 @@ PetscErrorCode VecGetSize(Vec v, PetscInt *n)
   return PETSC_SUCCESS;
 }
 
+/* the device copy changed: a host copy, if any, is out of date */
+static void VecDeviceModified(Vec v)
+{
+  if (v->mask == PETSC_OFFLOAD_BOTH) v->mask = PETSC_OFFLOAD_GPU;
+}
+
 PetscErrorCode VecSetRandomSeeded(Vec v, uint64_t seed)
 {
   PetscCheck(v, PETSC_ERR_ARG_NULL, "null vector");
   KSCall(ks_bv_random(SlepcAMDContext(), kprec(v->prec), v->n, v->d, seed));
+  VecDeviceModified(v);
   return PETSC_SUCCESS;
 }
 
+/* VecSetRandom with the generator's next stream (PetscRandom, see slepcimpl.h) */
+PetscErrorCode VecSetRandom(Vec v, PetscRandom r)
+{
+  PetscCheck(v && r, PETSC_ERR_ARG_NULL, "null argument");
+  r->count++;
+  return VecSetRandomSeeded(v, r->seed + 1000003ULL * r->count);
+}
+
+/* device pointer with write access: the host copy becomes stale */
 PetscErrorCode VecHIPGetArray(Vec v, void **d)
 {
   PetscCheck(v, PETSC_ERR_ARG_NULL, "null vector");
   *d = v->d;
+  VecDeviceModified(v);
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode VecHIPGetArrayRead(Vec v, const void **d)
+{
+  PetscCheck(v, PETSC_ERR_ARG_NULL, "null vector");
+  *d = v->d;
+  return PETSC_SUCCESS;
+}
+
+static PetscErrorCode VecHostAllocate(Vec v)
+{
+  PetscCheck(v->prec == SLEPC_AMD_F64, PETSC_ERR_SUP, "host access to FP32 vectors goes through VecGetValuesHost");
+  if (v->h) return PETSC_SUCCESS;
+  /* pinned, so copies into it can be enqueued in stream order */
+  KSCall(ks_host_alloc((void **)&v->h, sizeof(PetscScalar) * (size_t)(v->n ? v->n : 1)));
+  v->mask = PETSC_OFFLOAD_GPU;
+  return PETSC_SUCCESS;
+}
+
+static PetscErrorCode VecToHost(Vec v)
+{
+  PetscCall(VecHostAllocate(v));
+  if (v->mask != PETSC_OFFLOAD_BOTH) {
+    if (v->n) KSCall(ks_memcpy_dtoh(SlepcAMDContext(), v->h, v->d, sizeof(PetscScalar) * (size_t)v->n));
+    v->mask = PETSC_OFFLOAD_BOTH;
+  }
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode VecGetArrayRead(Vec v, const PetscScalar **a)
+{
+  PetscCheck(v && a, PETSC_ERR_ARG_NULL, "null argument");
+  PetscCall(VecToHost(v));
+  *a = v->h;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode VecRestoreArrayRead(Vec v, const PetscScalar **a)
+{
+  PetscCheck(v && a && *a == v->h, PETSC_ERR_ARG_WRONG, "array does not belong to the vector");
+  *a = NULL;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode VecGetArray(Vec v, PetscScalar **a)
+{
+  PetscCheck(v && a, PETSC_ERR_ARG_NULL, "null argument");
+  PetscCall(VecToHost(v));
+  *a = v->h;
+  return PETSC_SUCCESS;
+}
+
+/* the host values go back to the device (the host copy stays current) */
+PetscErrorCode VecRestoreArray(Vec v, PetscScalar **a)
+{
+  PetscCheck(v && a && *a == v->h, PETSC_ERR_ARG_WRONG, "array does not belong to the vector");
+  if (v->n) KSCall(ks_memcpy_htod(SlepcAMDContext(), v->d, v->h, sizeof(PetscScalar) * (size_t)v->n));
+  v->mask = PETSC_OFFLOAD_BOTH;
+  *a = NULL;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode VecHIPGetHostCopy(Vec v, PetscScalar **h)
+{
+  PetscCheck(v && h, PETSC_ERR_ARG_NULL, "null argument");
+  PetscCall(VecHostAllocate(v));
+  *h = v->h;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode VecHIPSetHostCurrent(Vec v)
+{
+  PetscCheck(v && v->h, PETSC_ERR_ARG_WRONGSTATE, "vector has no host copy");
+  v->mask = PETSC_OFFLOAD_BOTH;
   return PETSC_SUCCESS;
 }
 
@@ -78,6 +171,7 @@ PetscErrorCode VecSet(Vec v, PetscScalar alpha)
   PetscCheck(v, PETSC_ERR_ARG_NULL, "null vector");
   if (alpha == 0.0) {
     KSCall(ks_memset_zero(SlepcAMDContext(), v->d, SlepcPrecSize(v->prec) * (size_t)v->n));
+    VecDeviceModified(v);
     return PETSC_SUCCESS;
   }
   /* rare path: build on the host */
@@ -95,6 +189,7 @@ PetscErrorCode VecCopy(Vec x, Vec y)
   PetscCheck(x->n == y->n, PETSC_ERR_ARG_SIZ, "vector sizes %d and %d differ", x->n, y->n);
   if (x->prec == y->prec) KSCall(ks_bv_copy(SlepcAMDContext(), kprec(x->prec), x->n, x->d, y->d));
   else KSCall(ks_bv_convert(SlepcAMDContext(), x->n, kprec(x->prec), x->d, kprec(y->prec), y->d));
+  VecDeviceModified(y);
   return PETSC_SUCCESS;
 }
 
@@ -103,6 +198,7 @@ PetscErrorCode VecAXPY(Vec y, PetscScalar alpha, Vec x)
   PetscCheck(x && y, PETSC_ERR_ARG_NULL, "null vector");
   PetscCheck(x->n == y->n && x->prec == y->prec, PETSC_ERR_ARG_SIZ, "incompatible vectors");
   KSCall(ks_bv_axpy(SlepcAMDContext(), kprec(x->prec), x->n, alpha, x->d, y->d));
+  VecDeviceModified(y);
   return PETSC_SUCCESS;
 }
 
@@ -110,6 +206,7 @@ PetscErrorCode VecScale(Vec x, PetscScalar alpha)
 {
   PetscCheck(x, PETSC_ERR_ARG_NULL, "null vector");
   KSCall(ks_bv_scale(SlepcAMDContext(), kprec(x->prec), x->n, x->d, alpha));
+  VecDeviceModified(x);
   return PETSC_SUCCESS;
 }
 
@@ -126,6 +223,7 @@ PetscErrorCode VecSetValuesHost(Vec v, const PetscScalar *values)
 {
   PetscCheck(v && (values || !v->n), PETSC_ERR_ARG_NULL, "null argument");
   if (!v->n) return PETSC_SUCCESS;
+  VecDeviceModified(v);
   if (v->prec == SLEPC_AMD_F64) {
     KSCall(ks_memcpy_htod(SlepcAMDContext(), v->d, values, sizeof(double) * (size_t)v->n));
   } else {

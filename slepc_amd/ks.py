@@ -33,7 +33,8 @@ class Status(C.Structure):
                 ("error", C.c_int32), ("need_norm", C.c_int32), ("total_passes", C.c_int64),
                 ("onrm", C.c_double), ("nrm", C.c_double), ("beta_last", C.c_double),
                 ("scale", C.c_double), ("sumsq", C.c_double), ("exact_sq", C.c_double),
-                ("prescale", C.c_double), ("prescale_col", C.c_int32), ("prescale_pass", C.c_int32)]
+                ("prescale", C.c_double), ("prescale_col", C.c_int32), ("prescale_pass", C.c_int32),
+                ("halt_col", C.c_int32), ("halt_need", C.c_int32)]
 
 
 class ProfEntry(C.Structure):
@@ -62,6 +63,9 @@ def lib():
             "ks_bv_multvec": [vp, i32, i32, dbl, vp, i32, i32, vp, dbl, vp],
             "ks_bv_orthonormalize_column": [vp, i32, i32, vp, i32, i32, vp, i32, i32, i32, i32, dbl,
                                             i32, i32],
+            "ks_bv_orthonormalize_column_lean": [vp, i32, i32, vp, i32, i32, vp, i32, i32, i32, i32, dbl, i32,
+                                                 i32],
+            "ks_bv_orthonormalize_resume": [vp, i32, i32, vp, i32, i32, vp, i32, i32, i32, i32, dbl],
             "ks_bv_orthonormalize_column_noalloc": [vp, i32, i32, vp, i32, i32, vp, i32, i32, i32, i32, dbl,
                                                     i32, i32, vp],
             "ks_status_reset": [vp], "ks_status_get": [vp, C.POINTER(Status)],

@@ -29,7 +29,7 @@ class Result(C.Structure):
     _fields_ = [("nconv", C.c_int), ("nret", C.c_int), ("ngood", C.c_int), ("its", C.c_int),
                 ("reason", C.c_int), ("nmatmult", C.c_int64), ("gs_passes", C.c_int64),
                 ("t_arrival", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
-                ("t_host_ds", C.c_double)]
+                ("t_host_ds", C.c_double), ("error", C.c_int)]
 
 
 def clock() -> float:
@@ -97,7 +97,7 @@ class Server:
         out = dict(nconv=res.nconv, nret=k, ngood=res.ngood, its=res.its, reason=res.reason,
                    nmatmult=res.nmatmult, gs_passes=res.gs_passes, t_arrival=res.t_arrival,
                    t_admit=res.t_admit, t_done=res.t_done, latency=res.t_done - res.t_arrival, t_host_ds=res.t_host_ds,
-                   eigr=eigr[:k], eigi=eigi[:k], err=err[:k])
+                   error=res.error, eigr=eigr[:k], eigi=eigi[:k], err=err[:k])
         if want:
             out["X"] = X[:k].T.copy()
         return out
@@ -210,7 +210,7 @@ def run_stream_native(server: Server, requests, make_csr, tol: float = 1e-10, wa
         k = q.nret
         out = dict(nconv=q.nconv, nret=k, ngood=q.ngood, its=q.its, reason=q.reason, nmatmult=q.nmatmult,
                    gs_passes=q.gs_passes, t_arrival=q.t_arrival, t_admit=q.t_admit, t_done=q.t_done,
-                   latency=q.t_done - q.t_arrival, t_host_ds=q.t_host_ds,
+                   latency=q.t_done - q.t_arrival, t_host_ds=q.t_host_ds, error=q.error,
                    eigr=o["eigr"][:k], eigi=o["eigi"][:k], err=o["err"][:k])
         if want_vectors:
             out["X"] = o["X"][:k].T.copy()

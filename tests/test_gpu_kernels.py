@@ -257,6 +257,49 @@ def test_orthonormalize_refine_never_always(ks, ctx, oracle, refine):
     assert np.allclose(r["h"][:j + 1], r["H"], rtol=1e-10, atol=1e-13)
 
 
+@pytest.mark.parametrize("j,eta", [(12, 0.7071), (40, 0.7071), (12, 1.0), (40, 1.0), (60, 1.0)])
+def test_lean_orthonormalize_halts_and_resumes(ks, ctx, oracle, j, eta):
+    # the lean sequence enqueues two DGKS passes and no scaling.  A column with a
+    # tiny component outside the span takes two passes at the default eta; at
+    # eta = 1 the second pass's rounding-size coefficients also fall short of
+    # the criterion, so the lean sequence stops after two with the status
+    # naming the column and the resume runs the third.  Either way the result
+    # is bit for bit that of the full sequence: V, the coefficient column, the
+    # norm, lindep and the pass count (the oracle's)
+    rng = np.random.default_rng(300 + j)
+    n = 40961
+    V = np.zeros((n, j + 1), order="F")
+    V[:, :j] = np.linalg.qr(rng.standard_normal((n, j)))[0]
+    V[:, j] = 1e-12 * rng.standard_normal(n) + V[:, :j] @ rng.standard_normal(j)
+    ldbuf = j + 2
+
+    def run(lean):
+        dV = ks.DeviceArray.from_numpy(ctx, V)
+        buf = ks.DeviceArray.from_numpy(ctx, np.full((ldbuf, j + 2), 3.0))
+        ctx.status_reset()
+        f = ks.lib().ks_bv_orthonormalize_column_lean if lean else ks.lib().ks_bv_orthonormalize_column
+        ks.check(f(ctx.h, ks.F64, n, dV.ptr, n, j, buf.ptr, ldbuf, j, j, 0, eta, 1, 0), "orthonormalize")
+        st = ctx.status()
+        halted = (st.halt_col, st.halt_need)
+        if lean and st.halt_col >= 0:
+            ks.check(ks.lib().ks_bv_orthonormalize_resume(ctx.h, ks.F64, n, dV.ptr, n, j, buf.ptr, ldbuf, j, j, 0,
+                                                          eta), "resume")
+            st = ctx.status()
+        return dV.numpy(), buf.numpy(), st, halted
+
+    Va, ba, sa, _ = run(False)
+    Vb, bb, sb, halted = run(True)
+    H, nrm, lindep, passes = oracle.orthonormalize_column(np.asfortranarray(V.copy()), j, eta=eta)
+    assert sa.total_passes == passes and sa.lindep == int(lindep)
+    if eta == 1.0:
+        assert passes == 3 and halted == (j, 1)
+    else:
+        assert passes == 2 and halted[0] == -1
+    assert np.array_equal(Va, Vb) and np.array_equal(ba, bb)
+    assert sb.total_passes == sa.total_passes and sb.nrm == sa.nrm and sb.lindep == sa.lindep
+    assert sb.halt_col == -1 and sb.active == sa.active and sb.breakdown_col == sa.breakdown_col
+
+
 def test_random_matches_oracle_bitwise(ks, ctx, oracle):
     n = 100003
     d = ks.DeviceArray(ctx, (n,))
