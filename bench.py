@@ -11,6 +11,13 @@ value = converged eigenpairs/s, counting only pairs among the nev requested
 whose relative residual ||Ax - lambda x|| / |lambda|, recomputed on the device
 by EPSComputeError after the timed region, is <= 1e-10.
 
+The default run adds the other configurations of the metric to the same
+line (``--no-extras`` leaves them out): ``mixed_c3`` (the mixed-precision form
+against plain FP64 on C3), ``serving_c4`` (2,000 requests at ``--extra-qps``,
+served and one after another, with p50/p99 latency) and ``scale_c5`` (the R-MAT
+2^24 request served in mixed precision with eight C3 requests, beside the same
+requests in plain FP64).  Each is the summary of one of the forms below.
+
 Other forms, each printing its own JSON line:
 
 * ``--workload mixed`` -- nonsymmetric 2-D upwind convection-diffusion on a
@@ -42,6 +49,9 @@ HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
 METRIC = "converged eigenpairs/s at rel-residual<=1e-10; SpMV+BV HBM GB/s; p50/p99 latency"
 
+
+# R-MAT draws per vertex: the symmetrized graph then has average degree 2 x 8 = 16
+RMAT_EDGE_FACTOR = 8
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -112,6 +122,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-prof", action="store_true", help="time without per-kernel HIP events")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="fp64: only the 128^3 line, without the mixed, serving and at-scale lines")
+    ap.add_argument("--extra-qps", type=float, default=150.0, help="the serving line's arrival rate")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -120,15 +133,72 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    if args.workload == "mixed":
-        return run_mixed(args)
-    if args.workload == "serve":
-        return run_serve(args)
-    if args.workload == "scale":
-        return run_scale(args)
     if args.workload == "copycal":
         return run_copycal(args)
+    if args.workload != "fp64":
+        out = {"mixed": run_mixed, "serve": run_serve, "scale": run_scale}[args.workload](args)
+        print(json.dumps(out), flush=True)
+        return
+    out = run_fp64(args)
+    if not args.no_extras:
+        # the other configurations of the same metric, measured in the same run
+        t0 = time.time()
+        out["mixed_c3"] = _extra_mixed(args)
+        log(f"[bench] mixed-precision line in {time.time() - t0:.0f}s")
+        t0 = time.time()
+        out["serving_c4"] = _extra_serve(args)
+        log(f"[bench] serving line in {time.time() - t0:.0f}s")
+        t0 = time.time()
+        out["scale_c5"] = _extra_scale(args)
+        log(f"[bench] at-scale line in {time.time() - t0:.0f}s")
+    print(json.dumps(out), flush=True)
 
+
+def _sub_args(args, **kw):
+    a = argparse.Namespace(**vars(args))
+    for k, v in kw.items():
+        setattr(a, k, v)
+    return a
+
+
+def _extra_mixed(args):
+    """C3 (2048^2 convection-diffusion): mixed precision against plain FP64."""
+    o = run_mixed(_sub_args(args, steps=2, warmup=1))
+    f = o["forms"]
+    return {"workload": o["config"]["workload"], "eigenpairs_per_s_mixed": f["mixed"]["eigenpairs_per_s"],
+            "eigenpairs_per_s_fp64": f["fp64"]["eigenpairs_per_s"], "speedup_vs_fp64": o["speedup_vs_fp64"],
+            "solve_s": {k: f[k]["solve_s"] for k in f}, "restarts": {k: f[k]["restarts"][-1] for k in f},
+            "spmv": {k: f[k]["spmv"][-1] for k in f}, "mixed_phase": o["mixed_phase"],
+            "pair_rel_error_vs_fp64": o["pair_rel_error_vs_fp64"], "max_rel_residual": o["max_rel_residual"],
+            "verified": {k: f[k]["verified"][-1] for k in f}}
+
+
+def _extra_serve(args):
+    """C4: the 2,000-request Poisson stream, served, beside one after another."""
+    o = run_serve(_sub_args(args, requests=2000, qps=args.extra_qps, variants=False))
+    return {"workload": o["config"]["workload"], "requests": o["config"]["requests"], "qps": o["config"]["qps"],
+            "slots": o["config"]["slots"], "eigenpairs_per_s": o["served"]["eigenpairs_per_s"],
+            "wall_s": o["served"]["wall_s"], "verified_pairs": o["served"]["verified_pairs"],
+            "latency_ms": o["latency_ms"], "sequential": o["sequential"],
+            "speedup_vs_sequential": o["speedup_vs_sequential"],
+            "members_per_grouped_step": o["served"]["members_per_grouped_step"]}
+
+
+def _extra_scale(args):
+    """C5: the R-MAT 2^24 request in mixed precision served with eight mixed C3 requests."""
+    o = run_scale(args)
+    s = o["served"]
+    return {"workload": o["config"]["workload"], "rmat_n": o["config"]["rmat_n"],
+            "rmat_nnz": o["config"]["rmat_nnz"], "eigenpairs_per_s": s["eigenpairs_per_s"], "wall_s": s["wall_s"],
+            "verified_pairs": s["verified_pairs"], "latency_ms": o["latency_ms"], "requests": s["requests"],
+            "served_plain_fp64": o["served_plain_fp64"], "sequential_plain_fp64": o["sequential_plain_fp64"],
+            "speedup_vs_served_plain_fp64": round(s["eigenpairs_per_s"] / o["served_plain_fp64"]["eigenpairs_per_s"], 3)
+            if o["served_plain_fp64"]["eigenpairs_per_s"] else None,
+            "speedup_vs_sequential_fp64": o["speedup_vs_sequential_fp64"]}
+
+
+def run_fp64(args):
+    """C2: the default line (value, roofline, cpu_baseline)."""
     from slepc_amd import eps as E
     from slepc_amd import ks as K
     from slepc_amd import matgen as M
@@ -292,7 +362,7 @@ def main():
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def _base(args, value, unit, ms_per_step, steps, dtype, workload, extra):
@@ -393,7 +463,7 @@ def run_mixed(args):
                  "pair_rel_error_vs_fp64": [float(f"{e:.3e}") for e in pair_err],
                  "max_rel_residual": float(f"{max(res['mixed'][-1]['errs']):.3e}")})
     out["config"].update({"n": n, "nnz": int(csr[0][-1]), "nev": nev, "ncv": ncv, "tol": tol, "which": "LR"})
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def run_copycal(args):
@@ -443,16 +513,18 @@ def run_serve(args):
         r["_i"] = i
     # served: concurrent, admitted against the budget; first with each request
     # launching its own steps (streams only), then with grouped steps
-    srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=args.slots, grouped=False)
-    res_ng, wall_ng = S.run_stream_native(srv, reqs, lambda r: cache[r["_i"]], tol=1e-10)
-    srv.close()
-    good_ng = sum(r["ngood"] for r in res_ng)
-    log(f"[serve] streams only: {wall_ng:.2f}s, {good_ng} verified pairs")
-    # the same grouped serving driven from Python (arrivals and progress calls through ctypes)
-    srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=args.slots, grouped=True)
-    res_py, wall_py = S.run_stream(srv, reqs, lambda r: cache[r["_i"]], tol=1e-10)
-    srv.close()
-    log(f"[serve] grouped, Python-driven loop: {wall_py:.2f}s")
+    variants = getattr(args, "variants", True)
+    if variants:
+        srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=args.slots, grouped=False)
+        res_ng, wall_ng = S.run_stream_native(srv, reqs, lambda r: cache[r["_i"]], tol=1e-10)
+        srv.close()
+        good_ng = sum(r["ngood"] for r in res_ng)
+        log(f"[serve] streams only: {wall_ng:.2f}s, {good_ng} verified pairs")
+        # the same grouped serving driven from Python (arrivals and progress calls through ctypes)
+        srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=args.slots, grouped=True)
+        res_py, wall_py = S.run_stream(srv, reqs, lambda r: cache[r["_i"]], tol=1e-10)
+        srv.close()
+        log(f"[serve] grouped, Python-driven loop: {wall_py:.2f}s")
     srv = S.Server(budget=int(args.budget_gb * (1 << 30)), max_active=args.slots, grouped=True)
     results, wall = S.run_stream_native(srv, reqs, lambda r: cache[r["_i"]], tol=1e-10)
     stats = srv.stats()
@@ -490,22 +562,26 @@ def run_serve(args):
                             "members_per_grouped_step": round(stats["group_member_steps"] / max(1, stats["group_steps"]), 2)},
                  "served_python_driven": {"eigenpairs_per_s": round(sum(r["ngood"] for r in res_py) / wall_py, 4),
                                           "wall_s": round(wall_py, 3),
-                                          "latency_ms_p50": round(float(np.percentile([r["latency"] * 1e3 for r in res_py], 50)), 2)},
+                                          "latency_ms_p50": round(float(np.percentile([r["latency"] * 1e3 for r in res_py], 50)), 2)}
+                 if variants else None,
                  "served_streams_only": {"eigenpairs_per_s": round(good_ng / wall_ng, 4), "wall_s": round(wall_ng, 3),
                                          "verified_pairs": good_ng,
                                          "latency_ms_p50": round(float(np.percentile([r["latency"] * 1e3 for r in res_ng], 50)), 2),
-                                         "latency_ms_p99": round(float(np.percentile([r["latency"] * 1e3 for r in res_ng], 99)), 2)},
+                                         "latency_ms_p99": round(float(np.percentile([r["latency"] * 1e3 for r in res_ng], 99)), 2)}
+                 if variants else None,
                  "sequential": {"eigenpairs_per_s": round(seq_rate, 4), "wall_s": round(seq_wall, 3),
                                 "verified_pairs": seq_good,
                                 "latency_ms_p50": round(float(np.percentile(np.array(seq_lat) * 1e3, 50)), 2)},
                  "speedup_vs_sequential": round(rate / seq_rate, 3) if seq_rate else None})
     out["config"].update({"requests": len(reqs), "qps": args.qps, "slots": args.slots})
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def run_scale(args):
     """The at-scale mixed-precision serving case: the shifted normalized Laplacian
-    of an R-MAT graph (scale --rmat-scale, edge factor 16, FP32 values), nev 32
+    of an R-MAT graph (scale --rmat-scale, 8 n edges drawn and symmetrized, so
+    an average degree of 16 -- about 268M stored off-diagonal entries at scale
+    24 --, FP32 values), nev 32
     largest, ncv 96, FP32 phase to 1e-5 then FP64 restarts to 1e-10, served
     together with 8 convection-diffusion requests of the mixed-precision shape
     (2048^2, nev 8 largest real part, ncv 40); beside it the same nine requests
@@ -514,7 +590,7 @@ def run_scale(args):
     from slepc_amd import matgen as M
     from slepc_amd import serve as S
     t0 = time.time()
-    rmat = M.rmat_shifted_normalized_laplacian(args.rmat_scale, 16, seed=5)
+    rmat = M.rmat_shifted_normalized_laplacian(args.rmat_scale, RMAT_EDGE_FACTOR, seed=5)
     n5 = len(rmat[0]) - 1
     log(f"[scale] R-MAT scale {args.rmat_scale}: n={n5} nnz={int(rmat[0][-1])} built in {time.time() - t0:.1f}s")
     t0 = time.time()
@@ -558,7 +634,7 @@ def run_scale(args):
     rate = good / wall
     out = _base(args, rate, "eigenpairs/s", 1e3 * wall, 1, "f32+f64",
                 f"mixed-precision serving at scale: 2I - normalized Laplacian of R-MAT scale {args.rmat_scale} "
-                f"(a,b,c,d = .57,.19,.19,.05, edge factor 16, FP32 values), nev 32 largest, ncv 96, FP32 phase to "
+                f"(a,b,c,d = .57,.19,.19,.05, average degree 16, FP32 values), nev 32 largest, ncv 96, FP32 phase to "
                 f"1e-5 then FP64 to 1e-10, served with {len(cds)} mixed-precision convection-diffusion "
                 f"{args.mixed_grid}^2 requests (nev 8 LR, ncv 40)",
                 {"latency_ms": {"p50": round(float(np.percentile([o["latency"] * 1e3 for o in res], 50)), 2),
@@ -575,7 +651,7 @@ def run_scale(args):
                  "speedup_vs_sequential_fp64": round(rate / seq["eigenpairs_per_s"], 3) if seq else None})
     out["config"].update({"rmat_scale": args.rmat_scale, "rmat_n": n5, "rmat_nnz": int(rmat[0][-1]),
                           "convdiff_requests": len(cds), "convdiff_grid": args.mixed_grid})
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def C_byref(x):

@@ -378,8 +378,9 @@ def _rmat_laplacian_native(scale, edge_factor, seed, abcd):
 def rmat_shifted_normalized_laplacian(scale: int, edge_factor: int = 16, seed: int = 5,
                                       abcd=(0.57, 0.19, 0.19, 0.05), dtype=np.float32, native: bool = True):
     """M = 2I - (I - D^-1/2 A D^-1/2) = I + D^-1/2 A D^-1/2 of the R-MAT graph
-    symmetrized (A = pattern of E + E^T), with self-loops and duplicate edges
-    dropped; isolated vertices get M_ii = 1 (normalized Laplacian L_ii = 1 for
+    symmetrized (A = pattern of E + E^T, E the edge_factor * 2^scale drawn
+    edges, so the average degree is at most 2 edge_factor), with self-loops
+    and duplicate edges dropped; isolated vertices get M_ii = 1 (normalized Laplacian L_ii = 1 for
     them, the convention with D^-1/2 = 0 on isolated vertices).  Off-diagonal
     values are d_i^-1/2 d_j^-1/2 formed in FP64 and rounded once to FP32.
     Eigenvalues lie in [0, 2]; the largest of M are the smallest of the

@@ -90,6 +90,19 @@ def test_rmat_shifted_normalized_laplacian():
     assert np.allclose(A @ u, 2.0 * u)
 
 
+def test_rmat_average_degree():
+    # 8 n drawn edges, symmetrized: the average degree approaches 16 from below as
+    # the duplicates of the skewed quadrant choice thin out with the scale (the
+    # 2^24 workload: ~15.5, ~268M stored off-diagonal entries)
+    prev = 0.0
+    for scale in (12, 14):
+        rowptr = M.rmat_shifted_normalized_laplacian(scale, 8, seed=5)[0]
+        n = 1 << scale
+        deg = (int(rowptr[-1]) - n) / n
+        assert prev < deg <= 16.0 and deg > 12.5
+        prev = deg
+
+
 def test_rmat_native_generator_matches_numpy_restatement():
     # the OpenMP generator (slepc_amd/gen/rmat.c) and the numpy restatement of the
     # same counter-based steps build the same CSR, bit for bit
