@@ -1054,7 +1054,8 @@ typedef struct {
   int n, nev, ncv, mpd, max_it, which, hermitian, lock, nconv, its, reason;
   int harmonic, dsherm;           /* dsherm: Lanczos + DSHEP (Hermitian and Ritz extraction) */
   int twosided;
-  int sinvert, pc_jacobi, ksp_maxits;   /* shift-and-invert: (A - sigma I)^-1 by PCG */
+  int sinvert, pc_jacobi, ksp_maxits;   /* shift-and-invert: (A - sigma I)^-1 by PCG or BiCGStab */
+  int ksp_bcgs;
   double sigma, ksp_rtol, *dinv, *kwork;
   long ksp_its;
   int ksp_failed;
@@ -1100,10 +1101,72 @@ static int pcg_solve(oeps *eps, const double *b, double *x)
   OR_ERR(-91, "KSP did not converge in %d iterations", eps->ksp_maxits);
 }
 
+/* y = B (A - sigma I) x, the left-preconditioned operator of the BiCGStab solve */
+static void bcgs_op(oeps *eps, const double *x, double *y)
+{
+  or_spmv(eps->n, eps->rowptr, eps->colidx, eps->val, x, y);
+  for (int i = 0; i < eps->n; i++) y[i] = eps->dinv[i] * (y[i] - eps->sigma * x[i]);
+}
+
+/* BiCGStab (H. A. van der Vorst, SIAM J. Sci. Stat. Comput. 13 (1992) 631-644) on
+   B (A - sigma I) x = B b, left preconditioned with B = diag(A - sigma I)^-1
+   (PCJACOBI) or I, as PETSc's KSPBCGS with its default left preconditioning:
+   x0 = 0, r = B b, shadow rhat = r; per iteration beta = (rho/rho_old)(alpha/omega),
+   p = r + beta (p - omega v), v = K p, alpha = rho / (rhat, v), s = r - alpha v; if
+   |s| <= ttol then x += alpha p and stop; t = K s, omega = (t, s)/(t, t),
+   x += alpha p + omega s, r = s - omega t; converged when |r| <= max(rtol |B b|,
+   abstol).  A zero rho, (rhat, v), (t, t) or omega is a breakdown.  PETSc is not
+   part of the reference tree: this restates the published method; the
+   reference's default inner solver for shift-and-invert is a direct solve
+   (stsles.c:47-54), KSPBCGS being the iterative choice for an indefinite shifted
+   matrix. */
+static int bcgs_solve(oeps *eps, const double *b, double *x)
+{
+  int n = eps->n;
+  double *r = eps->kwork, *rh = r + n, *p = rh + n, *v = p + n, *sv = v + n, *t = sv + n;
+  double rho, rho_old = 1.0, alpha = 1.0, omega = 1.0, rr = 0.0;
+  for (int i = 0; i < n; i++) {
+    x[i] = 0.0; r[i] = eps->dinv[i] * b[i]; rh[i] = r[i]; p[i] = 0.0; v[i] = 0.0; rr += r[i] * r[i];
+  }
+  double ttol = fmax(eps->ksp_rtol * sqrt(rr), 1e-50);
+  if (sqrt(rr) <= ttol) return 0;
+  rho = rr;
+  for (int it = 0; it < eps->ksp_maxits; it++) {
+    double beta = (rho / rho_old) * (alpha / omega), rv = 0.0, ss = 0.0, ts = 0.0, tt = 0.0, rhr = 0.0;
+    for (int i = 0; i < n; i++) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+    bcgs_op(eps, p, v);
+    for (int i = 0; i < n; i++) rv += rh[i] * v[i];
+    if (rv == 0.0) OR_ERR(-91, "KSP breakdown: (rhat, v) = 0");
+    alpha = rho / rv;
+    for (int i = 0; i < n; i++) { sv[i] = r[i] - alpha * v[i]; ss += sv[i] * sv[i]; }
+    eps->ksp_its++;
+    if (sqrt(ss) <= ttol) {
+      for (int i = 0; i < n; i++) x[i] += alpha * p[i];
+      return 0;
+    }
+    bcgs_op(eps, sv, t);
+    for (int i = 0; i < n; i++) { ts += t[i] * sv[i]; tt += t[i] * t[i]; }
+    if (tt == 0.0) OR_ERR(-91, "KSP breakdown: (t, t) = 0");
+    omega = ts / tt;
+    rr = 0.0;
+    for (int i = 0; i < n; i++) {
+      x[i] += alpha * p[i] + omega * sv[i];
+      r[i] = sv[i] - omega * t[i];
+      rr += r[i] * r[i];
+      rhr += rh[i] * r[i];
+    }
+    if (sqrt(rr) <= ttol) return 0;
+    rho_old = rho;
+    rho = rhr;
+    if (rho == 0.0 || omega == 0.0) OR_ERR(-91, "KSP breakdown: rho or omega = 0");
+  }
+  OR_ERR(-91, "KSP did not converge in %d iterations", eps->ksp_maxits);
+}
+
 static void op_mult(oeps *eps, const double *x, double *y)
 {
   if (eps->sinvert) {
-    if (pcg_solve(eps, x, y)) eps->ksp_failed = 1;
+    if ((eps->ksp_bcgs ? bcgs_solve(eps, x, y) : pcg_solve(eps, x, y))) eps->ksp_failed = 1;
   } else or_spmv(eps->n, eps->rowptr, eps->colidx, eps->val, x, y);
   eps->nmatmult++;
 }
@@ -1535,6 +1598,7 @@ int or_eps_solve_twosided(int n, const int *rowptr, const int *colidx, const dou
     eps.ksp_rtol = opts->ksp_rtol > 0 ? opts->ksp_rtol : 1e-8;
     eps.ksp_maxits = 10000;
     eps.pc_jacobi = !opts->pc_none;
+    eps.ksp_bcgs = opts->ksp_bcgs != 0;
   }
 
   /* EPSSetUp (epssetup.c:344-345) and EPSSetDimensions_Default (epssetup.c:648-671) */
@@ -1577,7 +1641,7 @@ int or_eps_solve_twosided(int n, const int *rowptr, const int *colidx, const dou
   if (eps.sinvert) {
     eps.ds.sc.map = 1; eps.ds.sc.sigma = eps.sigma;
     eps.dinv = (double*)malloc(sizeof(double) * (size_t)n);
-    eps.kwork = (double*)malloc(sizeof(double) * 4 * (size_t)n);
+    eps.kwork = (double*)malloc(sizeof(double) * 6 * (size_t)n);
     if (!eps.dinv || !eps.kwork) { e = -55; goto done; }
     for (int i = 0; i < n; i++) {                                  /* PCJACOBI on A - sigma I */
       double d = 0.0;

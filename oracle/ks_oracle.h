@@ -45,6 +45,7 @@ typedef struct {
   int    st_sinvert;             /* 1: STSINVERT with shift = target, inner PCG */
   int    pc_none;                /* 1: PCG without the Jacobi preconditioner */
   double ksp_rtol;               /* <= 0 means 1e-8 */
+  int    ksp_bcgs;               /* 1: BiCGStab instead of CG for the inner solve */
 } or_opts;
 
 typedef struct {

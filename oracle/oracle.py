@@ -27,7 +27,7 @@ class Opts(C.Structure):
                 ("hermitian", C.c_int), ("keep", C.c_double), ("lock", C.c_int),
                 ("eta", C.c_double), ("refine", C.c_int), ("nthreads", C.c_int),
                 ("restart_cap", C.c_int), ("extraction", C.c_int), ("twosided", C.c_int),
-                ("st_sinvert", C.c_int), ("pc_none", C.c_int), ("ksp_rtol", C.c_double)]
+                ("st_sinvert", C.c_int), ("pc_none", C.c_int), ("ksp_rtol", C.c_double), ("ksp_bcgs", C.c_int)]
 
 
 class Info(C.Structure):
@@ -168,13 +168,13 @@ def eps_solve(csr, nev: int, ncv: int = 0, mpd: int = 0, tol: float = 1e-8,
               v0=None, deflation=None, max_it: int = 0, keep: float = 0.5, lock: bool = True,
               eta: float = 0.7071, refine: int = REFINE_IFNEEDED, nthreads: int = 0,
               restart_cap: int = 0, want_vectors: bool = True, extraction: int = 0, twosided: bool = False,
-              sinvert: bool = False, pc_none: bool = False, ksp_rtol: float = 0.0):
+              sinvert: bool = False, pc_none: bool = False, ksp_rtol: float = 0.0, ksp: str = "cg"):
     rowptr, colidx, val = _csr_args(csr)
     n = len(rowptr) - 1
     o = Opts(nev=nev, ncv=ncv, mpd=mpd, max_it=max_it, tol=tol, which=which, target=target,
              hermitian=int(hermitian), keep=keep, lock=int(lock), eta=eta, refine=refine,
              nthreads=nthreads, restart_cap=restart_cap, extraction=extraction, twosided=int(twosided),
-             st_sinvert=int(sinvert), pc_none=int(pc_none), ksp_rtol=ksp_rtol)
+             st_sinvert=int(sinvert), pc_none=int(pc_none), ksp_rtol=ksp_rtol, ksp_bcgs=int(ksp == "bcgs"))
     ncv_eff = ncv if ncv > 0 else max(2 * nev, nev + 15)
     ncv_eff = min(ncv_eff, n) if ncv_eff > 0 else n
     size = max(ncv_eff, nev) + 2

@@ -50,3 +50,33 @@ def test_sinvert_closed_form_near_shift(oracle):
 def test_sinvert_requires_target_which(oracle):
     with pytest.raises(RuntimeError):
         oracle.eps_solve(M.laplacian_1d(50), nev=2, which=oracle.LARGEST_MAGNITUDE, sinvert=True)
+
+
+# eps test2 "Interior eigenvalues" (src/eps/tests/test2.c, output/test2_1.out): the 1-D
+# Laplacian, target 2.1, whose shifted matrix A - 2.1 I is indefinite
+TEST2_INTERIOR = {30: "2.10130, 1.89870, 2.30286, 2.50131", 32: "2.09516, 2.28463, 1.90484, 2.47152"}
+
+
+@pytest.mark.parametrize("n", [30, 32])
+def test_interior_indefinite_sinvert_bcgs_golden(oracle, n):
+    # shift-and-invert at the target with BiCGStab as the inner solver reaches the
+    # reference's interior goldens; CG breaks down on the indefinite shifted matrix
+    csr = M.laplacian_1d(n)
+    r = oracle.eps_solve(csr, nev=4, tol=1e-10, which=oracle.TARGET_MAGNITUDE, target=2.1, sinvert=True,
+                         ksp="bcgs", ksp_rtol=1e-12)
+    assert r["reason"] == 1 and ", ".join("%.5f" % v for v in r["eigr"][:4]) == TEST2_INTERIOR[n]
+    closed = 2 - 2 * np.cos(np.arange(1, n + 1) * np.pi / (n + 1))
+    near = closed[np.argsort(np.abs(closed - 2.1))][:4]
+    assert np.allclose(r["eigr"][:4], near, rtol=1e-10, atol=0)
+    assert (r["err"][:4] < 1e-9).all() and r["ksp_its"] > 0
+    with pytest.raises(RuntimeError):
+        oracle.eps_solve(csr, nev=4, tol=1e-10, which=oracle.TARGET_MAGNITUDE, target=2.1, sinvert=True)
+
+
+def test_bcgs_matches_cg_on_a_definite_shift(oracle):
+    # on a positive definite A - sigma I both inner solvers give the same eigenpairs
+    csr = M.grid_graph_laplacian(10)
+    kw = dict(nev=4, ncv=20, tol=1e-10, which=oracle.TARGET_MAGNITUDE, target=-0.05, sinvert=True, ksp_rtol=1e-12)
+    a = oracle.eps_solve(csr, ksp="cg", **kw)
+    b = oracle.eps_solve(csr, ksp="bcgs", **kw)
+    assert np.allclose(a["eigr"][:4], b["eigr"][:4], rtol=1e-10, atol=1e-12)
