@@ -7,17 +7,34 @@
 // gradient method in PETSc's KSPCG form (Hestenes-Stiefel recurrences, the
 // preconditioned residual norm |B r| for the test rnorm <= max(rtol |B b|, abstol))
 // with B = diag(A - sigma I)^-1 (zero diagonal entries replaced by 1, as PCJACOBI
-// does).  Every scalar of the recurrence stays on the device; the host reads one
-// norm per iteration to test convergence.  Reductions add per-workgroup partial
-// sums in a fixed order in a one-workgroup launch, so a solve is deterministic.
+// does), and for an indefinite shifted matrix BiCGStab (van der Vorst 1992) in
+// PETSc's KSPBCGS form with the same left preconditioner.  Every scalar of a
+// recurrence and the convergence test stay on the device: the finishing
+// launch of an iteration decides, and once a solve has converged or failed the
+// later launches of the chunk exit at their first instruction (the products
+// through a status of the solver's own), so the host enqueues iterations in
+// chunks and reads one small state block per chunk.  Reductions add
+// per-workgroup partial sums in a fixed order in a one-workgroup launch, so a
+// solve is deterministic.
+#include <cstring>
+
 #include "ks_internal.hpp"
 
 namespace {
 
-#define KS_PCG_BLOCKS 512   // workgroups of the vector kernels (partials: 2 per workgroup)
+#define KS_PCG_BLOCKS 512   // workgroups of the vector kernels (partials: 3 per workgroup)
 
-// scalar slots in d_s: 0 rz, 1 previous rz, 2 p.q, 3 z.z
-enum { S_RZ = 0, S_RZOLD = 1, S_PQ = 2, S_ZZ = 3 };
+// the solver state after the vectors of the workspace: the gate of its
+// products (gate.active = 1 while iterating), the scalars, the iteration count
+// and the outcome (reason 0 while iterating, > 0 converged, < 0 failed)
+struct ksp_state {
+  ks_status gate;
+  double sc[16];
+  int32_t its, reason, sconv, maxits;
+};
+static_assert(sizeof(ksp_state) <= 64 * sizeof(double), "the solver state takes 64 doubles");
+enum { S_RZ = 0, S_RZOLD, S_PQ, S_ZZ, S_TTOL, S_RNORM,                                  // CG
+       S_RHO = 0, S_RHOOLD = 1, S_ALPHA = 6, S_OMEGA, S_RV, S_SS, S_TS, S_TT, S_RR, S_RHR };   // BiCGStab
 
 __device__ inline double block_sum(double v, double *lds) {
   // wave64 sum, then the four waves of the workgroup in order
@@ -32,12 +49,39 @@ __device__ inline double block_sum(double v, double *lds) {
   return t;
 }
 
+// partial sums of up to three products, per workgroup, stored at part[3 b + j]
+__device__ inline void store_partials(double *part, double a, double b, double c, double *lds) {
+  a = block_sum(a, lds);
+  b = block_sum(b, lds);
+  c = block_sum(c, lds);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = a;
+    part[3 * blockIdx.x + 1] = b;
+    part[3 * blockIdx.x + 2] = c;
+  }
+}
+
+// the three column sums of the partials, in workgroup order (thread 0)
+__device__ inline void sum_partials(const double *part, int nb, double *out, double *lds) {
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int q = threadIdx.x; q < nb; q += KS_BLOCK) {
+    a += part[3 * q];
+    b += part[3 * q + 1];
+    c += part[3 * q + 2];
+  }
+  out[0] = block_sum(a, lds);
+  out[1] = block_sum(b, lds);
+  out[2] = block_sum(c, lds);
+}
+
+#define KS_FOR_ROWS(i, n) for (int i = blockIdx.x * KS_BLOCK + threadIdx.x; i < n; i += gridDim.x * KS_BLOCK)
+
 // dinv[i] = 1 / (a_ii - sigma), 1 where that is zero; all ones without a preconditioner
 __global__ __launch_bounds__(KS_BLOCK) void k_jacobi(const int32_t *__restrict__ rowptr,
                                                     const int32_t *__restrict__ colidx,
                                                     const double *__restrict__ val, int n, double sigma, int jacobi,
                                                     double *__restrict__ dinv) {
-  for (int i = blockIdx.x * KS_BLOCK + threadIdx.x; i < n; i += gridDim.x * KS_BLOCK) {
+  KS_FOR_ROWS(i, n) {
     double d = 0.0;
     if (jacobi) {
       for (int32_t e = rowptr[i]; e < rowptr[i + 1]; e++)
@@ -48,13 +92,14 @@ __global__ __launch_bounds__(KS_BLOCK) void k_jacobi(const int32_t *__restrict__
   }
 }
 
+// ---------------------------------------------------------------- CG
 // x = 0, r = b, z = B r; partials of z.r and z.z
 __global__ __launch_bounds__(KS_BLOCK) void k_pcg_start(int n, const double *__restrict__ b, double *__restrict__ x,
                                                        double *__restrict__ r, double *__restrict__ z,
                                                        const double *__restrict__ dinv, double *__restrict__ part) {
   __shared__ double lds[KS_BLOCK / 64];
   double zr = 0.0, zz = 0.0;
-  for (int i = blockIdx.x * KS_BLOCK + threadIdx.x; i < n; i += gridDim.x * KS_BLOCK) {
+  KS_FOR_ROWS(i, n) {
     const double ri = b[i], zi = dinv[i] * ri;
     x[i] = 0.0;
     r[i] = ri;
@@ -62,73 +107,70 @@ __global__ __launch_bounds__(KS_BLOCK) void k_pcg_start(int n, const double *__r
     zr += zi * ri;
     zz += zi * zi;
   }
-  zr = block_sum(zr, lds);
-  zz = block_sum(zz, lds);
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = zr;
-    part[2 * blockIdx.x + 1] = zz;
-  }
+  store_partials(part, zr, zz, 0.0, lds);
 }
 
-// the new z.r and z.z from the partials; the old z.r moves to S_RZOLD
-__global__ __launch_bounds__(KS_BLOCK) void k_pcg_fin_rz(const double *__restrict__ part, int nb,
-                                                        double *__restrict__ s) {
+// first z.r and z.z; the convergence threshold from |B b|
+__global__ __launch_bounds__(KS_BLOCK) void k_pcg_fin_start(const double *__restrict__ part, int nb, ksp_state *S,
+                                                           double rtol, double abstol, int maxits) {
   __shared__ double lds[KS_BLOCK / 64];
-  double zr = 0.0, zz = 0.0;
-  for (int b = threadIdx.x; b < nb; b += KS_BLOCK) {
-    zr += part[2 * b];
-    zz += part[2 * b + 1];
-  }
-  zr = block_sum(zr, lds);
-  zz = block_sum(zz, lds);
+  double t[3];
+  sum_partials(part, nb, t, lds);
   if (threadIdx.x == 0) {
-    s[S_RZOLD] = s[S_RZ];
-    s[S_RZ] = zr;
-    s[S_ZZ] = zz;
+    const double rnorm0 = sqrt(t[1]);
+    S->sc[S_RZ] = t[0];
+    S->sc[S_ZZ] = t[1];
+    S->sc[S_RNORM] = rnorm0;
+    S->sc[S_TTOL] = fmax(rtol * rnorm0, abstol);
+    S->its = 0;
+    S->maxits = maxits;
+    S->reason = rnorm0 <= S->sc[S_TTOL] ? 1 : 0;
+    S->gate.active = S->reason ? 0 : 1;
   }
 }
 
 // p = z (first iteration) or z + (rz / rzold) p
 __global__ __launch_bounds__(KS_BLOCK) void k_pcg_p(int n, const double *__restrict__ z, double *__restrict__ p,
-                                                   const double *__restrict__ s, int first) {
-  const double beta = first ? 0.0 : s[S_RZ] / s[S_RZOLD];
-  for (int i = blockIdx.x * KS_BLOCK + threadIdx.x; i < n; i += gridDim.x * KS_BLOCK)
-    p[i] = first ? z[i] : z[i] + beta * p[i];
+                                                   const ksp_state *S) {
+  if (S->reason) return;
+  const bool first = S->its == 0;
+  const double beta = first ? 0.0 : S->sc[S_RZ] / S->sc[S_RZOLD];
+  KS_FOR_ROWS(i, n) p[i] = first ? z[i] : z[i] + beta * p[i];
 }
 
 // q -= sigma p (q = A p on entry); partials of p.q
 __global__ __launch_bounds__(KS_BLOCK) void k_pcg_q(int n, const double *__restrict__ p, double *__restrict__ q,
-                                                   double sigma, double *__restrict__ part) {
+                                                   double sigma, const ksp_state *S, double *__restrict__ part) {
   __shared__ double lds[KS_BLOCK / 64];
+  if (S->reason) return;
   double pq = 0.0;
-  for (int i = blockIdx.x * KS_BLOCK + threadIdx.x; i < n; i += gridDim.x * KS_BLOCK) {
+  KS_FOR_ROWS(i, n) {
     const double qi = q[i] - sigma * p[i];
     q[i] = qi;
     pq += p[i] * qi;
   }
-  pq = block_sum(pq, lds);
-  if (threadIdx.x == 0) part[2 * blockIdx.x] = pq;
+  store_partials(part, pq, 0.0, 0.0, lds);
 }
 
-__global__ __launch_bounds__(KS_BLOCK) void k_pcg_fin_pq(const double *__restrict__ part, int nb,
-                                                        double *__restrict__ s) {
+__global__ __launch_bounds__(KS_BLOCK) void k_pcg_fin_pq(const double *__restrict__ part, int nb, ksp_state *S) {
   __shared__ double lds[KS_BLOCK / 64];
-  double pq = 0.0;
-  for (int b = threadIdx.x; b < nb; b += KS_BLOCK) pq += part[2 * b];
-  pq = block_sum(pq, lds);
-  if (threadIdx.x == 0) s[S_PQ] = pq;
+  if (S->reason) return;
+  double t[3];
+  sum_partials(part, nb, t, lds);
+  if (threadIdx.x == 0) S->sc[S_PQ] = t[0];
 }
 
 // a = rz / pq; x += a p; r -= a q; z = B r; partials of z.r and z.z
 __global__ __launch_bounds__(KS_BLOCK) void k_pcg_xr(int n, const double *__restrict__ p,
                                                     const double *__restrict__ q, double *__restrict__ x,
                                                     double *__restrict__ r, double *__restrict__ z,
-                                                    const double *__restrict__ dinv, const double *__restrict__ s,
+                                                    const double *__restrict__ dinv, const ksp_state *S,
                                                     double *__restrict__ part) {
   __shared__ double lds[KS_BLOCK / 64];
-  const double a = s[S_RZ] / s[S_PQ];
+  if (S->reason) return;
+  const double a = S->sc[S_RZ] / S->sc[S_PQ];
   double zr = 0.0, zz = 0.0;
-  for (int i = blockIdx.x * KS_BLOCK + threadIdx.x; i < n; i += gridDim.x * KS_BLOCK) {
+  KS_FOR_ROWS(i, n) {
     x[i] += a * p[i];
     const double ri = r[i] - a * q[i], zi = dinv[i] * ri;
     r[i] = ri;
@@ -136,26 +178,236 @@ __global__ __launch_bounds__(KS_BLOCK) void k_pcg_xr(int n, const double *__rest
     zr += zi * ri;
     zz += zi * zi;
   }
-  zr = block_sum(zr, lds);
-  zz = block_sum(zz, lds);
+  store_partials(part, zr, zz, 0.0, lds);
+}
+
+// the new z.r and z.z, and the tests of the iteration: p^T (A - sigma I) p <= 0
+// (KSP_DIVERGED_INDEFINITE_MAT, -8), |B r| <= ttol (1), the iteration limit (-3)
+__global__ __launch_bounds__(KS_BLOCK) void k_pcg_fin_iter(const double *__restrict__ part, int nb, ksp_state *S) {
+  __shared__ double lds[KS_BLOCK / 64];
+  if (S->reason) return;
+  double t[3];
+  sum_partials(part, nb, t, lds);
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = zr;
-    part[2 * blockIdx.x + 1] = zz;
+    S->sc[S_RZOLD] = S->sc[S_RZ];
+    S->sc[S_RZ] = t[0];
+    S->sc[S_ZZ] = t[1];
+    S->sc[S_RNORM] = sqrt(t[1]);
+    S->its += 1;
+    if (!(S->sc[S_PQ] > 0.0)) S->reason = -8;
+    else if (S->sc[S_RNORM] <= S->sc[S_TTOL]) S->reason = 1;
+    else if (S->its >= S->maxits) S->reason = -3;
+    S->gate.active = S->reason ? 0 : 1;
   }
 }
 
-inline int pcg_blocks(int n) {
+// ---------------------------------------------------------------- BiCGStab
+// x = 0, r = B b, rhat = r, p = v = 0; partials of r.r
+__global__ __launch_bounds__(KS_BLOCK) void k_bcgs_start(int n, const double *__restrict__ b, double *__restrict__ x,
+                                                        double *__restrict__ r, double *__restrict__ rh,
+                                                        double *__restrict__ p, double *__restrict__ v,
+                                                        const double *__restrict__ dinv, double *__restrict__ part) {
+  __shared__ double lds[KS_BLOCK / 64];
+  double rr = 0.0;
+  KS_FOR_ROWS(i, n) {
+    const double ri = dinv[i] * b[i];
+    x[i] = 0.0;
+    r[i] = ri;
+    rh[i] = ri;
+    p[i] = 0.0;
+    v[i] = 0.0;
+    rr += ri * ri;
+  }
+  store_partials(part, rr, 0.0, 0.0, lds);
+}
+
+__global__ __launch_bounds__(KS_BLOCK) void k_bcgs_fin_start(const double *__restrict__ part, int nb, ksp_state *S,
+                                                            double rtol, double abstol, int maxits) {
+  __shared__ double lds[KS_BLOCK / 64];
+  double t[3];
+  sum_partials(part, nb, t, lds);
+  if (threadIdx.x == 0) {
+    const double rnorm0 = sqrt(t[0]);
+    S->sc[S_RHO] = t[0];   // (rhat, r) with rhat = r
+    S->sc[S_RHOOLD] = 1.0;
+    S->sc[S_ALPHA] = 1.0;
+    S->sc[S_OMEGA] = 1.0;
+    S->sc[S_RNORM] = rnorm0;
+    S->sc[S_TTOL] = fmax(rtol * rnorm0, abstol);
+    S->its = 0;
+    S->sconv = 0;
+    S->maxits = maxits;
+    S->reason = rnorm0 <= S->sc[S_TTOL] ? 1 : 0;
+    S->gate.active = S->reason ? 0 : 1;
+  }
+}
+
+// p = r + beta (p - omega v), beta = (rho / rho_old) (alpha / omega)
+__global__ __launch_bounds__(KS_BLOCK) void k_bcgs_p(int n, const double *__restrict__ r, double *__restrict__ p,
+                                                    const double *__restrict__ v, const ksp_state *S) {
+  if (S->reason) return;
+  const double beta = (S->sc[S_RHO] / S->sc[S_RHOOLD]) * (S->sc[S_ALPHA] / S->sc[S_OMEGA]);
+  const double om = S->sc[S_OMEGA];
+  KS_FOR_ROWS(i, n) p[i] = r[i] + beta * (p[i] - om * v[i]);
+}
+
+// y = B (y - sigma x) (y = A x on entry); partials of (h, y), (x, y), (y, y)
+__global__ __launch_bounds__(KS_BLOCK) void k_bcgs_apply(int n, const double *__restrict__ x, double *__restrict__ y,
+                                                        const double *__restrict__ h, const double *__restrict__ dinv,
+                                                        double sigma, const ksp_state *S, int skip_sconv,
+                                                        double *__restrict__ part) {
+  __shared__ double lds[KS_BLOCK / 64];
+  if (S->reason || (skip_sconv && S->sconv)) return;
+  double hy = 0.0, xy = 0.0, yy = 0.0;
+  KS_FOR_ROWS(i, n) {
+    const double yi = dinv[i] * (y[i] - sigma * x[i]);
+    y[i] = yi;
+    hy += h[i] * yi;
+    xy += x[i] * yi;
+    yy += yi * yi;
+  }
+  store_partials(part, hy, xy, yy, lds);
+}
+
+// alpha = rho / (rhat, v); a zero denominator is a breakdown (-5)
+__global__ __launch_bounds__(KS_BLOCK) void k_bcgs_fin_alpha(const double *__restrict__ part, int nb, ksp_state *S) {
+  __shared__ double lds[KS_BLOCK / 64];
+  if (S->reason) return;
+  double t[3];
+  sum_partials(part, nb, t, lds);
+  if (threadIdx.x == 0) {
+    S->sc[S_RV] = t[0];
+    if (t[0] == 0.0) {
+      S->reason = -5;
+      S->gate.active = 0;
+    } else S->sc[S_ALPHA] = S->sc[S_RHO] / t[0];
+  }
+}
+
+// s = r - alpha v; partials of (s, s)
+__global__ __launch_bounds__(KS_BLOCK) void k_bcgs_s(int n, const double *__restrict__ r, const double *__restrict__ v,
+                                                    double *__restrict__ sv, const ksp_state *S,
+                                                    double *__restrict__ part) {
+  __shared__ double lds[KS_BLOCK / 64];
+  if (S->reason) return;
+  const double a = S->sc[S_ALPHA];
+  double ss = 0.0;
+  KS_FOR_ROWS(i, n) {
+    const double si = r[i] - a * v[i];
+    sv[i] = si;
+    ss += si * si;
+  }
+  store_partials(part, ss, 0.0, 0.0, lds);
+}
+
+// |s| <= ttol: the iteration ends with x += alpha p (no product with s)
+__global__ __launch_bounds__(KS_BLOCK) void k_bcgs_fin_s(const double *__restrict__ part, int nb, ksp_state *S) {
+  __shared__ double lds[KS_BLOCK / 64];
+  if (S->reason) return;
+  double t[3];
+  sum_partials(part, nb, t, lds);
+  if (threadIdx.x == 0) {
+    S->sc[S_SS] = t[0];
+    S->sconv = sqrt(t[0]) <= S->sc[S_TTOL] ? 1 : 0;
+    S->gate.active = S->sconv ? 0 : 1;
+  }
+}
+
+// omega = (t, s) / (t, t); a zero (t, t) is a breakdown
+__global__ __launch_bounds__(KS_BLOCK) void k_bcgs_fin_omega(const double *__restrict__ part, int nb, ksp_state *S) {
+  __shared__ double lds[KS_BLOCK / 64];
+  if (S->reason || S->sconv) return;
+  double t[3];
+  sum_partials(part, nb, t, lds);
+  if (threadIdx.x == 0) {
+    // apply(x = s, y = t) returned (rhat, t), (s, t), (t, t)
+    S->sc[S_TS] = t[1];
+    S->sc[S_TT] = t[2];
+    if (t[2] == 0.0) {
+      S->reason = -5;
+      S->gate.active = 0;
+    } else S->sc[S_OMEGA] = t[1] / t[2];
+  }
+}
+
+// x += alpha p + omega s, r = s - omega t (or x += alpha p after an early |s|);
+// partials of (r, r) and (rhat, r)
+__global__ __launch_bounds__(KS_BLOCK) void k_bcgs_x(int n, double *__restrict__ x, const double *__restrict__ p,
+                                                    const double *__restrict__ sv, const double *__restrict__ t,
+                                                    double *__restrict__ r, const double *__restrict__ rh,
+                                                    const ksp_state *S, double *__restrict__ part) {
+  __shared__ double lds[KS_BLOCK / 64];
+  if (S->reason) return;
+  const double a = S->sc[S_ALPHA];
+  if (S->sconv) {
+    KS_FOR_ROWS(i, n) x[i] += a * p[i];
+    return;
+  }
+  const double om = S->sc[S_OMEGA];
+  double rr = 0.0, rhr = 0.0;
+  KS_FOR_ROWS(i, n) {
+    x[i] += a * p[i] + om * sv[i];
+    const double ri = sv[i] - om * t[i];
+    r[i] = ri;
+    rr += ri * ri;
+    rhr += rh[i] * ri;
+  }
+  store_partials(part, rr, rhr, 0.0, lds);
+}
+
+// the tests of the iteration: converged (1), breakdown (-5), iteration limit (-3)
+__global__ __launch_bounds__(KS_BLOCK) void k_bcgs_fin_iter(const double *__restrict__ part, int nb, ksp_state *S) {
+  __shared__ double lds[KS_BLOCK / 64];
+  if (S->reason) return;
+  double t[3];
+  const bool sconv = S->sconv != 0;
+  if (!sconv) sum_partials(part, nb, t, lds);
+  if (threadIdx.x == 0) {
+    S->its += 1;
+    if (sconv) {
+      S->sc[S_RNORM] = sqrt(S->sc[S_SS]);
+      S->reason = 1;
+    } else {
+      S->sc[S_RR] = t[0];
+      S->sc[S_RNORM] = sqrt(t[0]);
+      if (S->sc[S_RNORM] <= S->sc[S_TTOL]) S->reason = 1;
+      else {
+        S->sc[S_RHOOLD] = S->sc[S_RHO];
+        S->sc[S_RHO] = t[1];
+        if (t[1] == 0.0 || S->sc[S_OMEGA] == 0.0) S->reason = -5;
+        else if (S->its >= S->maxits) S->reason = -3;
+      }
+    }
+    S->gate.active = S->reason ? 0 : 1;
+  }
+}
+
+inline int ksp_blocks(int n) {
   int b = (n + KS_BLOCK - 1) / KS_BLOCK;
   if (b > KS_PCG_BLOCKS) b = KS_PCG_BLOCKS;
   return b < 1 ? 1 : b;
+}
+
+#define KS_KSP_CHUNK 8   // iterations enqueued between two reads of the solver state
+
+// waits for the stream, then the iteration count, the outcome and |B r|
+static int read_state(ks_ctx ctx, const ksp_state *S, int *its, int *reason, double *rnorm) {
+  ksp_state h;
+  KS_HIP(hipMemcpyAsync(ctx->h_scalar, S, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  KS_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(&h, ctx->h_scalar, sizeof(h));
+  *its = h.its;
+  *reason = h.reason;
+  *rnorm = h.sc[S_RNORM];
+  return KS_OK;
 }
 
 }  // namespace
 
 int ks_pcg_jacobi(ks_ctx ctx, ks_csr A, double sigma, int jacobi, double *d_dinv) {
   KS_CHECK(ctx && A && d_dinv, KS_ERR_ARG, "null argument");
-  KS_CHECK(A->vprec == KS_F64 && A->nrows == A->ncols, KS_ERR_ARG, "PCG needs a square FP64 matrix");
-  hipLaunchKernelGGL(k_jacobi, dim3(pcg_blocks(A->nrows)), dim3(KS_BLOCK), 0, ctx->stream, A->d_rowptr, A->d_colidx,
+  KS_CHECK(A->vprec == KS_F64 && A->nrows == A->ncols, KS_ERR_ARG, "the inner solve needs a square FP64 matrix");
+  hipLaunchKernelGGL(k_jacobi, dim3(ksp_blocks(A->nrows)), dim3(KS_BLOCK), 0, ctx->stream, A->d_rowptr, A->d_colidx,
                      (const double *)A->d_val, A->nrows, sigma, jacobi, d_dinv);
   KS_HIP(hipGetLastError());
   return KS_OK;
@@ -165,37 +417,72 @@ int ks_pcg_solve(ks_ctx ctx, ks_csr A, double sigma, const double *d_dinv, const
                  double rtol, double abstol, int maxits, double *d_work, int *its, double *rnorm, int *reason) {
   KS_CHECK(ctx && A && d_dinv && d_b && d_x && d_work && its && rnorm && reason, KS_ERR_ARG, "null PCG argument");
   KS_CHECK(A->vprec == KS_F64 && A->nrows == A->ncols, KS_ERR_ARG, "PCG needs a square FP64 matrix");
-  static_assert(2 * KS_PCG_BLOCKS <= (size_t)KS_MAXB * KS_NCOLP, "PCG partials exceed the workspace");
-  const int n = A->nrows, nb = pcg_blocks(n);
-  double *r = d_work, *z = r + n, *p = z + n, *q = p + n, *s = q + n;   // s: 4 scalars
-  double *part = ctx->d_partial, h[4];
+  KS_CHECK(maxits > 0, KS_ERR_ARG, "PCG needs maxits > 0");
+  static_assert(3 * KS_PCG_BLOCKS <= (size_t)KS_MAXB * KS_NCOLP, "partials exceed the workspace");
+  static_assert(sizeof(ksp_state) <= 64 * sizeof(double), "state exceeds the pinned scratch");
+  const int n = A->nrows, nb = ksp_blocks(n);
+  double *r = d_work, *z = r + n, *p = z + n, *q = p + n;
+  ksp_state *S = (ksp_state *)(q + n);
+  double *part = ctx->d_partial;
   hipStream_t st = ctx->stream;
-  *its = 0;
-  *reason = 0;
   hipLaunchKernelGGL(k_pcg_start, dim3(nb), dim3(KS_BLOCK), 0, st, n, d_b, d_x, r, z, d_dinv, part);
-  hipLaunchKernelGGL(k_pcg_fin_rz, dim3(1), dim3(KS_BLOCK), 0, st, (const double *)part, nb, s);
-  KS_HIP(hipMemcpyAsync(h, s, sizeof(h), hipMemcpyDeviceToHost, st));
-  KS_HIP(hipStreamSynchronize(st));
-  const double rnorm0 = sqrt(h[S_ZZ]);
-  const double ttol = fmax(rtol * rnorm0, abstol);
-  *rnorm = rnorm0;
-  if (rnorm0 <= ttol) { *reason = 1; return KS_OK; }
-  for (int i = 0; i < maxits; i++) {
-    hipLaunchKernelGGL(k_pcg_p, dim3(nb), dim3(KS_BLOCK), 0, st, n, (const double *)z, p, (const double *)s, i == 0);
-    int e = ks_spmv(ctx, A, p, q, KS_F64, 0);
-    if (e) return e;
-    hipLaunchKernelGGL(k_pcg_q, dim3(nb), dim3(KS_BLOCK), 0, st, n, (const double *)p, q, sigma, part);
-    hipLaunchKernelGGL(k_pcg_fin_pq, dim3(1), dim3(KS_BLOCK), 0, st, (const double *)part, nb, s);
-    hipLaunchKernelGGL(k_pcg_xr, dim3(nb), dim3(KS_BLOCK), 0, st, n, (const double *)p, (const double *)q, d_x, r, z,
-                       d_dinv, (const double *)s, part);
-    hipLaunchKernelGGL(k_pcg_fin_rz, dim3(1), dim3(KS_BLOCK), 0, st, (const double *)part, nb, s);
-    KS_HIP(hipMemcpyAsync(h, s, sizeof(h), hipMemcpyDeviceToHost, st));
-    KS_HIP(hipStreamSynchronize(st));
-    *its = i + 1;
-    *rnorm = sqrt(h[S_ZZ]);
-    if (!(h[S_PQ] > 0.0)) { *reason = -8; return KS_OK; }    // indefinite or breakdown (KSP_DIVERGED_INDEFINITE_MAT)
-    if (*rnorm <= ttol) { *reason = 1; return KS_OK; }
+  hipLaunchKernelGGL(k_pcg_fin_start, dim3(1), dim3(KS_BLOCK), 0, st, (const double *)part, nb, S, rtol, abstol,
+                     maxits);
+  for (int done = 0;; done += KS_KSP_CHUNK) {
+    KS_TRY(read_state(ctx, S, its, reason, rnorm));
+    if (*reason || done >= maxits) break;
+    for (int c = 0; c < KS_KSP_CHUNK; c++) {
+      hipLaunchKernelGGL(k_pcg_p, dim3(nb), dim3(KS_BLOCK), 0, st, n, (const double *)z, p, (const ksp_state *)S);
+      KS_TRY(ks_spmv_gated(ctx, A, p, q, KS_F64, &S->gate));
+      hipLaunchKernelGGL(k_pcg_q, dim3(nb), dim3(KS_BLOCK), 0, st, n, (const double *)p, q, sigma,
+                         (const ksp_state *)S, part);
+      hipLaunchKernelGGL(k_pcg_fin_pq, dim3(1), dim3(KS_BLOCK), 0, st, (const double *)part, nb, S);
+      hipLaunchKernelGGL(k_pcg_xr, dim3(nb), dim3(KS_BLOCK), 0, st, n, (const double *)p, (const double *)q, d_x, r,
+                         z, d_dinv, (const ksp_state *)S, part);
+      hipLaunchKernelGGL(k_pcg_fin_iter, dim3(1), dim3(KS_BLOCK), 0, st, (const double *)part, nb, S);
+    }
+    KS_HIP(hipGetLastError());
   }
-  *reason = -3;                                                // KSP_DIVERGED_ITS
+  if (!*reason) *reason = -3;   // KSP_DIVERGED_ITS
+  return KS_OK;
+}
+
+int ks_bcgs_solve(ks_ctx ctx, ks_csr A, double sigma, const double *d_dinv, const double *d_b, double *d_x,
+                  double rtol, double abstol, int maxits, double *d_work, int *its, double *rnorm, int *reason) {
+  KS_CHECK(ctx && A && d_dinv && d_b && d_x && d_work && its && rnorm && reason, KS_ERR_ARG,
+           "null BiCGStab argument");
+  KS_CHECK(A->vprec == KS_F64 && A->nrows == A->ncols, KS_ERR_ARG, "BiCGStab needs a square FP64 matrix");
+  KS_CHECK(maxits > 0, KS_ERR_ARG, "BiCGStab needs maxits > 0");
+  const int n = A->nrows, nb = ksp_blocks(n);
+  double *r = d_work, *rh = r + n, *p = rh + n, *v = p + n, *sv = v + n, *t = sv + n;
+  ksp_state *S = (ksp_state *)(t + n);
+  double *part = ctx->d_partial;
+  hipStream_t st = ctx->stream;
+  const dim3 g(nb), b1(KS_BLOCK), one(1);
+  hipLaunchKernelGGL(k_bcgs_start, g, b1, 0, st, n, d_b, d_x, r, rh, p, v, d_dinv, part);
+  hipLaunchKernelGGL(k_bcgs_fin_start, one, b1, 0, st, (const double *)part, nb, S, rtol, abstol, maxits);
+  for (int done = 0;; done += KS_KSP_CHUNK) {
+    KS_TRY(read_state(ctx, S, its, reason, rnorm));
+    if (*reason || done >= maxits) break;
+    for (int c = 0; c < KS_KSP_CHUNK; c++) {
+      hipLaunchKernelGGL(k_bcgs_p, g, b1, 0, st, n, (const double *)r, p, (const double *)v, (const ksp_state *)S);
+      KS_TRY(ks_spmv_gated(ctx, A, p, v, KS_F64, &S->gate));                         // v = A p
+      hipLaunchKernelGGL(k_bcgs_apply, g, b1, 0, st, n, (const double *)p, v, (const double *)rh, d_dinv, sigma,
+                         (const ksp_state *)S, 0, part);                               // v = B (A - sigma I) p
+      hipLaunchKernelGGL(k_bcgs_fin_alpha, one, b1, 0, st, (const double *)part, nb, S);
+      hipLaunchKernelGGL(k_bcgs_s, g, b1, 0, st, n, (const double *)r, (const double *)v, sv, (const ksp_state *)S,
+                         part);
+      hipLaunchKernelGGL(k_bcgs_fin_s, one, b1, 0, st, (const double *)part, nb, S);
+      KS_TRY(ks_spmv_gated(ctx, A, sv, t, KS_F64, &S->gate));                        // t = A s
+      hipLaunchKernelGGL(k_bcgs_apply, g, b1, 0, st, n, (const double *)sv, t, (const double *)rh, d_dinv, sigma,
+                         (const ksp_state *)S, 1, part);                               // t = B (A - sigma I) s
+      hipLaunchKernelGGL(k_bcgs_fin_omega, one, b1, 0, st, (const double *)part, nb, S);
+      hipLaunchKernelGGL(k_bcgs_x, g, b1, 0, st, n, d_x, (const double *)p, (const double *)sv, (const double *)t, r,
+                         (const double *)rh, (const ksp_state *)S, part);
+      hipLaunchKernelGGL(k_bcgs_fin_iter, one, b1, 0, st, (const double *)part, nb, S);
+    }
+    KS_HIP(hipGetLastError());
+  }
+  if (!*reason) *reason = -3;
   return KS_OK;
 }

@@ -6,8 +6,9 @@
  * STSHIFT with sigma = 0 on a standard problem hands out A itself
  * (STMatMAXPY_Private leaves T[0] = A, stsolve.c:598-610); eigenvalues need no
  * back-transform.  STSINVERT's operator is (A - sigma I)^-1, applied by an inner
- * solve (KSP) on the device: the preconditioned conjugate gradient method of
- * ks_pcg_solve with a Jacobi or no preconditioner.  Its eigenvalues theta map
+ * solve (KSP) on the device: preconditioned conjugate gradients (ks_pcg_solve,
+ * KSPCG) or BiCGStab (ks_bcgs_solve, KSPBCGS, for a shift inside the
+ * spectrum), with a Jacobi or no preconditioner.  Its eigenvalues theta map
  * back to lambda = sigma + 1/theta (STBackTransform_Sinvert, sinvert.c:32-50).
  * The default shift is the EPS target (STSetDefaultShift, from EPSSetTarget).
  */
@@ -85,8 +86,8 @@ PetscErrorCode STGetKSP(ST st, KSP *ksp)
 PetscErrorCode KSPSetType(KSP ksp, KSPType type)
 {
   PetscCheck(ksp && type, PETSC_ERR_ARG_NULL, "null argument");
-  PetscCheck(!strcmp(type, KSPCG), PETSC_ERR_SUP, "KSP type %s is not available (the device inner solver is %s)", type,
-             KSPCG);
+  PetscCheck(!strcmp(type, KSPCG) || !strcmp(type, KSPBCGS), PETSC_ERR_SUP,
+             "KSP type %s is not available (the device inner solvers are %s and %s)", type, KSPCG, KSPBCGS);
   strcpy(ksp->type, type);
   return PETSC_SUCCESS;
 }
@@ -128,6 +129,7 @@ static void KSPFreeWork(KSP ksp)
   if (ksp->d_work) (void)ks_free(ksp->d_work);
   ksp->d_dinv = ksp->d_work = NULL;
   ksp->n = 0;
+  ksp->nwork = 0;
 }
 
 /* ------------------------------------------------------------ STSINVERT operator */
@@ -141,13 +143,19 @@ static PetscErrorCode MatMult_STSinvert(Mat P, Vec x, Vec y)
   double rnorm = 0.0;
   PetscCheck(x->prec == SLEPC_AMD_F64 && y->prec == SLEPC_AMD_F64, PETSC_ERR_SUP,
              "shift-and-invert runs on FP64 vectors");
-  KSCall(ks_pcg_solve(SlepcAMDContext(), h->csr, st->sigma, ksp->d_dinv, (const double *)x->d, (double *)y->d,
-                      ksp->rtol, ksp->abstol, (int)ksp->maxits, ksp->d_work, &its, &rnorm, &reason));
+  const PetscBool cg = !strcmp(ksp->type, KSPCG) ? PETSC_TRUE : PETSC_FALSE;
+  if (cg)
+    KSCall(ks_pcg_solve(SlepcAMDContext(), h->csr, st->sigma, ksp->d_dinv, (const double *)x->d, (double *)y->d,
+                        ksp->rtol, ksp->abstol, (int)ksp->maxits, ksp->d_work, &its, &rnorm, &reason));
+  else
+    KSCall(ks_bcgs_solve(SlepcAMDContext(), h->csr, st->sigma, ksp->d_dinv, (const double *)x->d, (double *)y->d,
+                         ksp->rtol, ksp->abstol, (int)ksp->maxits, ksp->d_work, &its, &rnorm, &reason));
   ksp->its_total += its;
   ksp->nsolves++;
-  PetscCheck(reason > 0, PETSC_ERR_NOT_CONVERGED,
-             "KSP did not converge (reason %d, %d iterations, residual norm %g); the shifted matrix must be "
-             "positive definite on the Krylov space for CG", reason, its, rnorm);
+  PetscCheck(reason > 0, PETSC_ERR_NOT_CONVERGED, "KSP %s did not converge (reason %d, %d iterations, residual norm %g)%s",
+             ksp->type, reason, its, rnorm,
+             cg && reason == -8 ? "; A - sigma I is not positive definite: use KSPBCGS for a shift inside the spectrum"
+                                : "");
   return PETSC_SUCCESS;
 }
 
@@ -160,23 +168,36 @@ PetscErrorCode STSetUp(ST st)
   PetscCheck(st->A, PETSC_ERR_ORDER, "STSetMatrices must be called first");
   if (!st->sigma_set) st->sigma = strcmp(st->type, STSINVERT) ? 0.0 : st->defsigma;
   if (!strcmp(st->type, STSINVERT)) {
-    PetscCheck(!strcmp(st->A->type, MATHIPCSR), PETSC_ERR_SUP, "shift-and-invert needs a %s matrix", MATHIPCSR);
+    PetscCheck(!strcmp(st->A->hdr.type_name, MATHIPCSR), PETSC_ERR_SUP, "shift-and-invert needs a %s matrix", MATHIPCSR);
     h = (Mat_HIPCSR *)st->A->data;
     PetscCheck(h->csr && h->vprec == SLEPC_AMD_F64, PETSC_ERR_SUP, "shift-and-invert needs FP64 matrix values");
     PetscCall(STGetKSP(st, &ksp));
-    if (ksp->n != st->A->m) {
+    {
+      /* KSPSetFromOptions of the ST's solver (prefix st_) */
+      char v[SLEPC_NAME_LEN];
+      PetscBool set;
+      PetscCall(PetscOptionsGetString("-st_ksp_type", v, sizeof(v), &set));
+      if (set) PetscCall(KSPSetType(ksp, v));
+      PetscCall(PetscOptionsGetString("-st_pc_type", v, sizeof(v), &set));
+      if (set) PetscCall(PCSetType(&ksp->pc, v));
+    }
+    /* BiCGStab's r rhat p v s t (CG uses the first four), then the solver
+       state (64 doubles); sized for either, so KSPSetType may follow STSetUp */
+    const size_t nwork = 6 * (size_t)st->A->m + 64;
+    if (ksp->n != st->A->m || ksp->nwork < nwork) {
       KSPFreeWork(ksp);
       KSCall(ks_malloc((void **)&ksp->d_dinv, sizeof(double) * (size_t)st->A->m));
-      KSCall(ks_malloc((void **)&ksp->d_work, sizeof(double) * (4 * (size_t)st->A->m + 8)));
+      KSCall(ks_malloc((void **)&ksp->d_work, sizeof(double) * nwork));
       ksp->n = st->A->m;
+      ksp->nwork = nwork;
     }
     KSCall(ks_pcg_jacobi(SlepcAMDContext(), h->csr, st->sigma, !strcmp(ksp->pc.type, PCJACOBI), ksp->d_dinv));
     ksp->its_total = 0;
     ksp->nsolves = 0;
     if (!st->P) {
       PetscCall(PetscMallocZero(sizeof(struct _p_Mat), &st->P));
-      strcpy(st->P->type, "stsinvert");
-      st->P->ops.mult = MatMult_STSinvert;
+      snprintf(st->P->hdr.type_name, sizeof(st->P->hdr.type_name), "%s", "stsinvert");
+      st->P->ops->mult = MatMult_STSinvert;
     }
     st->P->m = st->P->n = st->A->m;
     st->P->data = st;
