@@ -151,6 +151,16 @@ int ks_pcg_solve(ks_ctx ctx, ks_csr A, double sigma, const double *d_dinv, const
    ((rhat, v) = 0, (t, t) = 0, rho = 0 or omega = 0). */
 int ks_bcgs_solve(ks_ctx ctx, ks_csr A, double sigma, const double *d_dinv, const double *d_b, double *d_x,
                   double rtol, double abstol, int maxits, double *d_work, int *its, double *rnorm, int *reason);
+/* QR of the n x k FP64 panel V (leading dimension ld) by TSQR: Householder QR
+   of row blocks of 2k rows (at least 64) in LDS, their triangular factors
+   reduced by the same factorization up a tree.  R (k x k, host memory, leading
+   dimension ldr) gets the factor; form_q != 0 overwrites V with Q, 0 leaves V
+   as it is.  Replaces BVOrthogonalize_LAPACK_TSQR and _TSQR_OnlyR
+   (src/sys/classes/bv/interface/bvlapack.c:347-560), which run LAPACK per
+   process and reduce across processes.  Q and R are LAPACK's Householder QR up
+   to the signs of R's rows.  Allocates its workspace (about 1.5 n k doubles)
+   and waits for the stream. */
+int ks_bv_tsqr(ks_ctx ctx, int n, double *d_V, int ld, int k, int form_q, double *R, int ldr);
 /* nrows, nnz, max row length and the kernel variant chosen (0 stream, 1 vector) */
 int ks_csr_info(ks_csr A, int *nrows, int64_t *nnz, int *max_row, int *variant, int *nblocks);
 /* y = A x.  x, y device vectors in xprec.  When gate != 0 the product is

@@ -153,7 +153,7 @@ PetscErrorCode MatLoadPETScBinary(const char *path, Mat *A);
 /* ------------------------------------------------------------ BV */
 typedef enum { BV_ORTHOG_CGS = 0, BV_ORTHOG_MGS = 1 } BVOrthogType;
 typedef enum { BV_ORTHOG_REFINE_IFNEEDED = 0, BV_ORTHOG_REFINE_NEVER = 1, BV_ORTHOG_REFINE_ALWAYS = 2 } BVOrthogRefineType;
-/* BVOrthogBlockType (include/slepcbv.h:84-88); TSQR and TSQRCHOL are not implemented on the device */
+/* BVOrthogBlockType (include/slepcbv.h:84-88) */
 typedef enum { BV_ORTHOG_BLOCK_GS = 0, BV_ORTHOG_BLOCK_CHOL = 1, BV_ORTHOG_BLOCK_TSQR = 2, BV_ORTHOG_BLOCK_TSQRCHOL = 3,
                BV_ORTHOG_BLOCK_SVQB = 4 } BVOrthogBlockType;
 PetscErrorCode BVRegister(const char name[], PetscErrorCode (*create)(BV));

@@ -47,6 +47,8 @@ extern void   dgemm_(const char*, const char*, const int*, const int*, const int
 extern double dnrm2_(const int*, const double*, const int*);
 extern void   dpotrf_(const char*, const int*, double*, const int*, int*);
 extern void   dtrtri_(const char*, const char*, const int*, double*, const int*, int*);
+extern void   dgeqrf_(const int*, const int*, double*, const int*, double*, double*, const int*, int*);
+extern void   dorgqr_(const int*, const int*, const int*, double*, const int*, const double*, double*, const int*, int*);
 extern void   dsyev_(const char*, const char*, const int*, double*, const int*, double*, double*, const int*,
                      int*);
 extern void   dgetrf_(const int*, const int*, double*, const int*, int*, int*);
@@ -405,7 +407,8 @@ int or_bv_dot(int n, const double *X, int ldx, int lx, int kx, const double *Y, 
    constraints): columns 0..l-1 kept, l..k-1 orthonormalized; R (ldr x k) gets
    columns l..k-1 of the factor.  block: 0 GS (bvorthog.c:507-547), 1 CHOL
    (bvorthog.c:581-596 with BVMatCholInv_LAPACK_Private bvlapack.c:136-202),
-   4 SVQB (bvorthog.c:646-663 with BVMatSVQB_LAPACK_Private bvlapack.c:261-345);
+   2 TSQR and 3 TSQRCHOL (bvorthog.c:615-663; on one process the TSQR of
+   bvlapack.c:347-406 is dgeqrf + dorgqr of the panel), 4 SVQB (bvorthog.c:646-663 with BVMatSVQB_LAPACK_Private bvlapack.c:261-345);
    the leading columns by block Gram-Schmidt first (bvorthog.c:489-502). */
 int or_bv_orthogonalize(int n, int ld, double *V, int l, int k, int block, double *R, int ldr)
 {
@@ -430,7 +433,7 @@ int or_bv_orthogonalize(int n, int ld, double *V, int l, int k, int block, doubl
     free(bv.buffer);
     return 0;
   }
-  if (block != 1 && block != 4) return -3;
+  if (block < 1 || block > 4) return -3;
   const int nn = k - l;
   double *W = (double*)calloc((size_t)k * k, sizeof(double));     /* the buffer as a k x k matrix */
   double *S = (double*)calloc((size_t)k * k, sizeof(double));
@@ -447,8 +450,47 @@ int or_bv_orthogonalize(int n, int ld, double *V, int l, int k, int block, doubl
     bv_multinplace(&bv, S, k, l, k);
     memset(S, 0, sizeof(double) * (size_t)k * k);
   }
-  bv_dot(V, ld, l, k, V, ld, l, k, n, W, k);   /* W(l:k, l:k) = V2^T V2 */
   int nn_ = nn, ldk = k, info = 0;
+  if (block == 2 || block == 3) {
+    /* TSQR on one process (bvorthog.c:615-663, bvlapack.c:347-406 and 510-560 with
+       size 1): a Householder QR of the panel, dgeqrf; R is its upper triangle.
+       TSQR forms Q with dorgqr, TSQRCHOL keeps V and applies R^-1 (BVMatTriInv). */
+    int n_ = n, ld_ = ld, lwork = -1;
+    double *A = block == 2 ? V + (size_t)l * ld : (double*)malloc(sizeof(double) * (size_t)ld * nn);
+    double *tau = (double*)malloc(sizeof(double) * (nn > 0 ? nn : 1)), q = 0.0;
+    if (!A || !tau) { free(tau); if (block == 3) free(A); free(W); free(S); return -1; }
+    if (block == 3) memcpy(A, V + (size_t)l * ld, sizeof(double) * (size_t)ld * nn);
+    dgeqrf_(&n_, &nn_, A, &ld_, tau, &q, &lwork, &info);
+    lwork = (int)q;
+    if (block == 2) {
+      int lw2 = -1;
+      dorgqr_(&n_, &nn_, &nn_, A, &ld_, tau, &q, &lw2, &info);
+      if ((int)q > lwork) lwork = (int)q;
+    }
+    double *work = (double*)malloc(sizeof(double) * (size_t)(lwork > 1 ? lwork : 1));
+    dgeqrf_(&n_, &nn_, A, &ld_, tau, work, &lwork, &info);
+    for (int j = l; j < k; j++)
+      for (int i = l; i < k; i++) W[i + (size_t)j * k] = i <= j ? A[(i - l) + (size_t)(j - l) * ld] : 0.0;
+    if (block == 2) dorgqr_(&n_, &nn_, &nn_, A, &ld_, tau, work, &lwork, &info);
+    free(work);
+    free(tau);
+    if (block == 3) free(A);
+    if (info) { free(W); free(S); return -7; }
+    if (block == 3) {   /* S(l:k, l:k) = R^-1, V2 = V2 S */
+      for (int j = l; j < k; j++) memcpy(S + (size_t)j * k + l, W + (size_t)j * k + l, sizeof(double) * nn);
+      dtrtri_("U", "N", &nn_, S + (size_t)l * k + l, &ldk, &info);
+      if (info) { free(W); free(S); return -5; }
+      bv.l = l; bv.k = k;
+      bv_multinplace(&bv, S, k, l, k);
+    }
+    if (R)
+      for (int j = l; j < k; j++)
+        for (int i = 0; i <= j; i++) R[i + (size_t)j * ldr] = W[i + (size_t)j * k];
+    free(W);
+    free(S);
+    return 0;
+  }
+  bv_dot(V, ld, l, k, V, ld, l, k, n, W, k);   /* W(l:k, l:k) = V2^T V2 */
   double *g = W + (size_t)l * k + l;
   if (block == 1) {
     for (int j = l; j < k; j++) memcpy(S + (size_t)j * k + l, W + (size_t)j * k + l, sizeof(double) * nn);
@@ -497,6 +539,58 @@ int or_bv_orthogonalize(int n, int ld, double *V, int l, int k, int block, doubl
   free(W);
   free(S);
   return 0;
+}
+
+/* The reference's TSQR over P processes (bvlapack.c:347-466): each process
+   factors its rows (PetscSplitOwnership: n/P rows, one more for the first n%P),
+   the P triangular factors are reduced to one, and each process's Q block is
+   its local Q times its slice of the reduction's Q.  Restated with a flat
+   reduction (one QR of the P stacked factors) in place of the MPI butterfly:
+   the same factorization up to the signs of R's rows, other rounding.  V is
+   n x k (leading dimension ld), l = 0; R (ldr x k) gets the factor. */
+int or_bv_tsqr_split(int n, int ld, double *V, int k, int P, double *R, int ldr)
+{
+  or_mkl_layer();
+  if (P < 1 || k < 1 || n < P * k) return -3;
+  const int m2 = P * k;
+  double *S = (double*)calloc((size_t)m2 * k, sizeof(double));      /* stacked factors */
+  double *tau = (double*)malloc(sizeof(double) * k);
+  int *r0 = (int*)malloc(sizeof(int) * (P + 1));
+  if (!S || !tau || !r0) { free(S); free(tau); free(r0); return -1; }
+  r0[0] = 0;
+  for (int p = 0; p < P; p++) r0[p + 1] = r0[p] + n / P + (p < n % P ? 1 : 0);
+  int info = 0, lwork = 64 * k + 64;
+  double *work = (double*)malloc(sizeof(double) * lwork);
+  for (int p = 0; p < P && !info; p++) {
+    int mp = r0[p + 1] - r0[p], kk = k;
+    double *A = V + r0[p];
+    dgeqrf_(&mp, &kk, A, &ld, tau, work, &lwork, &info);
+    for (int j = 0; j < k; j++)
+      for (int i = 0; i <= j; i++) S[(size_t)p * k + i + (size_t)j * m2] = A[i + (size_t)j * ld];
+    if (!info) dorgqr_(&mp, &kk, &kk, A, &ld, tau, work, &lwork, &info);
+  }
+  int m2_ = m2, kk = k;
+  if (!info) dgeqrf_(&m2_, &kk, S, &m2_, tau, work, &lwork, &info);
+  if (!info && R)
+    for (int j = 0; j < k; j++)
+      for (int i = 0; i < k; i++) R[i + (size_t)j * ldr] = i <= j ? S[i + (size_t)j * m2] : 0.0;
+  if (!info) dorgqr_(&m2_, &kk, &kk, S, &m2_, tau, work, &lwork, &info);
+  /* V block p = Q_p S(p k .. p k + k - 1, :) */
+  double *B = (double*)malloc(sizeof(double) * (size_t)(n / P + 1) * k);
+  for (int p = 0; p < P && !info && B; p++) {
+    int mp = r0[p + 1] - r0[p];
+    for (int j = 0; j < k; j++)
+      for (int i = 0; i < mp; i++) {
+        double a = 0.0;
+        for (int c = 0; c < k; c++) a += V[r0[p] + i + (size_t)c * ld] * S[(size_t)p * k + c + (size_t)j * m2];
+        B[i + (size_t)j * mp] = a;
+      }
+    for (int j = 0; j < k; j++)
+      for (int i = 0; i < mp; i++) V[r0[p] + i + (size_t)j * ld] = B[i + (size_t)j * mp];
+  }
+  int rc = info ? -7 : (B ? 0 : -1);
+  free(B); free(work); free(S); free(tau); free(r0);
+  return rc;
 }
 
 /* ------------------------------------------------------------------ DS */

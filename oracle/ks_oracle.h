@@ -81,6 +81,8 @@ int or_bv_dot(int n, const double *X, int ldx, int lx, int kx, const double *Y, 
               double *M, int ldm);
 /* BVOrthogonalize of V(:, l:k) with block type 0 (GS), 1 (CHOL) or 4 (SVQB); R: columns l..k-1 */
 int or_bv_orthogonalize(int n, int ld, double *V, int l, int k, int block, double *R, int ldr);
+/* TSQR over P row blocks, the reference's multi-process TSQR with a flat reduction */
+int or_bv_tsqr_split(int n, int ld, double *V, int k, int P, double *R, int ldr);
 int or_multinplace(int n, int ld, double *V, int nc, int l, int k,
                    const double *Q, int ldq, int s, int e);
 

@@ -117,7 +117,7 @@ def ds_translate_harmonic(H: np.ndarray, tau: float, beta: float):
     return A[:n, :n].copy(), g[:n].copy(), gamma.value
 
 
-BLOCK_GS, BLOCK_CHOL, BLOCK_SVQB = 0, 1, 4
+BLOCK_GS, BLOCK_CHOL, BLOCK_TSQR, BLOCK_TSQRCHOL, BLOCK_SVQB = 0, 1, 2, 3, 4
 
 
 def bv_orthogonalize(V: np.ndarray, l: int = 0, block: int = BLOCK_GS):
@@ -130,6 +130,18 @@ def bv_orthogonalize(V: np.ndarray, l: int = 0, block: int = BLOCK_GS):
                                    _p(R), C.c_int(k))
     if rc:
         raise RuntimeError(f"or_bv_orthogonalize failed ({rc})")
+    return R
+
+
+def bv_tsqr_split(V: np.ndarray, nblocks: int):
+    """TSQR of V (n, k) over ``nblocks`` row blocks split as PETSc splits rows over
+    processes, in place; returns R (k, k)."""
+    assert V.flags.f_contiguous and V.dtype == np.float64
+    n, k = V.shape
+    R = np.zeros((k, k), order="F")
+    rc = lib().or_bv_tsqr_split(C.c_int(n), C.c_int(n), _p(V), C.c_int(k), C.c_int(nblocks), _p(R), C.c_int(k))
+    if rc:
+        raise RuntimeError(f"or_bv_tsqr_split failed ({rc})")
     return R
 
 

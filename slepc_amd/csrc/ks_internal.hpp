@@ -16,7 +16,6 @@
 #ifndef KS_GS_SPLITMAX
 #define KS_GS_SPLITMAX 128           // Gram-Schmidt against <= this many columns: column-split kernel
 #endif
-#ifndef KS_GS_SPLIT_BLOCKS
 // Loads of data a kernel streams exactly once (basis columns in a sweep): the
 // nontemporal hint keeps them from displacing what is reused in L2 (the vector
 // being orthogonalized, partial sums).  Measured on the split sweeps at
@@ -31,6 +30,7 @@ __device__ __forceinline__ float2 ks_stream_load2(const float *p) {
   return make_float2(v.x, v.y);
 }
 
+#ifndef KS_GS_SPLIT_BLOCKS
 #define KS_GS_SPLIT_BLOCKS 6144      // workgroups of the column-split Gram-Schmidt
 #endif
 #ifndef KS_GS_RPL2_MAXCW

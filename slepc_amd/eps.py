@@ -148,6 +148,22 @@ class BV:
     def set_active(self, l: int, k: int):
         _chk(lib().BVSetActiveColumns(self.h, C.c_int(l), C.c_int(k)), "BVSetActiveColumns")
 
+    def insert_constraints(self, Cm) -> int:
+        """BVInsertConstraints with the columns of Cm (orthonormalized on insertion;
+        the regular columns' contents are lost).  Returns how many were kept."""
+        Cm = np.asarray(Cm, dtype=np.float64)
+        vs = [Vec.from_array(Cm[:, i]) for i in range(Cm.shape[1])]
+        nc = C.c_int(len(vs))
+        arr = (C.c_void_p * len(vs))(*[v.h for v in vs])
+        _chk(lib().BVInsertConstraints(self.h, C.byref(nc), arr), "BVInsertConstraints")
+        for v in vs:
+            v.destroy()
+        return nc.value
+
+    def get_constraints(self, nc: int) -> np.ndarray:
+        """The constraint columns (BVGetColumn with negative indices)."""
+        return self.get_columns(-nc, 0)
+
     def set_block(self, block: int):
         _chk(lib().BVSetOrthogonalization(self.h, C.c_int(0), C.c_int(0), C.c_double(0.7071), C.c_int(block)),
              "BVSetOrthogonalization")

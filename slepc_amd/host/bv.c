@@ -567,15 +567,38 @@ static PetscErrorCode OrthoBlockGS(BV V, PetscScalar *W, PetscInt ldw)
   return e;
 }
 
+/* V(:, l:k) = V(:, l:k) R^-1 for the upper triangular R in W(l:k, l:k)
+   (BVMatTriInv_LAPACK_Private, bvlapack.c:206-246, then BVMultInPlace) */
+static PetscErrorCode OrthoTriInv(BV V, PetscScalar *W, PetscInt ldw)
+{
+  const PetscInt l = V->l, k = V->k, n = k - l;
+  int n_ = (int)n, info = 0;
+  Mat S;
+  PetscScalar *sv;
+  PetscInt lds;
+  PetscCall(MatCreateSeqDense(k, k, NULL, &S));
+  PetscCall(DenseAt(S, &sv, &lds));
+  for (PetscInt j = 0; j < n; j++) memcpy(sv + (l + j) * lds + l, W + (l + j) * ldw + l, sizeof(PetscScalar) * (size_t)n);
+  {
+    int lds_ = (int)lds;
+    dtrtri_("U", "N", &n_, sv + l * lds + l, &lds_, &info);
+  }
+  if (info) { MatDestroy(&S); SETERRQ(PETSC_ERR_LIB, "Error in LAPACK subroutine trtri: info=%d", info); }
+  {
+    PetscErrorCode e = BVMultInPlace(V, S, l, k);
+    MatDestroy(&S);
+    PetscCall(e);
+  }
+  return PETSC_SUCCESS;
+}
+
 /* Cholesky QR (bvorthog.c:581-596, BVMatCholInv_LAPACK_Private bvlapack.c:136-202):
    G = V2^T V2, upper Cholesky G = R^T R (retried once on G + 50 eps I), V2 = V2 R^-1 */
 static PetscErrorCode OrthoChol(BV V, PetscScalar *W, PetscInt ldw)
 {
   const PetscInt l = V->l, k = V->k, n = k - l;
   int n_ = (int)n, ld_ = (int)ldw, info = 0;
-  Mat S;
-  PetscScalar *g, *sv, *save;
-  PetscInt lds;
+  PetscScalar *g, *save;
   PetscCall(BVDotRange(V, l, k, V, l, k, W + l * ldw + l, ldw));   /* W(l:k, l:k) = V2^T V2 */
   g = W + l * ldw + l;
   PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)n * (size_t)n, &save));
@@ -592,21 +615,22 @@ static PetscErrorCode OrthoChol(BV V, PetscScalar *W, PetscInt ldw)
   PetscCheck(info == 0, PETSC_ERR_LIB, "Error in LAPACK subroutine potrf: info=%d", info);
   for (PetscInt j = 0; j < n; j++)   /* zero below the diagonal */
     for (PetscInt i = j + 1; i < n; i++) g[i + j * ldw] = 0.0;
-  /* S(l:k, l:k) = R^-1 */
-  PetscCall(MatCreateSeqDense(k, k, NULL, &S));
-  PetscCall(DenseAt(S, &sv, &lds));
-  for (PetscInt j = 0; j < n; j++) memcpy(sv + (l + j) * lds + l, g + j * ldw, sizeof(PetscScalar) * (size_t)n);
-  {
-    int lds_ = (int)lds;
-    dtrtri_("U", "N", &n_, sv + l * lds + l, &lds_, &info);
-  }
-  if (info) { MatDestroy(&S); SETERRQ(PETSC_ERR_LIB, "Error in LAPACK subroutine trtri: info=%d", info); }
-  {
-    PetscErrorCode e = BVMultInPlace(V, S, l, k);
-    MatDestroy(&S);
-    PetscCall(e);
-  }
-  return PETSC_SUCCESS;
+  return OrthoTriInv(V, W, ldw);
+}
+
+/* TSQR and TSQRCHOL (bvorthog.c:615-663): the device TSQR of the panel
+   V(:, l:k) (ks_bv_tsqr); TSQR forms Q in place, TSQRCHOL keeps the factor
+   only and applies R^-1 (BVMatTriInv_LAPACK_Private, bvlapack.c:206-246) */
+static PetscErrorCode OrthoTSQR(BV V, PetscScalar *W, PetscInt ldw, PetscBool formq)
+{
+  const PetscInt l = V->l, k = V->k;
+  double *d;
+  PetscInt ld;
+  ks_ctx ctx;
+  PetscCall(BVHIPKSGetDeviceArray_Private(V, &d, &ld, &ctx));
+  KSCall(ks_bv_tsqr(ctx, (int)V->n, d + (size_t)(V->nc + l) * (size_t)ld, (int)ld, (int)(k - l), formq ? 1 : 0,
+                    W + l * ldw + l, (int)ldw));
+  return formq ? PETSC_SUCCESS : OrthoTriInv(V, W, ldw);
 }
 
 /* SVQB (bvorthog.c:646-663, BVMatSVQB_LAPACK_Private bvlapack.c:261-345):
@@ -663,7 +687,6 @@ PetscErrorCode BVOrthogonalize(BV V, Mat R)
     PetscCheck(m == n, PETSC_ERR_ARG_SIZ, "Mat argument is not square, it has %d rows and %d columns", m, n);
     PetscCheck(n >= k, PETSC_ERR_ARG_SIZ, "Mat size %d is smaller than the number of BV active columns %d", n, k);
   }
-  PetscCheck(!V->nc, PETSC_ERR_SUP, "Not implemented for BV with constraints, use BVOrthogonalizeColumn() instead");
   if (l >= k) return PETSC_SUCCESS;
   if (V->orthog_block == BV_ORTHOG_BLOCK_GS) {   /* column by column (bvorthog.c:507-547) */
     PetscScalar *r = NULL;
@@ -673,9 +696,10 @@ PetscErrorCode BVOrthogonalize(BV V, Mat R)
       PetscReal norm = 0.0;
       e = BVOrthogonalizeColumn(V, j, NULL, &norm, NULL);
       if (e) break;
-      if (r) {   /* every coefficient of the column, from the first one */
+      if (r) {   /* every coefficient of the column against the regular columns, from column 0
+                    (bvorthog.c:531-535 starts at -nc, the same column when there are no constraints) */
         PetscInt lsave = V->l;
-        V->l = -V->nc;
+        V->l = 0;
         e = BV_StoreCoefficients(V, j, NULL, r + j * ldr);
         V->l = lsave;
         if (e) break;
@@ -686,20 +710,42 @@ PetscErrorCode BVOrthogonalize(BV V, Mat R)
     }
     return e;
   }
-  PetscCheck(V->orthog_block == BV_ORTHOG_BLOCK_CHOL || V->orthog_block == BV_ORTHOG_BLOCK_SVQB, PETSC_ERR_SUP,
-             "block orthogonalization %d is not implemented on the device (GS, CHOL and SVQB are)", (int)V->orthog_block);
   PetscCheck(V->ops->dot, PETSC_ERR_SUP, "BV type %s has no block dot", BVTypeName(V));
-  ldw = k;
-  PetscCall(PetscMallocZero(sizeof(PetscScalar) * (size_t)ldw * (size_t)k, &W));
-  if (l > 0) e = OrthoBlockGS(V, W, ldw);
-  if (!e) e = V->orthog_block == BV_ORTHOG_BLOCK_CHOL ? OrthoChol(V, W, ldw) : OrthoSVQB(V, W, ldw);
-  if (!e && R) {   /* BV_StoreCoeffsBlock_Default: columns l..k-1, rows 0..j (CHOL) or 0..k-1 (SVQB) */
+  /* Constraint columns, which the reference refuses here (bvorthog.c:743), are
+     taken as further leading columns: the block Gram-Schmidt pass against the
+     leading columns then orthogonalizes against them too.  The view is shifted
+     so that storage column c is column c: nc = 0, l and k moved up by nc. */
+  const PetscInt nc = V->nc, msave = V->m;
+  if (nc) {
+    V->nc = 0;
+    V->m = msave + nc;
+    V->l = l + nc;
+    V->k = k + nc;
+  }
+  ldw = V->k;
+  W = (PetscScalar *)calloc((size_t)ldw * (size_t)ldw, sizeof(PetscScalar));
+  if (!W) e = PETSC_ERR_MEM;
+  if (!e && V->l > 0) e = OrthoBlockGS(V, W, ldw);
+  if (!e) switch (V->orthog_block) {
+      case BV_ORTHOG_BLOCK_CHOL: e = OrthoChol(V, W, ldw); break;
+      case BV_ORTHOG_BLOCK_TSQR: e = OrthoTSQR(V, W, ldw, PETSC_TRUE); break;
+      case BV_ORTHOG_BLOCK_TSQRCHOL: e = OrthoTSQR(V, W, ldw, PETSC_FALSE); break;
+      case BV_ORTHOG_BLOCK_SVQB: e = OrthoSVQB(V, W, ldw); break;
+      default: e = PETSC_ERR_ARG_OUTOFRANGE; break;
+    }
+  if (nc) {
+    V->nc = nc;
+    V->m = msave;
+    V->l = l;
+    V->k = k;
+  }
+  if (!e && R) {   /* BV_StoreCoeffsBlock_Default: columns l..k-1, rows 0..j (triangular) or 0..k-1 (SVQB) */
     PetscScalar *r;
     PetscInt ldr;
     PetscCall(DenseAt(R, &r, &ldr));
     for (PetscInt j = l; j < k; j++) {
-      const PetscInt rows = V->orthog_block == BV_ORTHOG_BLOCK_CHOL ? j + 1 : k;
-      for (PetscInt i = 0; i < rows; i++) r[i + j * ldr] = W[i + j * ldw];
+      const PetscInt rows = V->orthog_block == BV_ORTHOG_BLOCK_SVQB ? k : j + 1;
+      for (PetscInt i = 0; i < rows; i++) r[i + j * ldr] = W[(i + nc) + (j + nc) * ldw];
     }
   }
   free(W);

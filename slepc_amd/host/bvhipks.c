@@ -592,6 +592,19 @@ static PetscErrorCode CheckHIPKS(BV bv)
   return PETSC_SUCCESS;
 }
 
+/* the FP64 storage of a BVHIPKS (column c of the storage, constraints first,
+   at d + c ld) and its context, for the interface's device TSQR */
+PetscErrorCode BVHIPKSGetDeviceArray_Private(BV bv, double **d, PetscInt *ld, ks_ctx *ctx)
+{
+  PetscCall(CheckHIPKS(bv));
+  BV_HIPKS *h = (BV_HIPKS *)bv->data;
+  PetscCheck(h->prec == SLEPC_AMD_F64, PETSC_ERR_SUP, "TSQR runs on FP64 bases");
+  *d = (double *)h->dV;
+  *ld = bv->ld;
+  *ctx = h->ctx;
+  return PETSC_SUCCESS;
+}
+
 PetscErrorCode BVHIPKSSetPrecision(BV bv, SlepcAMDPrecision prec)
 {
   PetscCheck(bv, PETSC_ERR_ARG_NULL, "null BV");

@@ -309,6 +309,7 @@ typedef struct {
   PetscErrorCode (*matkrylovbegin)(BV, Mat, PetscInt k, PetscInt m);
   PetscErrorCode (*matkrylovend)(BV, PetscInt k, PetscInt *m, PetscReal *beta, PetscBool *breakdown);
 } BVKrylovOps;
+PetscErrorCode BVHIPKSGetDeviceArray_Private(BV bv, double **d, PetscInt *ld, ks_ctx *ctx);
 PetscErrorCode BVGetKrylovOps(BV bv, const BVKrylovOps **ops);
 
 /* BVHIPKS functions used by the solver and the serving loop */

@@ -1,8 +1,10 @@
 """Block orthogonalization on the MI355X (BVOrthogonalize with block types gs,
-chol and svqb), BVDot and BVMatProject on BVHIPKS bases, against the oracle
-and numpy.  The reference's test11 flow (leading columns first, then the
-active ones; src/sys/classes/bv/tests/test11.c) on its fill and at lengths the
-device handles with many workgroups, ragged row counts included."""
+chol, tsqr, tsqrchol and svqb), BVDot and BVMatProject on BVHIPKS bases,
+against the oracle and numpy.  The reference's test11 flow (leading columns
+first, then the active ones; src/sys/classes/bv/tests/test11.c) on its fill
+and at lengths the device handles with many workgroups, ragged row counts
+included; test11's TSQR case at n=180, k=7 (output/test11_12.out); a panel
+too ill-conditioned for Cholesky QR; constraint columns."""
 import numpy as np
 import pytest
 
@@ -11,7 +13,7 @@ from slepc_amd import matgen as M
 pytestmark = pytest.mark.gpu
 
 EPS = np.finfo(float).eps
-BLOCKS = {"gs": 0, "chol": 1, "svqb": 4}
+BLOCKS = {"gs": 0, "chol": 1, "tsqr": 2, "tsqrchol": 3, "svqb": 4}
 
 
 @pytest.fixture(scope="module")
@@ -58,6 +60,24 @@ def _match_signs(Q, R, Qo):
     return Q * s, (R.T * s).T
 
 
+def _positive_diagonal(Q, R):
+    """Householder QR (TSQR, TSQRCHOL) fixes the sign of each row of R by its own
+    rule, which a tree of factorizations need not share with one LAPACK call:
+    compare with R's diagonal made positive."""
+    s = np.sign(np.diag(R))
+    s[s == 0] = 1.0
+    return Q * s, (R.T * s).T
+
+
+def _normalize(name, Q, R, Qo, Ro):
+    if name == "svqb":
+        Q, R = _match_signs(Q, R, Qo)
+    elif name in ("tsqr", "tsqrchol"):
+        Q, R = _positive_diagonal(Q, R)
+        Qo, Ro = _positive_diagonal(Qo, Ro)
+    return Q, R, Qo, Ro
+
+
 def _oracle_flow(oracle, X, l, block):
     k = X.shape[1]
     Y = X.copy(order="F")
@@ -83,8 +103,7 @@ def test_test11_on_device(E, oracle, name):
     assert np.linalg.norm(G - I) < 100 * EPS
     assert np.linalg.norm(X - Q @ R) < 100 * EPS
     Qo, Ro = _oracle_flow(oracle, X, l, BLOCKS[name])
-    if name == "svqb":
-        Q, R = _match_signs(Q, R, Qo)
+    Q, R, Qo, Ro = _normalize(name, Q, R, Qo, Ro)
     assert np.abs(Q - Qo).max() < 1e-13
     assert np.abs(R - Ro).max() < 1e-13 * np.abs(Ro).max()
 
@@ -100,8 +119,7 @@ def test_large_vs_oracle(E, oracle, name, n, l, k):
     assert np.abs(G - np.eye(k)).max() < 1e-13
     assert np.abs(X - Q @ R).max() < 1e-12 * np.abs(X).max()
     Qo, Ro = _oracle_flow(oracle, X, l, BLOCKS[name])
-    if name == "svqb":
-        Q, R = _match_signs(Q, R, Qo)
+    Q, R, Qo, Ro = _normalize(name, Q, R, Qo, Ro)
     assert np.abs(Q - Qo).max() < 1e-11
     assert np.abs(R - Ro).max() < 1e-11 * np.abs(Ro).max()
 
@@ -155,8 +173,67 @@ def test_dot_fp32_basis(E):
     X.destroy()
 
 
-def test_tsqr_is_unsupported(E):
-    bv = E.BV(1000, 4)
+def test_test11_12_tsqr_on_device(E):
+    # test11 -resid -n 180 -l 0 -k 7 -bv_orthog_block tsqr (output/test11_12.out):
+    # "Level of orthogonality of Q < 100*eps", "Residual ||X-Q*R|| < 100*eps"
+    X = fill_test11(180, 7)
+    Q, R, G = _device_flow(E, X, 0, BLOCKS["tsqr"])
+    print(f"orth={np.linalg.norm(G - np.eye(7)):.2e} resid={np.linalg.norm(X - Q @ R):.2e}", flush=True)
+    assert np.linalg.norm(G - np.eye(7)) < 100 * EPS
+    assert np.linalg.norm(X - Q @ R) < 100 * EPS
+    assert np.allclose(np.tril(R, -1), 0.0, atol=0)
+
+
+@pytest.mark.parametrize("n", [4000, 300007])
+def test_tsqr_on_an_ill_conditioned_panel(E, oracle, n):
+    # condition number 1e12: TSQR keeps Q orthonormal to rounding and matches the
+    # oracle's Householder QR; the Cholesky QR of the same panel does not
+    rng = np.random.default_rng(n)
+    U, _ = np.linalg.qr(rng.standard_normal((n, 10)))
+    W, _ = np.linalg.qr(rng.standard_normal((10, 10)))
+    X = np.asfortranarray(U @ np.diag(np.logspace(0, -12, 10)) @ W)
+    Q, R, G = _device_flow(E, X, 0, BLOCKS["tsqr"])
+    assert np.abs(G - np.eye(10)).max() < 100 * EPS
+    assert np.linalg.norm(X - Q @ R) < 100 * EPS * np.linalg.norm(X)
+    Qo, Ro = _oracle_flow(oracle, X, 0, BLOCKS["tsqr"])
+    Q, R, Qo, Ro = _normalize("tsqr", Q, R, Qo, Ro)
+    assert np.abs(R - Ro).max() < 1e-13 * np.abs(Ro).max()
+    Qc, Rc, Gc = _device_flow(E, X, 0, BLOCKS["chol"])
+    assert np.abs(Gc - np.eye(10)).max() > 1e-6
+
+
+@pytest.mark.parametrize("name", list(BLOCKS))
+@pytest.mark.parametrize("n,l", [(5003, 0), (100003, 2)])
+def test_constraint_columns(E, name, n, l):
+    # three constraint columns: the active columns come out orthonormal, orthogonal
+    # to the constraints, and Q R reproduces their part outside the constraints' span
+    rng = np.random.default_rng(n + l)
+    k, nc = 9, 3
+    Cm = rng.standard_normal((n, nc))
+    X = np.asfortranarray(rng.standard_normal((n, k)))
+    bv = E.BV(n, k)
+    assert bv.insert_constraints(Cm) == nc
+    Cq = bv.get_constraints(nc)
+    assert np.abs(Cq.T @ Cq - np.eye(nc)).max() < 1e-13
+    bv.set_columns(X)
+    bv.set_block(BLOCKS[name])
+    R = np.zeros((k, k), order="F")
+    if l:
+        bv.set_active(0, l)
+        R[:, :l] = bv.orthogonalize()[:, :l]
+    bv.set_active(l, k)
+    R[:, l:] = bv.orthogonalize()[:, l:]
+    Q = bv.get_columns()
+    bv.destroy()
+    P = X - Cq @ (Cq.T @ X)
+    print(f"block={name} n={n} l={l} |C^T Q|={np.abs(Cq.T @ Q).max():.2e}", flush=True)
+    assert np.abs(Cq.T @ Q).max() < 1e-13
+    assert np.abs(Q.T @ Q - np.eye(k)).max() < 1e-13
+    assert np.abs(P - Q @ R).max() < 1e-12 * np.abs(X).max()
+
+
+def test_tsqr_needs_an_fp64_basis(E):
+    bv = E.BV(1000, 4, prec=E.F32)
     bv.set_columns(np.random.default_rng(0).standard_normal((1000, 4)))
     bv.set_block(E.BV.TSQR)
     with pytest.raises(E.SlepcError):
