@@ -103,7 +103,8 @@ def _measured_traffic(kernel, grid):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["fp64", "mixed", "serve", "scale", "copycal"], default="fp64")
+    ap.add_argument("--workload", choices=["fp64", "mixed", "serve", "scale", "copycal", "restartgemm"],
+                    default="fp64")
     ap.add_argument("--rmat-scale", type=int, default=24, help="scale: R-MAT graph scale")
     ap.add_argument("--cd-requests", type=int, default=8, help="scale: convection-diffusion requests served with it")
     ap.add_argument("--no-sequential", action="store_true", help="scale: skip the one-after-another FP64 baseline")
@@ -135,6 +136,8 @@ def main():
 
     if args.workload == "copycal":
         return run_copycal(args)
+    if args.workload == "restartgemm":
+        return run_restartgemm(args)
     if args.workload != "fp64":
         out = {"mixed": run_mixed, "serve": run_serve, "scale": run_scale}[args.workload](args)
         print(json.dumps(out), flush=True)
@@ -487,6 +490,43 @@ def run_copycal(args):
            "bytes_written_per_launch": 8 * n, "launches": e["launches"], "avg_us": round(us, 2),
            "GBs": round(16.0 * n / (us * 1e-6) / 1e9, 1)}
     print(json.dumps(out), flush=True)
+
+
+def run_restartgemm(args):
+    """The restart product V(:, 0:N) = V(:, 0:K) Q (BVMultInPlace, ks_bv_multinplace)
+    alone, at the Krylov-Schur restart shapes of the 128^3 solve (n = 2^21, K = 48,
+    N = 32) and of the R-MAT scale 24 solve (n = 2^24, K = 96, N = 64): bytes
+    (K + N) n 8 read and written, 2 n K N flops, FP64 MFMA."""
+    from slepc_amd import ks as K
+    ctx = K.Context(0)
+    rows = []
+    for n, k, nn in [(1 << 21, 48, 32), (1 << 24, 96, 64)]:
+        ld = ((n + 63) // 64) * 64
+        V = K.DeviceArray(ctx, (ld * k,))
+        K.random(ctx, K.F64, ld * k, V, 3)
+        rng = np.random.default_rng(k)
+        Q, _ = np.linalg.qr(rng.standard_normal((k, k)))
+        dQ = K.DeviceArray.from_numpy(ctx, np.asfortranarray(Q))
+        for _ in range(max(1, args.warmup)):
+            K.multinplace(ctx, K.F64, n, V, ld, k, dQ, k, 0, nn)
+        ctx.sync()
+        ctx.prof(True)
+        ctx.prof_reset()
+        for _ in range(max(1, args.steps)):
+            K.multinplace(ctx, K.F64, n, V, ld, k, dQ, k, 0, nn)
+        ctx.sync()
+        e = ctx.prof_get()["multinplace"]
+        ctx.prof(False)
+        us = 1e3 * e["ms"] / e["launches"]
+        gbs = 8.0 * n * (k + nn) / (us * 1e-6) / 1e9
+        tf = 2.0 * n * k * nn / (us * 1e-6) / 1e12
+        rows.append({"n": n, "K": k, "N": nn, "launches": e["launches"], "avg_us": round(us, 2),
+                     "GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "fp64_mfma_TFLOPs": round(tf, 2),
+                     "fp64_mfma_frac": round(tf / 78.6, 4)})
+        log(f"[restartgemm] n={n} K={k} N={nn}: {us:.1f} us, {gbs:.0f} GB/s, {tf:.2f} TFLOP/s")
+        V.free()
+        dQ.free()
+    print(json.dumps({"workload": "restartgemm", "kernel": "ks_bv_multinplace (FP64 MFMA)", "shapes": rows}), flush=True)
 
 
 def _er_job(r):

@@ -198,7 +198,7 @@ int ks_bv_orthonormalize_column(ks_ctx ctx, ks_prec prec, int n, void *V, int ld
    counterpart in the reference.  For callers that cannot allocate device
    memory (code under HIP graph capture, serving loops, harnesses that forbid
    hipMalloc).  The status, the finishing counter and the partial sums live in
-   one file-scope device array of the library (6.3 MB per GPU, held until the
+   one file-scope device array of the library (3.2 MB per GPU, held until the
    process ends), so k <= 128 (KS_ERR_RANGE beyond).  Work is enqueued on
    `stream` (a hipStream_t; NULL = the null stream); the status of the column
    (ks_status: norm, lindep, passes) is copied to d_status_out (device memory,

@@ -562,9 +562,8 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
   const unsigned r0 = (unsigned)b * (unsigned)rpb;
   const unsigned r1 = min((unsigned)n, r0 + (unsigned)rpb);
   int buf = 0;
-  for (unsigned i0 = r0; i0 < r1; i0 += STEP) {
-    T v[RPL][CW];
-    AT wi[RPL];
+  // the step's CW x RPL values of V and its RPL values of w
+  auto load_step = [&](unsigned i0, T (&v)[RPL][CW], AT (&wi)[RPL]) {
     if (i0 + STEP <= r1) {
       // every lane's rows exist (all steps but the last workgroup's last): no
       // per-lane guards, so the CW * RPL loads are issued back to back
@@ -587,7 +586,6 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
       }
 #pragma unroll
       for (int q = 0; q < RPL; q++) wi[q] = (AT)w[i0 + loff(q)];
-      __builtin_amdgcn_sched_barrier(0);   // all loads of the step before any use
     } else {
 #pragma unroll
       for (int u = 0; u < CW; u++) {
@@ -599,6 +597,9 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
 #pragma unroll
       for (int q = 0; q < RPL; q++) wi[q] = i0 + loff(q) < r1 ? (AT)w[i0 + loff(q)] : (AT)0;
     }
+  };
+  // the step's work on loaded values: the fused update through LDS, then the dots
+  auto work_step = [&](unsigned i0, T (&v)[RPL][CW], AT (&wi)[RPL]) {
     if (FUSED) {
 #pragma unroll
       for (int q = 0; q < RPL; q++) {
@@ -626,6 +627,13 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
       for (int u = 0; u < CW; u++) acc[u] += (AT)v[q][u] * wi[q];
       if (wave == 0) self += wi[q] * wi[q];
     }
+  };
+  for (unsigned i0 = r0; i0 < r1; i0 += STEP) {
+    T v[RPL][CW];
+    AT wi[RPL];
+    load_step(i0, v, wi);
+    __builtin_amdgcn_sched_barrier(0);   // all loads of the step before any use
+    work_step(i0, v, wi);
   }
   // lanes and workgroups combine in FP64; all CW sums of the wave by recursive halving
   double t[CW];
@@ -942,14 +950,128 @@ __global__ __launch_bounds__(KS_BLOCK) void k_multinplace_mfma(T *__restrict__ V
   }
 }
 
+// k-chunk of the persistent restart GEMM (a multiple of 4)
+#ifndef KS_MIP_LDS_KC
+#define KS_MIP_LDS_KC(R) 16
+#endif
+// The same product as a persistent kernel: each workgroup stages all of
+// Q(0:K, s:s+N) in LDS once and then walks row tiles (4 waves x 16R rows)
+// blockIdx.x, blockIdx.x + gridDim.x, ...; inside a tile the A fragments of
+// the next k-chunk are loaded while the MFMAs of the current one run, the
+// first chunk of the next tile is loaded before this tile's stores, and the
+// output goes through a per-wave LDS tile so that each store instruction
+// writes whole column runs (16R rows of 4/R columns: 512-byte runs for R = 4)
+// instead of 32-byte pieces.  The MFMA sequence of a tile, and so every
+// rounding, is the one of k_multinplace_mfma.
+template <typename T, int NT, int R>
+__global__ __launch_bounds__(KS_BLOCK) void k_multinplace_lds(T *__restrict__ V, long ld, int n, int K,
+                                                             const double *__restrict__ Q, int ldq, int s, int N,
+                                                             int Kp, int ntiles) {
+  extern __shared__ double smem[];
+  constexpr int WR = 16 * R;        // rows of a wave
+  constexpr int SS = WR + 1;        // padded column stride of the output tile
+  constexpr int JW = NT * 16;
+  constexpr int KC = KS_MIP_LDS_KC(R);   // k-chunk: R x KC/4 A fragments per lane in flight
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double *sQ = smem;                                                // Kp x JW, [k][j]
+  double *tile = smem + (size_t)Kp * JW + (size_t)wave * 16 * SS;   // 16 columns x SS
+  const int kq = lane >> 4;
+  const int nlast = n - 1;
+  for (int idx = tid; idx < Kp * JW; idx += KS_BLOCK) {
+    const int kk = idx / JW, j = idx - kk * JW;
+    sQ[idx] = (kk < K && j < N) ? Q[(size_t)kk + (size_t)(s + j) * ldq] : 0.0;
+  }
+  __syncthreads();
+  double a[R][KC / 4], an[R][KC / 4];
+  auto load = [&](double (&dst)[R][KC / 4], int rbase, int k0) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int row = min(rbase + r * 16 + (lane & 15), nlast);
+#pragma unroll
+      for (int kk = 0; kk < KC / 4; kk++) {
+        const int kc = min(k0 + 4 * kk + kq, K - 1);
+        dst[r][kk] = (double)ks_stream_load(V + (size_t)row + (size_t)kc * ld);
+      }
+    }
+  };
+  constexpr int CPI = 64 / WR;       // columns one store instruction covers
+  const int srow = lane % WR, scol = lane / WR;
+  int t0 = blockIdx.x;
+  if (t0 < ntiles && K > 0) load(a, (t0 * KS_WAVES + wave) * WR, 0);
+  for (; t0 < ntiles; t0 += gridDim.x) {
+    const int rbase = (t0 * KS_WAVES + wave) * WR;
+    v4d acc[R][NT];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int t = 0; t < NT; t++) acc[r][t] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += KC) {
+      const bool more = k0 + KC < K;
+      if (more) load(an, rbase, k0 + KC);
+#pragma unroll
+      for (int kk = 0; kk < KC / 4; kk++) {
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const double b = sQ[(size_t)(k0 + 4 * kk + kq) * JW + t * 16 + (lane & 15)];
+#pragma unroll
+          for (int r = 0; r < R; r++) acc[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r][kk], b, acc[r][t], 0, 0, 0);
+        }
+      }
+      if (more) {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+          for (int kk = 0; kk < KC / 4; kk++) a[r][kk] = an[r][kk];
+      }
+    }
+    // the next tile's first chunk is in flight while this tile is stored; this
+    // wave has read every input of its rows, and other waves own other rows
+    const int tn = t0 + gridDim.x;
+    if (tn < ntiles && K > 0) load(a, (tn * KS_WAVES + wave) * WR, 0);
+    // register q of lane l holds row l/16 + 4q, column l%16 of a 16 x 16 tile
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) tile[(lane & 15) * SS + r * 16 + (lane >> 4) + 4 * q] = acc[r][t][q];
+      __syncthreads();
+      const int row = rbase + srow;
+#pragma unroll
+      for (int c0 = 0; c0 < 16; c0 += CPI) {
+        const int c = t * 16 + c0 + scol;
+        if (row < n && c < N) V[(size_t)row + (size_t)(s + c) * ld] = (T)tile[(c0 + scol) * SS + srow];
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // row tiles per wavefront: more rows per staged Q chunk while the accumulators
 // (R * NT * 4 doubles per lane) stay within 64 registers' worth
+// workgroups of the persistent restart GEMM (two per CU resident at a time)
+#ifndef KS_MIP_GRID
+#define KS_MIP_GRID 1024
+#endif
+// Q fully in LDS while it takes at most this many bytes (with the output tiles)
+#ifndef KS_MIP_LDS_MAX
+#define KS_MIP_LDS_MAX (64 * 1024)
+#endif
+
 template <typename T, int NT>
 void launch_mip(ks_ctx ctx, T *V, long ld, int n, int K, const double *Q, int ldq, int s, int N) {
   constexpr int R = NT <= 2 ? 4 : (NT <= 4 ? 2 : 1);
   const int rows = KS_WAVES * 16 * R;
   dim3 grid((n + rows - 1) / rows), block(KS_BLOCK);
-  hipLaunchKernelGGL((k_multinplace_mfma<T, NT, R>), grid, block, 0, ctx->stream, V, ld, n, K, Q, ldq, s, N);
+  constexpr int KC = KS_MIP_LDS_KC(R);
+  const int Kp = (K + KC - 1) / KC * KC;
+  const size_t lds = sizeof(double) * ((size_t)Kp * NT * 16 + (size_t)KS_WAVES * 16 * (16 * R + 1));
+  const int ntiles = (int)grid.x;
+  if (lds <= KS_MIP_LDS_MAX)
+    hipLaunchKernelGGL((k_multinplace_lds<T, NT, R>), dim3(ntiles < KS_MIP_GRID ? ntiles : KS_MIP_GRID), block, lds,
+                       ctx->stream, V, ld, n, K, Q, ldq, s, N, Kp, ntiles);
+  else
+    hipLaunchKernelGGL((k_multinplace_mfma<T, NT, R>), grid, block, 0, ctx->stream, V, ld, n, K, Q, ldq, s, N);
 }
 
 template <typename T>
@@ -1329,7 +1451,7 @@ __global__ void k_noalloc_status_out(const ks_status *st, ks_status *out) {
 
 }  // namespace
 
-// 6.3 MB per device, held until the process ends.  Host code never passes the
+// 3.2 MB per device, held until the process ends.  Host code never passes the
 // array's name to a kernel: it uses the device address that hipGetSymbolAddress
 // returns for the current device.
 static __device__ __align__(256) double scratch[KS_NOALLOC_DOUBLES];

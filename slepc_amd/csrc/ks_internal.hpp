@@ -31,7 +31,7 @@ __device__ __forceinline__ float2 ks_stream_load2(const float *p) {
 }
 
 #ifndef KS_GS_SPLIT_BLOCKS
-#define KS_GS_SPLIT_BLOCKS 6144      // workgroups of the column-split Gram-Schmidt
+#define KS_GS_SPLIT_BLOCKS 3072      // workgroups of the column-split Gram-Schmidt
 #endif
 #ifndef KS_GS_RPL2_MAXCW
 #define KS_GS_RPL2_MAXCW 12          // FP64 column-split sweeps with <= this many columns per wave: 2 rows per lane

@@ -238,6 +238,9 @@ def test_dgks_second_pass_and_lindep(ks, ctx, oracle):
     assert r["passes"] == 2 and r["st"].total_passes == 2 and not r["st"].lindep
     assert np.allclose(r["h"][:4], r["H"], rtol=1e-8, atol=1e-15)
     assert np.abs(r["Vg"][:, :3].T @ r["Vg"][:, 3]).max() < 1e-14
+    # the column the second pass stores, normalized in that pass's own sweep
+    assert np.abs(r["Vg"][:, 3] - r["Vo"][:, 3]).max() < 1e-12
+    assert abs(np.linalg.norm(r["Vg"][:, 3]) - 1) < 1e-13
     V[:, 3] = 0.0
     r = _orth_case(ks, ctx, oracle, V, 3)
     assert r["lindep"] and r["st"].lindep == 1 and r["st"].breakdown_col == 3 and r["st"].active == 0
@@ -255,6 +258,62 @@ def test_orthonormalize_refine_never_always(ks, ctx, oracle, refine):
     assert r["st"].total_passes == r["passes"]
     assert abs(r["st"].nrm - r["nrm"]) <= 1e-13 * r["nrm"]
     assert np.allclose(r["h"][:j + 1], r["H"], rtol=1e-10, atol=1e-13)
+    assert np.abs(r["Vg"][:, j] - r["Vo"][:, j]).max() < 1e-12
+    assert abs(np.linalg.norm(r["Vg"][:, j]) - 1) < 1e-13
+
+
+def test_dgks_third_pass_vs_oracle(ks, ctx, oracle):
+    # eta = 1 and a column 1e-12 off the span: the second pass's coefficients are
+    # rounding-size but still shorten the vector, so DGKS takes its third pass;
+    # the stored column, h and the norm against the oracle's.  The column is a
+    # 1e-12 remainder of an O(1) vector, so two correct implementations that add
+    # in different orders agree to about u |v| / |w''| (1e-5 relative) only
+    rng = np.random.default_rng(9)
+    n, j = 30011, 20
+    V = np.zeros((n, j + 1), order="F")
+    V[:, :j] = np.linalg.qr(rng.standard_normal((n, j)))[0]
+    V[:, j] = V[:, :j] @ rng.standard_normal(j) + 1e-12 * rng.standard_normal(n)
+    Vo = np.asfortranarray(V.copy())
+    H, nrm, lindep, passes = oracle.orthonormalize_column(Vo, j, eta=1.0)
+    dV = ks.DeviceArray.from_numpy(ctx, V)
+    buf = ks.DeviceArray(ctx, (j + 2, j + 2))
+    ctx.status_reset()
+    ks.orthonormalize_column(ctx, ks.F64, n, dV, n, j, buf, j + 2, j, j, eta=1.0)
+    st = ctx.status()
+    Vg = dV.numpy()
+    print(f"passes device={st.total_passes} oracle={passes} nrm={st.nrm:.3e}/{nrm:.3e}", flush=True)
+    assert passes == 3 and st.total_passes == 3 and not st.lindep and not lindep
+    assert abs(st.nrm - nrm) <= 1e-4 * nrm
+    assert np.allclose(buf.numpy()[:j + 1, j], H, rtol=1e-8, atol=1e-15)
+    assert np.abs(Vg[:, j] - Vo[:, j]).max() < 1e-4 * np.abs(Vo[:, j]).max()
+    assert abs(np.linalg.norm(Vg[:, j]) - 1) < 1e-13
+    assert np.abs(Vg[:, :j].T @ Vg[:, j]).max() < 1e-14
+
+
+def test_dgks_second_pass_fp32(ks, ctx, oracle):
+    # FP32 basis, a column 1e-3 off the span: two passes, each storing FP32; the
+    # stored column against the FP64 oracle on the same FP32-representable data
+    rng = np.random.default_rng(10)
+    n, j = 20011, 12
+    V = np.zeros((n, j + 1), order="F")
+    V[:, :j] = np.linalg.qr(rng.standard_normal((n, j)))[0]
+    V[:, j] = V[:, :j] @ rng.standard_normal(j) + 1e-3 * rng.standard_normal(n) / np.sqrt(n)
+    V = np.asfortranarray(V.astype(np.float32).astype(np.float64))
+    Vo = np.asfortranarray(V.copy())
+    H, nrm, lindep, passes = oracle.orthonormalize_column(Vo, j)
+    dV = ks.DeviceArray.from_numpy(ctx, np.asfortranarray(V.astype(np.float32)))
+    buf = ks.DeviceArray(ctx, (j + 2, j + 2))
+    ctx.status_reset()
+    ks.orthonormalize_column(ctx, ks.F32, n, dV, n, j, buf, j + 2, j, j)
+    st = ctx.status()
+    Vg = dV.numpy().astype(np.float64)
+    assert passes == 2 and st.total_passes == 2 and st.done == 1 and not st.lindep
+    assert abs(st.nrm - nrm) <= 1e-3 * nrm
+    assert abs(np.linalg.norm(Vg[:, j]) - 1) < 1e-6
+    assert np.abs(Vg[:, :j].T @ Vg[:, j]).max() < 1e-6
+    # the first pass's FP32 update cancels |v| / |w'| = 3.5e3 of the vector, so the
+    # stored column carries u32 * 3.5e3 = 2e-4 relative error per operation
+    assert np.abs(Vg[:, j] - Vo[:, j]).max() < 1e-2 * np.abs(Vo[:, j]).max()
 
 
 @pytest.mark.parametrize("j,eta", [(12, 0.7071), (40, 0.7071), (12, 1.0), (40, 1.0), (60, 1.0)])
