@@ -26,7 +26,8 @@ enum {
   KS_ERR_NOMEM = -3,      /* device allocation failed */
   KS_ERR_NODEV = -4,      /* no gfx950 device visible */
   KS_ERR_INNER = -5,      /* invalid inner product (negative self dot beyond tolerance) */
-  KS_ERR_RANGE = -6       /* size beyond a kernel's supported maximum */
+  KS_ERR_RANGE = -6,      /* size beyond a kernel's supported maximum */
+  KS_ERR_PIVOT = -7       /* zero pivot in a factorization (ILU(0)) */
 };
 
 typedef enum { KS_F64 = 0, KS_F32 = 1 } ks_prec;
@@ -43,6 +44,7 @@ typedef struct ks_csr_s *ks_csr;
 typedef struct ks_pool_s *ks_pool;
 typedef struct ks_group_s *ks_group;
 typedef struct ks_graph_s *ks_graph;
+typedef struct ks_ilu_s *ks_ilu;
 
 /* Device-side state of the fused CGS+DGKS orthonormalization (the scalars of
    BVOrthogonalizeGS, src/sys/classes/bv/interface/bvorthog.c:145-217, kept on
@@ -161,6 +163,20 @@ int ks_bcgs_solve(ks_ctx ctx, ks_csr A, double sigma, const double *d_dinv, cons
    to the signs of R's rows.  Allocates its workspace (about 1.5 n k doubles)
    and waits for the stream. */
 int ks_bv_tsqr(ks_ctx ctx, int n, double *d_V, int ld, int k, int form_q, double *R, int ldr);
+/* ILU(0) of A - sigma I on A's pattern (Saad, Iterative Methods for Sparse Linear
+   Systems, Algorithm 10.4; PETSc's PCILU with levels 0 and the natural
+   ordering, its defaults), factored on the device one dependency level of rows
+   at a time; A's rows must list increasing columns and hold their diagonal.
+   Fails with KS_ERR_PIVOT on a zero pivot.  Applying it is one launch per level
+   of each factor (ks_ilu0_levels).  Replaces PCSetUp_ILU / PCApply_ILU as
+   KSPSolve reaches them from STApply_Sinvert
+   (src/sys/classes/st/impls/sinvert/sinvert.c:16-30). */
+int ks_ilu0_create(ks_ctx ctx, ks_csr A, double sigma, ks_ilu *out);
+int ks_ilu0_destroy(ks_ilu M);
+int ks_ilu0_levels(ks_ilu M, int *nlower, int *nupper);
+/* ks_bcgs_solve with the ILU(0) preconditioner B = (L U)^-1 */
+int ks_bcgs_solve_ilu(ks_ctx ctx, ks_csr A, double sigma, ks_ilu M, const double *d_b, double *d_x, double rtol,
+                      double abstol, int maxits, double *d_work, int *its, double *rnorm, int *reason);
 /* nrows, nnz, max row length and the kernel variant chosen (0 stream, 1 vector) */
 int ks_csr_info(ks_csr A, int *nrows, int64_t *nnz, int *max_row, int *variant, int *nblocks);
 /* y = A x.  x, y device vectors in xprec.  When gate != 0 the product is

@@ -277,6 +277,7 @@ typedef struct _p_PC *PC;
 #define KSPBCGS   "bcgs"
 #define PCJACOBI  "jacobi"
 #define PCNONE    "none"
+#define PCILU     "ilu"
 PetscErrorCode STCreate(ST *st);
 PetscErrorCode STSetType(ST st, STType type);
 PetscErrorCode STGetType(ST st, STType *type);

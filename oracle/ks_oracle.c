@@ -1149,8 +1149,10 @@ typedef struct {
   int harmonic, dsherm;           /* dsherm: Lanczos + DSHEP (Hermitian and Ritz extraction) */
   int twosided;
   int sinvert, pc_jacobi, ksp_maxits;   /* shift-and-invert: (A - sigma I)^-1 by PCG or BiCGStab */
-  int ksp_bcgs;
+  int ksp_bcgs, pc_ilu;
   double sigma, ksp_rtol, *dinv, *kwork;
+  int *lu_diag;                   /* PCILU: index of the diagonal entry of each row */
+  double *lu;                     /* PCILU: the ILU(0) factors of A - sigma I on A's pattern */
   long ksp_its;
   int ksp_failed;
   double tol, target, keep;
@@ -1195,11 +1197,86 @@ static int pcg_solve(oeps *eps, const double *b, double *x)
   OR_ERR(-91, "KSP did not converge in %d iterations", eps->ksp_maxits);
 }
 
+/* ILU(0) of A - sigma I (Y. Saad, Iterative Methods for Sparse Linear Systems,
+   2nd ed., Algorithm 10.4, the IKJ form): the factors L (unit lower) and U keep
+   the pattern of A, row by row in the natural order, as PETSc's PCILU with
+   levels 0 and the natural ordering (its defaults) on a sequential AIJ matrix.
+   Each row's columns are in increasing order and include the diagonal.  A zero
+   pivot fails, as PETSc's zero-pivot check does. */
+static int ilu0_factor(oeps *eps)
+{
+  const int n = eps->n, *rp = eps->rowptr, *ci = eps->colidx;
+  double *a = eps->lu;
+  for (int i = 0; i < n; i++) {
+    eps->lu_diag[i] = -1;
+    for (int q = rp[i]; q < rp[i + 1]; q++) {
+      a[q] = eps->val[q] - (ci[q] == i ? eps->sigma : 0.0);
+      if (ci[q] == i) eps->lu_diag[i] = q;
+      if (q > rp[i] && ci[q] <= ci[q - 1]) OR_ERR(-92, "ILU(0) needs increasing column indices in each row");
+    }
+    if (eps->lu_diag[i] < 0) OR_ERR(-92, "ILU(0) needs the diagonal entry of row %d", i);
+  }
+  for (int i = 1; i < n; i++)
+    for (int q = rp[i]; q < eps->lu_diag[i]; q++) {     /* k = ci[q] < i, in increasing order */
+      const int k = ci[q];
+      const double ukk = a[eps->lu_diag[k]];
+      if (ukk == 0.0) OR_ERR(-92, "ILU(0): zero pivot in row %d", k);
+      const double lik = a[q] / ukk;
+      a[q] = lik;
+      /* a_ij -= l_ik u_kj for the j > k of row i that row k also holds */
+      int t = eps->lu_diag[k] + 1;
+      for (int r = q + 1; r < rp[i + 1]; r++) {
+        const int j = ci[r];
+        while (t < rp[k + 1] && ci[t] < j) t++;
+        if (t < rp[k + 1] && ci[t] == j) a[r] -= lik * a[t];
+      }
+    }
+  if (a[eps->lu_diag[n - 1]] == 0.0) OR_ERR(-92, "ILU(0): zero pivot in row %d", n - 1);
+  return 0;
+}
+
+/* The factors alone, for the tests: lu (nnz, A's pattern) gets L below the
+   diagonal and U on and above it */
+int or_ilu0(int n, const int *rowptr, const int *colidx, const double *val, double sigma, double *lu)
+{
+  oeps eps;
+  memset(&eps, 0, sizeof(eps));
+  eps.n = n; eps.rowptr = rowptr; eps.colidx = colidx; eps.val = val; eps.sigma = sigma;
+  eps.lu = lu;
+  eps.lu_diag = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  if (!eps.lu_diag) return -55;
+  int e = ilu0_factor(&eps);
+  free(eps.lu_diag);
+  return e;
+}
+
+/* y = (L U)^-1 y in place: forward substitution with the unit lower factor, then
+   backward substitution with U */
+static void ilu0_apply(const oeps *eps, double *y)
+{
+  const int n = eps->n, *rp = eps->rowptr, *ci = eps->colidx;
+  const double *a = eps->lu;
+  for (int i = 0; i < n; i++) {
+    double s = y[i];
+    for (int q = rp[i]; q < eps->lu_diag[i]; q++) s -= a[q] * y[ci[q]];
+    y[i] = s;
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double s = y[i];
+    for (int q = eps->lu_diag[i] + 1; q < rp[i + 1]; q++) s -= a[q] * y[ci[q]];
+    y[i] = s / a[eps->lu_diag[i]];
+  }
+}
+
 /* y = B (A - sigma I) x, the left-preconditioned operator of the BiCGStab solve */
 static void bcgs_op(oeps *eps, const double *x, double *y)
 {
   or_spmv(eps->n, eps->rowptr, eps->colidx, eps->val, x, y);
-  for (int i = 0; i < eps->n; i++) y[i] = eps->dinv[i] * (y[i] - eps->sigma * x[i]);
+  if (eps->pc_ilu) {
+    for (int i = 0; i < eps->n; i++) y[i] -= eps->sigma * x[i];
+    ilu0_apply(eps, y);
+  } else
+    for (int i = 0; i < eps->n; i++) y[i] = eps->dinv[i] * (y[i] - eps->sigma * x[i]);
 }
 
 /* BiCGStab (H. A. van der Vorst, SIAM J. Sci. Stat. Comput. 13 (1992) 631-644) on
@@ -1219,8 +1296,10 @@ static int bcgs_solve(oeps *eps, const double *b, double *x)
   int n = eps->n;
   double *r = eps->kwork, *rh = r + n, *p = rh + n, *v = p + n, *sv = v + n, *t = sv + n;
   double rho, rho_old = 1.0, alpha = 1.0, omega = 1.0, rr = 0.0;
+  for (int i = 0; i < n; i++) r[i] = eps->pc_ilu ? b[i] : eps->dinv[i] * b[i];
+  if (eps->pc_ilu) ilu0_apply(eps, r);
   for (int i = 0; i < n; i++) {
-    x[i] = 0.0; r[i] = eps->dinv[i] * b[i]; rh[i] = r[i]; p[i] = 0.0; v[i] = 0.0; rr += r[i] * r[i];
+    x[i] = 0.0; rh[i] = r[i]; p[i] = 0.0; v[i] = 0.0; rr += r[i] * r[i];
   }
   double ttol = fmax(eps->ksp_rtol * sqrt(rr), 1e-50);
   if (sqrt(rr) <= ttol) return 0;
@@ -1693,6 +1772,8 @@ int or_eps_solve_twosided(int n, const int *rowptr, const int *colidx, const dou
     eps.ksp_maxits = 10000;
     eps.pc_jacobi = !opts->pc_none;
     eps.ksp_bcgs = opts->ksp_bcgs != 0;
+    eps.pc_ilu = opts->pc_ilu != 0;
+    if (eps.pc_ilu && !eps.ksp_bcgs) { e = -3; goto done; }   /* ILU(0) with BiCGStab only */
   }
 
   /* EPSSetUp (epssetup.c:344-345) and EPSSetDimensions_Default (epssetup.c:648-671) */
@@ -1737,6 +1818,12 @@ int or_eps_solve_twosided(int n, const int *rowptr, const int *colidx, const dou
     eps.dinv = (double*)malloc(sizeof(double) * (size_t)n);
     eps.kwork = (double*)malloc(sizeof(double) * 6 * (size_t)n);
     if (!eps.dinv || !eps.kwork) { e = -55; goto done; }
+    if (eps.pc_ilu) {
+      eps.lu = (double*)malloc(sizeof(double) * (size_t)rowptr[n]);
+      eps.lu_diag = (int*)malloc(sizeof(int) * (size_t)n);
+      if (!eps.lu || !eps.lu_diag) { e = -55; goto done; }
+      if ((e = ilu0_factor(&eps))) goto done;
+    }
     for (int i = 0; i < n; i++) {                                  /* PCJACOBI on A - sigma I */
       double d = 0.0;
       if (eps.pc_jacobi) { for (int q = rowptr[i]; q < rowptr[i + 1]; q++) if (colidx[q] == i) d += val[q]; d -= eps.sigma; }
@@ -1856,7 +1943,7 @@ int or_eps_solve_twosided(int n, const int *rowptr, const int *colidx, const dou
     info->nmatmult = eps.nmatmult; info->gs_passes = eps.V.passes; info->t_ds = eps.t_ds;
   }
 done:
-  free(eps.dinv); free(eps.kwork);
+  free(eps.dinv); free(eps.kwork); free(eps.lu); free(eps.lu_diag);
   if (eps.twosided) { free(eps.W.V); free(eps.W.buffer); free(eps.M); free(eps.trowptr); free(eps.tcolidx); free(eps.tval); }
   free(eps.V.V); free(eps.V.buffer); free(eps.eigr); free(eps.eigi); free(eps.errest); free(eps.perm);
   if (eps.ds.A) ds_free(&eps.ds);

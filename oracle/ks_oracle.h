@@ -46,6 +46,7 @@ typedef struct {
   int    pc_none;                /* 1: PCG without the Jacobi preconditioner */
   double ksp_rtol;               /* <= 0 means 1e-8 */
   int    ksp_bcgs;               /* 1: BiCGStab instead of CG for the inner solve */
+  int    pc_ilu;                 /* 1: ILU(0) preconditioner (BiCGStab only) */
 } or_opts;
 
 typedef struct {
@@ -83,6 +84,8 @@ int or_bv_dot(int n, const double *X, int ldx, int lx, int kx, const double *Y, 
 int or_bv_orthogonalize(int n, int ld, double *V, int l, int k, int block, double *R, int ldr);
 /* TSQR over P row blocks, the reference's multi-process TSQR with a flat reduction */
 int or_bv_tsqr_split(int n, int ld, double *V, int k, int P, double *R, int ldr);
+/* ILU(0) factors of A - sigma I on A's pattern (L below the diagonal, unit diagonal implied; U on and above) */
+int or_ilu0(int n, const int *rowptr, const int *colidx, const double *val, double sigma, double *lu);
 int or_multinplace(int n, int ld, double *V, int nc, int l, int k,
                    const double *Q, int ldq, int s, int e);
 

@@ -27,7 +27,8 @@ class Opts(C.Structure):
                 ("hermitian", C.c_int), ("keep", C.c_double), ("lock", C.c_int),
                 ("eta", C.c_double), ("refine", C.c_int), ("nthreads", C.c_int),
                 ("restart_cap", C.c_int), ("extraction", C.c_int), ("twosided", C.c_int),
-                ("st_sinvert", C.c_int), ("pc_none", C.c_int), ("ksp_rtol", C.c_double), ("ksp_bcgs", C.c_int)]
+                ("st_sinvert", C.c_int), ("pc_none", C.c_int), ("ksp_rtol", C.c_double), ("ksp_bcgs", C.c_int),
+                ("pc_ilu", C.c_int)]
 
 
 class Info(C.Structure):
@@ -145,6 +146,21 @@ def bv_tsqr_split(V: np.ndarray, nblocks: int):
     return R
 
 
+def ilu0(csr, sigma: float = 0.0) -> np.ndarray:
+    """ILU(0) factor values of A - sigma I on A's pattern (L strictly below the
+    diagonal, U on and above)."""
+    rowptr = np.ascontiguousarray(csr[0], dtype=np.int32)
+    colidx = np.ascontiguousarray(csr[1], dtype=np.int32)
+    val = np.ascontiguousarray(csr[2], dtype=np.float64)
+    lu = np.zeros(len(val))
+    ip = C.POINTER(C.c_int)
+    rc = lib().or_ilu0(C.c_int(len(rowptr) - 1), rowptr.ctypes.data_as(ip), colidx.ctypes.data_as(ip), _p(val),
+                       C.c_double(sigma), _p(lu))
+    if rc:
+        raise RuntimeError(f"or_ilu0 failed ({rc})")
+    return lu
+
+
 def bv_dot(X: np.ndarray, Y: np.ndarray, M: np.ndarray, lx: int = 0, ly: int = 0):
     """M(ly:, lx:) = Y(:, ly:)^T X(:, lx:) in place (BVDot with leading columns)."""
     assert X.flags.f_contiguous and Y.flags.f_contiguous and M.flags.f_contiguous
@@ -180,13 +196,15 @@ def eps_solve(csr, nev: int, ncv: int = 0, mpd: int = 0, tol: float = 1e-8,
               v0=None, deflation=None, max_it: int = 0, keep: float = 0.5, lock: bool = True,
               eta: float = 0.7071, refine: int = REFINE_IFNEEDED, nthreads: int = 0,
               restart_cap: int = 0, want_vectors: bool = True, extraction: int = 0, twosided: bool = False,
-              sinvert: bool = False, pc_none: bool = False, ksp_rtol: float = 0.0, ksp: str = "cg"):
+              sinvert: bool = False, pc_none: bool = False, ksp_rtol: float = 0.0, ksp: str = "cg",
+              pc: str = "jacobi"):
     rowptr, colidx, val = _csr_args(csr)
     n = len(rowptr) - 1
     o = Opts(nev=nev, ncv=ncv, mpd=mpd, max_it=max_it, tol=tol, which=which, target=target,
              hermitian=int(hermitian), keep=keep, lock=int(lock), eta=eta, refine=refine,
              nthreads=nthreads, restart_cap=restart_cap, extraction=extraction, twosided=int(twosided),
-             st_sinvert=int(sinvert), pc_none=int(pc_none), ksp_rtol=ksp_rtol, ksp_bcgs=int(ksp == "bcgs"))
+             st_sinvert=int(sinvert), pc_none=int(pc_none), ksp_rtol=ksp_rtol, ksp_bcgs=int(ksp == "bcgs"),
+             pc_ilu=int(pc == "ilu"))
     ncv_eff = ncv if ncv > 0 else max(2 * nev, nev + 15)
     ncv_eff = min(ncv_eff, n) if ncv_eff > 0 else n
     size = max(ncv_eff, nev) + 2

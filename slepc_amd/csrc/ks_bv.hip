@@ -1053,9 +1053,10 @@ __global__ __launch_bounds__(KS_BLOCK) void k_multinplace_lds(T *__restrict__ V,
 #ifndef KS_MIP_GRID
 #define KS_MIP_GRID 1024
 #endif
-// Q fully in LDS while it takes at most this many bytes (with the output tiles)
+// Q fully in LDS while it takes at most this many bytes (with the output tiles):
+// two workgroups per CU (160 KB of LDS) at the largest
 #ifndef KS_MIP_LDS_MAX
-#define KS_MIP_LDS_MAX (64 * 1024)
+#define KS_MIP_LDS_MAX (80 * 1024)
 #endif
 
 template <typename T, int NT>

@@ -16,7 +16,9 @@
 // chunks and reads one small state block per chunk.  Reductions add
 // per-workgroup partial sums in a fixed order in a one-workgroup launch, so a
 // solve is deterministic.
+#include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "ks_internal.hpp"
 
@@ -210,7 +212,7 @@ __global__ __launch_bounds__(KS_BLOCK) void k_bcgs_start(int n, const double *__
   __shared__ double lds[KS_BLOCK / 64];
   double rr = 0.0;
   KS_FOR_ROWS(i, n) {
-    const double ri = dinv[i] * b[i];
+    const double ri = dinv ? dinv[i] * b[i] : b[i];   // no dinv: b holds B b already
     x[i] = 0.0;
     r[i] = ri;
     rh[i] = ri;
@@ -251,7 +253,9 @@ __global__ __launch_bounds__(KS_BLOCK) void k_bcgs_p(int n, const double *__rest
   KS_FOR_ROWS(i, n) p[i] = r[i] + beta * (p[i] - om * v[i]);
 }
 
-// y = B (y - sigma x) (y = A x on entry); partials of (h, y), (x, y), (y, y)
+// y = B (y - sigma x) (y = A x on entry) with B = diag(dinv), or y as it is
+// when dinv is null (ILU: the shift and B applied by earlier launches);
+// partials of (h, y), (x, y), (y, y)
 __global__ __launch_bounds__(KS_BLOCK) void k_bcgs_apply(int n, const double *__restrict__ x, double *__restrict__ y,
                                                         const double *__restrict__ h, const double *__restrict__ dinv,
                                                         double sigma, const ksp_state *S, int skip_sconv,
@@ -260,8 +264,8 @@ __global__ __launch_bounds__(KS_BLOCK) void k_bcgs_apply(int n, const double *__
   if (S->reason || (skip_sconv && S->sconv)) return;
   double hy = 0.0, xy = 0.0, yy = 0.0;
   KS_FOR_ROWS(i, n) {
-    const double yi = dinv[i] * (y[i] - sigma * x[i]);
-    y[i] = yi;
+    const double yi = dinv ? dinv[i] * (y[i] - sigma * x[i]) : y[i];   // no dinv: y holds B (A - sigma I) x
+    if (dinv) y[i] = yi;
     hy += h[i] * yi;
     xy += x[i] * yi;
     yy += yi * yi;
@@ -382,6 +386,87 @@ __global__ __launch_bounds__(KS_BLOCK) void k_bcgs_fin_iter(const double *__rest
   }
 }
 
+// ---------------------------------------------------------------- ILU(0)
+// lu = values of A - sigma I on A's pattern
+__global__ __launch_bounds__(KS_BLOCK) void k_ilu0_init(const int32_t *__restrict__ rowptr,
+                                                       const int32_t *__restrict__ colidx,
+                                                       const double *__restrict__ val, int n, double sigma,
+                                                       double *__restrict__ lu) {
+  KS_FOR_ROWS(i, n)
+    for (int32_t q = rowptr[i]; q < rowptr[i + 1]; q++) lu[q] = val[q] - (colidx[q] == i ? sigma : 0.0);
+}
+
+// one level of the IKJ factorization (Saad, Algorithm 10.4): a thread per row of
+// the level; the rows it reads (its L part) belong to earlier levels.  The
+// operations of a row are the oracle's, in the same order.
+__global__ __launch_bounds__(64) void k_ilu0_factor_level(const int32_t *__restrict__ rows, int cnt,
+                                                         const int32_t *__restrict__ rowptr,
+                                                         const int32_t *__restrict__ colidx,
+                                                         const int32_t *__restrict__ diag, double *__restrict__ lu,
+                                                         int *zero_pivot) {
+  const int t = blockIdx.x * 64 + threadIdx.x;
+  if (t >= cnt) return;
+  const int i = rows[t];
+  const int32_t re = rowptr[i + 1];
+  for (int32_t q = rowptr[i]; q < diag[i]; q++) {
+    const int k = colidx[q];
+    const double ukk = lu[diag[k]];
+    if (ukk == 0.0) {
+      atomicExch(zero_pivot, 1);
+      return;
+    }
+    const double lik = lu[q] / ukk;
+    lu[q] = lik;
+    int32_t u = diag[k] + 1;
+    const int32_t ke = rowptr[k + 1];
+    for (int32_t r = q + 1; r < re; r++) {
+      const int j = colidx[r];
+      while (u < ke && colidx[u] < j) u++;
+      if (u < ke && colidx[u] == j) lu[r] -= lik * lu[u];
+    }
+  }
+}
+
+// one level of y = L^-1 y (unit lower factor), in place; skipped once the solve
+// has its outcome (or, skip_sconv, after an early |s|)
+__global__ __launch_bounds__(64) void k_ilu0_lower_level(const int32_t *__restrict__ rows, int cnt,
+                                                        const int32_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ colidx,
+                                                        const int32_t *__restrict__ diag,
+                                                        const double *__restrict__ lu, double *y, const ksp_state *S,
+                                                        int skip_sconv) {
+  if (S && (S->reason || (skip_sconv && S->sconv))) return;
+  const int t = blockIdx.x * 64 + threadIdx.x;
+  if (t >= cnt) return;
+  const int i = rows[t];
+  double s = y[i];
+  for (int32_t q = rowptr[i]; q < diag[i]; q++) s -= lu[q] * y[colidx[q]];
+  y[i] = s;
+}
+
+// one level of y = U^-1 y, in place
+__global__ __launch_bounds__(64) void k_ilu0_upper_level(const int32_t *__restrict__ rows, int cnt,
+                                                        const int32_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ colidx,
+                                                        const int32_t *__restrict__ diag,
+                                                        const double *__restrict__ lu, double *y, const ksp_state *S,
+                                                        int skip_sconv) {
+  if (S && (S->reason || (skip_sconv && S->sconv))) return;
+  const int t = blockIdx.x * 64 + threadIdx.x;
+  if (t >= cnt) return;
+  const int i = rows[t];
+  double s = y[i];
+  for (int32_t q = diag[i] + 1; q < rowptr[i + 1]; q++) s -= lu[q] * y[colidx[q]];
+  y[i] = s / lu[diag[i]];
+}
+
+// y = y - sigma x, skipped like the products
+__global__ __launch_bounds__(KS_BLOCK) void k_shift_gated(int n, const double *__restrict__ x, double *__restrict__ y,
+                                                         double sigma, const ksp_state *S, int skip_sconv) {
+  if (S->reason || (skip_sconv && S->sconv)) return;
+  KS_FOR_ROWS(i, n) y[i] -= sigma * x[i];
+}
+
 inline int ksp_blocks(int n) {
   int b = (n + KS_BLOCK - 1) / KS_BLOCK;
   if (b > KS_PCG_BLOCKS) b = KS_PCG_BLOCKS;
@@ -403,6 +488,136 @@ static int read_state(ks_ctx ctx, const ksp_state *S, int *its, int *reason, dou
 }
 
 }  // namespace
+
+struct ks_ilu_s {
+  int n = 0;
+  std::vector<int> lptr, uptr;        // level starts in lrows / urows (host)
+  int32_t *d_lrows = nullptr, *d_urows = nullptr, *d_diag = nullptr;
+  double *d_lu = nullptr;
+  int *d_flag = nullptr;
+  const int32_t *d_rowptr = nullptr, *d_colidx = nullptr;   // the operator's pattern (borrowed)
+};
+
+namespace {
+
+// y = (L U)^-1 y on the context stream, one launch per level of each factor
+void ilu0_apply(ks_ctx ctx, const ks_ilu M, double *y, const ksp_state *S, int skip_sconv) {
+  for (size_t l = 0; l + 1 < M->lptr.size(); l++) {
+    const int cnt = M->lptr[l + 1] - M->lptr[l];
+    hipLaunchKernelGGL(k_ilu0_lower_level, dim3((cnt + 63) / 64), dim3(64), 0, ctx->stream,
+                       (const int32_t *)M->d_lrows + M->lptr[l], cnt, M->d_rowptr, M->d_colidx,
+                       (const int32_t *)M->d_diag, (const double *)M->d_lu, y, S, skip_sconv);
+  }
+  for (size_t l = 0; l + 1 < M->uptr.size(); l++) {
+    const int cnt = M->uptr[l + 1] - M->uptr[l];
+    hipLaunchKernelGGL(k_ilu0_upper_level, dim3((cnt + 63) / 64), dim3(64), 0, ctx->stream,
+                       (const int32_t *)M->d_urows + M->uptr[l], cnt, M->d_rowptr, M->d_colidx,
+                       (const int32_t *)M->d_diag, (const double *)M->d_lu, y, S, skip_sconv);
+  }
+}
+
+// rows grouped by level (stable in row order) and the level starts
+void group_levels(const std::vector<int> &lev, std::vector<int32_t> &rows, std::vector<int> &ptr) {
+  int nl = 0;
+  for (int v : lev) nl = v + 1 > nl ? v + 1 : nl;
+  ptr.assign(nl + 1, 0);
+  for (int v : lev) ptr[v + 1]++;
+  for (int l = 0; l < nl; l++) ptr[l + 1] += ptr[l];
+  std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+  rows.resize(lev.size());
+  for (size_t i = 0; i < lev.size(); i++) rows[fill[lev[i]]++] = (int32_t)i;
+}
+
+}  // namespace
+
+int ks_ilu0_create(ks_ctx ctx, ks_csr A, double sigma, ks_ilu *out) {
+  KS_CHECK(ctx && A && out, KS_ERR_ARG, "null ILU argument");
+  KS_CHECK(A->vprec == KS_F64 && A->nrows == A->ncols, KS_ERR_ARG, "ILU(0) needs a square FP64 matrix");
+  *out = nullptr;
+  const int n = A->nrows;
+  const int64_t nnz = A->nnz;
+  std::vector<int32_t> rp(n + 1), ci(nnz), diag(n, -1);
+  KS_HIP(hipMemcpyAsync(rp.data(), A->d_rowptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
+  KS_HIP(hipMemcpyAsync(ci.data(), A->d_colidx, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, ctx->stream));
+  KS_HIP(hipStreamSynchronize(ctx->stream));
+  // the pattern's rows must list increasing columns and hold their diagonal
+  for (int i = 0; i < n; i++) {
+    for (int32_t q = rp[i]; q < rp[i + 1]; q++) {
+      KS_CHECK(q == rp[i] || ci[q] > ci[q - 1], KS_ERR_ARG, "ILU(0) needs increasing column indices (row %d)", i);
+      if (ci[q] == i) diag[i] = q;
+    }
+    KS_CHECK(diag[i] >= 0, KS_ERR_ARG, "ILU(0) needs the diagonal entry of row %d", i);
+  }
+  // levels: forward (row i after the rows of its L part), backward (after its U part)
+  std::vector<int> lev(n, 0);
+  for (int i = 0; i < n; i++)
+    for (int32_t q = rp[i]; q < diag[i]; q++) lev[i] = lev[ci[q]] + 1 > lev[i] ? lev[ci[q]] + 1 : lev[i];
+  ks_ilu M = new ks_ilu_s();
+  M->n = n;
+  M->d_rowptr = A->d_rowptr;
+  M->d_colidx = A->d_colidx;
+  std::vector<int32_t> lrows, urows;
+  group_levels(lev, lrows, M->lptr);
+  std::fill(lev.begin(), lev.end(), 0);
+  for (int i = n - 1; i >= 0; i--)
+    for (int32_t q = diag[i] + 1; q < rp[i + 1]; q++) lev[i] = lev[ci[q]] + 1 > lev[i] ? lev[ci[q]] + 1 : lev[i];
+  group_levels(lev, urows, M->uptr);
+  int e = KS_OK;
+  if (!e) e = ks_malloc((void **)&M->d_lrows, sizeof(int32_t) * n);
+  if (!e) e = ks_malloc((void **)&M->d_urows, sizeof(int32_t) * n);
+  if (!e) e = ks_malloc((void **)&M->d_diag, sizeof(int32_t) * n);
+  if (!e) e = ks_malloc((void **)&M->d_lu, sizeof(double) * (nnz > 0 ? nnz : 1));
+  if (!e) e = ks_malloc((void **)&M->d_flag, sizeof(int));
+  if (e) {
+    ks_ilu0_destroy(M);
+    return e;
+  }
+  hipStream_t st = ctx->stream;
+  KS_HIP(hipMemcpyAsync(M->d_lrows, lrows.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+  KS_HIP(hipMemcpyAsync(M->d_urows, urows.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+  KS_HIP(hipMemcpyAsync(M->d_diag, diag.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+  KS_TRY(ks_memset_zero(ctx, M->d_flag, sizeof(int)));
+  hipLaunchKernelGGL(k_ilu0_init, dim3(ksp_blocks(n)), dim3(KS_BLOCK), 0, st, A->d_rowptr, A->d_colidx,
+                     (const double *)A->d_val, n, sigma, M->d_lu);
+  for (size_t l = 0; l + 1 < M->lptr.size(); l++) {
+    const int cnt = M->lptr[l + 1] - M->lptr[l];
+    hipLaunchKernelGGL(k_ilu0_factor_level, dim3((cnt + 63) / 64), dim3(64), 0, st,
+                       (const int32_t *)M->d_lrows + M->lptr[l], cnt, A->d_rowptr, A->d_colidx,
+                       (const int32_t *)M->d_diag, M->d_lu, M->d_flag);
+  }
+  KS_HIP(hipGetLastError());
+  // the last pivot, and a zero pivot met on the way
+  int flag = 0;
+  double ulast = 0.0;
+  KS_HIP(hipMemcpyAsync(&flag, M->d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  KS_HIP(hipMemcpyAsync(&ulast, M->d_lu + diag[n - 1], sizeof(double), hipMemcpyDeviceToHost, st));
+  KS_HIP(hipStreamSynchronize(st));
+  if (flag || ulast == 0.0) {
+    fprintf(stderr, "ks_hip: ILU(0) met a zero pivot\n");
+    ks_ilu0_destroy(M);
+    return KS_ERR_PIVOT;
+  }
+  *out = M;
+  return KS_OK;
+}
+
+int ks_ilu0_destroy(ks_ilu M) {
+  if (!M) return KS_OK;
+  if (M->d_lrows) (void)ks_free(M->d_lrows);
+  if (M->d_urows) (void)ks_free(M->d_urows);
+  if (M->d_diag) (void)ks_free(M->d_diag);
+  if (M->d_lu) (void)ks_free(M->d_lu);
+  if (M->d_flag) (void)ks_free(M->d_flag);
+  delete M;
+  return KS_OK;
+}
+
+int ks_ilu0_levels(ks_ilu M, int *nlower, int *nupper) {
+  KS_CHECK(M && nlower && nupper, KS_ERR_ARG, "null argument");
+  *nlower = (int)M->lptr.size() - 1;
+  *nupper = (int)M->uptr.size() - 1;
+  return KS_OK;
+}
 
 int ks_pcg_jacobi(ks_ctx ctx, ks_csr A, double sigma, int jacobi, double *d_dinv) {
   KS_CHECK(ctx && A && d_dinv, KS_ERR_ARG, "null argument");
@@ -447,11 +662,12 @@ int ks_pcg_solve(ks_ctx ctx, ks_csr A, double sigma, const double *d_dinv, const
   return KS_OK;
 }
 
-int ks_bcgs_solve(ks_ctx ctx, ks_csr A, double sigma, const double *d_dinv, const double *d_b, double *d_x,
-                  double rtol, double abstol, int maxits, double *d_work, int *its, double *rnorm, int *reason) {
-  KS_CHECK(ctx && A && d_dinv && d_b && d_x && d_work && its && rnorm && reason, KS_ERR_ARG,
-           "null BiCGStab argument");
-  KS_CHECK(A->vprec == KS_F64 && A->nrows == A->ncols, KS_ERR_ARG, "BiCGStab needs a square FP64 matrix");
+namespace {
+
+// BiCGStab with B = diag(d_dinv) (ilu null) or B = (L U)^-1 (ilu)
+int bcgs_solve(ks_ctx ctx, ks_csr A, double sigma, const double *d_dinv, const ks_ilu ilu, const double *d_b,
+               double *d_x, double rtol, double abstol, int maxits, double *d_work, int *its, double *rnorm,
+               int *reason) {
   KS_CHECK(maxits > 0, KS_ERR_ARG, "BiCGStab needs maxits > 0");
   const int n = A->nrows, nb = ksp_blocks(n);
   double *r = d_work, *rh = r + n, *p = rh + n, *v = p + n, *sv = v + n, *t = sv + n;
@@ -459,22 +675,38 @@ int ks_bcgs_solve(ks_ctx ctx, ks_csr A, double sigma, const double *d_dinv, cons
   double *part = ctx->d_partial;
   hipStream_t st = ctx->stream;
   const dim3 g(nb), b1(KS_BLOCK), one(1);
-  hipLaunchKernelGGL(k_bcgs_start, g, b1, 0, st, n, d_b, d_x, r, rh, p, v, d_dinv, part);
+  if (ilu) {
+    KS_TRY(ks_memcpy_dtod(ctx, r, d_b, sizeof(double) * n));
+    ilu0_apply(ctx, ilu, r, nullptr, 0);                                             // r = B b
+    hipLaunchKernelGGL(k_bcgs_start, g, b1, 0, st, n, (const double *)r, d_x, r, rh, p, v, (const double *)nullptr,
+                       part);
+  } else {
+    hipLaunchKernelGGL(k_bcgs_start, g, b1, 0, st, n, d_b, d_x, r, rh, p, v, d_dinv, part);
+  }
   hipLaunchKernelGGL(k_bcgs_fin_start, one, b1, 0, st, (const double *)part, nb, S, rtol, abstol, maxits);
+  const double *dv = ilu ? nullptr : d_dinv;
   for (int done = 0;; done += KS_KSP_CHUNK) {
     KS_TRY(read_state(ctx, S, its, reason, rnorm));
     if (*reason || done >= maxits) break;
     for (int c = 0; c < KS_KSP_CHUNK; c++) {
       hipLaunchKernelGGL(k_bcgs_p, g, b1, 0, st, n, (const double *)r, p, (const double *)v, (const ksp_state *)S);
       KS_TRY(ks_spmv_gated(ctx, A, p, v, KS_F64, &S->gate));                         // v = A p
-      hipLaunchKernelGGL(k_bcgs_apply, g, b1, 0, st, n, (const double *)p, v, (const double *)rh, d_dinv, sigma,
+      if (ilu) {
+        hipLaunchKernelGGL(k_shift_gated, g, b1, 0, st, n, (const double *)p, v, sigma, (const ksp_state *)S, 0);
+        ilu0_apply(ctx, ilu, v, S, 0);
+      }
+      hipLaunchKernelGGL(k_bcgs_apply, g, b1, 0, st, n, (const double *)p, v, (const double *)rh, dv, sigma,
                          (const ksp_state *)S, 0, part);                               // v = B (A - sigma I) p
       hipLaunchKernelGGL(k_bcgs_fin_alpha, one, b1, 0, st, (const double *)part, nb, S);
       hipLaunchKernelGGL(k_bcgs_s, g, b1, 0, st, n, (const double *)r, (const double *)v, sv, (const ksp_state *)S,
                          part);
       hipLaunchKernelGGL(k_bcgs_fin_s, one, b1, 0, st, (const double *)part, nb, S);
       KS_TRY(ks_spmv_gated(ctx, A, sv, t, KS_F64, &S->gate));                        // t = A s
-      hipLaunchKernelGGL(k_bcgs_apply, g, b1, 0, st, n, (const double *)sv, t, (const double *)rh, d_dinv, sigma,
+      if (ilu) {
+        hipLaunchKernelGGL(k_shift_gated, g, b1, 0, st, n, (const double *)sv, t, sigma, (const ksp_state *)S, 1);
+        ilu0_apply(ctx, ilu, t, S, 1);
+      }
+      hipLaunchKernelGGL(k_bcgs_apply, g, b1, 0, st, n, (const double *)sv, t, (const double *)rh, dv, sigma,
                          (const ksp_state *)S, 1, part);                               // t = B (A - sigma I) s
       hipLaunchKernelGGL(k_bcgs_fin_omega, one, b1, 0, st, (const double *)part, nb, S);
       hipLaunchKernelGGL(k_bcgs_x, g, b1, 0, st, n, d_x, (const double *)p, (const double *)sv, (const double *)t, r,
@@ -485,4 +717,24 @@ int ks_bcgs_solve(ks_ctx ctx, ks_csr A, double sigma, const double *d_dinv, cons
   }
   if (!*reason) *reason = -3;
   return KS_OK;
+}
+
+}  // namespace
+
+int ks_bcgs_solve(ks_ctx ctx, ks_csr A, double sigma, const double *d_dinv, const double *d_b, double *d_x,
+                  double rtol, double abstol, int maxits, double *d_work, int *its, double *rnorm, int *reason) {
+  KS_CHECK(ctx && A && d_dinv && d_b && d_x && d_work && its && rnorm && reason, KS_ERR_ARG,
+           "null BiCGStab argument");
+  KS_CHECK(A->vprec == KS_F64 && A->nrows == A->ncols, KS_ERR_ARG, "BiCGStab needs a square FP64 matrix");
+  return bcgs_solve(ctx, A, sigma, d_dinv, nullptr, d_b, d_x, rtol, abstol, maxits, d_work, its, rnorm, reason);
+}
+
+int ks_bcgs_solve_ilu(ks_ctx ctx, ks_csr A, double sigma, ks_ilu M, const double *d_b, double *d_x, double rtol,
+                      double abstol, int maxits, double *d_work, int *its, double *rnorm, int *reason) {
+  KS_CHECK(ctx && A && M && d_b && d_x && d_work && its && rnorm && reason, KS_ERR_ARG,
+           "null BiCGStab argument");
+  KS_CHECK(A->vprec == KS_F64 && A->nrows == A->ncols && M->n == A->nrows, KS_ERR_ARG,
+           "BiCGStab with ILU(0) needs the square FP64 matrix the factors were made from");
+  KS_CHECK(M->d_rowptr == A->d_rowptr, KS_ERR_ARG, "the ILU(0) factors belong to another matrix");
+  return bcgs_solve(ctx, A, sigma, nullptr, M, d_b, d_x, rtol, abstol, maxits, d_work, its, rnorm, reason);
 }

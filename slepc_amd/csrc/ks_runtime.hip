@@ -68,6 +68,7 @@ const char *ks_error_string(int code) {
     case KS_ERR_NODEV: return "no gfx950 device";
     case KS_ERR_INNER: return "invalid inner product";
     case KS_ERR_RANGE: return "size beyond supported maximum";
+    case KS_ERR_PIVOT: return "zero pivot";
   }
   return "unknown error";
 }

@@ -334,7 +334,7 @@ class EPS:
               ksp_type=None):
         """STSetType (+ STSetShift) and the inner solver of shift-and-invert
         (STGetKSP -> KSPSetType ("cg" or "bcgs"), KSPSetTolerances, KSPGetPC ->
-        PCSetType)."""
+        PCSetType ("jacobi", "none", or "ilu" with "bcgs"))."""
         st = C.c_void_p()
         _chk(lib().EPSGetST(self.h, C.byref(st)), "EPSGetST")
         _chk(lib().STSetType(st, st_type.encode()), "STSetType")

@@ -363,6 +363,7 @@ struct _p_KSP {              /* the inner solver of STSINVERT (device CG or BiCG
   PetscReal rtol, abstol;
   PetscInt maxits, n;
   double *d_dinv, *d_work;   /* Jacobi diagonal, the solver's vectors + its state */
+  ks_ilu ilu;                /* PCILU: the ILU(0) factors of A - sigma I */
   size_t nwork;              /* doubles in d_work */
   int64_t its_total, nsolves;
 };

@@ -8,7 +8,8 @@
  * back-transform.  STSINVERT's operator is (A - sigma I)^-1, applied by an inner
  * solve (KSP) on the device: preconditioned conjugate gradients (ks_pcg_solve,
  * KSPCG) or BiCGStab (ks_bcgs_solve, KSPBCGS, for a shift inside the
- * spectrum), with a Jacobi or no preconditioner.  Its eigenvalues theta map
+ * spectrum), with a Jacobi or no preconditioner, or with BiCGStab the ILU(0)
+ * factors of A - sigma I (ks_ilu0_create, PCILU).  Its eigenvalues theta map
  * back to lambda = sigma + 1/theta (STBackTransform_Sinvert, sinvert.c:32-50).
  * The default shift is the EPS target (STSetDefaultShift, from EPSSetTarget).
  */
@@ -101,8 +102,8 @@ PetscErrorCode KSPGetPC(KSP ksp, PC *pc)
 PetscErrorCode PCSetType(PC pc, PCType type)
 {
   PetscCheck(pc && type, PETSC_ERR_ARG_NULL, "null argument");
-  PetscCheck(!strcmp(type, PCJACOBI) || !strcmp(type, PCNONE), PETSC_ERR_SUP,
-             "PC type %s is not available (jacobi or none)", type);
+  PetscCheck(!strcmp(type, PCJACOBI) || !strcmp(type, PCNONE) || !strcmp(type, PCILU), PETSC_ERR_SUP,
+             "PC type %s is not available (jacobi, ilu or none)", type);
   strcpy(pc->type, type);
   return PETSC_SUCCESS;
 }
@@ -125,6 +126,8 @@ PetscErrorCode KSPGetTotalIterations(KSP ksp, PetscInt *its)
 
 static void KSPFreeWork(KSP ksp)
 {
+  if (ksp->ilu) (void)ks_ilu0_destroy(ksp->ilu);
+  ksp->ilu = NULL;
   if (ksp->d_dinv) (void)ks_free(ksp->d_dinv);
   if (ksp->d_work) (void)ks_free(ksp->d_work);
   ksp->d_dinv = ksp->d_work = NULL;
@@ -144,9 +147,14 @@ static PetscErrorCode MatMult_STSinvert(Mat P, Vec x, Vec y)
   PetscCheck(x->prec == SLEPC_AMD_F64 && y->prec == SLEPC_AMD_F64, PETSC_ERR_SUP,
              "shift-and-invert runs on FP64 vectors");
   const PetscBool cg = !strcmp(ksp->type, KSPCG) ? PETSC_TRUE : PETSC_FALSE;
+  PetscCheck(strcmp(ksp->pc.type, PCILU) || (!cg && ksp->ilu), PETSC_ERR_ORDER,
+             "PC type %s needs KSP type %s set before STSetUp", PCILU, KSPBCGS);
   if (cg)
     KSCall(ks_pcg_solve(SlepcAMDContext(), h->csr, st->sigma, ksp->d_dinv, (const double *)x->d, (double *)y->d,
                         ksp->rtol, ksp->abstol, (int)ksp->maxits, ksp->d_work, &its, &rnorm, &reason));
+  else if (ksp->ilu)
+    KSCall(ks_bcgs_solve_ilu(SlepcAMDContext(), h->csr, st->sigma, ksp->ilu, (const double *)x->d, (double *)y->d,
+                             ksp->rtol, ksp->abstol, (int)ksp->maxits, ksp->d_work, &its, &rnorm, &reason));
   else
     KSCall(ks_bcgs_solve(SlepcAMDContext(), h->csr, st->sigma, ksp->d_dinv, (const double *)x->d, (double *)y->d,
                          ksp->rtol, ksp->abstol, (int)ksp->maxits, ksp->d_work, &its, &rnorm, &reason));
@@ -192,6 +200,13 @@ PetscErrorCode STSetUp(ST st)
       ksp->nwork = nwork;
     }
     KSCall(ks_pcg_jacobi(SlepcAMDContext(), h->csr, st->sigma, !strcmp(ksp->pc.type, PCJACOBI), ksp->d_dinv));
+    if (ksp->ilu) (void)ks_ilu0_destroy(ksp->ilu);
+    ksp->ilu = NULL;
+    if (!strcmp(ksp->pc.type, PCILU)) {
+      PetscCheck(!strcmp(ksp->type, KSPBCGS), PETSC_ERR_SUP, "PC type %s is available with KSP type %s", PCILU,
+                 KSPBCGS);
+      KSCall(ks_ilu0_create(SlepcAMDContext(), h->csr, st->sigma, &ksp->ilu));
+    }
     ksp->its_total = 0;
     ksp->nsolves = 0;
     if (!st->P) {

@@ -1,6 +1,7 @@
 """Shift-and-invert on the MI355X (STSINVERT; the inner solves by the device
-CG of ks_pcg_solve or BiCGStab of ks_bcgs_solve, through
-STGetKSP/KSPSetType/KSPSetTolerances/PCSetType): the reference's ex11 golden
+CG of ks_pcg_solve or BiCGStab of ks_bcgs_solve with Jacobi, none or the
+ILU(0) of ks_ilu0_create, through STGetKSP/KSPSetType/KSPSetTolerances/
+PCSetType): the reference's ex11 golden
 with -st_type sinvert -st_ksp_type cg -st_pc_type jacobi, the device against
 the oracle (restarts, inner iterations, eigenvalues), the interior goldens of
 eps test2 (target 2.1 inside the spectrum, A - sigma I indefinite) with
@@ -103,6 +104,46 @@ def test_interior_sinvert_bcgs_2d_vs_oracle(E, oracle):
     assert abs(r["ksp_its"] - o["ksp_its"]) <= 0.25 * o["ksp_its"]
     with pytest.raises(E.SlepcError):   # CG stops on the indefinite shifted matrix
         E.solve(csr, which=E.EPS.TARGET_MAGNITUDE, st=dict(st_type="sinvert", ksp_type="cg"), **kw)
+
+
+@pytest.mark.parametrize("n", [30, 32])
+def test_interior_bcgs_ilu_vs_oracle(E, oracle, n):
+    # ILU(0) factored on the device; on the tridiagonal A - 2.1 I it is the exact LU,
+    # so every inner solve takes one BiCGStab iteration
+    csr = M.laplacian_1d(n)
+    st = dict(st_type="sinvert", ksp_type="bcgs", pc_type="ilu", ksp_rtol=1e-12)
+    r = E.solve(csr, nev=4, tol=1e-10, which=E.EPS.TARGET_MAGNITUDE, target=2.1, st=st)
+    o = oracle.eps_solve(csr, nev=4, tol=1e-10, which=oracle.TARGET_MAGNITUDE, target=2.1, sinvert=True, ksp="bcgs",
+                         pc="ilu", ksp_rtol=1e-12)
+    print(f"n={n} its={r['its']}/{o['its']} ksp_its={r['ksp_its']}/{o['ksp_its']}", flush=True)
+    assert r["reason"] == 1 and ", ".join("%.5f" % v for v in r["eigr"][:4]) == TEST2_INTERIOR[n]
+    assert np.allclose(r["eigr"][:4], o["eigr"][:4], rtol=1e-10, atol=0)
+    assert abs(r["its"] - o["its"]) <= 1 and r["ksp_its"] == o["ksp_its"]
+
+
+def test_interior_sinvert_bcgs_ilu_2d_vs_oracle(E, oracle):
+    N, Mm = 64, 63
+    csr = M.laplacian_2d(N, Mm)
+    n = len(csr[0]) - 1
+    closed = np.sort(M.laplacian_2d_eigenvalues(N, Mm))
+    sigma = 0.5 * (closed[3] + closed[4])
+    kw = dict(nev=4, ncv=24, tol=1e-10, target=sigma, v0=M.uniform_vector(3, n), want_vectors=False)
+    r = E.solve(csr, which=E.EPS.TARGET_MAGNITUDE,
+                st=dict(st_type="sinvert", ksp_type="bcgs", pc_type="ilu", ksp_rtol=1e-12), **kw)
+    o = oracle.eps_solve(csr, which=oracle.TARGET_MAGNITUDE, sinvert=True, ksp="bcgs", pc="ilu", ksp_rtol=1e-12,
+                         **kw)
+    near = closed[np.argsort(np.abs(closed - sigma))][:4]
+    print(f"n={n} its={r['its']}/{o['its']} ksp_its={r['ksp_its']}/{o['ksp_its']}", flush=True)
+    assert r["reason"] == 1 and abs(r["its"] - o["its"]) <= 1
+    assert np.allclose(r["eigr"][:4], o["eigr"][:4], rtol=1e-10, atol=0)
+    assert np.allclose(np.sort(r["eigr"][:4]), np.sort(near), rtol=1e-10, atol=0)
+    assert abs(r["ksp_its"] - o["ksp_its"]) <= 0.25 * o["ksp_its"]
+
+
+def test_ilu_needs_bcgs(E):
+    with pytest.raises(E.SlepcError):
+        E.solve(M.laplacian_1d(50), nev=2, which=E.EPS.TARGET_MAGNITUDE, target=-0.1,
+                st=dict(st_type="sinvert", ksp_type="cg", pc_type="ilu"))
 
 
 def test_bcgs_matches_cg_on_a_definite_shift(E):

@@ -80,3 +80,62 @@ def test_bcgs_matches_cg_on_a_definite_shift(oracle):
     a = oracle.eps_solve(csr, ksp="cg", **kw)
     b = oracle.eps_solve(csr, ksp="bcgs", **kw)
     assert np.allclose(a["eigr"][:4], b["eigr"][:4], rtol=1e-10, atol=1e-12)
+
+
+def _lu_dense(csr, lu):
+    n = len(csr[0]) - 1
+    L, U = np.eye(n), np.zeros((n, n))
+    for i in range(n):
+        for q in range(csr[0][i], csr[0][i + 1]):
+            j = csr[1][q]
+            if j < i:
+                L[i, j] = lu[q]
+            else:
+                U[i, j] = lu[q]
+    return L, U
+
+
+def test_ilu0_factors(oracle):
+    # ILU(0)'s defining property: L U equals A - sigma I on A's pattern; on a
+    # tridiagonal matrix it has no fill to drop and is the exact LU factorization
+    sigma = 2.1
+    csr = M.laplacian_2d(9, 7)
+    n = len(csr[0]) - 1
+    L, U = _lu_dense(csr, oracle.ilu0(csr, sigma))
+    A = M.csr_to_dense(*csr) - sigma * np.eye(n)
+    P = L @ U
+    mask = A != 0
+    mask[np.arange(n), np.arange(n)] = True
+    assert np.abs((P - A)[mask]).max() < 1e-13
+    assert np.abs((P - A)[~mask]).max() > 1e-3            # the fill ILU(0) drops
+    csr1 = M.laplacian_1d(40)
+    L1, U1 = _lu_dense(csr1, oracle.ilu0(csr1, sigma))
+    A1 = M.csr_to_dense(*csr1) - sigma * np.eye(40)
+    assert np.abs(L1 @ U1 - A1).max() < 1e-13
+
+
+@pytest.mark.parametrize("n", [30, 32])
+def test_interior_bcgs_ilu_golden(oracle, n):
+    # with ILU(0) (the exact LU of the tridiagonal A - 2.1 I) each inner solve takes one iteration
+    csr = M.laplacian_1d(n)
+    r = oracle.eps_solve(csr, nev=4, tol=1e-10, which=oracle.TARGET_MAGNITUDE, target=2.1, sinvert=True,
+                         ksp="bcgs", pc="ilu", ksp_rtol=1e-12)
+    assert r["reason"] == 1 and ", ".join("%.5f" % v for v in r["eigr"][:4]) == TEST2_INTERIOR[n]
+    assert 0 < r["ksp_its"] <= r["nmatmult"]           # at most one iteration per application
+
+
+def test_ilu_fewer_iterations_than_jacobi(oracle):
+    # 64 x 63 Laplacian, shift between the 4th and 5th eigenvalues: the same eigenpairs,
+    # with ILU(0) in far fewer BiCGStab iterations
+    csr = M.laplacian_2d(64, 63)
+    n = len(csr[0]) - 1
+    closed = np.sort(M.laplacian_2d_eigenvalues(64, 63))
+    sigma = 0.5 * (closed[3] + closed[4])
+    kw = dict(nev=4, ncv=24, tol=1e-10, which=oracle.TARGET_MAGNITUDE, target=sigma, sinvert=True, ksp="bcgs",
+              ksp_rtol=1e-12, v0=M.uniform_vector(3, n), want_vectors=False)
+    a = oracle.eps_solve(csr, pc="jacobi", **kw)
+    b = oracle.eps_solve(csr, pc="ilu", **kw)
+    near = closed[np.argsort(np.abs(closed - sigma))][:4]
+    assert np.allclose(np.sort(b["eigr"][:4]), np.sort(near), rtol=1e-10, atol=0)
+    assert np.allclose(a["eigr"][:4], b["eigr"][:4], rtol=1e-10, atol=0)
+    assert b["ksp_its"] < a["ksp_its"] / 4
