@@ -78,6 +78,20 @@ __device__ __forceinline__ void spmv_block(int b, const int32_t *__restrict__ bl
   }
 }
 
+// Workgroups go to the 8 XCDs round-robin (workgroup i to XCD i mod 8), each
+// XCD with its own L2.  Row block xcd_block(i) gives XCD x a contiguous eighth
+// of the row blocks, so the x entries a block gathers from neighbouring rows
+// (a stencil's +-1, +-nx, +-nx*ny) are mostly in the L2 its XCD already filled.
+// A bijection on [0, nb) for any nb.
+#ifndef KS_SPMV_XCD
+#define KS_SPMV_XCD 1
+#endif
+__device__ __forceinline__ int xcd_block(int i, int nb) {
+  if (!KS_SPMV_XCD) return i;
+  const int q = nb >> 3, r = nb & 7, x = i & 7, j = i >> 3;
+  return x < r ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
+}
+
 template <typename VT, typename XT>
 __global__ __launch_bounds__(KS_BLOCK) void k_spmv_csr(const int32_t *__restrict__ blk,
                                                        const int32_t *__restrict__ rowptr,
@@ -87,7 +101,7 @@ __global__ __launch_bounds__(KS_BLOCK) void k_spmv_csr(const int32_t *__restrict
                                                        XT *__restrict__ y,
                                                        const ks_status *__restrict__ st) {
   if (st && !st->active) return;
-  spmv_block<VT, XT>(blockIdx.x, blk, rowptr, colidx, val, x, y);
+  spmv_block<VT, XT>(xcd_block(blockIdx.x, gridDim.x), blk, rowptr, colidx, val, x, y);
 }
 
 // the SpMVs of several independent solves in one launch: workgroup b belongs to

@@ -37,6 +37,12 @@ class Status(C.Structure):
                 ("halt_col", C.c_int32), ("halt_need", C.c_int32)]
 
 
+class GroupMember(C.Structure):
+    """ks_group_member: one solve's column step inside a grouped launch."""
+    _fields_ = [("ctx", C.c_void_p), ("A", C.c_void_p), ("V", C.c_void_p), ("ld", C.c_int), ("n", C.c_int),
+                ("k", C.c_int), ("buf", C.c_void_p), ("ldbuf", C.c_int), ("hcol", C.c_int), ("col", C.c_int)]
+
+
 class ProfEntry(C.Structure):
     _fields_ = [("launches", C.c_int64), ("skipped", C.c_int64), ("ms", C.c_double),
                 ("bytes", C.c_double), ("flops", C.c_double)]
@@ -77,6 +83,9 @@ def lib():
             "ks_prof_enable": [vp, i32], "ks_prof_enable_classes": [vp, C.c_uint], "ks_prof_collect": [vp],
             "ks_prof_get": [vp, i32, C.POINTER(ProfEntry)], "ks_prof_reset": [vp],
             "ks_mem_info": [C.POINTER(sz), C.POINTER(sz)],
+            "ks_group_create": [C.POINTER(vp)], "ks_group_destroy": [vp],
+            "ks_group_step": [vp, i32, vp, i32, dbl], "ks_group_wait_ctx": [vp, vp],
+            "ks_ctx_wait_group": [vp, vp],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -221,6 +230,29 @@ class CSR:
     def free(self):
         if self.h:
             lib().ks_csr_destroy(self.h)
+            self.h = None
+
+
+class Group:
+    """ks_group: one Arnoldi step of several solves per grouped launch."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        check(lib().ks_group_create(C.byref(self.h)), "ks_group_create")
+
+    def step(self, members, refine=REFINE_IFNEEDED, eta=0.7071):
+        arr = (GroupMember * len(members))(*members)
+        check(lib().ks_group_step(self.h, len(members), C.cast(arr, C.c_void_p), refine, eta), "ks_group_step")
+
+    def wait_ctx(self, ctx):
+        check(lib().ks_group_wait_ctx(self.h, ctx.h), "ks_group_wait_ctx")
+
+    def ctx_wait(self, ctx):
+        check(lib().ks_ctx_wait_group(ctx.h, self.h), "ks_ctx_wait_group")
+
+    def close(self):
+        if self.h:
+            lib().ks_group_destroy(self.h)
             self.h = None
 
 

@@ -381,3 +381,60 @@ def test_norm_scale_axpy(ks, ctx):
     assert np.allclose(dy.numpy(), 0.5 * (y - 3.0 * x), rtol=1e-15, atol=1e-15)
     ks.scale(ctx, ks.F64, 1000, dy, 0.0)
     assert not dy.numpy().any()
+
+
+def test_grouped_step_second_pass_matches_alone(ks, oracle):
+    # three solves in one grouped Arnoldi step, each column A v 1e-6 off the span of
+    # its basis (A = I + 1e-6 L), so every member takes the second DGKS pass inside
+    # the grouped launches; each member's stored column, h and pass count against the
+    # same step done alone on its own context
+    from slepc_amd import matgen as M
+    specs = [(5003, 8), (20011, 20), (70001, 40)]
+    ctxs, mats, Vs, bufs, alone = [], [], [], [], []
+    grp = ks.Group()
+    members = []
+    for idx, (n, k) in enumerate(specs):
+        rng = np.random.default_rng(n)
+        csr = M.er_laplacian(n, 8.0, 7 + idx)
+        rp, ci, va = csr
+        val = 1e-6 * np.asarray(va, dtype=np.float64)
+        for i in range(n):                      # + I on the diagonal (present in a Laplacian)
+            for q in range(rp[i], rp[i + 1]):
+                if ci[q] == i:
+                    val[q] += 1.0
+        ctx = ks.Context(0)
+        A = ks.CSR(ctx, (rp, ci, val))
+        V = np.zeros((n, k + 1), order="F")
+        V[:, :k] = np.linalg.qr(rng.standard_normal((n, k)))[0]
+        dV = ks.DeviceArray.from_numpy(ctx, V)
+        buf = ks.DeviceArray(ctx, (k + 2, k + 2))
+        # alone: SpMV of column k-1 into column k, then the orthonormalization
+        dVa = ks.DeviceArray.from_numpy(ctx, V)
+        bufa = ks.DeviceArray(ctx, (k + 2, k + 2))
+        ctx.status_reset()
+        check = ks.check
+        check(ks.lib().ks_spmv(ctx.h, A.h, dVa.offset((k - 1) * n), dVa.offset(k * n), ks.F64, 0), "ks_spmv")
+        ks.orthonormalize_column(ctx, ks.F64, n, dVa, n, k, bufa, k + 2, k, k)
+        sta = ctx.status()
+        alone.append((dVa.numpy()[:, k], bufa.numpy()[:k + 1, k], sta.total_passes, sta.nrm))
+        ctx.status_reset()
+        ctxs.append(ctx); mats.append(A); Vs.append(dV); bufs.append(buf)
+        members.append(ks.GroupMember(ctx=ctx.h.value, A=A.h.value, V=dV.ptr.value, ld=n, n=n, k=k,
+                                      buf=buf.ptr.value, ldbuf=k + 2, hcol=k, col=k))
+        grp.wait_ctx(ctx)
+    grp.step(members)
+    for idx, (n, k) in enumerate(specs):
+        grp.ctx_wait(ctxs[idx])
+        st = ctxs[idx].status()
+        col = Vs[idx].numpy()[:, k]
+        h = bufs[idx].numpy()[:k + 1, k]
+        col_a, h_a, passes_a, nrm_a = alone[idx]
+        print(f"member n={n} k={k}: passes {st.total_passes}/{passes_a} nrm {st.nrm:.6e}/{nrm_a:.6e}", flush=True)
+        assert st.total_passes == passes_a == 2 and st.done == 1 and not st.lindep
+        assert abs(st.nrm - nrm_a) <= 1e-9 * nrm_a
+        assert np.allclose(h, h_a, rtol=1e-10, atol=1e-18)
+        assert np.abs(col - col_a).max() < 1e-9 * np.abs(col_a).max()
+        assert abs(np.linalg.norm(col) - 1) < 1e-13
+    grp.close()
+    for c in ctxs:
+        c.close()
