@@ -45,6 +45,7 @@ typedef struct ks_pool_s *ks_pool;
 typedef struct ks_group_s *ks_group;
 typedef struct ks_graph_s *ks_graph;
 typedef struct ks_ilu_s *ks_ilu;
+typedef struct ks_lu_s *ks_lu;
 
 /* Device-side state of the fused CGS+DGKS orthonormalization (the scalars of
    BVOrthogonalizeGS, src/sys/classes/bv/interface/bvorthog.c:145-217, kept on
@@ -177,6 +178,18 @@ int ks_ilu0_levels(ks_ilu M, int *nlower, int *nupper);
 /* ks_bcgs_solve with the ILU(0) preconditioner B = (L U)^-1 */
 int ks_bcgs_solve_ilu(ks_ctx ctx, ks_csr A, double sigma, ks_ilu M, const double *d_b, double *d_x, double rtol,
                       double abstol, int maxits, double *d_work, int *its, double *rnorm, int *reason);
+/* The direct inner solve of shift-and-invert (KSPPREONLY with PCLU, the
+   reference's default: src/sys/classes/st/interface/stsles.c:47-54,
+   st/impls/sinvert/sinvert.c:131-154): LU with partial pivoting of A - sigma I
+   held dense in HBM (n^2 doubles, n <= KS_LU_MAXN), factored on the device by
+   LAPACK dgetrf's right-looking blocked algorithm (64-column panels, pivots as
+   idamax, trailing updates on FP64 MFMA).  ks_lu_solve gives x = (A - sigma I)^-1 b
+   (b and x device vectors of length n; they may be the same).  A zero pivot
+   fails the factorization with KS_ERR_PIVOT. */
+#define KS_LU_MAXN 32768
+int ks_lu_create(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out);
+int ks_lu_solve(ks_ctx ctx, ks_lu F, const double *d_b, double *d_x);
+int ks_lu_destroy(ks_lu F);
 /* nrows, nnz, max row length and the kernel variant chosen (0 stream, 1 vector) */
 int ks_csr_info(ks_csr A, int *nrows, int64_t *nnz, int *max_row, int *variant, int *nblocks);
 /* y = A x.  x, y device vectors in xprec.  When gate != 0 the product is

@@ -275,9 +275,11 @@ typedef struct _p_PC *PC;
 #define STSINVERT "sinvert"
 #define KSPCG     "cg"
 #define KSPBCGS   "bcgs"
+#define KSPPREONLY "preonly"
 #define PCJACOBI  "jacobi"
 #define PCNONE    "none"
 #define PCILU     "ilu"
+#define PCLU      "lu"
 PetscErrorCode STCreate(ST *st);
 PetscErrorCode STSetType(ST st, STType type);
 PetscErrorCode STGetType(ST st, STType *type);
@@ -404,6 +406,7 @@ typedef struct {
   PetscBool want_vectors;        /* keep the eigenvectors on the host */
   PetscReal mixed_tol;           /* > tol: FP32-basis first phase to this tolerance */
   double t_arrival;              /* on the SlepcServerClock clock; <= 0: submission time */
+  PetscBool twosided;            /* two-sided Krylov-Schur (non-Hermitian, FP64 values, no FP32 phase) */
 } SlepcRequest;
 typedef struct {
   PetscInt nconv, nret, ngood, its; /* converged, returned (<= nev), returned with residual <= tol, restarts */

@@ -20,7 +20,7 @@ CC = os.environ.get("CC", "gcc")
 MKLDIR = os.environ.get("MKLDIR", "/opt/conda/lib")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["ks_runtime.hip", "ks_spmv.hip", "ks_bv.hip", "ks_group.hip", "ks_transpose.hip", "ks_ksp.hip", "ks_tsqr.hip"]
+HIP_SOURCES = ["ks_runtime.hip", "ks_spmv.hip", "ks_bv.hip", "ks_group.hip", "ks_transpose.hip", "ks_ksp.hip", "ks_tsqr.hip", "ks_lu.hip"]
 HOST_SOURCES = ["sys.c", "vec.c", "mat.c", "bv.c", "bvhipks.c", "ds.c", "st.c", "eps.c", "krylovschur.c", "serve.c", "matload.c"]
 
 KS_LIB = os.path.join(HERE, "libks_hip.so")

@@ -483,19 +483,6 @@ PetscErrorCode EPSGetLeftStartVector(EPS eps, PetscInt i, PetscBool *breakdown)
   return PETSC_SUCCESS;
 }
 
-/* one Arnoldi expansion k..nv of V with Op into the DS matrix mat (BVMatArnoldi) */
-static PetscErrorCode TwoSidedExpand(EPS eps, BV V, Mat Op, DSMatType mat, PetscInt k, PetscInt *nv, PetscReal *beta,
-                                     PetscBool *breakdown)
-{
-  Mat H;
-  eps->nmatmult += *nv - k;
-  PetscCall(BVMatKrylov(V, Op, k, nv, beta, breakdown));
-  PetscCall(DSGetMat(eps->ds, mat, &H));
-  PetscCall(BVCopyHessenberg(V, H, k, *nv));
-  PetscCall(DSRestoreMat(eps->ds, mat, &H));
-  return PETSC_SUCCESS;
-}
-
 /* EPSTwoSidedRQUpdate1 (ks-twosided.c:25-77): oblique correction of the two residual
    vectors, u -= V M^-1 W^T u and w -= W M^-T V^T w, folded into S and T */
 static PetscErrorCode TwoSidedRQUpdate1(EPS eps, PetscInt nv, PetscReal beta, PetscReal betat)
@@ -585,99 +572,170 @@ static PetscErrorCode TwoSidedRQUpdate2(EPS eps, PetscInt k)
   return PETSC_SUCCESS;
 }
 
-/* EPSSolve_KrylovSchur_TwoSided (ks-twosided.c:138-241) */
-static PetscErrorCode EPSSolve_KrylovSchur_TwoSided(EPS eps)
+/* EPSSolve_KrylovSchur_TwoSided (ks-twosided.c:138-241), as a resumable state
+   machine like the default variant's: TSExpand enqueues the nv-step Arnoldi
+   expansion of V with Op on the device; TSAfterRight collects it and enqueues
+   the expansion of W with A^T up to the nv V reached (the reference expands W
+   after V, to V's nv); TSAfterLeft collects that and does the oblique
+   Rayleigh-quotient updates, the projected problem, the convergence test and
+   the restart, then expands again or finishes.  A serving loop polls the
+   device between the three. */
+static PetscErrorCode TSBegin(EPS eps, BV V, Mat Op, PetscInt k, PetscInt nv, const BVKrylovOps **ext, PetscInt *m,
+                              PetscReal *beta, PetscBool *breakdown)
+{
+  PetscCall(BVGetKrylovOps(V, ext));
+  if (*ext) PetscCall((*ext)->matkrylovbegin(V, Op, k, nv));
+  else {   /* the reference's per-step loop, run here; the next phase collects its results */
+    *m = nv;
+    PetscCall(BVMatKrylov(V, Op, k, m, beta, breakdown));
+  }
+  eps->nmatmult += nv - k;
+  return PETSC_SUCCESS;
+}
+
+static PetscErrorCode TSCollect(BV V, DS ds, DSMatType mat, PetscInt k, const BVKrylovOps *ext, PetscInt m,
+                                PetscReal mbeta, PetscBool mbreakdown, PetscInt *nv, PetscReal *beta,
+                                PetscBool *breakdown)
+{
+  Mat H;
+  if (ext) PetscCall(ext->matkrylovend(V, k, nv, beta, breakdown));
+  else {
+    *nv = m;
+    *beta = mbeta;
+    *breakdown = mbreakdown;
+  }
+  PetscCall(DSGetMat(ds, mat, &H));
+  PetscCall(BVCopyHessenberg(V, H, k, *nv));
+  PetscCall(DSRestoreMat(ds, mat, &H));
+  return PETSC_SUCCESS;
+}
+
+static PetscErrorCode TSExpand(EPS eps)
 {
   EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
-  Mat Op, U, Mm;
-  PetscReal norm, norm2, beta = 0.0, betat = 0.0;
-  PetscInt l, nv, nvt, k = 0, dsn, dsk;
-  PetscBool breakdownt = PETSC_FALSE, breakdown = PETSC_FALSE, breakdownl = PETSC_FALSE;
-  double t0;
+  PetscInt nv, k0 = eps->nconv + ctx->l;
+  eps->its++;
+  nv = PetscMin(eps->nconv + eps->mpd, eps->ncv);
+  PetscCall(DSSetDimensions(eps->ds, nv, eps->nconv, k0));
+  PetscCall(TSBegin(eps, eps->V, ctx->ts_Op, k0, nv, &ctx->kry_ext, &ctx->kry_m, &ctx->kry_beta, &ctx->kry_breakdown));
+  ctx->nv = nv;
+  ctx->phase = KS_PHASE_TS_RIGHT;
+  return PETSC_SUCCESS;
+}
 
+static PetscErrorCode TSAfterRight(EPS eps)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  PetscInt nv = ctx->nv, k0 = eps->nconv + ctx->l;
+  PetscCall(TSCollect(eps->V, eps->ds, DS_MAT_A, k0, ctx->kry_ext, ctx->kry_m, ctx->kry_beta, ctx->kry_breakdown,
+                      &nv, &ctx->ts_beta, &ctx->ts_breakdown));
+  PetscCall(DSSetDimensions(eps->ds, nv, eps->nconv, k0));
+  PetscCall(TSBegin(eps, eps->W, ctx->AT, k0, nv, &ctx->tkry_ext, &ctx->tkry_m, &ctx->tkry_beta,
+                    &ctx->tkry_breakdown));
+  ctx->nv = nv;
+  ctx->phase = KS_PHASE_TS_LEFT;
+  return PETSC_SUCCESS;
+}
+
+static PetscErrorCode TSStart(EPS eps)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
   PetscCall(EPSGetStartVector(eps, 0, NULL));
   PetscCall(EPSGetLeftStartVector(eps, 0, NULL));
-  l = 0;
+  ctx->l = 0;
   memset(ctx->M, 0, sizeof(PetscScalar) * (size_t)eps->ncv * (size_t)eps->ncv);
-  PetscCall(MatCreateSeqDenseView(eps->ncv, eps->ncv, ctx->M, eps->ncv, &Mm));
-  PetscCall(STGetOperator(eps->st, &Op));
-  while (eps->reason == EPS_CONVERGED_ITERATING) {
-    eps->its++;
-    /* nv-step Arnoldi factorizations for Op and Op^T */
-    nv = PetscMin(eps->nconv + eps->mpd, eps->ncv);
-    PetscCall(DSSetDimensions(eps->ds, nv, eps->nconv, eps->nconv + l));
-    PetscCall(TwoSidedExpand(eps, eps->V, Op, DS_MAT_A, eps->nconv + l, &nv, &beta, &breakdown));
-    nvt = nv;
-    PetscCall(DSSetDimensions(eps->ds, nv, eps->nconv, eps->nconv + l));
-    PetscCall(TwoSidedExpand(eps, eps->W, ctx->AT, DS_MAT_B, eps->nconv + l, &nvt, &betat, &breakdownt));
-    nv = PetscMin(nv, nvt);
-    PetscCall(DSSetDimensions(eps->ds, nv, eps->nconv, eps->nconv + l));
-    PetscCall(DSSetState(eps->ds, l == 0 ? DS_STATE_INTERMEDIATE : DS_STATE_RAW));
-    breakdown = (breakdown || breakdownt) ? PETSC_TRUE : PETSC_FALSE;
+  PetscCall(MatCreateSeqDenseView(eps->ncv, eps->ncv, ctx->M, eps->ncv, &ctx->ts_Mm));
+  PetscCall(STGetOperator(eps->st, &ctx->ts_Op));
+  return TSExpand(eps);
+}
 
-    /* update M, modify the Rayleigh quotients S and T */
-    PetscCall(BVSetActiveColumns(eps->V, eps->nconv + l, nv));
-    PetscCall(BVSetActiveColumns(eps->W, eps->nconv + l, nv));
-    PetscCall(BVMatProject(eps->V, NULL, eps->W, Mm));
-    PetscCall(TwoSidedRQUpdate1(eps, nv, beta, betat));
-
-    /* solve the projected problem */
-    t0 = SlepcWallTime();
-    PetscCall(DSSolve(eps->ds, eps->eigr, eps->eigi));
-    PetscCall(DSSort(eps->ds, eps->eigr, eps->eigi, NULL, NULL, NULL));
-    PetscCall(DSUpdateExtraRow(eps->ds));
-
-    /* check convergence */
-    PetscCall(BVNormColumn(eps->V, nv, NORM_2, &norm));
-    PetscCall(BVNormColumn(eps->W, nv, NORM_2, &norm2));
-    PetscCall(EPSKrylovConvergence(eps, PETSC_FALSE, eps->nconv, nv - eps->nconv, beta * norm, betat * norm2, 1.0, &k));
-    eps->t_ds += SlepcWallTime() - t0;
-    if (k >= eps->nev) eps->reason = EPS_CONVERGED_TOL;
-    else if (eps->its >= eps->max_it) eps->reason = EPS_DIVERGED_ITS;
-
-    /* update l */
-    if (eps->reason != EPS_CONVERGED_ITERATING || breakdown || k == nv) l = 0;
-    else {
-      l = PetscMax(1, (PetscInt)((nv - k) * ctx->keep));
-      PetscCall(DSGetTruncateSize(eps->ds, k, nv, &l));
-    }
-    if (!ctx->lock && l > 0) { l += k; k = 0; }
-
-    /* V(:,idx) = V*Q(:,idx), W(:,idx) = W*Z(:,idx) */
-    PetscCall(BVSetActiveColumns(eps->V, eps->nconv, nv));
-    PetscCall(BVSetActiveColumns(eps->W, eps->nconv, nv));
-    PetscCall(DSGetMat(eps->ds, DS_MAT_Q, &U));
-    PetscCall(BVMultInPlace(eps->V, U, eps->nconv, k + l));
-    PetscCall(DSRestoreMat(eps->ds, DS_MAT_Q, &U));
-    PetscCall(DSGetMat(eps->ds, DS_MAT_Z, &U));
-    PetscCall(BVMultInPlace(eps->W, U, eps->nconv, k + l));
-    PetscCall(DSRestoreMat(eps->ds, DS_MAT_Z, &U));
-    if (eps->reason == EPS_CONVERGED_ITERATING && !breakdown) {
-      PetscCall(BVCopyColumn(eps->V, nv, k + l));
-      PetscCall(BVCopyColumn(eps->W, nv, k + l));
-    }
-    if (eps->reason == EPS_CONVERGED_ITERATING) {
-      if (breakdown || k == nv) {
-        if (k < eps->nev) {
-          PetscCall(EPSGetStartVector(eps, k, &breakdown));
-          PetscCall(EPSGetLeftStartVector(eps, k, &breakdownl));
-          if (breakdown || breakdownl) eps->reason = EPS_DIVERGED_BREAKDOWN;
-        }
-      } else {
-        PetscCall(DSGetDimensions(eps->ds, &dsn, NULL, &dsk, NULL));
-        PetscCall(DSSetDimensions(eps->ds, dsn, k, dsk));
-        PetscCall(DSTruncate(eps->ds, k + l, PETSC_FALSE));
-      }
-      PetscCall(TwoSidedRQUpdate2(eps, k + l));
-    }
-    eps->nconv = k;
-    if (eps->monitor) PetscCall(eps->monitor(eps, eps->its, k, eps->eigr, eps->eigi, eps->errest, nv, eps->monitorctx));
-  }
-  PetscCall(STRestoreOperator(eps->st, &Op));
-  PetscCall(MatDestroy(&Mm));
+static PetscErrorCode TSFinish(EPS eps)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  PetscCall(STRestoreOperator(eps->st, &ctx->ts_Op));
+  ctx->ts_Op = NULL;
+  PetscCall(MatDestroy(&ctx->ts_Mm));
   PetscCall(DSTruncate(eps->ds, eps->nconv, PETSC_TRUE));
   ctx->phase = KS_PHASE_DONE;
   return PETSC_SUCCESS;
+}
+
+static PetscErrorCode TSAfterLeft(EPS eps)
+{
+  EPS_KRYLOVSCHUR *ctx = (EPS_KRYLOVSCHUR *)eps->data;
+  Mat U;
+  PetscReal norm, norm2, beta = ctx->ts_beta, betat = 0.0;
+  PetscInt l = ctx->l, nv = ctx->nv, nvt = ctx->nv, k = 0, dsn, dsk, k0 = eps->nconv + ctx->l;
+  PetscBool breakdownt = PETSC_FALSE, breakdown = ctx->ts_breakdown, breakdownl = PETSC_FALSE;
+  double t0;
+
+  PetscCall(TSCollect(eps->W, eps->ds, DS_MAT_B, k0, ctx->tkry_ext, ctx->tkry_m, ctx->tkry_beta,
+                      ctx->tkry_breakdown, &nvt, &betat, &breakdownt));
+  nv = PetscMin(nv, nvt);
+  PetscCall(DSSetDimensions(eps->ds, nv, eps->nconv, k0));
+  PetscCall(DSSetState(eps->ds, l == 0 ? DS_STATE_INTERMEDIATE : DS_STATE_RAW));
+  breakdown = (breakdown || breakdownt) ? PETSC_TRUE : PETSC_FALSE;
+
+  /* update M, modify the Rayleigh quotients S and T */
+  PetscCall(BVSetActiveColumns(eps->V, k0, nv));
+  PetscCall(BVSetActiveColumns(eps->W, k0, nv));
+  PetscCall(BVMatProject(eps->V, NULL, eps->W, ctx->ts_Mm));
+  PetscCall(TwoSidedRQUpdate1(eps, nv, beta, betat));
+
+  /* solve the projected problem */
+  t0 = SlepcWallTime();
+  PetscCall(DSSolve(eps->ds, eps->eigr, eps->eigi));
+  PetscCall(DSSort(eps->ds, eps->eigr, eps->eigi, NULL, NULL, NULL));
+  PetscCall(DSUpdateExtraRow(eps->ds));
+
+  /* check convergence */
+  PetscCall(BVNormColumn(eps->V, nv, NORM_2, &norm));
+  PetscCall(BVNormColumn(eps->W, nv, NORM_2, &norm2));
+  PetscCall(EPSKrylovConvergence(eps, PETSC_FALSE, eps->nconv, nv - eps->nconv, beta * norm, betat * norm2, 1.0, &k));
+  eps->t_ds += SlepcWallTime() - t0;
+  if (k >= eps->nev) eps->reason = EPS_CONVERGED_TOL;
+  else if (eps->its >= eps->max_it) eps->reason = EPS_DIVERGED_ITS;
+
+  /* update l */
+  if (eps->reason != EPS_CONVERGED_ITERATING || breakdown || k == nv) l = 0;
+  else {
+    l = PetscMax(1, (PetscInt)((nv - k) * ctx->keep));
+    PetscCall(DSGetTruncateSize(eps->ds, k, nv, &l));
+  }
+  if (!ctx->lock && l > 0) { l += k; k = 0; }
+
+  /* V(:,idx) = V*Q(:,idx), W(:,idx) = W*Z(:,idx) */
+  PetscCall(BVSetActiveColumns(eps->V, eps->nconv, nv));
+  PetscCall(BVSetActiveColumns(eps->W, eps->nconv, nv));
+  PetscCall(DSGetMat(eps->ds, DS_MAT_Q, &U));
+  PetscCall(BVMultInPlace(eps->V, U, eps->nconv, k + l));
+  PetscCall(DSRestoreMat(eps->ds, DS_MAT_Q, &U));
+  PetscCall(DSGetMat(eps->ds, DS_MAT_Z, &U));
+  PetscCall(BVMultInPlace(eps->W, U, eps->nconv, k + l));
+  PetscCall(DSRestoreMat(eps->ds, DS_MAT_Z, &U));
+  if (eps->reason == EPS_CONVERGED_ITERATING && !breakdown) {
+    PetscCall(BVCopyColumn(eps->V, nv, k + l));
+    PetscCall(BVCopyColumn(eps->W, nv, k + l));
+  }
+  if (eps->reason == EPS_CONVERGED_ITERATING) {
+    if (breakdown || k == nv) {
+      if (k < eps->nev) {
+        PetscCall(EPSGetStartVector(eps, k, &breakdown));
+        PetscCall(EPSGetLeftStartVector(eps, k, &breakdownl));
+        if (breakdown || breakdownl) eps->reason = EPS_DIVERGED_BREAKDOWN;
+      }
+    } else {
+      PetscCall(DSGetDimensions(eps->ds, &dsn, NULL, &dsk, NULL));
+      PetscCall(DSSetDimensions(eps->ds, dsn, k, dsk));
+      PetscCall(DSTruncate(eps->ds, k + l, PETSC_FALSE));
+    }
+    PetscCall(TwoSidedRQUpdate2(eps, k + l));
+  }
+  eps->nconv = k;
+  ctx->l = l;
+  if (eps->monitor) PetscCall(eps->monitor(eps, eps->its, k, eps->eigr, eps->eigi, eps->errest, nv, eps->monitorctx));
+  if (eps->reason != EPS_CONVERGED_ITERATING) return TSFinish(eps);
+  return TSExpand(eps);
 }
 
 static PetscErrorCode EPSSolveBegin_KrylovSchur(EPS eps)
@@ -686,7 +744,7 @@ static PetscErrorCode EPSSolveBegin_KrylovSchur(EPS eps)
   ctx->its32 = ctx->nconv32 = 0;
   ctx->nmatmult32 = ctx->passes32 = 0;
   ctx->t32 = 0.0;
-  if (eps->twosided) return EPSSolve_KrylovSchur_TwoSided(eps);   /* runs to the end */
+  if (eps->twosided) return TSStart(eps);
   if (ctx->mixed_tol > 0.0 && ctx->mixed_tol > eps->tol && BVPrecision(eps->V) == SLEPC_AMD_F64) return MixedBegin(eps);
   return KSStart(eps);
 }
@@ -703,6 +761,8 @@ static PetscErrorCode EPSSolveStep_KrylovSchur(EPS eps, PetscBool *done)
       PetscCall(KSStart(eps));
     }
   } else if (ctx->phase == KS_PHASE_EXPANDING) PetscCall(KSAfterExpand(eps));
+  else if (ctx->phase == KS_PHASE_TS_RIGHT) PetscCall(TSAfterRight(eps));
+  else if (ctx->phase == KS_PHASE_TS_LEFT) PetscCall(TSAfterLeft(eps));
   *done = ctx->phase == KS_PHASE_DONE;
   return PETSC_SUCCESS;
 }

@@ -74,6 +74,10 @@ static size_t RequestBytes(const SlepcRequest *r)
     b += ROUND(4 * ld * m) + ROUND(8 * m * m) + ROUND(8 * (m + 2)) + ROUND(8 * m * m) + 7 * ROUND(4 * n);
     if (!r->val32) b += ROUND(4 * (n + 1)) + ROUND(4 * (nnz ? nnz : 1)) * 2 + ROUND(4 * (n + 2));
   }
+  if (r->twosided) {   /* A^T (CSR with its row-block schedule), the left basis, buffers and vectors, M */
+    b += ROUND(4 * (n + 1)) + ROUND(4 * (nnz ? nnz : 1)) + ROUND(8 * (nnz ? nnz : 1)) + ROUND(4 * (n + 2));
+    b += ROUND(8 * ld * m) + ROUND(8 * m * m) + ROUND(8 * (m + 2)) + ROUND(8 * m * m) + 7 * ROUND(8 * n);
+  }
 #undef ROUND
   return b + 64 * 1024;
 }
@@ -109,6 +113,8 @@ PetscErrorCode SlepcServerSubmit(SlepcServer s, const SlepcRequest *r, PetscInt 
   PetscCheck(r->n > 0 && r->rowptr && r->colidx && (r->val || r->val32), PETSC_ERR_ARG_NULL, "request without a matrix");
   PetscCheck(r->nev >= 1 && r->ncv > r->nev && r->ncv <= r->n && r->ncv + 1 <= KS_MAX_COLS, PETSC_ERR_ARG_OUTOFRANGE,
              "request nev=%d ncv=%d n=%d not supported", (int)r->nev, (int)r->ncv, (int)r->n);
+  PetscCheck(!r->twosided || (r->val && !r->val32 && !(r->mixed_tol > r->tol)), PETSC_ERR_SUP,
+             "a two-sided request needs FP64 values and no FP32 phase");
   if (s->nreq == s->capreq) {
     PetscInt cap = s->capreq ? 2 * s->capreq : 64;
     Request *nr = (Request *)realloc(s->req, sizeof(Request) * (size_t)cap);
@@ -155,7 +161,9 @@ static PetscErrorCode RequestAdmit(SlepcServer s, PetscInt id, int slot)
   if (q->in.mixed_tol > 0.0) PetscCall(EPSKrylovSchurSetMixedPrecision(q->eps, q->in.mixed_tol));
   PetscCall(EPSGetBV(q->eps, &V));
   PetscCall(BVSetType(V, BVHIPKS));
-  PetscCall(EPSKrylovSchurSetGrouped_Private(q->eps, s->grouped));
+  /* the two-sided variant expands two bases per restart; it runs on its own stream */
+  if (q->in.twosided) PetscCall(EPSSetTwoSided(q->eps, PETSC_TRUE));
+  PetscCall(EPSKrylovSchurSetGrouped_Private(q->eps, s->grouped && !q->in.twosided));
   if (q->in.v0) {
     PetscCall(VecCreateHIP(q->in.n, SLEPC_AMD_F64, &v0));
     PetscCall(VecSetValuesHost(v0, q->in.v0));

@@ -356,14 +356,16 @@ PetscErrorCode DSCreate_NHEPTS(DS ds);
 /* ---------------------------------------------------------------- ST */
 struct _p_PC {
   char type[SLEPC_NAME_LEN];
+  PetscBool set;             /* PCSetType was called (else KSPSetType picks the PC that goes with it) */
 };
-struct _p_KSP {              /* the inner solver of STSINVERT (device CG or BiCGStab) */
+struct _p_KSP {              /* the inner solver of STSINVERT (device LU, CG or BiCGStab) */
   char type[SLEPC_NAME_LEN];
   struct _p_PC pc;
   PetscReal rtol, abstol;
   PetscInt maxits, n;
   double *d_dinv, *d_work;   /* Jacobi diagonal, the solver's vectors + its state */
   ks_ilu ilu;                /* PCILU: the ILU(0) factors of A - sigma I */
+  ks_lu lu;                  /* KSPPREONLY + PCLU: the LU factors of A - sigma I */
   size_t nwork;              /* doubles in d_work */
   int64_t its_total, nsolves;
 };
@@ -423,7 +425,8 @@ struct _p_EPS {
   BV stepV;                  /* the basis whose device work the next step collects */
 };
 
-typedef enum { KS_PHASE_IDLE = 0, KS_PHASE_EXPANDING = 1, KS_PHASE_DONE = 2, KS_PHASE_MIXED32 = 3 } KSPhase;
+typedef enum { KS_PHASE_IDLE = 0, KS_PHASE_EXPANDING = 1, KS_PHASE_DONE = 2, KS_PHASE_MIXED32 = 3,
+               KS_PHASE_TS_RIGHT = 4, KS_PHASE_TS_LEFT = 5 } KSPhase;
 
 typedef struct {            /* EPS_KRYLOVSCHUR (krylovschur.h:76-105), default variant fields */
   PetscReal keep;
@@ -441,9 +444,18 @@ typedef struct {            /* EPS_KRYLOVSCHUR (krylovschur.h:76-105), default v
   /* harmonic extraction: the translation vector g (ncv + 1 entries) */
   PetscBool harmonic;
   PetscScalar *g;
-  /* two-sided variant: A^T and the oblique projection M = W^T V (ncv x ncv) */
+  /* two-sided variant: A^T and the oblique projection M = W^T V (ncv x ncv);
+     while its restart loop runs: the operator, a view of M, the right
+     expansion's results, and the left expansion in flight (tkry_*) */
   Mat AT;
   PetscScalar *M;
+  Mat ts_Op, ts_Mm;
+  PetscReal ts_beta;
+  PetscBool ts_breakdown;
+  const BVKrylovOps *tkry_ext;
+  PetscInt tkry_m;
+  PetscReal tkry_beta;
+  PetscBool tkry_breakdown;
   /* the expansion in flight: through the basis' Krylov extension, or (NULL)
      already run by the per-step loop, with its results here */
   const BVKrylovOps *kry_ext;

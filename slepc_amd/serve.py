@@ -22,7 +22,7 @@ class Request(C.Structure):
                 ("val32", C.c_void_p),
                 ("nev", C.c_int), ("ncv", C.c_int), ("max_it", C.c_int), ("tol", C.c_double),
                 ("which", C.c_int), ("hermitian", C.c_int), ("v0", C.c_void_p), ("want_vectors", C.c_int),
-                ("mixed_tol", C.c_double), ("t_arrival", C.c_double)]
+                ("mixed_tol", C.c_double), ("t_arrival", C.c_double), ("twosided", C.c_int)]
 
 
 class Result(C.Structure):
@@ -53,9 +53,9 @@ class Server:
 
     def submit(self, csr, nev: int, ncv: int, tol: float = 1e-10, which: int = 1, hermitian: bool = True,
                v0=None, want_vectors: bool = False, t_arrival: float = 0.0, max_it: int = 0,
-               mixed_tol: float = 0.0) -> int:
+               mixed_tol: float = 0.0, twosided: bool = False) -> int:
         """FP32 values in csr[2] keep the operator in FP32; mixed_tol > tol adds the
-        FP32-basis first phase."""
+        FP32-basis first phase; twosided runs two-sided Krylov-Schur (FP64 values)."""
         rowptr = np.ascontiguousarray(csr[0], dtype=np.int32)
         colidx = np.ascontiguousarray(csr[1], dtype=np.int32)
         f32 = np.asarray(csr[2]).dtype == np.float32
@@ -66,7 +66,7 @@ class Server:
                     val=None if f32 else val.ctypes.data, val32=val.ctypes.data if f32 else None, nev=nev,
                     ncv=ncv, max_it=max_it, tol=tol, which=which, hermitian=int(hermitian),
                     v0=None if v0a is None else v0a.ctypes.data, want_vectors=int(want_vectors),
-                    mixed_tol=mixed_tol, t_arrival=t_arrival)
+                    mixed_tol=mixed_tol, t_arrival=t_arrival, twosided=int(twosided))
         rid = C.c_int()
         _eps._chk(_eps.lib().SlepcServerSubmit(self.h, C.byref(r), C.byref(rid)), "SlepcServerSubmit")
         self._keep[rid.value] = (rowptr, colidx, val, v0a)   # the C side reads them until the solve ends
@@ -151,7 +151,8 @@ def run_stream(server: Server, requests, make_csr, tol: float = 1e-10, want_vect
             csr, v0 = mats[nxt]
             ids[nxt] = server.submit(csr, r["nev"], r["ncv"], tol=tol, v0=v0, want_vectors=want_vectors,
                                      t_arrival=t0 + time_scale * r["t_arrival"], which=r.get("which", 1),
-                                     hermitian=r.get("hermitian", True), mixed_tol=r.get("mixed_tol", 0.0))
+                                     hermitian=r.get("hermitian", True), mixed_tol=r.get("mixed_tol", 0.0),
+                                     twosided=r.get("twosided", False))
             pending.add(nxt)
             mats[nxt] = None
             nxt += 1
@@ -194,7 +195,8 @@ def run_stream_native(server: Server, requests, make_csr, tol: float = 1e-10, wa
                           val=None if f32 else val.ctypes.data, val32=val.ctypes.data if f32 else None,
                           nev=r["nev"], ncv=r["ncv"], max_it=0, tol=tol, which=r.get("which", 1),
                           hermitian=int(r.get("hermitian", True)), v0=v0.ctypes.data, want_vectors=int(want_vectors),
-                          mixed_tol=r.get("mixed_tol", 0.0), t_arrival=time_scale * r["t_arrival"])
+                          mixed_tol=r.get("mixed_tol", 0.0), t_arrival=time_scale * r["t_arrival"],
+                          twosided=int(r.get("twosided", False)))
         o = dict(eigr=np.zeros(r["nev"]), eigi=np.zeros(r["nev"]), err=np.zeros(r["nev"]))
         if want_vectors:
             o["X"] = np.zeros((r["nev"], r["n"]))

@@ -141,3 +141,31 @@ def test_serving_mixed_fp32_operator_alongside_nonsymmetric(E):
         ev = np.linalg.eigvalsh(A) if r["hermitian"] else np.linalg.eigvals(A)
         ev = ev[np.argsort(-ev.real)][:k]
         assert np.allclose(np.sort_complex(b), np.sort_complex(ev), rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("driver", ["python", "native"])
+def test_serving_twosided_requests(E, driver):
+    # two-sided Krylov-Schur requests (the stepped variant: right expansion, left
+    # expansion, restart, each a step of the loop) served beside one-sided ones;
+    # each result the same as the request solved alone
+    from slepc_amd import serve as S
+    mats = [M.convection_diffusion_2d(N, 10.0, seed=3 + i) for i, N in enumerate((40, 56))] + [M.brusselator(50)]
+    reqs = []
+    for i, c in enumerate(mats):
+        for ts in (True, False):
+            reqs.append(dict(n=len(c[0]) - 1, nev=4, ncv=24, t_arrival=0.001 * len(reqs), v0_seed=20 + i,
+                             which=E.EPS.LARGEST_REAL, hermitian=False, twosided=ts, _csr=c))
+    srv = S.Server(budget=1 << 30, max_active=4)
+    run = S.run_stream if driver == "python" else S.run_stream_native
+    res, wall = run(srv, reqs, lambda r: r["_csr"], tol=1e-10)
+    srv.close()
+    for r, out in zip(reqs, res):
+        print(f"n={r['n']} twosided={r['twosided']} its={out['its']} ngood={out['ngood']}", flush=True)
+        assert out["error"] == 0 and out["reason"] == 1 and out["ngood"] == r["nev"], (r["n"], out)
+        alone = E.solve(r["_csr"], nev=r["nev"], ncv=r["ncv"], tol=1e-10, which=r["which"], hermitian=False,
+                        v0=M.uniform_vector(r["v0_seed"], r["n"]), twosided=r["twosided"], want_vectors=False)
+        k = r["nev"]
+        a = np.asarray(alone["eigr"][:k]) + 1j * np.asarray(alone["eigi"][:k])
+        b = np.asarray(out["eigr"]) + 1j * np.asarray(out["eigi"])
+        assert np.allclose(np.sort_complex(a), np.sort_complex(b), rtol=1e-10, atol=0)
+        assert out["its"] == alone["its"]
