@@ -1149,8 +1149,10 @@ typedef struct {
   int harmonic, dsherm;           /* dsherm: Lanczos + DSHEP (Hermitian and Ritz extraction) */
   int twosided;
   int sinvert, pc_jacobi, ksp_maxits;   /* shift-and-invert: (A - sigma I)^-1 by PCG or BiCGStab */
-  int ksp_bcgs, pc_ilu;
+  int ksp_bcgs, pc_ilu, ksp_lu;
   double sigma, ksp_rtol, *dinv, *kwork;
+  double *dlu;                    /* KSPPREONLY + PCLU: the dense LU factors of A - sigma I (n x n, by columns) */
+  int *ipiv;                      /* KSPPREONLY + PCLU: row k swapped with row ipiv[k] at step k */
   int *lu_diag;                   /* PCILU: index of the diagonal entry of each row */
   double *lu;                     /* PCILU: the ILU(0) factors of A - sigma I on A's pattern */
   long ksp_its;
@@ -1250,6 +1252,83 @@ int or_ilu0(int n, const int *rowptr, const int *colidx, const double *val, doub
   return e;
 }
 
+/* The direct inner solve KSPPREONLY + PCLU, the reference's default for
+   shift-and-invert (STSetDefaultKSP_Default, stsles.c:47-54): one application of
+   the LU factors of A - sigma I.  PETSc (not part of the reference tree) factors
+   the sparse matrix with a fill-reducing ordering; this restates the dense LU with
+   partial pivoting of LAPACK's dgetf2 (right-looking, the pivot of column k is the
+   first entry of largest magnitude on or below the diagonal, as idamax picks it;
+   the column below the pivot is scaled by the pivot's reciprocal) and the solve of
+   dgetrs (the row interchanges in order, then unit lower and upper triangular
+   substitution).  In exact arithmetic both give (A - sigma I)^-1 b.  A pivot of
+   magnitude below 100 machine epsilons fails as a zero pivot, the test and default
+   tolerance of PETSc's factorizations (MatFactorInfo zeropivot). */
+static int lu_factor(oeps *eps)
+{
+  const int n = eps->n;
+  double *a = eps->dlu;
+  memset(a, 0, sizeof(double) * (size_t)n * n);
+  for (int i = 0; i < n; i++)
+    for (int q = eps->rowptr[i]; q < eps->rowptr[i + 1]; q++) a[i + (size_t)eps->colidx[q] * n] += eps->val[q];
+  for (int i = 0; i < n; i++) a[i + (size_t)i * n] -= eps->sigma;
+  for (int k = 0; k < n; k++) {
+    double *ck = a + (size_t)k * n;
+    int p = k;
+    double m = fabs(ck[k]);
+    for (int i = k + 1; i < n; i++)
+      if (fabs(ck[i]) > m) { m = fabs(ck[i]); p = i; }
+    eps->ipiv[k] = p;
+    if (!(m >= 100.0 * MACH_EPS)) OR_ERR(-92, "LU: zero pivot in column %d (|pivot| %g)", k, m);
+    if (p != k)
+      for (int j = 0; j < n; j++) {
+        double t = a[k + (size_t)j * n];
+        a[k + (size_t)j * n] = a[p + (size_t)j * n];
+        a[p + (size_t)j * n] = t;
+      }
+    const double r = 1.0 / ck[k];
+    for (int i = k + 1; i < n; i++) ck[i] *= r;
+#pragma omp parallel for schedule(static) if (n - k > 256)
+    for (int j = k + 1; j < n; j++) {               /* rank-1 update of the trailing matrix */
+      double *cj = a + (size_t)j * n;
+      const double akj = cj[k];
+      if (akj != 0.0)
+        for (int i = k + 1; i < n; i++) cj[i] -= ck[i] * akj;
+    }
+  }
+  return 0;
+}
+
+static void lu_solve(const oeps *eps, const double *b, double *x)
+{
+  const int n = eps->n;
+  const double *a = eps->dlu;
+  memcpy(x, b, sizeof(double) * (size_t)n);
+  for (int k = 0; k < n; k++)
+    if (eps->ipiv[k] != k) { double t = x[k]; x[k] = x[eps->ipiv[k]]; x[eps->ipiv[k]] = t; }
+  for (int j = 0; j < n; j++) {                     /* L y = P b, by columns */
+    const double xj = x[j];
+    if (xj != 0.0)
+      for (int i = j + 1; i < n; i++) x[i] -= a[i + (size_t)j * n] * xj;
+  }
+  for (int j = n - 1; j >= 0; j--) {                /* U x = y, by columns */
+    x[j] /= a[j + (size_t)j * n];
+    const double xj = x[j];
+    if (xj != 0.0)
+      for (int i = 0; i < j; i++) x[i] -= a[i + (size_t)j * n] * xj;
+  }
+}
+
+/* The dense factors alone, for the tests: lu (n x n, by columns) gets L below the
+   diagonal and U on and above it, ipiv the interchanges (0-based) */
+int or_lu(int n, const int *rowptr, const int *colidx, const double *val, double sigma, double *lu, int *ipiv)
+{
+  oeps eps;
+  memset(&eps, 0, sizeof(eps));
+  eps.n = n; eps.rowptr = rowptr; eps.colidx = colidx; eps.val = val; eps.sigma = sigma;
+  eps.dlu = lu; eps.ipiv = ipiv;
+  return lu_factor(&eps);
+}
+
 /* y = (L U)^-1 y in place: forward substitution with the unit lower factor, then
    backward substitution with U */
 static void ilu0_apply(const oeps *eps, double *y)
@@ -1338,7 +1417,10 @@ static int bcgs_solve(oeps *eps, const double *b, double *x)
 
 static void op_mult(oeps *eps, const double *x, double *y)
 {
-  if (eps->sinvert) {
+  if (eps->sinvert && eps->ksp_lu) {
+    lu_solve(eps, x, y);
+    eps->ksp_its++;
+  } else if (eps->sinvert) {
     if ((eps->ksp_bcgs ? bcgs_solve(eps, x, y) : pcg_solve(eps, x, y))) eps->ksp_failed = 1;
   } else or_spmv(eps->n, eps->rowptr, eps->colidx, eps->val, x, y);
   eps->nmatmult++;
@@ -1773,7 +1855,9 @@ int or_eps_solve_twosided(int n, const int *rowptr, const int *colidx, const dou
     eps.pc_jacobi = !opts->pc_none;
     eps.ksp_bcgs = opts->ksp_bcgs != 0;
     eps.pc_ilu = opts->pc_ilu != 0;
+    eps.ksp_lu = opts->ksp_lu != 0;
     if (eps.pc_ilu && !eps.ksp_bcgs) { e = -3; goto done; }   /* ILU(0) with BiCGStab only */
+    if (eps.ksp_lu && (eps.pc_ilu || eps.ksp_bcgs)) { e = -3; goto done; }
   }
 
   /* EPSSetUp (epssetup.c:344-345) and EPSSetDimensions_Default (epssetup.c:648-671) */
@@ -1823,6 +1907,12 @@ int or_eps_solve_twosided(int n, const int *rowptr, const int *colidx, const dou
       eps.lu_diag = (int*)malloc(sizeof(int) * (size_t)n);
       if (!eps.lu || !eps.lu_diag) { e = -55; goto done; }
       if ((e = ilu0_factor(&eps))) goto done;
+    }
+    if (eps.ksp_lu) {
+      eps.dlu = (double*)malloc(sizeof(double) * (size_t)n * n);
+      eps.ipiv = (int*)malloc(sizeof(int) * (size_t)n);
+      if (!eps.dlu || !eps.ipiv) { e = -55; goto done; }
+      if ((e = lu_factor(&eps))) goto done;
     }
     for (int i = 0; i < n; i++) {                                  /* PCJACOBI on A - sigma I */
       double d = 0.0;
@@ -1943,7 +2033,7 @@ int or_eps_solve_twosided(int n, const int *rowptr, const int *colidx, const dou
     info->nmatmult = eps.nmatmult; info->gs_passes = eps.V.passes; info->t_ds = eps.t_ds;
   }
 done:
-  free(eps.dinv); free(eps.kwork); free(eps.lu); free(eps.lu_diag);
+  free(eps.dinv); free(eps.kwork); free(eps.lu); free(eps.lu_diag); free(eps.dlu); free(eps.ipiv);
   if (eps.twosided) { free(eps.W.V); free(eps.W.buffer); free(eps.M); free(eps.trowptr); free(eps.tcolidx); free(eps.tval); }
   free(eps.V.V); free(eps.V.buffer); free(eps.eigr); free(eps.eigi); free(eps.errest); free(eps.perm);
   if (eps.ds.A) ds_free(&eps.ds);

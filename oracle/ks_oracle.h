@@ -42,11 +42,12 @@ typedef struct {
   int    restart_cap;            /* > 0: stop after this many restarts (bounded timing sample) */
   int    extraction;             /* 0 EPS_RITZ, 1 EPS_HARMONIC (translation by target) */
   int    twosided;               /* 1: two-sided variant (left eigenvectors too) */
-  int    st_sinvert;             /* 1: STSINVERT with shift = target, inner PCG */
+  int    st_sinvert;             /* 1: STSINVERT with shift = target (inner PCG unless set below) */
   int    pc_none;                /* 1: PCG without the Jacobi preconditioner */
   double ksp_rtol;               /* <= 0 means 1e-8 */
   int    ksp_bcgs;               /* 1: BiCGStab instead of CG for the inner solve */
   int    pc_ilu;                 /* 1: ILU(0) preconditioner (BiCGStab only) */
+  int    ksp_lu;                 /* 1: KSPPREONLY + PCLU, the dense LU direct solve */
 } or_opts;
 
 typedef struct {
@@ -86,6 +87,7 @@ int or_bv_orthogonalize(int n, int ld, double *V, int l, int k, int block, doubl
 int or_bv_tsqr_split(int n, int ld, double *V, int k, int P, double *R, int ldr);
 /* ILU(0) factors of A - sigma I on A's pattern (L below the diagonal, unit diagonal implied; U on and above) */
 int or_ilu0(int n, const int *rowptr, const int *colidx, const double *val, double sigma, double *lu);
+int or_lu(int n, const int *rowptr, const int *colidx, const double *val, double sigma, double *lu, int *ipiv);
 int or_multinplace(int n, int ld, double *V, int nc, int l, int k,
                    const double *Q, int ldq, int s, int e);
 

@@ -28,7 +28,7 @@ class Opts(C.Structure):
                 ("eta", C.c_double), ("refine", C.c_int), ("nthreads", C.c_int),
                 ("restart_cap", C.c_int), ("extraction", C.c_int), ("twosided", C.c_int),
                 ("st_sinvert", C.c_int), ("pc_none", C.c_int), ("ksp_rtol", C.c_double), ("ksp_bcgs", C.c_int),
-                ("pc_ilu", C.c_int)]
+                ("pc_ilu", C.c_int), ("ksp_lu", C.c_int)]
 
 
 class Info(C.Structure):
@@ -161,6 +161,23 @@ def ilu0(csr, sigma: float = 0.0) -> np.ndarray:
     return lu
 
 
+def lu(csr, sigma: float = 0.0):
+    """Dense LU with partial pivoting of A - sigma I: (LU by columns, L strictly
+    below the diagonal and U on and above; ipiv, 0-based interchanges)."""
+    rowptr = np.ascontiguousarray(csr[0], dtype=np.int32)
+    colidx = np.ascontiguousarray(csr[1], dtype=np.int32)
+    val = np.ascontiguousarray(csr[2], dtype=np.float64)
+    n = len(rowptr) - 1
+    a = np.zeros((n, n), order="F")
+    ipiv = np.zeros(n, dtype=np.int32)
+    ip = C.POINTER(C.c_int)
+    rc = lib().or_lu(C.c_int(n), rowptr.ctypes.data_as(ip), colidx.ctypes.data_as(ip), _p(val), C.c_double(sigma),
+                     _p(a), ipiv.ctypes.data_as(ip))
+    if rc:
+        raise RuntimeError(f"or_lu failed ({rc})")
+    return a, ipiv
+
+
 def bv_dot(X: np.ndarray, Y: np.ndarray, M: np.ndarray, lx: int = 0, ly: int = 0):
     """M(ly:, lx:) = Y(:, ly:)^T X(:, lx:) in place (BVDot with leading columns)."""
     assert X.flags.f_contiguous and Y.flags.f_contiguous and M.flags.f_contiguous
@@ -204,7 +221,7 @@ def eps_solve(csr, nev: int, ncv: int = 0, mpd: int = 0, tol: float = 1e-8,
              hermitian=int(hermitian), keep=keep, lock=int(lock), eta=eta, refine=refine,
              nthreads=nthreads, restart_cap=restart_cap, extraction=extraction, twosided=int(twosided),
              st_sinvert=int(sinvert), pc_none=int(pc_none), ksp_rtol=ksp_rtol, ksp_bcgs=int(ksp == "bcgs"),
-             pc_ilu=int(pc == "ilu"))
+             pc_ilu=int(pc == "ilu"), ksp_lu=int(ksp == "preonly"))
     ncv_eff = ncv if ncv > 0 else max(2 * nev, nev + 15)
     ncv_eff = min(ncv_eff, n) if ncv_eff > 0 else n
     size = max(ncv_eff, nev) + 2

@@ -7,16 +7,25 @@
 // rounded up to 64, the padding an identity block, so every tile is whole) and
 // factored by the right-looking blocked algorithm of LAPACK's dgetrf with
 // panels of 64 columns:
-//   * panel: per column, one workgroup finds the pivot (largest magnitude,
-//     lowest row on ties, as idamax) and swaps it in within the panel; a
-//     row-parallel launch scales the column by the pivot's reciprocal and
-//     updates the panel's remaining columns (dgetf2's dscal and dger);
-//   * the panel's row interchanges applied to the columns outside it (dlaswp);
+//   * panel: one launch per column.  The pivot rows are found and used in
+//     place (implicit pivoting): every workgroup reduces the partial maxima of
+//     column j the previous launch left (largest magnitude, lowest row on
+//     ties), scales its rows of column j by the pivot's reciprocal, updates
+//     their panel columns right of j (dgetf2's dscal and dger) and leaves its
+//     partial maximum of column j+1.  Rows already chosen as pivots are
+//     skipped.  At the panel's end one thread turns the pivot rows into
+//     LAPACK's sequence of interchanges, which are then applied to every
+//     column (dlaswp) -- the same rows and values dgetf2 arrives at by
+//     swapping as it goes;
 //   * U12 = L11^-1 A12, one wavefront per column with L11 in LDS (dtrsm);
-//   * A22 -= L21 U12 on v_mfma_f64_16x16x4f64 in 64 x 64 tiles (dgemm).
+//   * A22 -= L21 U12 on v_mfma_f64_16x16x4f64 in 64 x 64 tiles (dgemm), the
+//     operands ordered so that 16 lanes hold 16 consecutive rows of a column
+//     of A22 (128-byte segments for its read and write).
 // The solve applies the row permutation as a gather, then the unit-lower and
-// upper triangular solves block by block: a 64 x 64 diagonal block by one
-// wavefront, then a row-parallel update of the rest of the vector.
+// upper triangular solves block by block, one launch per 64-row block: every
+// workgroup stages the diagonal block in LDS and solves it (one wavefront),
+// then updates its rows of the rest of the vector; the solved block goes to a
+// second vector, so no workgroup reads what another writes in the launch.
 // Memory is n^2 doubles (for n = 32,768: 8.6 GB), so the factorization serves
 // matrices of up to that size; larger ones take an iterative inner solver.
 #include <vector>
@@ -26,7 +35,9 @@
 namespace {
 
 #define KS_LU_NB 64
-#define KS_LU_PIV_NT 1024
+#define KS_LU_PG_MAX 512   // most workgroups of a panel step (ldn <= 131,072)
+// a pivot below this magnitude is a zero pivot (PETSc's default zeropivot, 100 eps)
+#define KS_LU_ZEROPIVOT (100.0 * 2.220446049250313e-16)
 
 // A = 0, then A(i, j) = a_ij - sigma [i == j] from the CSR, and the padding rows
 // and columns an identity block
@@ -44,78 +55,158 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_fill(const int32_t *__restrict_
   }
 }
 
-// pivot of column j (rows j..ldn-1), its row swapped with row j inside the panel
-// columns j0..j0+KS_LU_NB-1; zero pivot -> *flag = 1
-__global__ __launch_bounds__(KS_LU_PIV_NT) void k_lu_pivot(double *__restrict__ A, int ldn, int j, int j0,
-                                                          int *__restrict__ ipiv, int *flag) {
-  __shared__ double smax[KS_LU_PIV_NT / 64];
-  __shared__ int sidx[KS_LU_PIV_NT / 64];
-  __shared__ int sp;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const double *col = A + (size_t)j * ldn;
-  double best = -1.0;
-  int bi = ldn;
-  for (int i = j + tid; i < ldn; i += KS_LU_PIV_NT) {
-    const double a = fabs(col[i]);
-    if (a > best) {   // rows visited in increasing order: the first maximum stays
-      best = a;
-      bi = i;
-    }
-  }
+// magnitude for the pivot search: |a|; a NaN row is still a candidate but any
+// number wins over it
+__device__ inline double piv_mag(double a) { return a == a ? fabs(a) : -0.5; }
+
+__device__ inline bool piv_better(double m1, int i1, double m2, int i2) {
+  return m1 > m2 || (m1 == m2 && i1 < i2);
+}
+
+// workgroup reduction of (magnitude, row) to thread 0; sm/si hold KS_WAVES entries
+__device__ inline void piv_reduce(double &m, int &i, double *sm, int *si) {
   for (int off = 32; off > 0; off >>= 1) {
-    const double ob = __shfl_down(best, off, 64);
-    const int oi = __shfl_down(bi, off, 64);
-    if (ob > best || (ob == best && oi < bi)) {
-      best = ob;
-      bi = oi;
+    const double om = __shfl_down(m, off, 64);
+    const int oi = __shfl_down(i, off, 64);
+    if (piv_better(om, oi, m, i)) {
+      m = om;
+      i = oi;
     }
   }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) {
-    smax[wave] = best;
-    sidx[wave] = bi;
+    sm[wave] = m;
+    si[wave] = i;
   }
   __syncthreads();
-  if (tid == 0) {
-    double b = smax[0];
-    int p = sidx[0];
-    for (int w = 1; w < KS_LU_PIV_NT / 64; w++)
-      if (smax[w] > b || (smax[w] == b && sidx[w] < p)) {
-        b = smax[w];
-        p = sidx[w];
+  if (threadIdx.x == 0)
+    for (int w = 1; w < KS_WAVES; w++)
+      if (piv_better(sm[w], si[w], m, i)) {
+        m = sm[w];
+        i = si[w];
       }
-    if (!(b > 0.0)) {
-      *flag = 1;
-      p = j;
+}
+
+// Rows of a panel step: thread t of workgroup g owns rows j0 + g * KS_BLOCK + t +
+// s * G * KS_BLOCK (s = 0, 1, ...), the same rows for every column of the panel.
+// used[r] == j0 + 1: row r is the pivot of an earlier column of this panel.
+// pm/pi[slot][g]: workgroup g's largest magnitude (and its row) of a column.
+
+// the partial maxima of the panel's first column j0 (slot 0)
+__global__ __launch_bounds__(KS_BLOCK) void k_lu_colmax(const double *__restrict__ A, int ldn, int j0,
+                                                       double *__restrict__ pm, int *__restrict__ pi) {
+  __shared__ double sm[KS_WAVES];
+  __shared__ int si[KS_WAVES];
+  double m = -1.0;
+  int idx = INT_MAX;
+  for (int r = j0 + blockIdx.x * KS_BLOCK + threadIdx.x; r < ldn; r += gridDim.x * KS_BLOCK) {
+    const double a = piv_mag(A[(size_t)r + (size_t)j0 * ldn]);
+    if (piv_better(a, r, m, idx)) {
+      m = a;
+      idx = r;
     }
-    ipiv[j] = p;
-    sp = p;
+  }
+  piv_reduce(m, idx, sm, si);
+  if (threadIdx.x == 0) {
+    pm[blockIdx.x] = m;
+    pi[blockIdx.x] = idx;
+  }
+}
+
+// column j of the panel j0: the pivot row p from the partial maxima (slot j & 1),
+// l_rj = a_rj / a_pj for the other candidate rows r, a_rc -= l_rj a_pc for the
+// panel columns c > j, and the partial maxima of column j+1 (slot (j + 1) & 1).
+// A pivot below KS_LU_ZEROPIVOT sets *flag.
+__global__ __launch_bounds__(KS_BLOCK) void k_lu_step(double *__restrict__ A, int ldn, int j0, int j,
+                                                     int *__restrict__ used, double *__restrict__ pm,
+                                                     int *__restrict__ pi, int *__restrict__ prow, int *flag) {
+  __shared__ double prv[KS_LU_NB];
+  __shared__ double sm[KS_WAVES];
+  __shared__ int si[KS_WAVES];
+  __shared__ int sp;
+  const int tid = threadIdx.x, G = gridDim.x, slot = j & 1;
+  double m = -1.0;
+  int idx = INT_MAX;
+  for (int g = tid; g < G; g += KS_BLOCK) {
+    const double a = pm[slot * KS_LU_PG_MAX + g];
+    const int i = pi[slot * KS_LU_PG_MAX + g];
+    if (piv_better(a, i, m, idx)) {
+      m = a;
+      idx = i;
+    }
+  }
+  piv_reduce(m, idx, sm, si);
+  if (tid == 0) {
+    sp = idx;
+    if (blockIdx.x == 0) {
+      prow[j - j0] = idx;
+      if (!(m >= KS_LU_ZEROPIVOT)) *flag = 1;
+    }
   }
   __syncthreads();
-  const int p = sp;
-  if (p != j && tid < KS_LU_NB) {
-    double *a = A + (size_t)(j0 + tid) * ldn;
-    const double t = a[j];
-    a[j] = a[p];
-    a[p] = t;
+  const int p = sp;   // a candidate row always exists: rows j..ldn-1 outnumber the panel's pivots
+  const int jend = j0 + KS_LU_NB;
+  if (tid < jend - j) prv[tid] = A[(size_t)p + (size_t)(j + tid) * ldn];
+  __syncthreads();
+  const double rinv = 1.0 / prv[0];
+  const int stamp = j0 + 1;
+  double m2 = -1.0;
+  int i2 = INT_MAX;
+  for (int r = j0 + blockIdx.x * KS_BLOCK + tid; r < ldn; r += G * KS_BLOCK) {
+    if (used[r] == stamp) continue;
+    if (r == p) {
+      used[r] = stamp;
+      continue;
+    }
+    double *row = A + r;
+    const double l = row[(size_t)j * ldn] * rinv;
+    row[(size_t)j * ldn] = l;
+    for (int c = j + 1; c < jend; c++) row[(size_t)c * ldn] -= l * prv[c - j];
+    if (j + 1 < jend) {
+      const double a = piv_mag(row[(size_t)(j + 1) * ldn]);
+      if (piv_better(a, r, m2, i2)) {
+        m2 = a;
+        i2 = r;
+      }
+    }
+  }
+  if (j + 1 < jend) {
+    piv_reduce(m2, i2, sm, si);
+    if (tid == 0) {
+      pm[(slot ^ 1) * KS_LU_PG_MAX + blockIdx.x] = m2;
+      pi[(slot ^ 1) * KS_LU_PG_MAX + blockIdx.x] = i2;
+    }
   }
 }
 
-// rows i > j: l_ij = a_ij / a_jj (as the reciprocal's product, dgetf2's dscal),
-// then a_ic -= l_ij a_jc for the panel's columns c = j+1 .. j0+KS_LU_NB-1
-__global__ __launch_bounds__(KS_BLOCK) void k_lu_panel_col(double *__restrict__ A, int ldn, int j, int j0) {
-  const int i = j + 1 + blockIdx.x * KS_BLOCK + threadIdx.x;
-  if (i >= ldn) return;
-  const double r = 1.0 / A[(size_t)j + (size_t)j * ldn];
-  const double l = A[(size_t)i + (size_t)j * ldn] * r;
-  A[(size_t)i + (size_t)j * ldn] = l;
-  for (int c = j + 1; c < j0 + KS_LU_NB; c++) A[(size_t)i + (size_t)c * ldn] -= l * A[(size_t)j + (size_t)c * ldn];
+// the pivot rows of the panel (physical rows, chosen in place) as LAPACK's
+// interchanges: at step j rows j and ipiv[j] swap.  Only rows that started in
+// the panel's own 64 positions can sit elsewhere when picked: lpos follows
+// where those rows are, at which of the 64 positions holds which row.
+__global__ __launch_bounds__(64) void k_lu_ipiv(const int *__restrict__ prow, int j0, int *__restrict__ ipiv) {
+  __shared__ int lpos[KS_LU_NB], at[KS_LU_NB];
+  lpos[threadIdx.x] = j0 + threadIdx.x;
+  at[threadIdx.x] = j0 + threadIdx.x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < KS_LU_NB; k++) {
+      const int j = j0 + k, p = prow[k];
+      const bool pin = p - j0 < KS_LU_NB;
+      const int P = pin ? lpos[p - j0] : p;   // where row p stands now (>= j)
+      ipiv[j] = P;
+      const int x = at[k];                    // the row standing at j, one of the panel's own
+      lpos[x - j0] = P;
+      if (pin) lpos[p - j0] = j;
+      at[k] = p;
+      if (P - j0 < KS_LU_NB) at[P - j0] = x;
+    }
+  }
 }
 
-// the panel's interchanges (rows j0..j0+NB-1 with ipiv) on every column outside it
+// the panel's interchanges (rows j0..j0+NB-1 with ipiv, in order) on every column
 __global__ __launch_bounds__(KS_BLOCK) void k_lu_laswp(double *__restrict__ A, int ldn, int j0,
                                                       const int *__restrict__ ipiv) {
-  const int t = blockIdx.x * KS_BLOCK + threadIdx.x;
-  const int c = t < j0 ? t : t + KS_LU_NB;
+  const int c = blockIdx.x * KS_BLOCK + threadIdx.x;
   if (c >= ldn) return;
   double *a = A + (size_t)c * ldn;
   for (int j = j0; j < j0 + KS_LU_NB; j++) {
@@ -152,48 +243,63 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_trsm(double *__restrict__ A, in
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// A22 -= L21 U12, one 64 x 64 tile of A22 per workgroup: the 64 x 64 blocks of
-// L21 and U12 in LDS, each wavefront a 32 x 32 quarter (2 x 2 MFMA tiles over
-// the 16 k-steps of 4)
+// A22 -= L21 U12, one 64 x 64 tile of A22 per workgroup: L21 and U12 staged
+// in LDS in two halves of 32 along k (33 KB, several workgroups a CU), each
+// wavefront a 32 x 32 quarter (2 x 2 MFMA tiles).  The MFMA's first operand is
+// U12 (its 16 rows the tile's columns), the second L21 (its 16 columns the
+// tile's rows), so register q of lane l holds column l/16 + 4q, row l%16 of a
+// 16 x 16 tile of A22: 16 lanes read and write 128 contiguous bytes.
+#define KS_LU_KH 32
 __global__ __launch_bounds__(KS_BLOCK) void k_lu_gemm(double *__restrict__ A, int ldn, int j0) {
-  __shared__ double sL[KS_LU_NB][KS_LU_NB + 1];   // sL[k][r] = L21(r, k)
-  __shared__ double sU[KS_LU_NB][KS_LU_NB + 1];   // sU[k][c] = U12(k, c)
+  __shared__ double sL[KS_LU_KH][KS_LU_NB + 1];   // sL[k][r] = L21(r, h + k)
+  __shared__ double sU[KS_LU_KH][KS_LU_NB + 1];   // sU[k][c] = U12(h + k, c)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int base = j0 + KS_LU_NB;
   const int r0 = base + blockIdx.x * KS_LU_NB, c0 = base + blockIdx.y * KS_LU_NB;
-  for (int e = tid; e < KS_LU_NB * KS_LU_NB; e += KS_BLOCK) {
-    const int i = e % KS_LU_NB, k = e / KS_LU_NB;
-    sL[k][i] = A[(size_t)(r0 + i) + (size_t)(j0 + k) * ldn];
-    sU[i][k] = A[(size_t)(j0 + i) + (size_t)(c0 + k) * ldn];
-  }
-  __syncthreads();
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  v4d acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
   const int lr = lane & 15, lk = lane >> 4;
-#pragma unroll 4
-  for (int kk = 0; kk < KS_LU_NB / 4; kk++) {
-    const int k = 4 * kk + lk;
-    const double a0 = sL[k][wr + lr], a1 = sL[k][wr + 16 + lr];
-    const double b0 = sU[k][wc + lr], b1 = sU[k][wc + 16 + lr];
-    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-  }
-  // register q of lane l holds row l/16 + 4q, column l%16 of a 16 x 16 tile
+  // this lane's 16 entries of A22, read before the operands are staged
+  double cv[2][2][4];
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++)
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int r = r0 + wr + 16 * a + lk + 4 * q, c = c0 + wc + 16 * b + lr;
-        A[(size_t)r + (size_t)c * ldn] -= acc[a][b][q];
-      }
+      for (int q = 0; q < 4; q++)
+        cv[a][b][q] = A[(size_t)(r0 + wr + 16 * b + lr) + (size_t)(c0 + wc + 16 * a + lk + 4 * q) * ldn];
+  v4d acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int h = 0; h < KS_LU_NB; h += KS_LU_KH) {
+    if (h) __syncthreads();
+    for (int e = tid; e < KS_LU_NB * KS_LU_KH; e += KS_BLOCK) {
+      const int i = e % KS_LU_NB, k = e / KS_LU_NB;     // L21: 64 rows of a column
+      sL[k][i] = A[(size_t)(r0 + i) + (size_t)(j0 + h + k) * ldn];
+      const int kk = e % KS_LU_KH, c = e / KS_LU_KH;   // U12: 32 rows of a column
+      sU[kk][c] = A[(size_t)(j0 + h + kk) + (size_t)(c0 + c) * ldn];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KS_LU_KH / 4; kk++) {
+      const int k = 4 * kk + lk;
+      const double u0 = sU[k][wc + lr], u1 = sU[k][wc + 16 + lr];
+      const double l0 = sL[k][wr + lr], l1 = sL[k][wr + 16 + lr];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(u0, l0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(u0, l1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(u1, l0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(u1, l1, acc[1][1], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        A[(size_t)(r0 + wr + 16 * b + lr) + (size_t)(c0 + wc + 16 * a + lk + 4 * q) * ldn] =
+            cv[a][b][q] - acc[a][b][q];
 }
 
 // x(i) = b(perm(i)) for i < n, 0 in the padding
@@ -203,57 +309,95 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_permute(const int *__restrict__
     x[i] = i < n ? b[perm[i]] : 0.0;
 }
 
-// the 64 x 64 diagonal block b0..b0+63 of L (unit lower, LOWER = 1) or U (upper)
-// solved in place in x by one wavefront, lane i holding x(b0+i)
-template <bool LOWER>
-__global__ __launch_bounds__(64) void k_lu_diag(const double *__restrict__ A, int ldn, int b0, double *__restrict__ x) {
-  const int lane = threadIdx.x;
-  double xi = x[b0 + lane];
-  if (LOWER) {
-    for (int k = 0; k < KS_LU_NB - 1; k++) {
-      const double xk = __shfl(xi, k, 64);
-      if (lane > k) xi -= A[(size_t)(b0 + lane) + (size_t)(b0 + k) * ldn] * xk;
-    }
-  } else {
-    for (int k = KS_LU_NB - 1; k >= 0; k--) {
-      if (lane == k) xi /= A[(size_t)(b0 + k) + (size_t)(b0 + k) * ldn];
-      const double xk = __shfl(xi, k, 64);
-      if (lane < k) xi -= A[(size_t)(b0 + lane) + (size_t)(b0 + k) * ldn] * xk;
-    }
-  }
-  x[b0 + lane] = xi;
+__device__ inline double readlane_f64(double v, int k) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)bits, k), hi = __builtin_amdgcn_readlane((int)(bits >> 32), k);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// x(r) -= sum_k A(r, b0+k) x(b0+k) for the rows r in [r0, r1): the rest of the
-// vector after the diagonal block b0 is solved
-__global__ __launch_bounds__(KS_BLOCK) void k_lu_update(const double *__restrict__ A, int ldn, int b0, int r0, int r1,
-                                                       double *__restrict__ x) {
+#define KS_LU_TRI_ROWS (KS_BLOCK - 64)   // rows a workgroup updates (wavefronts 1..3)
+
+// One block step of a triangular solve.  Wavefront 0 solves the 64 x 64
+// diagonal block at b0 of L (unit lower, LOWER) or U from xin(b0..b0+63), lane
+// i holding row i of the block in registers and the solved entries passed on
+// by readlane (column-oriented substitution, the order of LAPACK's dtrsv);
+// workgroup 0 writes the block to out(b0..) (its rows below n_out).  The other
+// wavefronts load their rows of the rest, r >= b0 + 64 (LOWER) or r < b0,
+// meanwhile and then take xin(r) -= A(r, b0:b0+64) x_block (in the order of k
+// that dtrsv follows).  xin's block rows are read and never written in the
+// launch, and out is not xin.
+template <bool LOWER>
+__global__ __launch_bounds__(KS_BLOCK) void k_lu_tri_step(const double *__restrict__ A, int ldn, int b0, int n_out,
+                                                         double *__restrict__ xin, double *__restrict__ out) {
   __shared__ double xb[KS_LU_NB];
-  if (threadIdx.x < KS_LU_NB) xb[threadIdx.x] = x[b0 + threadIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r0 = LOWER ? b0 + KS_LU_NB : 0, r1 = LOWER ? ldn : b0;
+  const int r = r0 + blockIdx.x * KS_LU_TRI_ROWS + tid - 64;   // this thread's row (wavefronts 1..3)
+  double arow[KS_LU_NB], s = 0.0;
+  if (wave == 0) {
+    double d[KS_LU_NB];
+#pragma unroll
+    for (int k = 0; k < KS_LU_NB; k++) d[k] = A[(size_t)(b0 + lane) + (size_t)(b0 + k) * ldn];
+    double xi = xin[b0 + lane];
+    if (LOWER) {
+#pragma unroll
+      for (int k = 0; k < KS_LU_NB - 1; k++) {
+        const double xk = readlane_f64(xi, k);
+        if (lane > k) xi -= d[k] * xk;
+      }
+    } else {
+#pragma unroll
+      for (int k = KS_LU_NB - 1; k >= 0; k--) {
+        if (lane == k) xi /= d[k];
+        const double xk = readlane_f64(xi, k);
+        if (lane < k) xi -= d[k] * xk;
+      }
+    }
+    xb[lane] = xi;
+    if (blockIdx.x == 0 && b0 + lane < n_out) out[b0 + lane] = xi;
+  } else if (r < r1) {
+    s = xin[r];
+#pragma unroll
+    for (int k = 0; k < KS_LU_NB; k++) arow[k] = A[(size_t)r + (size_t)(b0 + k) * ldn];
+  }
   __syncthreads();
-  const int r = r0 + blockIdx.x * KS_BLOCK + threadIdx.x;
-  if (r >= r1) return;
-  double s = x[r];
-#pragma unroll 8
-  for (int k = 0; k < KS_LU_NB; k++) s -= A[(size_t)r + (size_t)(b0 + k) * ldn] * xb[k];
-  x[r] = s;
+  if (wave == 0) return;
+  if (r < r1) {
+    if (LOWER) {
+#pragma unroll
+      for (int k = 0; k < KS_LU_NB; k++) s -= arow[k] * xb[k];
+    } else {
+#pragma unroll
+      for (int k = KS_LU_NB - 1; k >= 0; k--) s -= arow[k] * xb[k];
+    }
+    xin[r] = s;
+  }
+  for (int rr = r + gridDim.x * KS_LU_TRI_ROWS; rr < r1; rr += gridDim.x * KS_LU_TRI_ROWS) {
+    double t = xin[rr];
+    if (LOWER) {
+      for (int k = 0; k < KS_LU_NB; k++) t -= A[(size_t)rr + (size_t)(b0 + k) * ldn] * xb[k];
+    } else {
+      for (int k = KS_LU_NB - 1; k >= 0; k--) t -= A[(size_t)rr + (size_t)(b0 + k) * ldn] * xb[k];
+    }
+    xin[rr] = t;
+  }
 }
 
 }  // namespace
 
 struct ks_lu_s {
   int n = 0, ldn = 0;
-  double *d_A = nullptr, *d_x = nullptr;
-  int *d_ipiv = nullptr, *d_perm = nullptr, *d_flag = nullptr;
+  double *d_A = nullptr, *d_x = nullptr, *d_y = nullptr;
+  int *d_ipiv = nullptr, *d_perm = nullptr;
 };
 
 extern "C" int ks_lu_destroy(ks_lu F) {
   if (!F) return KS_OK;
   if (F->d_A) (void)ks_free(F->d_A);
   if (F->d_x) (void)ks_free(F->d_x);
+  if (F->d_y) (void)ks_free(F->d_y);
   if (F->d_ipiv) (void)ks_free(F->d_ipiv);
   if (F->d_perm) (void)ks_free(F->d_perm);
-  if (F->d_flag) (void)ks_free(F->d_flag);
   delete F;
   return KS_OK;
 }
@@ -263,38 +407,45 @@ extern "C" int ks_lu_create(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out) {
   KS_CHECK(A->vprec == KS_F64 && A->nrows == A->ncols, KS_ERR_ARG, "the LU needs a square FP64 matrix");
   KS_CHECK(A->nrows <= KS_LU_MAXN, KS_ERR_RANGE, "the dense LU takes at most %d rows (%d given)", KS_LU_MAXN,
            A->nrows);
+  static_assert((KS_LU_MAXN + KS_LU_NB - 1) / KS_LU_NB * KS_LU_NB <= KS_LU_PG_MAX * KS_BLOCK,
+                "a panel step's workgroups must cover every row");
   *out = nullptr;
   ks_lu F = new ks_lu_s();
   const int n = A->nrows, ldn = (n + KS_LU_NB - 1) / KS_LU_NB * KS_LU_NB;
   F->n = n;
   F->ldn = ldn;
+  // factorization scratch: used[ldn], prow[64], flag, pi[2][PG_MAX] (ints), pm[2][PG_MAX] (doubles)
+  const size_t nint = (size_t)ldn + KS_LU_NB + 1 + 2 * KS_LU_PG_MAX;
+  const size_t scratch_bytes = ((nint * sizeof(int) + 255) / 256) * 256 + 2 * KS_LU_PG_MAX * sizeof(double);
+  void *scratch = nullptr;
   int e = ks_malloc((void **)&F->d_A, sizeof(double) * (size_t)ldn * ldn);
   if (!e) e = ks_malloc((void **)&F->d_x, sizeof(double) * ldn);
+  if (!e) e = ks_malloc((void **)&F->d_y, sizeof(double) * ldn);
   if (!e) e = ks_malloc((void **)&F->d_ipiv, sizeof(int) * ldn);
   if (!e) e = ks_malloc((void **)&F->d_perm, sizeof(int) * ldn);
-  if (!e) e = ks_malloc((void **)&F->d_flag, sizeof(int));
+  if (!e) e = ks_malloc(&scratch, scratch_bytes);
   if (!e) e = ks_memset_zero(ctx, F->d_A, sizeof(double) * (size_t)ldn * ldn);
-  if (!e) e = ks_memset_zero(ctx, F->d_flag, sizeof(int));
+  if (!e) e = ks_memset_zero(ctx, scratch, scratch_bytes);
   if (e) {
+    if (scratch) (void)ks_free(scratch);
     ks_lu_destroy(F);
     return e;
   }
+  int *d_used = (int *)scratch, *d_prow = d_used + ldn, *d_flag = d_prow + KS_LU_NB, *d_pi = d_flag + 1;
+  double *d_pm = (double *)((char *)scratch + ((nint * sizeof(int) + 255) / 256) * 256);
   hipStream_t st = ctx->stream;
   const int fill_blocks = (ldn + KS_BLOCK - 1) / KS_BLOCK;
   hipLaunchKernelGGL(k_lu_fill, dim3(fill_blocks), dim3(KS_BLOCK), 0, st, A->d_rowptr, A->d_colidx,
                      (const double *)A->d_val, n, ldn, sigma, F->d_A);
   for (int j0 = 0; j0 < ldn; j0 += KS_LU_NB) {
-    for (int j = j0; j < j0 + KS_LU_NB; j++) {
-      hipLaunchKernelGGL(k_lu_pivot, dim3(1), dim3(KS_LU_PIV_NT), 0, st, F->d_A, ldn, j, j0, F->d_ipiv, F->d_flag);
-      const int rows = ldn - j - 1;
-      if (rows > 0)
-        hipLaunchKernelGGL(k_lu_panel_col, dim3((rows + KS_BLOCK - 1) / KS_BLOCK), dim3(KS_BLOCK), 0, st, F->d_A, ldn,
-                           j, j0);
-    }
-    const int other = ldn - KS_LU_NB;
-    if (other > 0)
-      hipLaunchKernelGGL(k_lu_laswp, dim3((other + KS_BLOCK - 1) / KS_BLOCK), dim3(KS_BLOCK), 0, st, F->d_A, ldn, j0,
-                         (const int *)F->d_ipiv);
+    const int rows = ldn - j0, G = (rows + KS_BLOCK - 1) / KS_BLOCK;
+    hipLaunchKernelGGL(k_lu_colmax, dim3(G), dim3(KS_BLOCK), 0, st, (const double *)F->d_A, ldn, j0, d_pm, d_pi);
+    for (int j = j0; j < j0 + KS_LU_NB; j++)
+      hipLaunchKernelGGL(k_lu_step, dim3(G), dim3(KS_BLOCK), 0, st, F->d_A, ldn, j0, j, d_used, d_pm, d_pi, d_prow,
+                         d_flag);
+    hipLaunchKernelGGL(k_lu_ipiv, dim3(1), dim3(KS_LU_NB), 0, st, (const int *)d_prow, j0, F->d_ipiv);
+    hipLaunchKernelGGL(k_lu_laswp, dim3(fill_blocks), dim3(KS_BLOCK), 0, st, F->d_A, ldn, j0,
+                       (const int *)F->d_ipiv);
     const int rest = ldn - j0 - KS_LU_NB;
     if (rest > 0) {
       const int tb = (rest + KS_WAVES - 1) / KS_WAVES;
@@ -302,13 +453,18 @@ extern "C" int ks_lu_create(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out) {
       hipLaunchKernelGGL(k_lu_gemm, dim3(rest / KS_LU_NB, rest / KS_LU_NB), dim3(KS_BLOCK), 0, st, F->d_A, ldn, j0);
     }
   }
-  KS_HIP(hipGetLastError());
+  hipError_t herr = hipGetLastError();
   // the row permutation P (P A = L U): the interchanges applied in order to 0..ldn-1
   std::vector<int> ipiv(ldn), perm(ldn);
   int flag = 0;
-  KS_HIP(hipMemcpyAsync(ipiv.data(), F->d_ipiv, sizeof(int) * ldn, hipMemcpyDeviceToHost, st));
-  KS_HIP(hipMemcpyAsync(&flag, F->d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
-  KS_HIP(hipStreamSynchronize(st));
+  if (herr == hipSuccess) herr = hipMemcpyAsync(ipiv.data(), F->d_ipiv, sizeof(int) * ldn, hipMemcpyDeviceToHost, st);
+  if (herr == hipSuccess) herr = hipMemcpyAsync(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (herr == hipSuccess) herr = hipStreamSynchronize(st);
+  (void)ks_free(scratch);
+  if (herr != hipSuccess) {
+    ks_lu_destroy(F);
+    KS_HIP(herr);
+  }
   if (flag) {
     fprintf(stderr, "ks_hip: the LU of A - sigma I met a zero pivot\n");
     ks_lu_destroy(F);
@@ -328,8 +484,12 @@ extern "C" int ks_lu_create(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out) {
       ks_lu_destroy(F);
       return KS_ERR_PIVOT;
     }
-  KS_HIP(hipMemcpyAsync(F->d_perm, perm.data(), sizeof(int) * ldn, hipMemcpyHostToDevice, st));
-  KS_HIP(hipStreamSynchronize(st));
+  herr = hipMemcpyAsync(F->d_perm, perm.data(), sizeof(int) * ldn, hipMemcpyHostToDevice, st);
+  if (herr == hipSuccess) herr = hipStreamSynchronize(st);
+  if (herr != hipSuccess) {
+    ks_lu_destroy(F);
+    KS_HIP(herr);
+  }
   *out = F;
   return KS_OK;
 }
@@ -338,23 +498,18 @@ extern "C" int ks_lu_solve(ks_ctx ctx, ks_lu F, const double *d_b, double *d_x) 
   KS_CHECK(ctx && F && d_b && d_x, KS_ERR_ARG, "null LU solve argument");
   const int n = F->n, ldn = F->ldn;
   hipStream_t st = ctx->stream;
-  double *x = F->d_x;
   hipLaunchKernelGGL(k_lu_permute, dim3((ldn + KS_BLOCK - 1) / KS_BLOCK), dim3(KS_BLOCK), 0, st,
-                     (const int *)F->d_perm, d_b, n, ldn, x);
-  for (int b0 = 0; b0 < ldn; b0 += KS_LU_NB) {   // L y = P b
-    hipLaunchKernelGGL(k_lu_diag<true>, dim3(1), dim3(64), 0, st, (const double *)F->d_A, ldn, b0, x);
-    const int r0 = b0 + KS_LU_NB, rows = ldn - r0;
-    if (rows > 0)
-      hipLaunchKernelGGL(k_lu_update, dim3((rows + KS_BLOCK - 1) / KS_BLOCK), dim3(KS_BLOCK), 0, st,
-                         (const double *)F->d_A, ldn, b0, r0, ldn, x);
-  }
-  for (int b0 = ldn - KS_LU_NB; b0 >= 0; b0 -= KS_LU_NB) {   // U x = y
-    hipLaunchKernelGGL(k_lu_diag<false>, dim3(1), dim3(64), 0, st, (const double *)F->d_A, ldn, b0, x);
-    if (b0 > 0)
-      hipLaunchKernelGGL(k_lu_update, dim3((b0 + KS_BLOCK - 1) / KS_BLOCK), dim3(KS_BLOCK), 0, st,
-                         (const double *)F->d_A, ldn, b0, 0, b0, x);
-  }
-  KS_HIP(hipMemcpyAsync(d_x, x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+                     (const int *)F->d_perm, d_b, n, ldn, F->d_x);
+  auto grid = [](int rows) {
+    const int g = (rows + KS_LU_TRI_ROWS - 1) / KS_LU_TRI_ROWS;
+    return g < 1 ? 1 : (g > 512 ? 512 : g);
+  };
+  for (int b0 = 0; b0 < ldn; b0 += KS_LU_NB)   // L y = P b: F->d_x updated in place, y to F->d_y
+    hipLaunchKernelGGL(k_lu_tri_step<true>, dim3(grid(ldn - b0 - KS_LU_NB)), dim3(KS_BLOCK), 0, st,
+                       (const double *)F->d_A, ldn, b0, ldn, F->d_x, F->d_y);
+  for (int b0 = ldn - KS_LU_NB; b0 >= 0; b0 -= KS_LU_NB)   // U x = y: F->d_y updated in place, x to d_x
+    hipLaunchKernelGGL(k_lu_tri_step<false>, dim3(grid(b0)), dim3(KS_BLOCK), 0, st, (const double *)F->d_A, ldn, b0,
+                       n, F->d_y, d_x);
   KS_HIP(hipGetLastError());
   return KS_OK;
 }

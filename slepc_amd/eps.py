@@ -333,8 +333,9 @@ class EPS:
     def setST(self, st_type: str = "shift", shift=None, ksp_rtol=None, ksp_maxits=None, pc_type=None,
               ksp_type=None):
         """STSetType (+ STSetShift) and the inner solver of shift-and-invert
-        (STGetKSP -> KSPSetType ("cg" or "bcgs"), KSPSetTolerances, KSPGetPC ->
-        PCSetType ("jacobi", "none", or "ilu" with "bcgs"))."""
+        (STGetKSP -> KSPSetType ("preonly", the default direct solve with
+        "lu"; "cg" or "bcgs"), KSPSetTolerances, KSPGetPC -> PCSetType ("lu"
+        with "preonly"; "jacobi", "none", or "ilu" with "bcgs"))."""
         st = C.c_void_p()
         _chk(lib().EPSGetST(self.h, C.byref(st)), "EPSGetST")
         _chk(lib().STSetType(st, st_type.encode()), "STSetType")

@@ -6,7 +6,9 @@
  * STSHIFT with sigma = 0 on a standard problem hands out A itself
  * (STMatMAXPY_Private leaves T[0] = A, stsolve.c:598-610); eigenvalues need no
  * back-transform.  STSINVERT's operator is (A - sigma I)^-1, applied by an inner
- * solve (KSP) on the device: preconditioned conjugate gradients (ks_pcg_solve,
+ * solve (KSP) on the device: by default, as in the reference (stsles.c:47-54),
+ * the direct solve KSPPREONLY + PCLU (ks_lu_create / ks_lu_solve, a dense LU
+ * with partial pivoting); or preconditioned conjugate gradients (ks_pcg_solve,
  * KSPCG) or BiCGStab (ks_bcgs_solve, KSPBCGS, for a shift inside the
  * spectrum), with a Jacobi or no preconditioner, or with BiCGStab the ILU(0)
  * factors of A - sigma I (ks_ilu0_create, PCILU).  Its eigenvalues theta map
@@ -74,8 +76,8 @@ PetscErrorCode STGetKSP(ST st, KSP *ksp)
 {
   if (!st->ksp) {
     PetscCall(PetscMallocZero(sizeof(struct _p_KSP), &st->ksp));
-    strcpy(st->ksp->type, KSPCG);
-    strcpy(st->ksp->pc.type, PCJACOBI);
+    strcpy(st->ksp->type, KSPPREONLY);   /* STSetDefaultKSP_Default (stsles.c:47-54) */
+    strcpy(st->ksp->pc.type, PCLU);
     st->ksp->rtol = SLEPC_DEFAULT_TOL;   /* stsles.c:407 */
     st->ksp->abstol = 1e-50;
     st->ksp->maxits = 10000;
@@ -87,9 +89,13 @@ PetscErrorCode STGetKSP(ST st, KSP *ksp)
 PetscErrorCode KSPSetType(KSP ksp, KSPType type)
 {
   PetscCheck(ksp && type, PETSC_ERR_ARG_NULL, "null argument");
-  PetscCheck(!strcmp(type, KSPCG) || !strcmp(type, KSPBCGS), PETSC_ERR_SUP,
-             "KSP type %s is not available (the device inner solvers are %s and %s)", type, KSPCG, KSPBCGS);
+  PetscCheck(!strcmp(type, KSPCG) || !strcmp(type, KSPBCGS) || !strcmp(type, KSPPREONLY), PETSC_ERR_SUP,
+             "KSP type %s is not available (the device inner solvers are %s, %s and %s)", type, KSPPREONLY, KSPCG,
+             KSPBCGS);
   strcpy(ksp->type, type);
+  /* the LU factors serve only as the whole solve here (KSPPREONLY): an
+     iterative solver left with the default PCLU takes Jacobi instead */
+  if (!ksp->pc.set) strcpy(ksp->pc.type, !strcmp(type, KSPPREONLY) ? PCLU : PCJACOBI);
   return PETSC_SUCCESS;
 }
 
@@ -102,9 +108,10 @@ PetscErrorCode KSPGetPC(KSP ksp, PC *pc)
 PetscErrorCode PCSetType(PC pc, PCType type)
 {
   PetscCheck(pc && type, PETSC_ERR_ARG_NULL, "null argument");
-  PetscCheck(!strcmp(type, PCJACOBI) || !strcmp(type, PCNONE) || !strcmp(type, PCILU), PETSC_ERR_SUP,
-             "PC type %s is not available (jacobi, ilu or none)", type);
+  PetscCheck(!strcmp(type, PCJACOBI) || !strcmp(type, PCNONE) || !strcmp(type, PCILU) || !strcmp(type, PCLU),
+             PETSC_ERR_SUP, "PC type %s is not available (lu, jacobi, ilu or none)", type);
   strcpy(pc->type, type);
+  pc->set = PETSC_TRUE;
   return PETSC_SUCCESS;
 }
 
@@ -128,6 +135,8 @@ static void KSPFreeWork(KSP ksp)
 {
   if (ksp->ilu) (void)ks_ilu0_destroy(ksp->ilu);
   ksp->ilu = NULL;
+  if (ksp->lu) (void)ks_lu_destroy(ksp->lu);
+  ksp->lu = NULL;
   if (ksp->d_dinv) (void)ks_free(ksp->d_dinv);
   if (ksp->d_work) (void)ks_free(ksp->d_work);
   ksp->d_dinv = ksp->d_work = NULL;
@@ -147,6 +156,14 @@ static PetscErrorCode MatMult_STSinvert(Mat P, Vec x, Vec y)
   PetscCheck(x->prec == SLEPC_AMD_F64 && y->prec == SLEPC_AMD_F64, PETSC_ERR_SUP,
              "shift-and-invert runs on FP64 vectors");
   const PetscBool cg = !strcmp(ksp->type, KSPCG) ? PETSC_TRUE : PETSC_FALSE;
+  if (!strcmp(ksp->type, KSPPREONLY)) {   /* KSPSolve_PREONLY: one application of the LU factors */
+    PetscCheck(ksp->lu, PETSC_ERR_ORDER, "KSP type %s with PC type %s needs STSetUp after they are set", KSPPREONLY,
+               PCLU);
+    KSCall(ks_lu_solve(SlepcAMDContext(), ksp->lu, (const double *)x->d, (double *)y->d));
+    ksp->its_total += 1;
+    ksp->nsolves++;
+    return PETSC_SUCCESS;
+  }
   PetscCheck(strcmp(ksp->pc.type, PCILU) || (!cg && ksp->ilu), PETSC_ERR_ORDER,
              "PC type %s needs KSP type %s set before STSetUp", PCILU, KSPBCGS);
   if (cg)
@@ -202,6 +219,18 @@ PetscErrorCode STSetUp(ST st)
     KSCall(ks_pcg_jacobi(SlepcAMDContext(), h->csr, st->sigma, !strcmp(ksp->pc.type, PCJACOBI), ksp->d_dinv));
     if (ksp->ilu) (void)ks_ilu0_destroy(ksp->ilu);
     ksp->ilu = NULL;
+    if (ksp->lu) (void)ks_lu_destroy(ksp->lu);
+    ksp->lu = NULL;
+    PetscCheck(!strcmp(ksp->type, KSPPREONLY) == !strcmp(ksp->pc.type, PCLU), PETSC_ERR_SUP,
+               "the device inner solvers pair KSP type %s with PC type %s only (given %s with %s)", KSPPREONLY, PCLU,
+               ksp->type, ksp->pc.type);
+    if (!strcmp(ksp->type, KSPPREONLY)) {
+      PetscCheck(st->A->m <= KS_LU_MAXN, PETSC_ERR_SUP,
+                 "the direct inner solve (%s + %s) holds A - sigma I dense and takes n <= %d (n = %d): choose %s or %s "
+                 "with -st_ksp_type",
+                 KSPPREONLY, PCLU, KS_LU_MAXN, (int)st->A->m, KSPCG, KSPBCGS);
+      KSCall(ks_lu_create(SlepcAMDContext(), h->csr, st->sigma, &ksp->lu));
+    }
     if (!strcmp(ksp->pc.type, PCILU)) {
       PetscCheck(!strcmp(ksp->type, KSPBCGS), PETSC_ERR_SUP, "PC type %s is available with KSP type %s", PCILU,
                  KSPBCGS);

@@ -86,6 +86,7 @@ def lib():
             "ks_group_create": [C.POINTER(vp)], "ks_group_destroy": [vp],
             "ks_group_step": [vp, i32, vp, i32, dbl], "ks_group_wait_ctx": [vp, vp],
             "ks_ctx_wait_group": [vp, vp],
+            "ks_lu_create": [vp, vp, dbl, C.POINTER(vp)], "ks_lu_solve": [vp, vp, vp, vp], "ks_lu_destroy": [vp],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -230,6 +231,23 @@ class CSR:
     def free(self):
         if self.h:
             lib().ks_csr_destroy(self.h)
+            self.h = None
+
+
+class LU:
+    """ks_lu: the dense LU factors (partial pivoting) of A - sigma I on the device."""
+
+    def __init__(self, ctx: Context, A: CSR, sigma: float = 0.0):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        check(lib().ks_lu_create(ctx.h, A.h, C.c_double(sigma), C.byref(self.h)), "ks_lu_create")
+
+    def solve(self, b: DeviceArray, x: DeviceArray):
+        check(lib().ks_lu_solve(self.ctx.h, self.h, b.ptr, x.ptr), "ks_lu_solve")
+
+    def close(self):
+        if self.h:
+            lib().ks_lu_destroy(self.h)
             self.h = None
 
 

@@ -30,7 +30,7 @@ def test_ex11_sinvert_golden_vs_oracle(E, oracle, pc):
     csr = M.grid_graph_laplacian(10)
     ones = np.ones((100, 1))
     r = E.solve(csr, nev=4, ncv=20, which=E.EPS.TARGET_MAGNITUDE, target=0.0, deflation=ones,
-                st=dict(st_type="sinvert", pc_type=pc))
+                st=dict(st_type="sinvert", ksp_type="cg", pc_type=pc))
     o = oracle.eps_solve(csr, nev=4, ncv=20, which=oracle.TARGET_MAGNITUDE, target=0.0, deflation=ones, sinvert=True,
                          pc_none=(pc == "none"))
     print(f"pc={pc} its={r['its']}/{o['its']} ksp_its={r['ksp_its']}/{o['ksp_its']}", flush=True)
@@ -48,7 +48,7 @@ def test_sinvert_closed_form_at_device_size(E):
     csr = M.laplacian_2d(N, Mm)
     n = len(csr[0]) - 1
     r = E.solve(csr, nev=4, ncv=20, tol=1e-10, which=E.EPS.TARGET_MAGNITUDE, target=sigma,
-                v0=M.uniform_vector(2, n), want_vectors=False, st=dict(st_type="sinvert", ksp_rtol=1e-12))
+                v0=M.uniform_vector(2, n), want_vectors=False, st=dict(st_type="sinvert", ksp_type="cg", ksp_rtol=1e-12))
     closed = np.sort(M.laplacian_2d_eigenvalues(N, Mm))
     near = closed[np.argsort(np.abs(closed - sigma))][:4]
     print(f"n={n} its={r['its']} ksp_its={r['ksp_its']} eig={r['eigr'][:4]}", flush=True)
@@ -166,4 +166,136 @@ def test_sinvert_indefinite_shift_reports_failed_solve(E):
     # runs out of iterations) and the solve fails with an error instead of a wrong answer
     with pytest.raises(E.SlepcError):
         E.solve(M.laplacian_1d(200), nev=2, which=E.EPS.TARGET_MAGNITUDE, target=2.0,
-                st=dict(st_type="sinvert", ksp_maxits=500))
+                st=dict(st_type="sinvert", ksp_type="cg", ksp_maxits=500))
+
+
+# ---- the direct inner solve, KSPPREONLY + PCLU: the reference's default for STSINVERT
+# (stsles.c:47-54), here the dense LU with partial pivoting of ks_lu_create / ks_lu_solve
+
+@pytest.fixture(scope="module")
+def ks():
+    from slepc_amd import ks as K
+    return K
+
+
+@pytest.mark.parametrize("n,sigma,kind", [(1, 0.5, "rand"), (63, 0.0, "rand"), (64, 0.0, "rand"), (200, 2.0, "lap"),
+                                          (1000, 0.3, "rand"), (4096, 2.0, "lap")])
+def test_lu_solve_vs_oracle_factors(ks, oracle, n, sigma, kind):
+    # ragged sizes (the factors are padded to a multiple of 64), a 1-D Laplacian shifted to a
+    # zero diagonal (every column pivots), random nonsymmetric rows; two right-hand sides on
+    # the same factors, each against the oracle's factors and the backward error bound
+    import scipy.linalg as sl
+    csr = M.laplacian_1d(n) if kind == "lap" else M.random_sparse_nonsymmetric(n, min(1.0, 8.0 / n), seed=n)
+    A = M.csr_to_dense(*csr) - sigma * np.eye(n)
+    lu, ipiv = oracle.lu(csr, sigma)
+    ctx = ks.Context(0)
+    dA = ks.CSR(ctx, csr)
+    F = ks.LU(ctx, dA, sigma)
+    rng = np.random.default_rng(n)
+    cond = np.linalg.cond(A, 1) if n <= 1000 else 1e4   # the shifted Laplacian of 4,096 rows: about 5e3
+    for rhs in range(2):
+        b = rng.standard_normal(n)
+        db = ks.DeviceArray.from_numpy(ctx, b)
+        dx = ks.DeviceArray(ctx, (n,))
+        F.solve(db, dx)
+        ctx.sync()
+        x = dx.numpy()
+        xr = sl.lu_solve((lu, ipiv), b)
+        bw = np.abs(A @ x - b).max() / (np.abs(A).sum(axis=1).max() * np.abs(x).max() + np.abs(b).max())
+        fw = np.abs(x - xr).max() / np.abs(xr).max()
+        print(f"n={n} {kind} rhs={rhs} backward={bw:.2e} forward={fw:.2e} cond={cond:.1e}", flush=True)
+        assert bw < 1e-14 * max(10, n ** 0.5)
+        assert fw < 1e-14 * cond * 100
+        db.free()
+        dx.free()
+    F.close()
+    dA.free()
+    ctx.close()
+
+
+def test_lu_zero_pivot_reported(ks, E):
+    # diag(1, ..., 10) - 3 I has a zero column: the factorization reports the zero pivot
+    # (KS_ERR_PIVOT) and so does the eigensolve that asked for it
+    n = 10
+    csr = (np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.arange(1.0, n + 1))
+    ctx = ks.Context(0)
+    dA = ks.CSR(ctx, csr)
+    with pytest.raises(ks.KSError):
+        ks.LU(ctx, dA, 3.0)
+    dA.free()
+    ctx.close()
+    with pytest.raises(E.SlepcError):
+        E.solve(csr, nev=2, which=E.EPS.TARGET_MAGNITUDE, target=3.0, st=dict(st_type="sinvert"))
+
+
+def test_lu_size_limit_names_the_iterative_solvers(E):
+    with pytest.raises(E.SlepcError):
+        E.solve(M.laplacian_1d(40000), nev=2, which=E.EPS.TARGET_MAGNITUDE, target=-0.01, st=dict(st_type="sinvert"))
+
+
+@pytest.mark.parametrize("n", [30, 32])
+def test_interior_direct_default_vs_oracle(E, oracle, n):
+    # eps test2's interior goldens with the default inner solve
+    csr = M.laplacian_1d(n)
+    r = E.solve(csr, nev=4, tol=1e-10, which=E.EPS.TARGET_MAGNITUDE, target=2.1, st=dict(st_type="sinvert"))
+    o = oracle.eps_solve(csr, nev=4, tol=1e-10, which=oracle.TARGET_MAGNITUDE, target=2.1, sinvert=True,
+                         ksp="preonly", pc="lu")
+    print(f"n={n} its={r['its']}/{o['its']} solves={r['ksp_its']}/{o['ksp_its']}", flush=True)
+    assert r["reason"] == 1 and ", ".join("%.5f" % v for v in r["eigr"][:4]) == TEST2_INTERIOR[n]
+    assert np.allclose(r["eigr"][:4], o["eigr"][:4], rtol=1e-12, atol=0)
+    assert r["its"] == o["its"] and r["ksp_its"] == o["ksp_its"]
+
+
+def test_ex11_direct_vs_oracle(E, oracle):
+    # ex11's shift 0 is an eigenvalue (the deflated constant vector): the LU meets a zero
+    # pivot there; a shift just below it gives ex11's eigenvalues
+    csr = M.grid_graph_laplacian(10)
+    ones = np.ones((100, 1))
+    kw = dict(nev=4, ncv=20, deflation=ones)
+    with pytest.raises(E.SlepcError):
+        E.solve(csr, which=E.EPS.TARGET_MAGNITUDE, target=0.0, st=dict(st_type="sinvert"), **kw)
+    r = E.solve(csr, which=E.EPS.TARGET_MAGNITUDE, target=-0.01, st=dict(st_type="sinvert"), **kw)
+    o = oracle.eps_solve(csr, which=oracle.TARGET_MAGNITUDE, target=-0.01, sinvert=True, ksp="preonly", pc="lu", **kw)
+    assert r["reason"] == 1 and ", ".join("%.5f" % v for v in r["eigr"][:4]) == EX11
+    print(f"its={r['its']}/{o['its']} err={r['err'][:4]}", flush=True)
+    assert r["its"] == o["its"]
+    assert np.allclose(r["eigr"][:4], o["eigr"][:4], rtol=1e-10, atol=0)
+    # the outer tolerance bounds the residual of theta = 1/(lambda - sigma); for A it grows by
+    # about |lambda - sigma| |A - sigma I| / |lambda| (near 10 here)
+    assert (r["err"][:4] < 1e-6).all()
+
+
+def test_markov_nonsymmetric_direct_vs_oracle(E, oracle):
+    # eps test11's Markov model (m = 15, N = 120) near 0.5: A - 0.5 I nonsymmetric and
+    # indefinite; the four eigenvalues nearest the shift against numpy's
+    csr = M.markov_model(15)
+    w = np.linalg.eigvals(M.csr_to_dense(*csr))
+    near = np.sort(w[np.argsort(np.abs(w - 0.5))][:4].real)
+    r = E.solve(csr, nev=4, tol=1e-10, which=E.EPS.TARGET_MAGNITUDE, target=0.5, hermitian=False,
+                v0=np.ones(120), st=dict(st_type="sinvert"))
+    o = oracle.eps_solve(csr, nev=4, tol=1e-10, which=oracle.TARGET_MAGNITUDE, target=0.5, hermitian=False,
+                         v0=np.ones(120), sinvert=True, ksp="preonly", pc="lu")
+    print(f"its={r['its']}/{o['its']} eig={r['eigr'][:4]}", flush=True)
+    assert r["reason"] == 1 and r["its"] == o["its"]
+    assert np.allclose(np.sort(r["eigr"][:4]), near, rtol=1e-10, atol=0)
+    assert np.allclose(r["eigr"][:4], o["eigr"][:4], rtol=1e-10, atol=0)
+
+
+def test_deep_interior_direct_at_device_size(E):
+    # 128 x 127 5-point Laplacian (16,256 rows), the shift halfway between its 2,000th and
+    # 2,001st eigenvalues: 2,000 negative eigenvalues in A - sigma I, out of reach of the
+    # iterative inner solvers; the dense LU (16,256^2, 2.1 GB) solves it exactly
+    import time
+    N, Mm = 128, 127
+    csr = M.laplacian_2d(N, Mm)
+    n = len(csr[0]) - 1
+    closed = np.sort(M.laplacian_2d_eigenvalues(N, Mm))
+    sigma = 0.5 * (closed[1999] + closed[2000])
+    t0 = time.time()
+    r = E.solve(csr, nev=6, ncv=24, tol=1e-10, which=E.EPS.TARGET_MAGNITUDE, target=sigma,
+                v0=M.uniform_vector(5, n), want_vectors=False, st=dict(st_type="sinvert"))
+    near = np.sort(closed[np.argsort(np.abs(closed - sigma))][:6])
+    print(f"n={n} sigma={sigma:.6f} its={r['its']} solves={r['ksp_its']} {time.time() - t0:.2f}s "
+          f"eig={r['eigr'][:6]}", flush=True)
+    assert r["reason"] == 1
+    assert np.allclose(np.sort(r["eigr"][:6]), near, rtol=1e-10, atol=0)

@@ -139,3 +139,100 @@ def test_ilu_fewer_iterations_than_jacobi(oracle):
     assert np.allclose(np.sort(b["eigr"][:4]), np.sort(near), rtol=1e-10, atol=0)
     assert np.allclose(a["eigr"][:4], b["eigr"][:4], rtol=1e-10, atol=0)
     assert b["ksp_its"] < a["ksp_its"] / 4
+
+
+# The direct inner solve KSPPREONLY + PCLU (the reference's default for STSINVERT,
+# stsles.c:47-54): the oracle's dense LU with partial pivoting is pinned to LAPACK's
+# dgetrf through scipy (the same pivot rows, the same factors), and its solves to the
+# goldens above and to numpy's eigenvalues of a nonsymmetric matrix.
+
+def _dense(csr):
+    rp, ci, v = csr
+    n = len(rp) - 1
+    A = np.zeros((n, n))
+    for i in range(n):
+        np.add.at(A[i], ci[rp[i]:rp[i + 1]], v[rp[i]:rp[i + 1]])
+    return A
+
+
+def _random_sparse(n, seed, zero_diag_every=3):
+    # nonsymmetric, about 8 entries a row, every third diagonal entry zero (pivoting needed)
+    rng = np.random.default_rng(seed)
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        js = np.unique(np.concatenate([rng.integers(0, n, 7), [i]]))
+        for j in js:
+            rows.append(i); cols.append(j)
+            vals.append(0.0 if (j == i and i % zero_diag_every == 0) else rng.standard_normal())
+    rowptr = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(rowptr, np.array(rows) + 1, 1)
+    return np.cumsum(rowptr).astype(np.int32), np.array(cols, dtype=np.int32), np.array(vals)
+
+
+@pytest.mark.parametrize("n,sigma", [(1, 0.5), (7, 0.0), (130, 0.25), (301, -1.5)])
+def test_lu_factors_match_lapack(oracle, n, sigma):
+    import scipy.linalg as sl
+    csr = _random_sparse(n, seed=n)
+    lu, ipiv = oracle.lu(csr, sigma)
+    A = _dense(csr) - sigma * np.eye(n)
+    lu_ref, piv_ref = sl.lu_factor(A)
+    assert (ipiv == piv_ref).all()
+    assert np.allclose(lu, lu_ref, rtol=1e-11, atol=1e-12 * np.abs(lu_ref).max())
+    # and P A = L U rebuilt from the factors
+    L = np.tril(lu, -1) + np.eye(n)
+    U = np.triu(lu)
+    PA = A.copy()
+    for k in range(n):
+        PA[[k, ipiv[k]]] = PA[[ipiv[k], k]]
+    assert np.abs(PA - L @ U).max() <= 1e-13 * max(1.0, np.abs(A).max()) * n
+
+
+def test_lu_zero_pivot_fails(oracle):
+    # diag(1, 2, ..., 10) - 3 I is singular: its third column is zero
+    n = 10
+    csr = (np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.arange(1.0, n + 1))
+    with pytest.raises(RuntimeError):
+        oracle.lu(csr, 3.0)
+    with pytest.raises(RuntimeError):
+        oracle.eps_solve(csr, nev=2, which=oracle.TARGET_MAGNITUDE, target=3.0, sinvert=True, ksp="preonly", pc="lu")
+
+
+def test_ex11_direct_singular_shift_and_near_shift(oracle):
+    # ex11's shift 0 is an eigenvalue of the graph Laplacian (the deflated constant vector):
+    # the LU of A - 0 I meets a zero pivot, where the reference's ex11 runs CG on the
+    # deflated space; a shift just below it factors and finds ex11's eigenvalues
+    csr = M.grid_graph_laplacian(10)
+    kw = dict(nev=4, ncv=20, which=oracle.TARGET_MAGNITUDE, deflation=np.ones((100, 1)), sinvert=True, ksp="preonly",
+              pc="lu")
+    with pytest.raises(RuntimeError):
+        oracle.eps_solve(csr, target=0.0, **kw)
+    r = oracle.eps_solve(csr, target=-0.01, **kw)
+    assert r["reason"] == 1 and ", ".join("%.5f" % v for v in r["eigr"][:4]) == EX11
+    assert 0 < r["ksp_its"] <= r["nmatmult"]      # one application of the factors per solve
+    # the outer tolerance bounds the residual of theta = 1/(lambda - sigma); for A it grows by
+    # about |lambda - sigma| |A - sigma I| / |lambda| (near 10 here)
+    assert (r["err"][:4] < 1e-6).all()
+
+
+@pytest.mark.parametrize("n", [30, 32])
+def test_interior_direct_golden(oracle, n):
+    csr = M.laplacian_1d(n)
+    r = oracle.eps_solve(csr, nev=4, tol=1e-10, which=oracle.TARGET_MAGNITUDE, target=2.1, sinvert=True,
+                         ksp="preonly", pc="lu")
+    assert r["reason"] == 1 and ", ".join("%.5f" % v for v in r["eigr"][:4]) == TEST2_INTERIOR[n]
+    closed = 2 - 2 * np.cos(np.arange(1, n + 1) * np.pi / (n + 1))
+    near = closed[np.argsort(np.abs(closed - 2.1))][:4]
+    assert np.allclose(r["eigr"][:4], near, rtol=1e-12, atol=0)
+
+
+def test_markov_nonsymmetric_direct_vs_numpy(oracle):
+    # the Markov model of eps test11 (m = 15, N = 120) near 0.5: A - 0.5 I is nonsymmetric
+    # and indefinite, which the LU takes as it is
+    csr = M.markov_model(15)
+    w = np.linalg.eigvals(_dense(csr))
+    near = w[np.argsort(np.abs(w - 0.5))][:4]
+    r = oracle.eps_solve(csr, nev=4, tol=1e-10, which=oracle.TARGET_MAGNITUDE, target=0.5, hermitian=False,
+                         v0=np.ones(120), sinvert=True, ksp="preonly", pc="lu")
+    assert r["reason"] == 1 and r["nconv"] >= 4
+    assert np.allclose(np.sort(r["eigr"][:4]), np.sort(near.real), rtol=1e-10, atol=0)
+    assert np.allclose(r["eigi"][:4], 0.0, atol=1e-12)
