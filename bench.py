@@ -29,6 +29,12 @@ Other forms, each printing its own JSON line:
   ncv = 2 nev + 16, tol 1e-10) arriving by a Poisson process at ``--qps``,
   served concurrently under an HBM budget, timed beside the same requests
   solved one after another.
+* ``--workload lu`` -- the direct inner solve of shift-and-invert (KSPPREONLY +
+  PCLU, ks_lu_create / ks_lu_solve) alone on 2-D 5-point Laplacians of 128 x 127
+  and 181 x 180 points (16,256 and 32,580 rows) shifted into mid-spectrum:
+  factorization FP64 TFLOP/s (2/3 ldn^3 flops) and solve GB/s (ldn^2 8 bytes of
+  factors read), then the shift-and-invert eigensolve of the smaller grid at a
+  shift with 2,000 eigenvalues below it (nev 6, tol 1e-10).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -103,7 +109,7 @@ def _measured_traffic(kernel, grid):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["fp64", "mixed", "serve", "scale", "copycal", "restartgemm"],
+    ap.add_argument("--workload", choices=["fp64", "mixed", "serve", "scale", "copycal", "restartgemm", "lu"],
                     default="fp64")
     ap.add_argument("--rmat-scale", type=int, default=24, help="scale: R-MAT graph scale")
     ap.add_argument("--cd-requests", type=int, default=8, help="scale: convection-diffusion requests served with it")
@@ -138,6 +144,8 @@ def main():
         return run_copycal(args)
     if args.workload == "restartgemm":
         return run_restartgemm(args)
+    if args.workload == "lu":
+        return run_lu(args)
     if args.workload != "fp64":
         out = {"mixed": run_mixed, "serve": run_serve, "scale": run_scale}[args.workload](args)
         print(json.dumps(out), flush=True)
@@ -527,6 +535,71 @@ def run_restartgemm(args):
         V.free()
         dQ.free()
     print(json.dumps({"workload": "restartgemm", "kernel": "ks_bv_multinplace (FP64 MFMA)", "shapes": rows}), flush=True)
+
+
+def run_lu(args):
+    """The dense LU of A - sigma I (ks_lu_create) and its solve (ks_lu_solve) on 2-D
+    Laplacians shifted into mid-spectrum, then the shift-and-invert eigensolve that
+    uses them."""
+    from slepc_amd import eps as E
+    from slepc_amd import ks as K
+    from slepc_amd import matgen as M
+    ctx = K.Context(0)
+    rows = []
+    for N in (128, 181):
+        csr = M.laplacian_2d(N, N - 1)
+        n = len(csr[0]) - 1
+        ldn = (n + 63) // 64 * 64
+        closed = np.sort(M.laplacian_2d_eigenvalues(N, N - 1))
+        sigma = 0.5 * (closed[1999] + closed[2000])
+        dA = K.CSR(ctx, csr)
+        b = K.DeviceArray.from_numpy(ctx, M.uniform_vector(7, n))
+        x = K.DeviceArray(ctx, (n,))
+        tf = []
+        for _ in range(max(1, args.steps)):
+            ctx.sync()
+            t0 = time.perf_counter()
+            F = K.LU(ctx, dA, sigma)          # returns after the factorization completes
+            tf.append(time.perf_counter() - t0)
+            F.solve(b, x)
+            ctx.sync()
+            nsolve = 20
+            t0 = time.perf_counter()
+            for _ in range(nsolve):
+                F.solve(b, x)
+            ctx.sync()
+            ts = (time.perf_counter() - t0) / nsolve
+            F.close()
+        t_f = min(tf)
+        flops = 2.0 / 3.0 * ldn ** 3
+        rows.append({"n": n, "ldn": ldn, "sigma": round(sigma, 6), "factor_ms": round(1e3 * t_f, 2),
+                     "factor_fp64_TFLOPs": round(flops / t_f / 1e12, 2),
+                     "factor_fp64_frac": round(flops / t_f / 1e12 / 78.6, 4), "solve_ms": round(1e3 * ts, 3),
+                     "solve_GBs": round(8.0 * ldn * ldn / ts / 1e9, 1),
+                     "solve_hbm_frac": round(8.0 * ldn * ldn / ts / 1e9 / HBM_PEAK_GBS, 4)})
+        log(f"[lu] n={n}: factor {1e3 * t_f:.1f} ms ({flops / t_f / 1e12:.2f} TFLOP/s), solve {1e3 * ts:.2f} ms "
+            f"({8.0 * ldn * ldn / ts / 1e9:.0f} GB/s)")
+        b.free()
+        x.free()
+        dA.free()
+    ctx.close()
+    E.initialize(0)
+    N = 128
+    csr = M.laplacian_2d(N, N - 1)
+    n = len(csr[0]) - 1
+    closed = np.sort(M.laplacian_2d_eigenvalues(N, N - 1))
+    sigma = 0.5 * (closed[1999] + closed[2000])
+    t0 = time.perf_counter()
+    r = E.solve(csr, nev=6, ncv=24, tol=1e-10, which=E.EPS.TARGET_MAGNITUDE, target=sigma,
+                v0=M.uniform_vector(5, n), want_vectors=False, st=dict(st_type="sinvert"))
+    t_eps = time.perf_counter() - t0
+    near = np.sort(closed[np.argsort(np.abs(closed - sigma))][:6])
+    ok = bool(r["reason"] == 1 and np.allclose(np.sort(r["eigr"][:6]), near, rtol=1e-10, atol=0))
+    log(f"[lu] sinvert eigensolve n={n}: {t_eps:.2f} s, {r['its']} restarts, {r['ksp_its']} solves, verified={ok}")
+    print(json.dumps({"workload": "lu", "kernel": "ks_lu_create / ks_lu_solve (dense LU, FP64 MFMA)", "shapes": rows,
+                      "sinvert_eigensolve": {"n": n, "sigma": round(sigma, 6), "nev": 6, "seconds": round(t_eps, 3),
+                                             "restarts": r["its"], "solves": r["ksp_its"], "verified": ok}}),
+          flush=True)
 
 
 def _er_job(r):
