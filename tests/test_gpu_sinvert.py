@@ -213,6 +213,40 @@ def test_lu_solve_vs_oracle_factors(ks, oracle, n, sigma, kind):
     ctx.close()
 
 
+def test_lu_at_its_largest_size(ks):
+    # n = 32,768 (the largest the dense factorization takes; 8.6 GB of factors): the 1-D
+    # Laplacian shifted to a zero diagonal, against the banded LAPACK solve
+    import scipy.linalg as sl
+    n, sigma = 32768, 2.0 + 1e-3
+    csr = M.laplacian_1d(n)
+    ctx = ks.Context(0)
+    dA = ks.CSR(ctx, csr)
+    F = ks.LU(ctx, dA, sigma)
+    b = M.uniform_vector(11, n)
+    db = ks.DeviceArray.from_numpy(ctx, b)
+    dx = ks.DeviceArray(ctx, (n,))
+    F.solve(db, dx)
+    ctx.sync()
+    x = dx.numpy()
+    ab = np.zeros((3, n))
+    ab[0, 1:] = -1.0
+    ab[1, :] = 2.0 - sigma
+    ab[2, :-1] = -1.0
+    xr = sl.solve_banded((1, 1), ab, b)
+    r = 2.0 * x - sigma * x
+    r[1:] -= x[:-1]
+    r[:-1] -= x[1:]
+    bw = np.abs(r - b).max() / (4.0 * np.abs(x).max() + np.abs(b).max())
+    fw = np.abs(x - xr).max() / np.abs(xr).max()
+    print(f"n={n} backward={bw:.2e} forward={fw:.2e}", flush=True)
+    assert bw < 1e-14 * n ** 0.5 and fw < 1e-9
+    F.close()
+    db.free()
+    dx.free()
+    dA.free()
+    ctx.close()
+
+
 def test_lu_zero_pivot_reported(ks, E):
     # diag(1, ..., 10) - 3 I has a zero column: the factorization reports the zero pivot
     # (KS_ERR_PIVOT) and so does the eigensolve that asked for it
