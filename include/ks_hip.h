@@ -135,6 +135,11 @@ int ks_csr_convert(ks_csr A, ks_ctx ctx, ks_prec to, ks_csr *out);
    two-sided Krylov-Schur (src/eps/impls/krylov/krylovschur/ks-twosided.c:138) by an
    explicit matrix, so products with A^T use the same SpMV kernel. */
 int ks_csr_transpose(ks_csr A, ks_ctx ctx, ks_csr *out);
+/* Copies a device matrix back to host arrays (rowptr: nrows + 1 entries; colidx
+   and val: nnz entries, val in the matrix's precision) and waits for the copy.
+   No counterpart in the reference (whose matrices live on the host); for callers
+   and tests that read back what the device built, such as the transpose above. */
+int ks_csr_download(ks_csr A, ks_ctx ctx, int32_t *rowptr, int32_t *colidx, void *val);
 /* The inner solve of shift-and-invert: (A - sigma I) x = b by preconditioned
    conjugate gradients (KSPCG's recurrences, preconditioned residual norm,
    rnorm <= max(rtol |B b|, abstol), at most maxits iterations) with

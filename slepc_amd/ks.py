@@ -87,6 +87,7 @@ def lib():
             "ks_group_step": [vp, i32, vp, i32, dbl], "ks_group_wait_ctx": [vp, vp],
             "ks_ctx_wait_group": [vp, vp],
             "ks_lu_create": [vp, vp, dbl, C.POINTER(vp)], "ks_lu_solve": [vp, vp, vp, vp], "ks_lu_destroy": [vp],
+            "ks_csr_transpose": [vp, vp, C.POINTER(vp)], "ks_csr_download": [vp, vp, vp, vp, vp],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -227,6 +228,26 @@ class CSR:
         check(lib().ks_csr_info(self.h, C.byref(n), C.byref(nnz), C.byref(mr), C.byref(var), C.byref(nb)),
               "ks_csr_info")
         return dict(n=n.value, nnz=nnz.value, max_row=mr.value, variant=var.value, nblocks=nb.value)
+
+    def transpose(self) -> "CSR":
+        """A^T built on the device (ks_csr_transpose)."""
+        h = C.c_void_p()
+        check(lib().ks_csr_transpose(self.h, self.ctx.h, C.byref(h)), "ks_csr_transpose")
+        T = CSR.__new__(CSR)
+        T.ctx, T.h, T.prec = self.ctx, h, self.prec
+        info = T.info()
+        T.n, T.nnz, T.ncols = info["n"], info["nnz"], self.n
+        T.rowptr = T.colidx = T.val = None
+        return T
+
+    def download(self):
+        """(rowptr, colidx, val) read back from the device (ks_csr_download)."""
+        rp = np.zeros(self.n + 1, dtype=np.int32)
+        ci = np.zeros(max(self.nnz, 1), dtype=np.int32)
+        va = np.zeros(max(self.nnz, 1), dtype=np.float64 if self.prec == F64 else np.float32)
+        check(lib().ks_csr_download(self.h, self.ctx.h, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
+                                    va.ctypes.data_as(C.c_void_p)), "ks_csr_download")
+        return rp, ci[:self.nnz], va[:self.nnz]
 
     def free(self):
         if self.h:

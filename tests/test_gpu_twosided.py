@@ -68,6 +68,47 @@ def test_mult_transpose_vs_numpy(E, name):
     A2.destroy()
 
 
+def _random_pattern(nr, nc, seed, dense_rows=(), dense_cols=()):
+    # rows of varied length (some empty), a few dense rows and columns
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(nr):
+        if i in dense_rows:
+            c = np.arange(nc)
+        else:
+            k = int(rng.integers(0, 6)) if i % 7 else 0
+            c = np.unique(rng.integers(0, nc, k))
+            if dense_cols:
+                c = np.union1d(c, [d for d in dense_cols if rng.random() < 0.5])
+        rows.append(np.asarray(c, dtype=np.int64))
+    rowptr = np.zeros(nr + 1, dtype=np.int32)
+    rowptr[1:] = np.cumsum([len(c) for c in rows])
+    col = np.concatenate(rows).astype(np.int32) if rowptr[-1] else np.zeros(0, dtype=np.int32)
+    return rowptr, col, rng.standard_normal(len(col))
+
+
+@pytest.mark.parametrize("nr,nc,kind", [(1, 1, "plain"), (50, 1, "plain"), (3000, 257, "dense"),
+                                        (20000, 70000, "plain"), (5000, 1 << 20, "plain"), (1 << 16, 1 << 16, "dense")])
+def test_transpose_entry_for_entry(nr, nc, kind):
+    # the device transpose (ks_csr_transpose, hand-written radix sort) against scipy's: the
+    # same row pointer, column indices in increasing order and values bit for bit
+    import scipy.sparse as sp
+    from slepc_amd import ks as K
+    dense = dict(dense_rows=(0, nr // 2), dense_cols=(0, nc - 1)) if kind == "dense" else {}
+    csr = _random_pattern(nr, nc, seed=nr + nc, **dense)
+    ctx = K.Context(0)
+    A = K.CSR(ctx, csr, ncols=nc)
+    T = A.transpose()
+    rp, ci, va = T.download()
+    ref = sp.csr_matrix((csr[2], csr[1], csr[0]), shape=(nr, nc)).T.tocsr()
+    ref.sort_indices()
+    print(f"nr={nr} nc={nc} nnz={len(csr[1])}", flush=True)
+    assert np.array_equal(rp, ref.indptr) and np.array_equal(ci, ref.indices) and np.array_equal(va, ref.data)
+    T.free()
+    A.free()
+    ctx.close()
+
+
 @pytest.mark.parametrize("twosided", [False, True])
 @pytest.mark.parametrize("lock", [True, False])
 def test_ex5_markov_golden(E, twosided, lock):
