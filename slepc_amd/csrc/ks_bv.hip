@@ -773,13 +773,30 @@ __device__ __forceinline__ int grp_member(const int *b0, int nmem, int b) {
   return m;
 }
 
-template <int CW, bool FUSED>
+// each workgroup runs its member's sweep with the member's own column split, so
+// a member's arithmetic does not depend on which other solves share the launch
+template <bool FUSED>
 __global__ __launch_bounds__(KS_BLOCK) void k_grp_gs(ks_grp_gs g, int pass) {
   const int m = grp_member(g.b0_gs, g.nmem, blockIdx.x);
   const ks_grp_gs_member &M = g.m[m];
   if (!gs_pass_gate<FUSED>(M.st, M.col, pass)) return;
-  gs_split_block<double, CW, FUSED, 1, false>((int)blockIdx.x - g.b0_gs[m], M.nb, M.V, M.ld, M.k, M.w, M.n, M.rpb,
-                                           M.cbuf, M.partial, gs_prescale<FUSED>(M.st, M.col, pass));
+  const int b = (int)blockIdx.x - g.b0_gs[m];
+  const double pre = gs_prescale<FUSED>(M.st, M.col, pass);
+  switch (M.cw) {
+#define KS_GRP_CW(C)                                                                                                 \
+  case C:                                                                                                            \
+    gs_split_block<double, C, FUSED, 1, false>(b, M.nb, M.V, M.ld, M.k, M.w, M.n, M.rpb, M.cbuf, M.partial, pre);    \
+    break;
+    KS_GRP_CW(2)
+    KS_GRP_CW(4)
+    KS_GRP_CW(8)
+    KS_GRP_CW(12)
+    KS_GRP_CW(16)
+    KS_GRP_CW(24)
+    default:
+      gs_split_block<double, 32, FUSED, 1, false>(b, M.nb, M.V, M.ld, M.k, M.w, M.n, M.rpb, M.cbuf, M.partial, pre);
+#undef KS_GRP_CW
+  }
 }
 
 template <bool FUSED>
@@ -1506,27 +1523,16 @@ int ks_gs_split_geometry(int n, int k, int *cw, int *rpb, int *nb) {
   return KS_OK;
 }
 
-void ks_group_launch_gs(hipStream_t stream, const ks_grp_gs &g, int cw, int refine, double eta) {
+void ks_group_launch_gs(hipStream_t stream, const ks_grp_gs &g, int refine, double eta) {
   const double deftol = 10.0 * 2.220446049250313e-16;
   const int passes = refine == KS_REFINE_NEVER ? 1 : (refine == KS_REFINE_ALWAYS ? 2 : 3);
   const dim3 block(KS_BLOCK), ggs(g.b0_gs[g.nmem]), gfin(g.b0_fin[g.nmem]), gsc(g.b0_sc[g.nmem]);
-#define KS_GRP_SWEEP(FUSED, PASS)                                                                                 \
-  switch (cw) {                                                                                                   \
-    case 2: hipLaunchKernelGGL((k_grp_gs<2, FUSED>), ggs, block, 0, stream, g, PASS); break;                     \
-    case 4: hipLaunchKernelGGL((k_grp_gs<4, FUSED>), ggs, block, 0, stream, g, PASS); break;                     \
-    case 8: hipLaunchKernelGGL((k_grp_gs<8, FUSED>), ggs, block, 0, stream, g, PASS); break;                     \
-    case 12: hipLaunchKernelGGL((k_grp_gs<12, FUSED>), ggs, block, 0, stream, g, PASS); break;                   \
-    case 16: hipLaunchKernelGGL((k_grp_gs<16, FUSED>), ggs, block, 0, stream, g, PASS); break;                   \
-    case 24: hipLaunchKernelGGL((k_grp_gs<24, FUSED>), ggs, block, 0, stream, g, PASS); break;                   \
-    default: hipLaunchKernelGGL((k_grp_gs<32, FUSED>), ggs, block, 0, stream, g, PASS); break;                   \
-  }
-  KS_GRP_SWEEP(false, 0)
+  hipLaunchKernelGGL(k_grp_gs<false>, ggs, block, 0, stream, g, 0);
   hipLaunchKernelGGL((k_grp_fin<false>), gfin, block, 0, stream, g, 0, refine, eta, deftol);
   for (int p = 1; p <= passes; p++) {
-    KS_GRP_SWEEP(true, p)
+    hipLaunchKernelGGL(k_grp_gs<true>, ggs, block, 0, stream, g, p);
     hipLaunchKernelGGL((k_grp_fin<true>), gfin, block, 0, stream, g, p, refine, eta, deftol);
   }
-#undef KS_GRP_SWEEP
   hipLaunchKernelGGL(k_grp_scale, gsc, dim3(256), 0, stream, g);
 }
 

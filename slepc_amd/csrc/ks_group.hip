@@ -69,7 +69,6 @@ int ks_group_step(ks_group g, int nmem, const ks_group_member *mem, ks_refine re
   std::memset(&sp, 0, sizeof(sp));
   std::memset(&gs, 0, sizeof(gs));
   sp.nmem = gs.nmem = nmem;
-  int cw = 2;
   for (int i = 0; i < nmem; i++) {
     const ks_group_member &M = mem[i];
     KS_CHECK(M.ctx && M.A && M.V && M.buf, KS_ERR_ARG, "member %d: null pointer", i);
@@ -85,7 +84,6 @@ int ks_group_step(ks_group g, int nmem, const ks_group_member *mem, ks_refine re
     int e = ks_gs_split_geometry(M.n, M.k, &mcw, &rpb, &nb);
     if (e) return e;
     KS_CHECK(nb <= KS_MAXB, KS_ERR_RANGE, "member %d: vector length %d too long", i, M.n);
-    if (mcw > cw) cw = mcw;
     const double *V = (const double *)M.V;
     double *x = (double *)M.V + (size_t)(M.k - 1) * (size_t)M.ld;
     double *w = (double *)M.V + (size_t)M.k * (size_t)M.ld;
@@ -112,12 +110,13 @@ int ks_group_step(ks_group g, int nmem, const ks_group_member *mem, ks_refine re
     G.rpb = rpb;
     G.nb = nb;
     G.col = M.col;
+    G.cw = mcw;
     gs.b0_gs[i + 1] = gs.b0_gs[i] + nb;
     gs.b0_fin[i + 1] = gs.b0_fin[i] + M.k + 1;
     gs.b0_sc[i + 1] = gs.b0_sc[i] + ks_ew_blocks(M.n);
   }
   ks_group_launch_spmv(g->stream, sp);
-  ks_group_launch_gs(g->stream, gs, cw, (int)refine, eta);
+  ks_group_launch_gs(g->stream, gs, (int)refine, eta);
   KS_HIP(hipGetLastError());
   KS_HIP(hipEventRecord(g->ev[slot], g->stream));
   g->head++;

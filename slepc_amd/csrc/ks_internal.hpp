@@ -188,6 +188,7 @@ struct ks_grp_gs_member {
   unsigned *counter;
   long ld;
   int n, k, rpb, nb, col;
+  int cw;                          // the member's own column split (as when it is grouped alone)
 };
 struct ks_grp_gs {
   int nmem;
@@ -197,7 +198,7 @@ struct ks_grp_gs {
   ks_grp_gs_member m[KS_GROUP_MAX];
 };
 void ks_group_launch_spmv(hipStream_t stream, const ks_grp_spmv &g);
-void ks_group_launch_gs(hipStream_t stream, const ks_grp_gs &g, int cw, int refine, double eta);
+void ks_group_launch_gs(hipStream_t stream, const ks_grp_gs &g, int refine, double eta);
 int ks_gs_split_geometry(int n, int k, int *cw, int *rpb, int *nb);   // KS_ERR_RANGE beyond the split kernel
 int ks_csr_blocks(ks_csr A, const int32_t **blk, const int32_t **rowptr, const int32_t **colidx, const void **val,
                   int *nblocks, ks_prec *vprec);

@@ -70,6 +70,28 @@ def test_serving_alone_vs_mix(E, grouped, driver):
         assert _max_angle_sin(res["X"], alone["X"][:, :k]) < 1e-6
 
 
+def test_serving_mix_bitwise_equals_served_alone(E):
+    # grouped serving gives each member its own column split, so a request served in a mix
+    # of others returns exactly what it returns when it is the only request served
+    from slepc_amd import serve as S
+    reqs = _stream(8, seed=21, nmax=60000)
+    srv = S.Server(budget=2 << 30, max_active=4, grouped=True)
+    mix, _ = S.run_stream_native(srv, reqs, _csr, tol=1e-10)
+    st = srv.stats()
+    srv.close()
+    if E.krylov_extension():
+        assert st["group_member_steps"] > st["group_steps"] > 0
+    for r, res in zip(reqs, mix):
+        one = dict(r, t_arrival=0.0)
+        srv = S.Server(budget=2 << 30, max_active=4, grouped=True)
+        alone, _ = S.run_stream_native(srv, [one], _csr, tol=1e-10)
+        srv.close()
+        a = alone[0]
+        print(f"n={r['n']} nev={r['nev']} its={res['its']}/{a['its']}", flush=True)
+        assert res["reason"] == 1 and a["reason"] == 1 and res["its"] == a["its"]
+        assert np.array_equal(res["eigr"], a["eigr"]) and np.array_equal(res["err"], a["err"])
+
+
 def test_serving_small_requests_vs_lapack(E):
     from slepc_amd import serve as S
     reqs = _stream(6, seed=9, nmax=3000)
