@@ -5,27 +5,31 @@
 // (src/sys/classes/st/interface/stsles.c:47-54, sinvert.c:131-154), a sparse LU.
 // Here the shifted matrix is expanded to a dense ldn x ldn array (ldn = n
 // rounded up to 64, the padding an identity block, so every tile is whole) and
-// factored by the right-looking blocked algorithm of LAPACK's dgetrf with
-// panels of 64 columns:
-//   * panel: one launch per column.  The pivot rows are found and used in
+// factored by the right-looking blocked algorithm of LAPACK's dgetrf: panels
+// of 512 columns, each factored as sub-panels of 64 (dgetrf2's recursion cut
+// at 64 columns), the columns right of a panel updated once per panel:
+//   * sub-panel: one launch per column.  The pivot rows are found and used in
 //     place (implicit pivoting): every workgroup reduces the partial maxima of
 //     column j the previous launch left (largest magnitude, lowest row on
 //     ties), scales its rows of column j by the pivot's reciprocal, updates
-//     their panel columns right of j (dgetf2's dscal and dger) and leaves its
-//     partial maximum of column j+1.  Rows already chosen as pivots are
-//     skipped.  At the panel's end one thread turns the pivot rows into
+//     their sub-panel columns right of j (dgetf2's dscal and dger) and leaves
+//     its partial maximum of column j+1.  Rows already chosen as pivots are
+//     skipped.  At the sub-panel's end one thread turns the pivot rows into
 //     LAPACK's sequence of interchanges, which are then applied to every
 //     column (dlaswp) -- the same rows and values dgetf2 arrives at by
 //     swapping as it goes;
-//   * U12 = L11^-1 A12, one wavefront per column with L11 in LDS (dtrsm);
-//   * A22 -= L21 U12 on v_mfma_f64_16x16x4f64 in 64 x 64 tiles (dgemm), the
-//     operands ordered so that 16 lanes hold 16 consecutive rows of a column
-//     of A22 (128-byte segments for its read and write).
+//   * U12 = L11^-1 A12 per 64-row block, one wavefront per column with L11 in
+//     LDS (dtrsm);
+//   * C -= L U on v_mfma_f64_16x16x4f64 in 64 x 64 tiles (dgemm), k staged
+//     through LDS 32 at a time with the next slice loaded into registers
+//     during the MFMAs, the operands ordered so that 16 lanes hold 16
+//     consecutive rows of a column of C (128-byte reads and writes).
 // The solve applies the row permutation as a gather, then the unit-lower and
-// upper triangular solves block by block, one launch per 64-row block: every
-// workgroup stages the diagonal block in LDS and solves it (one wavefront),
-// then updates its rows of the rest of the vector; the solved block goes to a
-// second vector, so no workgroup reads what another writes in the launch.
+// upper triangular solves block by block, one launch per 64-row block:
+// wavefront 0 of every workgroup solves the diagonal block from registers
+// while the others load their rows of the off-diagonal block, then they update
+// those rows; the solved block goes to a second vector, so no workgroup reads
+// what another writes in the launch.
 // Memory is n^2 doubles (for n = 32,768: 8.6 GB), so the factorization serves
 // matrices of up to that size; larger ones take an iterative inner solver.
 #include <vector>
@@ -35,6 +39,7 @@
 namespace {
 
 #define KS_LU_NB 64
+#define KS_LU_NB2 512       // columns of a panel of the trailing update (a multiple of 64)
 #define KS_LU_PG_MAX 512   // most workgroups of a panel step (ldn <= 131,072)
 // a pivot below this magnitude is a zero pivot (PETSc's default zeropivot, 100 eps)
 #define KS_LU_ZEROPIVOT (100.0 * 2.220446049250313e-16)
@@ -203,12 +208,13 @@ __global__ __launch_bounds__(64) void k_lu_ipiv(const int *__restrict__ prow, in
   }
 }
 
-// the panel's interchanges (rows j0..j0+NB-1 with ipiv, in order) on every column
+// the panel's interchanges (rows j0..j0+NB-1 with ipiv, in order) on the columns
+// cb .. cb+nc-1, a thread per column
 __global__ __launch_bounds__(KS_BLOCK) void k_lu_laswp(double *__restrict__ A, int ldn, int j0,
-                                                      const int *__restrict__ ipiv) {
-  const int c = blockIdx.x * KS_BLOCK + threadIdx.x;
-  if (c >= ldn) return;
-  double *a = A + (size_t)c * ldn;
+                                                      const int *__restrict__ ipiv, int cb, int nc) {
+  const int t = blockIdx.x * KS_BLOCK + threadIdx.x;
+  if (t >= nc) return;
+  double *a = A + (size_t)(cb + t) * ldn;
   for (int j = j0; j < j0 + KS_LU_NB; j++) {
     const int p = ipiv[j];
     if (p != j) {
@@ -219,9 +225,15 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_laswp(double *__restrict__ A, i
   }
 }
 
-// U12 = L11^-1 A12 for the columns right of the panel: a wavefront per column,
-// lane i holding row j0+i; L11 (unit lower) staged in LDS
-__global__ __launch_bounds__(KS_BLOCK) void k_lu_trsm(double *__restrict__ A, int ldn, int j0) {
+__device__ inline double readlane_f64(double v, int k) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)bits, k), hi = __builtin_amdgcn_readlane((int)(bits >> 32), k);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// U12 = L11^-1 A12 for the columns cb .. cb+ncols-1: a wavefront per column,
+// lane i holding row j0+i; L11 (the unit lower block at (j0, j0)) staged in LDS
+__global__ __launch_bounds__(KS_BLOCK) void k_lu_trsm(double *__restrict__ A, int ldn, int j0, int cb, int ncols) {
   __shared__ double L[KS_LU_NB][KS_LU_NB + 1];   // L[k][i] = A(j0+i, j0+k)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int e = tid; e < KS_LU_NB * KS_LU_NB; e += KS_BLOCK) {
@@ -229,12 +241,11 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_trsm(double *__restrict__ A, in
     L[k][i] = A[(size_t)(j0 + i) + (size_t)(j0 + k) * ldn];
   }
   __syncthreads();
-  const int ncols = ldn - j0 - KS_LU_NB;
   for (int cc = blockIdx.x * KS_WAVES + wave; cc < ncols; cc += gridDim.x * KS_WAVES) {
-    double *a = A + (size_t)(j0 + KS_LU_NB + cc) * ldn + j0;
+    double *a = A + (size_t)(cb + cc) * ldn + j0;
     double x = a[lane];
     for (int k = 0; k < KS_LU_NB - 1; k++) {
-      const double xk = __shfl(x, k, 64);
+      const double xk = readlane_f64(x, k);
       if (lane > k) x -= L[k][lane] * xk;
     }
     a[lane] = x;
@@ -243,22 +254,24 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_trsm(double *__restrict__ A, in
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// A22 -= L21 U12, one 64 x 64 tile of A22 per workgroup: L21 and U12 staged
-// in LDS in two halves of 32 along k (33 KB, several workgroups a CU), each
+// C -= L U for the 64 x 64 tile (rb + 64 bx, cb + 64 by) of A, with L the
+// columns kb .. kb+K-1 of the tile's rows and U the rows kb .. kb+K-1 of its
+// columns (the trailing update A22 -= L21 U12 of the factorization).  L and U
+// are staged in LDS 32 along k at a time (33 KB, several workgroups a CU), each
 // wavefront a 32 x 32 quarter (2 x 2 MFMA tiles).  The MFMA's first operand is
-// U12 (its 16 rows the tile's columns), the second L21 (its 16 columns the
-// tile's rows), so register q of lane l holds column l/16 + 4q, row l%16 of a
-// 16 x 16 tile of A22: 16 lanes read and write 128 contiguous bytes.
+// U (its 16 rows the tile's columns), the second L (its 16 columns the tile's
+// rows), so register q of lane l holds column l/16 + 4q, row l%16 of a 16 x 16
+// tile: 16 lanes read and write 128 contiguous bytes.  K is a multiple of 32.
 #define KS_LU_KH 32
-__global__ __launch_bounds__(KS_BLOCK) void k_lu_gemm(double *__restrict__ A, int ldn, int j0) {
-  __shared__ double sL[KS_LU_KH][KS_LU_NB + 1];   // sL[k][r] = L21(r, h + k)
-  __shared__ double sU[KS_LU_KH][KS_LU_NB + 1];   // sU[k][c] = U12(h + k, c)
+__global__ __launch_bounds__(KS_BLOCK) void k_lu_gemm(double *__restrict__ A, int ldn, int kb, int K, int rb,
+                                                     int cb) {
+  __shared__ double sL[KS_LU_KH][KS_LU_NB + 1];   // sL[k][r] = L(r, h + k)
+  __shared__ double sU[KS_LU_KH][KS_LU_NB + 1];   // sU[k][c] = U(h + k, c)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int base = j0 + KS_LU_NB;
-  const int r0 = base + blockIdx.x * KS_LU_NB, c0 = base + blockIdx.y * KS_LU_NB;
+  const int r0 = rb + blockIdx.x * KS_LU_NB, c0 = cb + blockIdx.y * KS_LU_NB;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
-  // this lane's 16 entries of A22, read before the operands are staged
+  // this lane's 16 entries of C, read before the operands are staged
   double cv[2][2][4];
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -272,15 +285,31 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_gemm(double *__restrict__ A, in
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-  for (int h = 0; h < KS_LU_NB; h += KS_LU_KH) {
+  // the operands of a 32-wide k slice: thread e-slots tid, tid + 256, ... (8 each);
+  // the next slice is loaded into registers while the MFMAs of the current one run
+  constexpr int PER = KS_LU_NB * KS_LU_KH / KS_BLOCK;
+  double pl[PER], pu[PER];
+  auto load = [&](int h) {
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int e = tid + u * KS_BLOCK;
+      const int i = e % KS_LU_NB, k = e / KS_LU_NB;     // L: 64 rows of a column
+      pl[u] = A[(size_t)(r0 + i) + (size_t)(kb + h + k) * ldn];
+      const int kk = e % KS_LU_KH, c = e / KS_LU_KH;   // U: 32 rows of a column
+      pu[u] = A[(size_t)(kb + h + kk) + (size_t)(c0 + c) * ldn];
+    }
+  };
+  load(0);
+  for (int h = 0; h < K; h += KS_LU_KH) {
     if (h) __syncthreads();
-    for (int e = tid; e < KS_LU_NB * KS_LU_KH; e += KS_BLOCK) {
-      const int i = e % KS_LU_NB, k = e / KS_LU_NB;     // L21: 64 rows of a column
-      sL[k][i] = A[(size_t)(r0 + i) + (size_t)(j0 + h + k) * ldn];
-      const int kk = e % KS_LU_KH, c = e / KS_LU_KH;   // U12: 32 rows of a column
-      sU[kk][c] = A[(size_t)(j0 + h + kk) + (size_t)(c0 + c) * ldn];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int e = tid + u * KS_BLOCK;
+      sL[e / KS_LU_NB][e % KS_LU_NB] = pl[u];
+      sU[e % KS_LU_KH][e / KS_LU_KH] = pu[u];
     }
     __syncthreads();
+    if (h + KS_LU_KH < K) load(h + KS_LU_KH);
 #pragma unroll
     for (int kk = 0; kk < KS_LU_KH / 4; kk++) {
       const int k = 4 * kk + lk;
@@ -307,12 +336,6 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_permute(const int *__restrict__
                                                         int n, int ldn, double *__restrict__ x) {
   for (int i = blockIdx.x * KS_BLOCK + threadIdx.x; i < ldn; i += gridDim.x * KS_BLOCK)
     x[i] = i < n ? b[perm[i]] : 0.0;
-}
-
-__device__ inline double readlane_f64(double v, int k) {
-  const long long bits = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)bits, k), hi = __builtin_amdgcn_readlane((int)(bits >> 32), k);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 #define KS_LU_TRI_ROWS (KS_BLOCK - 64)   // rows a workgroup updates (wavefronts 1..3)
@@ -437,29 +460,58 @@ extern "C" int ks_lu_create(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out) {
   const int fill_blocks = (ldn + KS_BLOCK - 1) / KS_BLOCK;
   hipLaunchKernelGGL(k_lu_fill, dim3(fill_blocks), dim3(KS_BLOCK), 0, st, A->d_rowptr, A->d_colidx,
                      (const double *)A->d_val, n, ldn, sigma, F->d_A);
-  for (int j0 = 0; j0 < ldn; j0 += KS_LU_NB) {
-    const int rows = ldn - j0, G = (rows + KS_BLOCK - 1) / KS_BLOCK;
-    hipLaunchKernelGGL(k_lu_colmax, dim3(G), dim3(KS_BLOCK), 0, st, (const double *)F->d_A, ldn, j0, d_pm, d_pi);
-    for (int j = j0; j < j0 + KS_LU_NB; j++)
-      hipLaunchKernelGGL(k_lu_step, dim3(G), dim3(KS_BLOCK), 0, st, F->d_A, ldn, j0, j, d_used, d_pm, d_pi, d_prow,
-                         d_flag);
-    hipLaunchKernelGGL(k_lu_ipiv, dim3(1), dim3(KS_LU_NB), 0, st, (const int *)d_prow, j0, F->d_ipiv);
-    hipLaunchKernelGGL(k_lu_laswp, dim3(fill_blocks), dim3(KS_BLOCK), 0, st, F->d_A, ldn, j0,
-                       (const int *)F->d_ipiv);
-    const int rest = ldn - j0 - KS_LU_NB;
-    if (rest > 0) {
-      const int tb = (rest + KS_WAVES - 1) / KS_WAVES;
-      hipLaunchKernelGGL(k_lu_trsm, dim3(tb < 1024 ? tb : 1024), dim3(KS_BLOCK), 0, st, F->d_A, ldn, j0);
-      hipLaunchKernelGGL(k_lu_gemm, dim3(rest / KS_LU_NB, rest / KS_LU_NB), dim3(KS_BLOCK), 0, st, F->d_A, ldn, j0);
+  // panels of KS_LU_NB2 columns, each factored as sub-panels of KS_LU_NB (dgetrf2's
+  // recursion cut at 64): a sub-panel is first brought up to date with the earlier
+  // sub-panels of its panel, and the columns right of the panel take the whole
+  // panel's update at once (K = KS_LU_NB2), so A22 is read and written once per
+  // KS_LU_NB2 columns
+  auto trsm = [&](int j0, int cb, int ncols) {
+    const int tb = (ncols + KS_WAVES - 1) / KS_WAVES;
+    hipLaunchKernelGGL(k_lu_trsm, dim3(tb < 1024 ? tb : 1024), dim3(KS_BLOCK), 0, st, F->d_A, ldn, j0, cb, ncols);
+  };
+  auto gemm = [&](int kb, int K, int rb, int nrows, int cb, int ncols) {
+    if (nrows > 0 && ncols > 0)
+      hipLaunchKernelGGL(k_lu_gemm, dim3(nrows / KS_LU_NB, ncols / KS_LU_NB), dim3(KS_BLOCK), 0, st, F->d_A, ldn, kb,
+                         K, rb, cb);
+  };
+  for (int J0 = 0; J0 < ldn; J0 += KS_LU_NB2) {
+    const int W = ldn - J0 < KS_LU_NB2 ? ldn - J0 : KS_LU_NB2;
+    for (int s = 0; s < W; s += KS_LU_NB) {
+      const int j0 = J0 + s;
+      if (s > 0) {   // the sub-panel's rows J0 .. j0-1 become U, its rows below take the update
+        for (int t = 0; t < s; t += KS_LU_NB) {
+          trsm(J0 + t, j0, KS_LU_NB);
+          gemm(J0 + t, KS_LU_NB, J0 + t + KS_LU_NB, s - t - KS_LU_NB, j0, KS_LU_NB);
+        }
+        gemm(J0, s, j0, ldn - j0, j0, KS_LU_NB);
+      }
+      const int G = (ldn - j0 + KS_BLOCK - 1) / KS_BLOCK;
+      hipLaunchKernelGGL(k_lu_colmax, dim3(G), dim3(KS_BLOCK), 0, st, (const double *)F->d_A, ldn, j0, d_pm, d_pi);
+      for (int j = j0; j < j0 + KS_LU_NB; j++)
+        hipLaunchKernelGGL(k_lu_step, dim3(G), dim3(KS_BLOCK), 0, st, F->d_A, ldn, j0, j, d_used, d_pm, d_pi, d_prow,
+                           d_flag);
+      hipLaunchKernelGGL(k_lu_ipiv, dim3(1), dim3(KS_LU_NB), 0, st, (const int *)d_prow, j0, F->d_ipiv);
+      hipLaunchKernelGGL(k_lu_laswp, dim3(fill_blocks), dim3(KS_BLOCK), 0, st, F->d_A, ldn, j0,
+                         (const int *)F->d_ipiv, 0, ldn);
+    }
+    const int cb = J0 + W, nc = ldn - cb;
+    if (nc > 0) {   // U12 of the panel's rows, then A22 -= L21 U12 with K = W
+      for (int t = 0; t < W; t += KS_LU_NB) {
+        trsm(J0 + t, cb, nc);
+        gemm(J0 + t, KS_LU_NB, J0 + t + KS_LU_NB, W - t - KS_LU_NB, cb, nc);
+      }
+      gemm(J0, W, cb, ldn - cb, cb, nc);
     }
   }
-  hipError_t herr = hipGetLastError();
+  hipError_t herr = hipSuccess;
+  if (herr == hipSuccess) herr = hipGetLastError();
   // the row permutation P (P A = L U): the interchanges applied in order to 0..ldn-1
   std::vector<int> ipiv(ldn), perm(ldn);
   int flag = 0;
   if (herr == hipSuccess) herr = hipMemcpyAsync(ipiv.data(), F->d_ipiv, sizeof(int) * ldn, hipMemcpyDeviceToHost, st);
   if (herr == hipSuccess) herr = hipMemcpyAsync(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, st);
   if (herr == hipSuccess) herr = hipStreamSynchronize(st);
+  else (void)hipStreamSynchronize(st);   // nothing may still run on what is freed below
   (void)ks_free(scratch);
   if (herr != hipSuccess) {
     ks_lu_destroy(F);
