@@ -157,18 +157,27 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_step(double *__restrict__ A, in
   const int stamp = j0 + 1;
   double m2 = -1.0;
   int i2 = INT_MAX;
+  const int nc = jend - j;   // columns j .. jend-1 of the row
   for (int r = j0 + blockIdx.x * KS_BLOCK + tid; r < ldn; r += G * KS_BLOCK) {
-    if (used[r] == stamp) continue;
+    // the row's flag and its nc values in one round of independent loads
+    double *row = A + r;
+    const int u = used[r];
+    double v[KS_LU_NB];
+#pragma unroll
+    for (int c = 0; c < KS_LU_NB; c++)
+      if (c < nc) v[c] = row[(size_t)(j + c) * ldn];
+    if (u == stamp) continue;
     if (r == p) {
       used[r] = stamp;
       continue;
     }
-    double *row = A + r;
-    const double l = row[(size_t)j * ldn] * rinv;
+    const double l = v[0] * rinv;
     row[(size_t)j * ldn] = l;
-    for (int c = j + 1; c < jend; c++) row[(size_t)c * ldn] -= l * prv[c - j];
-    if (j + 1 < jend) {
-      const double a = piv_mag(row[(size_t)(j + 1) * ldn]);
+#pragma unroll
+    for (int c = 1; c < KS_LU_NB; c++)
+      if (c < nc) row[(size_t)(j + c) * ldn] = v[c] - l * prv[c];
+    if (nc > 1) {
+      const double a = piv_mag(v[1] - l * prv[1]);
       if (piv_better(a, r, m2, i2)) {
         m2 = a;
         i2 = r;
