@@ -92,8 +92,9 @@ __device__ inline void piv_reduce(double &m, int &i, double *sm, int *si) {
       }
 }
 
-// Rows of a panel step: thread t of workgroup g owns rows j0 + g * KS_BLOCK + t +
-// s * G * KS_BLOCK (s = 0, 1, ...), the same rows for every column of the panel.
+// Rows of a panel step: thread t of workgroup g owns row j0 + g * KS_BLOCK + t (the
+// grid has G = ceil((ldn - j0) / KS_BLOCK) workgroups), the same row for every
+// column of the panel.
 // used[r] == j0 + 1: row r is the pivot of an earlier column of this panel.
 // pm/pi[slot][g]: workgroup g's largest magnitude (and its row) of a column.
 
@@ -130,6 +131,20 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_step(double *__restrict__ A, in
   __shared__ int si[KS_WAVES];
   __shared__ int sp;
   const int tid = threadIdx.x, G = gridDim.x, slot = j & 1;
+  const int jend = j0 + KS_LU_NB, nc = jend - j, stamp = j0 + 1;
+  // this thread's row (the grid covers rows j0 .. ldn-1 one row a thread): its flag
+  // and its nc values (columns j .. jend-1) loaded in the same round trip as the
+  // partial maxima
+  const int r = j0 + blockIdx.x * KS_BLOCK + tid;
+  const bool have = r < ldn;
+  int u = stamp;
+  double v[KS_LU_NB];
+  if (have) {
+    u = used[r];
+#pragma unroll
+    for (int c = 0; c < KS_LU_NB; c++)
+      if (c < nc) v[c] = A[(size_t)r + (size_t)(j + c) * ldn];
+  }
   double m = -1.0;
   int idx = INT_MAX;
   for (int g = tid; g < G; g += KS_BLOCK) {
@@ -150,36 +165,23 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_step(double *__restrict__ A, in
   }
   __syncthreads();
   const int p = sp;   // a candidate row always exists: rows j..ldn-1 outnumber the panel's pivots
-  const int jend = j0 + KS_LU_NB;
-  if (tid < jend - j) prv[tid] = A[(size_t)p + (size_t)(j + tid) * ldn];
+  if (tid < nc) prv[tid] = A[(size_t)p + (size_t)(j + tid) * ldn];
   __syncthreads();
   const double rinv = 1.0 / prv[0];
-  const int stamp = j0 + 1;
   double m2 = -1.0;
   int i2 = INT_MAX;
-  const int nc = jend - j;   // columns j .. jend-1 of the row
-  for (int r = j0 + blockIdx.x * KS_BLOCK + tid; r < ldn; r += G * KS_BLOCK) {
-    // the row's flag and its nc values in one round of independent loads
-    double *row = A + r;
-    const int u = used[r];
-    double v[KS_LU_NB];
-#pragma unroll
-    for (int c = 0; c < KS_LU_NB; c++)
-      if (c < nc) v[c] = row[(size_t)(j + c) * ldn];
-    if (u == stamp) continue;
+  if (have && u != stamp) {
     if (r == p) {
       used[r] = stamp;
-      continue;
-    }
-    const double l = v[0] * rinv;
-    row[(size_t)j * ldn] = l;
+    } else {
+      double *row = A + r;
+      const double l = v[0] * rinv;
+      row[(size_t)j * ldn] = l;
 #pragma unroll
-    for (int c = 1; c < KS_LU_NB; c++)
-      if (c < nc) row[(size_t)(j + c) * ldn] = v[c] - l * prv[c];
-    if (nc > 1) {
-      const double a = piv_mag(v[1] - l * prv[1]);
-      if (piv_better(a, r, m2, i2)) {
-        m2 = a;
+      for (int c = 1; c < KS_LU_NB; c++)
+        if (c < nc) row[(size_t)(j + c) * ldn] = v[c] - l * prv[c];
+      if (nc > 1) {
+        m2 = piv_mag(v[1] - l * prv[1]);
         i2 = r;
       }
     }
