@@ -800,12 +800,13 @@ __global__ __launch_bounds__(KS_BLOCK) void k_grp_gs(ks_grp_gs g, int pass) {
 }
 
 template <bool FUSED>
-__global__ __launch_bounds__(KS_BLOCK) void k_grp_fin(ks_grp_gs g, int pass, int refine, double eta, double deftol) {
+__global__ __launch_bounds__(KS_BLOCK) void k_grp_fin(ks_grp_gs g, int pass, int refine, double eta, double deftol,
+                                                     int halt_pass) {
   const int m = grp_member(g.b0_fin, g.nmem, blockIdx.x);
   const ks_grp_gs_member &M = g.m[m];
   if (!gs_pass_gate<FUSED>(M.st, M.col, pass)) return;
   gs_fin_block<FUSED, KS_BLOCK>((int)blockIdx.x - g.b0_fin[m], M.partial, M.nb, M.k, M.sums, M.counter, M.cbuf, M.hcol, M.st,
-                      M.col, refine, eta, 1, 0, deftol, 1, 0);
+                      M.col, refine, eta, 1, 0, deftol, 1, halt_pass);
 }
 
 __global__ void k_grp_scale(ks_grp_gs g) {
@@ -1523,17 +1524,25 @@ int ks_gs_split_geometry(int n, int k, int *cw, int *rpb, int *nb) {
   return KS_OK;
 }
 
+// The lean sequence of the single-solve path (with refinement on): the first
+// pass, then two fused passes whose last sweep stores the normalized column; a
+// member that needs a third pass or a separate scaling stops its expansion there
+// (the finish of pass 2 halts it, later gated launches exit), and BVMatKrylovEnd
+// completes that column and runs the rest of its expansion on its own stream.
+// Seven launches a grouped step with the SpMV instead of ten.
 void ks_group_launch_gs(hipStream_t stream, const ks_grp_gs &g, int refine, double eta) {
   const double deftol = 10.0 * 2.220446049250313e-16;
+  const bool lean = refine != KS_REFINE_NEVER;
   const int passes = refine == KS_REFINE_NEVER ? 1 : (refine == KS_REFINE_ALWAYS ? 2 : 3);
+  const int last = lean ? 2 : passes, halt_pass = lean ? 2 : 0;
   const dim3 block(KS_BLOCK), ggs(g.b0_gs[g.nmem]), gfin(g.b0_fin[g.nmem]), gsc(g.b0_sc[g.nmem]);
   hipLaunchKernelGGL(k_grp_gs<false>, ggs, block, 0, stream, g, 0);
-  hipLaunchKernelGGL((k_grp_fin<false>), gfin, block, 0, stream, g, 0, refine, eta, deftol);
-  for (int p = 1; p <= passes; p++) {
+  hipLaunchKernelGGL((k_grp_fin<false>), gfin, block, 0, stream, g, 0, refine, eta, deftol, 0);
+  for (int p = 1; p <= last; p++) {
     hipLaunchKernelGGL(k_grp_gs<true>, ggs, block, 0, stream, g, p);
-    hipLaunchKernelGGL((k_grp_fin<true>), gfin, block, 0, stream, g, p, refine, eta, deftol);
+    hipLaunchKernelGGL((k_grp_fin<true>), gfin, block, 0, stream, g, p, refine, eta, deftol, halt_pass);
   }
-  hipLaunchKernelGGL(k_grp_scale, gsc, dim3(256), 0, stream, g);
+  if (!lean) hipLaunchKernelGGL(k_grp_scale, gsc, dim3(256), 0, stream, g);
 }
 
 extern "C" {
