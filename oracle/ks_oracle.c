@@ -24,6 +24,13 @@
  *   EPS driver                src/eps/interface/epssolve.c:119-208,666-873
  *                             src/eps/interface/epssetup.c:286-418,648-732
  *                             src/eps/interface/epsdefault.c:28-50,105-169,224-302
+ *   shift-and-invert          src/sys/classes/st/impls/sinvert/sinvert.c:16-105,
+ *                             src/sys/classes/st/interface/stsolve.c:244-262, stsles.c:47-54
+ *   its inner solves          PETSc (not in the reference tree), restated from the
+ *                             published methods: KSPCG, KSPBCGS, PCILU levels 0 (Saad,
+ *                             Algorithm 10.4) and KSPPREONLY + PCLU as LAPACK's
+ *                             dgetf2/dgetrs (the LU pinned to scipy's dgetrf: same
+ *                             pivots and factors), the eigen-results pinned by goldens
  *
  * Third-party arithmetic at the boundary: BLAS/LAPACK (MKL via libmkl_rt,
  * LP64) for gemv/gemm/nrm2/lartg/rot/steqr/gehrd/orghr/hseqr/trexc/trevc/lascl,
