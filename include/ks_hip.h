@@ -195,6 +195,11 @@ int ks_bcgs_solve_ilu(ks_ctx ctx, ks_csr A, double sigma, ks_ilu M, const double
 int ks_lu_create(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out);
 int ks_lu_solve(ks_ctx ctx, ks_lu F, const double *d_b, double *d_x);
 int ks_lu_destroy(ks_lu F);
+/* Waits for the context's stream and reports whether every solve so far ran
+   clean (the triangular solves' bounded waits never ran out): KS_OK or
+   KS_ERR_HIP.  No counterpart in the reference (its LU solves are synchronous
+   host code); for callers that check the asynchronous solves, such as tests. */
+int ks_lu_check(ks_ctx ctx, ks_lu F);
 /* nrows, nnz, max row length and the kernel variant chosen (0 stream, 1 vector) */
 int ks_csr_info(ks_csr A, int *nrows, int64_t *nnz, int *max_row, int *variant, int *nblocks);
 /* y = A x.  x, y device vectors in xprec.  When gate != 0 the product is

@@ -24,12 +24,11 @@
 //     through LDS 32 at a time with the next slice loaded into registers
 //     during the MFMAs, the operands ordered so that 16 lanes hold 16
 //     consecutive rows of a column of C (128-byte reads and writes).
-// The solve applies the row permutation as a gather, then the unit-lower and
-// upper triangular solves block by block, one launch per 64-row block:
-// wavefront 0 of every workgroup solves the diagonal block from registers
-// while the others load their rows of the off-diagonal block, then they update
-// those rows; the solved block goes to a second vector, so no workgroup reads
-// what another writes in the launch.
+// The solve is two launches, the unit-lower and the upper triangular solve,
+// each with one workgroup per 64-row block row, pipelined through per-block
+// flags (k_lu_tri_sf): a workgroup accumulates the blocks left (right) of its
+// diagonal as their solutions are published, then solves its diagonal block and
+// publishes it; the row permutation is applied as a gather on the way in.
 // Memory is n^2 doubles (for n = 32,768: 8.6 GB), so the factorization serves
 // matrices of up to that size; larger ones take an iterative inner solver.
 #include <vector>
@@ -342,37 +341,83 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_gemm(double *__restrict__ A, in
             cv[a][b][q] - acc[a][b][q];
 }
 
-// x(i) = b(perm(i)) for i < n, 0 in the padding
-__global__ __launch_bounds__(KS_BLOCK) void k_lu_permute(const int *__restrict__ perm, const double *__restrict__ b,
-                                                        int n, int ldn, double *__restrict__ x) {
-  for (int i = blockIdx.x * KS_BLOCK + threadIdx.x; i < ldn; i += gridDim.x * KS_BLOCK)
-    x[i] = i < n ? b[perm[i]] : 0.0;
-}
-
-#define KS_LU_TRI_ROWS (KS_BLOCK - 64)   // rows a workgroup updates (wavefronts 1..3)
-
-// One block step of a triangular solve.  Wavefront 0 solves the 64 x 64
-// diagonal block at b0 of L (unit lower, LOWER) or U from xin(b0..b0+63), lane
-// i holding row i of the block in registers and the solved entries passed on
-// by readlane (column-oriented substitution, the order of LAPACK's dtrsv);
-// workgroup 0 writes the block to out(b0..) (its rows below n_out).  The other
-// wavefronts load their rows of the rest, r >= b0 + 64 (LOWER) or r < b0,
-// meanwhile and then take xin(r) -= A(r, b0:b0+64) x_block (in the order of k
-// that dtrsv follows).  xin's block rows are read and never written in the
-// launch, and out is not xin.
+// One launch for a whole triangular solve, block rows pipelined through flags.
+// Workgroup g takes block row i = g (forward, L) or i = nblk-1-g (backward, U),
+// so a workgroup only waits for lower-numbered ones (dispatched before it).  For
+// each solved block j (j < i forward, j > i backward, in the order they are
+// solved) it loads its 64 x 64 block L(i, j) or U(i, j) -- issued before the wait
+// -- waits until block j is published, and accumulates the block times x_j:
+// thread t takes row t%64 and 16 of the 64 columns.  Then wavefront 0 solves the
+// diagonal block from registers (column-oriented, in dtrsv's order within the
+// block), writes x_i to xs and the output, and the block is published: a release
+// store of the solve's epoch into flag[i].  Waits poll flag[j] with agent-scope
+// loads (coherent across the XCDs' caches) followed by one acquire fence, and are
+// bounded: past
+// KS_LU_SPIN polls the workgroup sets *err and goes on, so every workgroup exits.
+// Forward: the right-hand side is b gathered through the row permutation
+// (rows >= n zero); backward: the right-hand side is the forward solve's xs.
+#define KS_LU_SPIN (1 << 22)
 template <bool LOWER>
-__global__ __launch_bounds__(KS_BLOCK) void k_lu_tri_step(const double *__restrict__ A, int ldn, int b0, int n_out,
-                                                         double *__restrict__ xin, double *__restrict__ out) {
-  __shared__ double xb[KS_LU_NB];
+__global__ __launch_bounds__(KS_BLOCK) void k_lu_tri_sf(const double *__restrict__ A, int ldn, int n,
+                                                       const int *__restrict__ perm, const double *__restrict__ rhs,
+                                                       double *xs, double *__restrict__ out, int *flag, int epoch,
+                                                       int *err) {
+  __shared__ double part[KS_WAVES][KS_LU_NB];
+  __shared__ double xj[KS_LU_NB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int r0 = LOWER ? b0 + KS_LU_NB : 0, r1 = LOWER ? ldn : b0;
-  const int r = r0 + blockIdx.x * KS_LU_TRI_ROWS + tid - 64;   // this thread's row (wavefronts 1..3)
-  double arow[KS_LU_NB], s = 0.0;
+  const int nblk = ldn / KS_LU_NB;
+  const int i = LOWER ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
+  const int r0 = i * KS_LU_NB, row = r0 + lane, cq = wave * 16;   // this thread's row and 16 columns
+  double acc = 0.0;
+  const int jn = LOWER ? i : nblk - 1 - i;   // blocks to take, in solve order
+  // wavefront 0's diagonal block, loaded before the waits
+  double d[KS_LU_NB];
   if (wave == 0) {
-    double d[KS_LU_NB];
 #pragma unroll
-    for (int k = 0; k < KS_LU_NB; k++) d[k] = A[(size_t)(b0 + lane) + (size_t)(b0 + k) * ldn];
-    double xi = xin[b0 + lane];
+    for (int k = 0; k < KS_LU_NB; k++) d[k] = A[(size_t)row + (size_t)(r0 + k) * ldn];
+  }
+  double a[16];
+  auto load_block = [&](int j) {
+#pragma unroll
+    for (int c = 0; c < 16; c++) a[c] = A[(size_t)row + (size_t)(j * KS_LU_NB + cq + c) * ldn];
+  };
+  if (jn > 0) load_block(LOWER ? 0 : nblk - 1);
+  for (int t = 0; t < jn; t++) {
+    const int j = LOWER ? t : nblk - 1 - t;
+    if (tid == 0) {
+      int spins = 0;
+      // coherent polls without cache invalidation, then one acquire fence
+      while (__hip_atomic_load(&flag[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+        if (++spins > KS_LU_SPIN) {
+          atomicOr(err, 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    if (tid < KS_LU_NB) xj[tid] = xs[j * KS_LU_NB + tid];
+    __syncthreads();
+    double cur[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) cur[c] = a[c];
+    if (t + 1 < jn) load_block(LOWER ? t + 1 : nblk - 2 - t);   // the next block in flight meanwhile
+    if (LOWER) {
+#pragma unroll
+      for (int c = 0; c < 16; c++) acc += cur[c] * xj[cq + c];
+    } else {
+#pragma unroll
+      for (int c = 15; c >= 0; c--) acc += cur[c] * xj[cq + c];
+    }
+  }
+  part[wave][lane] = acc;
+  __syncthreads();
+  if (wave == 0) {
+    const double b = LOWER ? (row < n ? rhs[perm[row]] : 0.0) : rhs[row];
+    const double sum = LOWER ? ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]
+                             : ((part[3][lane] + part[2][lane]) + part[1][lane]) + part[0][lane];
+    double xi = b - sum;
     if (LOWER) {
 #pragma unroll
       for (int k = 0; k < KS_LU_NB - 1; k++) {
@@ -387,34 +432,11 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_tri_step(const double *__restri
         if (lane < k) xi -= d[k] * xk;
       }
     }
-    xb[lane] = xi;
-    if (blockIdx.x == 0 && b0 + lane < n_out) out[b0 + lane] = xi;
-  } else if (r < r1) {
-    s = xin[r];
-#pragma unroll
-    for (int k = 0; k < KS_LU_NB; k++) arow[k] = A[(size_t)r + (size_t)(b0 + k) * ldn];
+    xs[row] = xi;
+    if (out && row < n) out[row] = xi;
   }
   __syncthreads();
-  if (wave == 0) return;
-  if (r < r1) {
-    if (LOWER) {
-#pragma unroll
-      for (int k = 0; k < KS_LU_NB; k++) s -= arow[k] * xb[k];
-    } else {
-#pragma unroll
-      for (int k = KS_LU_NB - 1; k >= 0; k--) s -= arow[k] * xb[k];
-    }
-    xin[r] = s;
-  }
-  for (int rr = r + gridDim.x * KS_LU_TRI_ROWS; rr < r1; rr += gridDim.x * KS_LU_TRI_ROWS) {
-    double t = xin[rr];
-    if (LOWER) {
-      for (int k = 0; k < KS_LU_NB; k++) t -= A[(size_t)rr + (size_t)(b0 + k) * ldn] * xb[k];
-    } else {
-      for (int k = KS_LU_NB - 1; k >= 0; k--) t -= A[(size_t)rr + (size_t)(b0 + k) * ldn] * xb[k];
-    }
-    xin[rr] = t;
-  }
+  if (tid == 0) __hip_atomic_store(&flag[i], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace
@@ -423,6 +445,8 @@ struct ks_lu_s {
   int n = 0, ldn = 0;
   double *d_A = nullptr, *d_x = nullptr, *d_y = nullptr;
   int *d_ipiv = nullptr, *d_perm = nullptr;
+  int *d_flags = nullptr;   // [2][nblk] publication epochs of the two solves, then the error word
+  int epoch = 0;
 };
 
 extern "C" int ks_lu_destroy(ks_lu F) {
@@ -432,6 +456,7 @@ extern "C" int ks_lu_destroy(ks_lu F) {
   if (F->d_y) (void)ks_free(F->d_y);
   if (F->d_ipiv) (void)ks_free(F->d_ipiv);
   if (F->d_perm) (void)ks_free(F->d_perm);
+  if (F->d_flags) (void)ks_free(F->d_flags);
   delete F;
   return KS_OK;
 }
@@ -457,6 +482,8 @@ extern "C" int ks_lu_create(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out) {
   if (!e) e = ks_malloc((void **)&F->d_y, sizeof(double) * ldn);
   if (!e) e = ks_malloc((void **)&F->d_ipiv, sizeof(int) * ldn);
   if (!e) e = ks_malloc((void **)&F->d_perm, sizeof(int) * ldn);
+  if (!e) e = ks_malloc((void **)&F->d_flags, sizeof(int) * (2 * (ldn / KS_LU_NB) + 1));
+  if (!e) e = ks_memset_zero(ctx, F->d_flags, sizeof(int) * (2 * (ldn / KS_LU_NB) + 1));
   if (!e) e = ks_malloc(&scratch, scratch_bytes);
   if (!e) e = ks_memset_zero(ctx, F->d_A, sizeof(double) * (size_t)ldn * ldn);
   if (!e) e = ks_memset_zero(ctx, scratch, scratch_bytes);
@@ -559,20 +586,25 @@ extern "C" int ks_lu_create(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out) {
 
 extern "C" int ks_lu_solve(ks_ctx ctx, ks_lu F, const double *d_b, double *d_x) {
   KS_CHECK(ctx && F && d_b && d_x, KS_ERR_ARG, "null LU solve argument");
-  const int n = F->n, ldn = F->ldn;
+  const int n = F->n, ldn = F->ldn, nblk = ldn / KS_LU_NB;
   hipStream_t st = ctx->stream;
-  hipLaunchKernelGGL(k_lu_permute, dim3((ldn + KS_BLOCK - 1) / KS_BLOCK), dim3(KS_BLOCK), 0, st,
-                     (const int *)F->d_perm, d_b, n, ldn, F->d_x);
-  auto grid = [](int rows) {
-    const int g = (rows + KS_LU_TRI_ROWS - 1) / KS_LU_TRI_ROWS;
-    return g < 1 ? 1 : (g > 512 ? 512 : g);
-  };
-  for (int b0 = 0; b0 < ldn; b0 += KS_LU_NB)   // L y = P b: F->d_x updated in place, y to F->d_y
-    hipLaunchKernelGGL(k_lu_tri_step<true>, dim3(grid(ldn - b0 - KS_LU_NB)), dim3(KS_BLOCK), 0, st,
-                       (const double *)F->d_A, ldn, b0, ldn, F->d_x, F->d_y);
-  for (int b0 = ldn - KS_LU_NB; b0 >= 0; b0 -= KS_LU_NB)   // U x = y: F->d_y updated in place, x to d_x
-    hipLaunchKernelGGL(k_lu_tri_step<false>, dim3(grid(b0)), dim3(KS_BLOCK), 0, st, (const double *)F->d_A, ldn, b0,
-                       n, F->d_y, d_x);
+  const int epoch = ++F->epoch;   // flags of earlier solves hold smaller epochs
+  int *fflag = F->d_flags, *bflag = F->d_flags + nblk, *err = F->d_flags + 2 * nblk;
+  // L y = P b (y in d_y), then U x = y (x in d_x and the caller's vector)
+  hipLaunchKernelGGL(k_lu_tri_sf<true>, dim3(nblk), dim3(KS_BLOCK), 0, st, (const double *)F->d_A, ldn, n,
+                     (const int *)F->d_perm, d_b, F->d_y, (double *)nullptr, fflag, epoch, err);
+  hipLaunchKernelGGL(k_lu_tri_sf<false>, dim3(nblk), dim3(KS_BLOCK), 0, st, (const double *)F->d_A, ldn, n,
+                     (const int *)F->d_perm, (const double *)F->d_y, F->d_x, d_x, bflag, epoch, err);
   KS_HIP(hipGetLastError());
+  return KS_OK;
+}
+
+extern "C" int ks_lu_check(ks_ctx ctx, ks_lu F) {
+  KS_CHECK(ctx && F, KS_ERR_ARG, "null LU argument");
+  int err = 0;
+  KS_HIP(hipMemcpyAsync(&err, F->d_flags + 2 * (F->ldn / KS_LU_NB), sizeof(int), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  KS_HIP(hipStreamSynchronize(ctx->stream));
+  KS_CHECK(!err, KS_ERR_HIP, "a triangular solve's wait for a block ran out of polls");
   return KS_OK;
 }

@@ -87,6 +87,7 @@ def lib():
             "ks_group_step": [vp, i32, vp, i32, dbl], "ks_group_wait_ctx": [vp, vp],
             "ks_ctx_wait_group": [vp, vp],
             "ks_lu_create": [vp, vp, dbl, C.POINTER(vp)], "ks_lu_solve": [vp, vp, vp, vp], "ks_lu_destroy": [vp],
+            "ks_lu_check": [vp, vp],
             "ks_csr_transpose": [vp, vp, C.POINTER(vp)], "ks_csr_download": [vp, vp, vp, vp, vp],
         }
         for name, args in sigs.items():
@@ -265,6 +266,10 @@ class LU:
 
     def solve(self, b: DeviceArray, x: DeviceArray):
         check(lib().ks_lu_solve(self.ctx.h, self.h, b.ptr, x.ptr), "ks_lu_solve")
+
+    def check(self):
+        """Waits and raises if a solve's bounded wait ran out (ks_lu_check)."""
+        check(lib().ks_lu_check(self.ctx.h, self.h), "ks_lu_check")
 
     def close(self):
         if self.h:

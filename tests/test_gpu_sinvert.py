@@ -208,6 +208,7 @@ def test_lu_solve_vs_oracle_factors(ks, oracle, n, sigma, kind):
         assert fw < 1e-14 * cond * 100
         db.free()
         dx.free()
+    F.check()   # every bounded wait of the pipelined triangular solves was met
     F.close()
     dA.free()
     ctx.close()
@@ -240,6 +241,7 @@ def test_lu_at_its_largest_size(ks):
     fw = np.abs(x - xr).max() / np.abs(xr).max()
     print(f"n={n} backward={bw:.2e} forward={fw:.2e}", flush=True)
     assert bw < 1e-14 * n ** 0.5 and fw < 1e-9
+    F.check()
     F.close()
     db.free()
     dx.free()
