@@ -356,7 +356,7 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_gemm(double *__restrict__ A, in
 // KS_LU_SPIN polls the workgroup sets *err and goes on, so every workgroup exits.
 // Forward: the right-hand side is b gathered through the row permutation
 // (rows >= n zero); backward: the right-hand side is the forward solve's xs.
-#define KS_LU_SPIN (1 << 22)
+#define KS_LU_SPIN (1 << 26)   // polls of about 50 ns: seconds, far beyond any legitimate wait
 template <bool LOWER>
 __global__ __launch_bounds__(KS_BLOCK) void k_lu_tri_sf(const double *__restrict__ A, int ldn, int n,
                                                        const int *__restrict__ perm, const double *__restrict__ rhs,

@@ -135,7 +135,10 @@ static void KSPFreeWork(KSP ksp)
 {
   if (ksp->ilu) (void)ks_ilu0_destroy(ksp->ilu);
   ksp->ilu = NULL;
-  if (ksp->lu) (void)ks_lu_destroy(ksp->lu);
+  if (ksp->lu) {
+    if (ks_lu_check(SlepcAMDContext(), ksp->lu)) fprintf(stderr, "slepc_amd: a direct inner solve did not complete\n");
+    (void)ks_lu_destroy(ksp->lu);
+  }
   ksp->lu = NULL;
   if (ksp->d_dinv) (void)ks_free(ksp->d_dinv);
   if (ksp->d_work) (void)ks_free(ksp->d_work);
@@ -162,6 +165,8 @@ static PetscErrorCode MatMult_STSinvert(Mat P, Vec x, Vec y)
     KSCall(ks_lu_solve(SlepcAMDContext(), ksp->lu, (const double *)x->d, (double *)y->d));
     ksp->its_total += 1;
     ksp->nsolves++;
+    /* the solves run asynchronously; every 64th confirms their bounded waits were met */
+    if (ksp->nsolves % 64 == 0) KSCall(ks_lu_check(SlepcAMDContext(), ksp->lu));
     return PETSC_SUCCESS;
   }
   PetscCheck(strcmp(ksp->pc.type, PCILU) || (!cg && ksp->ilu), PETSC_ERR_ORDER,
