@@ -187,10 +187,11 @@ int ks_bcgs_solve_ilu(ks_ctx ctx, ks_csr A, double sigma, ks_ilu M, const double
    reference's default: src/sys/classes/st/interface/stsles.c:47-54,
    st/impls/sinvert/sinvert.c:131-154): LU with partial pivoting of A - sigma I
    held dense in HBM (n^2 doubles, n <= KS_LU_MAXN), factored on the device by
-   LAPACK dgetrf's right-looking blocked algorithm (64-column panels, pivots as
-   idamax, trailing updates on FP64 MFMA).  ks_lu_solve gives x = (A - sigma I)^-1 b
-   (b and x device vectors of length n; they may be the same).  A zero pivot
-   fails the factorization with KS_ERR_PIVOT. */
+   LAPACK dgetrf's right-looking blocked algorithm (512-column panels of
+   64-column sub-panels, pivots as idamax, updates on FP64 MFMA).  ks_lu_solve
+   gives x = (A - sigma I)^-1 b (b and x device vectors of length n; they may be
+   the same), stream-ordered, one launch per triangular solve.  A pivot below
+   100 machine epsilons fails the factorization with KS_ERR_PIVOT. */
 #define KS_LU_MAXN 32768
 int ks_lu_create(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out);
 int ks_lu_solve(ks_ctx ctx, ks_lu F, const double *d_b, double *d_x);
