@@ -16,7 +16,9 @@ line (``--no-extras`` leaves them out): ``mixed_c3`` (the mixed-precision form
 against plain FP64 on C3), ``serving_c4`` (2,000 requests at ``--extra-qps``,
 served and one after another, with p50/p99 latency) and ``scale_c5`` (the R-MAT
 2^24 request served in mixed precision with eight C3 requests, beside the same
-requests in plain FP64).  Each is the summary of one of the forms below.
+requests in plain FP64) and ``sinvert_lu`` (the ``--workload lu`` line: the
+shift-and-invert direct inner solve).  Each is the summary of one of the forms
+below.
 
 Other forms, each printing its own JSON line:
 
@@ -162,7 +164,8 @@ def main():
     if args.workload == "restartgemm":
         return run_restartgemm(args)
     if args.workload == "lu":
-        return run_lu(args)
+        print(json.dumps(run_lu(args)), flush=True)
+        return
     if args.workload != "fp64":
         out = {"mixed": run_mixed, "serve": run_serve, "scale": run_scale}[args.workload](args)
         print(json.dumps(out), flush=True)
@@ -179,6 +182,9 @@ def main():
         t0 = time.time()
         out["scale_c5"] = _extra_scale(args)
         log(f"[bench] at-scale line in {time.time() - t0:.0f}s")
+        t0 = time.time()
+        out["sinvert_lu"] = run_lu(_sub_args(args, steps=1))
+        log(f"[bench] shift-and-invert (dense LU) line in {time.time() - t0:.0f}s")
     print(json.dumps(out), flush=True)
 
 
@@ -613,10 +619,9 @@ def run_lu(args):
     near = np.sort(closed[np.argsort(np.abs(closed - sigma))][:6])
     ok = bool(r["reason"] == 1 and np.allclose(np.sort(r["eigr"][:6]), near, rtol=1e-10, atol=0))
     log(f"[lu] sinvert eigensolve n={n}: {t_eps:.2f} s, {r['its']} restarts, {r['ksp_its']} solves, verified={ok}")
-    print(json.dumps({"workload": "lu", "kernel": "ks_lu_create / ks_lu_solve (dense LU, FP64 MFMA)", "shapes": rows,
-                      "sinvert_eigensolve": {"n": n, "sigma": round(sigma, 6), "nev": 6, "seconds": round(t_eps, 3),
-                                             "restarts": r["its"], "solves": r["ksp_its"], "verified": ok}}),
-          flush=True)
+    return {"workload": "lu", "kernel": "ks_lu_create / ks_lu_solve (dense LU, FP64 MFMA)", "shapes": rows,
+            "sinvert_eigensolve": {"n": n, "sigma": round(sigma, 6), "nev": 6, "seconds": round(t_eps, 3),
+                                   "restarts": r["its"], "solves": r["ksp_its"], "verified": ok}}
 
 
 def _er_job(r):
