@@ -213,6 +213,13 @@ extern "C" int *ks_prof_begin(ks_ctx ctx, int cls, double bytes, double flops, b
 extern "C" void ks_prof_end(ks_ctx ctx);
 
 // ---------------------------------------------------------------- device helpers
+// lane k's value of v, to every lane (k wave-uniform), as two 32-bit readlanes
+__device__ __forceinline__ double ks_readlane_f64(double v, int k) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)bits, k), hi = __builtin_amdgcn_readlane((int)(bits >> 32), k);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ __forceinline__ double ks_wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

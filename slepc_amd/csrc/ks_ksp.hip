@@ -428,36 +428,67 @@ __global__ __launch_bounds__(64) void k_ilu0_factor_level(const int32_t *__restr
 }
 
 // one level of y = L^-1 y (unit lower factor), in place; skipped once the solve
-// has its outcome (or, skip_sconv, after an early |s|)
-__global__ __launch_bounds__(64) void k_ilu0_lower_level(const int32_t *__restrict__ rows, int cnt,
-                                                        const int32_t *__restrict__ rowptr,
-                                                        const int32_t *__restrict__ colidx,
-                                                        const int32_t *__restrict__ diag,
-                                                        const double *__restrict__ lu, double *y, const ksp_state *S,
-                                                        int skip_sconv) {
+// One triangular solve of y = (L U)^-1 y in one launch (LOWER: the unit lower
+// factor, forward; else the upper factor, backward), in place, one wavefront per
+// row: row i goes to wavefront i (forward) or n-1-i (backward), so a row waits
+// only for rows of lower-numbered wavefronts, dispatched before it.  The lanes
+// take the row's entries of its factor part (64 at a time), wait until each
+// entry's column is solved and load its value; lane order then subtracts them
+// in column order, as the oracle's loop does.  A solved value is published as
+// two 64-bit words, each a 32-bit half of the double beside the solve's epoch,
+// written and polled with agent-scope atomics: a reader takes a value once both
+// words carry the current epoch, so no fence is needed.  Waits are bounded
+// (KS_ILU_SPIN polls, or sooner once another wait has run out): then bit 1 of
+// *err is set and the row goes on, so every row publishes and every wavefront
+// exits; the solve reports the failure.  Skipped like the products when the
+// solver's state says so (the state is read once, at the start).
+#define KS_ILU_SPIN (1 << 26)
+template <bool LOWER>
+__global__ __launch_bounds__(KS_BLOCK) void k_ilu0_sf(int n, const int32_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ colidx,
+                                                     const int32_t *__restrict__ diag, const double *__restrict__ lu,
+                                                     double *y, unsigned long long *slot, unsigned epoch, int *err,
+                                                     const ksp_state *S, int skip_sconv) {
   if (S && (S->reason || (skip_sconv && S->sconv))) return;
-  const int t = blockIdx.x * 64 + threadIdx.x;
-  if (t >= cnt) return;
-  const int i = rows[t];
+  const int w = blockIdx.x * KS_WAVES + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (w >= n) return;
+  const int i = LOWER ? w : n - 1 - w;
+  const int q0 = LOWER ? rowptr[i] : diag[i] + 1, q1 = LOWER ? diag[i] : rowptr[i + 1];
+  const unsigned long long tag = (unsigned long long)epoch << 32;
   double s = y[i];
-  for (int32_t q = rowptr[i]; q < diag[i]; q++) s -= lu[q] * y[colidx[q]];
-  y[i] = s;
-}
-
-// one level of y = U^-1 y, in place
-__global__ __launch_bounds__(64) void k_ilu0_upper_level(const int32_t *__restrict__ rows, int cnt,
-                                                        const int32_t *__restrict__ rowptr,
-                                                        const int32_t *__restrict__ colidx,
-                                                        const int32_t *__restrict__ diag,
-                                                        const double *__restrict__ lu, double *y, const ksp_state *S,
-                                                        int skip_sconv) {
-  if (S && (S->reason || (skip_sconv && S->sconv))) return;
-  const int t = blockIdx.x * 64 + threadIdx.x;
-  if (t >= cnt) return;
-  const int i = rows[t];
-  double s = y[i];
-  for (int32_t q = diag[i] + 1; q < rowptr[i + 1]; q++) s -= lu[q] * y[colidx[q]];
-  y[i] = s / lu[diag[i]];
+  for (int qb = q0; qb < q1; qb += 64) {
+    const int q = qb + lane;
+    double a = 0.0, v = 0.0;
+    if (q < q1) {
+      a = lu[q];
+      const unsigned long long *sl = slot + 2 * (size_t)colidx[q];
+      unsigned long long lo, hi;
+      int spins = 0;
+      for (;;) {
+        lo = __hip_atomic_load(sl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        hi = __hip_atomic_load(sl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((lo >> 32) == epoch && (hi >> 32) == epoch) break;
+        // past the bound, or once another wait has run out, the row goes on (and publishes)
+        if (++spins > KS_ILU_SPIN || ((spins & 1023) == 0 && (__hip_atomic_load(err, __ATOMIC_RELAXED,
+                                                                                  __HIP_MEMORY_SCOPE_AGENT) & 2))) {
+          atomicOr(err, 2);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      v = __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
+    }
+    const int cnt = q1 - qb < 64 ? q1 - qb : 64;
+    for (int t = 0; t < cnt; t++) s -= ks_readlane_f64(a, t) * ks_readlane_f64(v, t);
+  }
+  const double x = LOWER ? s : s / lu[diag[i]];
+  if (lane == 0) {
+    y[i] = x;
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
+    __hip_atomic_store(slot + 2 * (size_t)i, tag | (bits & 0xffffffffull), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(slot + 2 * (size_t)i + 1, tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // y = y - sigma x, skipped like the products
@@ -491,29 +522,30 @@ static int read_state(ks_ctx ctx, const ksp_state *S, int *its, int *reason, dou
 
 struct ks_ilu_s {
   int n = 0;
-  std::vector<int> lptr, uptr;        // level starts in lrows / urows (host)
-  int32_t *d_lrows = nullptr, *d_urows = nullptr, *d_diag = nullptr;
+  std::vector<int> lptr, uptr;        // level starts (host): the factorization's schedule; the
+                                      // backward solve's depth (ks_ilu0_levels)
+  int32_t *d_lrows = nullptr, *d_diag = nullptr;
   double *d_lu = nullptr;
-  int *d_flag = nullptr;
+  int *d_flag = nullptr;              // bit 0: a zero pivot; bit 1: a solve's wait ran out
+  unsigned long long *d_slot = nullptr;   // [n][2] published values of the triangular solves
+  unsigned epoch = 0;
   const int32_t *d_rowptr = nullptr, *d_colidx = nullptr;   // the operator's pattern (borrowed)
 };
 
 namespace {
 
-// y = (L U)^-1 y on the context stream, one launch per level of each factor
-void ilu0_apply(ks_ctx ctx, const ks_ilu M, double *y, const ksp_state *S, int skip_sconv) {
-  for (size_t l = 0; l + 1 < M->lptr.size(); l++) {
-    const int cnt = M->lptr[l + 1] - M->lptr[l];
-    hipLaunchKernelGGL(k_ilu0_lower_level, dim3((cnt + 63) / 64), dim3(64), 0, ctx->stream,
-                       (const int32_t *)M->d_lrows + M->lptr[l], cnt, M->d_rowptr, M->d_colidx,
-                       (const int32_t *)M->d_diag, (const double *)M->d_lu, y, S, skip_sconv);
-  }
-  for (size_t l = 0; l + 1 < M->uptr.size(); l++) {
-    const int cnt = M->uptr[l + 1] - M->uptr[l];
-    hipLaunchKernelGGL(k_ilu0_upper_level, dim3((cnt + 63) / 64), dim3(64), 0, ctx->stream,
-                       (const int32_t *)M->d_urows + M->uptr[l], cnt, M->d_rowptr, M->d_colidx,
-                       (const int32_t *)M->d_diag, (const double *)M->d_lu, y, S, skip_sconv);
-  }
+// y = (L U)^-1 y on the context stream: two launches, the forward and the
+// backward solve, each a new epoch of the published values
+void ilu0_apply(ks_ctx ctx, ks_ilu M, double *y, const ksp_state *S, int skip_sconv) {
+  const int blocks = (M->n + KS_WAVES - 1) / KS_WAVES;
+  M->epoch += 1;
+  hipLaunchKernelGGL(k_ilu0_sf<true>, dim3(blocks), dim3(KS_BLOCK), 0, ctx->stream, M->n, M->d_rowptr,
+                     M->d_colidx, (const int32_t *)M->d_diag, (const double *)M->d_lu, y, M->d_slot, M->epoch,
+                     M->d_flag, S, skip_sconv);
+  M->epoch += 1;
+  hipLaunchKernelGGL(k_ilu0_sf<false>, dim3(blocks), dim3(KS_BLOCK), 0, ctx->stream, M->n, M->d_rowptr,
+                     M->d_colidx, (const int32_t *)M->d_diag, (const double *)M->d_lu, y, M->d_slot, M->epoch,
+                     M->d_flag, S, skip_sconv);
 }
 
 // rows grouped by level (stable in row order) and the level starts
@@ -564,17 +596,17 @@ int ks_ilu0_create(ks_ctx ctx, ks_csr A, double sigma, ks_ilu *out) {
   group_levels(lev, urows, M->uptr);
   int e = KS_OK;
   if (!e) e = ks_malloc((void **)&M->d_lrows, sizeof(int32_t) * n);
-  if (!e) e = ks_malloc((void **)&M->d_urows, sizeof(int32_t) * n);
   if (!e) e = ks_malloc((void **)&M->d_diag, sizeof(int32_t) * n);
   if (!e) e = ks_malloc((void **)&M->d_lu, sizeof(double) * (nnz > 0 ? nnz : 1));
   if (!e) e = ks_malloc((void **)&M->d_flag, sizeof(int));
+  if (!e) e = ks_malloc((void **)&M->d_slot, sizeof(unsigned long long) * 2 * (size_t)n);
+  if (!e) e = ks_memset_zero(ctx, M->d_slot, sizeof(unsigned long long) * 2 * (size_t)n);
   if (e) {
     ks_ilu0_destroy(M);
     return e;
   }
   hipStream_t st = ctx->stream;
   KS_HIP(hipMemcpyAsync(M->d_lrows, lrows.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
-  KS_HIP(hipMemcpyAsync(M->d_urows, urows.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
   KS_HIP(hipMemcpyAsync(M->d_diag, diag.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
   KS_TRY(ks_memset_zero(ctx, M->d_flag, sizeof(int)));
   hipLaunchKernelGGL(k_ilu0_init, dim3(ksp_blocks(n)), dim3(KS_BLOCK), 0, st, A->d_rowptr, A->d_colidx,
@@ -604,7 +636,7 @@ int ks_ilu0_create(ks_ctx ctx, ks_csr A, double sigma, ks_ilu *out) {
 int ks_ilu0_destroy(ks_ilu M) {
   if (!M) return KS_OK;
   if (M->d_lrows) (void)ks_free(M->d_lrows);
-  if (M->d_urows) (void)ks_free(M->d_urows);
+  if (M->d_slot) (void)ks_free(M->d_slot);
   if (M->d_diag) (void)ks_free(M->d_diag);
   if (M->d_lu) (void)ks_free(M->d_lu);
   if (M->d_flag) (void)ks_free(M->d_flag);
@@ -736,5 +768,12 @@ int ks_bcgs_solve_ilu(ks_ctx ctx, ks_csr A, double sigma, ks_ilu M, const double
   KS_CHECK(A->vprec == KS_F64 && A->nrows == A->ncols && M->n == A->nrows, KS_ERR_ARG,
            "BiCGStab with ILU(0) needs the square FP64 matrix the factors were made from");
   KS_CHECK(M->d_rowptr == A->d_rowptr, KS_ERR_ARG, "the ILU(0) factors belong to another matrix");
-  return bcgs_solve(ctx, A, sigma, nullptr, M, d_b, d_x, rtol, abstol, maxits, d_work, its, rnorm, reason);
+  const int e = bcgs_solve(ctx, A, sigma, nullptr, M, d_b, d_x, rtol, abstol, maxits, d_work, its, rnorm, reason);
+  if (e) return e;
+  // the triangular solves' waits all met (bit 1 of the flag word); the solve has synchronized
+  int flag = 0;
+  KS_HIP(hipMemcpyAsync(&flag, M->d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  KS_HIP(hipStreamSynchronize(ctx->stream));
+  KS_CHECK(!(flag & 2), KS_ERR_HIP, "an ILU(0) triangular solve's wait ran out of polls");
+  return KS_OK;
 }

@@ -235,12 +235,6 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_laswp(double *__restrict__ A, i
   }
 }
 
-__device__ inline double readlane_f64(double v, int k) {
-  const long long bits = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)bits, k), hi = __builtin_amdgcn_readlane((int)(bits >> 32), k);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
 // U12 = L11^-1 A12 for the columns cb .. cb+ncols-1: a wavefront per column,
 // lane i holding row j0+i; L11 (the unit lower block at (j0, j0)) staged in LDS
 __global__ __launch_bounds__(KS_BLOCK) void k_lu_trsm(double *__restrict__ A, int ldn, int j0, int cb, int ncols) {
@@ -255,7 +249,7 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_trsm(double *__restrict__ A, in
     double *a = A + (size_t)(cb + cc) * ldn + j0;
     double x = a[lane];
     for (int k = 0; k < KS_LU_NB - 1; k++) {
-      const double xk = readlane_f64(x, k);
+      const double xk = ks_readlane_f64(x, k);
       if (lane > k) x -= L[k][lane] * xk;
     }
     a[lane] = x;
@@ -421,14 +415,14 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_tri_sf(const double *__restrict
     if (LOWER) {
 #pragma unroll
       for (int k = 0; k < KS_LU_NB - 1; k++) {
-        const double xk = readlane_f64(xi, k);
+        const double xk = ks_readlane_f64(xi, k);
         if (lane > k) xi -= d[k] * xk;
       }
     } else {
 #pragma unroll
       for (int k = KS_LU_NB - 1; k >= 0; k--) {
         if (lane == k) xi /= d[k];
-        const double xk = readlane_f64(xi, k);
+        const double xk = ks_readlane_f64(xi, k);
         if (lane < k) xi -= d[k] * xk;
       }
     }
