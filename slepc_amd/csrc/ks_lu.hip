@@ -25,10 +25,11 @@
 //     during the MFMAs, the operands ordered so that 16 lanes hold 16
 //     consecutive rows of a column of C (128-byte reads and writes).
 // The solve is two launches, the unit-lower and the upper triangular solve,
-// each with one workgroup per 64-row block row, pipelined through per-block
-// flags (k_lu_tri_sf): a workgroup accumulates the blocks left (right) of its
-// diagonal as their solutions are published, then solves its diagonal block and
-// publishes it; the row permutation is applied as a gather on the way in.
+// each with one workgroup per 64-row block row, pipelined through the published
+// block solutions (k_lu_tri_sf): a workgroup accumulates the blocks left
+// (right) of its diagonal as their solutions are published, then solves its
+// diagonal block and publishes it; the row permutation is applied as a gather
+// on the way in.
 // Memory is n^2 doubles (for n = 32,768: 8.6 GB), so the factorization serves
 // matrices of up to that size; larger ones take an iterative inner solver.
 #include <vector>
@@ -335,29 +336,31 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_gemm(double *__restrict__ A, in
             cv[a][b][q] - acc[a][b][q];
 }
 
-// One launch for a whole triangular solve, block rows pipelined through flags.
-// Workgroup g takes block row i = g (forward, L) or i = nblk-1-g (backward, U),
-// so a workgroup only waits for lower-numbered ones (dispatched before it).  For
-// each solved block j (j < i forward, j > i backward, in the order they are
-// solved) it loads its 64 x 64 block L(i, j) or U(i, j) -- issued before the wait
-// -- waits until block j is published, and accumulates the block times x_j:
-// thread t takes row t%64 and 16 of the 64 columns.  Then wavefront 0 solves the
-// diagonal block from registers (column-oriented, in dtrsv's order within the
-// block), writes x_i to xs and the output, and the block is published: a release
-// store of the solve's epoch into flag[i].  Waits poll flag[j] with agent-scope
-// loads (coherent across the XCDs' caches) followed by one acquire fence, and are
-// bounded: past
-// KS_LU_SPIN polls the workgroup sets *err and goes on, so every workgroup exits.
-// Forward: the right-hand side is b gathered through the row permutation
-// (rows >= n zero); backward: the right-hand side is the forward solve's xs.
+// One launch for a whole triangular solve, block rows pipelined through their
+// published solutions.  Workgroup g takes block row i = g (forward, L) or
+// i = nblk-1-g (backward, U), so a workgroup only waits for lower-numbered ones
+// (dispatched before it).  For each solved block j (j < i forward, j > i
+// backward, in the order they are solved) it loads its 64 x 64 block L(i, j) or
+// U(i, j) -- issued before the wait -- waits for x_j and accumulates the block
+// times x_j: thread t takes row t%64 and 16 of the 64 columns.  Then wavefront 0
+// solves the diagonal block from registers (column-oriented, in dtrsv's order
+// within the block), writes x_i to xs (forward) or the output (backward, rows
+// below n) and publishes it.  A value is published as two 64-bit words, each a
+// 32-bit half of the double beside the solve's epoch, written and polled with
+// agent-scope atomics (coherent across the XCDs' caches): a reader takes a value
+// once both words carry the current epoch, so no fence is needed.  Waits are
+// bounded (KS_LU_SPIN polls, or sooner once another wait has run out): then bit
+// 0 of *err is set and the workgroup goes on, so every workgroup publishes and
+// exits.  Forward: the right-hand side is b gathered through the row
+// permutation (rows >= n zero); backward: the forward solve's xs.
 #define KS_LU_SPIN (1 << 26)   // polls of about 50 ns: seconds, far beyond any legitimate wait
 template <bool LOWER>
 __global__ __launch_bounds__(KS_BLOCK) void k_lu_tri_sf(const double *__restrict__ A, int ldn, int n,
                                                        const int *__restrict__ perm, const double *__restrict__ rhs,
-                                                       double *xs, double *__restrict__ out, int *flag, int epoch,
-                                                       int *err) {
+                                                       double *__restrict__ xs, double *__restrict__ out,
+                                                       unsigned long long *slot, unsigned epoch, int *err) {
   __shared__ double part[KS_WAVES][KS_LU_NB];
-  __shared__ double xj[KS_LU_NB];
+  __shared__ double xj[2][KS_LU_NB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nblk = ldn / KS_LU_NB;
   const int i = LOWER ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
@@ -377,32 +380,35 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_tri_sf(const double *__restrict
   };
   if (jn > 0) load_block(LOWER ? 0 : nblk - 1);
   for (int t = 0; t < jn; t++) {
-    const int j = LOWER ? t : nblk - 1 - t;
-    if (tid == 0) {
+    const int j = LOWER ? t : nblk - 1 - t, buf = t & 1;
+    if (tid < KS_LU_NB) {   // x_j, element by element, as it is published
+      const unsigned long long *sl = slot + 2 * (size_t)(j * KS_LU_NB + tid);
+      unsigned long long lo, hi;
       int spins = 0;
-      // coherent polls without cache invalidation, then one acquire fence
-      while (__hip_atomic_load(&flag[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-        if (++spins > KS_LU_SPIN) {
+      for (;;) {
+        lo = __hip_atomic_load(sl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        hi = __hip_atomic_load(sl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((lo >> 32) == epoch && (hi >> 32) == epoch) break;
+        if (++spins > KS_LU_SPIN || ((spins & 1023) == 0 && (__hip_atomic_load(err, __ATOMIC_RELAXED,
+                                                                                  __HIP_MEMORY_SCOPE_AGENT) & 1))) {
           atomicOr(err, 1);
           break;
         }
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(1);
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      xj[buf][tid] = __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
     }
-    __syncthreads();
-    if (tid < KS_LU_NB) xj[tid] = xs[j * KS_LU_NB + tid];
-    __syncthreads();
+    __syncthreads();   // (also: every thread is past the use of this buffer two blocks back)
     double cur[16];
 #pragma unroll
     for (int c = 0; c < 16; c++) cur[c] = a[c];
     if (t + 1 < jn) load_block(LOWER ? t + 1 : nblk - 2 - t);   // the next block in flight meanwhile
     if (LOWER) {
 #pragma unroll
-      for (int c = 0; c < 16; c++) acc += cur[c] * xj[cq + c];
+      for (int c = 0; c < 16; c++) acc += cur[c] * xj[buf][cq + c];
     } else {
 #pragma unroll
-      for (int c = 15; c >= 0; c--) acc += cur[c] * xj[cq + c];
+      for (int c = 15; c >= 0; c--) acc += cur[c] * xj[buf][cq + c];
     }
   }
   part[wave][lane] = acc;
@@ -426,11 +432,14 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_tri_sf(const double *__restrict
         if (lane < k) xi -= d[k] * xk;
       }
     }
-    xs[row] = xi;
-    if (out && row < n) out[row] = xi;
+    if (LOWER) xs[row] = xi;
+    else if (row < n) out[row] = xi;
+    const unsigned long long tag = (unsigned long long)epoch << 32;
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(xi);
+    __hip_atomic_store(slot + 2 * (size_t)row, tag | (bits & 0xffffffffull), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(slot + 2 * (size_t)row + 1, tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  __syncthreads();
-  if (tid == 0) __hip_atomic_store(&flag[i], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace
@@ -439,8 +448,9 @@ struct ks_lu_s {
   int n = 0, ldn = 0;
   double *d_A = nullptr, *d_x = nullptr, *d_y = nullptr;
   int *d_ipiv = nullptr, *d_perm = nullptr;
-  int *d_flags = nullptr;   // [2][nblk] publication epochs of the two solves, then the error word
-  int epoch = 0;
+  unsigned long long *d_slot = nullptr;   // [ldn][2] published values of the triangular solves
+  int *d_err = nullptr;                    // bit 0: a solve's wait ran out
+  unsigned epoch = 0;
 };
 
 extern "C" int ks_lu_destroy(ks_lu F) {
@@ -450,7 +460,8 @@ extern "C" int ks_lu_destroy(ks_lu F) {
   if (F->d_y) (void)ks_free(F->d_y);
   if (F->d_ipiv) (void)ks_free(F->d_ipiv);
   if (F->d_perm) (void)ks_free(F->d_perm);
-  if (F->d_flags) (void)ks_free(F->d_flags);
+  if (F->d_slot) (void)ks_free(F->d_slot);
+  if (F->d_err) (void)ks_free(F->d_err);
   delete F;
   return KS_OK;
 }
@@ -476,8 +487,10 @@ extern "C" int ks_lu_create(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out) {
   if (!e) e = ks_malloc((void **)&F->d_y, sizeof(double) * ldn);
   if (!e) e = ks_malloc((void **)&F->d_ipiv, sizeof(int) * ldn);
   if (!e) e = ks_malloc((void **)&F->d_perm, sizeof(int) * ldn);
-  if (!e) e = ks_malloc((void **)&F->d_flags, sizeof(int) * (2 * (ldn / KS_LU_NB) + 1));
-  if (!e) e = ks_memset_zero(ctx, F->d_flags, sizeof(int) * (2 * (ldn / KS_LU_NB) + 1));
+  if (!e) e = ks_malloc((void **)&F->d_slot, sizeof(unsigned long long) * 2 * (size_t)ldn);
+  if (!e) e = ks_malloc((void **)&F->d_err, sizeof(int));
+  if (!e) e = ks_memset_zero(ctx, F->d_slot, sizeof(unsigned long long) * 2 * (size_t)ldn);
+  if (!e) e = ks_memset_zero(ctx, F->d_err, sizeof(int));
   if (!e) e = ks_malloc(&scratch, scratch_bytes);
   if (!e) e = ks_memset_zero(ctx, F->d_A, sizeof(double) * (size_t)ldn * ldn);
   if (!e) e = ks_memset_zero(ctx, scratch, scratch_bytes);
@@ -582,13 +595,14 @@ extern "C" int ks_lu_solve(ks_ctx ctx, ks_lu F, const double *d_b, double *d_x) 
   KS_CHECK(ctx && F && d_b && d_x, KS_ERR_ARG, "null LU solve argument");
   const int n = F->n, ldn = F->ldn, nblk = ldn / KS_LU_NB;
   hipStream_t st = ctx->stream;
-  const int epoch = ++F->epoch;   // flags of earlier solves hold smaller epochs
-  int *fflag = F->d_flags, *bflag = F->d_flags + nblk, *err = F->d_flags + 2 * nblk;
-  // L y = P b (y in d_y), then U x = y (x in d_x and the caller's vector)
+  // L y = P b (y in d_y), then U x = y (x in the caller's vector), each with an
+  // epoch of its own (the published values of earlier solves hold older ones)
+  const unsigned ef = ++F->epoch, eb = ++F->epoch;
   hipLaunchKernelGGL(k_lu_tri_sf<true>, dim3(nblk), dim3(KS_BLOCK), 0, st, (const double *)F->d_A, ldn, n,
-                     (const int *)F->d_perm, d_b, F->d_y, (double *)nullptr, fflag, epoch, err);
+                     (const int *)F->d_perm, d_b, F->d_y, (double *)nullptr, F->d_slot, ef, F->d_err);
   hipLaunchKernelGGL(k_lu_tri_sf<false>, dim3(nblk), dim3(KS_BLOCK), 0, st, (const double *)F->d_A, ldn, n,
-                     (const int *)F->d_perm, (const double *)F->d_y, F->d_x, d_x, bflag, epoch, err);
+                     (const int *)F->d_perm, (const double *)F->d_y, (double *)nullptr, d_x, F->d_slot, eb,
+                     F->d_err);
   KS_HIP(hipGetLastError());
   return KS_OK;
 }
@@ -596,8 +610,7 @@ extern "C" int ks_lu_solve(ks_ctx ctx, ks_lu F, const double *d_b, double *d_x) 
 extern "C" int ks_lu_check(ks_ctx ctx, ks_lu F) {
   KS_CHECK(ctx && F, KS_ERR_ARG, "null LU argument");
   int err = 0;
-  KS_HIP(hipMemcpyAsync(&err, F->d_flags + 2 * (F->ldn / KS_LU_NB), sizeof(int), hipMemcpyDeviceToHost,
-                        ctx->stream));
+  KS_HIP(hipMemcpyAsync(&err, F->d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   KS_HIP(hipStreamSynchronize(ctx->stream));
   KS_CHECK(!err, KS_ERR_HIP, "a triangular solve's wait for a block ran out of polls");
   return KS_OK;
