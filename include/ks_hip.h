@@ -27,7 +27,8 @@ enum {
   KS_ERR_NODEV = -4,      /* no gfx950 device visible */
   KS_ERR_INNER = -5,      /* invalid inner product (negative self dot beyond tolerance) */
   KS_ERR_RANGE = -6,      /* size beyond a kernel's supported maximum */
-  KS_ERR_PIVOT = -7       /* zero pivot in a factorization (ILU(0)) */
+  KS_ERR_PIVOT = -7,      /* zero pivot in a factorization (ILU(0), LU) */
+  KS_ERR_TIMEOUT = -8     /* a bounded wait of a kernel on other workgroups ran out */
 };
 
 typedef enum { KS_F64 = 0, KS_F32 = 1 } ks_prec;
@@ -173,8 +174,9 @@ int ks_bv_tsqr(ks_ctx ctx, int n, double *d_V, int ld, int k, int form_q, double
    Systems, Algorithm 10.4; PETSc's PCILU with levels 0 and the natural
    ordering, its defaults), factored on the device one dependency level of rows
    at a time; A's rows must list increasing columns and hold their diagonal.
-   Fails with KS_ERR_PIVOT on a zero pivot.  Applying it is one launch per level
-   of each factor (ks_ilu0_levels).  Replaces PCSetUp_ILU / PCApply_ILU as
+   Fails with KS_ERR_PIVOT on a zero pivot.  Applying it is one launch for each
+   factor, rows pipelined through their published values (ks_ilu0_levels gives
+   the dependency depth of each).  Replaces PCSetUp_ILU / PCApply_ILU as
    KSPSolve reaches them from STApply_Sinvert
    (src/sys/classes/st/impls/sinvert/sinvert.c:16-30). */
 int ks_ilu0_create(ks_ctx ctx, ks_csr A, double sigma, ks_ilu *out);
@@ -198,7 +200,7 @@ int ks_lu_solve(ks_ctx ctx, ks_lu F, const double *d_b, double *d_x);
 int ks_lu_destroy(ks_lu F);
 /* Waits for the context's stream and reports whether every solve so far ran
    clean (the triangular solves' bounded waits never ran out): KS_OK or
-   KS_ERR_HIP.  No counterpart in the reference (its LU solves are synchronous
+   KS_ERR_TIMEOUT.  No counterpart in the reference (its LU solves are synchronous
    host code); for callers that check the asynchronous solves, such as tests. */
 int ks_lu_check(ks_ctx ctx, ks_lu F);
 /* nrows, nnz, max row length and the kernel variant chosen (0 stream, 1 vector) */

@@ -774,6 +774,6 @@ int ks_bcgs_solve_ilu(ks_ctx ctx, ks_csr A, double sigma, ks_ilu M, const double
   int flag = 0;
   KS_HIP(hipMemcpyAsync(&flag, M->d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   KS_HIP(hipStreamSynchronize(ctx->stream));
-  KS_CHECK(!(flag & 2), KS_ERR_HIP, "an ILU(0) triangular solve's wait ran out of polls");
+  KS_CHECK(!(flag & 2), KS_ERR_TIMEOUT, "an ILU(0) triangular solve's wait ran out of polls");
   return KS_OK;
 }

@@ -32,6 +32,7 @@
 // on the way in.
 // Memory is n^2 doubles (for n = 32,768: 8.6 GB), so the factorization serves
 // matrices of up to that size; larger ones take an iterative inner solver.
+#include <cstdlib>
 #include <vector>
 
 #include "ks_internal.hpp"
@@ -40,7 +41,8 @@ namespace {
 
 #define KS_LU_NB 64
 #define KS_LU_NB2 512       // columns of a panel of the trailing update (a multiple of 64)
-#define KS_LU_PG_MAX 512   // most workgroups of a panel step (ldn <= 131,072)
+#define KS_LU_PG_MAX 512
+#define KS_LU_SPIN (1 << 26)   // bound of every wait: polls of about 50 ns, seconds in all   // most workgroups of a panel step (ldn <= 131,072)
 // a pivot below this magnitude is a zero pivot (PETSc's default zeropivot, 100 eps)
 #define KS_LU_ZEROPIVOT (100.0 * 2.220446049250313e-16)
 
@@ -69,6 +71,7 @@ __device__ inline bool piv_better(double m1, int i1, double m2, int i2) {
 }
 
 // workgroup reduction of (magnitude, row) to thread 0; sm/si hold KS_WAVES entries
+template <int NW = KS_WAVES>
 __device__ inline void piv_reduce(double &m, int &i, double *sm, int *si) {
   for (int off = 32; off > 0; off >>= 1) {
     const double om = __shfl_down(m, off, 64);
@@ -85,7 +88,7 @@ __device__ inline void piv_reduce(double &m, int &i, double *sm, int *si) {
   }
   __syncthreads();
   if (threadIdx.x == 0)
-    for (int w = 1; w < KS_WAVES; w++)
+    for (int w = 1; w < NW; w++)
       if (piv_better(sm[w], si[w], m, i)) {
         m = sm[w];
         i = si[w];
@@ -193,6 +196,138 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_step(double *__restrict__ A, in
       pi[(slot ^ 1) * KS_LU_PG_MAX + blockIdx.x] = i2;
     }
   }
+}
+
+// Epoch-tagged publication: a 32-bit payload beside a 32-bit epoch in one 64-bit
+// word, stored and polled with agent-scope atomics (coherent across the XCDs'
+// caches); a double takes two words.  A reader that finds the current epoch in
+// every word of a value has the value, with no fence.
+__device__ inline void pub_store(unsigned long long *w, unsigned epoch, unsigned payload) {
+  __hip_atomic_store(w, ((unsigned long long)epoch << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void pub_store_f64(unsigned long long *w, unsigned epoch, double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  pub_store(w, epoch, (unsigned)(b & 0xffffffffull));
+  pub_store(w + 1, epoch, (unsigned)(b >> 32));
+}
+// polls until the nw words at w carry epoch (bounded; sooner out once *err shows
+// another wait ran out), then returns their payloads in p[]
+template <int NWORD>
+__device__ inline void pub_wait(const unsigned long long *w, unsigned epoch, unsigned *p, int *err) {
+  int spins = 0;
+  for (;;) {
+    bool ok = true;
+#pragma unroll
+    for (int q = 0; q < NWORD; q++) {
+      const unsigned long long v = __hip_atomic_load(w + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      p[q] = (unsigned)(v & 0xffffffffull);
+      ok = ok && (unsigned)(v >> 32) == epoch;
+    }
+    if (ok) return;
+    if (++spins > KS_LU_SPIN ||
+        ((spins & 1023) == 0 && (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1))) {
+      atomicOr(err, 1);
+      return;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// The whole sub-panel in one cooperative launch (all G workgroups resident):
+// thread t of workgroup g keeps row j0 + g * KS_LU_PT + t of the 64 panel
+// columns in LDS for the 64 column steps, which exchange their data through
+// epoch-tagged words instead of launch boundaries.  Per column k (epoch
+// e0 + k): each workgroup finds its best candidate (largest magnitude, lowest
+// row) and publishes it -- its (magnitude, row) as three words, the row's
+// values in columns k..63 as two words each, one value a thread -- into slot
+// k & 1; every workgroup then waits for the G candidates, takes the best
+// (ties to the lower row), waits for the winner's values and updates its rows
+// as k_lu_step does (the same operations in the same order: the factors are
+// those of the column-step launches).  A workgroup writes slot k & 1 again
+// only at column k + 2, after every workgroup has published column k + 1,
+// which each does after reading column k.
+#define KS_LU_PT 128                                  // rows (threads) a workgroup
+#define KS_LU_PW (KS_LU_PT / 64)
+#define KS_LU_CO_MAX 256                              // workgroups at most (ldn <= 32,768)
+#define KS_LU_CAND 3                                  // words of a candidate: magnitude (2), row
+__global__ __launch_bounds__(KS_LU_PT) void k_lu_panel(double *__restrict__ A, int ldn, int j0,
+                                                      unsigned long long *cand, unsigned long long *crow,
+                                                      unsigned e0, int *__restrict__ prow, int *flag, int *err) {
+  __shared__ double rowv[KS_LU_NB][KS_LU_PT];   // 64 KB: the workgroup's rows of the panel
+  __shared__ double pv[KS_LU_NB];
+  __shared__ double sm[KS_LU_PW];
+  __shared__ int si[KS_LU_PW];
+  __shared__ int s_best, s_p;
+  const int tid = threadIdx.x, G = gridDim.x, g = blockIdx.x;
+  const int r = j0 + g * KS_LU_PT + tid;
+  const bool have = r < ldn;
+#pragma unroll 8
+  for (int c = 0; c < KS_LU_NB; c++) rowv[c][tid] = have ? A[(size_t)r + (size_t)(j0 + c) * ldn] : 0.0;
+  bool used = !have;
+  __syncthreads();
+  for (int k = 0; k < KS_LU_NB; k++) {
+    const unsigned ep = e0 + (unsigned)k;
+    const int slot = k & 1;
+    unsigned long long *mycand = cand + ((size_t)slot * KS_LU_CO_MAX + g) * KS_LU_CAND;
+    unsigned long long *myrow = crow + ((size_t)slot * KS_LU_CO_MAX + g) * 2 * KS_LU_NB;
+    double m = used ? -1.0 : piv_mag(rowv[k][tid]);
+    int idx = used ? INT_MAX : r;
+    piv_reduce<KS_LU_PW>(m, idx, sm, si);
+    if (tid == 0) {
+      s_best = idx;
+      pub_store_f64(mycand, ep, m);
+      pub_store(mycand + 2, ep, (unsigned)idx);
+    }
+    __syncthreads();
+    {   // the candidate's values, columns k..63, one a thread (the row's index -> its thread)
+      const int b = s_best;
+      const int c = k + tid;
+      if (c < KS_LU_NB) pub_store_f64(myrow + 2 * c, ep, b != INT_MAX ? rowv[c][b - (j0 + g * KS_LU_PT)] : 0.0);
+    }
+    // the best of the G candidates (each workgroup alike)
+    double mb = -1.0;
+    int ib = INT_MAX;
+    for (int q = tid; q < G; q += KS_LU_PT) {
+      unsigned w[KS_LU_CAND];
+      pub_wait<KS_LU_CAND>(cand + ((size_t)slot * KS_LU_CO_MAX + q) * KS_LU_CAND, ep, w, err);
+      const double a = __longlong_as_double((long long)(((unsigned long long)w[1] << 32) | w[0]));
+      const int i = (int)w[2];
+      if (piv_better(a, i, mb, ib)) {
+        mb = a;
+        ib = i;
+      }
+    }
+    piv_reduce<KS_LU_PW>(mb, ib, sm, si);
+    if (tid == 0) {
+      s_p = ib;
+      if (g == 0) {
+        prow[k] = ib;
+        if (!(mb >= KS_LU_ZEROPIVOT)) *flag = 1;
+      }
+    }
+    __syncthreads();
+    const int p = s_p;
+    const int gw = p == INT_MAX ? g : (p - j0) / KS_LU_PT;   // a candidate row always exists
+    if (tid >= k && tid < KS_LU_NB) {
+      unsigned w[2];
+      pub_wait<2>(crow + ((size_t)slot * KS_LU_CO_MAX + gw) * 2 * KS_LU_NB + 2 * tid, ep, w, err);
+      pv[tid] = __longlong_as_double((long long)(((unsigned long long)w[1] << 32) | w[0]));
+    }
+    __syncthreads();
+    if (!used) {
+      if (r == p) {
+        used = true;
+      } else {
+        const double l = rowv[k][tid] * (1.0 / pv[k]);
+        rowv[k][tid] = l;
+        for (int c = k + 1; c < KS_LU_NB; c++) rowv[c][tid] -= l * pv[c];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll 8
+  for (int c = 0; c < KS_LU_NB; c++)
+    if (have) A[(size_t)r + (size_t)(j0 + c) * ldn] = rowv[c][tid];
 }
 
 // the pivot rows of the panel (physical rows, chosen in place) as LAPACK's
@@ -353,7 +488,6 @@ __global__ __launch_bounds__(KS_BLOCK) void k_lu_gemm(double *__restrict__ A, in
 // 0 of *err is set and the workgroup goes on, so every workgroup publishes and
 // exits.  Forward: the right-hand side is b gathered through the row
 // permutation (rows >= n zero); backward: the forward solve's xs.
-#define KS_LU_SPIN (1 << 26)   // polls of about 50 ns: seconds, far beyond any legitimate wait
 template <bool LOWER>
 __global__ __launch_bounds__(KS_BLOCK) void k_lu_tri_sf(const double *__restrict__ A, int ldn, int n,
                                                        const int *__restrict__ perm, const double *__restrict__ rhs,
@@ -466,21 +600,43 @@ extern "C" int ks_lu_destroy(ks_lu F) {
   return KS_OK;
 }
 
+namespace {
+int lu_factor(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out, bool coop);
+}
+
+// The cooperative sub-panels wait on each other; should another stream's work
+// keep some of them from running until a wait runs out, the factorization is
+// made again with the column-step launches, which wait on nothing.
 extern "C" int ks_lu_create(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out) {
+  // KS_LU_COOPERATIVE=0 in the environment selects the column-step launches
+  const char *env = getenv("KS_LU_COOPERATIVE");
+  if (env && env[0] == '0') return lu_factor(ctx, A, sigma, out, false);
+  const int e = lu_factor(ctx, A, sigma, out, true);
+  if (e != KS_ERR_TIMEOUT) return e;
+  return lu_factor(ctx, A, sigma, out, false);
+}
+
+namespace {
+int lu_factor(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out, bool coop) {
   KS_CHECK(ctx && A && out, KS_ERR_ARG, "null LU argument");
   KS_CHECK(A->vprec == KS_F64 && A->nrows == A->ncols, KS_ERR_ARG, "the LU needs a square FP64 matrix");
   KS_CHECK(A->nrows <= KS_LU_MAXN, KS_ERR_RANGE, "the dense LU takes at most %d rows (%d given)", KS_LU_MAXN,
            A->nrows);
   static_assert((KS_LU_MAXN + KS_LU_NB - 1) / KS_LU_NB * KS_LU_NB <= KS_LU_PG_MAX * KS_BLOCK,
                 "a panel step's workgroups must cover every row");
+  static_assert((KS_LU_MAXN + KS_LU_NB - 1) / KS_LU_NB * KS_LU_NB <= KS_LU_CO_MAX * KS_LU_PT,
+                "a cooperative panel's workgroups must cover every row");
   *out = nullptr;
   ks_lu F = new ks_lu_s();
   const int n = A->nrows, ldn = (n + KS_LU_NB - 1) / KS_LU_NB * KS_LU_NB;
   F->n = n;
   F->ldn = ldn;
-  // factorization scratch: used[ldn], prow[64], flag, pi[2][PG_MAX] (ints), pm[2][PG_MAX] (doubles)
-  const size_t nint = (size_t)ldn + KS_LU_NB + 1 + 2 * KS_LU_PG_MAX;
-  const size_t scratch_bytes = ((nint * sizeof(int) + 255) / 256) * 256 + 2 * KS_LU_PG_MAX * sizeof(double);
+  // factorization scratch: used[ldn], prow[64], flag, err, pi[2][PG_MAX] (ints); pm[2][PG_MAX]
+  // (doubles); the cooperative panels' published words cand[2][CO_MAX][3], crow[2][CO_MAX][2 * 64]
+  const size_t nint = (size_t)ldn + KS_LU_NB + 2 + 2 * KS_LU_PG_MAX;
+  const size_t off_dbl = ((nint * sizeof(int) + 255) / 256) * 256;
+  const size_t nwords = 2 * KS_LU_CO_MAX * (KS_LU_CAND + 2 * KS_LU_NB);
+  const size_t scratch_bytes = off_dbl + 2 * KS_LU_PG_MAX * sizeof(double) + nwords * sizeof(unsigned long long);
   void *scratch = nullptr;
   int e = ks_malloc((void **)&F->d_A, sizeof(double) * (size_t)ldn * ldn);
   if (!e) e = ks_malloc((void **)&F->d_x, sizeof(double) * ldn);
@@ -499,8 +655,13 @@ extern "C" int ks_lu_create(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out) {
     ks_lu_destroy(F);
     return e;
   }
-  int *d_used = (int *)scratch, *d_prow = d_used + ldn, *d_flag = d_prow + KS_LU_NB, *d_pi = d_flag + 1;
-  double *d_pm = (double *)((char *)scratch + ((nint * sizeof(int) + 255) / 256) * 256);
+  int *d_used = (int *)scratch, *d_prow = d_used + ldn, *d_flag = d_prow + KS_LU_NB, *d_perr = d_flag + 1,
+      *d_pi = d_perr + 1;
+  double *d_pm = (double *)((char *)scratch + off_dbl);
+  unsigned long long *d_cand = (unsigned long long *)(d_pm + 2 * KS_LU_PG_MAX);
+  unsigned long long *d_crow = d_cand + 2 * KS_LU_CO_MAX * KS_LU_CAND;
+  unsigned e0 = 1;     // the next cooperative sub-panel's first epoch (the words start at 0)
+  // coop: cooperative launches wanted and available (else the column-step launches)
   hipStream_t st = ctx->stream;
   const int fill_blocks = (ldn + KS_BLOCK - 1) / KS_BLOCK;
   hipLaunchKernelGGL(k_lu_fill, dim3(fill_blocks), dim3(KS_BLOCK), 0, st, A->d_rowptr, A->d_colidx,
@@ -530,11 +691,30 @@ extern "C" int ks_lu_create(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out) {
         }
         gemm(J0, s, j0, ldn - j0, j0, KS_LU_NB);
       }
-      const int G = (ldn - j0 + KS_BLOCK - 1) / KS_BLOCK;
-      hipLaunchKernelGGL(k_lu_colmax, dim3(G), dim3(KS_BLOCK), 0, st, (const double *)F->d_A, ldn, j0, d_pm, d_pi);
-      for (int j = j0; j < j0 + KS_LU_NB; j++)
-        hipLaunchKernelGGL(k_lu_step, dim3(G), dim3(KS_BLOCK), 0, st, F->d_A, ldn, j0, j, d_used, d_pm, d_pi, d_prow,
-                           d_flag);
+      if (coop) {   // the sub-panel in one cooperative launch
+        const int Gc = (ldn - j0 + KS_LU_PT - 1) / KS_LU_PT;
+        double *a_A = F->d_A;
+        int a_ldn = ldn, a_j0 = j0;
+        unsigned long long *a_cand = d_cand, *a_crow = d_crow;
+        unsigned a_e0 = e0;
+        int *a_prow = d_prow, *a_flag = d_flag, *a_err = d_perr;
+        void *args[] = {&a_A, &a_ldn, &a_j0, &a_cand, &a_crow, &a_e0, &a_prow, &a_flag, &a_err};
+        if (hipLaunchCooperativeKernel((const void *)k_lu_panel, dim3(Gc), dim3(KS_LU_PT), args, 0, st) ==
+            hipSuccess)
+          e0 += KS_LU_NB;
+        else {
+          (void)hipGetLastError();
+          coop = false;
+        }
+      }
+      if (!coop) {
+        const int G = (ldn - j0 + KS_BLOCK - 1) / KS_BLOCK;
+        hipLaunchKernelGGL(k_lu_colmax, dim3(G), dim3(KS_BLOCK), 0, st, (const double *)F->d_A, ldn, j0, d_pm,
+                           d_pi);
+        for (int j = j0; j < j0 + KS_LU_NB; j++)
+          hipLaunchKernelGGL(k_lu_step, dim3(G), dim3(KS_BLOCK), 0, st, F->d_A, ldn, j0, j, d_used, d_pm, d_pi,
+                             d_prow, d_flag);
+      }
       hipLaunchKernelGGL(k_lu_ipiv, dim3(1), dim3(KS_LU_NB), 0, st, (const int *)d_prow, j0, F->d_ipiv);
       hipLaunchKernelGGL(k_lu_laswp, dim3(fill_blocks), dim3(KS_BLOCK), 0, st, F->d_A, ldn, j0,
                          (const int *)F->d_ipiv, 0, ldn);
@@ -554,13 +734,19 @@ extern "C" int ks_lu_create(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out) {
   std::vector<int> ipiv(ldn), perm(ldn);
   int flag = 0;
   if (herr == hipSuccess) herr = hipMemcpyAsync(ipiv.data(), F->d_ipiv, sizeof(int) * ldn, hipMemcpyDeviceToHost, st);
+  int perr = 0;
   if (herr == hipSuccess) herr = hipMemcpyAsync(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (herr == hipSuccess) herr = hipMemcpyAsync(&perr, d_perr, sizeof(int), hipMemcpyDeviceToHost, st);
   if (herr == hipSuccess) herr = hipStreamSynchronize(st);
   else (void)hipStreamSynchronize(st);   // nothing may still run on what is freed below
   (void)ks_free(scratch);
   if (herr != hipSuccess) {
     ks_lu_destroy(F);
     KS_HIP(herr);
+  }
+  if (perr) {   // the caller makes it again without cooperative launches
+    ks_lu_destroy(F);
+    return KS_ERR_TIMEOUT;
   }
   if (flag) {
     fprintf(stderr, "ks_hip: the LU of A - sigma I met a zero pivot\n");
@@ -591,6 +777,8 @@ extern "C" int ks_lu_create(ks_ctx ctx, ks_csr A, double sigma, ks_lu *out) {
   return KS_OK;
 }
 
+}  // namespace
+
 extern "C" int ks_lu_solve(ks_ctx ctx, ks_lu F, const double *d_b, double *d_x) {
   KS_CHECK(ctx && F && d_b && d_x, KS_ERR_ARG, "null LU solve argument");
   const int n = F->n, ldn = F->ldn, nblk = ldn / KS_LU_NB;
@@ -612,6 +800,6 @@ extern "C" int ks_lu_check(ks_ctx ctx, ks_lu F) {
   int err = 0;
   KS_HIP(hipMemcpyAsync(&err, F->d_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   KS_HIP(hipStreamSynchronize(ctx->stream));
-  KS_CHECK(!err, KS_ERR_HIP, "a triangular solve's wait for a block ran out of polls");
+  KS_CHECK(!err, KS_ERR_TIMEOUT, "a triangular solve's wait for a block ran out of polls");
   return KS_OK;
 }

@@ -69,6 +69,7 @@ const char *ks_error_string(int code) {
     case KS_ERR_INNER: return "invalid inner product";
     case KS_ERR_RANGE: return "size beyond supported maximum";
     case KS_ERR_PIVOT: return "zero pivot";
+    case KS_ERR_TIMEOUT: return "a bounded device wait ran out";
   }
   return "unknown error";
 }

@@ -214,6 +214,33 @@ def test_lu_solve_vs_oracle_factors(ks, oracle, n, sigma, kind):
     ctx.close()
 
 
+def test_lu_cooperative_and_column_step_panels_agree(ks, monkeypatch):
+    # the sub-panels factored by one cooperative launch each (the default) and by one launch
+    # per column (KS_LU_COOPERATIVE=0) do the same operations in the same order
+    n, sigma = 3000, 0.3
+    csr = M.random_sparse_nonsymmetric(n, 8.0 / n, seed=7)
+    b = M.uniform_vector(5, n)
+    ctx = ks.Context(0)
+    dA = ks.CSR(ctx, csr)
+    db = ks.DeviceArray.from_numpy(ctx, b)
+    xs = []
+    for coop in ("1", "0"):
+        monkeypatch.setenv("KS_LU_COOPERATIVE", coop)
+        F = ks.LU(ctx, dA, sigma)
+        dx = ks.DeviceArray(ctx, (n,))
+        F.solve(db, dx)
+        F.check()
+        xs.append(dx.numpy())
+        dx.free()
+        F.close()
+    assert np.array_equal(xs[0], xs[1])
+    A = M.csr_to_dense(*csr) - sigma * np.eye(n)
+    assert np.abs(A @ xs[0] - b).max() / (np.abs(A).sum(axis=1).max() * np.abs(xs[0]).max()) < 1e-13
+    db.free()
+    dA.free()
+    ctx.close()
+
+
 def test_lu_at_its_largest_size(ks):
     # n = 32,768 (the largest the dense factorization takes; 8.6 GB of factors): the 1-D
     # Laplacian shifted to a zero diagonal, against the banded LAPACK solve
