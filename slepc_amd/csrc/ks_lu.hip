@@ -8,13 +8,15 @@
 // factored by the right-looking blocked algorithm of LAPACK's dgetrf: panels
 // of 512 columns, each factored as sub-panels of 64 (dgetrf2's recursion cut
 // at 64 columns), the columns right of a panel updated once per panel:
-//   * sub-panel: one launch per column.  The pivot rows are found and used in
-//     place (implicit pivoting): every workgroup reduces the partial maxima of
-//     column j the previous launch left (largest magnitude, lowest row on
-//     ties), scales its rows of column j by the pivot's reciprocal, updates
-//     their sub-panel columns right of j (dgetf2's dscal and dger) and leaves
-//     its partial maximum of column j+1.  Rows already chosen as pivots are
-//     skipped.  At the sub-panel's end one thread turns the pivot rows into
+//   * sub-panel: one cooperative launch (k_lu_panel), the 64 column steps
+//     exchanging candidates through epoch-tagged words; or, should that launch
+//     be refused or a wait run out, one launch per column (k_lu_step) -- the
+//     same operations in the same order.  The pivot rows are found and used in
+//     place (implicit pivoting): per column the best candidate over all rows
+//     (largest magnitude, lowest row on ties) is the pivot, the other rows'
+//     entries of column j are scaled by its reciprocal and their sub-panel
+//     columns right of j updated (dgetf2's dscal and dger).  Rows already
+//     chosen as pivots are skipped.  At the sub-panel's end one thread turns the pivot rows into
 //     LAPACK's sequence of interchanges, which are then applied to every
 //     column (dlaswp) -- the same rows and values dgetf2 arrives at by
 //     swapping as it goes;
