@@ -674,15 +674,11 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_split(const T *__restrict__ V, 
 // the last workgroup to finish -- counted with one device-scope atomic after
 // a fence, cheap in this kernel since it has written only k+1 sums -- takes
 // the DGKS step on them.  The counter resets itself.
-template <bool FUSED, int NT>
-__device__ __forceinline__ void gs_fin_block(int b, const double *__restrict__ partial, int nb, int k, double *sums,
-                                             unsigned *counter, double *cbuf, double *hcol, ks_status *st, int col,
-                                             int refine, double eta, int do_scale, int safe, double deftol,
-                                             int prescale_ok, int halt_pass) {
+// column b's nb partial sums added in a fixed order; the sum is valid in thread 0
+template <int NT>
+__device__ __forceinline__ double gs_fin_colsum(int b, const double *__restrict__ partial, int nb) {
   constexpr int NW = NT / 64;
   __shared__ double red[NW];
-  __shared__ double fin[KS_NCOLP];
-  __shared__ unsigned ticket;
   const double *p = partial + (size_t)b * nb;
   // FIN_U predicated loads per thread issued together: one memory round trip
   // covers a column of up to FIN_U * NT partials (6144 with 256 threads)
@@ -704,12 +700,23 @@ __device__ __forceinline__ void gs_fin_block(int b, const double *__restrict__ p
   double v = ks_wave_sum((a[0] + a[1]) + a[2]);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
+  double s = 0.0;
   if (threadIdx.x == 0) {
-    double s = 0.0;
 #pragma unroll
     for (int w = 0; w < NW; w++) s += red[w];
-    sums[b] = s;
   }
+  return s;
+}
+
+template <bool FUSED, int NT>
+__device__ __forceinline__ void gs_fin_block(int b, const double *__restrict__ partial, int nb, int k, double *sums,
+                                             unsigned *counter, double *cbuf, double *hcol, ks_status *st, int col,
+                                             int refine, double eta, int do_scale, int safe, double deftol,
+                                             int prescale_ok, int halt_pass) {
+  __shared__ double fin[KS_NCOLP];
+  __shared__ unsigned ticket;
+  const double s = gs_fin_colsum<NT>(b, partial, nb);
+  if (threadIdx.x == 0) sums[b] = s;
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) ticket = atomicAdd(counter, 1u);
@@ -734,6 +741,67 @@ __global__ __launch_bounds__(KS_FIN_BLOCK) void k_gs_fin(const double *__restric
   if (!gs_pass_gate<FUSED>(st, col, pass)) return;
   gs_fin_block<FUSED, KS_FIN_BLOCK>(blockIdx.x, partial, nb, k, sums, counter, cbuf, hcol, st, col, refine, eta,
                                     do_scale, safe, deftol, prescale_ok, halt_pass);
+}
+
+// The same finish without the ticket and its fences: workgroups 0..k publish
+// their column sums as epoch-tagged words, and workgroup k+1 -- dispatched
+// after them -- waits for the k+1 sums (bounded; KS_ERR_TIMEOUT in the status
+// if a sum never comes), runs gs_finish and advances the epoch kept in
+// pub[0] for the next launch (every workgroup has read it by then).
+#define KS_FIN_SPIN (1 << 22)
+template <bool FUSED, int NT>
+__device__ __forceinline__ void gs_fin_pub_block(int b, const double *__restrict__ partial, int nb, int k,
+                                                 unsigned long long *pub, double *cbuf, double *hcol, ks_status *st,
+                                                 int col, int refine, double eta, int do_scale, int safe,
+                                                 double deftol, int prescale_ok, int halt_pass) {
+  unsigned *hdr = (unsigned *)pub;   // [0] the epoch of the last launch
+  const unsigned e_last = __hip_atomic_load(hdr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned epoch = e_last + 1u == 0u ? 1u : e_last + 1u;   // 0 is the cleared words' epoch
+  unsigned long long *sumw = pub + 1;
+  const int tid = threadIdx.x;
+  if (b <= k) {
+    const double s = gs_fin_colsum<NT>(b, partial, nb);
+    if (tid == 0) ks_pub_store_f64(sumw + 2 * b, epoch, s);
+    return;
+  }
+  __shared__ double fin[KS_NCOLP];
+  __shared__ int late;
+  if (tid == 0) late = 0;
+  __syncthreads();
+  for (int j = tid; j <= k; j += NT) {
+    unsigned long long lo, hi;
+    int spins = 0;
+    for (;;) {
+      lo = ks_pub_load(sumw + 2 * j);
+      hi = ks_pub_load(sumw + 2 * j + 1);
+      if ((unsigned)(lo >> 32) == epoch && (unsigned)(hi >> 32) == epoch) break;
+      if (++spins > KS_FIN_SPIN) {
+        late = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    fin[j] = ks_pub_f64(lo, hi);
+  }
+  __syncthreads();
+  gs_finish<FUSED>(fin, k, cbuf, hcol, st, col, refine, eta, do_scale, safe, deftol, prescale_ok, halt_pass);
+  if (tid == 0) {
+    __hip_atomic_store(hdr, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (late) {
+      st->error = KS_ERR_TIMEOUT;
+      st->active = 0;
+    }
+  }
+}
+template <bool FUSED>
+__global__ __launch_bounds__(KS_FIN_BLOCK) void k_gs_fin_pub(const double *__restrict__ partial, int nb, int k,
+                                                            unsigned long long *pub, double *cbuf, double *hcol,
+                                                            ks_status *st, int col, int pass, int refine, double eta,
+                                                            int do_scale, int safe, double deftol, int prescale_ok,
+                                                            int halt_pass) {
+  if (!gs_pass_gate<FUSED>(st, col, pass)) return;
+  gs_fin_pub_block<FUSED, KS_FIN_BLOCK>(blockIdx.x, partial, nb, k, pub, cbuf, hcol, st, col, refine, eta, do_scale,
+                                        safe, deftol, prescale_ok, halt_pass);
 }
 
 template <typename T>
@@ -805,8 +873,13 @@ __global__ __launch_bounds__(KS_BLOCK) void k_grp_fin(ks_grp_gs g, int pass, int
   const int m = grp_member(g.b0_fin, g.nmem, blockIdx.x);
   const ks_grp_gs_member &M = g.m[m];
   if (!gs_pass_gate<FUSED>(M.st, M.col, pass)) return;
-  gs_fin_block<FUSED, KS_BLOCK>((int)blockIdx.x - g.b0_fin[m], M.partial, M.nb, M.k, M.sums, M.counter, M.cbuf, M.hcol, M.st,
-                      M.col, refine, eta, 1, 0, deftol, 1, halt_pass);
+  const int b = (int)blockIdx.x - g.b0_fin[m];
+  if (M.pub)
+    gs_fin_pub_block<FUSED, KS_BLOCK>(b, M.partial, M.nb, M.k, M.pub, M.cbuf, M.hcol, M.st, M.col, refine, eta, 1, 0,
+                                      deftol, 1, halt_pass);
+  else
+    gs_fin_block<FUSED, KS_BLOCK>(b, M.partial, M.nb, M.k, M.sums, M.counter, M.cbuf, M.hcol, M.st, M.col, refine,
+                                  eta, 1, 0, deftol, 1, halt_pass);
 }
 
 __global__ void k_grp_scale(ks_grp_gs g) {
@@ -1366,9 +1439,14 @@ void launch_gs_fin(ks_ctx ctx, const gs_geom &g, int k, double *cbuf, double *hc
                    double eta, int do_scale, int safe, int halt_pass) {
   const double deftol = 10.0 * 2.220446049250313e-16;
   ks_prof_begin(ctx, KS_PROF_GS_FIN, (double)g.nb * (k + 1) * 8.0, (double)g.nb * (k + 1), false);
-  hipLaunchKernelGGL((k_gs_fin<FUSED>), dim3(k + 1), dim3(KS_FIN_BLOCK), 0, ctx->stream, ctx->d_partial, g.nb, k,
-                     ctx->d_gpartial, ctx->d_counters, cbuf, hcol, ctx->d_status, col, pass, refine, eta, do_scale,
-                     safe, deftol, g.cw != 0 || g.kc != 0, halt_pass);
+  if (ctx->d_finpub)
+    hipLaunchKernelGGL((k_gs_fin_pub<FUSED>), dim3(k + 2), dim3(KS_FIN_BLOCK), 0, ctx->stream, ctx->d_partial, g.nb,
+                       k, ctx->d_finpub, cbuf, hcol, ctx->d_status, col, pass, refine, eta, do_scale, safe, deftol,
+                       g.cw != 0 || g.kc != 0, halt_pass);
+  else
+    hipLaunchKernelGGL((k_gs_fin<FUSED>), dim3(k + 1), dim3(KS_FIN_BLOCK), 0, ctx->stream, ctx->d_partial, g.nb, k,
+                       ctx->d_gpartial, ctx->d_counters, cbuf, hcol, ctx->d_status, col, pass, refine, eta, do_scale,
+                       safe, deftol, g.cw != 0 || g.kc != 0, halt_pass);
   ks_prof_end(ctx);
 }
 

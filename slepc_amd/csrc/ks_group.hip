@@ -104,6 +104,7 @@ int ks_group_step(ks_group g, int nmem, const ks_group_member *mem, ks_refine re
     G.sums = M.ctx->d_gpartial;
     G.st = M.ctx->d_status;
     G.counter = M.ctx->d_counters;
+    G.pub = M.ctx->d_finpub;
     G.ld = M.ld;
     G.n = M.n;
     G.k = M.k;
@@ -112,7 +113,7 @@ int ks_group_step(ks_group g, int nmem, const ks_group_member *mem, ks_refine re
     G.col = M.col;
     G.cw = mcw;
     gs.b0_gs[i + 1] = gs.b0_gs[i] + nb;
-    gs.b0_fin[i + 1] = gs.b0_fin[i] + M.k + 1;
+    gs.b0_fin[i + 1] = gs.b0_fin[i] + M.k + (G.pub ? 2 : 1);
     gs.b0_sc[i + 1] = gs.b0_sc[i] + ks_ew_blocks(M.n);
   }
   ks_group_launch_spmv(g->stream, sp);

@@ -90,6 +90,7 @@ struct ks_ctx_s {
   double *d_partial = nullptr;     // [KS_MAXB][KS_NCOLP]
   double *d_gpartial = nullptr;    // [KS_MAXG][KS_NCOLP]
   unsigned *d_counters = nullptr;  // [KS_MAXG + 1], zero between launches
+  unsigned long long *d_finpub = nullptr;   // k_gs_fin's epoch and epoch-tagged sums [1 + 2 * KS_NCOLP]
   double *d_scalar = nullptr;      // small device scratch (norm results)
   double *h_scalar = nullptr;      // pinned host scratch
   ks_status *h_status = nullptr;   // pinned mirror of d_status, filled by ks_status_post
@@ -186,6 +187,7 @@ struct ks_grp_gs_member {
   double *w, *cbuf, *hcol, *partial, *sums;
   ks_status *st;
   unsigned *counter;
+  unsigned long long *pub;         // the context's epoch-tagged finish area (k+2 finishing workgroups), or null
   long ld;
   int n, k, rpb, nb, col;
   int cw;                          // the member's own column split (as when it is grouped alone)
@@ -193,7 +195,7 @@ struct ks_grp_gs_member {
 struct ks_grp_gs {
   int nmem;
   int b0_gs[KS_GROUP_MAX + 1];     // sweep workgroups
-  int b0_fin[KS_GROUP_MAX + 1];    // finishing workgroups (k+1 per member)
+  int b0_fin[KS_GROUP_MAX + 1];    // finishing workgroups (k+2 per member, k+1 with a ticket)
   int b0_sc[KS_GROUP_MAX + 1];     // scaling workgroups
   ks_grp_gs_member m[KS_GROUP_MAX];
 };
@@ -218,6 +220,25 @@ __device__ __forceinline__ double ks_readlane_f64(double v, int k) {
   const long long bits = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)bits, k), hi = __builtin_amdgcn_readlane((int)(bits >> 32), k);
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Epoch-tagged publication: a 64-bit word holds a 32-bit payload (low half)
+// and the epoch it was written in (high half), stored and loaded as one relaxed
+// agent-scope atomic, so a reader that sees the current epoch sees the payload
+// written with it, with no fence on either side.  A double takes two words.
+__device__ __forceinline__ void ks_pub_store(unsigned long long *w, unsigned epoch, unsigned payload) {
+  __hip_atomic_store(w, ((unsigned long long)epoch << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void ks_pub_store_f64(unsigned long long *w, unsigned epoch, double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  ks_pub_store(w, epoch, (unsigned)(b & 0xffffffffull));
+  ks_pub_store(w + 1, epoch, (unsigned)(b >> 32));
+}
+__device__ __forceinline__ unsigned long long ks_pub_load(const unsigned long long *w) {
+  return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ks_pub_f64(unsigned long long lo, unsigned long long hi) {
+  return __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
 }
 
 __device__ __forceinline__ double ks_wave_sum(double v) {

@@ -104,11 +104,16 @@ int ks_ctx_create(ks_ctx *out) {
   KS_HIP(hipMalloc(&c->d_partial, sizeof(double) * (size_t)KS_MAXB * KS_NCOLP));
   KS_HIP(hipMalloc(&c->d_gpartial, sizeof(double) * (size_t)KS_MAXG * KS_NCOLP));
   KS_HIP(hipMalloc(&c->d_counters, sizeof(unsigned) * (KS_MAXG + 1)));
+  {
+    const char *t = getenv("KS_GS_FIN_TICKET");   // 1: finishing launches count with a ticket instead
+    if (!(t && t[0] == '1')) KS_HIP(hipMalloc(&c->d_finpub, sizeof(unsigned long long) * (1 + 2 * KS_NCOLP)));
+  }
   KS_HIP(hipMalloc(&c->d_scalar, sizeof(double) * 64));
   KS_HIP(hipHostMalloc(&c->h_scalar, sizeof(double) * 64, hipHostMallocDefault));
   KS_HIP(hipHostMalloc((void **)&c->h_status, sizeof(ks_status), hipHostMallocDefault));
   int e = ks_memset_zero(c, c->d_counters, sizeof(unsigned) * (KS_MAXG + 1));
   if (e) return e;
+  if (c->d_finpub && (e = ks_memset_zero(c, c->d_finpub, sizeof(unsigned long long) * (1 + 2 * KS_NCOLP)))) return e;
   hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(64), 0, c->stream, c->d_status);
   KS_HIP(hipGetLastError());
   KS_HIP(hipStreamSynchronize(c->stream));
@@ -124,6 +129,7 @@ int ks_ctx_destroy(ks_ctx c) {
   (void)hipFree(c->d_status);
   (void)hipFree(c->d_partial);
   (void)hipFree(c->d_gpartial);
+  if (c->d_finpub) (void)hipFree(c->d_finpub);
   (void)hipFree(c->d_counters);
   (void)hipFree(c->d_scalar);
   if (c->d_ran) (void)hipFree(c->d_ran);
