@@ -202,6 +202,47 @@ def test_orthonormalize_widths(ks, ctx, oracle, j):
         assert np.abs(r["Vg"][:, :j].T @ r["Vg"][:, j]).max() < 1e-14
 
 
+@pytest.mark.parametrize("prec,j", [("f64", 9), ("f64", 24), ("f64", 48), ("f64", 96), ("f64", 200),
+                                    ("f32", 40), ("f32", 150)])
+def test_finish_published_equals_ticket_bitwise(ks, ctx, prec, j):
+    # the finishing launch publishes the column sums as epoch-tagged words and
+    # takes the DGKS step in one more workgroup; a context made with
+    # KS_GS_FIN_TICKET=1 counts with a ticket and fences instead: the same sums,
+    # so the same column, coefficients and status bit for bit (three calls in a
+    # row, so later calls run on advanced epochs)
+    import os
+    rng = np.random.default_rng(300 + j)
+    n = 40009
+    dt = np.float64 if prec == "f64" else np.float32
+    p = ks.F64 if prec == "f64" else ks.F32
+    os.environ["KS_GS_FIN_TICKET"] = "1"
+    try:
+        ctx_t = ks.Context(0)
+    finally:
+        del os.environ["KS_GS_FIN_TICKET"]
+    try:
+        for rep in range(3):
+            V = np.zeros((n, j + 1), order="F")
+            V[:, :j] = np.linalg.qr(rng.standard_normal((n, j)))[0]
+            V[:, j] = rng.standard_normal(n) + V[:, :j] @ rng.standard_normal(j)
+            V = np.asfortranarray(V.astype(dt))
+            out = []
+            for c in (ctx, ctx_t):
+                dV = ks.DeviceArray.from_numpy(c, V)
+                buf = ks.DeviceArray(c, (j + 2, j + 2))
+                c.status_reset()
+                ks.orthonormalize_column(c, p, n, dV, n, j, buf, j + 2, j, j)
+                st = c.status()
+                out.append((dV.numpy()[:, j], buf.numpy()[:j + 1, j], st.total_passes, st.nrm, st.lindep, st.error))
+            (v0, h0, p0, n0, l0, e0), (v1, h1, p1, n1, l1, e1) = out
+            assert e0 == 0 and e1 == 0
+            assert np.array_equal(v0, v1) and np.array_equal(h0, h1)
+            assert (p0, n0, l0) == (p1, n1, l1)
+            assert abs(np.linalg.norm(v0.astype(np.float64)) - 1) < (1e-13 if prec == "f64" else 1e-5)
+    finally:
+        ctx_t.close()
+
+
 @pytest.mark.parametrize("j", [5, 40, 60, 150])
 def test_orthonormalize_fp32(ks, ctx, oracle, j):
     # FP32 basis: FP32 loads/stores, FP64 sums and DGKS scalars; checked against the
