@@ -523,6 +523,15 @@ def run_copycal(args):
     print(json.dumps(out), flush=True)
 
 
+def _basis_ld(n, esz):
+    """The leading dimension the GPU basis uses (BVHIPKSLeadingDimension): n rounded
+    up to 64, plus 2 KB when that column stride is a multiple of 4 KB."""
+    ld = ((n + 63) // 64) * 64
+    if (ld * esz) % 4096 == 0:
+        ld += 2048 // esz
+    return ld
+
+
 def run_restartgemm(args):
     """The restart product V(:, 0:N) = V(:, 0:K) Q (BVMultInPlace, ks_bv_multinplace)
     alone, at the Krylov-Schur restart shapes of the 128^3 solve (n = 2^21, K = 48,
@@ -532,7 +541,7 @@ def run_restartgemm(args):
     ctx = K.Context(0)
     rows = []
     for n, k, nn in [(1 << 21, 48, 32), (1 << 24, 96, 64)]:
-        ld = ((n + 63) // 64) * 64
+        ld = _basis_ld(n, 8)
         V = K.DeviceArray(ctx, (ld * k,))
         K.random(ctx, K.F64, ld * k, V, 3)
         rng = np.random.default_rng(k)

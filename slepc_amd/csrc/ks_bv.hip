@@ -1363,7 +1363,8 @@ int gs_geometry(int n, int k, size_t esize, gs_geom *g) {
     if (esize == 8 && g->cw <= KS_GS_RPL2_MAXCW) g->rpl = 2;
     const long step = esize == 4 ? 128 : 64 * g->rpl;                   // rows per workgroup step, even
     const long tiles = (n + step - 1) / step;
-    const long nb = tiles < KS_GS_SPLIT_BLOCKS ? tiles : KS_GS_SPLIT_BLOCKS;
+    const long cap = esize == 4 ? KS_GS_SPLIT_BLOCKS32 : KS_GS_SPLIT_BLOCKS;
+    const long nb = tiles < cap ? tiles : cap;
     const long tpb = (tiles + nb - 1) / nb;
     g->cpb = (int)(tpb * step);                                         // rows per workgroup
     g->nb = (int)((n + (long)g->cpb - 1) / g->cpb);
@@ -1510,7 +1511,7 @@ int orthonormalize_t(ks_ctx ctx, int n, T *V, long ld, int k, double *buf, int l
 #define KS_NOALLOC_COUNTER 32
 #define KS_NOALLOC_SUMS 64
 #define KS_NOALLOC_PARTIAL 448
-#define KS_NOALLOC_PARTIAL_CAP ((size_t)(KS_NOALLOC_MAXK + 1) * KS_GS_SPLIT_BLOCKS)
+#define KS_NOALLOC_PARTIAL_CAP ((size_t)(KS_NOALLOC_MAXK + 1) * 3072)   // partial sums of up to 3072 workgroups
 #define KS_NOALLOC_DOUBLES (KS_NOALLOC_PARTIAL + KS_NOALLOC_PARTIAL_CAP)
 static_assert(sizeof(ks_status) <= 32 * sizeof(double), "status slot too small");
 static_assert(KS_NOALLOC_SUMS + KS_NCOLP <= KS_NOALLOC_PARTIAL, "column-sum slot too small");

@@ -31,7 +31,10 @@ __device__ __forceinline__ float2 ks_stream_load2(const float *p) {
 }
 
 #ifndef KS_GS_SPLIT_BLOCKS
-#define KS_GS_SPLIT_BLOCKS 3072      // workgroups of the column-split Gram-Schmidt
+#define KS_GS_SPLIT_BLOCKS 1024      // workgroups of the column-split Gram-Schmidt (FP64 basis)
+#endif
+#ifndef KS_GS_SPLIT_BLOCKS32
+#define KS_GS_SPLIT_BLOCKS32 2048    // the same for an FP32 basis
 #endif
 #ifndef KS_GS_RPL2_MAXCW
 #define KS_GS_RPL2_MAXCW 12          // FP64 column-split sweeps with <= this many columns per wave: 2 rows per lane

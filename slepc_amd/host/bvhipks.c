@@ -8,7 +8,8 @@
  * members of its struct _BVOps (include/slepc/private/bvimpl.h:25-113); all of
  * its own state is behind bv->data:
  *
- *   storage   V: ld x (nc+m) column-major in HBM, ld = n rounded up to 64; the
+ *   storage   V: ld x (nc+m) column-major in HBM, ld = n rounded up to 64 (plus 2 KB when
+ *             that stride is a multiple of 4 KB, BVHIPKSLeadingDimension); the
  *             coefficient buffer is the interface's bv->buffer (a device Vec
  *             with a pinned host copy); Q staging for BVMultInPlace in HBM
  *   ops       dotvec/multvec (tall-skinny kernels), scale/norm/copycolumn,
@@ -560,7 +561,7 @@ PetscErrorCode BVCreate_HIPKS(BV bv)
   h->prec = BVPrecision(bv);
   h->ext = PETSC_TRUE;
   bv->hip = PETSC_TRUE;
-  bv->ld = ((bv->n + 63) / 64) * 64;
+  bv->ld = BVHIPKSLeadingDimension(bv->n, esz(bv));
   h->ncols = bv->nc + bv->m;
   KSCall(ks_malloc(&h->dV, esz(bv) * (size_t)bv->ld * (size_t)h->ncols));
   KSCall(ks_memset_zero(h->ctx, h->dV, esz(bv) * (size_t)bv->ld * (size_t)h->ncols));

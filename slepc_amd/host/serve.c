@@ -63,7 +63,8 @@ struct _p_SlepcServer {
 static size_t RequestBytes(const SlepcRequest *r)
 {
   const size_t al = 256;
-  size_t n = (size_t)r->n, nnz = (size_t)r->rowptr[r->n], ld = (n + 63) / 64 * 64;
+  size_t n = (size_t)r->n, nnz = (size_t)r->rowptr[r->n], ld = (size_t)BVHIPKSLeadingDimension(r->n, 8);
+  size_t ld32 = (size_t)BVHIPKSLeadingDimension(r->n, 4);
   size_t m = (size_t)r->ncv + 1, b = 0, vs = r->val32 ? 4 : 8;
   int mixed = r->mixed_tol > 0.0 && r->mixed_tol > r->tol;
 #define ROUND(x) (((x) + al - 1) / al * al)
@@ -71,7 +72,7 @@ static size_t RequestBytes(const SlepcRequest *r)
   b += ROUND(8 * ld * m) + ROUND(8 * m * m) + ROUND(8 * (m + 2)) + ROUND(8 * m * m);
   b += 7 * ROUND(8 * n);
   if (mixed) {   /* the FP32 phase: its basis, buffers and vectors, and an FP32 copy of FP64 values */
-    b += ROUND(4 * ld * m) + ROUND(8 * m * m) + ROUND(8 * (m + 2)) + ROUND(8 * m * m) + 7 * ROUND(4 * n);
+    b += ROUND(4 * ld32 * m) + ROUND(8 * m * m) + ROUND(8 * (m + 2)) + ROUND(8 * m * m) + 7 * ROUND(4 * n);
     if (!r->val32) b += ROUND(4 * (n + 1)) + ROUND(4 * (nnz ? nnz : 1)) * 2 + ROUND(4 * (n + 2));
   }
   if (r->twosided) {   /* A^T (CSR with its row-block schedule), the left basis, buffers and vectors, M */
