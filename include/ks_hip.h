@@ -72,6 +72,8 @@ typedef struct {
   int32_t prescale_pass;   /* ... and this pass */
   int32_t halt_col;        /* a lean orthonormalization stopped the expansion at this column (-1: none) ... */
   int32_t halt_need;       /* ... and needs 1: the third DGKS pass, 2: the scaling (ks_bv_orthonormalize_resume) */
+  int32_t defer_col;       /* the first fused pass of this column did not store its vector (-1: none): */
+  int32_t defer_pass;      /* ... the sweep of the pass after this one forms it again from the same loads */
 } ks_status;
 
 /* ---------------------------------------------------------------- runtime */

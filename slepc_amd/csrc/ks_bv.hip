@@ -243,17 +243,26 @@ __device__ inline void gs_predict_last(ks_status *st, int col, int refine, doubl
 // halt_pass > 0 (the lean sequence): the finish of that pass stops the
 // expansion when the column needs a launch the lean sequence left out -- a
 // further DGKS pass, or a scaling its last sweep did not apply.
+// coef1 (first pass, split sweeps only; null: never): when the first fused pass
+// is certain not to be the last -- its finish will decide from |v| and the
+// coefficients known here, as gs_predict_last reads them -- that pass leaves
+// its vector unstored and the next sweep forms it again from the loads it
+// makes anyway (the same operations, so the same bits), saving a write of the
+// column.  c of the first pass is kept in coef1 for that.
 template <bool FUSED>
 __device__ void gs_finish(const double *fin, int k, double *cbuf, double *hcol, ks_status *st, int col,
                           int refine, double eta, int do_scale, int safe, double deftol, int prescale_ok,
-                          int halt_pass) {
+                          int halt_pass, double *coef1 = nullptr) {
   __shared__ int s_continue;
   const int tid = threadIdx.x;
   if (!FUSED) {
     // first pass: BV_CleanCoefficients, c, |v| (BVDotColumnInc + BV_SquareRoot)
     for (int c = tid; c <= k; c += blockDim.x) hcol[c] = 0.0;
     __syncthreads();
-    for (int c = tid; c < k; c += blockDim.x) cbuf[c] = fin[c];
+    for (int c = tid; c < k; c += blockDim.x) {
+      cbuf[c] = fin[c];
+      if (coef1) coef1[c] = fin[c];
+    }
     if (tid == 0) {
       const double self = fin[k];
       double sumsq = 0.0;
@@ -270,6 +279,22 @@ __device__ void gs_finish(const double *fin, int k, double *cbuf, double *hcol, 
       st->lindep = 0;
       if (!st->error) gs_predict_last(st, col, refine, eta, do_scale, safe, prescale_ok);
       else st->prescale_col = -1;
+      st->defer_col = -1;
+      if (coef1 && !st->error && !safe && refine != KS_REFINE_NEVER) {
+        // the first fused pass's finish (l = 1) continues when ALWAYS (l < 2), or
+        // when the estimate is positive and below eta |v| (IFNEEDED, l < 3)
+        bool more = refine == KS_REFINE_ALWAYS;
+        const double onrm = st->onrm;
+        double e = onrm * onrm - st->sumsq;
+        if (!more && e > 0.0) {
+          e = sqrt(e);
+          more = refine == KS_REFINE_IFNEEDED && e != 0.0 && fabs(e) < eta * fabs(onrm);
+        }
+        if (more) {
+          st->defer_col = col;
+          st->defer_pass = 1;
+        }
+      }
     }
     return;
   }
@@ -457,6 +482,13 @@ __device__ __forceinline__ double gs_prescale(const ks_status *st, int col, int 
   return (FUSED && st->prescale_col == col && st->prescale_pass == pass) ? st->prescale : 1.0;
 }
 
+// the deferred-store role of this fused pass's sweep (see gs_split_block)
+template <bool FUSED>
+__device__ __forceinline__ int gs_defer(const ks_status *st, int col, int pass) {
+  if (!FUSED || st->defer_col != col) return 0;
+  return pass == st->defer_pass ? 1 : (pass == st->defer_pass + 1 ? 2 : 0);
+}
+
 template <typename T, int KC, bool FUSED>
 __global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, long ld, int k, T *__restrict__ w,
                                                     int n, int rpb, const double *__restrict__ cbuf,
@@ -538,22 +570,34 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_reg(const T *__restrict__ V, lo
 // 8-byte aligned columns): each lane takes two adjacent rows with one 8-byte
 // load per column, so a wave still moves 512 contiguous bytes per column per
 // step, as with FP64 data.
-template <typename T, int CW, bool FUSED, int RPL, bool PAIR>
+// defer (fused passes): 1 -- this pass leaves its vector unstored; 2 -- the
+// previous pass did, so each step first forms that pass's vector from the
+// loaded rows and coef1 (the previous coefficients), with the operations that
+// pass used, then applies this pass's coefficients to it.
+// REFORM: the sweep can take defer == 2 (its extra sums cost registers, so only
+// the launches that may follow a deferred pass are built with it).
+template <typename T, int CW, bool FUSED, int RPL, bool PAIR, bool REFORM = false>
 __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restrict__ V, long ld, int k,
                                                T *__restrict__ w, int n, int rpb, const double *__restrict__ cbuf,
-                                               double *__restrict__ partial, double wscale) {
+                                               double *__restrict__ partial, double wscale, int defer = 0,
+                                               const double *__restrict__ coef1 = nullptr) {
   static_assert(!PAIR || (sizeof(T) == 4 && RPL == 2), "row pairs are two adjacent FP32 rows");
   using AT = typename acc_t<T>::type;   // FP32 data: FP32 products and per-thread sums
   constexpr int STEP = 64 * RPL;        // rows per step; each lane takes RPL of them
   __shared__ AT coef[KS_WAVES * CW];
   __shared__ AT red[2][KS_WAVES][STEP];
+  __shared__ AT coefp[REFORM ? KS_WAVES * CW : 1];          // coef1, when defer == 2
+  __shared__ AT redp[REFORM ? 2 : 1][KS_WAVES][REFORM ? STEP : 1];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: column guards are scalar branches
   const int c0 = wave * CW;
   // this lane's q-th row of a step: adjacent pairs (one 8-byte load) or 64 apart
   auto loff = [&](int q) { return PAIR ? RPL * lane + q : 64 * q + lane; };
   if (FUSED)
-    for (int c = tid; c < KS_WAVES * CW; c += KS_BLOCK) coef[c] = (AT)(c < k ? cbuf[c] : 0.0);
+    for (int c = tid; c < KS_WAVES * CW; c += KS_BLOCK) {
+      coef[c] = (AT)(c < k ? cbuf[c] : 0.0);
+      if (REFORM && defer == 2) coefp[c] = (AT)(c < k ? coef1[c] : 0.0);
+    }
   __syncthreads();
   AT acc[CW];
 #pragma unroll
@@ -607,17 +651,26 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
 #pragma unroll
         for (int u = 0; u < CW; u++) s += (AT)v[q][u] * coef[c0 + u];
         red[buf][wave][loff(q)] = s;
+        if (REFORM && defer == 2) {
+          AT sp = (AT)0;
+#pragma unroll
+          for (int u = 0; u < CW; u++) sp += (AT)v[q][u] * coefp[c0 + u];
+          redp[buf][wave][loff(q)] = sp;
+        }
       }
       __syncthreads();
 #pragma unroll
       for (int q = 0; q < RPL; q++) {
         const int r = loff(q);
         const unsigned i = i0 + r;
+        AT base = wi[q];
+        if (REFORM && defer == 2)   // the previous pass's vector, as that pass formed it
+          base = (AT)(T)(base - (((redp[buf][0][r] + redp[buf][1][r]) + redp[buf][2][r]) + redp[buf][3][r]));
         const AT tot = ((red[buf][0][r] + red[buf][1][r]) + red[buf][2][r]) + red[buf][3][r];
-        const T t = (T)(wi[q] - tot);
+        const T t = (T)(base - tot);
         wi[q] = i < r1 ? (AT)t : (AT)0;
         // the final pass of a column stores it normalized, rounded as k_scale_status would
-        if (wave == 0 && i < r1) w[i] = wscale == 1.0 ? t : (T)((AT)t * (AT)wscale);
+        if (defer != 1 && wave == 0 && i < r1) w[i] = wscale == 1.0 ? t : (T)((AT)t * (AT)wscale);
       }
       buf ^= 1;
     }
@@ -654,15 +707,16 @@ __device__ __forceinline__ void gs_split_block(int b, int nb, const T *__restric
   }
 }
 
-template <typename T, int CW, bool FUSED, int RPL, bool PAIR>
+template <typename T, int CW, bool FUSED, int RPL, bool PAIR, bool REFORM>
 __global__ __launch_bounds__(KS_BLOCK) void k_gs_split(const T *__restrict__ V, long ld, int k, T *__restrict__ w,
                                                       int n, int rpb, const double *__restrict__ cbuf,
                                                       double *__restrict__ partial, const ks_status *st, int col,
-                                                      int pass, int *ran) {
+                                                      int pass, int *ran, const double *__restrict__ coef1) {
   if (!gs_pass_gate<FUSED>(st, col, pass)) return;
   if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = 1;
-  gs_split_block<T, CW, FUSED, RPL, PAIR>(blockIdx.x, gridDim.x, V, ld, k, w, n, rpb, cbuf, partial,
-                                          gs_prescale<FUSED>(st, col, pass));
+  gs_split_block<T, CW, FUSED, RPL, PAIR, REFORM>(blockIdx.x, gridDim.x, V, ld, k, w, n, rpb, cbuf, partial,
+                                                  gs_prescale<FUSED>(st, col, pass), gs_defer<FUSED>(st, col, pass),
+                                                  coef1);
 }
 
 // The finish of a Gram-Schmidt pass: a separate launch, ordered by the stream
@@ -712,7 +766,7 @@ template <bool FUSED, int NT>
 __device__ __forceinline__ void gs_fin_block(int b, const double *__restrict__ partial, int nb, int k, double *sums,
                                              unsigned *counter, double *cbuf, double *hcol, ks_status *st, int col,
                                              int refine, double eta, int do_scale, int safe, double deftol,
-                                             int prescale_ok, int halt_pass) {
+                                             int prescale_ok, int halt_pass, double *coef1 = nullptr) {
   __shared__ double fin[KS_NCOLP];
   __shared__ unsigned ticket;
   const double s = gs_fin_colsum<NT>(b, partial, nb);
@@ -726,7 +780,7 @@ __device__ __forceinline__ void gs_fin_block(int b, const double *__restrict__ p
   for (int c = threadIdx.x; c <= k; c += blockDim.x) fin[c] = __builtin_nontemporal_load(&sums[c]);
   if (threadIdx.x == 0) *counter = 0u;
   __syncthreads();
-  gs_finish<FUSED>(fin, k, cbuf, hcol, st, col, refine, eta, do_scale, safe, deftol, prescale_ok, halt_pass);
+  gs_finish<FUSED>(fin, k, cbuf, hcol, st, col, refine, eta, do_scale, safe, deftol, prescale_ok, halt_pass, coef1);
 }
 
 // one 256-thread workgroup per column (1024-thread workgroups, which read a
@@ -737,10 +791,10 @@ __global__ __launch_bounds__(KS_FIN_BLOCK) void k_gs_fin(const double *__restric
                                                         double *sums, unsigned *counter, double *cbuf, double *hcol,
                                                         ks_status *st, int col, int pass, int refine, double eta,
                                                         int do_scale, int safe, double deftol, int prescale_ok,
-                                                        int halt_pass) {
+                                                        int halt_pass, double *coef1) {
   if (!gs_pass_gate<FUSED>(st, col, pass)) return;
   gs_fin_block<FUSED, KS_FIN_BLOCK>(blockIdx.x, partial, nb, k, sums, counter, cbuf, hcol, st, col, refine, eta,
-                                    do_scale, safe, deftol, prescale_ok, halt_pass);
+                                    do_scale, safe, deftol, prescale_ok, halt_pass, coef1);
 }
 
 // The same finish without the ticket and its fences: workgroups 0..k publish
@@ -753,7 +807,8 @@ template <bool FUSED, int NT>
 __device__ __forceinline__ void gs_fin_pub_block(int b, const double *__restrict__ partial, int nb, int k,
                                                  unsigned long long *pub, double *cbuf, double *hcol, ks_status *st,
                                                  int col, int refine, double eta, int do_scale, int safe,
-                                                 double deftol, int prescale_ok, int halt_pass) {
+                                                 double deftol, int prescale_ok, int halt_pass,
+                                                 double *coef1 = nullptr) {
   unsigned *hdr = (unsigned *)pub;   // [0] the epoch of the last launch
   const unsigned e_last = __hip_atomic_load(hdr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const unsigned epoch = e_last + 1u == 0u ? 1u : e_last + 1u;   // 0 is the cleared words' epoch
@@ -784,7 +839,7 @@ __device__ __forceinline__ void gs_fin_pub_block(int b, const double *__restrict
     fin[j] = ks_pub_f64(lo, hi);
   }
   __syncthreads();
-  gs_finish<FUSED>(fin, k, cbuf, hcol, st, col, refine, eta, do_scale, safe, deftol, prescale_ok, halt_pass);
+  gs_finish<FUSED>(fin, k, cbuf, hcol, st, col, refine, eta, do_scale, safe, deftol, prescale_ok, halt_pass, coef1);
   if (tid == 0) {
     __hip_atomic_store(hdr, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (late) {
@@ -798,10 +853,10 @@ __global__ __launch_bounds__(KS_FIN_BLOCK) void k_gs_fin_pub(const double *__res
                                                             unsigned long long *pub, double *cbuf, double *hcol,
                                                             ks_status *st, int col, int pass, int refine, double eta,
                                                             int do_scale, int safe, double deftol, int prescale_ok,
-                                                            int halt_pass) {
+                                                            int halt_pass, double *coef1) {
   if (!gs_pass_gate<FUSED>(st, col, pass)) return;
   gs_fin_pub_block<FUSED, KS_FIN_BLOCK>(blockIdx.x, partial, nb, k, pub, cbuf, hcol, st, col, refine, eta, do_scale,
-                                        safe, deftol, prescale_ok, halt_pass);
+                                        safe, deftol, prescale_ok, halt_pass, coef1);
 }
 
 template <typename T>
@@ -850,10 +905,12 @@ __global__ __launch_bounds__(KS_BLOCK) void k_grp_gs(ks_grp_gs g, int pass) {
   if (!gs_pass_gate<FUSED>(M.st, M.col, pass)) return;
   const int b = (int)blockIdx.x - g.b0_gs[m];
   const double pre = gs_prescale<FUSED>(M.st, M.col, pass);
+  const int defer = gs_defer<FUSED>(M.st, M.col, pass);
   switch (M.cw) {
 #define KS_GRP_CW(C)                                                                                                 \
   case C:                                                                                                            \
-    gs_split_block<double, C, FUSED, 1, false>(b, M.nb, M.V, M.ld, M.k, M.w, M.n, M.rpb, M.cbuf, M.partial, pre);    \
+    gs_split_block<double, C, FUSED, 1, false>(b, M.nb, M.V, M.ld, M.k, M.w, M.n, M.rpb, M.cbuf, M.partial, pre,     \
+                                               defer, M.coef1);                                                      \
     break;
     KS_GRP_CW(2)
     KS_GRP_CW(4)
@@ -862,7 +919,8 @@ __global__ __launch_bounds__(KS_BLOCK) void k_grp_gs(ks_grp_gs g, int pass) {
     KS_GRP_CW(16)
     KS_GRP_CW(24)
     default:
-      gs_split_block<double, 32, FUSED, 1, false>(b, M.nb, M.V, M.ld, M.k, M.w, M.n, M.rpb, M.cbuf, M.partial, pre);
+      gs_split_block<double, 32, FUSED, 1, false>(b, M.nb, M.V, M.ld, M.k, M.w, M.n, M.rpb, M.cbuf, M.partial, pre,
+                                                  defer, M.coef1);
 #undef KS_GRP_CW
   }
 }
@@ -876,10 +934,10 @@ __global__ __launch_bounds__(KS_BLOCK) void k_grp_fin(ks_grp_gs g, int pass, int
   const int b = (int)blockIdx.x - g.b0_fin[m];
   if (M.pub)
     gs_fin_pub_block<FUSED, KS_BLOCK>(b, M.partial, M.nb, M.k, M.pub, M.cbuf, M.hcol, M.st, M.col, refine, eta, 1, 0,
-                                      deftol, 1, halt_pass);
+                                      deftol, 1, halt_pass, M.coef1);
   else
     gs_fin_block<FUSED, KS_BLOCK>(b, M.partial, M.nb, M.k, M.sums, M.counter, M.cbuf, M.hcol, M.st, M.col, refine,
-                                  eta, 1, 0, deftol, 1, halt_pass);
+                                  eta, 1, 0, deftol, 1, halt_pass, M.coef1);
 }
 
 __global__ void k_grp_scale(ks_grp_gs g) {
@@ -1394,26 +1452,36 @@ int gs_geometry(int n, int k, size_t esize, gs_geom *g) {
   return KS_OK;
 }
 
+template <typename T, bool FUSED, bool RF>
+void launch_split(ks_ctx ctx, const gs_geom &g, const T *V, long ld, int k, T *w, int n, const double *cbuf, int col,
+                  int pass, int *ran) {
+  dim3 grid(g.nb), block(KS_BLOCK);
+#define KS_GSS_CASE(CC)                                                                                           \
+  case CC:                                                                                                        \
+    if (g.pair)                                                                                                   \
+      hipLaunchKernelGGL((k_gs_split<T, CC, FUSED, 2, sizeof(T) == 4, RF>), grid, block, 0, ctx->stream, V, ld, k, w,  \
+                         n, g.cpb, cbuf, ctx->d_partial, ctx->d_status, col, pass, ran, ctx->d_coef1);                          \
+    else if (sizeof(T) == 8 && g.rpl == 2)                                                                        \
+      hipLaunchKernelGGL((k_gs_split<T, CC, FUSED, 2, false, RF>), grid, block, 0, ctx->stream, V, ld, k, w, n, g.cpb, \
+                         cbuf, ctx->d_partial, ctx->d_status, col, pass, ran, ctx->d_coef1);                                    \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_gs_split<T, CC, FUSED, 1, false, RF>), grid, block, 0, ctx->stream, V, ld, k, w, n, g.cpb, \
+                         cbuf, ctx->d_partial, ctx->d_status, col, pass, ran, ctx->d_coef1);                                    \
+    break;
+    switch (g.cw) { KS_GSS_CASE(2) KS_GSS_CASE(4) KS_GSS_CASE(8) KS_GSS_CASE(12) KS_GSS_CASE(16) KS_GSS_CASE(24)
+                    KS_GSS_CASE(32) }
+#undef KS_GSS_CASE
+}
+
 template <typename T, bool FUSED>
 void launch_gs(ks_ctx ctx, const gs_geom &g, const T *V, long ld, int k, T *w, int n, const double *cbuf, int col,
                int pass, int *ran) {
   dim3 grid(g.nb), block(KS_BLOCK);
   if (g.cw) {
-#define KS_GSS_CASE(CC)                                                                                           \
-  case CC:                                                                                                        \
-    if (g.pair)                                                                                                   \
-      hipLaunchKernelGGL((k_gs_split<T, CC, FUSED, 2, sizeof(T) == 4>), grid, block, 0, ctx->stream, V, ld, k, w,  \
-                         n, g.cpb, cbuf, ctx->d_partial, ctx->d_status, col, pass, ran);                          \
-    else if (sizeof(T) == 8 && g.rpl == 2)                                                                        \
-      hipLaunchKernelGGL((k_gs_split<T, CC, FUSED, 2, false>), grid, block, 0, ctx->stream, V, ld, k, w, n, g.cpb, \
-                         cbuf, ctx->d_partial, ctx->d_status, col, pass, ran);                                    \
-    else                                                                                                          \
-      hipLaunchKernelGGL((k_gs_split<T, CC, FUSED, 1, false>), grid, block, 0, ctx->stream, V, ld, k, w, n, g.cpb, \
-                         cbuf, ctx->d_partial, ctx->d_status, col, pass, ran);                                    \
-    break;
-    switch (g.cw) { KS_GSS_CASE(2) KS_GSS_CASE(4) KS_GSS_CASE(8) KS_GSS_CASE(12) KS_GSS_CASE(16) KS_GSS_CASE(24)
-                    KS_GSS_CASE(32) }
-#undef KS_GSS_CASE
+    if (FUSED && pass == 2)   // may follow a deferred first pass
+      launch_split<T, FUSED, true>(ctx, g, V, ld, k, w, n, cbuf, col, pass, ran);
+    else
+      launch_split<T, FUSED, false>(ctx, g, V, ld, k, w, n, cbuf, col, pass, ran);
     return;
   }
   if (g.kc) {
@@ -1440,14 +1508,15 @@ void launch_gs_fin(ks_ctx ctx, const gs_geom &g, int k, double *cbuf, double *hc
                    double eta, int do_scale, int safe, int halt_pass) {
   const double deftol = 10.0 * 2.220446049250313e-16;
   ks_prof_begin(ctx, KS_PROF_GS_FIN, (double)g.nb * (k + 1) * 8.0, (double)g.nb * (k + 1), false);
+  double *coef1 = g.cw ? ctx->d_coef1 : nullptr;   // deferred first-pass store: split sweeps only
   if (ctx->d_finpub)
     hipLaunchKernelGGL((k_gs_fin_pub<FUSED>), dim3(k + 2), dim3(KS_FIN_BLOCK), 0, ctx->stream, ctx->d_partial, g.nb,
                        k, ctx->d_finpub, cbuf, hcol, ctx->d_status, col, pass, refine, eta, do_scale, safe, deftol,
-                       g.cw != 0 || g.kc != 0, halt_pass);
+                       g.cw != 0 || g.kc != 0, halt_pass, coef1);
   else
     hipLaunchKernelGGL((k_gs_fin<FUSED>), dim3(k + 1), dim3(KS_FIN_BLOCK), 0, ctx->stream, ctx->d_partial, g.nb, k,
                        ctx->d_gpartial, ctx->d_counters, cbuf, hcol, ctx->d_status, col, pass, refine, eta, do_scale,
-                       safe, deftol, g.cw != 0 || g.kc != 0, halt_pass);
+                       safe, deftol, g.cw != 0 || g.kc != 0, halt_pass, coef1);
   ks_prof_end(ctx);
 }
 
@@ -1484,9 +1553,15 @@ int orthonormalize_t(ks_ctx ctx, int n, T *V, long ld, int k, double *buf, int l
     launch_gs_fin<false>(ctx, g, k, cbuf, hcol, col, 0, refine, eta, do_scale, safe, 0);
   }
   // each pass: update with its c and the next pass's dots in the same sweep
+  // with the first pass's store deferrable (split sweeps, a refining pass
+  // follows) that pass is counted without its write -- it stores only in the
+  // rare column it is also the last pass of -- and the next one with the
+  // re-formed product's flops
+  const bool deferrable = g.cw && ctx->d_coef1 && !safe && refine != KS_REFINE_NEVER;
   for (int p = p_first; p <= p_last; p++) {
-    ran = ks_prof_begin(ctx, p == 1 ? KS_PROF_GS_UPD1 : KS_PROF_GS_UPDR, (k + 2.0) * n * s,
-                        2.0 * k * n + 2.0 * (k + 1.0) * n, true);
+    const double bytes = (p == 1 && deferrable ? k + 1.0 : k + 2.0) * n * s;
+    const double flops = 2.0 * k * n + 2.0 * (k + 1.0) * n + (p == 2 && deferrable ? 2.0 * k * n : 0.0);
+    ran = ks_prof_begin(ctx, p == 1 ? KS_PROF_GS_UPD1 : KS_PROF_GS_UPDR, bytes, flops, true);
     launch_gs<T, true>(ctx, g, V, ld, k, w, n, cbuf, col, p, ran);
     ks_prof_end(ctx);
     launch_gs_fin<true>(ctx, g, k, cbuf, hcol, col, p, refine, eta, do_scale, safe, halt_pass);
@@ -1539,6 +1614,7 @@ __global__ void k_noalloc_reset(ks_status *st, unsigned *counter) {
     st->prescale_pass = 0;
     st->halt_col = -1;
     st->halt_need = 0;
+    st->defer_col = -1;
     *counter = 0u;
   }
 }

@@ -105,6 +105,7 @@ int ks_group_step(ks_group g, int nmem, const ks_group_member *mem, ks_refine re
     G.st = M.ctx->d_status;
     G.counter = M.ctx->d_counters;
     G.pub = M.ctx->d_finpub;
+    G.coef1 = nullptr;   // grouped passes store every pass's vector (their sweep kernel takes no re-forming pass)
     G.ld = M.ld;
     G.n = M.n;
     G.k = M.k;

@@ -94,6 +94,7 @@ struct ks_ctx_s {
   double *d_gpartial = nullptr;    // [KS_MAXG][KS_NCOLP]
   unsigned *d_counters = nullptr;  // [KS_MAXG + 1], zero between launches
   unsigned long long *d_finpub = nullptr;   // k_gs_fin's epoch and epoch-tagged sums [1 + 2 * KS_NCOLP]
+  double *d_coef1 = nullptr;                // first-pass coefficients kept for a deferred store [KS_NCOLP]
   double *d_scalar = nullptr;      // small device scratch (norm results)
   double *h_scalar = nullptr;      // pinned host scratch
   ks_status *h_status = nullptr;   // pinned mirror of d_status, filled by ks_status_post
@@ -191,6 +192,7 @@ struct ks_grp_gs_member {
   ks_status *st;
   unsigned *counter;
   unsigned long long *pub;         // the context's epoch-tagged finish area (k+2 finishing workgroups), or null
+  double *coef1;                   // first-pass coefficient copy for a deferred store (null: none)
   long ld;
   int n, k, rpb, nb, col;
   int cw;                          // the member's own column split (as when it is grouped alone)

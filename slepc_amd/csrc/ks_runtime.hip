@@ -41,6 +41,7 @@ __global__ void k_status_reset(ks_status *st) {
     st->prescale_col = -1;
     st->prescale_pass = 0;
     st->halt_col = -1;
+    st->defer_col = -1;
     st->halt_need = 0;
   }
 }
@@ -104,6 +105,7 @@ int ks_ctx_create(ks_ctx *out) {
   KS_HIP(hipMalloc(&c->d_partial, sizeof(double) * (size_t)KS_MAXB * KS_NCOLP));
   KS_HIP(hipMalloc(&c->d_gpartial, sizeof(double) * (size_t)KS_MAXG * KS_NCOLP));
   KS_HIP(hipMalloc(&c->d_counters, sizeof(unsigned) * (KS_MAXG + 1)));
+  if (!getenv("KS_GS_NODEFER")) KS_HIP(hipMalloc(&c->d_coef1, sizeof(double) * KS_NCOLP));
   {
     const char *t = getenv("KS_GS_FIN_TICKET");   // 1: finishing launches count with a ticket instead
     if (!(t && t[0] == '1')) KS_HIP(hipMalloc(&c->d_finpub, sizeof(unsigned long long) * (1 + 2 * KS_NCOLP)));
@@ -130,6 +132,7 @@ int ks_ctx_destroy(ks_ctx c) {
   (void)hipFree(c->d_partial);
   (void)hipFree(c->d_gpartial);
   if (c->d_finpub) (void)hipFree(c->d_finpub);
+  if (c->d_coef1) (void)hipFree(c->d_coef1);
   (void)hipFree(c->d_counters);
   (void)hipFree(c->d_scalar);
   if (c->d_ran) (void)hipFree(c->d_ran);

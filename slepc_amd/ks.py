@@ -34,7 +34,8 @@ class Status(C.Structure):
                 ("onrm", C.c_double), ("nrm", C.c_double), ("beta_last", C.c_double),
                 ("scale", C.c_double), ("sumsq", C.c_double), ("exact_sq", C.c_double),
                 ("prescale", C.c_double), ("prescale_col", C.c_int32), ("prescale_pass", C.c_int32),
-                ("halt_col", C.c_int32), ("halt_need", C.c_int32)]
+                ("halt_col", C.c_int32), ("halt_need", C.c_int32), ("defer_col", C.c_int32),
+                ("defer_pass", C.c_int32)]
 
 
 class GroupMember(C.Structure):
