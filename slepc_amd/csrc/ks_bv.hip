@@ -718,6 +718,18 @@ __global__ __launch_bounds__(KS_BLOCK) void k_gs_split(const T *__restrict__ V, 
                                                   gs_prescale<FUSED>(st, col, pass), gs_defer<FUSED>(st, col, pass),
                                                   coef1);
 }
+// the re-forming sweep held to the occupancy of the plain fused sweep where that
+// is four waves per SIMD (its extra sums would otherwise take it to three)
+template <typename T, int CW, bool FUSED, int RPL, bool PAIR, bool REFORM>
+__global__ __launch_bounds__(KS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_gs_split_w4(
+    const T *__restrict__ V, long ld, int k, T *__restrict__ w, int n, int rpb, const double *__restrict__ cbuf,
+    double *__restrict__ partial, const ks_status *st, int col, int pass, int *ran, const double *__restrict__ coef1) {
+  if (!gs_pass_gate<FUSED>(st, col, pass)) return;
+  if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = 1;
+  gs_split_block<T, CW, FUSED, RPL, PAIR, REFORM>(blockIdx.x, gridDim.x, V, ld, k, w, n, rpb, cbuf, partial,
+                                                  gs_prescale<FUSED>(st, col, pass), gs_defer<FUSED>(st, col, pass),
+                                                  coef1);
+}
 
 // The finish of a Gram-Schmidt pass: a separate launch, ordered by the stream
 // after every partial sum of the sweep is written (so the sweep kernels, which
@@ -1461,6 +1473,9 @@ void launch_split(ks_ctx ctx, const gs_geom &g, const T *V, long ld, int k, T *w
     if (g.pair)                                                                                                   \
       hipLaunchKernelGGL((k_gs_split<T, CC, FUSED, 2, sizeof(T) == 4, RF>), grid, block, 0, ctx->stream, V, ld, k, w,  \
                          n, g.cpb, cbuf, ctx->d_partial, ctx->d_status, col, pass, ran, ctx->d_coef1);                          \
+    else if (sizeof(T) == 8 && g.rpl == 2 && RF && CC == 12)                                                       \
+      hipLaunchKernelGGL((k_gs_split_w4<T, CC, FUSED, 2, false, RF>), grid, block, 0, ctx->stream, V, ld, k, w, n,     \
+                         g.cpb, cbuf, ctx->d_partial, ctx->d_status, col, pass, ran, ctx->d_coef1);                 \
     else if (sizeof(T) == 8 && g.rpl == 2)                                                                        \
       hipLaunchKernelGGL((k_gs_split<T, CC, FUSED, 2, false, RF>), grid, block, 0, ctx->stream, V, ld, k, w, n, g.cpb, \
                          cbuf, ctx->d_partial, ctx->d_status, col, pass, ran, ctx->d_coef1);                                    \
